@@ -1,0 +1,230 @@
+/*
+ * nuif.h - C ABI of the MI355X-native KinectFusion / voxel-hashing fusion path.
+ *
+ * This library replaces the reference's OpenCL runtime layer
+ * (src/OpenCLUtilities: NuiOpenCLGlobal, NuiOpenCLKernelManager,
+ * NuiGPUMemManager, NuiOpenCLFoundationUtils) and its OpenCL C kernels
+ * (the .cl files of src/Kernels) for the fusion path of src/SLAM.  Every entry point is
+ * plain C: device pointers, host pointers, sizes and a stream handle.  All
+ * functions return NUIF_OK (0) or a negative status; nuif_last_error() gives
+ * the message of the calling thread's last failure.  Launch entry points are
+ * asynchronous on the given stream (NULL selects the library's default
+ * stream of the current device).
+ *
+ * Citations: K/ = src/Kernels/, OCL/ = src/SLAM/VisualOdometry/DeviceSpecific/OpenCL/,
+ * CLU/ = src/OpenCLUtilities/.
+ */
+#ifndef NUIF_H
+#define NUIF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NUIF_OK 0
+#define NUIF_ERR_INVALID -1   /* bad argument / shape */
+#define NUIF_ERR_HIP -2       /* HIP runtime error */
+#define NUIF_ERR_OOM -3       /* device allocation failed */
+#define NUIF_ERR_NODEV -4     /* no GPU / not initialised */
+#define NUIF_ERR_TRACKING -5  /* ICP failure (singular normal equations) */
+
+/* ---- shared structures (layouts fixed by K/gpu_def.h:5-34) ---------------- */
+
+/* K/gpu_def.h:13-27 NuiCLCameraParams (40 bytes) */
+typedef struct NuiCLCameraParams {
+    float fx, fy, fx_inv, fy_inv, cx, cy;
+    uint32_t depthImageWidth, depthImageHeight;
+    float sensorDepthWorldMin, sensorDepthWorldMax;
+} NuiCLCameraParams;
+
+/* K/gpu_def.h:29-34 NuiCLRigidTransform (84 bytes): world = R * cam + t, row-major */
+typedef struct NuiCLRigidTransform {
+    float R[9];
+    float R_inv[9];
+    float t[3];
+} NuiCLRigidTransform;
+
+/* K/gpu_def.h:5-11 TsdfParams (40 bytes) */
+typedef struct TsdfParams {
+    float resolution[3];
+    float dimension[3];
+    float cell_size[3];
+    float tranc_dist;
+} TsdfParams;
+
+/* Dense TSDF voxel storage (4 or 8 bytes per voxel, layout ((y*Nx + x)*Nz + z)). */
+#define NUIF_TSDF_S16 0   /* reference short2: int16 tsdf*32766 (RTZ), int16 weight (K/utils.cl:69-84) */
+#define NUIF_TSDF_F16 1   /* FP16 tsdf + uint16 weight, FP32 arithmetic, round-to-nearest */
+#define NUIF_TSDF_F32 2   /* FP32 tsdf + FP32 weight */
+
+typedef struct nuif_stream_s* nuif_stream_t;   /* a hipStream_t */
+typedef struct nuif_event_s* nuif_event_t;     /* a hipEvent_t */
+typedef struct nuif_graph_s* nuif_graph_t;     /* an instantiated hipGraphExec_t */
+
+/* ---- runtime: replaces CLU/NuiOpenCLGlobal.cpp:75-220 (context + queue) ---- */
+int nuif_init(int device);                          /* select + initialise a GPU */
+int nuif_device_count(int* count);
+int nuif_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes, int* clock_khz);
+const char* nuif_last_error(void);
+const char* nuif_version(void);
+
+/* ---- memory: replaces CLU/NuiGPUMemManager (CreateBufferCL / ReleaseMemObjectCL) */
+int nuif_malloc(void** dptr, size_t bytes);         /* 256-byte aligned, zero-filled */
+int nuif_free(void* dptr);
+int nuif_mem_in_use(size_t* bytes);                 /* sum of live nuif_malloc bytes (admission) */
+int nuif_host_alloc(void** hptr, size_t bytes);     /* pinned host memory */
+int nuif_host_free(void* hptr);
+int nuif_memcpy_h2d(void* dst, const void* src, size_t bytes, nuif_stream_t s);
+int nuif_memcpy_d2h(void* dst, const void* src, size_t bytes, nuif_stream_t s);
+int nuif_memcpy_d2d(void* dst, const void* src, size_t bytes, nuif_stream_t s);
+/* byte fill by kernel (safe inside graph capture); CLU/NuiOpenCLFoundationUtils.cpp:8-150 */
+int nuif_fill_u8(void* dst, uint8_t value, size_t bytes, nuif_stream_t s);
+int nuif_fill_u32(void* dst, uint32_t value, size_t count, nuif_stream_t s);
+
+/* ---- streams, events, graphs --------------------------------------------- */
+int nuif_stream_create(nuif_stream_t* s);
+int nuif_stream_destroy(nuif_stream_t s);
+int nuif_stream_synchronize(nuif_stream_t s);
+int nuif_device_synchronize(void);
+int nuif_event_create(nuif_event_t* e);
+int nuif_event_destroy(nuif_event_t e);
+int nuif_event_record(nuif_event_t e, nuif_stream_t s);
+int nuif_event_synchronize(nuif_event_t e);
+int nuif_event_elapsed_ms(nuif_event_t start, nuif_event_t stop, float* ms);
+int nuif_stream_wait_event(nuif_stream_t s, nuif_event_t e);
+int nuif_graph_begin(nuif_stream_t s);                       /* start capture on s */
+int nuif_graph_end(nuif_stream_t s, nuif_graph_t* g);        /* end capture + instantiate */
+int nuif_graph_launch(nuif_graph_t g, nuif_stream_t s);
+int nuif_graph_destroy(nuif_graph_t g);
+
+/* ---- frame preprocessing (one entry per reference kernel) ------------------ */
+
+/* K/filters.cl:9-23 depth_passing_filter_kernel: depth[i] = raw[i]*0.001 m, NaN unless dmin < d < dmax */
+int nuif_depth_passing_filter(const uint16_t* raw, float* depth, size_t n, float dmin, float dmax,
+                              nuif_stream_t s);
+
+/* K/bilateralFilter.cl:68-119 bilateral_filter_depth_kernel; gaussian: 2*radius+1 host floats
+ * (OCL/NuiKinfuOpenCLFrame.cpp:165-189), radius <= 7 */
+int nuif_bilateral_filter_depth(const float* in, float* out, int width, int height, const float* gaussian,
+                                int radius, float sigma_depth2_inv_half, float depth_threshold, nuif_stream_t s);
+
+/* K/transformMaps.cl:44-88 half_sample_kernel; dst is (dst_width x dst_height), src 2x that */
+int nuif_half_sample(const float* src, float* dst, int dst_width, int dst_height, int radius,
+                     float depth_threshold, nuif_stream_t s);
+
+/* K/transformMaps.cl:18-42 depth2vertex_kernel; vmap packed float3 AoS; div = 2^level */
+int nuif_depth2vertex(const float* depth, float* vmap, int width, int height, const NuiCLCameraParams* cam,
+                      int div, nuif_stream_t s);
+
+/* K/normalEstimation.cl:770-822 estimate_normals_simple_kernel */
+int nuif_estimate_normals_simple(const float* vmap, float* nmap, int width, int height, float depth_threshold,
+                                 nuif_stream_t s);
+
+/* K/transformMaps.cl:91-127 transform_maps_kernel; pose: device NuiCLRigidTransform */
+int nuif_transform_maps(const float* vmap, const float* nmap, float* vmap_dst, float* nmap_dst, int width,
+                        int height, const NuiCLRigidTransform* pose, nuif_stream_t s);
+
+/* K/tsdfVolume.cl:8-29 scaleDepthsKernel (in place) */
+int nuif_scale_depths(float* depth, int width, int height, const NuiCLCameraParams* cam, nuif_stream_t s);
+
+/* Fused frame preprocessing: OCL/NuiKinfuOpenCLFrame.cpp:314-367 UpdateVertexBuffers
+ * (H1 passing + H3 bilateral + H5 level 0) and OCL/NuiKinfuOpenCLDepthTracker.cpp:140-236
+ * (H4 pyramid + H5 per level) in as few launches as the data flow allows.
+ *   depth_out      : passing-filter depth (W*H); if scale_for_tsdf, already scaled to ray
+ *                    length (K/tsdfVolume.cl:8-29) for the dense scene's integration
+ *   depth_pyr[l]   : filtered depth per level (level 0 = bilateral output)
+ *   vmap_pyr[l]    : camera-space vertices per level (float3 AoS)               */
+int nuif_frame_preprocess(const uint16_t* raw, float* depth_out, float* const* depth_pyr, float* const* vmap_pyr,
+                          int levels, int width, int height, const NuiCLCameraParams* cam, const float* gaussian,
+                          int radius, float sigma_depth2_inv_half, float depth_threshold, int scale_for_tsdf,
+                          nuif_stream_t s);
+
+/* ---- projective ICP (K/icp.cl:162-326, OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504) ---- */
+
+#define NUIF_ICP_TERMS 29
+#define NUIF_ICP_MAX_LEVELS 4
+
+/* Host-visible view of the device tracking state (nuif_icp_state_read). */
+typedef struct nuif_icp_result {
+    float R[9];          /* current pose estimate, camera -> world */
+    float t[3];
+    float error;         /* m_error of the last executed iteration */
+    float count;         /* m_count of the last executed iteration */
+    int32_t status;      /* 0 ok, 1 tracking failure (singular system) */
+    int32_t iterations;  /* iterations that ran (not skipped by a break) */
+    int32_t integrate;   /* integration gate decision of the last frame */
+    int32_t frame;       /* frames finalised */
+    double sums[NUIF_ICP_TERMS]; /* 29 sums of the last executed iteration */
+} nuif_icp_result;
+
+typedef struct nuif_icp_config {
+    int levels;                            /* pyramid levels (<= 4) */
+    int iterations[NUIF_ICP_MAX_LEVELS];   /* per level, level 0 = finest */
+    float dist_threshold;                  /* NuiTrackerConfig::dist_threshold */
+    float gate_threshold;                  /* integration metric threshold (0 = always) */
+} nuif_icp_config;
+
+size_t nuif_icp_state_bytes(void);
+size_t nuif_icp_partials_bytes(int width, int height);
+/* Reset tracking state: estimate = pose, last-integration pose = pose. */
+int nuif_icp_state_init(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s);
+int nuif_icp_state_read(const void* state, nuif_icp_result* out, nuif_stream_t s); /* synchronises s */
+/* Begin a frame: estimate <- pose, clear break/failure flags. */
+int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s);
+/* One iteration at `level`: association + 29-term reduction (partials) and, in the last
+ * workgroup, the reference's LM accept/reject, FP64 6x6 solve and pose update.
+ * vmap_level: current-frame vertices at the level; ref_vmap/ref_nmap: world maps at full
+ * resolution; prev_pose: previous frame's transform. count_threshold per quirk Q10. */
+int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* cam, const float* vmap_level,
+                       int level, const float* ref_vmap, const float* ref_nmap,
+                       const NuiCLRigidTransform* prev_pose, float dist_threshold, uint32_t count_threshold,
+                       int iter, nuif_stream_t s);
+/* Finish the frame: on success pose <- estimate (with R_inv), and the integration gate
+ * (VO/NuiKinfuTrackingEngine.cpp:113-127) is evaluated into the state.  If history is not
+ * NULL, row (frame % capacity) of NUIF_POSE_ROW floats receives
+ * [R (9, row-major), t (3), ok (1/0), error, count, integrated (1/0)]. */
+#define NUIF_POSE_ROW 16
+int nuif_icp_end(void* state, NuiCLRigidTransform* pose, float gate_threshold, float* history, int capacity,
+                 nuif_stream_t s);
+/* All levels and iterations of one frame, no host synchronisation. */
+int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
+                      const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
+                      const nuif_icp_config* cfg, float* history, int capacity, nuif_stream_t s);
+/* Device int32 flags inside a state block: integration gate of the last frame, and
+ * tracking success of the last frame (raycast runs only if set). */
+const int32_t* nuif_icp_gate_flag(const void* state);
+const int32_t* nuif_icp_ok_flag(const void* state);
+/* Count thresholds of the reference's stale-global-size rule (quirk Q10). */
+int nuif_icp_count_thresholds(int width, int height, const nuif_icp_config* cfg, uint32_t* thresholds);
+
+/* ---- dense TSDF scene (OCL/NuiKinfuOpenCLScene.cpp, K/tsdfVolume.cl, K/raycast.cl) ---- */
+
+size_t nuif_tsdf_voxel_bytes(int storage);
+/* K/tsdfVolume.cl:192-211 initializeVolume: (32767,0) / (0,0) and colour 0 */
+int nuif_tsdf_reset(void* volume, uint8_t* color_volume, const TsdfParams* params, int storage, nuif_stream_t s);
+/* K/tsdfVolume.cl:31-190 integrateTsdfVolumeKernel.  depth: ray-length depth (scaled);
+ * bgra/color_volume may be NULL; gate: optional device int32, integrate only if non-zero;
+ * updated: optional device counter, incremented by the number of voxels updated. */
+int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
+                        const TsdfParams* params, const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose,
+                        int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
+                        nuif_stream_t s);
+/* K/raycast.cl:128-277 raycastKernel: world vmap/nmap (float3 AoS) and optional cmap */
+int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
+                      const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
+                      uint8_t* cmap, int storage, nuif_stream_t s);
+/* Same, skipped entirely when the device int32 *enable is zero (tracking failure). */
+int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
+                            const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap,
+                            float* nmap, uint8_t* cmap, int storage, const int32_t* enable, nuif_stream_t s);
+/* Effective truncation distance (OCL/NuiKinfuOpenCLScene.cpp:41: >= 2.1 * max cell). */
+int nuif_tsdf_params(int nx, int ny, int nz, float dx, float dy, float dz, float trunc, TsdfParams* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUIF_H */

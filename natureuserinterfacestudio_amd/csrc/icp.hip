@@ -1,0 +1,513 @@
+// Projective point-to-plane ICP (rows H8-H11).
+//
+// The reference runs per iteration: icp_block_kernel (128-wide LDS trees of
+// 29 floats), compute_sums, a blocking read-back, and the Levenberg-Marquardt
+// bookkeeping + Eigen FP64 6x6 solve on the host
+// (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504).  Here every iteration is two
+// device launches and the whole pyramid runs without a host round trip:
+//
+//   icp_partials_kernel  grid-stride over the level's pixels; each thread keeps
+//                        Kahan-compensated FP32 sums of the 29 terms, the block
+//                        transposes them through LDS and reduces each term with
+//                        a compensated 32-wide pass, combined in FP64.
+//   icp_solve_kernel     one workgroup: FP64 sum over block partials, the
+//                        reference's accept/reject state machine (quirks
+//                        Q10-Q12 kept), FP64 LDL^T solve of the damped 6x6
+//                        system, Euler-angle increment composed in FP64.
+//
+// The tracking state (estimate, last good pose, lambda, break flags, last
+// integration pose, integration gate) lives in one device block so that
+// integrate/raycast launches can consume the ICP outcome without the host.
+
+#include "nuif_internal.h"
+
+namespace nuif {
+
+constexpr int kTerms = NUIF_ICP_TERMS;
+constexpr int kBlock = 256;
+constexpr int kMaxBlocks = 1024;
+
+struct IcpState {
+    float R[9], t[3];          // current estimate (Rcurr, tcurr)
+    float Rg[9], tg[3];        // last accepted (Rgood, tgood)
+    float Rlast[9], tlast[3];  // pose of the last integration
+    float lambda, err_old;
+    float error, count;
+    int32_t brk[NUIF_ICP_MAX_LEVELS];
+    int32_t failed, iterations, gate, frame, ok, pad;
+    double good[27];
+    double sums[kTerms];
+};
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// K/icp.cl:192-220 interpolateBilinear_withHoles.  Reads past the row end as
+// the reference does (quirk Q14); a read past the buffer end is a hole.
+__device__ __forceinline__ f3 bilinear_holes(const float* __restrict__ src, float px, float py, int W, int H) {
+    const int x = (int)floorf(px), y = (int)floorf(py);
+    const float dx = px - (float)x, dy = py - (float)y;
+    const long long n = (long long)W * H;
+    const long long ia = x + (long long)y * W, ib = ia + 1, ic = ia + W, id = ic + 1;
+    const f3 NaN3 = mk3(kNaN, kNaN, kNaN);
+    if (ia < 0 || id >= n) return NaN3;
+    const f3 a = ld3(src, ia), b = ld3(src, ib), c = ld3(src, ic), d = ld3(src, id);
+    if (nan3(a) || nan3(b) || nan3(c) || nan3(d)) return NaN3;
+    f3 r;
+    r.x = a.x * (1.0f - dx) * (1.0f - dy) + b.x * dx * (1.0f - dy) + c.x * (1.0f - dx) * dy + d.x * dx * dy;
+    r.y = a.y * (1.0f - dx) * (1.0f - dy) + b.y * dx * (1.0f - dy) + c.y * (1.0f - dx) * dy + d.y * dx * dy;
+    r.z = a.z * (1.0f - dx) * (1.0f - dy) + b.z * dx * (1.0f - dy) + c.z * (1.0f - dx) * dy + d.z * dx * dy;
+    return r;
+}
+
+// LDS: 29 terms x 8 segments x 33 (padded) floats, reused as doubles later
+constexpr int kSeg = 8, kSegLen = 32, kSegPad = 33;
+
+__global__ void __launch_bounds__(kBlock)
+    icp_partials_kernel(const IcpState* __restrict__ st, const float* __restrict__ vcur, int npx,
+                        NuiCLCameraParams cam, const float* __restrict__ vref, const float* __restrict__ nref,
+                        const NuiCLRigidTransform* __restrict__ prev, float distThr, int level,
+                        double* __restrict__ partials) {
+    if (st->failed || st->brk[level]) return;
+    __shared__ float lds[kTerms * kSeg * kSegPad];
+    __shared__ double part[kTerms * kSeg];
+
+    float R[9], t[3], Ri[9], tp[3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        R[k] = st->R[k];
+        Ri[k] = prev->R_inv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        t[k] = st->t[k];
+        tp[k] = prev->t[k];
+    }
+    float acc[kTerms], cmp[kTerms];
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) acc[k] = cmp[k] = 0.f;
+
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    for (int p = blockIdx.x * kBlock + threadIdx.x; p < npx; p += gridDim.x * kBlock) {
+        const f3 vc = ld3(vcur, p);
+        if (nan3(vc)) continue;
+        // K/icp.cl:243-289
+        const f3 pv = add3(rot3(R, vc), mk3(t[0], t[1], t[2]));
+        const f3 pp = rot3(Ri, sub3(pv, mk3(tp[0], tp[1], tp[2])));
+        const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
+        if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) continue;
+        const f3 rn = bilinear_holes(nref, px, py, W, H);
+        if (nan3(rn)) continue;
+        const f3 rv = bilinear_holes(vref, px, py, W, H);
+        if (nan3(rv)) continue;
+        const f3 diff = sub3(rv, pv);
+        const float dist = sqrtf(dot3(diff, diff));
+        if (!(dist < distThr)) continue;
+        const f3 row0 = cross3(rn, pv);
+        const float c[7] = {row0.x, row0.y, row0.z, rn.x, rn.y, rn.z, dot3(rn, diff)};
+        int s = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = i; j < 7; ++j) {
+                kahan_add(acc[s], cmp[s], c[i] * c[j]);
+                ++s;
+            }
+        kahan_add(acc[27], cmp[27], c[6] * c[6]);
+        acc[28] += 1.0f;  // counts are exact in FP32 up to 2^24
+    }
+
+    // transpose through LDS: term k, thread tid -> lds[k][tid / 32][tid % 32]
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) lds[k * kSeg * kSegPad + (tid / kSegLen) * kSegPad + (tid % kSegLen)] = acc[k];
+    __syncthreads();
+    if (tid < kTerms * kSeg) {
+        const int k = tid / kSeg, sg = tid % kSeg;
+        const float* src = &lds[k * kSeg * kSegPad + sg * kSegPad];
+        float s = 0.f, c = 0.f;
+        for (int j = 0; j < kSegLen; ++j) kahan_add(s, c, src[j]);
+        part[k * kSeg + sg] = (double)s - (double)c;
+    }
+    __syncthreads();
+    if (tid < kTerms) {
+        double s = 0.0;
+        for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
+        partials[(size_t)blockIdx.x * kTerms + tid] = s;
+    }
+}
+
+// ---- small FP64 linear algebra for the solve ----------------------------------
+
+// A = L D L^T (unit lower L), in place; returns false if a pivot is not positive.
+__device__ bool ldlt6(double A[36], double* det) {
+    double d = 1.0;
+    for (int j = 0; j < 6; ++j) {
+        double Dj = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) Dj -= A[j * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
+        A[j * 6 + j] = Dj;
+        d *= Dj;
+        if (!(Dj > 0.0)) {
+            *det = d;
+            return false;
+        }
+        for (int i = j + 1; i < 6; ++i) {
+            double v = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) v -= A[i * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
+            A[i * 6 + j] = v / Dj;
+        }
+    }
+    *det = d;
+    return true;
+}
+
+__device__ void ldlt6_solve(const double F[36], const double b[6], double x[6]) {
+    double y[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= F[i * 6 + k] * y[k];
+        y[i] = s;
+    }
+    for (int i = 0; i < 6; ++i) y[i] /= F[i * 6 + i];
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int k = i + 1; k < 6; ++k) s -= F[k * 6 + i] * x[k];
+        x[i] = s;
+    }
+}
+
+__device__ void matmul3d(const double a[9], const double b[9], double c[9]) {
+    double r[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+    for (int k = 0; k < 9; ++k) c[k] = r[k];
+}
+
+// Eigen-style float cofactor inverse (OCL/NuiKinfuOpenCLCameraState.cpp:67-69)
+__device__ void inverse3f(const float m[9], float out[9]) {
+    const float c00 = m[4] * m[8] - m[5] * m[7];
+    const float c01 = m[5] * m[6] - m[3] * m[8];
+    const float c02 = m[3] * m[7] - m[4] * m[6];
+    const float det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    const float inv = 1.0f / det;
+    out[0] = c00 * inv;
+    out[3] = c01 * inv;
+    out[6] = c02 * inv;
+    out[1] = (m[2] * m[7] - m[1] * m[8]) * inv;
+    out[4] = (m[0] * m[8] - m[2] * m[6]) * inv;
+    out[7] = (m[1] * m[6] - m[0] * m[7]) * inv;
+    out[2] = (m[1] * m[5] - m[2] * m[4]) * inv;
+    out[5] = (m[2] * m[3] - m[0] * m[5]) * inv;
+    out[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    icp_solve_kernel(IcpState* __restrict__ st, const double* __restrict__ partials, int nblocks, int level,
+                     int iter, uint32_t count_threshold) {
+    if (st->failed || st->brk[level]) return;
+    __shared__ double part[kTerms * kSeg];
+    __shared__ double sums[kTerms];
+    const int tid = threadIdx.x;
+    if (tid < kTerms * kSeg) {
+        const int k = tid / kSeg, j = tid % kSeg;
+        double s = 0.0;
+        for (int b = j; b < nblocks; b += kSeg) s += partials[(size_t)b * kTerms + k];
+        part[k * kSeg + j] = s;
+    }
+    __syncthreads();
+    if (tid < kTerms) {
+        double s = 0.0;
+        for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
+        sums[tid] = s;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+
+    // OCL/NuiKinfuOpenCLDepthTracker.cpp:395-458: error/count, accept or reject
+    for (int k = 0; k < kTerms; ++k) st->sums[k] = sums[k];
+    const float cnt = (float)sums[28];
+    float err = cnt > 0.f ? sqrtf((float)sums[27]) / cnt : 3.402823466e38f;
+    st->error = err;
+    st->count = cnt;
+    st->iterations += 1;
+    if (iter == 0) {  // per-level initialisation (:299-306)
+        st->lambda = 1.0f;
+        st->err_old = 1e20f;
+        for (int k = 0; k < 9; ++k) st->Rg[k] = st->R[k];
+        for (int k = 0; k < 3; ++k) st->tg[k] = st->t[k];
+    }
+    float lambda = st->lambda;
+    if (cnt < (float)count_threshold || err > st->err_old) {
+        lambda *= 10.0f;
+        for (int k = 0; k < 9; ++k) st->R[k] = st->Rg[k];
+        for (int k = 0; k < 3; ++k) st->t[k] = st->tg[k];
+        st->lambda = lambda;
+        if (iter == 0) {
+            st->brk[level] = 1;
+            return;
+        }
+    } else {
+        lambda /= 10.0f;
+        st->lambda = lambda;
+        for (int k = 0; k < 9; ++k) st->Rg[k] = st->R[k];
+        for (int k = 0; k < 3; ++k) st->tg[k] = st->t[k];
+        st->err_old = err;
+        for (int k = 0; k < 27; ++k) st->good[k] = sums[k];
+    }
+    // damped normal equations, normalised by the current count (quirk Q12)
+    double A[36], b[6];
+    const double dc = (double)cnt;
+    for (int i = 0, s = 0; i < 6; ++i) {
+        for (int j = i; j < 7; ++j, ++s) {
+            const double v = st->good[s] / dc;
+            if (j == 6)
+                b[i] = v;
+            else
+                A[j * 6 + i] = A[i * 6 + j] = v;
+        }
+        A[i * 6 + i] *= (double)(1.0f + lambda);
+    }
+    double det;
+    const bool spd = ldlt6(A, &det);
+    if (!spd || fabs(det) < 1e-15 || isnan(det)) {
+        st->failed = 1;
+        return;
+    }
+    double x[6];
+    ldlt6_solve(A, b, x);
+    const float al = (float)x[0], be = (float)x[1], ga = (float)x[2];
+    if (al * al + be * be + ga * ga < 1e-6f) {  // converged: break before applying (:482-484)
+        st->brk[level] = 1;
+        return;
+    }
+    // Rinc = Rz(gamma) Ry(beta) Rx(alpha); t <- Rinc^-1 t + t_inc; R <- Rinc^-1 R   (:486-497)
+    const double sa = sin((double)al), ca = cos((double)al);
+    const double sb = sin((double)be), cb = cos((double)be);
+    const double sg = sin((double)ga), cg = cos((double)ga);
+    const double Rz[9] = {cg, -sg, 0, sg, cg, 0, 0, 0, 1};
+    const double Ry[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
+    const double Rx[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca};
+    double Rzy[9], Rinc[9];
+    matmul3d(Rz, Ry, Rzy);
+    matmul3d(Rzy, Rx, Rinc);
+    double Ri[9];  // inverse of a rotation: its transpose
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rinc[3 * j + i];
+    double Rc[9], tc[3];
+    for (int k = 0; k < 9; ++k) Rc[k] = st->R[k];
+    for (int k = 0; k < 3; ++k) tc[k] = st->t[k];
+    double Rn[9];
+    matmul3d(Ri, Rc, Rn);
+    const double tinc[3] = {(double)(float)x[3], (double)(float)x[4], (double)(float)x[5]};
+    for (int i = 0; i < 3; ++i)
+        st->t[i] = (float)(Ri[3 * i] * tc[0] + Ri[3 * i + 1] * tc[1] + Ri[3 * i + 2] * tc[2] + tinc[i]);
+    for (int k = 0; k < 9; ++k) st->R[k] = (float)Rn[k];
+}
+
+__global__ void icp_begin_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
+    for (int k = 0; k < 9; ++k) st->R[k] = pose->R[k];
+    for (int k = 0; k < 3; ++k) st->t[k] = pose->t[k];
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
+    st->failed = 0;
+    st->iterations = 0;
+    st->ok = 0;
+}
+
+__global__ void icp_state_init_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
+    for (int k = 0; k < 9; ++k) st->R[k] = st->Rg[k] = st->Rlast[k] = pose->R[k];
+    for (int k = 0; k < 3; ++k) st->t[k] = st->tg[k] = st->tlast[k] = pose->t[k];
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
+    st->failed = 0;
+    st->iterations = 0;
+    st->gate = 1;
+    st->ok = 1;
+    st->frame = 0;
+    st->lambda = 1.0f;
+    st->err_old = 1e20f;
+    st->error = 0.f;
+    st->count = 0.f;
+}
+
+// src/Foundation/NuiMatrixUtilities.cpp:7-52 rodrigues2(R).norm() for a rotation
+__device__ float rodrigues_norm(const float Rm[9]) {
+    const double rx = Rm[7] - Rm[5], ry = Rm[2] - Rm[6], rz = Rm[3] - Rm[1];
+    const double s = sqrt((rx * rx + ry * ry + rz * rz) * 0.25);
+    double c = ((double)Rm[0] + Rm[4] + Rm[8] - 1) * 0.5;
+    c = c > 1. ? 1. : c < -1. ? -1. : c;
+    const double theta = acos(c);
+    if (s < 1e-5 && c > 0) return 0.f;
+    return (float)theta;
+}
+
+__global__ void icp_end_kernel(IcpState* st, NuiCLRigidTransform* pose, float gate_threshold, float* history,
+                               int capacity) {
+    const int frame = st->frame;
+    st->frame = frame + 1;
+    float* row = history ? history + (size_t)(frame % capacity) * NUIF_POSE_ROW : nullptr;
+    if (row) {
+        const float* R = st->failed ? pose->R : st->R;
+        const float* t = st->failed ? pose->t : st->t;
+        for (int k = 0; k < 9; ++k) row[k] = R[k];
+        for (int k = 0; k < 3; ++k) row[9 + k] = t[k];
+        row[12] = st->failed ? 0.f : 1.f;
+        row[13] = st->error;
+        row[14] = st->count;
+        row[15] = 0.f;
+    }
+    if (st->failed) {  // RunTracking returns false: no pose update, integrate or raycast
+        st->ok = 0;
+        st->gate = 0;
+        return;
+    }
+    st->ok = 1;
+    for (int k = 0; k < 9; ++k) pose->R[k] = st->R[k];
+    for (int k = 0; k < 3; ++k) pose->t[k] = st->t[k];
+    inverse3f(st->R, pose->R_inv);
+    // VO/NuiKinfuTrackingEngine.cpp:113-127 integration gate
+    float Rrel[9];
+    {
+        const float* Ri = pose->R_inv;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                Rrel[3 * i + j] = Ri[3 * i] * st->Rlast[j] + Ri[3 * i + 1] * st->Rlast[3 + j] + Ri[3 * i + 2] * st->Rlast[6 + j];
+    }
+    const float rnorm = rodrigues_norm(Rrel);
+    const float d0 = st->t[0] - st->tlast[0], d1 = st->t[1] - st->tlast[1], d2 = st->t[2] - st->tlast[2];
+    const float tnorm = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+    const int gate = (rnorm + 1.f * tnorm) / 2 >= gate_threshold;
+    st->gate = gate;
+    if (row) row[15] = (float)gate;
+    if (gate) {
+        for (int k = 0; k < 9; ++k) st->Rlast[k] = st->R[k];
+        for (int k = 0; k < 3; ++k) st->tlast[k] = st->t[k];
+    }
+}
+
+}  // namespace nuif
+
+using namespace nuif;
+
+extern "C" {
+
+size_t nuif_icp_state_bytes(void) { return (sizeof(IcpState) + 255) / 256 * 256; }
+
+size_t nuif_icp_partials_bytes(int, int) { return (size_t)kMaxBlocks * kTerms * sizeof(double); }
+
+const int32_t* nuif_icp_gate_flag(const void* state) {
+    return state ? &reinterpret_cast<const IcpState*>(state)->gate : nullptr;
+}
+
+const int32_t* nuif_icp_ok_flag(const void* state) {
+    return state ? &reinterpret_cast<const IcpState*>(state)->ok : nullptr;
+}
+
+int nuif_icp_state_init(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && pose, "nuif_icp_state_init: null argument");
+    icp_state_init_kernel<<<1, 1, 0, stream_of(s)>>>((IcpState*)state, pose);
+    NUIF_LAUNCHED("icp_state_init_kernel");
+    return NUIF_OK;
+}
+
+int nuif_icp_state_read(const void* state, nuif_icp_result* out, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && out, "nuif_icp_state_read: null argument");
+    IcpState h;
+    NUIF_HIP(hipMemcpyAsync(&h, state, sizeof(IcpState), hipMemcpyDeviceToHost, stream_of(s)));
+    NUIF_HIP(hipStreamSynchronize(stream_of(s)));
+    for (int k = 0; k < 9; ++k) out->R[k] = h.R[k];
+    for (int k = 0; k < 3; ++k) out->t[k] = h.t[k];
+    out->error = h.error;
+    out->count = h.count;
+    out->status = h.failed ? 1 : 0;
+    out->iterations = h.iterations;
+    out->integrate = h.gate;
+    out->frame = h.frame;
+    for (int k = 0; k < kTerms; ++k) out->sums[k] = h.sums[k];
+    return NUIF_OK;
+}
+
+int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && pose, "nuif_icp_begin: null argument");
+    icp_begin_kernel<<<1, 1, 0, stream_of(s)>>>((IcpState*)state, pose);
+    NUIF_LAUNCHED("icp_begin_kernel");
+    return NUIF_OK;
+}
+
+int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* cam, const float* vmap_level,
+                       int level, const float* ref_vmap, const float* ref_nmap, const NuiCLRigidTransform* prev_pose,
+                       float dist_threshold, uint32_t count_threshold, int iter, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && partials && cam && vmap_level && ref_vmap && ref_nmap && prev_pose,
+                   "nuif_icp_iteration: null argument");
+    NUIF_CHECK_ARG(level >= 0 && level < NUIF_ICP_MAX_LEVELS && iter >= 0, "nuif_icp_iteration: bad level/iter");
+    const int Wl = (int)cam->depthImageWidth >> level, Hl = (int)cam->depthImageHeight >> level;
+    NUIF_CHECK_ARG(Wl > 0 && Hl > 0, "nuif_icp_iteration: level %d too small", level);
+    const int npx = Wl * Hl;
+    const int nblocks = std::min<int>(div_up(npx, kBlock), kMaxBlocks);
+    hipStream_t st = stream_of(s);
+    icp_partials_kernel<<<nblocks, kBlock, 0, st>>>((const IcpState*)state, vmap_level, npx, *cam, ref_vmap,
+                                                     ref_nmap, prev_pose, dist_threshold, level, (double*)partials);
+    NUIF_LAUNCHED("icp_partials_kernel");
+    icp_solve_kernel<<<1, kBlock, 0, st>>>((IcpState*)state, (const double*)partials, nblocks, level, iter,
+                                           count_threshold);
+    NUIF_LAUNCHED("icp_solve_kernel");
+    return NUIF_OK;
+}
+
+int nuif_icp_end(void* state, NuiCLRigidTransform* pose, float gate_threshold, float* history, int capacity,
+                 nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && pose, "nuif_icp_end: null argument");
+    NUIF_CHECK_ARG(!history || capacity > 0, "nuif_icp_end: history capacity must be positive");
+    icp_end_kernel<<<1, 1, 0, stream_of(s)>>>((IcpState*)state, pose, gate_threshold, history, capacity);
+    NUIF_LAUNCHED("icp_end_kernel");
+    return NUIF_OK;
+}
+
+int nuif_icp_count_thresholds(int W, int H, const nuif_icp_config* cfg, uint32_t* thr) {
+    NUIF_CHECK_ARG(cfg && thr && W > 0 && H > 0, "nuif_icp_count_thresholds: bad arguments");
+    NUIF_CHECK_ARG(cfg->levels >= 1 && cfg->levels <= NUIF_ICP_MAX_LEVELS, "bad level count %d", cfg->levels);
+    // OCL/NuiKinfuOpenCLDepthTracker.cpp:309: (kernelGlobalSize[0] >> 2l) * 0.1 where
+    // kernelGlobalSize[0] is left over from the previous level's compute_sums launch (Q10)
+    size_t kgs = (size_t)W * H;
+    for (int level = cfg->levels - 1; level >= 0; --level) {
+        thr[level] = (uint32_t)((double)(kgs >> (2 * level)) * 0.1);
+        if (cfg->iterations[level] > 0) {
+            const size_t groups = ((size_t)(W >> level) * (H >> level) + 127) / 128;
+            kgs = (groups + 127) / 128 * 128;
+        }
+    }
+    return NUIF_OK;
+}
+
+int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
+                      const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
+                      const nuif_icp_config* cfg, float* history, int capacity, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && partials && cam && vmap_pyr && ref_vmap && ref_nmap && pose && cfg,
+                   "nuif_icp_estimate: null argument");
+    uint32_t thr[NUIF_ICP_MAX_LEVELS];
+    int rc = nuif_icp_count_thresholds((int)cam->depthImageWidth, (int)cam->depthImageHeight, cfg, thr);
+    if (rc) return rc;
+    const float L = (float)cfg->levels;
+    const float dist = cfg->dist_threshold + cfg->dist_threshold / L;  // :300-304 (Q11)
+    rc = nuif_icp_begin(state, pose, s);
+    if (rc) return rc;
+    for (int level = cfg->levels - 1; level >= 0; --level) {
+        NUIF_CHECK_ARG(vmap_pyr[level], "nuif_icp_estimate: vmap level %d is NULL", level);
+        for (int it = 0; it < cfg->iterations[level]; ++it) {
+            rc = nuif_icp_iteration(state, partials, cam, vmap_pyr[level], level, ref_vmap, ref_nmap, pose, dist,
+                                    thr[level], it, s);
+            if (rc) return rc;
+        }
+    }
+    return nuif_icp_end(state, pose, cfg->gate_threshold, history, capacity, s);
+}
+
+}  // extern "C"

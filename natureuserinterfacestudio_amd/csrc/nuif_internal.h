@@ -1,0 +1,91 @@
+// Internal helpers shared by the HIP translation units of libnuif.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "nuif.h"
+
+namespace nuif {
+
+// ---- error reporting (thread-local last error, status codes) ----------------
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+hipStream_t stream_of(nuif_stream_t s);
+bool initialised();
+
+#define NUIF_REQUIRE_INIT()                                      \
+    do {                                                         \
+        if (!nuif::initialised()) {                              \
+            nuif::set_error("nuif_init() has not been called");  \
+            return NUIF_ERR_NODEV;                               \
+        }                                                        \
+    } while (0)
+
+#define NUIF_CHECK_ARG(cond, ...)        \
+    do {                                 \
+        if (!(cond)) {                   \
+            nuif::set_error(__VA_ARGS__); \
+            return NUIF_ERR_INVALID;     \
+        }                                \
+    } while (0)
+
+#define NUIF_HIP(call)                                            \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return nuif::hip_fail(e_, #call);   \
+    } while (0)
+
+#define NUIF_LAUNCHED(name)                                       \
+    do {                                                          \
+        hipError_t e_ = hipGetLastError();                        \
+        if (e_ != hipSuccess) return nuif::hip_fail(e_, name);    \
+    } while (0)
+
+inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ---- device math, mirroring the reference's float evaluation order ----------
+// Everything is compiled with -ffp-contract=off: no implicit FMA, so each
+// expression rounds exactly as written (K/*.cl evaluation order).
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 mul3(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ bool nan3(f3 v) { return isnan(v.x) || isnan(v.y) || isnan(v.z); }
+__device__ __forceinline__ f3 ld3(const float* p, size_t i) { return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ void st3(float* p, size_t i, f3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+// normalize()/fast_normalize(): a zero vector is returned unchanged
+__device__ __forceinline__ f3 normalize3(f3 v) {
+    if (v.x == 0.f && v.y == 0.f && v.z == 0.f) return v;
+    float d = dot3(v, v);
+    float inv = 1.0f / sqrtf(d);
+    return mul3(v, inv);
+}
+// OpenCL float -> int conversions: round toward zero; out of range / NaN -> INT_MIN
+__device__ __forceinline__ int to_int(float v) {
+    if (!(v > -2147483648.0f && v < 2147483647.0f)) return INT32_MIN;
+    return (int)v;
+}
+__device__ __forceinline__ f3 rot3(const float* R, f3 v) {
+    return f3{dot3(mk3(R[0], R[1], R[2]), v), dot3(mk3(R[3], R[4], R[5]), v), dot3(mk3(R[6], R[7], R[8]), v)};
+}
+
+constexpr float kNaN = __builtin_nanf("");
+
+}  // namespace nuif

@@ -1,0 +1,571 @@
+// Dense TSDF volume: reset (H14), integration (H13) and raycast (H15).
+//
+// Volume layout is the reference's ((y*Nx + x)*Nz + z), z contiguous, so a
+// z-run of one (x,y) column is a contiguous byte range.
+//
+// Integration.  The reference walks one (x,y) column per work-item along z,
+// updating the projection incrementally (K/tsdfVolume.cl:74-189); adjacent
+// work-items then touch addresses Nz*4 bytes apart, i.e. no coalescing.  Here
+// one wavefront owns an 8x8 tile of columns and processes z in chunks of 64:
+//   phase 1  every lane walks its own column for 64 steps with exactly the
+//            reference's incremental arithmetic (bit-identical projections) and
+//            parks the new tsdf sample (or "no update") in LDS;
+//   phase 2  the wave sweeps the 64 columns of the tile with 16-byte vector
+//            read-modify-writes, 1 KiB contiguous per instruction, 16 loads in
+//            flight per wave before the first store.
+// Voxels that the reference does not update are neither read nor written
+// unless they share a 16-byte vector with one that is (then they are written
+// back unchanged).  Algorithmic traffic: 2 x bytes/voxel per updated voxel +
+// the depth image; the kernel is HBM-bound.
+//
+// Raycast.  One lane per pixel, 8x8 pixel tiles per wavefront so that
+// neighbouring rays share cache lines; the fixed-step march, zero-crossing
+// test, trilinear refinement and gradient normal follow K/raycast.cl:128-277
+// operation by operation.
+
+#include <hip/hip_fp16.h>
+
+#include "nuif_internal.h"
+
+namespace nuif {
+
+// ---- voxel storage policies ----------------------------------------------------
+
+// Reference short2: int16 tsdf * 32766 (round toward zero), int16 weight (K/utils.cl:69-84).
+struct StS16 {
+    using word = uint32_t;
+    static constexpr int kBytes = 4;
+    __device__ static word init() { return 0x00007fffu; }  // (SHORT_NAN 32767, 0)
+    __device__ static float tsdf(word w) { return (float)(int16_t)(w & 0xffffu) / 32766.0f; }
+    __device__ static int weight(word w) { return (int16_t)(w >> 16); }
+    __device__ static word pack(float f, int wt) {
+        const int16_t q = (int16_t)(int)(f * 32766.0f);
+        return (uint32_t)(uint16_t)q | ((uint32_t)(uint16_t)wt << 16);
+    }
+};
+// FP16 tsdf (round to nearest even) + uint16 weight.
+struct StF16 {
+    using word = uint32_t;
+    static constexpr int kBytes = 4;
+    __device__ static word init() { return (uint32_t)__half_as_ushort(__float2half_rn(1.0f)); }
+    __device__ static float tsdf(word w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
+    __device__ static int weight(word w) { return (int)(w >> 16); }
+    __device__ static word pack(float f, int wt) {
+        return (uint32_t)__half_as_ushort(__float2half_rn(f)) | ((uint32_t)(uint16_t)wt << 16);
+    }
+};
+// FP32 tsdf + FP32 weight.
+struct StF32 {
+    using word = uint2;
+    static constexpr int kBytes = 8;
+    __device__ static word init() { return make_uint2(__float_as_uint(1.0f), 0u); }
+    __device__ static float tsdf(word w) { return __uint_as_float(w.x); }
+    __device__ static int weight(word w) { return (int)__uint_as_float(w.y); }
+    __device__ static word pack(float f, int wt) { return make_uint2(__float_as_uint(f), __float_as_uint((float)wt)); }
+};
+
+constexpr int kMaxWeight = 128;  // K/tsdfVolume.cl:4
+
+template <class S>
+__device__ __forceinline__ typename S::word update_voxel(typename S::word old, float tsdf) {
+    const float prev = S::tsdf(old);
+    const int w = S::weight(old);
+    const float fnew = (prev * (float)w + 1.0f * tsdf) / (float)(w + 1);
+    return S::pack(fnew, min(w + 1, kMaxWeight));
+}
+
+__device__ __forceinline__ uchar4 update_color(uchar4 vc, uchar4 nc, int maxcw) {
+    const float Wrk = 1.f;
+    const float nx = ((float)(vc.x * vc.w) + Wrk * (float)nc.x) / ((float)vc.w + Wrk);
+    const float ny = ((float)(vc.y * vc.w) + Wrk * (float)nc.y) / ((float)vc.w + Wrk);
+    const float nz = ((float)(vc.z * vc.w) + Wrk * (float)nc.z) / ((float)vc.w + Wrk);
+    const unsigned char wn = (unsigned char)(vc.w + 1);
+    uchar4 r;
+    r.x = (unsigned char)(int)fminf(fmaxf(nx, 0.0f), 255.f);
+    r.y = (unsigned char)(int)fminf(fmaxf(ny, 0.0f), 255.f);
+    r.z = (unsigned char)(int)fminf(fmaxf(nz, 0.0f), 255.f);
+    r.w = (unsigned char)min(maxcw, (int)wn);
+    return r;
+}
+
+// ---- reset ------------------------------------------------------------------
+
+template <class S>
+__global__ void reset_kernel(typename S::word* __restrict__ vol, uint32_t* __restrict__ cvol, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const typename S::word w0 = S::init();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        vol[i] = w0;
+        if (cvol) cvol[i] = 0u;
+    }
+}
+
+// ---- integration ---------------------------------------------------------------
+
+constexpr int kZChunk = 64;
+constexpr int kLdsPitch = kZChunk + 1;
+
+struct ColumnWalk {  // the reference's per-column incremental state
+    float vgz, part, vx, vy, vz, zs, incx, incy;
+};
+
+__device__ __forceinline__ ColumnWalk column_setup(int x, int y, int Nx, int Ny, const TsdfParams& p,
+                                                   const NuiCLCameraParams& cam, const NuiCLRigidTransform& g) {
+    ColumnWalk c;
+    const float vgx = ((float)(x - Nx / 2) + 0.5f) * p.cell_size[0] - g.t[0];
+    const float vgy = ((float)(y - Ny / 2) + 0.5f) * p.cell_size[1] - g.t[1];
+    c.vgz = (0.5f) * p.cell_size[2] - g.t[2];
+    c.part = vgx * vgx + vgy * vgy;
+    c.vx = (g.R_inv[0] * vgx + g.R_inv[1] * vgy + g.R_inv[2] * c.vgz) * cam.fx;
+    c.vy = (g.R_inv[3] * vgx + g.R_inv[4] * vgy + g.R_inv[5] * c.vgz) * cam.fy;
+    c.vz = (g.R_inv[6] * vgx + g.R_inv[7] * vgy + g.R_inv[8] * c.vgz);
+    c.zs = 0;
+    c.incx = g.R_inv[2] * p.cell_size[2] * cam.fx;
+    c.incy = g.R_inv[5] * p.cell_size[2] * cam.fy;
+    return c;
+}
+
+// One step of K/tsdfVolume.cl:74-120 for the current z; returns the new tsdf
+// sample or NaN when the reference skips the voxel; *pix receives the pixel
+// for colour integration (|sdf| < trunc) or -1.
+__device__ __forceinline__ float column_step(const ColumnWalk& c, const float* __restrict__ depth,
+                                             const NuiCLCameraParams& cam, float R8, float trunc, float tr_inv,
+                                             int* pix) {
+    *pix = -1;
+    const float inv_z = 1.0f / (c.vz + R8 * c.zs);
+    if (inv_z < 0.f) return kNaN;
+    const int u = to_int(roundf(c.vx * inv_z + cam.cx));
+    const int v = to_int(roundf(c.vy * inv_z + cam.cy));
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    if (!(u >= 0 && v >= 0 && u < W && v < H)) return kNaN;
+    const int id = v * W + u;
+    const float Dp = depth[id];
+    const float sdf = Dp - sqrtf(c.vgz * c.vgz + c.part);
+    if (isnan(Dp) || !(sdf >= -trunc)) return kNaN;
+    if (fabsf(sdf) < trunc) *pix = id;
+    return fminf(1.0f, sdf * tr_inv);
+}
+
+__device__ __forceinline__ void column_advance(ColumnWalk& c, float cz) {
+    c.vgz += cz;
+    c.zs += cz;
+    c.vx += c.incx;
+    c.vy += c.incy;
+}
+
+// Fast path: Nx, Ny multiples of 8, Nz multiple of 64.
+template <class S, bool COLOR>
+__global__ void __launch_bounds__(64)
+    integrate_tile_kernel(const float* __restrict__ depth, const uchar4* __restrict__ bgra,
+                          typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
+                          NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
+                          const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated) {
+    if (gate && *gate == 0) return;
+    using word = typename S::word;
+    constexpr int VPL = 16 / S::kBytes;      // voxels per lane per 16-byte vector
+    constexpr int LPC = kZChunk / VPL;        // lanes covering one column chunk
+    constexpr int CPI = 64 / LPC;             // columns per vector instruction
+    constexpr int NIT = 64 / CPI;             // instructions per chunk
+    constexpr int BATCH = 8;
+    __shared__ float ts[64 * kLdsPitch];
+    __shared__ int cp[COLOR ? 64 * kLdsPitch : 1];
+
+    const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
+    const int lane = threadIdx.x;
+    const int x = blockIdx.x * 8 + (lane & 7), y = blockIdx.y * 8 + (lane >> 3);
+    const NuiCLRigidTransform g = *pose;
+    ColumnWalk c = column_setup(x, y, Nx, Ny, p, cam, g);
+    const float trunc = p.tranc_dist, tr_inv = 1.f / p.tranc_dist, cz = p.cell_size[2], R8 = g.R_inv[8];
+
+    const int sub = lane % LPC;  // position inside a column chunk (phase 2)
+    unsigned nupd = 0;
+    for (int zc = 0; zc < Nz; zc += kZChunk) {
+        // phase 1: serial walk of this lane's column
+        bool any = false;
+        for (int k = 0; k < kZChunk; ++k) {
+            int pix;
+            const float v = column_step(c, depth, cam, R8, trunc, tr_inv, &pix);
+            ts[lane * kLdsPitch + k] = v;
+            if (COLOR) cp[lane * kLdsPitch + k] = pix;
+            any |= !isnan(v);
+            nupd += !isnan(v);
+            column_advance(c, cz);
+        }
+        if (!__any(any)) continue;  // the whole tile misses the frustum / band in this chunk
+        __syncthreads();
+        // phase 2: coalesced read-modify-write, CPI columns per instruction
+        for (int it0 = 0; it0 < NIT; it0 += BATCH) {
+            word buf[BATCH][VPL];
+            bool act[BATCH];
+#pragma unroll
+            for (int j = 0; j < BATCH; ++j) {
+                const int col = (it0 + j) * CPI + lane / LPC;
+                const float* tv = &ts[col * kLdsPitch + sub * VPL];
+                bool a = false;
+#pragma unroll
+                for (int q = 0; q < VPL; ++q) a |= !isnan(tv[q]);
+                act[j] = a;
+                if (a) {
+                    const int gx = blockIdx.x * 8 + (col & 7), gy = blockIdx.y * 8 + (col >> 3);
+                    const size_t base = ((size_t)gy * Nx + gx) * Nz + zc + sub * VPL;
+                    const uint4 raw = *reinterpret_cast<const uint4*>(vol + base);
+                    const word* wv = reinterpret_cast<const word*>(&raw);
+#pragma unroll
+                    for (int q = 0; q < VPL; ++q) buf[j][q] = wv[q];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < BATCH; ++j) {
+                if (!act[j]) continue;
+                const int col = (it0 + j) * CPI + lane / LPC;
+                const float* tv = &ts[col * kLdsPitch + sub * VPL];
+                const int gx = blockIdx.x * 8 + (col & 7), gy = blockIdx.y * 8 + (col >> 3);
+                const size_t base = ((size_t)gy * Nx + gx) * Nz + zc + sub * VPL;
+                uint4 outv;
+                word* ov = reinterpret_cast<word*>(&outv);
+#pragma unroll
+                for (int q = 0; q < VPL; ++q) ov[q] = isnan(tv[q]) ? buf[j][q] : update_voxel<S>(buf[j][q], tv[q]);
+                *reinterpret_cast<uint4*>(vol + base) = outv;
+                if (COLOR) {
+                    const int* pv = &cp[col * kLdsPitch + sub * VPL];
+#pragma unroll
+                    for (int q = 0; q < VPL; ++q) {
+                        const int pix = pv[q];
+                        if (isnan(tv[q]) || pix < 0) continue;
+                        const uchar4 nc = bgra[pix];
+                        if (nc.w == 0) continue;
+                        cvol[base + q] = update_color(cvol[base + q], nc, maxcw);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (updated) {
+        for (int off = 32; off > 0; off >>= 1) nupd += __shfl_xor(nupd, off);
+        if (lane == 0 && nupd) atomicAdd(updated, (unsigned long long)nupd);
+    }
+}
+
+// Generic path (any resolution): the reference's one-column-per-thread walk.
+template <class S, bool COLOR>
+__global__ void integrate_column_kernel(const float* __restrict__ depth, const uchar4* __restrict__ bgra,
+                                        typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
+                                        NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
+                                        int maxcw, const int32_t* __restrict__ gate,
+                                        unsigned long long* __restrict__ updated) {
+    if (gate && *gate == 0) return;
+    const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= Nx || y >= Ny) return;
+    const NuiCLRigidTransform g = *pose;
+    ColumnWalk c = column_setup(x, y, Nx, Ny, p, cam, g);
+    const float trunc = p.tranc_dist, tr_inv = 1.f / p.tranc_dist, cz = p.cell_size[2], R8 = g.R_inv[8];
+    const size_t base = ((size_t)y * Nx + x) * Nz;
+    unsigned nupd = 0;
+    for (int z = 0; z < Nz; ++z, column_advance(c, cz)) {
+        int pix;
+        const float v = column_step(c, depth, cam, R8, trunc, tr_inv, &pix);
+        if (isnan(v)) continue;
+        ++nupd;
+        vol[base + z] = update_voxel<S>(vol[base + z], v);
+        if (COLOR && pix >= 0) {
+            const uchar4 nc = bgra[pix];
+            if (nc.w != 0) cvol[base + z] = update_color(cvol[base + z], nc, maxcw);
+        }
+    }
+    if (updated && nupd) atomicAdd(updated, (unsigned long long)nupd);
+}
+
+// ---- raycast -------------------------------------------------------------------
+
+template <class S> struct VolView {
+    const typename S::word* __restrict__ v;
+    int Nx, Ny, Nz;
+    __device__ __forceinline__ float sample(long long idx) const {
+        const typename S::word w = v[idx];
+        return S::weight(w) > 0 ? S::tsdf(w) : kNaN;
+    }
+};
+
+__device__ __forceinline__ void get_voxel(f3 pt, const TsdfParams& p, int g[3]) {
+    g[0] = to_int(roundf(pt.x / p.cell_size[0]));
+    g[1] = to_int(roundf(pt.y / p.cell_size[1]));
+    g[2] = to_int(roundf(pt.z / p.cell_size[2]));
+}
+
+// K/raycast.cl:36-120 interpolateTrilineary, corner weights exactly as written
+// (the x and y fractions land on swapped corners relative to the y-major layout).
+template <class S> __device__ float trilinear(f3 pt, const VolView<S>& vv, const TsdfParams& p) {
+    int g[3];
+    get_voxel(pt, p, g);
+    if (g[0] <= 0 || g[0] >= vv.Nx - 1) return kNaN;
+    if (g[1] <= 0 || g[1] >= vv.Ny - 1) return kNaN;
+    if (g[2] <= 0 || g[2] >= vv.Nz - 1) return kNaN;
+    const float vx = ((float)g[0] + 0.5f) * p.cell_size[0];
+    const float vy = ((float)g[1] + 0.5f) * p.cell_size[1];
+    const float vz = ((float)g[2] + 0.5f) * p.cell_size[2];
+    g[0] = pt.x < vx ? g[0] - 1 : g[0];
+    g[1] = pt.y < vy ? g[1] - 1 : g[1];
+    g[2] = pt.z < vz ? g[2] - 1 : g[2];
+    const float a = (pt.x - ((float)g[0] + 0.5f) * p.cell_size[0]) / p.cell_size[0];
+    const float b = (pt.y - ((float)g[1] + 0.5f) * p.cell_size[1]) / p.cell_size[1];
+    const float c = (pt.z - ((float)g[2] + 0.5f) * p.cell_size[2]) / p.cell_size[2];
+    const long long sx = vv.Nz, sy = (long long)vv.Nx * vv.Nz;
+    const long long idx = ((long long)g[1] * vv.Nx + g[0]) * vv.Nz + g[2];
+    const float f000 = vv.sample(idx);
+    if (isnan(f000)) return kNaN;
+    const float f001 = vv.sample(idx + 1);
+    if (isnan(f001)) return kNaN;
+    const float f011 = vv.sample(idx + sx + 1);
+    if (isnan(f011)) return kNaN;
+    const float f010 = vv.sample(idx + sx);
+    if (isnan(f010)) return kNaN;
+    const float f110 = vv.sample(idx + sx + sy);
+    if (isnan(f110)) return kNaN;
+    const float f111 = vv.sample(idx + sx + sy + 1);
+    if (isnan(f111)) return kNaN;
+    const float f101 = vv.sample(idx + sy + 1);
+    if (isnan(f101)) return kNaN;
+    const float f100 = vv.sample(idx + sy);
+    if (isnan(f100)) return kNaN;
+    return f000 * (1 - a) * (1 - b) * (1 - c) + f001 * (1 - a) * (1 - b) * c + f010 * (1 - a) * b * (1 - c) +
+           f011 * (1 - a) * b * c + f100 * a * (1 - b) * (1 - c) + f101 * a * (1 - b) * c +
+           f110 * a * b * (1 - c) + f111 * a * b * c;
+}
+
+__device__ __forceinline__ float min_time(const float* vmax, f3 o, f3 d) {
+    const float tx = fabs((double)d.x) < 1e-14 ? 0.f : ((d.x > 0 ? 0.f : vmax[0]) - o.x) / d.x;
+    const float ty = fabs((double)d.y) < 1e-14 ? 0.f : ((d.y > 0 ? 0.f : vmax[1]) - o.y) / d.y;
+    const float tz = fabs((double)d.z) < 1e-14 ? 0.f : ((d.z > 0 ? 0.f : vmax[2]) - o.z) / d.z;
+    return fmaxf(fmaxf(tx, ty), tz);
+}
+__device__ __forceinline__ float max_time(const float* vmax, f3 o, f3 d) {
+    const float tx = fabs((double)d.x) < 1e-14 ? 1e14f : ((d.x > 0 ? vmax[0] : 0.f) - o.x) / d.x;
+    const float ty = fabs((double)d.y) < 1e-14 ? 1e14f : ((d.y > 0 ? vmax[1] : 0.f) - o.y) / d.y;
+    const float tz = fabs((double)d.z) < 1e-14 ? 1e14f : ((d.z > 0 ? vmax[2] : 0.f) - o.z) / d.z;
+    return fminf(fminf(tx, ty), tz);
+}
+
+template <class S>
+__global__ void __launch_bounds__(64)
+    raycast_kernel(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol, TsdfParams p,
+                   NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
+                   float* __restrict__ nmap, uchar4* __restrict__ cmap, const int32_t* __restrict__ enable) {
+    if (enable && *enable == 0) return;
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    if (x >= W || y >= H) return;
+    const size_t id = (size_t)y * W + x;
+    const f3 NaN3 = mk3(kNaN, kNaN, kNaN);
+    VolView<S> vv{vol, (int)p.resolution[0], (int)p.resolution[1], (int)p.resolution[2]};
+    const NuiCLRigidTransform g = *pose;
+    const f3 off = mk3(p.dimension[0] / 2.0f, p.dimension[1] / 2.0f, 0.f);
+    const f3 rs = add3(mk3(g.t[0], g.t[1], g.t[2]), off);
+    f3 rn = mk3(((float)x - cam.cx) * cam.fx_inv, ((float)y - cam.cy) * cam.fy_inv, 1.f);
+    rn = add3(add3(rot3(g.R, rn), mk3(g.t[0], g.t[1], g.t[2])), off);
+    const f3 dir = normalize3(sub3(rn, rs));
+    float tcur = min_time(p.dimension, rs, dir);
+    const float texit = max_time(p.dimension, rs, dir);
+    tcur = fmaxf(tcur, 0.f);
+    f3 vout = NaN3, nout = NaN3;
+    uchar4 cout = make_uchar4(0, 0, 0, 0);
+    if (tcur < texit) {
+        int gv[3];
+        get_voxel(add3(rs, mul3(dir, tcur)), p, gv);
+        gv[0] = max(0, min(gv[0], vv.Nx - 1));
+        gv[1] = max(0, min(gv[1], vv.Ny - 1));
+        gv[2] = max(0, min(gv[2], vv.Nz - 1));
+        long long idx = ((long long)gv[1] * vv.Nx + gv[0]) * vv.Nz + gv[2];
+        float tsdf = vv.sample(idx);
+        const float max_t = 3.f * (p.dimension[0] + p.dimension[1] + p.dimension[2]);
+        const float step = p.tranc_dist * 0.8f;
+        for (; tcur < max_t; tcur += step) {
+            const float tprev = tsdf;
+            get_voxel(add3(rs, mul3(dir, tcur + step)), p, gv);
+            if (!(gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny && gv[2] < vv.Nz)) break;
+            idx = ((long long)gv[1] * vv.Nx + gv[0]) * vv.Nz + gv[2];
+            tsdf = vv.sample(idx);
+            if (isnan(tsdf) || isnan(tprev)) continue;
+            if (tprev < 0.f && tsdf > 0.f) break;
+            if (tprev >= 0.f && tsdf <= 0.f) {
+                const float Ftdt = trilinear<S>(add3(rs, mul3(dir, tcur + step)), vv, p);
+                if (isnan(Ftdt)) break;
+                const float Ft = trilinear<S>(add3(rs, mul3(dir, tcur)), vv, p);
+                if (isnan(Ft)) break;
+                const float Ts = tcur - step * Ft / (Ftdt - Ft);
+                const f3 vf = add3(rs, mul3(dir, Ts));
+                vout = sub3(vf, off);
+                if (cvol && cmap) cout = cvol[idx];
+                int gg[3];
+                get_voxel(add3(rs, mul3(dir, tcur)), p, gg);
+                if (gg[0] > 1 && gg[1] > 1 && gg[2] > 1 && gg[0] < vv.Nx - 2 && gg[1] < vv.Ny - 2 && gg[2] < vv.Nz - 2) {
+                    f3 n, q;
+                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S>(q, vv, p);
+                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S>(q, vv, p);
+                    n.x = fx1 - fx2;
+                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S>(q, vv, p);
+                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S>(q, vv, p);
+                    n.y = fy1 - fy2;
+                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S>(q, vv, p);
+                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S>(q, vv, p);
+                    n.z = fz1 - fz2;
+                    nout = normalize3(n);
+                }
+                break;
+            }
+        }
+    }
+    st3(vmap, id, vout);
+    st3(nmap, id, nout);
+    if (cmap) cmap[id] = cout;
+}
+
+// ---- host launchers ------------------------------------------------------------
+
+template <class S>
+int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
+                     const TsdfParams& p, const NuiCLCameraParams& cam, const NuiCLRigidTransform* pose, int maxcw,
+                     const int32_t* gate, unsigned long long* updated, hipStream_t st) {
+    const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
+    const bool color = bgra && color_volume;
+    auto* vol = reinterpret_cast<typename S::word*>(volume);
+    auto* cv = reinterpret_cast<uchar4*>(color_volume);
+    auto* bg = reinterpret_cast<const uchar4*>(bgra);
+    if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0) {
+        dim3 grid(Nx / 8, Ny / 8);
+        if (color)
+            integrate_tile_kernel<S, true><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated);
+        else
+            integrate_tile_kernel<S, false><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated);
+        NUIF_LAUNCHED("integrate_tile_kernel");
+    } else {
+        dim3 block(16, 16), grid(div_up(Nx, 16), div_up(Ny, 16));
+        if (color)
+            integrate_column_kernel<S, true><<<grid, block, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated);
+        else
+            integrate_column_kernel<S, false><<<grid, block, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated);
+        NUIF_LAUNCHED("integrate_column_kernel");
+    }
+    return NUIF_OK;
+}
+
+template <class S>
+int launch_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams& p,
+                   const NuiCLCameraParams& cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
+                   uint8_t* cmap, const int32_t* enable, hipStream_t st) {
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    dim3 grid(div_up(W, 8), div_up(H, 8));
+    raycast_kernel<S><<<grid, 64, 0, st>>>(reinterpret_cast<const typename S::word*>(volume),
+                                           reinterpret_cast<const uchar4*>(color_volume), p, cam, pose, vmap, nmap,
+                                           reinterpret_cast<uchar4*>(cmap), enable);
+    NUIF_LAUNCHED("raycast_kernel");
+    return NUIF_OK;
+}
+
+static int check_params(const TsdfParams* p) {
+    NUIF_CHECK_ARG(p != nullptr, "null TsdfParams");
+    for (int k = 0; k < 3; ++k) {
+        const float r = p->resolution[k];
+        NUIF_CHECK_ARG(r >= 1.f && r <= 4096.f && r == (float)(int)r, "bad resolution %g", (double)r);
+        NUIF_CHECK_ARG(p->cell_size[k] > 0.f && p->dimension[k] > 0.f, "bad cell size / dimension");
+    }
+    NUIF_CHECK_ARG(p->tranc_dist > 0.f, "bad truncation distance");
+    return NUIF_OK;
+}
+
+}  // namespace nuif
+
+using namespace nuif;
+
+extern "C" {
+
+size_t nuif_tsdf_voxel_bytes(int storage) {
+    switch (storage) {
+        case NUIF_TSDF_S16: return 4;
+        case NUIF_TSDF_F16: return 4;
+        case NUIF_TSDF_F32: return 8;
+        default: return 0;
+    }
+}
+
+int nuif_tsdf_params(int nx, int ny, int nz, float dx, float dy, float dz, float trunc, TsdfParams* out) {
+    NUIF_CHECK_ARG(out && nx > 0 && ny > 0 && nz > 0 && dx > 0 && dy > 0 && dz > 0, "nuif_tsdf_params: bad arguments");
+    out->resolution[0] = (float)nx;
+    out->resolution[1] = (float)ny;
+    out->resolution[2] = (float)nz;
+    out->dimension[0] = dx;
+    out->dimension[1] = dy;
+    out->dimension[2] = dz;
+    out->cell_size[0] = dx / (float)nx;
+    out->cell_size[1] = dy / (float)ny;
+    out->cell_size[2] = dz / (float)nz;
+    const float c = std::max(out->cell_size[0], std::max(out->cell_size[1], out->cell_size[2]));
+    out->tranc_dist = std::max(trunc, 2.1f * c);  // OCL/NuiKinfuOpenCLScene.cpp:41 (Q16)
+    return NUIF_OK;
+}
+
+int nuif_tsdf_reset(void* volume, uint8_t* color_volume, const TsdfParams* params, int storage, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(params);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(volume != nullptr, "nuif_tsdf_reset: null volume");
+    const size_t n = (size_t)params->resolution[0] * (size_t)params->resolution[1] * (size_t)params->resolution[2];
+    const unsigned blocks = std::min<unsigned>(div_up(n, 256), 65536u);
+    hipStream_t st = stream_of(s);
+    uint32_t* cv = reinterpret_cast<uint32_t*>(color_volume);
+    switch (storage) {
+        case NUIF_TSDF_S16: reset_kernel<StS16><<<blocks, 256, 0, st>>>((uint32_t*)volume, cv, n); break;
+        case NUIF_TSDF_F16: reset_kernel<StF16><<<blocks, 256, 0, st>>>((uint32_t*)volume, cv, n); break;
+        case NUIF_TSDF_F32: reset_kernel<StF32><<<blocks, 256, 0, st>>>((uint2*)volume, cv, n); break;
+        default: set_error("nuif_tsdf_reset: unknown storage %d", storage); return NUIF_ERR_INVALID;
+    }
+    NUIF_LAUNCHED("reset_kernel");
+    return NUIF_OK;
+}
+
+int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
+                        const TsdfParams* params, const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose,
+                        int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
+                        nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(params);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(depth && volume && cam && pose, "nuif_tsdf_integrate: null argument");
+    NUIF_CHECK_ARG(cam->depthImageWidth > 0 && cam->depthImageHeight > 0, "nuif_tsdf_integrate: empty image");
+    hipStream_t st = stream_of(s);
+    switch (storage) {
+        case NUIF_TSDF_S16:
+            return launch_integrate<StS16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, st);
+        case NUIF_TSDF_F16:
+            return launch_integrate<StF16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, st);
+        case NUIF_TSDF_F32:
+            return launch_integrate<StF32>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, st);
+        default: set_error("nuif_tsdf_integrate: unknown storage %d", storage); return NUIF_ERR_INVALID;
+    }
+}
+
+int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
+                            const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
+                            uint8_t* cmap, int storage, const int32_t* enable, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(params);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(volume && cam && pose && vmap && nmap, "nuif_tsdf_raycast: null argument");
+    NUIF_CHECK_ARG(cam->depthImageWidth > 0 && cam->depthImageHeight > 0, "nuif_tsdf_raycast: empty image");
+    hipStream_t st = stream_of(s);
+    switch (storage) {
+        case NUIF_TSDF_S16: return launch_raycast<StS16>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable, st);
+        case NUIF_TSDF_F16: return launch_raycast<StF16>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable, st);
+        case NUIF_TSDF_F32: return launch_raycast<StF32>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable, st);
+        default: set_error("nuif_tsdf_raycast: unknown storage %d", storage); return NUIF_ERR_INVALID;
+    }
+}
+
+int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
+                      const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
+                      uint8_t* cmap, int storage, nuif_stream_t s) {
+    return nuif_tsdf_raycast_gated(volume, color_volume, params, cam, pose, vmap, nmap, cmap, storage, nullptr, s);
+}
+
+}  // extern "C"

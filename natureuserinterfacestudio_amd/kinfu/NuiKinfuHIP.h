@@ -1,0 +1,263 @@
+// GPU implementations of the fusion interfaces on top of the nuif C ABI
+// (replacing OCL/NuiKinfuOpenCL{Frame,FeedbackFrame,CameraState,DepthTracker,
+// Scene}.*).  Every object of one tracking session shares the session's HIP
+// stream; nothing is global.  All device work is enqueued asynchronously; the
+// engine decides where to synchronise.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nuif.h"
+#include "NuiKinfuConfig.h"
+#include "NuiKinfuInterfaces.h"
+
+namespace nuif_kinfu {
+
+// RAII device buffer (replaces NuiGPUMemManager::CreateBufferCL / ReleaseMemObjectCL)
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t bytes);
+    ~DeviceBuffer();
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
+    void* get() const { return p_; }
+    template <class T> T* as() const { return static_cast<T*>(p_); }
+    size_t bytes() const { return n_; }
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Throws std::runtime_error with nuif_last_error() on a non-zero status.
+void check(int rc, const char* what);
+
+// OCL/NuiKinfuOpenCLCameraState.*: device camera params + rigid transform
+class NuiKinfuHIPCameraState : public NuiKinfuCameraState {
+public:
+    explicit NuiKinfuHIPCameraState(nuif_stream_t s);
+    ~NuiKinfuHIPCameraState() override;
+    void UpdateCameraTransform(const Matrix3frm& rot, const Vector3f& tran) override;
+    void UpdateCameraTransform(const Matrix3frm& rot, const Vector3f& tran, const Vector3f& offsetTran) override;
+    void UpdateCameraParams(const NuiCameraParams& cameraParams, UINT nWidth, UINT nHeight) override;
+    // host pose <- device pose (after asynchronous tracking)
+    void SyncFromDevice();
+    NuiCLRigidTransform* GetCameraTransformBuffer() const { return m_pose.as<NuiCLRigidTransform>(); }
+    const NuiCLCameraParams& GetCameraParams() const { return m_cam; }
+
+private:
+    void WriteToBuffer(const Matrix3frm& rot, const Vector3f& tran);
+    nuif_stream_t m_stream;
+    DeviceBuffer m_pose;
+    NuiCLRigidTransform* m_staging = nullptr;  // pinned
+    NuiCLCameraParams m_cam{};
+};
+
+// OCL/NuiKinfuOpenCLFrame.*: raw depth upload, passing + bilateral filter,
+// vertex pyramid (the pyramid of OCL/NuiKinfuOpenCLDepthTracker.cpp:140-236 is
+// produced here in the same fused preprocessing pass).
+class NuiKinfuHIPFrame : public NuiKinfuFrame {
+public:
+    NuiKinfuHIPFrame(const NuiTrackerConfig& config, UINT nWidth, UINT nHeight, nuif_stream_t s);
+    ~NuiKinfuHIPFrame() override;
+    void UpdateVertexBuffers(UINT16* pDepths, UINT nNum, NuiKinfuCameraState* pCameraState) override;
+    void UpdateColorBuffers(BGRQUAD* pColors, UINT nNum) override;
+    // device-resident inputs (no host copy)
+    void UpdateVertexBuffersDevice(const UINT16* dDepths, NuiKinfuCameraState* pCameraState);
+    void UpdateColorBuffersDevice(const uint8_t* dBgra);
+    UINT GetWidth() const override { return m_W; }
+    UINT GetHeight() const override { return m_H; }
+    int GetLevels() const { return m_levels; }
+    // the scene needing ray-length depth (dense TSDF) sets this before the first frame
+    void SetScaleDepthForTsdf(bool v) { m_scaleForTsdf = v; }
+    bool IsDepthScaled() const { return m_scaleForTsdf; }
+    float* GetDepthBuffer() const { return m_depth.as<float>(); }
+    float* GetFilteredDepthBuffer() const { return m_dpyr[0].as<float>(); }
+    float* GetVertexBuffer(int level = 0) const { return m_vpyr[level].as<float>(); }
+    const float* const* GetVertexPyramid() const { return m_vptrs.data(); }
+    uint8_t* GetColorBuffer() const { return m_hasColor ? m_colors.as<uint8_t>() : nullptr; }
+    UINT16* GetRawBuffer() const { return m_raw.as<UINT16>(); }
+    float GetDepthThreshold() const { return m_cfg.depth_threshold; }
+
+private:
+    void Preprocess(NuiKinfuCameraState* pCameraState);
+    NuiTrackerConfig m_cfg;
+    UINT m_W, m_H;
+    int m_levels;
+    nuif_stream_t m_stream;
+    bool m_scaleForTsdf = false;
+    bool m_hasColor = false;
+    DeviceBuffer m_raw, m_depth, m_colors;
+    std::vector<DeviceBuffer> m_dpyr, m_vpyr;
+    std::vector<float*> m_dptrs;
+    std::vector<const float*> m_vptrs;
+    std::vector<float> m_gaussian;
+};
+
+// OCL/NuiKinfuOpenCLFeedbackFrame.*: the reference (model) maps used by ICP
+class NuiKinfuHIPFeedbackFrame : public NuiKinfuFeedbackFrame {
+public:
+    NuiKinfuHIPFeedbackFrame(UINT nWidth, UINT nHeight, nuif_stream_t s);
+    void UpdateBuffers(NuiKinfuFrame* frame, NuiKinfuCameraState* pCameraState) override;
+    UINT GetWidth() const override { return m_W; }
+    UINT GetHeight() const override { return m_H; }
+    float* GetVertexBuffer() const { return m_vmap.as<float>(); }
+    float* GetNormalBuffer() const { return m_nmap.as<float>(); }
+    uint8_t* GetColorBuffer() const { return m_colors.as<uint8_t>(); }
+
+private:
+    UINT m_W, m_H;
+    nuif_stream_t m_stream;
+    DeviceBuffer m_vmap, m_nmap, m_colors;
+};
+
+// OCL/NuiKinfuOpenCLDepthTracker.*: pyramid projective ICP, all on device
+class NuiKinfuHIPDepthTracker : public NuiKinfuTracker {
+public:
+    NuiKinfuHIPDepthTracker(const NuiTrackerConfig& config, UINT nWidth, UINT nHeight, nuif_stream_t s);
+    bool EstimatePose(NuiKinfuFrame* pFrame, NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState,
+                      Affine3f* hint) override;
+    void log(const std::string& fileName) const override { m_cfg.log(fileName); }
+    float getError() const override;
+    int getCount() const override;
+    // tracking state (device) and its flags, consumed by the scene
+    void* GetState() const { return m_state.get(); }
+    const int32_t* GateFlag() const { return nuif_icp_gate_flag(m_state.get()); }
+    const int32_t* OkFlag() const { return nuif_icp_ok_flag(m_state.get()); }
+    void ResetState(NuiKinfuHIPCameraState* cam);
+    // frame without ICP (first frame): records the pose and advances the frame count
+    void RecordPose(NuiKinfuHIPCameraState* cam);
+    void SetIntegrationThreshold(float v) { m_icfg.gate_threshold = v; }
+    void SetHistory(float* dHistory, int capacity) { m_history = dHistory; m_historyCap = capacity; }
+    nuif_icp_result ReadState() const;
+
+private:
+    NuiTrackerConfig m_cfg;
+    nuif_icp_config m_icfg{};
+    UINT m_W, m_H;
+    nuif_stream_t m_stream;
+    DeviceBuffer m_state, m_partials;
+    float* m_history = nullptr;
+    int m_historyCap = 0;
+};
+
+// OCL/NuiKinfuOpenCLScene.*: dense TSDF volume
+class NuiKinfuHIPScene : public NuiKinfuScene {
+public:
+    NuiKinfuHIPScene(const NuiKinfuVolumeConfig& config, int storage, nuif_stream_t s);
+    bool log(const std::string& fileName) const override { return m_config.log(fileName); }
+    bool hasColorData() const override { return m_config.bHas_color_volume; }
+    float getVoxelLeafSize() const override;
+    void reset() override;
+    bool integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) override;
+    void raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) override;
+    bool Volume2CLMesh(NuiKinfuMesh* pData) override { return false; }
+    bool Volume2Mesh(NuiKinfuMesh* pMesh) override { return false; }
+    // device predicates produced by the tracker (integration gate / tracking success)
+    void SetGates(const int32_t* integrate, const int32_t* raycast) {
+        m_gateIntegrate = integrate;
+        m_gateRaycast = raycast;
+    }
+    const TsdfParams& GetParams() const { return m_params; }
+    int GetStorage() const { return m_storage; }
+    void* GetVolume() const { return m_volume.get(); }
+    uint8_t* GetColorVolume() const { return m_color.as<uint8_t>(); }
+    size_t VolumeBytes() const { return m_volume.bytes(); }
+    // running count of voxels updated by integration since construction (synchronises)
+    unsigned long long UpdatedVoxels() const;
+
+private:
+    NuiKinfuVolumeConfig m_config;
+    int m_storage;
+    nuif_stream_t m_stream;
+    TsdfParams m_params{};
+    DeviceBuffer m_volume, m_color, m_counter;
+    const int32_t* m_gateIntegrate = nullptr;
+    const int32_t* m_gateRaycast = nullptr;
+};
+
+}  // namespace nuif_kinfu
+
+namespace NuiKinfuEngine {
+
+// DS/NuiKinfuTrackingFactory.h:23-50
+class NuiKinfuTrackingFactory {
+public:
+    static NuiKinfuTrackingFactory& Instance() {
+        static NuiKinfuTrackingFactory s;
+        return s;
+    }
+    static void BuildTrackingEngine(NuiKinfuTracker** pTracker, NuiKinfuFrame** pFrame,
+                                    NuiKinfuFeedbackFrame** pFeedbackFrame, NuiKinfuCameraState** pCameraState,
+                                    const NuiTrackerConfig& trackerConfig, bool bAcceleratedFeedback,
+                                    unsigned int nWidth, unsigned int nHeight, nuif_stream_t stream);
+};
+
+// VO/NuiKinfuTrackingEngine.*: per-frame orchestration
+class NuiKinfuTrackingEngine {
+public:
+    explicit NuiKinfuTrackingEngine(nuif_stream_t stream = nullptr);
+    ~NuiKinfuTrackingEngine();
+    bool isInit() const { return m_pTracker != nullptr; }
+    void initialize(const NuiTrackerConfig& trackerConfig, bool bAcceleratedFeedback, UINT nWidth, UINT nHeight);
+    void log(const std::string& fileName) const;
+    void setTranslateBasis(const Vector3f& basis) { m_translateBasis = basis; }
+    const Vector3f& getTranslateBasis() const { return m_translateBasis; }
+    void reset();
+    // VO/NuiKinfuTrackingEngine.cpp:68-142; synchronous: returns the tracking outcome
+    bool RunTracking(INT64 timeStamp, UINT16* pDepths, BGRQUAD* pColors, UINT nPointNum, NuiKinfuScene* pVolume,
+                     const NuiCameraParams& cameraParams);
+    // Device-resident inputs; sync=false enqueues the frame and returns at once
+    // (outcome via getStatus()); with useGraph the frame pipeline is replayed
+    // from a captured hipGraph after its first execution.
+    bool RunTrackingDevice(INT64 timeStamp, const UINT16* dDepths, const uint8_t* dBgra, NuiKinfuScene* pVolume,
+                           const NuiCameraParams& cameraParams, bool sync);
+    void setUseGraph(bool v) { m_useGraph = v; }
+    bool synchronize();  // waits for queued frames; false if any tracking failure occurred
+    const NuiCameraPos& getCameraPose(int time = -1) const;
+    float getTrackerError() const;
+    int getTrackerCount() const;
+    int getFrameID() const { return m_frames; }
+    void setIntegrationMetricThreshold(float threshold);
+    nuif_stream_t stream() const { return m_stream; }
+    nuif_kinfu::NuiKinfuHIPFeedbackFrame* feedbackFrame() const { return m_pFeedbackFrame; }
+    nuif_kinfu::NuiKinfuHIPDepthTracker* tracker() const { return m_pTracker; }
+    const float* poseHistoryDevice() const { return m_history.as<float>(); }
+    int poseHistoryCapacity() const { return m_historyCap; }
+    // Per-stage device timing of tracked frames (events around preprocessing, ICP,
+    // integration and raycast on the session stream; frames then run un-captured).
+    void setStageTiming(bool on);
+    // Sums of the stage durations (ms) of the frames timed since the last call,
+    // and their number; synchronises the stream.
+    int takeStageTimes(double ms[4]);
+
+private:
+    void enqueueTrackedFrame(nuif_kinfu::NuiKinfuHIPScene* scene);
+    void recordStage(int k);
+    bool m_stageTiming = false;
+    std::vector<nuif_event_t> m_events;  // 5 per timed frame
+    size_t m_eventsUsed = 0;
+    nuif_stream_t m_stream;
+    bool m_ownStream = false;
+    nuif_kinfu::NuiKinfuHIPDepthTracker* m_pTracker = nullptr;
+    nuif_kinfu::NuiKinfuHIPFrame* m_pFrame = nullptr;
+    nuif_kinfu::NuiKinfuHIPFeedbackFrame* m_pFeedbackFrame = nullptr;
+    nuif_kinfu::NuiKinfuHIPCameraState* m_pCameraState = nullptr;
+    Vector3f m_translateBasis;
+    float m_integrationThreshold = 0.15f;  // VO/NuiKinfuTrackingEngine.cpp:22,32
+    int m_frames = 0;
+    bool m_useGraph = false;
+    nuif_graph_t m_graph = nullptr;
+    nuif_kinfu::NuiKinfuHIPScene* m_graphScene = nullptr;
+    nuif_kinfu::DeviceBuffer m_history;
+    int m_historyCap = 1 << 16;
+    mutable std::vector<NuiCameraPos> m_poses;
+    mutable bool m_posesDirty = false;
+};
+
+}  // namespace NuiKinfuEngine

@@ -1,0 +1,430 @@
+"""ctypes binding of the C ABI (include/nuif.h) plus small device-array helpers.
+
+This is the Python counterpart of the binding a host application would add;
+the product computation all happens in libnuif.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import libkinfu, libnuif
+
+c_void_p = C.c_void_p
+c_int = C.c_int
+c_float = C.c_float
+c_size_t = C.c_size_t
+
+
+class NuiCLCameraParams(C.Structure):
+    _fields_ = [("fx", c_float), ("fy", c_float), ("fx_inv", c_float), ("fy_inv", c_float), ("cx", c_float),
+                ("cy", c_float), ("depthImageWidth", C.c_uint32), ("depthImageHeight", C.c_uint32),
+                ("sensorDepthWorldMin", c_float), ("sensorDepthWorldMax", c_float)]
+
+
+class NuiCLRigidTransform(C.Structure):
+    _fields_ = [("R", c_float * 9), ("R_inv", c_float * 9), ("t", c_float * 3)]
+
+
+class TsdfParams(C.Structure):
+    _fields_ = [("resolution", c_float * 3), ("dimension", c_float * 3), ("cell_size", c_float * 3),
+                ("tranc_dist", c_float)]
+
+
+ICP_TERMS = 29
+ICP_MAX_LEVELS = 4
+POSE_ROW = 16
+TSDF_S16, TSDF_F16, TSDF_F32 = 0, 1, 2
+
+
+class IcpResult(C.Structure):
+    _fields_ = [("R", c_float * 9), ("t", c_float * 3), ("error", c_float), ("count", c_float),
+                ("status", C.c_int32), ("iterations", C.c_int32), ("integrate", C.c_int32), ("frame", C.c_int32),
+                ("sums", C.c_double * ICP_TERMS)]
+
+
+class IcpConfig(C.Structure):
+    _fields_ = [("levels", c_int), ("iterations", c_int * ICP_MAX_LEVELS), ("dist_threshold", c_float),
+                ("gate_threshold", c_float)]
+
+
+class KinfuDesc(C.Structure):
+    _fields_ = [("width", c_int), ("height", c_int), ("fx", c_float), ("fy", c_float), ("cx", c_float),
+                ("cy", c_float), ("depth_min", c_float), ("depth_max", c_float), ("filter_radius", c_int),
+                ("sigma_space2_inv_half", c_float), ("sigma_depth2_inv_half", c_float),
+                ("depth_threshold", c_float), ("dist_threshold", c_float), ("levels", c_int),
+                ("iterations", c_int * 4), ("integration_threshold", c_float), ("resolution", c_int * 3),
+                ("dimensions", c_float * 3), ("tranc_dist", c_float), ("color", c_int), ("storage", c_int),
+                ("use_graph", c_int)]
+
+
+_SIGS = {
+    "nuif_init": (c_int, [c_int]),
+    "nuif_device_count": (c_int, [C.POINTER(c_int)]),
+    "nuif_device_info": (c_int, [C.c_char_p, c_int, C.POINTER(c_int), C.POINTER(c_size_t), C.POINTER(c_int)]),
+    "nuif_last_error": (C.c_char_p, []),
+    "nuif_version": (C.c_char_p, []),
+    "nuif_malloc": (c_int, [C.POINTER(c_void_p), c_size_t]),
+    "nuif_free": (c_int, [c_void_p]),
+    "nuif_mem_in_use": (c_int, [C.POINTER(c_size_t)]),
+    "nuif_host_alloc": (c_int, [C.POINTER(c_void_p), c_size_t]),
+    "nuif_host_free": (c_int, [c_void_p]),
+    "nuif_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "nuif_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "nuif_memcpy_d2d": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "nuif_fill_u8": (c_int, [c_void_p, C.c_uint8, c_size_t, c_void_p]),
+    "nuif_fill_u32": (c_int, [c_void_p, C.c_uint32, c_size_t, c_void_p]),
+    "nuif_stream_create": (c_int, [C.POINTER(c_void_p)]),
+    "nuif_stream_destroy": (c_int, [c_void_p]),
+    "nuif_stream_synchronize": (c_int, [c_void_p]),
+    "nuif_device_synchronize": (c_int, []),
+    "nuif_event_create": (c_int, [C.POINTER(c_void_p)]),
+    "nuif_event_destroy": (c_int, [c_void_p]),
+    "nuif_event_record": (c_int, [c_void_p, c_void_p]),
+    "nuif_event_synchronize": (c_int, [c_void_p]),
+    "nuif_event_elapsed_ms": (c_int, [c_void_p, c_void_p, C.POINTER(c_float)]),
+    "nuif_stream_wait_event": (c_int, [c_void_p, c_void_p]),
+    "nuif_graph_begin": (c_int, [c_void_p]),
+    "nuif_graph_end": (c_int, [c_void_p, C.POINTER(c_void_p)]),
+    "nuif_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "nuif_graph_destroy": (c_int, [c_void_p]),
+    "nuif_depth_passing_filter": (c_int, [c_void_p, c_void_p, c_size_t, c_float, c_float, c_void_p]),
+    "nuif_bilateral_filter_depth": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_float, c_float,
+                                            c_void_p]),
+    "nuif_half_sample": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_float, c_void_p]),
+    "nuif_depth2vertex": (c_int, [c_void_p, c_void_p, c_int, c_int, C.POINTER(NuiCLCameraParams), c_int,
+                                  c_void_p]),
+    "nuif_estimate_normals_simple": (c_int, [c_void_p, c_void_p, c_int, c_int, c_float, c_void_p]),
+    "nuif_transform_maps": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "nuif_scale_depths": (c_int, [c_void_p, c_int, c_int, C.POINTER(NuiCLCameraParams), c_void_p]),
+    "nuif_frame_preprocess": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                      C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_float, c_float, c_int,
+                                      c_void_p]),
+    "nuif_icp_state_bytes": (c_size_t, []),
+    "nuif_icp_partials_bytes": (c_size_t, [c_int, c_int]),
+    "nuif_icp_state_init": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nuif_icp_state_read": (c_int, [c_void_p, C.POINTER(IcpResult), c_void_p]),
+    "nuif_icp_begin": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nuif_icp_iteration": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_void_p,
+                                   c_void_p, c_void_p, c_float, C.c_uint32, c_int, c_void_p]),
+    "nuif_icp_end": (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_int, c_void_p]),
+    "nuif_icp_estimate": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
+                                  c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
+    "nuif_icp_gate_flag": (c_void_p, [c_void_p]),
+    "nuif_icp_ok_flag": (c_void_p, [c_void_p]),
+    "nuif_icp_count_thresholds": (c_int, [c_int, c_int, C.POINTER(IcpConfig), C.POINTER(C.c_uint32)]),
+    "nuif_tsdf_voxel_bytes": (c_size_t, [c_int]),
+    "nuif_tsdf_params": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float, c_float, C.POINTER(TsdfParams)]),
+    "nuif_tsdf_reset": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p]),
+    "nuif_tsdf_integrate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(TsdfParams),
+                                    C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                    c_void_p]),
+    "nuif_tsdf_raycast": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "nuif_tsdf_raycast_gated": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+}
+
+_KSIGS = {
+    "nuifk_last_error": (C.c_char_p, []),
+    "nuifk_create": (c_int, [C.POINTER(KinfuDesc), C.POINTER(c_void_p)]),
+    "nuifk_destroy": (c_int, [c_void_p]),
+    "nuifk_run": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int]),
+    "nuifk_synchronize": (c_int, [c_void_p]),
+    "nuifk_reset": (c_int, [c_void_p]),
+    "nuifk_poses": (c_int, [c_void_p, c_void_p, c_int]),
+    "nuifk_state": (c_int, [c_void_p, C.POINTER(IcpResult)]),
+    "nuifk_read_volume": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "nuifk_read_maps": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nuifk_stream": (c_void_p, [c_void_p]),
+    "nuifk_volume_ptr": (c_void_p, [c_void_p]),
+    "nuifk_volume_bytes": (c_size_t, [c_void_p]),
+    "nuifk_trunc": (c_float, [c_void_p]),
+    "nuifk_set_stage_timing": (c_int, [c_void_p, c_int]),
+    "nuifk_take_stage_times": (c_int, [c_void_p, C.POINTER(C.c_double)]),
+    "nuifk_updated_voxels": (c_int, [c_void_p, C.POINTER(C.c_ulonglong)]),
+    "nuifk_load_tracker_config": (c_int, [C.c_char_p, C.POINTER(KinfuDesc)]),
+    "nuifk_save_tracker_config": (c_int, [C.c_char_p, C.POINTER(KinfuDesc)]),
+}
+
+_bound = {}
+
+
+def _bind(libfn, sigs):
+    lib = libfn()
+    key = id(lib)
+    if key in _bound:
+        return lib
+    for name, (res, args) in sigs.items():
+        if not hasattr(lib, name):
+            continue
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _bound[key] = True
+    return lib
+
+
+def L():
+    return _bind(libnuif, _SIGS)
+
+
+def K():
+    L()
+    return _bind(libkinfu, _KSIGS)
+
+
+def declared_symbols():
+    """Names of every function declared in include/nuif.h that this binding knows."""
+    return sorted(_SIGS)
+
+
+class NuifError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = L().nuif_last_error()
+        raise NuifError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+    return rc
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        check(L().nuif_init(device), "nuif_init")
+        _initialised = True
+
+
+def device_info():
+    init()
+    name = C.create_string_buffer(256)
+    cus, hbm, clk = c_int(), c_size_t(), c_int()
+    check(L().nuif_device_info(name, 256, C.byref(cus), C.byref(hbm), C.byref(clk)), "nuif_device_info")
+    return {"name": name.value.decode(), "cu": cus.value, "hbm_bytes": hbm.value, "clock_khz": clk.value}
+
+
+class DeviceArray:
+    """A device allocation with a numpy shape/dtype for transfers."""
+
+    def __init__(self, shape, dtype):
+        init()
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (shape,)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = c_void_p()
+        check(L().nuif_malloc(C.byref(p), max(self.nbytes, 1)), "nuif_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.shape, arr.dtype)
+        d.upload(arr, stream)
+        return d
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes, (arr.nbytes, self.nbytes)
+        check(L().nuif_memcpy_h2d(self.ptr, arr.ctypes.data, self.nbytes, stream), "h2d")
+        check(L().nuif_stream_synchronize(stream), "sync")
+
+    def download(self, stream=None):
+        out = np.empty(self.shape, self.dtype)
+        check(L().nuif_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, stream), "d2h")
+        check(L().nuif_stream_synchronize(stream), "sync")
+        return out
+
+    def free(self):
+        if self.ptr:
+            L().nuif_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def camera_params(W, H, fx, fy, cx, cy, dmin=0.5, dmax=4.5):
+    """NuiCLCameraParams as OCL/NuiKinfuOpenCLCameraState.cpp:33-62 fills it."""
+    c = NuiCLCameraParams()
+    c.fx = fx
+    c.fy = fy
+    c.fx_inv = np.float32(1.0) / np.float32(fx)
+    c.fy_inv = np.float32(1.0) / np.float32(fy)
+    c.cx = cx
+    c.cy = cy
+    c.depthImageWidth = W
+    c.depthImageHeight = H
+    c.sensorDepthWorldMin = dmin
+    c.sensorDepthWorldMax = dmax
+    return c
+
+
+def inverse3_f32(R):
+    """Eigen-style float cofactor inverse of a row-major 3x3 (matches the device helper)."""
+    m = np.asarray(R, np.float32).reshape(9)
+    f = np.float32
+    c00 = f(m[4] * m[8]) - f(m[5] * m[7])
+    c01 = f(m[5] * m[6]) - f(m[3] * m[8])
+    c02 = f(m[3] * m[7]) - f(m[4] * m[6])
+    det = f(f(f(m[0] * c00) + f(m[1] * c01)) + f(m[2] * c02))
+    inv = f(1.0) / det
+    out = np.empty(9, np.float32)
+    out[0] = c00 * inv
+    out[3] = c01 * inv
+    out[6] = c02 * inv
+    out[1] = f(f(m[2] * m[7]) - f(m[1] * m[8])) * inv
+    out[4] = f(f(m[0] * m[8]) - f(m[2] * m[6])) * inv
+    out[7] = f(f(m[1] * m[6]) - f(m[0] * m[7])) * inv
+    out[2] = f(f(m[1] * m[5]) - f(m[2] * m[4])) * inv
+    out[5] = f(f(m[2] * m[3]) - f(m[0] * m[5])) * inv
+    out[8] = f(f(m[0] * m[4]) - f(m[1] * m[3])) * inv
+    return out
+
+
+def rigid(R, t):
+    g = NuiCLRigidTransform()
+    R = np.asarray(R, np.float32).reshape(9)
+    Ri = inverse3_f32(R)
+    for k in range(9):
+        g.R[k] = R[k]
+        g.R_inv[k] = Ri[k]
+    for k in range(3):
+        g.t[k] = float(np.float32(t[k]))
+    return g
+
+
+def upload_struct(st):
+    d = DeviceArray((C.sizeof(st),), np.uint8)
+    check(L().nuif_memcpy_h2d(d.ptr, C.addressof(st), C.sizeof(st), None), "h2d")
+    check(L().nuif_stream_synchronize(None), "sync")
+    return d
+
+
+def tsdf_params(N, dim, trunc):
+    p = TsdfParams()
+    check(L().nuif_tsdf_params(N, N, N, dim, dim, dim, trunc, C.byref(p)), "nuif_tsdf_params")
+    return p
+
+
+class Event:
+    def __init__(self):
+        init()
+        e = c_void_p()
+        check(L().nuif_event_create(C.byref(e)), "event")
+        self.ptr = e.value
+
+    def record(self, stream=None):
+        check(L().nuif_event_record(self.ptr, stream), "event record")
+
+    def synchronize(self):
+        check(L().nuif_event_synchronize(self.ptr), "event sync")
+
+    def elapsed_ms(self, later):
+        ms = c_float()
+        check(L().nuif_event_elapsed_ms(self.ptr, later.ptr, C.byref(ms)), "elapsed")
+        return ms.value
+
+
+class Engine:
+    """One tracking session: NuiKinfuTrackingEngine + dense NuiKinfuHIPScene."""
+
+    def __init__(self, desc):
+        init()
+        K()
+        self._h = c_void_p()
+        rc = K().nuifk_create(C.byref(desc), C.byref(self._h))
+        if rc != 0:
+            raise NuifError("nuifk_create: " + (K().nuifk_last_error() or b"").decode())
+        self.desc = desc
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise NuifError(f"{what}: " + (K().nuifk_last_error() or b"").decode())
+        return rc
+
+    def run(self, depth, bgra=None, sync=True):
+        """depth: numpy uint16 (host) or DeviceArray (device)."""
+        if isinstance(depth, DeviceArray):
+            bp = bgra.ptr if isinstance(bgra, DeviceArray) else None
+            return self._check(K().nuifk_run(self._h, depth.ptr, 1, bp, 1 if sync else 0), "run")
+        depth = np.ascontiguousarray(depth, np.uint16)
+        bp = np.ascontiguousarray(bgra, np.uint8).ctypes.data if bgra is not None else None
+        return self._check(K().nuifk_run(self._h, depth.ctypes.data, 0, bp, 1), "run")
+
+    def synchronize(self):
+        return self._check(K().nuifk_synchronize(self._h), "synchronize")
+
+    def poses(self, n):
+        out = np.zeros((n, POSE_ROW), np.float32)
+        m = self._check(K().nuifk_poses(self._h, out.ctypes.data, n), "poses")
+        return out[:m]
+
+    def state(self):
+        r = IcpResult()
+        self._check(K().nuifk_state(self._h, C.byref(r)), "state")
+        return r
+
+    def volume_raw(self):
+        nb = K().nuifk_volume_bytes(self._h)
+        out = np.empty(nb, np.uint8)
+        self._check(K().nuifk_read_volume(self._h, out.ctypes.data, nb), "read volume")
+        return out
+
+    def maps(self):
+        n = self.desc.width * self.desc.height
+        v = np.empty((n, 3), np.float32)
+        nm = np.empty((n, 3), np.float32)
+        self._check(K().nuifk_read_maps(self._h, v.ctypes.data, nm.ctypes.data), "maps")
+        return v, nm
+
+    def set_stage_timing(self, on):
+        self._check(K().nuifk_set_stage_timing(self._h, 1 if on else 0), "stage timing")
+
+    def take_stage_times(self):
+        """(frames, [preprocess, icp, integrate, raycast] summed ms)."""
+        ms = (C.c_double * 4)()
+        n = self._check(K().nuifk_take_stage_times(self._h, ms), "stage times")
+        return n, list(ms)
+
+    def updated_voxels(self):
+        v = C.c_ulonglong()
+        self._check(K().nuifk_updated_voxels(self._h, C.byref(v)), "updated voxels")
+        return v.value
+
+    @property
+    def stream(self):
+        return K().nuifk_stream(self._h)
+
+    @property
+    def trunc(self):
+        return K().nuifk_trunc(self._h)
+
+    def close(self):
+        if self._h:
+            K().nuifk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def decode_volume(raw, storage):
+    """(tsdf float32 normalised, weight int32) arrays from a raw volume dump."""
+    if storage == TSDF_S16:
+        v = raw.view(np.int16).reshape(-1, 2)
+        return v[:, 0].astype(np.float32) / np.float32(32766.0), v[:, 1].astype(np.int32)
+    if storage == TSDF_F16:
+        v = raw.view(np.uint16).reshape(-1, 2)
+        return v[:, 0].view(np.float16).astype(np.float32), v[:, 1].astype(np.int32)
+    v = raw.view(np.float32).reshape(-1, 2)
+    return v[:, 0].copy(), v[:, 1].astype(np.int32)

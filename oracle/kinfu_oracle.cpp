@@ -1,0 +1,1050 @@
+// TEST INFRASTRUCTURE ONLY (see oracle_common.h).  CPU restatement of the
+// reference's dense KinectFusion path: frame preprocessing, projective ICP
+// with its Levenberg-Marquardt host loop, dense TSDF integration and raycast,
+// and the per-frame orchestration of RunTracking.  Each function names the
+// reference file:line it follows.  Exported with C linkage in two flavours:
+// *_f (float, reference-compatible) and *_d (FP64 oracle).
+
+#include "oracle_common.h"
+
+#include <cstdio>
+
+namespace oracle {
+
+// ---------------------------------------------------------------- preprocess
+
+// K/filters.cl:9-23 depth_passing_filter_kernel: u16 mm -> m, NaN unless min < d < max
+template <class T> void passing_filter(const uint16_t* raw, T* out, size_t n, T dmin, T dmax) {
+    for (size_t i = 0; i < n; ++i) {
+        T d = (T)raw[i] * (T)0.001;
+        if (!(d > dmin && d < dmax)) d = qnan<T>();
+        out[i] = d;
+    }
+}
+
+// OCL/NuiKinfuOpenCLFrame.cpp:165-189 GenerateGaussianBuffer: g[i] = exp(-(i-r)^2 * s)
+template <class T> void gaussian_table(int radius, T sigma_space, T* g) {
+    for (int i = 0; i < 2 * radius + 1; ++i) {
+        int x = i - radius;
+        g[i] = std::exp(-(T)(x * x) * sigma_space);
+    }
+}
+
+// K/bilateralFilter.cl:68-119 bilateral_filter_depth_kernel
+template <class T>
+void bilateral(const T* in, T* out, int W, int H, const T* g, int radius, T sigma_depth, T thr) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            T c = in[y * W + x];
+            if (std::isnan(c)) {
+                out[y * W + x] = qnan<T>();
+                continue;
+            }
+            int x0 = std::max(x - radius, 0), x1 = std::min(x + radius + 1, W);
+            int y0 = std::max(y - radius, 0), y1 = std::min(y + radius + 1, H);
+            T sd = 0, sw = 0;
+            for (int cy = y0; cy < y1; ++cy)
+                for (int cx = x0; cx < x1; ++cx) {
+                    T n = in[cy * W + cx];
+                    if (std::isnan(n)) continue;
+                    T diff = std::fabs(c - n);
+                    if (n > T(0) && diff < thr) {
+                        T d2 = diff * diff;
+                        T w = g[cx + radius - x] * g[cy + radius - y] * std::exp(-(d2 * sigma_depth));
+                        sd += n * w;
+                        sw += w;
+                    }
+                }
+            out[y * W + x] = sd / sw;
+        }
+}
+
+// K/transformMaps.cl:44-88 half_sample_kernel (source size taken as 2x the destination)
+template <class T> void half_sample(const T* src, T* dst, int Wd, int Hd, int radius, T thr) {
+    const int Ws = Wd << 1, Hs = Hd << 1;
+    for (int y = 0; y < Hd; ++y)
+        for (int x = 0; x < Wd; ++x) {
+            int sx = x << 1, sy = y << 1;
+            T c = src[sy * Ws + sx];
+            int x0 = std::max(sx - radius, 0), x1 = std::min(sx + radius + 1, Ws);
+            int y0 = std::max(sy - radius, 0), y1 = std::min(sy + radius + 1, Hs);
+            T s = 0;
+            int cnt = 0;
+            for (int cy = y0; cy < y1; ++cy)
+                for (int cx = x0; cx < x1; ++cx) {
+                    T n = src[cy * Ws + cx];
+                    if (!std::isnan(n) && (std::isnan(c) || std::fabs(c - n) < thr)) {
+                        s += n;
+                        cnt += 1;
+                    }
+                }
+            dst[y * Wd + x] = cnt > 0 ? s / (T)cnt : qnan<T>();
+        }
+}
+
+// K/transformMaps.cl:18-42 depth2vertex_kernel with div = 2^level
+template <class T> void depth2vertex(const T* depth, T* vmap, int W, int H, const Cam<T>& cam, int div) {
+    const T fxi = (T)div * cam.fx_inv, fyi = (T)div * cam.fy_inv;
+    const T cx = cam.cx / (T)div, cy = cam.cy / (T)div;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t i = (size_t)y * W + x;
+            T d = depth[i];
+            V3<T> v = mk(qnan<T>(), qnan<T>(), qnan<T>());
+            if (!std::isnan(d)) v = mk(((T)x - cx) * fxi, ((T)y - cy) * fyi, T(1)) * d;
+            store3(vmap, i, v);
+        }
+}
+
+// K/normalEstimation.cl:770-822 estimate_normals_simple_kernel
+template <class T> void normals_simple(const T* vmap, T* nmap, int W, int H, T thr) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t i = (size_t)y * W + x;
+            V3<T> c = load3(vmap, i);
+            if (isnan3(c)) {
+                store3(nmap, i, mk(qnan<T>(), qnan<T>(), qnan<T>()));
+                continue;
+            }
+            auto pick = [&](bool ok, size_t j) {
+                if (!ok) return c;
+                V3<T> n = load3(vmap, j);
+                if (isnan3(n) || std::fabs(c.z - n.z) > thr) return c;
+                return n;
+            };
+            V3<T> l = pick(x > 0, i - 1), r = pick(x < W - 1, i + 1);
+            V3<T> u = pick(y > 0, i - W), d = pick(y < H - 1, i + W);
+            store3(nmap, i, normalize3(cross(r - l, d - u)));
+        }
+}
+
+// K/transformMaps.cl:91-127 transform_maps_kernel
+template <class T> void transform_maps(const T* v, const T* n, T* vd, T* nd, size_t npx, const Rigid<T>& g) {
+    const V3<T> NaN3 = mk(qnan<T>(), qnan<T>(), qnan<T>());
+    for (size_t i = 0; i < npx; ++i) {
+        V3<T> vs = load3(v, i);
+        if (isnan3(vs)) {
+            store3(vd, i, NaN3);
+            store3(nd, i, NaN3);
+            continue;
+        }
+        store3(vd, i, transform(g, vs));
+        V3<T> ns = load3(n, i);
+        store3(nd, i, isnan3(ns) ? NaN3 : rotate(g.R, ns));
+    }
+}
+
+// K/tsdfVolume.cl:8-29 scaleDepthsKernel (in place): depth -> ray length
+template <class T> void scale_depths(T* depth, int W, int H, const Cam<T>& cam) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            T d = depth[(size_t)y * W + x];
+            if (std::isnan(d)) continue;
+            T xl = ((T)x - cam.cx) * cam.fx_inv, yl = ((T)y - cam.cy) * cam.fy_inv;
+            T lambda = std::sqrt(xl * xl + yl * yl + T(1));
+            depth[(size_t)y * W + x] = d * lambda;
+        }
+}
+
+// ---------------------------------------------------------------- ICP
+
+constexpr int kTerms = 29;  // 27 unique normal-equation terms + r^2 + count (K/icp.cl:130-160)
+
+// K/icp.cl:192-220 interpolateBilinear_withHoles.  The reference reads p.x+1 /
+// p.y+1 without bounds (quirk Q14); a linear index inside the buffer reads the
+// same element as the reference, one past the buffer is treated as a hole.
+template <class T> V3<T> bilinear_holes(const T* src, T px, T py, int W, int H) {
+    const V3<T> NaN3 = mk(qnan<T>(), qnan<T>(), qnan<T>());
+    int x = (int)std::floor(px), y = (int)std::floor(py);
+    T dx = px - (T)x, dy = py - (T)y;
+    const long long n = (long long)W * H;
+    long long ia = x + (long long)y * W, ib = (x + 1) + (long long)y * W;
+    long long ic = x + (long long)(y + 1) * W, id = (x + 1) + (long long)(y + 1) * W;
+    if (ia >= n || ib >= n || ic >= n || id >= n) return NaN3;
+    V3<T> a = load3(src, ia), b = load3(src, ib), c = load3(src, ic), d = load3(src, id);
+    if (isnan3(a) || isnan3(b) || isnan3(c) || isnan3(d)) return NaN3;
+    V3<T> r;
+    r.x = a.x * (T(1) - dx) * (T(1) - dy) + b.x * dx * (T(1) - dy) + c.x * (T(1) - dx) * dy + d.x * dx * dy;
+    r.y = a.y * (T(1) - dx) * (T(1) - dy) + b.y * dx * (T(1) - dy) + c.y * (T(1) - dx) * dy + d.y * dx * dy;
+    r.z = a.z * (T(1) - dx) * (T(1) - dy) + b.z * dx * (T(1) - dy) + c.z * (T(1) - dx) * dy + d.z * dx * dy;
+    return r;
+}
+
+// K/icp.cl:222-292 icp_block_kernel, one pixel: fills the 29 terms (zeros if rejected)
+template <class T>
+void icp_pixel(const Cam<T>& cam, V3<T> vc, const T R[9], const T t[3], const T* vref, const T* nref,
+               const Rigid<T>& prev, T distThr, T out[kTerms]) {
+    for (int k = 0; k < kTerms; ++k) out[k] = 0;
+    if (isnan3(vc)) return;
+    V3<T> pv = rotate(R, vc) + mk(t[0], t[1], t[2]);
+    V3<T> pp = transform_inverse(prev, pv);
+    T px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
+    if (!(px >= 0 && px < (T)cam.W && py >= 0 && py < (T)cam.H)) return;
+    V3<T> rn = bilinear_holes(nref, px, py, cam.W, cam.H);
+    if (isnan3(rn)) return;
+    V3<T> rv = bilinear_holes(vref, px, py, cam.W, cam.H);
+    if (isnan3(rv)) return;
+    V3<T> diff = rv - pv;
+    T dist = std::sqrt(dot(diff, diff));
+    if (!(dist < distThr)) return;
+    V3<T> row0 = cross(rn, pv);
+    T c[7] = {row0.x, row0.y, row0.z, rn.x, rn.y, rn.z, dot(rn, diff)};
+    int s = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j) out[s++] = c[i] * c[j];
+    out[s++] = c[6] * c[6];
+    out[s++] = T(1);
+}
+
+// In-place 128-wide tree of K/icp.cl:162-190 reduce(): buf holds 128 x 29 values
+template <class T> void tree128(T* buf) {
+    for (int d = 64; d > 0; d >>= 1)
+        for (int l = 0; l < d; ++l)
+            for (int i = 0; i < kTerms; ++i) buf[l * kTerms + i] += buf[(l + d) * kTerms + i];
+}
+
+// One ICP iteration's reduction exactly as the reference orders it:
+// icp_block_kernel (128-wide trees per group) -> compute_sums (128-wide trees
+// over groups) -> host sequential sum over the second-level blocks
+// (OCL/NuiKinfuOpenCLDepthTracker.cpp:318-404).  Returns the per-block
+// results in `blocks` (nblocks x 29) for the host loop.
+template <class T>
+int icp_reduce(const Cam<T>& cam, const T* vcur, int Wl, int Hl, const T R[9], const T t[3], const T* vref,
+               const T* nref, const Rigid<T>& prev, T distThr, std::vector<T>& blocks) {
+    const int npx = Wl * Hl;
+    const int groups = (npx + 127) / 128;
+    std::vector<T> gsum((size_t)groups * kTerms);
+#pragma omp parallel
+    {
+        std::vector<T> buf(128 * kTerms);
+#pragma omp for schedule(static)
+        for (int gi = 0; gi < groups; ++gi) {
+            for (int l = 0; l < 128; ++l) {
+                int p = gi * 128 + l;
+                T* o = &buf[l * kTerms];
+                if (p < npx)
+                    icp_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, o);
+                else
+                    for (int k = 0; k < kTerms; ++k) o[k] = 0;  // quirk Q13: partial group taken as zeros
+            }
+            tree128(buf.data());
+            std::memcpy(&gsum[(size_t)gi * kTerms], buf.data(), kTerms * sizeof(T));
+        }
+    }
+    const int nblocks = (groups + 127) / 128;
+    blocks.assign((size_t)nblocks * kTerms, T(0));
+    std::vector<T> buf(128 * kTerms);
+    for (int b = 0; b < nblocks; ++b) {
+        for (int l = 0; l < 128; ++l) {
+            int gi = b * 128 + l;
+            for (int k = 0; k < kTerms; ++k) buf[l * kTerms + k] = gi < groups ? gsum[(size_t)gi * kTerms + k] : T(0);
+        }
+        tree128(buf.data());
+        std::memcpy(&blocks[(size_t)b * kTerms], buf.data(), kTerms * sizeof(T));
+    }
+    return nblocks;
+}
+
+// Eigen::AngleAxis<T>(angle, unit axis e_k).toRotationMatrix()
+template <class T> void axis_rot(T angle, int k, T Rm[9]) {
+    T s = std::sin(angle), c = std::cos(angle);
+    T ax[3] = {0, 0, 0};
+    ax[k] = 1;
+    T sa[3] = {s * ax[0], s * ax[1], s * ax[2]};
+    T ca[3] = {(T(1) - c) * ax[0], (T(1) - c) * ax[1], (T(1) - c) * ax[2]};
+    Rm[1] = ca[0] * ax[1] - sa[2];
+    Rm[3] = ca[0] * ax[1] + sa[2];
+    Rm[2] = ca[0] * ax[2] + sa[1];
+    Rm[6] = ca[0] * ax[2] - sa[1];
+    Rm[5] = ca[1] * ax[2] - sa[0];
+    Rm[7] = ca[1] * ax[2] + sa[0];
+    Rm[0] = ca[0] * ax[0] + c;
+    Rm[4] = ca[1] * ax[1] + c;
+    Rm[8] = ca[2] * ax[2] + c;
+}
+
+// 6x6 SPD solve by Cholesky (A = L L^T) in double; det = prod(L_ii)^2.
+// Stands in for Eigen's A.determinant() / A.llt().solve(b)
+// (OCL/NuiKinfuOpenCLDepthTracker.cpp:465-475).  Returns false if not SPD.
+inline bool cholesky6(const double A[36], const double b[6], double x[6], double* det) {
+    double L[36] = {0};
+    for (int j = 0; j < 6; ++j) {
+        double s = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) s -= L[j * 6 + k] * L[j * 6 + k];
+        if (!(s > 0)) {
+            *det = 0;
+            return false;
+        }
+        L[j * 6 + j] = std::sqrt(s);
+        for (int i = j + 1; i < 6; ++i) {
+            double t = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) t -= L[i * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = t / L[j * 6 + j];
+        }
+    }
+    double d = 1;
+    for (int j = 0; j < 6; ++j) d *= L[j * 6 + j];
+    *det = d * d;
+    double y[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * y[k];
+        y[i] = s / L[i * 6 + i];
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int k = i + 1; k < 6; ++k) s -= L[k * 6 + i] * x[k];
+        x[i] = s / L[i * 6 + i];
+    }
+    return true;
+}
+
+// Determinant by partially pivoted LU (for the nullspace check when the
+// damped matrix is not positive definite).
+inline double det6(const double Ain[36]) {
+    double A[36];
+    std::memcpy(A, Ain, sizeof(A));
+    double det = 1;
+    for (int c = 0; c < 6; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 6; ++r)
+            if (std::fabs(A[r * 6 + c]) > std::fabs(A[p * 6 + c])) p = r;
+        if (A[p * 6 + c] == 0) return 0;
+        if (p != c) {
+            for (int k = 0; k < 6; ++k) std::swap(A[c * 6 + k], A[p * 6 + k]);
+            det = -det;
+        }
+        det *= A[c * 6 + c];
+        for (int r = c + 1; r < 6; ++r) {
+            double f = A[r * 6 + c] / A[c * 6 + c];
+            for (int k = c; k < 6; ++k) A[r * 6 + k] -= f * A[c * 6 + k];
+        }
+    }
+    return det;
+}
+
+struct IcpConfig {
+    int levels;
+    int iters[8];
+    float dist_threshold;
+};
+
+struct IcpResult {
+    int ok;          // 0: tracking failure (singular system)
+    float error;     // m_error of the last iteration
+    float count;     // m_count of the last iteration
+    int total_iters; // iterations executed (kernel launches)
+};
+
+// OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504 IterativeClosestPoint.
+// vpyr[l] / (Wl,Hl) are the current frame's vertex pyramid; vref/nref the
+// world-space reference maps at full resolution; prev the camera transform
+// of the previous frame; R,t in/out the pose estimate.
+template <class T>
+IcpResult icp(const Cam<T>& cam, const IcpConfig& cfg, const T* const* vpyr, const T* vref, const T* nref,
+              const Rigid<T>& prev, T R[9], T t[3]) {
+    IcpResult res{1, 0, 0, 0};
+    const int L = cfg.levels;
+    const T step = (T)cfg.dist_threshold / (T)L;
+    const T distThr = (T)cfg.dist_threshold + step;
+    size_t kgs = (size_t)cam.W * cam.H;  // kernelGlobalSize[0]; stale across levels (quirk Q10)
+    T goodCores[kTerms] = {0};
+    T err = 0, cnt = 0;
+    std::vector<T> blocks;
+    for (int level = L - 1; level >= 0; --level) {
+        const unsigned thr = (unsigned)((double)(kgs >> (2 * level)) * 0.1);
+        T lambda = 1;
+        T Rg[9], tg[3];
+        std::memcpy(Rg, R, sizeof(Rg));
+        std::memcpy(tg, t, sizeof(tg));
+        T errOld = (T)1e20f;
+        const int Wl = cam.W >> level, Hl = cam.H >> level;
+        for (int it = 0; it < cfg.iters[level]; ++it) {
+            int nb = icp_reduce(cam, vpyr[level], Wl, Hl, R, t, vref, nref, prev, distThr, blocks);
+            res.total_iters++;
+            const int groups = (Wl * Hl + 127) / 128;
+            kgs = (size_t)((groups + 127) / 128) * 128;
+            err = 0;
+            cnt = 0;
+            for (int n = 0; n < nb; ++n) {
+                err += blocks[(size_t)n * kTerms + kTerms - 2];
+                cnt += blocks[(size_t)n * kTerms + kTerms - 1];
+            }
+            err = cnt > 0 ? std::sqrt(err) / cnt : std::numeric_limits<T>::max();
+            if (cnt < (T)thr || err > errOld) {
+                lambda *= T(10);
+                std::memcpy(R, Rg, sizeof(Rg));
+                std::memcpy(t, tg, sizeof(tg));
+                if (it == 0) break;
+            } else {
+                lambda /= T(10);
+                std::memcpy(Rg, R, sizeof(Rg));
+                std::memcpy(tg, t, sizeof(tg));
+                errOld = err;
+                for (int s = 0; s < 27; ++s) {
+                    T acc = 0;
+                    for (int n = 0; n < nb; ++n) acc += blocks[(size_t)n * kTerms + s];
+                    goodCores[s] = acc;
+                }
+            }
+            double A[36], b[6];
+            for (int i = 0, s = 0; i < 6; ++i) {
+                for (int j = i; j < 7; ++j) {
+                    double v = (double)(goodCores[s] / cnt);
+                    if (j == 6)
+                        b[i] = v;
+                    else
+                        A[j * 6 + i] = A[i * 6 + j] = v;
+                    s++;
+                }
+                A[i * 6 + i] *= (double)(T(1) + lambda);
+            }
+            double x[6], det;
+            bool spd = cholesky6(A, b, x, &det);
+            if (!spd) det = det6(A);
+            if (std::fabs(det) < 1e-15 || std::isnan(det) || !spd) {
+                res.ok = 0;
+                res.error = (float)err;
+                res.count = (float)cnt;
+                return res;
+            }
+            T alpha = (T)x[0], beta = (T)x[1], gamma = (T)x[2];
+            if (alpha * alpha + beta * beta + gamma * gamma < (T)1e-6f) break;
+            T Rz[9], Ry[9], Rx[9], Rzy[9], Rinc[9], RincInv[9];
+            axis_rot(gamma, 2, Rz);
+            axis_rot(beta, 1, Ry);
+            axis_rot(alpha, 0, Rx);
+            matmul3(Rz, Ry, Rzy);
+            matmul3(Rzy, Rx, Rinc);
+            inverse3(Rinc, RincInv);
+            T tn[3];
+            matvec3(RincInv, t, tn);
+            t[0] = tn[0] + (T)x[3];
+            t[1] = tn[1] + (T)x[4];
+            t[2] = tn[2] + (T)x[5];
+            matmul3(RincInv, R, R);
+        }
+    }
+    res.error = (float)err;
+    res.count = (float)cnt;
+    return res;
+}
+
+// src/Foundation/NuiMatrixUtilities.cpp:7-52 rodrigues2(...).norm(), for an
+// (already orthonormal) rotation: 0 for tiny rotations, else the angle.
+template <class T> T rodrigues_norm(const T Rm[9]) {
+    double rx = Rm[7] - Rm[5], ry = Rm[2] - Rm[6], rz = Rm[3] - Rm[1];
+    double s = std::sqrt((rx * rx + ry * ry + rz * rz) * 0.25);
+    double c = ((double)Rm[0] + Rm[4] + Rm[8] - 1) * 0.5;
+    c = c > 1. ? 1. : c < -1. ? -1. : c;
+    double theta = std::acos(c);
+    if (s < 1e-5 && c > 0) return 0;
+    return (T)theta;
+}
+
+// ---------------------------------------------------------------- dense TSDF
+
+// Voxel storage policies.  S16: the reference's short2 (int16 tsdf x 32766 with
+// round-toward-zero re-quantisation, int16 weight; K/utils.cl:69-84).  F64:
+// the FP64 oracle's exact storage.
+struct VolS16 {
+    std::vector<int16_t> v;  // 2 per voxel
+    void init(size_t n) { v.assign(2 * n, 0); for (size_t i = 0; i < n; ++i) v[2 * i] = 32767; }
+    template <class T> T tsdf(size_t i) const { return (T)v[2 * i] / (T)32766; }
+    int weight(size_t i) const { return v[2 * i + 1]; }
+    template <class T> void set(size_t i, T f, int w) {
+        v[2 * i] = (int16_t)(int)(f * (T)32766);  // convert_short: RTZ
+        v[2 * i + 1] = (int16_t)w;
+    }
+};
+struct VolF64 {
+    std::vector<double> f;
+    std::vector<int> w;
+    void init(size_t n) { f.assign(n, 32767.0 / 32766.0); w.assign(n, 0); }
+    template <class T> T tsdf(size_t i) const { return (T)f[i]; }
+    int weight(size_t i) const { return w[i]; }
+    template <class T> void set(size_t i, T v, int wt) { f[i] = (double)v; w[i] = wt; }
+};
+
+// K/tsdfVolume.cl:31-190 integrateTsdfVolumeKernel (voxelWrap = 0), one
+// column per (x,y) walking z with the reference's incremental updates.
+// depth: ray-length depth (after scale_depths); bgra may be NULL.
+template <class T, class Vol>
+long long integrate(const T* depth, const uint8_t* bgra, Vol& vol, uint8_t* color_vol, const Tsdf<T>& p,
+                    const Cam<T>& cam, const Rigid<T>& g, int max_color_weight) {
+    const int Nx = p.res[0], Ny = p.res[1], Nz = p.res[2];
+    const T tr_inv = T(1) / p.trunc;
+    long long nupd = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nupd)
+    for (int y = 0; y < Ny; ++y)
+        for (int x = 0; x < Nx; ++x) {
+            const T vgx = ((T)(x - Nx / 2) + T(0.5)) * p.cell[0] - g.t[0];
+            const T vgy = ((T)(y - Ny / 2) + T(0.5)) * p.cell[1] - g.t[1];
+            T vgz = T(0.5) * p.cell[2] - g.t[2];
+            const T part = vgx * vgx + vgy * vgy;
+            T vx = (g.R_inv[0] * vgx + g.R_inv[1] * vgy + g.R_inv[2] * vgz) * cam.fx;
+            T vy = (g.R_inv[3] * vgx + g.R_inv[4] * vgy + g.R_inv[5] * vgz) * cam.fy;
+            const T vz = g.R_inv[6] * vgx + g.R_inv[7] * vgy + g.R_inv[8] * vgz;
+            T zs = 0;
+            const T incx = g.R_inv[2] * p.cell[2] * cam.fx;
+            const T incy = g.R_inv[5] * p.cell[2] * cam.fy;
+            const size_t base = ((size_t)y * Nx + x) * Nz;
+            for (int z = 0; z < Nz; ++z, vgz += p.cell[2], zs += p.cell[2], vx += incx, vy += incy) {
+                const T inv_z = T(1) / (vz + g.R_inv[8] * zs);
+                if (inv_z < 0) continue;
+                const int u = to_int(cl_round(vx * inv_z + cam.cx));
+                const int v = to_int(cl_round(vy * inv_z + cam.cy));
+                if (!(u >= 0 && v >= 0 && u < cam.W && v < cam.H)) continue;
+                const size_t pix = (size_t)v * cam.W + u;
+                const T Dp = depth[pix];
+                const T sdf = Dp - std::sqrt(vgz * vgz + part);
+                if (std::isnan(Dp) || !(sdf >= -p.trunc)) continue;
+                const T tsdf = std::min(T(1), sdf * tr_inv);
+                const size_t idx = base + z;
+                const T prev = vol.template tsdf<T>(idx);
+                const int w = vol.weight(idx);
+                const T fnew = (prev * (T)w + T(1) * tsdf) / (T)(w + 1);
+                vol.set(idx, fnew, std::min(w + 1, 128));
+                ++nupd;
+                if (color_vol && bgra && std::fabs(sdf) < p.trunc) {
+                    const uint8_t* nc = bgra + 4 * pix;
+                    if (nc[3] != 0) {
+                        uint8_t* vc = color_vol + 4 * idx;
+                        const float Wrk = 1.f;
+                        float c[3];
+                        for (int k = 0; k < 3; ++k)
+                            c[k] = ((float)(vc[k] * vc[3]) + Wrk * (float)nc[k]) / ((float)vc[3] + Wrk);
+                        uint8_t wn = (uint8_t)(vc[3] + 1);
+                        for (int k = 0; k < 3; ++k) vc[k] = (uint8_t)(int)std::min(std::max(c[k], 0.f), 255.f);
+                        vc[3] = (uint8_t)std::min(max_color_weight, (int)wn);
+                    }
+                }
+            }
+        }
+    return nupd;
+}
+
+// K/raycast.cl:22-29 getVoxel: round(p / cell)
+template <class T> inline void get_voxel(V3<T> p, const Tsdf<T>& prm, int g[3]) {
+    g[0] = to_int(cl_round(p.x / prm.cell[0]));
+    g[1] = to_int(cl_round(p.y / prm.cell[1]));
+    g[2] = to_int(cl_round(p.z / prm.cell[2]));
+}
+
+template <class T, class Vol> inline T sample(const Vol& vol, size_t idx) {
+    return vol.weight(idx) > 0 ? vol.template tsdf<T>(idx) : qnan<T>();
+}
+
+// K/raycast.cl:36-120 interpolateTrilineary.  The reference names its eight
+// corners for an x-major layout while the volume is y-major
+// ((y*Nx + x)*Nz + z), so the x and y fractions weight the swapped corners;
+// this is reproduced as written.
+template <class T, class Vol> T trilinear(V3<T> pt, const Vol& vol, const Tsdf<T>& prm) {
+    int g[3];
+    get_voxel(pt, prm, g);
+    const int Nx = prm.res[0], Ny = prm.res[1], Nz = prm.res[2];
+    if (g[0] <= 0 || g[0] >= Nx - 1) return qnan<T>();
+    if (g[1] <= 0 || g[1] >= Ny - 1) return qnan<T>();
+    if (g[2] <= 0 || g[2] >= Nz - 1) return qnan<T>();
+    const T vx = ((T)g[0] + T(0.5)) * prm.cell[0];
+    const T vy = ((T)g[1] + T(0.5)) * prm.cell[1];
+    const T vz = ((T)g[2] + T(0.5)) * prm.cell[2];
+    g[0] = pt.x < vx ? g[0] - 1 : g[0];
+    g[1] = pt.y < vy ? g[1] - 1 : g[1];
+    g[2] = pt.z < vz ? g[2] - 1 : g[2];
+    const T a = (pt.x - ((T)g[0] + T(0.5)) * prm.cell[0]) / prm.cell[0];
+    const T b = (pt.y - ((T)g[1] + T(0.5)) * prm.cell[1]) / prm.cell[1];
+    const T c = (pt.z - ((T)g[2] + T(0.5)) * prm.cell[2]) / prm.cell[2];
+    const long long sx = Nz, sy = (long long)Nx * Nz;
+    long long idx = ((long long)g[1] * Nx + g[0]) * Nz + g[2];
+    T f000 = sample<T>(vol, idx);             if (std::isnan(f000)) return qnan<T>();
+    T f001 = sample<T>(vol, idx + 1);         if (std::isnan(f001)) return qnan<T>();
+    T f011 = sample<T>(vol, idx + sx + 1);    if (std::isnan(f011)) return qnan<T>();
+    T f010 = sample<T>(vol, idx + sx);        if (std::isnan(f010)) return qnan<T>();
+    T f110 = sample<T>(vol, idx + sx + sy);   if (std::isnan(f110)) return qnan<T>();
+    T f111 = sample<T>(vol, idx + sx + sy + 1); if (std::isnan(f111)) return qnan<T>();
+    T f101 = sample<T>(vol, idx + sy + 1);    if (std::isnan(f101)) return qnan<T>();
+    T f100 = sample<T>(vol, idx + sy);        if (std::isnan(f100)) return qnan<T>();
+    return f000 * (1 - a) * (1 - b) * (1 - c) + f001 * (1 - a) * (1 - b) * c + f010 * (1 - a) * b * (1 - c) +
+           f011 * (1 - a) * b * c + f100 * a * (1 - b) * (1 - c) + f101 * a * (1 - b) * c +
+           f110 * a * b * (1 - c) + f111 * a * b * c;
+}
+
+template <class T> inline T min_time(const T* vmax, V3<T> o, V3<T> d) {
+    T tx = std::fabs(d.x) < 1e-14 ? T(0) : ((d.x > 0 ? T(0) : vmax[0]) - o.x) / d.x;
+    T ty = std::fabs(d.y) < 1e-14 ? T(0) : ((d.y > 0 ? T(0) : vmax[1]) - o.y) / d.y;
+    T tz = std::fabs(d.z) < 1e-14 ? T(0) : ((d.z > 0 ? T(0) : vmax[2]) - o.z) / d.z;
+    return std::max(std::max(tx, ty), tz);
+}
+template <class T> inline T max_time(const T* vmax, V3<T> o, V3<T> d) {
+    T tx = std::fabs(d.x) < 1e-14 ? T(1e14) : ((d.x > 0 ? vmax[0] : T(0)) - o.x) / d.x;
+    T ty = std::fabs(d.y) < 1e-14 ? T(1e14) : ((d.y > 0 ? vmax[1] : T(0)) - o.y) / d.y;
+    T tz = std::fabs(d.z) < 1e-14 ? T(1e14) : ((d.z > 0 ? vmax[2] : T(0)) - o.z) / d.z;
+    return std::min(std::min(tx, ty), tz);
+}
+
+// K/raycast.cl:128-277 raycastKernel (voxelWrap = 0).  Outputs world-space
+// vmap/nmap (NaN where no surface) and, if both colour buffers are given, cmap.
+template <class T, class Vol>
+void raycast(const Vol& vol, const uint8_t* color_vol, const Tsdf<T>& prm, const Cam<T>& cam, const Rigid<T>& g,
+             T* vmap, T* nmap, uint8_t* cmap) {
+    const int W = cam.W, H = cam.H;
+    const int Nx = prm.res[0], Ny = prm.res[1], Nz = prm.res[2];
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t id = (size_t)y * W + x;
+            const V3<T> NaN3 = mk(qnan<T>(), qnan<T>(), qnan<T>());
+            store3(vmap, id, NaN3);
+            store3(nmap, id, NaN3);
+            if (cmap) std::memset(cmap + 4 * id, 0, 4);
+            const V3<T> off = mk((T)(prm.dim[0] / 2.0), (T)(prm.dim[1] / 2.0), T(0));
+            V3<T> rs = mk(g.t[0], g.t[1], g.t[2]);
+            rs = rs + off;
+            V3<T> rn = mk(((T)x - cam.cx) * cam.fx_inv, ((T)y - cam.cy) * cam.fy_inv, T(1));
+            rn = transform(g, rn);
+            rn = rn + off;
+            V3<T> dir = normalize3(rn - rs);
+            T tcur = min_time(prm.dim, rs, dir);
+            const T texit = max_time(prm.dim, rs, dir);
+            tcur = std::max(tcur, T(0));
+            if (tcur >= texit) continue;
+            int gv[3];
+            get_voxel(rs + dir * tcur, prm, gv);
+            gv[0] = std::max(0, std::min(gv[0], Nx - 1));
+            gv[1] = std::max(0, std::min(gv[1], Ny - 1));
+            gv[2] = std::max(0, std::min(gv[2], Nz - 1));
+            size_t idx = ((size_t)gv[1] * Nx + gv[0]) * Nz + gv[2];
+            T tsdf = sample<T>(vol, idx);
+            const T max_t = T(3) * (prm.dim[0] + prm.dim[1] + prm.dim[2]);
+            const T step = prm.trunc * T(0.8);
+            for (; tcur < max_t; tcur += step) {
+                const T tprev = tsdf;
+                get_voxel(rs + dir * (tcur + step), prm, gv);
+                if (!(gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < Nx && gv[1] < Ny && gv[2] < Nz)) break;
+                idx = ((size_t)gv[1] * Nx + gv[0]) * Nz + gv[2];
+                tsdf = sample<T>(vol, idx);
+                if (std::isnan(tsdf) || std::isnan(tprev)) continue;
+                if (tprev < 0 && tsdf > 0) break;
+                if (tprev >= 0 && tsdf <= 0) {
+                    const T Ftdt = trilinear(rs + dir * (tcur + step), vol, prm);
+                    if (std::isnan(Ftdt)) break;
+                    const T Ft = trilinear(rs + dir * tcur, vol, prm);
+                    if (std::isnan(Ft)) break;
+                    const T Ts = tcur - step * Ft / (Ftdt - Ft);
+                    const V3<T> vf = rs + dir * Ts;
+                    store3(vmap, id, vf - off);
+                    if (color_vol && cmap) std::memcpy(cmap + 4 * id, color_vol + 4 * idx, 4);
+                    int gg[3];
+                    get_voxel(rs + dir * tcur, prm, gg);
+                    if (gg[0] > 1 && gg[1] > 1 && gg[2] > 1 && gg[0] < Nx - 2 && gg[1] < Ny - 2 && gg[2] < Nz - 2) {
+                        V3<T> n, q;
+                        q = vf; q.x += prm.cell[0]; T fx1 = trilinear(q, vol, prm);
+                        q = vf; q.x -= prm.cell[0]; T fx2 = trilinear(q, vol, prm);
+                        n.x = fx1 - fx2;
+                        q = vf; q.y += prm.cell[1]; T fy1 = trilinear(q, vol, prm);
+                        q = vf; q.y -= prm.cell[1]; T fy2 = trilinear(q, vol, prm);
+                        n.y = fy1 - fy2;
+                        q = vf; q.z += prm.cell[2]; T fz1 = trilinear(q, vol, prm);
+                        q = vf; q.z -= prm.cell[2]; T fz2 = trilinear(q, vol, prm);
+                        n.z = fz1 - fz2;
+                        store3(nmap, id, normalize3(n));
+                    }
+                    break;
+                }
+            }
+        }
+}
+
+}  // namespace oracle
+
+// ============================================================ C interface
+using namespace oracle;
+
+namespace {
+template <class T> Cam<T> make_cam(const float* c10, int W, int H) {
+    // c10: fx, fy, cx, cy, dmin, dmax (+ unused)
+    Cam<T> c;
+    c.fx = (T)c10[0];
+    c.fy = (T)c10[1];
+    c.fx_inv = T(1) / c.fx;
+    c.fy_inv = T(1) / c.fy;
+    c.cx = (T)c10[2];
+    c.cy = (T)c10[3];
+    c.W = W;
+    c.H = H;
+    c.dmin = (T)c10[4];
+    c.dmax = (T)c10[5];
+    return c;
+}
+template <class T> Tsdf<T> make_tsdf(int N, float dim, float trunc) {
+    Tsdf<T> p;
+    for (int k = 0; k < 3; ++k) {
+        p.res[k] = N;
+        p.dim[k] = (T)dim;
+        p.cell[k] = (T)dim / (T)N;
+    }
+    // OCL/NuiKinfuOpenCLScene.cpp:41: trunc >= 2.1 * max cell (quirk Q16)
+    T c = std::max(p.cell[0], std::max(p.cell[1], p.cell[2]));
+    p.trunc = std::max((T)trunc, (T)2.1f * c);
+    return p;
+}
+}  // namespace
+
+#define OR_EXPORT extern "C" __attribute__((visibility("default")))
+
+// ---- per-stage entry points, float (compat) flavour
+OR_EXPORT void or_passing_filter_f(const uint16_t* raw, float* out, long long n, float dmin, float dmax) {
+    passing_filter<float>(raw, out, (size_t)n, dmin, dmax);
+}
+OR_EXPORT void or_passing_filter_d(const uint16_t* raw, double* out, long long n, double dmin, double dmax) {
+    passing_filter<double>(raw, out, (size_t)n, dmin, dmax);
+}
+OR_EXPORT void or_gaussian_table_f(int radius, float sigma, float* g) { gaussian_table<float>(radius, sigma, g); }
+OR_EXPORT void or_bilateral_f(const float* in, float* out, int W, int H, const float* g, int radius, float sd,
+                              float thr) {
+    bilateral<float>(in, out, W, H, g, radius, sd, thr);
+}
+OR_EXPORT void or_bilateral_d(const double* in, double* out, int W, int H, const double* g, int radius, double sd,
+                              double thr) {
+    bilateral<double>(in, out, W, H, g, radius, sd, thr);
+}
+OR_EXPORT void or_half_sample_f(const float* s, float* d, int Wd, int Hd, int radius, float thr) {
+    half_sample<float>(s, d, Wd, Hd, radius, thr);
+}
+OR_EXPORT void or_half_sample_d(const double* s, double* d, int Wd, int Hd, int radius, double thr) {
+    half_sample<double>(s, d, Wd, Hd, radius, thr);
+}
+OR_EXPORT void or_depth2vertex_f(const float* depth, float* vmap, int W, int H, const float* cam6, int Wfull,
+                                 int Hfull, int div) {
+    depth2vertex<float>(depth, vmap, W, H, make_cam<float>(cam6, Wfull, Hfull), div);
+}
+OR_EXPORT void or_depth2vertex_d(const double* depth, double* vmap, int W, int H, const float* cam6, int Wfull,
+                                 int Hfull, int div) {
+    depth2vertex<double>(depth, vmap, W, H, make_cam<double>(cam6, Wfull, Hfull), div);
+}
+OR_EXPORT void or_normals_simple_f(const float* v, float* n, int W, int H, float thr) {
+    normals_simple<float>(v, n, W, H, thr);
+}
+OR_EXPORT void or_normals_simple_d(const double* v, double* n, int W, int H, double thr) {
+    normals_simple<double>(v, n, W, H, thr);
+}
+OR_EXPORT void or_transform_maps_f(const float* v, const float* n, float* vd, float* nd, long long npx,
+                                   const float* R9, const float* t3) {
+    transform_maps<float>(v, n, vd, nd, (size_t)npx, make_rigid<float>(R9, t3));
+}
+OR_EXPORT void or_scale_depths_f(float* depth, int W, int H, const float* cam6) {
+    scale_depths<float>(depth, W, H, make_cam<float>(cam6, W, H));
+}
+OR_EXPORT void or_scale_depths_d(double* depth, int W, int H, const float* cam6) {
+    scale_depths<double>(depth, W, H, make_cam<double>(cam6, W, H));
+}
+// 29 ICP sums (reference reduction order) at level `level` for pose (R9,t3)
+OR_EXPORT void or_icp_sums_f(const float* cam6, int W, int H, const float* vcur, int level, const float* R9,
+                             const float* t3, const float* vref, const float* nref, const float* prevR9,
+                             const float* prevt3, float distThr, double* sums29) {
+    Cam<float> cam = make_cam<float>(cam6, W, H);
+    std::vector<float> blocks;
+    int nb = icp_reduce<float>(cam, vcur, W >> level, H >> level, R9, t3, vref, nref,
+                               make_rigid<float>(prevR9, prevt3), distThr, blocks);
+    for (int k = 0; k < kTerms; ++k) {
+        float acc = 0;
+        for (int n = 0; n < nb; ++n) acc += blocks[(size_t)n * kTerms + k];
+        sums29[k] = acc;
+    }
+}
+OR_EXPORT void or_icp_sums_d(const float* cam6, int W, int H, const double* vcur, int level, const double* R9,
+                             const double* t3, const double* vref, const double* nref, const double* prevR9,
+                             const double* prevt3, double distThr, double* sums29) {
+    Cam<double> cam = make_cam<double>(cam6, W, H);
+    std::vector<double> blocks;
+    int nb = icp_reduce<double>(cam, vcur, W >> level, H >> level, R9, t3, vref, nref,
+                                make_rigid<double>(prevR9, prevt3), distThr, blocks);
+    for (int k = 0; k < kTerms; ++k) {
+        double acc = 0;
+        for (int n = 0; n < nb; ++n) acc += blocks[(size_t)n * kTerms + k];
+        sums29[k] = acc;
+    }
+}
+// Eigen-equivalent helpers exposed for their own closed-form tests
+OR_EXPORT int or_cholesky6(const double* A, const double* b, double* x, double* det) {
+    return cholesky6(A, b, x, det) ? 1 : 0;
+}
+OR_EXPORT double or_det6(const double* A) { return det6(A); }
+OR_EXPORT void or_axis_rot_d(double angle, int axis, double* R9) { axis_rot<double>(angle, axis, R9); }
+OR_EXPORT void or_inverse3_d(const double* m, double* out) { inverse3<double>(m, out); }
+OR_EXPORT double or_rodrigues_norm_d(const double* R9) { return rodrigues_norm<double>(R9); }
+
+// ================================================================ engine
+
+namespace {
+
+struct EngineCfg {
+    int W, H;
+    float cam[6];          // fx, fy, cx, cy, dmin, dmax
+    int filter_radius;
+    float sigma_space, sigma_depth, depth_threshold, dist_threshold;
+    int levels;
+    int iters[8];
+    int N;                 // dense volume resolution (cubic)
+    float dim;             // metres
+    float trunc;           // metres (before the 2.1-cell clamp)
+    float gate;            // integration metric threshold (0: integrate every frame)
+    int max_color_weight;
+};
+
+// Per-frame orchestration of src/SLAM/VisualOdometry/NuiKinfuTrackingEngine.cpp:68-142
+// with the OpenCL frame / feedback frame / depth tracker / dense scene.
+template <class T, class Vol> struct Engine {
+    EngineCfg c;
+    Cam<T> cam;
+    Tsdf<T> prm;
+    Vol vol;
+    std::vector<uint8_t> cvol;
+    std::vector<T> g, depth, filt, vref, nref;
+    std::vector<std::vector<T>> dpyr, vpyr;
+    T R[9], t[3];
+    T Rlast[9], tlast[3];
+    int frames = 0;
+    IcpResult last{1, 0, 0, 0};
+    long long nupd_last = 0;
+    bool integrated_last = false;
+
+    explicit Engine(const EngineCfg& cfg) : c(cfg) {
+        cam = make_cam<T>(c.cam, c.W, c.H);
+        prm = make_tsdf<T>(c.N, c.dim, c.trunc);
+        vol.init((size_t)c.N * c.N * c.N);
+        g.resize(2 * c.filter_radius + 1);
+        {
+            // host table in float as in OCL/NuiKinfuOpenCLFrame.cpp:181-186
+            std::vector<float> gf(g.size());
+            gaussian_table<float>(c.filter_radius, c.sigma_space, gf.data());
+            for (size_t i = 0; i < g.size(); ++i) g[i] = (T)gf[i];
+        }
+        const size_t n = (size_t)c.W * c.H;
+        depth.resize(n);
+        filt.resize(n);
+        vref.resize(3 * n);
+        nref.resize(3 * n);
+        dpyr.resize(c.levels);
+        vpyr.resize(c.levels);
+        for (int l = 0; l < c.levels; ++l) {
+            size_t nl = (size_t)(c.W >> l) * (c.H >> l);
+            dpyr[l].resize(nl);
+            vpyr[l].resize(3 * nl);
+        }
+        const T I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        std::memcpy(R, I, sizeof(I));
+        std::memcpy(Rlast, I, sizeof(I));
+        t[0] = t[1] = t[2] = 0;
+        tlast[0] = tlast[1] = tlast[2] = 0;
+    }
+
+    // returns 1 ok, 0 tracking failure
+    int run(const uint16_t* raw, const uint8_t* bgra) {
+        const int W = c.W, H = c.H;
+        const size_t n = (size_t)W * H;
+        // NuiKinfuOpenCLFrame::UpdateVertexBuffers (OCL/NuiKinfuOpenCLFrame.cpp:314-367)
+        passing_filter<T>(raw, depth.data(), n, cam.dmin, cam.dmax);
+        bilateral<T>(depth.data(), dpyr[0].data(), W, H, g.data(), c.filter_radius, (T)c.sigma_depth,
+                     (T)c.depth_threshold);
+        depth2vertex<T>(dpyr[0].data(), vpyr[0].data(), W, H, cam, 1);
+        const Rigid<T> pose = make_rigid<T>(R, t);
+        integrated_last = false;
+        if (frames == 0) {
+            // NuiKinfuOpenCLFeedbackFrame::UpdateBuffers: normals + camera->world
+            std::vector<T> nm(3 * n);
+            normals_simple<T>(vpyr[0].data(), nm.data(), W, H, (T)c.depth_threshold);
+            transform_maps<T>(vpyr[0].data(), nm.data(), vref.data(), nref.data(), n, pose);
+            scale_depths<T>(depth.data(), W, H, cam);
+            nupd_last = integrate<T, Vol>(depth.data(), bgra, vol, bgra ? cvol.data() : nullptr, prm, cam, pose,
+                                          c.max_color_weight);
+            integrated_last = true;
+        } else {
+            // NuiKinfuOpenCLDepthTracker::EstimatePose: pyramid, vertices, ICP
+            for (int l = 1; l < c.levels; ++l) {
+                half_sample<T>(dpyr[l - 1].data(), dpyr[l].data(), W >> l, H >> l, 1, (T)c.depth_threshold);
+                depth2vertex<T>(dpyr[l].data(), vpyr[l].data(), W >> l, H >> l, cam, 1 << l);
+            }
+            IcpConfig ic;
+            ic.levels = c.levels;
+            for (int l = 0; l < 8; ++l) ic.iters[l] = c.iters[l];
+            ic.dist_threshold = c.dist_threshold;
+            std::vector<const T*> vp(c.levels);
+            for (int l = 0; l < c.levels; ++l) vp[l] = vpyr[l].data();
+            T Rn[9], tn[3];
+            std::memcpy(Rn, R, sizeof(Rn));
+            std::memcpy(tn, t, sizeof(tn));
+            last = icp<T>(cam, ic, vp.data(), vref.data(), nref.data(), pose, Rn, tn);
+            if (!last.ok) return 0;
+            std::memcpy(R, Rn, sizeof(Rn));
+            std::memcpy(t, tn, sizeof(tn));
+            const Rigid<T> np = make_rigid<T>(R, t);
+            // integration gate (VO/NuiKinfuTrackingEngine.cpp:113-127)
+            T Rinv[9], Rrel[9];
+            inverse3(R, Rinv);
+            matmul3(Rinv, Rlast, Rrel);
+            T rnorm = rodrigues_norm<T>(Rrel);
+            T d0 = t[0] - tlast[0], d1 = t[1] - tlast[1], d2 = t[2] - tlast[2];
+            T tnorm = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            if ((rnorm + tnorm) / T(2) >= (T)c.gate) {
+                scale_depths<T>(depth.data(), W, H, cam);
+                nupd_last = integrate<T, Vol>(depth.data(), bgra, vol, bgra ? cvol.data() : nullptr, prm, cam, np,
+                                              c.max_color_weight);
+                integrated_last = true;
+                std::memcpy(Rlast, R, sizeof(Rlast));
+                std::memcpy(tlast, t, sizeof(tlast));
+            }
+            raycast<T, Vol>(vol, bgra ? cvol.data() : nullptr, prm, cam, np, vref.data(), nref.data(), nullptr);
+        }
+        frames++;
+        return 1;
+    }
+};
+
+template <class T, class Vol> struct EngineBox {
+    Engine<T, Vol> e;
+    explicit EngineBox(const EngineCfg& c) : e(c) {}
+};
+
+}  // namespace
+
+// Engine handle API.  prec: 0 = float compat (short2 volume), 1 = FP64 oracle.
+struct OrEngine {
+    int prec;
+    void* p;
+};
+
+OR_EXPORT void* or_engine_create(int prec, int W, int H, const float* cam6, int filter_radius, float sigma_space,
+                                 float sigma_depth, float depth_threshold, float dist_threshold, int levels,
+                                 const int* iters, int N, float dim, float trunc, float gate, int color) {
+    EngineCfg c;
+    c.W = W;
+    c.H = H;
+    std::memcpy(c.cam, cam6, sizeof(c.cam));
+    c.filter_radius = filter_radius;
+    c.sigma_space = sigma_space;
+    c.sigma_depth = sigma_depth;
+    c.depth_threshold = depth_threshold;
+    c.dist_threshold = dist_threshold;
+    c.levels = levels;
+    for (int l = 0; l < 8; ++l) c.iters[l] = l < levels ? iters[l] : 0;
+    c.N = N;
+    c.dim = dim;
+    c.trunc = trunc;
+    c.gate = gate;
+    c.max_color_weight = 128;
+    OrEngine* h = new OrEngine;
+    h->prec = prec;
+    if (prec == 0) {
+        auto* b = new EngineBox<float, VolS16>(c);
+        if (color) b->e.cvol.assign((size_t)N * N * N * 4, 0);
+        h->p = b;
+    } else {
+        auto* b = new EngineBox<double, VolF64>(c);
+        if (color) b->e.cvol.assign((size_t)N * N * N * 4, 0);
+        h->p = b;
+    }
+    return h;
+}
+
+OR_EXPORT void or_engine_destroy(void* hp) {
+    OrEngine* h = (OrEngine*)hp;
+    if (!h) return;
+    if (h->prec == 0)
+        delete (EngineBox<float, VolS16>*)h->p;
+    else
+        delete (EngineBox<double, VolF64>*)h->p;
+    delete h;
+}
+
+// Runs one frame. pose12 out: R (row-major) + t as double. Returns 1 ok / 0 failure.
+OR_EXPORT int or_engine_run(void* hp, const uint16_t* raw, const uint8_t* bgra, double* pose12, double* icp_info3) {
+    OrEngine* h = (OrEngine*)hp;
+    int ok;
+    auto fill = [&](auto& e) {
+        if (pose12) {
+            for (int k = 0; k < 9; ++k) pose12[k] = (double)e.R[k];
+            for (int k = 0; k < 3; ++k) pose12[9 + k] = (double)e.t[k];
+        }
+        if (icp_info3) {
+            icp_info3[0] = e.last.error;
+            icp_info3[1] = e.last.count;
+            icp_info3[2] = (double)e.nupd_last;
+        }
+    };
+    if (h->prec == 0) {
+        auto& e = ((EngineBox<float, VolS16>*)h->p)->e;
+        ok = e.run(raw, bgra);
+        fill(e);
+    } else {
+        auto& e = ((EngineBox<double, VolF64>*)h->p)->e;
+        ok = e.run(raw, bgra);
+        fill(e);
+    }
+    return ok;
+}
+
+// Volume readback: tsdf (float, normalised) and weight per voxel in the
+// reference layout ((y*N + x)*N + z).
+OR_EXPORT void or_engine_volume(void* hp, float* tsdf, int* weight) {
+    OrEngine* h = (OrEngine*)hp;
+    auto rd = [&](auto& e) {
+        size_t n = (size_t)e.c.N * e.c.N * e.c.N;
+        for (size_t i = 0; i < n; ++i) {
+            if (tsdf) tsdf[i] = e.vol.template tsdf<float>(i);
+            if (weight) weight[i] = e.vol.weight(i);
+        }
+    };
+    if (h->prec == 0)
+        rd(((EngineBox<float, VolS16>*)h->p)->e);
+    else
+        rd(((EngineBox<double, VolF64>*)h->p)->e);
+}
+
+// Raw short2 volume of the compat engine (2 x int16 per voxel).
+OR_EXPORT int or_engine_volume_raw(void* hp, int16_t* out) {
+    OrEngine* h = (OrEngine*)hp;
+    if (h->prec != 0) return -1;
+    auto& e = ((EngineBox<float, VolS16>*)h->p)->e;
+    std::memcpy(out, e.vol.v.data(), e.vol.v.size() * sizeof(int16_t));
+    return 0;
+}
+
+// Reference maps (world vmap/nmap, float) after the last frame.
+OR_EXPORT void or_engine_refmaps(void* hp, float* vmap, float* nmap) {
+    OrEngine* h = (OrEngine*)hp;
+    auto rd = [&](auto& e) {
+        for (size_t i = 0; i < e.vref.size(); ++i) {
+            if (vmap) vmap[i] = (float)e.vref[i];
+            if (nmap) nmap[i] = (float)e.nref[i];
+        }
+    };
+    if (h->prec == 0)
+        rd(((EngineBox<float, VolS16>*)h->p)->e);
+    else
+        rd(((EngineBox<double, VolF64>*)h->p)->e);
+}
+
+// ---- stateless dense stages on caller-provided short2 volumes (float compat)
+OR_EXPORT long long or_integrate_s16_f(const float* depth_scaled, const uint8_t* bgra, int16_t* vol2,
+                                       uint8_t* color_vol, int N, float dim, float trunc, const float* cam6, int W,
+                                       int H, const float* R9, const float* t3, int max_color_weight) {
+    VolS16 v;
+    v.v.assign(vol2, vol2 + 2 * (size_t)N * N * N);
+    long long nupd = integrate<float, VolS16>(depth_scaled, bgra, v, color_vol, make_tsdf<float>(N, dim, trunc),
+                                              make_cam<float>(cam6, W, H), make_rigid<float>(R9, t3),
+                                              max_color_weight);
+    std::memcpy(vol2, v.v.data(), v.v.size() * sizeof(int16_t));
+    return nupd;
+}
+OR_EXPORT void or_raycast_s16_f(const int16_t* vol2, const uint8_t* color_vol, int N, float dim, float trunc,
+                                const float* cam6, int W, int H, const float* R9, const float* t3, float* vmap,
+                                float* nmap, uint8_t* cmap) {
+    VolS16 v;
+    v.v.assign(vol2, vol2 + 2 * (size_t)N * N * N);
+    raycast<float, VolS16>(v, color_vol, make_tsdf<float>(N, dim, trunc), make_cam<float>(cam6, W, H),
+                           make_rigid<float>(R9, t3), vmap, nmap, cmap);
+}
+OR_EXPORT float or_effective_trunc(int N, float dim, float trunc) { return make_tsdf<float>(N, dim, trunc).trunc; }

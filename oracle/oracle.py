@@ -1,0 +1,120 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (liboracle.so).
+
+Used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as the
+checker.  The product library never loads it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+i16p = np.ctypeslib.ndpointer(np.int16, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+ci, cf, cd, cll = C.c_int, C.c_float, C.c_double, C.c_longlong
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_PATH):
+            raise RuntimeError(f"{_PATH} missing: run `make -C oracle`")
+        _lib = C.CDLL(_PATH)
+        sig = {
+            "or_passing_filter_f": (None, [u16p, f32p, cll, cf, cf]),
+            "or_passing_filter_d": (None, [u16p, f64p, cll, cd, cd]),
+            "or_gaussian_table_f": (None, [ci, cf, f32p]),
+            "or_bilateral_f": (None, [f32p, f32p, ci, ci, f32p, ci, cf, cf]),
+            "or_bilateral_d": (None, [f64p, f64p, ci, ci, f64p, ci, cd, cd]),
+            "or_half_sample_f": (None, [f32p, f32p, ci, ci, ci, cf]),
+            "or_half_sample_d": (None, [f64p, f64p, ci, ci, ci, cd]),
+            "or_depth2vertex_f": (None, [f32p, f32p, ci, ci, f32p, ci, ci, ci]),
+            "or_depth2vertex_d": (None, [f64p, f64p, ci, ci, f32p, ci, ci, ci]),
+            "or_normals_simple_f": (None, [f32p, f32p, ci, ci, cf]),
+            "or_normals_simple_d": (None, [f64p, f64p, ci, ci, cd]),
+            "or_transform_maps_f": (None, [f32p, f32p, f32p, f32p, cll, f32p, f32p]),
+            "or_scale_depths_f": (None, [f32p, ci, ci, f32p]),
+            "or_scale_depths_d": (None, [f64p, ci, ci, f32p]),
+            "or_icp_sums_f": (None, [f32p, ci, ci, f32p, ci, f32p, f32p, f32p, f32p, f32p, f32p, cf, f64p]),
+            "or_icp_sums_d": (None, [f32p, ci, ci, f64p, ci, f64p, f64p, f64p, f64p, f64p, f64p, cd, f64p]),
+            "or_cholesky6": (ci, [f64p, f64p, f64p, f64p]),
+            "or_det6": (cd, [f64p]),
+            "or_axis_rot_d": (None, [cd, ci, f64p]),
+            "or_inverse3_d": (None, [f64p, f64p]),
+            "or_rodrigues_norm_d": (cd, [f64p]),
+            "or_engine_create": (vp, [ci, ci, ci, f32p, ci, cf, cf, cf, cf, ci, i32p, ci, cf, cf, cf, ci]),
+            "or_engine_destroy": (None, [vp]),
+            "or_engine_run": (ci, [vp, u16p, vp, f64p, f64p]),
+            "or_engine_volume": (None, [vp, f32p, i32p]),
+            "or_engine_volume_raw": (ci, [vp, i16p]),
+            "or_engine_refmaps": (None, [vp, f32p, f32p]),
+            "or_integrate_s16_f": (cll, [f32p, vp, i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, ci]),
+            "or_raycast_s16_f": (None, [i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, f32p, f32p, vp]),
+            "or_effective_trunc": (cf, [ci, cf, cf]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(_lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+class OracleEngine:
+    """RunTracking of the reference over a dense scene, on the CPU.
+
+    prec 0: float arithmetic of the reference with short2 volume; prec 1: FP64 oracle.
+    """
+
+    def __init__(self, wl, prec=0, gate=0.0, color=False, iterations=None):
+        its = np.array(list(iterations or wl.iterations) + [0] * 8, np.int32)[:8]
+        self.wl = wl
+        self.prec = prec
+        self.h = lib().or_engine_create(prec, wl.width, wl.height, wl.cam6(), 3, 0.08, 50.0, 0.3, 0.1,
+                                        len(iterations or wl.iterations), its, wl.volume_res, wl.volume_dim,
+                                        wl.trunc, gate, 1 if color else 0)
+
+    def run(self, depth, bgra=None):
+        pose = np.zeros(12, np.float64)
+        info = np.zeros(3, np.float64)
+        ok = lib().or_engine_run(self.h, np.ascontiguousarray(depth, np.uint16),
+                                 None if bgra is None else np.ascontiguousarray(bgra).ctypes.data, pose, info)
+        return ok, pose, info
+
+    def volume(self):
+        n = self.wl.volume_res ** 3
+        t = np.empty(n, np.float32)
+        w = np.empty(n, np.int32)
+        lib().or_engine_volume(self.h, t, w)
+        return t, w
+
+    def volume_raw(self):
+        n = self.wl.volume_res ** 3
+        out = np.empty(2 * n, np.int16)
+        lib().or_engine_volume_raw(self.h, out)
+        return out
+
+    def refmaps(self):
+        n = self.wl.width * self.wl.height * 3
+        v = np.empty(n, np.float32)
+        nm = np.empty(n, np.float32)
+        lib().or_engine_refmaps(self.h, v, nm)
+        return v.reshape(-1, 3), nm.reshape(-1, 3)
+
+    def close(self):
+        if self.h:
+            lib().or_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

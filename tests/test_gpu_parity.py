@@ -1,0 +1,375 @@
+"""GPU parity of the HIP kernels against the CPU oracle (float "compat"
+flavour, which follows the reference's float arithmetic operation by
+operation) and, for end-to-end tracking, against the FP64 oracle.
+
+Tolerances (stated per test):
+  * integer / decision outputs (NaN patterns, counts, raw int16 voxels): exact,
+    with a small budget for voxels or pixels whose float decision sits on a
+    rounding boundary (expf on the device differs from glibc by <= 1 ulp);
+  * poses: <= 0.1 mm and <= 0.01 deg per frame against the float oracle;
+  * TSDF: <= 1e-3 of the truncation distance; raycast depth <= 0.5 mm.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from natureuserinterfacestudio_amd import nuif
+from natureuserinterfacestudio_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def ulp_close(a, b, ulps=4):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    nan_a, nan_b = np.isnan(a), np.isnan(b)
+    assert np.array_equal(nan_a, nan_b), f"NaN patterns differ at {np.argwhere(nan_a != nan_b)[:5]}"
+    m = ~nan_a
+    tol = ulps * np.spacing(np.maximum(np.abs(a[m]), np.abs(b[m])))
+    bad = np.abs(a[m] - b[m]) > tol
+    assert not bad.any(), f"{bad.sum()} values beyond {ulps} ulp; max diff {np.abs(a[m] - b[m]).max()}"
+
+
+def rot_y(deg):
+    a = np.radians(deg)
+    return np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]], np.float32)
+
+
+def pose_err(Ra, ta, Rb, tb):
+    dt = np.linalg.norm(np.asarray(ta, np.float64) - np.asarray(tb, np.float64))
+    Rr = np.asarray(Ra, np.float64).T @ np.asarray(Rb, np.float64)
+    ang = np.degrees(np.arccos(np.clip((np.trace(Rr) - 1) / 2, -1, 1)))
+    return dt, ang
+
+
+# ------------------------------------------------------------------ preprocessing
+
+def test_passing_filter_bit_exact(gpu):
+    announce("passing filter 320x240 frame 3")
+    d, _, _ = W.SMALL.render(3)
+    raw = nuif.DeviceArray.from_host(d)
+    out = nuif.DeviceArray(d.shape, np.float32)
+    nuif.check(gpu.L().nuif_depth_passing_filter(raw.ptr, out.ptr, d.size, 0.5, 4.5, None))
+    ref = np.empty(d.size, np.float32)
+    O.lib().or_passing_filter_f(d.reshape(-1), ref, d.size, 0.5, 4.5)
+    np.testing.assert_array_equal(out.download().reshape(-1), ref)
+
+
+def test_bilateral_matches_oracle(gpu):
+    announce("bilateral 320x240 frame 5, radius 3")
+    d, _, _ = W.SMALL.render(5)
+    dep = np.empty(d.size, np.float32)
+    O.lib().or_passing_filter_f(d.reshape(-1), dep, d.size, 0.5, 4.5)
+    g = np.empty(7, np.float32)
+    O.lib().or_gaussian_table_f(3, 0.08, g)
+    ref = np.empty_like(dep)
+    O.lib().or_bilateral_f(dep, ref, 320, 240, g, 3, 50.0, 0.3)
+    din = nuif.DeviceArray.from_host(dep)
+    dout = nuif.DeviceArray(dep.shape, np.float32)
+    nuif.check(gpu.L().nuif_bilateral_filter_depth(din.ptr, dout.ptr, 320, 240, g.ctypes.data, 3, 50.0, 0.3, None))
+    ulp_close(dout.download(), ref, ulps=8)  # device expf vs glibc expf
+
+
+def test_fused_preprocess_matches_oracle_chain(gpu):
+    for wl in (W.SMALL, W.FULL):
+        announce(f"fused preprocess {wl.name}")
+        Wd, Hd = wl.width, wl.height
+        d, _, _ = wl.render(7)
+        cam = wl.camera_params()
+        g = np.empty(7, np.float32)
+        O.lib().or_gaussian_table_f(3, 0.08, g)
+        raw = nuif.DeviceArray.from_host(d)
+        dout = nuif.DeviceArray((Hd, Wd), np.float32)
+        dp = [nuif.DeviceArray(((Hd >> l) * (Wd >> l),), np.float32) for l in range(3)]
+        vp = [nuif.DeviceArray(((Hd >> l) * (Wd >> l), 3), np.float32) for l in range(3)]
+        dptr = (C.c_void_p * 3)(*[a.ptr for a in dp])
+        vptr = (C.c_void_p * 3)(*[a.ptr for a in vp])
+        nuif.check(gpu.L().nuif_frame_preprocess(raw.ptr, dout.ptr, dptr, vptr, 3, Wd, Hd, C.byref(cam),
+                                                 g.ctypes.data, 3, 50.0, 0.3, 1, None))
+        dep = np.empty(d.size, np.float32)
+        O.lib().or_passing_filter_f(d.reshape(-1), dep, d.size, 0.5, 4.5)
+        filt = np.empty_like(dep)
+        O.lib().or_bilateral_f(dep, filt, Wd, Hd, g, 3, 50.0, 0.3)
+        scaled = dep.copy()
+        O.lib().or_scale_depths_f(scaled, Wd, Hd, wl.cam6())
+        np.testing.assert_array_equal(dout.download().reshape(-1), scaled)
+        gdep = [a.download() for a in dp]
+        ulp_close(gdep[0], filt, ulps=8)
+        # downstream levels are checked from the GPU's own level-0 output, so the
+        # comparison isolates half_sample + depth2vertex
+        src = gdep[0]
+        for l in range(3):
+            wl_, hl = Wd >> l, Hd >> l
+            if l > 0:
+                h = np.empty(wl_ * hl, np.float32)
+                O.lib().or_half_sample_f(np.ascontiguousarray(gdep[l - 1]), h, wl_, hl, 1, 0.3)
+                ulp_close(gdep[l], h, ulps=1)
+                src = gdep[l]
+            v = np.empty(wl_ * hl * 3, np.float32)
+            O.lib().or_depth2vertex_f(np.ascontiguousarray(src), v, wl_, hl, wl.cam6(), Wd, Hd, 1 << l)
+            np.testing.assert_array_equal(vp[l].download().reshape(-1), v)
+
+
+def test_normals_and_transform_bit_exact(gpu):
+    announce("normals + transform 640x480 frame 2")
+    wl = W.FULL
+    d, _, _ = wl.render(2)
+    dep = np.empty(d.size, np.float32)
+    O.lib().or_passing_filter_f(d.reshape(-1), dep, d.size, 0.5, 4.5)
+    v = np.empty(d.size * 3, np.float32)
+    O.lib().or_depth2vertex_f(dep, v, wl.width, wl.height, wl.cam6(), wl.width, wl.height, 1)
+    nref = np.empty_like(v)
+    O.lib().or_normals_simple_f(v, nref, wl.width, wl.height, 0.3)
+    dv = nuif.DeviceArray.from_host(v)
+    dn = nuif.DeviceArray(v.shape, np.float32)
+    nuif.check(gpu.L().nuif_estimate_normals_simple(dv.ptr, dn.ptr, wl.width, wl.height, 0.3, None))
+    np.testing.assert_array_equal(dn.download(), nref)
+    R = rot_y(12.0)
+    t = np.array([0.1, -0.05, 0.2], np.float32)
+    pose = nuif.upload_struct(nuif.rigid(R, t))
+    vd = nuif.DeviceArray(v.shape, np.float32)
+    nd = nuif.DeviceArray(v.shape, np.float32)
+    nuif.check(gpu.L().nuif_transform_maps(dv.ptr, dn.ptr, vd.ptr, nd.ptr, wl.width, wl.height, pose.ptr, None))
+    vr, nr = np.empty_like(v), np.empty_like(v)
+    O.lib().or_transform_maps_f(v.reshape(-1, 3).copy(), nref.reshape(-1, 3).copy(), vr.reshape(-1, 3),
+                                nr.reshape(-1, 3), d.size, R.reshape(-1).copy(), t)
+    np.testing.assert_array_equal(vd.download(), vr)
+    np.testing.assert_array_equal(nd.download(), nr)
+
+
+# ------------------------------------------------------------------ ICP
+
+def _oracle_state(wl, frames):
+    """Oracle engine run over `frames` frames; returns it and its reference maps."""
+    e = O.OracleEngine(wl, prec=0, gate=0.0)
+    for f in range(frames):
+        d, _, _ = wl.render(f)
+        e.run(d)
+    return e
+
+
+def test_icp_iteration_sums(gpu):
+    announce("ICP sums, 320x240, frame 2 vs oracle maps of frame 1, all levels")
+    wl = W.SMALL
+    e = _oracle_state(wl, 2)
+    vref, nref = e.refmaps()
+    d, _, gt = wl.render(2)
+    # current vertex pyramid from the oracle chain
+    dep = np.empty(d.size, np.float32)
+    O.lib().or_passing_filter_f(d.reshape(-1), dep, d.size, 0.5, 4.5)
+    g = np.empty(7, np.float32)
+    O.lib().or_gaussian_table_f(3, 0.08, g)
+    filt = np.empty_like(dep)
+    O.lib().or_bilateral_f(dep, filt, 320, 240, g, 3, 50.0, 0.3)
+    R = gt[:, :3].astype(np.float32).copy()
+    t = gt[:, 3].astype(np.float32).copy()
+    prevR = rot_y(-2.0)
+    prevt = np.array([0.03, 0.0, 0.001], np.float32)
+    cam = wl.camera_params()
+    dvr, dnr = nuif.DeviceArray.from_host(vref), nuif.DeviceArray.from_host(nref)
+    prev = nuif.upload_struct(nuif.rigid(prevR, prevt))
+    st = nuif.DeviceArray((gpu.L().nuif_icp_state_bytes(),), np.uint8)
+    part = nuif.DeviceArray((gpu.L().nuif_icp_partials_bytes(320, 240),), np.uint8)
+    src = filt
+    for level in range(3):
+        wl_, hl = 320 >> level, 240 >> level
+        if level > 0:
+            h = np.empty(wl_ * hl, np.float32)
+            O.lib().or_half_sample_f(np.ascontiguousarray(src), h, wl_, hl, 1, 0.3)
+            src = h
+        v = np.empty(wl_ * hl * 3, np.float32)
+        O.lib().or_depth2vertex_f(np.ascontiguousarray(src), v, wl_, hl, wl.cam6(), 320, 240, 1 << level)
+        dv = nuif.DeviceArray.from_host(v)
+        start = nuif.upload_struct(nuif.rigid(R, t))
+        nuif.check(gpu.L().nuif_icp_state_init(st.ptr, start.ptr, None))
+        nuif.check(gpu.L().nuif_icp_begin(st.ptr, start.ptr, None))
+        nuif.check(gpu.L().nuif_icp_iteration(st.ptr, part.ptr, C.byref(cam), dv.ptr, level, dvr.ptr, dnr.ptr,
+                                              prev.ptr, 0.1333, 0, 0, None))
+        res = nuif.IcpResult()
+        nuif.check(gpu.L().nuif_icp_state_read(st.ptr, C.byref(res), None))
+        s_gpu = np.array(res.sums)
+        s_f = np.empty(29)
+        O.lib().or_icp_sums_f(wl.cam6(), 320, 240, v, level, R.reshape(-1).copy(), t, vref.reshape(-1).copy(),
+                              nref.reshape(-1).copy(), prevR.reshape(-1).copy(), prevt, 0.1333, s_f)
+        s_d = np.empty(29)
+        O.lib().or_icp_sums_d(wl.cam6(), 320, 240, v.astype(np.float64), level, R.reshape(-1).astype(np.float64),
+                              t.astype(np.float64), vref.reshape(-1).astype(np.float64),
+                              nref.reshape(-1).astype(np.float64), prevR.reshape(-1).astype(np.float64),
+                              prevt.astype(np.float64), 0.1333, s_d)
+        assert s_gpu[28] > 100
+        # correspondence decisions are float decisions identical to the reference's
+        assert s_gpu[28] == s_f[28]
+        # the compensated FP32 + FP64 sums are at least as close to the FP64 sums as the
+        # reference's float tree
+        scale = np.abs(s_d[:28]).max()
+        assert np.abs(s_gpu[:28] - s_d[:28]).max() <= max(np.abs(s_f[:28] - s_d[:28]).max(), 1e-6 * scale) + 1e-9
+
+
+# ------------------------------------------------------------------ dense TSDF
+
+def _small_volume_inputs(wl, frame):
+    """Ray-length depth of a frame (passing filter + scaleDepths of the oracle)."""
+    d, _, _ = wl.render(frame)
+    dep = np.empty(d.size, np.float32)
+    O.lib().or_passing_filter_f(d.reshape(-1), dep, d.size, 0.5, 4.5)
+    O.lib().or_scale_depths_f(dep, wl.width, wl.height, wl.cam6())
+    return dep
+
+
+@pytest.mark.parametrize("res", [128, 96])  # 96^3 runs the generic column kernel (96 % 64 != 0)
+def test_integrate_bit_exact_s16(gpu, res):
+    announce(f"integrate S16 {res}^3 320x240, 3 frames with motion")
+    wl = W.SMALL
+    N = res
+    p = nuif.tsdf_params(N, wl.volume_dim, wl.trunc)
+    vol = np.zeros((N ** 3, 2), np.int16)
+    vol[:, 0] = 32767
+    dvol = nuif.DeviceArray.from_host(vol)
+    cam = wl.camera_params()
+    for k, f in enumerate([0, 4, 9]):
+        R = rot_y(-2.0 * f).T.copy()
+        t = np.array([0.01 * k, -0.005 * k, 0.02 * k], np.float32)
+        dep = _small_volume_inputs(wl, f)
+        ddep = nuif.DeviceArray.from_host(dep)
+        pose = nuif.upload_struct(nuif.rigid(R, t))
+        nuif.check(gpu.L().nuif_tsdf_integrate(ddep.ptr, None, dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr,
+                                               0, 128, None, None, None))
+        O.lib().or_integrate_s16_f(dep, None, vol.reshape(-1), None, N, wl.volume_dim, wl.trunc, wl.cam6(),
+                                   wl.width, wl.height, R.reshape(-1).copy(), t, 128)
+    got = dvol.download()
+    same = np.all(got == vol, axis=1)
+    assert (vol[:, 1] > 0).sum() > 10000
+    assert same.mean() == 1.0, f"{(~same).sum()} voxels differ"
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_integrate_f16_f32_close_to_fp32_math(gpu, storage):
+    announce(f"integrate storage {storage} 128^3 vs float oracle")
+    wl = W.SMALL
+    N = 128
+    p = nuif.tsdf_params(N, wl.volume_dim, wl.trunc)
+    vb = gpu.L().nuif_tsdf_voxel_bytes(storage)
+    dvol = nuif.DeviceArray((N ** 3 * vb,), np.uint8)
+    nuif.check(gpu.L().nuif_tsdf_reset(dvol.ptr, None, C.byref(p), storage, None))
+    cam = wl.camera_params()
+    vol = np.zeros((N ** 3, 2), np.int16)
+    vol[:, 0] = 32767
+    R = np.eye(3, dtype=np.float32)
+    t = np.zeros(3, np.float32)
+    dep = _small_volume_inputs(wl, 0)
+    ddep = nuif.DeviceArray.from_host(dep)
+    pose = nuif.upload_struct(nuif.rigid(R, t))
+    for _ in range(3):
+        nuif.check(gpu.L().nuif_tsdf_integrate(ddep.ptr, None, dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr,
+                                               storage, 128, None, None, None))
+        O.lib().or_integrate_s16_f(dep, None, vol.reshape(-1), None, N, wl.volume_dim, wl.trunc, wl.cam6(),
+                                   wl.width, wl.height, R.reshape(-1).copy(), t, 128)
+    tg, wg = nuif.decode_volume(dvol.download(), storage)
+    tr, wr = vol[:, 0] / np.float32(32766), vol[:, 1]
+    np.testing.assert_array_equal(wg, wr)
+    m = wr > 0
+    # FP16 storage: half-precision rounding of values in [-1, 1] (<= 2^-11); S16 RTZ bias <= 3/32766
+    tol = 1e-3 if storage == 1 else 2e-4
+    assert np.abs(tg[m] - tr[m]).max() <= tol
+
+
+def test_raycast_matches_oracle(gpu):
+    announce("raycast S16 128^3 320x240 after 4 integrations")
+    wl = W.SMALL
+    e = _oracle_state(wl, 4)
+    vol = e.volume_raw()
+    N = wl.volume_res
+    p = nuif.tsdf_params(N, wl.volume_dim, wl.trunc)
+    cam = wl.camera_params()
+    _, _, gt = wl.render(4)
+    R = gt[:, :3].astype(np.float32).copy()
+    t = gt[:, 3].astype(np.float32).copy()
+    dvol = nuif.DeviceArray.from_host(vol)
+    pose = nuif.upload_struct(nuif.rigid(R, t))
+    vm = nuif.DeviceArray((wl.width * wl.height, 3), np.float32)
+    nm = nuif.DeviceArray((wl.width * wl.height, 3), np.float32)
+    nuif.check(gpu.L().nuif_tsdf_raycast(dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr, vm.ptr, nm.ptr, None, 0,
+                                         None))
+    vr = np.empty(wl.width * wl.height * 3, np.float32)
+    nr = np.empty_like(vr)
+    O.lib().or_raycast_s16_f(vol, None, N, wl.volume_dim, wl.trunc, wl.cam6(), wl.width, wl.height,
+                             R.reshape(-1).copy(), t, vr, nr, None)
+    vg, ng = vm.download().reshape(-1), nm.download().reshape(-1)
+    assert (~np.isnan(vr)).sum() > 3 * 5000
+    np.testing.assert_array_equal(np.isnan(vg), np.isnan(vr))
+    np.testing.assert_array_equal(vg, vr)
+    np.testing.assert_array_equal(np.isnan(ng), np.isnan(nr))
+    np.testing.assert_array_equal(ng, nr)
+
+
+# ------------------------------------------------------------------ end to end
+
+def _run_engine(wl, frames, storage=0, use_graph=False, gate=0.0):
+    eng = nuif.Engine(wl.kinfu_desc(storage=storage, gate=gate, use_graph=use_graph))
+    for f in range(frames):
+        d, _, _ = wl.render(f)
+        dd = nuif.DeviceArray.from_host(d)
+        eng.run(dd, sync=False)
+    eng.synchronize()
+    return eng, eng.poses(frames)
+
+
+def test_engine_small_matches_float_oracle(gpu):
+    announce("engine 320x240 128^3, 12 frames vs float oracle")
+    wl = W.SMALL
+    n = 12
+    eng, poses = _run_engine(wl, n)
+    ref = O.OracleEngine(wl, prec=0, gate=0.0)
+    for f in range(n):
+        ok, pr, _ = ref.run(wl.render(f)[0])
+        assert ok == 1 and poses[f, 12] == 1.0
+        dt, ang = pose_err(poses[f, :9].reshape(3, 3), poses[f, 9:12], pr[:9].reshape(3, 3), pr[9:])
+        assert dt < 1e-4 and ang < 0.01, (f, dt, ang)
+
+
+def test_engine_graph_replay_identical(gpu):
+    announce("engine 320x240 graph vs stream, 8 frames")
+    wl = W.SMALL
+    _, p1 = _run_engine(wl, 8, use_graph=False)
+    e2, p2 = _run_engine(wl, 8, use_graph=True)
+    np.testing.assert_array_equal(p1, p2)
+
+
+def test_engine_full_size_against_oracles(gpu):
+    announce("engine 640x480 512^3, 4 frames vs float and FP64 oracles")
+    wl = W.FULL
+    n = 4
+    eng, poses = _run_engine(wl, n)
+    o32 = O.OracleEngine(wl, prec=0, gate=0.0)
+    o64 = O.OracleEngine(wl, prec=1, gate=0.0)
+    for f in range(n):
+        d = wl.render(f)[0]
+        _, p32, _ = o32.run(d)
+        _, p64, _ = o64.run(d)
+        dt, ang = pose_err(poses[f, :9].reshape(3, 3), poses[f, 9:12], p32[:9].reshape(3, 3), p32[9:])
+        assert dt < 1e-4 and ang < 0.01, ("float oracle", f, dt, ang)
+        dt, ang = pose_err(poses[f, :9].reshape(3, 3), poses[f, 9:12], p64[:9].reshape(3, 3), p64[9:])
+        assert dt < 1e-4 and ang < 0.01, ("FP64 oracle", f, dt, ang)
+    # TSDF against the float oracle's short2 volume: identical voxels
+    raw = eng.volume_raw().view(np.int16).reshape(-1, 2)
+    ref = o32.volume_raw().reshape(-1, 2)
+    same = np.all(raw == ref, axis=1)
+    assert same.mean() > 0.99999, (~same).sum()
+    t_g, w_g = raw[:, 0] / 32766.0, raw[:, 1]
+    t_r = ref[:, 0] / 32766.0
+    m = (w_g > 0) & (ref[:, 1] > 0)
+    assert np.abs(t_g[m] - t_r[m]).max() <= 1e-3
+    # raycast depth (reference maps for the next frame) within 0.5 mm
+    vg, _ = eng.maps()
+    vr, _ = o32.refmaps()
+    ok = ~np.isnan(vg[:, 2]) & ~np.isnan(vr[:, 2])
+    assert ok.sum() > 50000
+    assert np.quantile(np.abs(vg[ok] - vr[ok]).max(1), 0.999) < 5e-4

@@ -1,0 +1,390 @@
+"""Pins the CPU oracle against closed forms, invariants and brute force.
+
+The reference ships no test vectors for this path, so each oracle function is
+checked against what the mathematics fixes, chosen so that a plausible slip
+(a dropped term, a wrong sign or index, a transposed operand, a non-strict
+comparison) fails.  CPU only.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle as O
+from natureuserinterfacestudio_amd import workloads as W
+
+CAM = np.array([262.5, 262.5, 159.5, 119.5, 0.5, 4.5], np.float32)
+
+
+def lib():
+    return O.lib()
+
+
+# ---------------------------------------------------------------- H1 passing filter
+
+def test_passing_filter_strict_range():
+    raw = np.array([0, 500, 501, 1234, 4499, 4500, 4501, 65535], np.uint16)
+    out = np.empty(raw.size, np.float32)
+    lib().or_passing_filter_f(raw, out, raw.size, 0.5, 4.5)
+    valid = ~np.isnan(out)
+    # K/filters.cl:4-7: strict min < d < max; 500 mm and 4500 mm are rejected
+    assert valid.tolist() == [False, False, True, True, True, False, False, False]
+    np.testing.assert_allclose(out[valid], raw[valid] / 1000.0, rtol=2e-7)
+
+
+# ---------------------------------------------------------------- H2/H3 bilateral
+
+def gauss(r=3, s=0.08):
+    g = np.empty(2 * r + 1, np.float32)
+    lib().or_gaussian_table_f(r, s, g)
+    return g
+
+
+def test_gaussian_table():
+    g = gauss()
+    assert g[3] == 1.0
+    np.testing.assert_array_equal(g, g[::-1])
+    np.testing.assert_allclose(g[2], np.exp(-0.08), rtol=1e-6)
+    np.testing.assert_allclose(g[0], np.exp(-9 * 0.08), rtol=1e-6)
+
+
+def bilateral_f(img, r=3, sd=50.0, thr=0.3):
+    H, W = img.shape
+    out = np.empty_like(img)
+    lib().or_bilateral_f(np.ascontiguousarray(img), out, W, H, gauss(r), r, sd, thr)
+    return out
+
+
+def test_bilateral_constant_plane_is_identity():
+    img = np.full((24, 32), 2.0, np.float32)
+    np.testing.assert_array_equal(bilateral_f(img), img)
+
+
+def test_bilateral_step_edge_keeps_sides_and_nan():
+    img = np.full((24, 32), 1.0, np.float32)
+    img[:, 16:] = 1.5  # 0.5 m jump > depth threshold 0.3: sides never mix
+    img[5, 5] = np.nan
+    out = bilateral_f(img)
+    assert np.isnan(out[5, 5])
+    m = ~np.isnan(img)
+    np.testing.assert_allclose(out[m], img[m], rtol=4e-7)  # a weighted mean of equal values
+
+
+def test_bilateral_brute_force_double():
+    rng = np.random.default_rng(7)
+    img = (1.0 + 0.2 * rng.random((16, 16))).astype(np.float64)
+    img[rng.random((16, 16)) < 0.1] = np.nan
+    img[3, 4] = 0.0  # non-positive neighbours never contribute
+    r, s_sp, s_d, thr = 3, 0.08, 50.0, 0.3
+    g = np.exp(-(np.arange(-r, r + 1) ** 2) * s_sp)
+    out = np.empty_like(img)
+    lib().or_bilateral_d(img, out, 16, 16, g, r, s_d, thr)
+    ref = np.full_like(img, np.nan)
+    for y in range(16):
+        for x in range(16):
+            c = img[y, x]
+            if np.isnan(c):
+                continue
+            num = den = 0.0
+            for yy in range(max(0, y - r), min(16, y + r + 1)):
+                for xx in range(max(0, x - r), min(16, x + r + 1)):
+                    n = img[yy, xx]
+                    if np.isnan(n) or not (n > 0) or not (abs(c - n) < thr):
+                        continue
+                    w = g[xx - x + r] * g[yy - y + r] * np.exp(-(c - n) ** 2 * s_d)
+                    num += n * w
+                    den += w
+            ref[y, x] = num / den if den > 0 else np.nan  # 0/0 for a zero centre
+    np.testing.assert_allclose(out, ref, rtol=1e-13, equal_nan=True)
+
+
+# ---------------------------------------------------------------- H4 pyramid
+
+def half(src, thr=0.3):
+    Hs, Ws = src.shape
+    dst = np.empty((Hs // 2, Ws // 2), np.float32)
+    lib().or_half_sample_f(np.ascontiguousarray(src, np.float32), dst, Ws // 2, Hs // 2, 1, thr)
+    return dst
+
+
+def test_half_sample_closed_forms():
+    src = np.full((8, 8), 2.0, np.float32)
+    np.testing.assert_array_equal(half(src), np.full((4, 4), 2.0, np.float32))
+    src = np.arange(64, dtype=np.float32).reshape(8, 8) * 0.01 + 1.0
+    out = half(src)
+    # interior pixel (1,1) <- mean of the 3x3 around source (2,2)
+    np.testing.assert_allclose(out[1, 1], src[1:4, 1:4].mean(), rtol=1e-6)
+    # corner (0,0) <- window clipped to the image: rows/cols 0..1
+    np.testing.assert_allclose(out[0, 0], src[0:2, 0:2].mean(), rtol=1e-6)
+    # NaN centre: mean of all valid neighbours regardless of the threshold
+    src2 = src.copy()
+    src2[2, 2] = np.nan
+    src2[1, 1] = 9.0
+    m = src2[1:4, 1:4]
+    np.testing.assert_allclose(half(src2)[1, 1], np.nanmean(m), rtol=1e-6)
+    # neighbour beyond the threshold is excluded
+    src3 = src.copy()
+    src3[1, 1] = 9.0
+    w = src3[1:4, 1:4].copy().reshape(-1)
+    np.testing.assert_allclose(half(src3)[1, 1], w[w < 5].mean(), rtol=1e-6)
+    # all NaN -> NaN
+    assert np.isnan(half(np.full((4, 4), np.nan, np.float32))).all()
+
+
+# ---------------------------------------------------------------- H5 / H6 / H7 / H12
+
+def test_depth2vertex_intrinsics_per_level():
+    W_, H_ = 320, 240
+    for level in (0, 1, 2):
+        w, h, div = W_ >> level, H_ >> level, 1 << level
+        d = np.full((h, w), 2.0, np.float32)
+        v = np.empty((h * w * 3,), np.float32)
+        lib().or_depth2vertex_f(d, v, w, h, CAM, W_, H_, div)
+        v = v.reshape(h, w, 3)
+        ys, xs = np.mgrid[0:h, 0:w]
+        np.testing.assert_allclose(v[..., 0], 2.0 * (xs * div - CAM[2]) / CAM[0], rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(v[..., 1], 2.0 * (ys * div - CAM[3]) / CAM[1], rtol=1e-5, atol=1e-6)
+        np.testing.assert_array_equal(v[..., 2], 2.0)
+
+
+def test_normals_of_tilted_plane():
+    a, b, c = 0.3, -0.2, 2.0
+    ys, xs = np.mgrid[0:12, 0:16].astype(np.float64) * 0.01
+    v = np.stack([xs, ys, a * xs + b * ys + c], -1)
+    n = np.empty_like(v)
+    lib().or_normals_simple_d(np.ascontiguousarray(v), n, 16, 12, 0.3)
+    expect = np.array([-a, -b, 1.0]) / np.sqrt(a * a + b * b + 1)
+    np.testing.assert_allclose(n[1:-1, 1:-1].reshape(-1, 3), np.tile(expect, (10 * 14, 1)), atol=1e-12)
+    v[4, 4] = np.nan
+    lib().or_normals_simple_d(np.ascontiguousarray(v), n, 16, 12, 0.3)
+    assert np.isnan(n[4, 4]).all() and not np.isnan(n[4, 5]).any()
+
+
+def test_transform_maps_round_trip():
+    rng = np.random.default_rng(3)
+    ang = 0.3
+    R = np.array([[np.cos(ang), 0, np.sin(ang)], [0, 1, 0], [-np.sin(ang), 0, np.cos(ang)]], np.float32)
+    t = np.array([0.1, -0.2, 0.3], np.float32)
+    v = rng.random((50, 3)).astype(np.float32)
+    n = rng.random((50, 3)).astype(np.float32)
+    v[7] = np.nan
+    vd, nd = np.empty_like(v), np.empty_like(n)
+    lib().or_transform_maps_f(v, n, vd, nd, 50, R.reshape(-1).copy(), t)
+    ok = ~np.isnan(v[:, 0])
+    np.testing.assert_allclose(vd[ok], v[ok] @ R.T + t, atol=1e-6)
+    np.testing.assert_allclose(nd[ok], n[ok] @ R.T, atol=1e-6)
+    assert np.isnan(vd[7]).all() and np.isnan(nd[7]).all()
+
+
+def test_scale_depths_closed_form():
+    d = np.full((240, 320), 2.0, np.float64)
+    lib().or_scale_depths_d(d, 320, 240, CAM)
+    ys, xs = np.mgrid[0:240, 0:320]
+    xn, yn = (xs - CAM[2]) / CAM[0], (ys - CAM[3]) / CAM[1]
+    np.testing.assert_allclose(d, 2.0 * np.sqrt(1 + xn ** 2 + yn ** 2), rtol=1e-6)
+
+
+# ---------------------------------------------------------------- H8-H11 ICP
+
+def _plane_maps(W_=64, H_=48):
+    cam = np.array([52.5, 52.5, 31.5, 23.5, 0.5, 4.5], np.float32)
+    ys, xs = np.mgrid[0:H_, 0:W_]
+    z = 1.5 + 0.2 * (xs / W_) + 0.1 * np.sin(ys / 5.0)
+    v = np.stack([(xs - cam[2]) / cam[0] * z, (ys - cam[3]) / cam[1] * z, z], -1)
+    n = np.empty_like(v)
+    lib().or_normals_simple_d(np.ascontiguousarray(v), n, W_, H_, 0.3)
+    return cam, v, n
+
+
+def test_icp_sums_identity_closed_form():
+    """Current frame == reference frame on a plane z = 2 with power-of-two intrinsics:
+    every vertex projects exactly onto its own pixel, so b = n.(q - p) = 0, r^2 = 0 and
+    A = sum J^T J with J = [n x p, n], n = (0,0,1).  Pixels of the last row are
+    rejected (the bilinear lookup reads row y+1), pixels of the last column are kept
+    (the x+1 read wraps into the next row, quirk Q14) except the one whose x+1, y+1
+    corner lies past the end of the buffer."""
+    W_, H_ = 64, 48
+    cam = np.array([64.0, 64.0, 32.0, 24.0, 0.5, 4.5], np.float32)
+    ys, xs = np.mgrid[0:H_, 0:W_]
+    z = np.full((H_, W_), 2.0)
+    v = np.stack([(xs - cam[2]) / cam[0] * z, (ys - cam[3]) / cam[1] * z, z], -1)
+    n = np.zeros_like(v)
+    n[..., 2] = 1.0
+    I = np.eye(3).reshape(-1)
+    t0 = np.zeros(3)
+    sums = np.empty(29)
+    vv, nn = np.ascontiguousarray(v.reshape(-1, 3)), np.ascontiguousarray(n.reshape(-1, 3))
+    lib().or_icp_sums_d(cam, W_, H_, vv, 0, I, t0, vv, nn, I, t0, 0.1, sums)
+    ok = (ys < H_ - 1).reshape(-1)
+    ok[(H_ - 2) * W_ + W_ - 1] = False
+    J = np.concatenate([np.cross(nn[ok], vv[ok]), nn[ok]], 1)
+    A = J.T @ J
+    k = 0
+    for i in range(6):
+        for j in range(i, 7):
+            expect = A[i, j] if j < 6 else 0.0
+            assert np.isclose(sums[k], expect, rtol=1e-12, atol=1e-12), (i, j, sums[k], expect)
+            k += 1
+    assert sums[27] == 0.0
+    assert sums[28] == ok.sum() == W_ * (H_ - 1) - 1
+
+
+def test_icp_sums_residual_sign():
+    # shifting the estimate by +dz along the optical axis gives b = n.(q - p) = -n_z dz per pixel
+    W_, H_ = 64, 48
+    cam, v, n = _plane_maps(W_, H_)
+    I = np.eye(3).reshape(-1)
+    dz = 0.004
+    sums = np.empty(29)
+    vv = np.ascontiguousarray(v.reshape(-1, 3))
+    nn = np.ascontiguousarray(n.reshape(-1, 3))
+    lib().or_icp_sums_d(cam, W_, H_, vv, 0, I, np.array([0, 0, dz]), vv, nn, I, np.zeros(3), 0.1, sums)
+    # b_z term (index 26 = coresp[5]*coresp[6]) must be negative: -sum n_z^2 dz (approx.)
+    assert sums[26] < 0
+    assert sums[28] > 0.8 * (W_ - 1) * (H_ - 1)
+
+
+def test_cholesky_det_vs_numpy():
+    rng = np.random.default_rng(11)
+    M = rng.random((6, 6))
+    A = M @ M.T + 0.5 * np.eye(6)
+    b = rng.random(6)
+    x = np.empty(6)
+    det = np.empty(1)
+    assert lib().or_cholesky6(np.ascontiguousarray(A), b, x, det) == 1
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10)
+    np.testing.assert_allclose(det[0], np.linalg.det(A), rtol=1e-10)
+    np.testing.assert_allclose(lib().or_det6(np.ascontiguousarray(A)), np.linalg.det(A), rtol=1e-10)
+    S = A.copy()
+    S[5, :] = S[4, :]
+    assert abs(lib().or_det6(np.ascontiguousarray(S))) < 1e-12
+
+
+def test_axis_rotation_inverse_rodrigues():
+    for axis in range(3):
+        R = np.empty(9)
+        lib().or_axis_rot_d(0.3, axis, R)
+        R = R.reshape(3, 3)
+        e = np.zeros(3)
+        e[axis] = 1
+        np.testing.assert_allclose(R @ e, e, atol=1e-15)  # axis is invariant
+        assert np.isclose(np.linalg.det(R), 1)
+        np.testing.assert_allclose(np.trace(R), 1 + 2 * np.cos(0.3))
+        # right-handed: Rz rotates x towards y
+        if axis == 2:
+            np.testing.assert_allclose(R @ [1, 0, 0], [np.cos(0.3), np.sin(0.3), 0], atol=1e-15)
+        inv = np.empty(9)
+        lib().or_inverse3_d(np.ascontiguousarray(R.reshape(-1)), inv)
+        np.testing.assert_allclose(inv.reshape(3, 3), R.T, atol=1e-14)
+        assert np.isclose(lib().or_rodrigues_norm_d(np.ascontiguousarray(R.reshape(-1))), 0.3)
+    assert lib().or_rodrigues_norm_d(np.eye(3).reshape(-1).copy()) == 0.0
+
+
+def test_icp_recovers_known_motion():
+    """Noise-free frames: frame 1 tracked against frame 0's own maps lands on the
+    ground-truth pose (a sign or index slip in the terms, the solve or the Euler
+    composition diverges instead)."""
+    wl = dataclasses.replace(W.SMALL, noise=0, hole_frac=0.0)
+    for prec in (0, 1):
+        e = O.OracleEngine(wl, prec=prec, gate=0.0)
+        for f in range(2):
+            d, _, gt = wl.render(f)
+            ok, pose, _ = e.run(d)
+            assert ok == 1
+        R, t = pose[:9].reshape(3, 3), pose[9:]
+        assert np.linalg.norm(t - gt[:, 3]) < 3e-3
+        ang = np.degrees(np.arccos(np.clip((np.trace(R.T @ gt[:, :3]) - 1) / 2, -1, 1)))
+        assert ang < 0.1
+        # the motion itself is ~2 deg and ~4 cm, so this is not the identity
+        assert np.linalg.norm(gt[:, 3]) > 0.03
+
+
+def test_icp_count_threshold_rule():
+    # K/icp thresholds of the stale-global-size rule (quirk Q10) for 640x480, 3 levels
+    import ctypes as C
+    from natureuserinterfacestudio_amd import nuif
+    cfg = nuif.IcpConfig()
+    cfg.levels = 3
+    for i, n in enumerate([10, 5, 4]):
+        cfg.iterations[i] = n
+    thr = (C.c_uint32 * 4)()
+    assert nuif.L().nuif_icp_count_thresholds(640, 480, C.byref(cfg), thr) == 0
+    assert list(thr)[:3] == [64, 6, 1920]
+
+
+# ---------------------------------------------------------------- H12-H15 dense TSDF
+
+def _plane_volume(N=64, dim=2.0, trunc=0.05, D=1.2, frames=1):
+    """Integrates a fronto-parallel plane at depth D seen from the identity pose."""
+    W_, H_ = 160, 120
+    cam = np.array([131.25, 131.25, 79.5, 59.5, 0.5, 4.5], np.float32)
+    depth = np.full((H_, W_), D, np.float32)
+    lib().or_scale_depths_f(depth, W_, H_, cam)
+    vol = np.zeros((N ** 3, 2), np.int16)
+    vol[:, 0] = 32767
+    I = np.eye(3, dtype=np.float32).reshape(-1)
+    t0 = np.zeros(3, np.float32)
+    for _ in range(frames):
+        nupd = lib().or_integrate_s16_f(depth, None, vol.reshape(-1), None, N, dim, trunc, cam, W_, H_, I, t0, 128)
+    return cam, depth, vol, nupd, (W_, H_)
+
+
+def test_integrate_plane_closed_form():
+    N, dim, trunc, D = 64, 2.0, 0.05, 1.2
+    cam, depth, vol, nupd, (W_, H_) = _plane_volume(N, dim, trunc, D)
+    trunc = lib().or_effective_trunc(N, dim, trunc)  # 2.1 cells here (quirk Q16)
+    c = dim / N
+    y, x, z = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    X = (x - N // 2 + 0.5) * c
+    Y = (y - N // 2 + 0.5) * c
+    Z = (z + 0.5) * c
+    u = np.floor(X / Z * cam[0] + cam[2] + 0.5)
+    v = np.floor(Y / Z * cam[1] + cam[3] + 0.5)
+    inimg = (u >= 0) & (v >= 0) & (u < W_) & (v < H_)
+    xn, yn = (u - cam[2]) / cam[0], (v - cam[3]) / cam[1]
+    Dray = D * np.sqrt(1 + xn ** 2 + yn ** 2)
+    sdf = Dray - np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    upd = inimg & (sdf >= -trunc)
+    w = vol[:, 1].reshape(N, N, N)
+    # voxels whose projection lies within 1e-3 px of a pixel boundary may round either way
+    fu, fv = X / Z * cam[0] + cam[2] + 0.5, Y / Z * cam[1] + cam[3] + 0.5
+    edge = (np.abs(fu - np.round(fu)) < 1e-3) | (np.abs(fv - np.round(fv)) < 1e-3)
+    assert np.array_equal((w > 0)[~edge], upd[~edge])
+    assert nupd == (w > 0).sum()
+    tsdf = vol[:, 0].reshape(N, N, N).astype(np.float64) / 32766
+    expect = np.trunc(np.minimum(1.0, sdf / trunc) * 32766) / 32766  # one update: RTZ of the sample
+    m = upd & ~edge
+    np.testing.assert_allclose(tsdf[m], expect[m], atol=2.0 / 32766)
+
+
+def test_integrate_weight_cap_and_running_mean():
+    N = 32
+    cam, depth, vol, _, _ = _plane_volume(N=N, frames=130)
+    w = vol[:, 1]
+    assert w.max() == 128 and set(np.unique(w)) <= {0, 128}
+
+
+def test_raycast_plane_depth_and_normal():
+    N, dim, trunc, D = 128, 2.0, 0.05, 1.2
+    cam, depth, vol, _, (W_, H_) = _plane_volume(N, dim, trunc, D, frames=3)
+    I = np.eye(3, dtype=np.float32).reshape(-1)
+    vmap = np.empty(W_ * H_ * 3, np.float32)
+    nmap = np.empty_like(vmap)
+    lib().or_raycast_s16_f(vol.reshape(-1), None, N, dim, trunc, cam, W_, H_, I, np.zeros(3, np.float32), vmap, nmap,
+                           None)
+    vmap, nmap = vmap.reshape(H_, W_, 3), nmap.reshape(H_, W_, 3)
+    core = vmap[40:80, 60:100]
+    assert not np.isnan(core).any()
+    np.testing.assert_allclose(core[..., 2], D, atol=1.5e-3)  # plane depth, well inside a voxel (15.6 mm)
+    ncore = nmap[40:80, 60:100].reshape(-1, 3)
+    ok = ~np.isnan(ncore[:, 0])
+    assert ok.mean() > 0.9
+    # the field is the distance along each pixel's ray, so off-axis voxels tilt the
+    # gradient slightly; the normal still faces the camera
+    np.testing.assert_allclose(ncore[ok], np.tile([0, 0, -1.0], (ok.sum(), 1)), atol=3e-2)
+
+
+def test_effective_truncation_clamp():
+    # OCL/NuiKinfuOpenCLScene.cpp:41 (quirk Q16): 128^3 over 2 m -> 2.1 cells = 32.8 mm
+    assert np.isclose(lib().or_effective_trunc(128, 2.0, 0.03), 2.1 * 2.0 / 128, rtol=1e-6)
+    assert np.isclose(lib().or_effective_trunc(512, 3.0, 0.025), 0.025)
