@@ -160,11 +160,16 @@ typedef struct nuif_icp_result {
     double sums[NUIF_ICP_TERMS]; /* 29 sums of the last executed iteration */
 } nuif_icp_result;
 
+/* Summation order of the 29 ICP sums */
+#define NUIF_ICP_REDUCE_REFERENCE 0    /* the reference's float trees and host loop, bit for bit */
+#define NUIF_ICP_REDUCE_COMPENSATED 1  /* Kahan FP32 per lane, FP64 across blocks, FP64 LDL^T */
+
 typedef struct nuif_icp_config {
     int levels;                            /* pyramid levels (<= 4) */
     int iterations[NUIF_ICP_MAX_LEVELS];   /* per level, level 0 = finest */
     float dist_threshold;                  /* NuiTrackerConfig::dist_threshold */
     float gate_threshold;                  /* integration metric threshold (0 = always) */
+    int reduction;                         /* NUIF_ICP_REDUCE_* */
 } nuif_icp_config;
 
 size_t nuif_icp_state_bytes(void);
@@ -181,7 +186,7 @@ int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s
 int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* cam, const float* vmap_level,
                        int level, const float* ref_vmap, const float* ref_nmap,
                        const NuiCLRigidTransform* prev_pose, float dist_threshold, uint32_t count_threshold,
-                       int iter, nuif_stream_t s);
+                       int iter, int reduction, nuif_stream_t s);
 /* Finish the frame: on success pose <- estimate (with R_inv), and the integration gate
  * (VO/NuiKinfuTrackingEngine.cpp:113-127) is evaluated into the state.  If history is not
  * NULL, row (frame % capacity) of NUIF_POSE_ROW floats receives
@@ -222,6 +227,31 @@ int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, con
                             float* nmap, uint8_t* cmap, int storage, const int32_t* enable, nuif_stream_t s);
 /* Effective truncation distance (OCL/NuiKinfuOpenCLScene.cpp:41: >= 2.1 * max cell). */
 int nuif_tsdf_params(int nx, int ny, int nz, float dx, float dy, float dz, float trunc, TsdfParams* out);
+
+/* ---- prefix sum and compaction (OCL/NuiOpenCLPrefixSum.cpp:75-300, K/prefixSum.cl:163-232) ---- */
+
+size_t nuif_scan_scratch_bytes(size_t n);
+/* out[i] = sum of in[0..i-1]; total (device, optional) = sum of all; in == out allowed */
+int nuif_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total, void* scratch,
+                            nuif_stream_t s);
+/* the same over (flags[i] > 0) */
+int nuif_exclusive_scan_flags(const uint8_t* flags, uint32_t* out, size_t n, uint32_t* total, void* scratch,
+                              nuif_stream_t s);
+/* ids[scan[i]] = i for every i with flags[i] > 0 (order preserving; the reference's
+ * compactifyValidPrefixSum_kernel is racy, quirk Q2) */
+int nuif_compact_flags(const uint8_t* flags, const uint32_t* scan, size_t n, int32_t* ids, nuif_stream_t s);
+
+/* ---- dense marching cubes (K/marchingCube.cl:100-256, OCL/NuiKinfuOpenCLScene.cpp:617-786) ---- */
+
+/* Reference cap on output vertices (K/gpu_def.h:3) */
+#define NUIF_MC_MAX_VERTICES 5000000
+size_t nuif_mc_scratch_bytes(const TsdfParams* params);
+/* Triangles of the zero level set: vertices float3 (3 per triangle) and colours float4 RGBA
+ * (nearest corner of the colour volume, or (0,0,0,1) without one) written in a fixed tile
+ * order; *total (device uint32) receives the vertex count before capping at `capacity`. */
+int nuif_mc_dense(const void* volume, const uint8_t* color_volume, const TsdfParams* params, int storage,
+                  float* vertices, float* colors, uint32_t capacity, uint32_t* total, void* scratch,
+                  nuif_stream_t s);
 
 #ifdef __cplusplus
 }

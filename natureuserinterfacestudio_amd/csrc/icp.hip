@@ -19,6 +19,7 @@
 // integration pose, integration gate) lives in one device block so that
 // integrate/raycast launches can consume the ICP outcome without the host.
 
+#include "det_math.h"
 #include "nuif_internal.h"
 
 namespace nuif {
@@ -35,9 +36,15 @@ struct IcpState {
     float error, count;
     int32_t brk[NUIF_ICP_MAX_LEVELS];
     int32_t failed, iterations, gate, frame, ok, pad;
-    double good[27];
-    double sums[kTerms];
+    double good[27];       // compensated mode: accepted sums
+    double sums[kTerms];   // sums of the last executed iteration (both modes)
+    float goodf[27];       // reference mode: goodCores as the reference keeps them (float)
+    float pad2;
 };
+
+// Reduction orders (nuif_icp_config::reduction)
+constexpr int kRefOrder = 0;      // the reference's float trees, bit for bit
+constexpr int kCompensated = 1;   // Kahan FP32 per thread + FP64 across blocks
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     const float y = v - c;
@@ -141,7 +148,146 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// One pixel's 29 terms of K/icp.cl:243-289 (zeros when rejected).
+__device__ __forceinline__ void icp_terms(const f3 vc, const float* R, const float* t, const float* Ri,
+                                          const float* tp, const NuiCLCameraParams& cam,
+                                          const float* __restrict__ vref, const float* __restrict__ nref,
+                                          float distThr, float out[kTerms]) {
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) out[k] = 0.f;
+    if (nan3(vc)) return;
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    const f3 pv = add3(rot3(R, vc), mk3(t[0], t[1], t[2]));
+    const f3 pp = rot3(Ri, sub3(pv, mk3(tp[0], tp[1], tp[2])));
+    const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
+    if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) return;
+    const f3 rn = bilinear_holes(nref, px, py, W, H);
+    if (nan3(rn)) return;
+    const f3 rv = bilinear_holes(vref, px, py, W, H);
+    if (nan3(rv)) return;
+    const f3 diff = sub3(rv, pv);
+    const float dist = sqrtf(dot3(diff, diff));
+    if (!(dist < distThr)) return;
+    const f3 row0 = cross3(rn, pv);
+    const float c[7] = {row0.x, row0.y, row0.z, rn.x, rn.y, rn.z, dot3(rn, diff)};
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 7; ++j) out[s++] = c[i] * c[j];
+    out[27] = c[6] * c[6];
+    out[28] = 1.0f;
+}
+
+// K/icp.cl:162-190 reduce(): in-place 128-wide tree over buf[128][29]
+__device__ __forceinline__ void tree128(float* buf, int lid) {
+    for (int d = 64; d > 0; d >>= 1) {
+        __syncthreads();
+        if (lid < d) {
+            float* a = buf + lid * kTerms;
+            const float* b = buf + (lid + d) * kTerms;
+            for (int i = 0; i < kTerms; ++i) a[i] += b[i];
+        }
+    }
+    __syncthreads();
+}
+
+// Reference-order partials: one 128-item work-group per 128 pixels, the
+// reference's LDS tree, 29 floats per group (K/icp.cl:222-292).  A partial
+// last group counts its missing items as zeros.
+__global__ void __launch_bounds__(128)
+    icp_ref_partials_kernel(const IcpState* __restrict__ st, const float* __restrict__ vcur, int npx,
+                            NuiCLCameraParams cam, const float* __restrict__ vref, const float* __restrict__ nref,
+                            const NuiCLRigidTransform* __restrict__ prev, float distThr, int level,
+                            float* __restrict__ partials) {
+    if (st->failed || st->brk[level]) return;
+    __shared__ float buf[128 * kTerms];
+    float R[9], t[3], Ri[9], tp[3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        R[k] = st->R[k];
+        Ri[k] = prev->R_inv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        t[k] = st->t[k];
+        tp[k] = prev->t[k];
+    }
+    const int lid = threadIdx.x;
+    const int p = blockIdx.x * 128 + lid;
+    float v[kTerms];
+    const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
+    icp_terms(vc, R, t, Ri, tp, cam, vref, nref, distThr, v);
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) buf[lid * kTerms + k] = v[k];
+    tree128(buf, lid);
+    if (lid < kTerms) partials[(size_t)blockIdx.x * kTerms + lid] = buf[lid];
+}
+
 // ---- small FP64 linear algebra for the solve ----------------------------------
+
+// Cholesky A = L L^T in place (lower), as Eigen's LLT; returns false if not SPD.
+__device__ bool llt6(double A[36], double* det) {
+    double d = 1.0;
+    for (int j = 0; j < 6; ++j) {
+        double s = A[j * 6 + j];
+        for (int k = 0; k < j; ++k) s -= A[j * 6 + k] * A[j * 6 + k];
+        if (!(s > 0)) {
+            *det = 0.0;
+            return false;
+        }
+        const double l = sqrt(s);
+        A[j * 6 + j] = l;
+        d *= l;
+        for (int i = j + 1; i < 6; ++i) {
+            double u = A[i * 6 + j];
+            for (int k = 0; k < j; ++k) u -= A[i * 6 + k] * A[j * 6 + k];
+            A[i * 6 + j] = u / l;
+        }
+    }
+    *det = d * d;
+    return true;
+}
+
+__device__ void llt6_solve(const double L[36], const double b[6], double x[6]) {
+    double y[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int k = 0; k < i; ++k) s -= L[i * 6 + k] * y[k];
+        y[i] = s / L[i * 6 + i];
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int k = i + 1; k < 6; ++k) s -= L[k * 6 + i] * x[k];
+        x[i] = s / L[i * 6 + i];
+    }
+}
+
+// Float 3x3 helpers mirroring Eigen's float evaluation of the increment
+__device__ void axis_rotf(float angle, int k, float Rm[9]) {
+    float s, c;
+    nuif_det::sincosf_det(angle, &s, &c);
+    float ax[3] = {0.f, 0.f, 0.f};
+    ax[k] = 1.f;
+    const float sa[3] = {s * ax[0], s * ax[1], s * ax[2]};
+    const float ca[3] = {(1.f - c) * ax[0], (1.f - c) * ax[1], (1.f - c) * ax[2]};
+    Rm[1] = ca[0] * ax[1] - sa[2];
+    Rm[3] = ca[0] * ax[1] + sa[2];
+    Rm[2] = ca[0] * ax[2] + sa[1];
+    Rm[6] = ca[0] * ax[2] - sa[1];
+    Rm[5] = ca[1] * ax[2] - sa[0];
+    Rm[7] = ca[1] * ax[2] + sa[0];
+    Rm[0] = ca[0] * ax[0] + c;
+    Rm[4] = ca[1] * ax[1] + c;
+    Rm[8] = ca[2] * ax[2] + c;
+}
+
+__device__ void matmul3f(const float a[9], const float b[9], float c[9]) {
+    float r[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+    for (int k = 0; k < 9; ++k) c[k] = r[k];
+}
 
 // A = L D L^T (unit lower L), in place; returns false if a pivot is not positive.
 __device__ bool ldlt6(double A[36], double* det) {
@@ -205,32 +351,69 @@ __device__ void inverse3f(const float m[9], float out[9]) {
     out[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
 }
 
+// The host loop of OCL/NuiKinfuOpenCLDepthTracker.cpp:382-497 for one iteration,
+// on one workgroup.  Reference order: compute_sums' 128-wide trees over the group
+// partials, then the host's sequential float sum over the second-level blocks,
+// float goodCores, Eigen-like LLT in double and the float Euler increment.
+// Compensated order: FP64 sums, LDL^T in FP64 and the increment composed in FP64.
+template <int MODE>
 __global__ void __launch_bounds__(kBlock)
-    icp_solve_kernel(IcpState* __restrict__ st, const double* __restrict__ partials, int nblocks, int level,
+    icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials_v, int ngroups, int level,
                      int iter, uint32_t count_threshold) {
     if (st->failed || st->brk[level]) return;
+    __shared__ float buf[128 * kTerms];
     __shared__ double part[kTerms * kSeg];
     __shared__ double sums[kTerms];
+    __shared__ float sumsf[kTerms];
     const int tid = threadIdx.x;
-    if (tid < kTerms * kSeg) {
-        const int k = tid / kSeg, j = tid % kSeg;
-        double s = 0.0;
-        for (int b = j; b < nblocks; b += kSeg) s += partials[(size_t)b * kTerms + k];
-        part[k * kSeg + j] = s;
+    if (MODE == kRefOrder) {
+        const float* partials = static_cast<const float*>(partials_v);
+        const int nb2 = (ngroups + 127) / 128;  // compute_sums work-groups
+        if (tid < kTerms) sumsf[tid] = 0.f;
+        for (int b = 0; b < nb2; ++b) {
+            if (tid < 128) {
+                const int gi = b * 128 + tid;
+                for (int k = 0; k < kTerms; ++k)
+                    buf[tid * kTerms + k] = gi < ngroups ? partials[(size_t)gi * kTerms + k] : 0.f;
+            }
+            // tree128 with the whole 256-thread block taking part in the barriers
+            for (int d = 64; d > 0; d >>= 1) {
+                __syncthreads();
+                if (tid < d) {
+                    float* a = buf + tid * kTerms;
+                    const float* c = buf + (tid + d) * kTerms;
+                    for (int i = 0; i < kTerms; ++i) a[i] += c[i];
+                }
+            }
+            __syncthreads();
+            if (tid < kTerms) sumsf[tid] += buf[tid];  // host: sequential float sum over blocks
+            __syncthreads();
+        }
+        if (tid < kTerms) sums[tid] = (double)sumsf[tid];
+        __syncthreads();
+    } else {
+        const double* partials = static_cast<const double*>(partials_v);
+        if (tid < kTerms * kSeg) {
+            const int k = tid / kSeg, j = tid % kSeg;
+            double s = 0.0;
+            for (int b = j; b < ngroups; b += kSeg) s += partials[(size_t)b * kTerms + k];
+            part[k * kSeg + j] = s;
+        }
+        __syncthreads();
+        if (tid < kTerms) {
+            double s = 0.0;
+            for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
+            sums[tid] = s;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (tid < kTerms) {
-        double s = 0.0;
-        for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
-        sums[tid] = s;
-    }
-    __syncthreads();
     if (tid != 0) return;
 
-    // OCL/NuiKinfuOpenCLDepthTracker.cpp:395-458: error/count, accept or reject
+    // :395-458: error/count, accept or reject
     for (int k = 0; k < kTerms; ++k) st->sums[k] = sums[k];
     const float cnt = (float)sums[28];
-    float err = cnt > 0.f ? sqrtf((float)sums[27]) / cnt : 3.402823466e38f;
+    const float errsum = MODE == kRefOrder ? sumsf[27] : (float)sums[27];
+    const float err = cnt > 0.f ? sqrtf(errsum) / cnt : 3.402823466e38f;
     st->error = err;
     st->count = cnt;
     st->iterations += 1;
@@ -256,14 +439,16 @@ __global__ void __launch_bounds__(kBlock)
         for (int k = 0; k < 9; ++k) st->Rg[k] = st->R[k];
         for (int k = 0; k < 3; ++k) st->tg[k] = st->t[k];
         st->err_old = err;
-        for (int k = 0; k < 27; ++k) st->good[k] = sums[k];
+        for (int k = 0; k < 27; ++k) {
+            st->good[k] = sums[k];
+            st->goodf[k] = MODE == kRefOrder ? sumsf[k] : (float)sums[k];
+        }
     }
     // damped normal equations, normalised by the current count (quirk Q12)
     double A[36], b[6];
-    const double dc = (double)cnt;
     for (int i = 0, s = 0; i < 6; ++i) {
         for (int j = i; j < 7; ++j, ++s) {
-            const double v = st->good[s] / dc;
+            const double v = MODE == kRefOrder ? (double)(st->goodf[s] / cnt) : st->good[s] / (double)cnt;
             if (j == 6)
                 b[i] = v;
             else
@@ -271,41 +456,59 @@ __global__ void __launch_bounds__(kBlock)
         }
         A[i * 6 + i] *= (double)(1.0f + lambda);
     }
-    double det;
-    const bool spd = ldlt6(A, &det);
+    double det, x[6];
+    const bool spd = MODE == kRefOrder ? llt6(A, &det) : ldlt6(A, &det);
     if (!spd || fabs(det) < 1e-15 || isnan(det)) {
         st->failed = 1;
         return;
     }
-    double x[6];
-    ldlt6_solve(A, b, x);
+    if (MODE == kRefOrder)
+        llt6_solve(A, b, x);
+    else
+        ldlt6_solve(A, b, x);
     const float al = (float)x[0], be = (float)x[1], ga = (float)x[2];
     if (al * al + be * be + ga * ga < 1e-6f) {  // converged: break before applying (:482-484)
         st->brk[level] = 1;
         return;
     }
     // Rinc = Rz(gamma) Ry(beta) Rx(alpha); t <- Rinc^-1 t + t_inc; R <- Rinc^-1 R   (:486-497)
-    const double sa = sin((double)al), ca = cos((double)al);
-    const double sb = sin((double)be), cb = cos((double)be);
-    const double sg = sin((double)ga), cg = cos((double)ga);
-    const double Rz[9] = {cg, -sg, 0, sg, cg, 0, 0, 0, 1};
-    const double Ry[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
-    const double Rx[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca};
-    double Rzy[9], Rinc[9];
-    matmul3d(Rz, Ry, Rzy);
-    matmul3d(Rzy, Rx, Rinc);
-    double Ri[9];  // inverse of a rotation: its transpose
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rinc[3 * j + i];
-    double Rc[9], tc[3];
-    for (int k = 0; k < 9; ++k) Rc[k] = st->R[k];
-    for (int k = 0; k < 3; ++k) tc[k] = st->t[k];
-    double Rn[9];
-    matmul3d(Ri, Rc, Rn);
-    const double tinc[3] = {(double)(float)x[3], (double)(float)x[4], (double)(float)x[5]};
-    for (int i = 0; i < 3; ++i)
-        st->t[i] = (float)(Ri[3 * i] * tc[0] + Ri[3 * i + 1] * tc[1] + Ri[3 * i + 2] * tc[2] + tinc[i]);
-    for (int k = 0; k < 9; ++k) st->R[k] = (float)Rn[k];
+    const float tinc[3] = {(float)x[3], (float)x[4], (float)x[5]};
+    if (MODE == kRefOrder) {
+        float Rz[9], Ry[9], Rx[9], Rzy[9], Rinc[9], Ri[9];
+        axis_rotf(ga, 2, Rz);
+        axis_rotf(be, 1, Ry);
+        axis_rotf(al, 0, Rx);
+        matmul3f(Rz, Ry, Rzy);
+        matmul3f(Rzy, Rx, Rinc);
+        inverse3f(Rinc, Ri);
+        float tn[3];
+        for (int i = 0; i < 3; ++i) tn[i] = Ri[3 * i] * st->t[0] + Ri[3 * i + 1] * st->t[1] + Ri[3 * i + 2] * st->t[2];
+        for (int i = 0; i < 3; ++i) st->t[i] = tn[i] + tinc[i];
+        float Rn[9];
+        matmul3f(Ri, st->R, Rn);
+        for (int k = 0; k < 9; ++k) st->R[k] = Rn[k];
+    } else {
+        const double sa = sin((double)al), ca = cos((double)al);
+        const double sb = sin((double)be), cb = cos((double)be);
+        const double sg = sin((double)ga), cg = cos((double)ga);
+        const double Rz[9] = {cg, -sg, 0, sg, cg, 0, 0, 0, 1};
+        const double Ry[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
+        const double Rx[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca};
+        double Rzy[9], Rinc[9];
+        matmul3d(Rz, Ry, Rzy);
+        matmul3d(Rzy, Rx, Rinc);
+        double Ri[9];  // inverse of a rotation: its transpose
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rinc[3 * j + i];
+        double Rc[9], tc[3];
+        for (int k = 0; k < 9; ++k) Rc[k] = st->R[k];
+        for (int k = 0; k < 3; ++k) tc[k] = st->t[k];
+        double Rn[9];
+        matmul3d(Ri, Rc, Rn);
+        for (int i = 0; i < 3; ++i)
+            st->t[i] = (float)(Ri[3 * i] * tc[0] + Ri[3 * i + 1] * tc[1] + Ri[3 * i + 2] * tc[2] + (double)tinc[i]);
+        for (int k = 0; k < 9; ++k) st->R[k] = (float)Rn[k];
+    }
 }
 
 __global__ void icp_begin_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
@@ -395,7 +598,11 @@ extern "C" {
 
 size_t nuif_icp_state_bytes(void) { return (sizeof(IcpState) + 255) / 256 * 256; }
 
-size_t nuif_icp_partials_bytes(int, int) { return (size_t)kMaxBlocks * kTerms * sizeof(double); }
+size_t nuif_icp_partials_bytes(int width, int height) {
+    const size_t compensated = (size_t)kMaxBlocks * kTerms * sizeof(double);
+    const size_t reference = ((size_t)width * height + 127) / 128 * kTerms * sizeof(float);
+    return std::max(compensated, reference);
+}
 
 const int32_t* nuif_icp_gate_flag(const void* state) {
     return state ? &reinterpret_cast<const IcpState*>(state)->gate : nullptr;
@@ -441,21 +648,32 @@ int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s
 
 int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* cam, const float* vmap_level,
                        int level, const float* ref_vmap, const float* ref_nmap, const NuiCLRigidTransform* prev_pose,
-                       float dist_threshold, uint32_t count_threshold, int iter, nuif_stream_t s) {
+                       float dist_threshold, uint32_t count_threshold, int iter, int reduction, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(state && partials && cam && vmap_level && ref_vmap && ref_nmap && prev_pose,
                    "nuif_icp_iteration: null argument");
     NUIF_CHECK_ARG(level >= 0 && level < NUIF_ICP_MAX_LEVELS && iter >= 0, "nuif_icp_iteration: bad level/iter");
+    NUIF_CHECK_ARG(reduction == NUIF_ICP_REDUCE_REFERENCE || reduction == NUIF_ICP_REDUCE_COMPENSATED,
+                   "nuif_icp_iteration: unknown reduction %d", reduction);
     const int Wl = (int)cam->depthImageWidth >> level, Hl = (int)cam->depthImageHeight >> level;
     NUIF_CHECK_ARG(Wl > 0 && Hl > 0, "nuif_icp_iteration: level %d too small", level);
     const int npx = Wl * Hl;
-    const int nblocks = std::min<int>(div_up(npx, kBlock), kMaxBlocks);
     hipStream_t st = stream_of(s);
-    icp_partials_kernel<<<nblocks, kBlock, 0, st>>>((const IcpState*)state, vmap_level, npx, *cam, ref_vmap,
-                                                     ref_nmap, prev_pose, dist_threshold, level, (double*)partials);
-    NUIF_LAUNCHED("icp_partials_kernel");
-    icp_solve_kernel<<<1, kBlock, 0, st>>>((IcpState*)state, (const double*)partials, nblocks, level, iter,
-                                           count_threshold);
+    if (reduction == NUIF_ICP_REDUCE_REFERENCE) {
+        const int groups = (int)div_up(npx, 128);
+        icp_ref_partials_kernel<<<groups, 128, 0, st>>>((const IcpState*)state, vmap_level, npx, *cam, ref_vmap,
+                                                        ref_nmap, prev_pose, dist_threshold, level, partials);
+        NUIF_LAUNCHED("icp_ref_partials_kernel");
+        icp_solve_kernel<kRefOrder><<<1, kBlock, 0, st>>>((IcpState*)state, partials, groups, level, iter,
+                                                           count_threshold);
+    } else {
+        const int nblocks = std::min<int>(div_up(npx, kBlock), kMaxBlocks);
+        icp_partials_kernel<<<nblocks, kBlock, 0, st>>>((const IcpState*)state, vmap_level, npx, *cam, ref_vmap,
+                                                         ref_nmap, prev_pose, dist_threshold, level, (double*)partials);
+        NUIF_LAUNCHED("icp_partials_kernel");
+        icp_solve_kernel<kCompensated><<<1, kBlock, 0, st>>>((IcpState*)state, partials, nblocks, level, iter,
+                                                              count_threshold);
+    }
     NUIF_LAUNCHED("icp_solve_kernel");
     return NUIF_OK;
 }
@@ -503,7 +721,7 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
         NUIF_CHECK_ARG(vmap_pyr[level], "nuif_icp_estimate: vmap level %d is NULL", level);
         for (int it = 0; it < cfg->iterations[level]; ++it) {
             rc = nuif_icp_iteration(state, partials, cam, vmap_pyr[level], level, ref_vmap, ref_nmap, pose, dist,
-                                    thr[level], it, s);
+                                    thr[level], it, cfg->reduction, s);
             if (rc) return rc;
         }
     }

@@ -7,6 +7,7 @@
 // are bandwidth-trivial (a few MB per frame); the fused preprocessing kernel
 // exists to cut launches and to read the raw depth once.
 
+#include "det_math.h"
 #include "nuif_internal.h"
 
 namespace nuif {
@@ -47,7 +48,7 @@ __device__ __forceinline__ float bilateral_px(const float* tile, int tw, int x, 
             const float diff = fabsf(c - n);
             if (n > 0.0f && diff < thr) {
                 const float d2 = diff * diff;
-                const float w = g.g[cx + r - x] * g.g[cy + r - y] * expf(-(d2 * sigma_depth));
+                const float w = g.g[cx + r - x] * g.g[cy + r - y] * nuif_det::expf_det(-(d2 * sigma_depth));
                 sd += n * w;
                 sw += w;
             }

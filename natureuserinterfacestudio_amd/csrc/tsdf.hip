@@ -103,6 +103,7 @@ __global__ void reset_kernel(typename S::word* __restrict__ vol, uint32_t* __res
 // ---- integration ---------------------------------------------------------------
 
 constexpr int kZChunk = 64;
+constexpr int kRayBatch = 8;  // raycast samples whose loads are in flight together
 constexpr int kLdsPitch = kZChunk + 1;
 
 struct ColumnWalk {  // the reference's per-column incremental state
@@ -153,6 +154,34 @@ __device__ __forceinline__ void column_advance(ColumnWalk& c, float cz) {
     c.vy += c.incy;
 }
 
+// True when none of the next n steps of the walk can land inside the image, so
+// column_step would return NaN for all of them.  The camera-space point of the
+// column is affine in z, hence so are the projection's numerators and its depth;
+// with the depth of one sign over the segment the projected coordinates are
+// monotonic and the end points bound them.  The margins cover the rounding of
+// the serial accumulation (relative 1e-4 of the magnitudes, far above 64 ulps).
+__device__ __forceinline__ bool column_chunk_outside(const ColumnWalk& c, float R8, float cz, int n,
+                                                     const NuiCLCameraParams& cam) {
+    const float steps = (float)(n - 1);
+    const float z0 = c.vz + R8 * c.zs;
+    const float z1 = c.vz + R8 * (c.zs + steps * cz);
+    const float ez = 1e-4f * (fabsf(c.vz) + fabsf(R8 * c.zs) + fabsf(R8 * steps * cz)) + 1e-6f;
+    if (z0 < -ez && z1 < -ez) return true;  // behind the camera over the whole chunk
+    if (!(z0 > ez && z1 > ez)) return false;
+    const float W = (float)cam.depthImageWidth, H = (float)cam.depthImageHeight;
+    const float x0 = c.vx, x1 = c.vx + steps * c.incx;
+    const float y0 = c.vy, y1 = c.vy + steps * c.incy;
+    const float u0 = x0 / z0, u1 = x1 / z1, v0 = y0 / z0, v1 = y1 / z1;
+    const float mu = 2.f + 1e-4f * (fabsf(x0) + fabsf(steps * c.incx)) / fminf(z0, z1);
+    const float mv = 2.f + 1e-4f * (fabsf(y0) + fabsf(steps * c.incy)) / fminf(z0, z1);
+    // pixel u is hit iff round(u' + cx) in [0, W-1], i.e. u' + cx in [-0.5, W - 0.5)
+    const float ulo = fminf(u0, u1) + cam.cx, uhi = fmaxf(u0, u1) + cam.cx;
+    const float vlo = fminf(v0, v1) + cam.cy, vhi = fmaxf(v0, v1) + cam.cy;
+    if (uhi < -0.5f - mu || ulo > W - 0.5f + mu) return true;
+    if (vhi < -0.5f - mv || vlo > H - 0.5f + mv) return true;
+    return false;
+}
+
 // Fast path: Nx, Ny multiples of 8, Nz multiple of 64.
 template <class S, bool COLOR>
 __global__ void __launch_bounds__(64)
@@ -180,16 +209,30 @@ __global__ void __launch_bounds__(64)
     const int sub = lane % LPC;  // position inside a column chunk (phase 2)
     unsigned nupd = 0;
     for (int zc = 0; zc < Nz; zc += kZChunk) {
-        // phase 1: serial walk of this lane's column
+        // phase 1: serial walk of this lane's column; chunks that cannot project
+        // into the image only advance the walk
+        const bool outside = column_chunk_outside(c, R8, cz, kZChunk, cam);
+        if (__all(outside)) {
+            for (int k = 0; k < kZChunk; ++k) column_advance(c, cz);
+            continue;
+        }
         bool any = false;
-        for (int k = 0; k < kZChunk; ++k) {
-            int pix;
-            const float v = column_step(c, depth, cam, R8, trunc, tr_inv, &pix);
-            ts[lane * kLdsPitch + k] = v;
-            if (COLOR) cp[lane * kLdsPitch + k] = pix;
-            any |= !isnan(v);
-            nupd += !isnan(v);
-            column_advance(c, cz);
+        if (outside) {
+            for (int k = 0; k < kZChunk; ++k) {
+                ts[lane * kLdsPitch + k] = kNaN;
+                if (COLOR) cp[lane * kLdsPitch + k] = -1;
+                column_advance(c, cz);
+            }
+        } else {
+            for (int k = 0; k < kZChunk; ++k) {
+                int pix;
+                const float v = column_step(c, depth, cam, R8, trunc, tr_inv, &pix);
+                ts[lane * kLdsPitch + k] = v;
+                if (COLOR) cp[lane * kLdsPitch + k] = pix;
+                any |= !isnan(v);
+                nupd += !isnan(v);
+                column_advance(c, cz);
+            }
         }
         if (!__any(any)) continue;  // the whole tile misses the frustum / band in this chunk
         __syncthreads();
@@ -380,15 +423,46 @@ __global__ void __launch_bounds__(64)
         float tsdf = vv.sample(idx);
         const float max_t = 3.f * (p.dimension[0] + p.dimension[1] + p.dimension[2]);
         const float step = p.tranc_dist * 0.8f;
-        for (; tcur < max_t; tcur += step) {
-            const float tprev = tsdf;
-            get_voxel(add3(rs, mul3(dir, tcur + step)), p, gv);
-            if (!(gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny && gv[2] < vv.Nz)) break;
-            idx = ((long long)gv[1] * vv.Nx + gv[0]) * vv.Nz + gv[2];
-            tsdf = vv.sample(idx);
-            if (isnan(tsdf) || isnan(tprev)) continue;
-            if (tprev < 0.f && tsdf > 0.f) break;
-            if (tprev >= 0.f && tsdf <= 0.f) {
+        // The march of K/raycast.cl:193-276, staged kRayBatch samples at a time: the
+        // sample positions of the next steps do not depend on the samples, so their
+        // loads are issued together and then consumed in order with the reference's
+        // exit tests (loads past the exit are in bounds and simply unused).
+        bool marching = true, crossing = false;
+        while (marching) {
+            float tb[kRayBatch], sv[kRayBatch];
+            long long ib[kRayBatch];
+            bool inb[kRayBatch];
+            float tn = tcur;
+#pragma unroll
+            for (int k = 0; k < kRayBatch; ++k) {
+                tb[k] = tn;
+                get_voxel(add3(rs, mul3(dir, tn + step)), p, gv);
+                inb[k] = gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny && gv[2] < vv.Nz;
+                ib[k] = inb[k] ? ((long long)gv[1] * vv.Nx + gv[0]) * vv.Nz + gv[2] : 0;
+                tn += step;  // the reference's `time_curr += time_step`
+            }
+#pragma unroll
+            for (int k = 0; k < kRayBatch; ++k) sv[k] = inb[k] ? vv.sample(ib[k]) : kNaN;
+            for (int k = 0; k < kRayBatch && marching; ++k) {
+                tcur = tb[k];
+                if (!(tcur < max_t) || !inb[k]) {  // loop guard / checkInds: leave the march
+                    marching = false;
+                    break;
+                }
+                const float tprev = tsdf;
+                idx = ib[k];
+                tsdf = sv[k];
+                if (isnan(tsdf) || isnan(tprev)) continue;
+                if (tprev < 0.f && tsdf > 0.f) marching = false;  // leaving a surface from behind
+                else if (tprev >= 0.f && tsdf <= 0.f) {           // zero crossing at tcur
+                    marching = false;
+                    crossing = true;
+                }
+            }
+            if (marching) tcur = tn;
+        }
+        if (crossing) {
+            do {
                 const float Ftdt = trilinear<S>(add3(rs, mul3(dir, tcur + step)), vv, p);
                 if (isnan(Ftdt)) break;
                 const float Ft = trilinear<S>(add3(rs, mul3(dir, tcur)), vv, p);
@@ -412,8 +486,7 @@ __global__ void __launch_bounds__(64)
                     n.z = fz1 - fz2;
                     nout = normalize3(n);
                 }
-                break;
-            }
+            } while (false);
         }
     }
     st3(vmap, id, vout);

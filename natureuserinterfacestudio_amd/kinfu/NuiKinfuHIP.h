@@ -133,6 +133,9 @@ public:
     // frame without ICP (first frame): records the pose and advances the frame count
     void RecordPose(NuiKinfuHIPCameraState* cam);
     void SetIntegrationThreshold(float v) { m_icfg.gate_threshold = v; }
+    // NUIF_ICP_REDUCE_REFERENCE (default: the reference's float summation order) or
+    // NUIF_ICP_REDUCE_COMPENSATED (Kahan FP32 + FP64)
+    void SetReduction(int mode) { m_icfg.reduction = mode; }
     void SetHistory(float* dHistory, int capacity) { m_history = dHistory; m_historyCap = capacity; }
     nuif_icp_result ReadState() const;
 

@@ -54,6 +54,7 @@ typedef struct nuifk_desc {
     int color;                      // colour volume on/off
     int storage;                    // NUIF_TSDF_*
     int use_graph;
+    int icp_reduction;              // NUIF_ICP_REDUCE_*
 } nuifk_desc;
 
 const char* nuifk_last_error(void) { return t_err.c_str(); }
@@ -80,6 +81,7 @@ int nuifk_create(const nuifk_desc* d, void** out) {
         s->engine->setIntegrationMetricThreshold(d->integration_threshold);
         s->engine->initialize(tc, false, (UINT)d->width, (UINT)d->height);
         s->engine->setUseGraph(d->use_graph != 0);
+        s->engine->tracker()->SetReduction(d->icp_reduction);
         s->scene = std::make_unique<NuiKinfuHIPScene>(vc, d->storage, s->engine->stream());
         s->cam.m_intrinsics = NuiCameraIntrinsics(d->fx, d->fy, d->cx, d->cy);
         s->cam.m_sensorDepthMin = d->depth_min;

@@ -44,7 +44,10 @@ class IcpResult(C.Structure):
 
 class IcpConfig(C.Structure):
     _fields_ = [("levels", c_int), ("iterations", c_int * ICP_MAX_LEVELS), ("dist_threshold", c_float),
-                ("gate_threshold", c_float)]
+                ("gate_threshold", c_float), ("reduction", c_int)]
+
+
+ICP_REDUCE_REFERENCE, ICP_REDUCE_COMPENSATED = 0, 1
 
 
 class KinfuDesc(C.Structure):
@@ -54,7 +57,7 @@ class KinfuDesc(C.Structure):
                 ("depth_threshold", c_float), ("dist_threshold", c_float), ("levels", c_int),
                 ("iterations", c_int * 4), ("integration_threshold", c_float), ("resolution", c_int * 3),
                 ("dimensions", c_float * 3), ("tranc_dist", c_float), ("color", c_int), ("storage", c_int),
-                ("use_graph", c_int)]
+                ("use_graph", c_int), ("icp_reduction", c_int)]
 
 
 _SIGS = {
@@ -105,7 +108,7 @@ _SIGS = {
     "nuif_icp_state_read": (c_int, [c_void_p, C.POINTER(IcpResult), c_void_p]),
     "nuif_icp_begin": (c_int, [c_void_p, c_void_p, c_void_p]),
     "nuif_icp_iteration": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_void_p,
-                                   c_void_p, c_void_p, c_float, C.c_uint32, c_int, c_void_p]),
+                                   c_void_p, c_void_p, c_float, C.c_uint32, c_int, c_int, c_void_p]),
     "nuif_icp_end": (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_int, c_void_p]),
     "nuif_icp_estimate": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                   c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
@@ -122,6 +125,13 @@ _SIGS = {
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "nuif_tsdf_raycast_gated": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),
                                         c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "nuif_scan_scratch_bytes": (c_size_t, [c_size_t]),
+    "nuif_exclusive_scan_u32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
+    "nuif_exclusive_scan_flags": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
+    "nuif_compact_flags": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
+    "nuif_mc_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
+    "nuif_mc_dense": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p, C.c_uint32,
+                              c_void_p, c_void_p, c_void_p]),
 }
 
 _KSIGS = {

@@ -85,7 +85,8 @@ class Workload:
     def camera_params(self):
         return nuif.camera_params(self.width, self.height, self.fx, self.fy, self.cx, self.cy, 0.5, 4.5)
 
-    def kinfu_desc(self, storage=nuif.TSDF_S16, gate=0.0, color=False, use_graph=False, iterations=None):
+    def kinfu_desc(self, storage=nuif.TSDF_S16, gate=0.0, color=False, use_graph=False, iterations=None,
+                   reduction=None):
         d = nuif.KinfuDesc()
         d.width, d.height = self.width, self.height
         d.fx, d.fy, d.cx, d.cy = self.fx, self.fy, self.cx, self.cy
@@ -107,6 +108,11 @@ class Workload:
         d.color = 1 if color else 0
         d.storage = storage
         d.use_graph = 1 if use_graph else 0
+        # the reference-precision volume pairs with the reference's summation order;
+        # the FP16 / FP32 variants use the compensated sums
+        if reduction is None:
+            reduction = nuif.ICP_REDUCE_REFERENCE if storage == nuif.TSDF_S16 else nuif.ICP_REDUCE_COMPENSATED
+        d.icp_reduction = reduction
         return d
 
 

@@ -59,6 +59,12 @@ def lib():
             "or_integrate_s16_f": (cll, [f32p, vp, i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, ci]),
             "or_raycast_s16_f": (None, [i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, f32p, f32p, vp]),
             "or_effective_trunc": (cf, [ci, cf, cf]),
+            "or_mc_count_s16": (cll, [i16p, ci]),
+            "or_mc_emit_s16": (cll, [i16p, vp, ci, cf, f32p, vp, cll]),
+            "or_mc_mesh_dedup_s16": (cll, [i16p, ci, cf, f32p, cll, i32p, cll, C.POINTER(cll)]),
+            "or_mc_table": (None, [np.ctypeslib.ndpointer(np.int8), np.ctypeslib.ndpointer(np.uint8)]),
+            "or_expf_det": (cf, [cf]),
+            "or_sincosf_det": (None, [cf, C.POINTER(cf), C.POINTER(cf)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(_lib, name)

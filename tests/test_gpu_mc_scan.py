@@ -1,0 +1,100 @@
+"""GPU parity of the device-wide scan / compaction and the dense marching cubes.
+
+Scan values, counts and compacted ids are integers: exact.  Marching cubes:
+the triangle count is exact; the vertices are the oracle's vertices bit for
+bit as a multiset (the order is the tile order here and nondeterministic in
+the reference, quirk Q25)."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from natureuserinterfacestudio_amd import nuif
+from natureuserinterfacestudio_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def sig(gpu):
+    return gpu.L()
+
+
+@pytest.mark.parametrize("n", [1, 1023, 1024, 1025, 300001, 1 << 20, 1572864])
+def test_exclusive_scan_u32_and_flags(gpu, n):
+    announce(f"scan n={n}")
+    L = sig(gpu)
+    rng = np.random.default_rng(n)
+    vals = rng.integers(0, 1000, n, dtype=np.uint32)
+    flags = (rng.random(n) < 0.3).astype(np.uint8) * rng.integers(1, 4, n).astype(np.uint8)
+    dv, df = nuif.DeviceArray.from_host(vals), nuif.DeviceArray.from_host(flags)
+    out = nuif.DeviceArray((n,), np.uint32)
+    tot = nuif.DeviceArray((1,), np.uint32)
+    scratch = nuif.DeviceArray((L.nuif_scan_scratch_bytes(n),), np.uint8)
+    nuif.check(L.nuif_exclusive_scan_u32(dv.ptr, out.ptr, n, tot.ptr, scratch.ptr, None))
+    ex = np.concatenate([[0], np.cumsum(vals, dtype=np.uint64)[:-1]]).astype(np.uint32)
+    np.testing.assert_array_equal(out.download(), ex)
+    assert tot.download()[0] == np.uint32(vals.sum(dtype=np.uint64))
+    nuif.check(L.nuif_exclusive_scan_flags(df.ptr, out.ptr, n, tot.ptr, scratch.ptr, None))
+    f = (flags > 0).astype(np.uint64)
+    ex = np.concatenate([[0], np.cumsum(f)[:-1]]).astype(np.uint32)
+    np.testing.assert_array_equal(out.download(), ex)
+    cnt = int(f.sum())
+    assert tot.download()[0] == cnt
+    if cnt:
+        ids = nuif.DeviceArray((cnt,), np.int32)
+        nuif.check(L.nuif_compact_flags(df.ptr, out.ptr, n, ids.ptr, None))
+        np.testing.assert_array_equal(ids.download(), np.nonzero(flags)[0].astype(np.int32))
+    # in place
+    dv2 = nuif.DeviceArray.from_host(vals)
+    nuif.check(L.nuif_exclusive_scan_u32(dv2.ptr, dv2.ptr, n, tot.ptr, scratch.ptr, None))
+    np.testing.assert_array_equal(dv2.download(),
+                                  np.concatenate([[0], np.cumsum(vals, dtype=np.uint64)[:-1]]).astype(np.uint32))
+
+
+def _oracle_volume(wl, frames):
+    e = O.OracleEngine(wl, prec=0, gate=0.0)
+    for f in range(frames):
+        e.run(wl.render(f)[0])
+    return e.volume_raw()
+
+
+@pytest.mark.parametrize("res", [128, 100])
+def test_marching_cubes_matches_oracle(gpu, res):
+    import dataclasses
+    announce(f"marching cubes {res}^3 after 3 frames")
+    L = sig(gpu)
+    wl = dataclasses.replace(W.SMALL, volume_res=res)
+    vol = _oracle_volume(wl, 3)
+    N = res
+    total_ref = O.lib().or_mc_count_s16(vol, N)
+    assert total_ref > 3000
+    ref = np.empty((total_ref, 3), np.float32)
+    O.lib().or_mc_emit_s16(vol, None, N, wl.volume_dim, ref, None, total_ref)
+    p = nuif.tsdf_params(N, wl.volume_dim, wl.trunc)
+    dvol = nuif.DeviceArray.from_host(vol)
+    verts = nuif.DeviceArray((total_ref + 16, 3), np.float32)
+    cols = nuif.DeviceArray((total_ref + 16, 4), np.float32)
+    tot = nuif.DeviceArray((1,), np.uint32)
+    scratch = nuif.DeviceArray((L.nuif_mc_scratch_bytes(C.byref(p)),), np.uint8)
+    nuif.check(L.nuif_mc_dense(dvol.ptr, None, C.byref(p), 0, verts.ptr, cols.ptr, total_ref + 16, tot.ptr,
+                               scratch.ptr, None))
+    total = int(tot.download()[0])
+    assert total == total_ref  # triangle count: exact
+    got = verts.download()[:total]
+    # same triangles: compare sorted triangle rows (9 floats each), bit for bit
+    def tris(v):
+        t = v.reshape(-1, 9)
+        return t[np.lexsort(t.T[::-1])]
+    np.testing.assert_array_equal(tris(got), tris(ref))
+    c = cols.download()[:total]
+    np.testing.assert_array_equal(c, np.tile([0, 0, 0, 1.0], (total, 1)).astype(np.float32))
+    # capped output writes only `capacity` vertices but reports the full count
+    nuif.check(L.nuif_mc_dense(dvol.ptr, None, C.byref(p), 0, verts.ptr, None, 999, tot.ptr, scratch.ptr, None))
+    assert int(tot.download()[0]) == total_ref
