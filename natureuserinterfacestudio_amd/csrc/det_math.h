@@ -12,6 +12,7 @@
 #include <cmath>
 
 #if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define NUIF_HD __host__ __device__ __forceinline__
 #else
 #define NUIF_HD inline
