@@ -45,13 +45,78 @@ def cpu_baseline(wl, frames):
                       f"float oracle with the reference's short2 volume, OpenMP"}
 
 
+def run_single(nuif, wl, frames, storage, graph, warmup, steps, reduction=None, stage_timing=True):
+    """One session, frames back to back on its stream.  Returns throughput, the
+    per-stage device times (event-timed when graphs are off) and updated voxels."""
+    eng = nuif.Engine(wl.kinfu_desc(storage=storage, gate=0.0, use_graph=bool(graph), reduction=reduction))
+    for f in range(warmup):
+        eng.run(frames[f], sync=False)
+    eng.synchronize()
+    eng.set_stage_timing(stage_timing and not graph)
+    eng.take_stage_times()
+    upd0 = eng.updated_voxels()
+    start, stop = nuif.Event(), nuif.Event()
+    nuif.check(nuif.L().nuif_device_synchronize(), "sync")
+    t0 = time.perf_counter()
+    start.record(eng.stream)
+    for f in range(warmup, warmup + steps):
+        eng.run(frames[f], sync=False)
+    stop.record(eng.stream)
+    stop.synchronize()
+    nuif.check(nuif.L().nuif_device_synchronize(), "sync")
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_ms(stop)
+    ok = eng.synchronize()
+    timed, stage_ms = eng.take_stage_times()
+    upd = eng.updated_voxels() - upd0
+    eng.close()
+    return {"fps": steps / (gpu_ms / 1e3), "ms_per_frame": gpu_ms / steps, "wall_s": wall, "ok": bool(ok),
+            "stage_avg": [x / max(timed, 1) for x in stage_ms] if timed else None,
+            "updated_per_frame": upd / max(steps, 1)}
+
+
+def run_server(nuif, wl, frames, sessions, storage, warmup, rounds, budget):
+    """`sessions` resident sessions; each round submits one frame per session and
+    waits for all of them (closed loop).  Session s plays frames s, s+1, ..."""
+    srv = nuif.Server(budget)
+    desc = wl.kinfu_desc(storage=storage, gate=0.0, use_graph=True)
+    ids = []
+    for _ in range(sessions):
+        sid = srv.open(desc)
+        if sid is None:
+            break
+        ids.append(sid)
+    for r in range(warmup):  # first frames, graph capture
+        srv.submit(ids, [frames[r + i] for i in range(len(ids))])
+        srv.wait()
+    # timed rounds, back to back (each round waits for the previous one)
+    nuif.check(nuif.L().nuif_device_synchronize(), "sync")
+    lat = []
+    t0 = time.perf_counter()
+    for r in range(warmup, warmup + rounds):
+        srv.submit(ids, [frames[r + i] for i in range(len(ids))])
+        lat.append(srv.wait())
+    nuif.check(nuif.L().nuif_device_synchronize(), "sync")
+    t_all = time.perf_counter() - t0
+    ok = all(srv.sessions[i].synchronize() == 1 for i in ids)
+    lat = np.concatenate(lat)
+    out = {"sessions": len(ids), "fps": len(ids) * rounds / t_all, "round_ms": t_all / rounds * 1e3,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "max_ms": float(lat.max()), "bytes_in_use": int(srv.bytes_in_use), "ok": bool(ok)}
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=60, help="timed frames (single) / rounds (server)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--storage", choices=sorted(STORAGES), default="s16")
-    ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--storage", choices=sorted(STORAGES), default="s16", help="single-session volume storage")
+    ap.add_argument("--graph", type=int, default=0, help="single session replayed from a hipGraph")
+    ap.add_argument("--sessions", type=int, default=32, help="server sessions (0: single session only)")
+    ap.add_argument("--server-storage", choices=sorted(STORAGES), default="f16")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="server device-memory budget")
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -66,70 +131,65 @@ def main():
     wl = W.FULL
     nuif.init(0)
     info = nuif.device_info()
-    n = args.warmup + args.steps
+    n = args.warmup + args.steps + max(args.sessions - 1, 0)
     if n > wl.frames:
         wl = dataclasses.replace(wl, frames=n)
     print(f"[bench] {info['name']} CUs={info['cu']}; rendering {n} frames of {wl.name}", file=sys.stderr, flush=True)
     frames = [nuif.DeviceArray.from_host(wl.render(f)[0]) for f in range(n)]
+
+    # 1) one session, reference-precision volume, stage-timed: the roofline of the integration kernel
     storage = STORAGES[args.storage]
-    eng = nuif.Engine(wl.kinfu_desc(storage=storage, gate=0.0, use_graph=bool(args.graph)))
-
-    for f in range(args.warmup):
-        eng.run(frames[f], sync=False)
-    eng.synchronize()
-    eng.set_stage_timing(not args.graph)
-    eng.take_stage_times()
-    upd0 = eng.updated_voxels()
-
-    start, stop = nuif.Event(), nuif.Event()
-    nuif.check(nuif.L().nuif_device_synchronize(), "sync")
-    t0 = time.perf_counter()
-    start.record(eng.stream)
-    for f in range(args.warmup, n):
-        eng.run(frames[f], sync=False)
-    stop.record(eng.stream)
-    stop.synchronize()
-    nuif.check(nuif.L().nuif_device_synchronize(), "sync")
-    wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_ms(stop)
-    ok = eng.synchronize()
-    timed_frames, stage_ms = eng.take_stage_times()
-    upd = eng.updated_voxels() - upd0
-
-    ms_per_step = gpu_ms / args.steps
-    value = args.steps / (gpu_ms / 1e3)
-    stage_avg = [s / max(timed_frames, 1) for s in stage_ms]
-    names = ["preprocess", "icp", "integrate", "raycast"]
-
-    # dominant kernel: integration (one launch per frame).  Algorithmic bytes per
-    # launch: read + write of every updated voxel (B bytes each) + the depth image.
+    print(f"[bench] single session, storage {args.storage}", file=sys.stderr, flush=True)
+    single = run_single(nuif, wl, frames, storage, args.graph, args.warmup, args.steps)
     B = nuif.L().nuif_tsdf_voxel_bytes(storage)
-    upd_per = upd / max(args.steps, 1)
-    integ_bytes = 2 * B * upd_per + 4 * wl.width * wl.height
+    integ_bytes = 2 * B * single["updated_per_frame"] + 4 * wl.width * wl.height
+    names = ["preprocess", "icp", "integrate", "raycast"]
     roof = None
-    if timed_frames:
-        t_int = stage_avg[2] / 1e3
+    if single["stage_avg"]:
+        t_int = single["stage_avg"][2] / 1e3
         achieved = integ_bytes / t_int / 1e9
-        roof = {"kernel": "integrate_tile_kernel", "bound": "hbm", "achieved": round(achieved, 2),
+        roof = {"kernel": f"integrate_tile_kernel<{args.storage}>", "bound": "hbm", "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "algorithmic_bytes": int(integ_bytes), "avg_ms": round(stage_avg[2], 4),
-                "updated_voxels_per_launch": int(upd_per)}
+                "algorithmic_bytes": int(integ_bytes), "avg_ms": round(single["stage_avg"][2], 4),
+                "updated_voxels_per_launch": int(single["updated_per_frame"])}
+
+    server = fp32 = None
+    if args.sessions > 0:
+        # 2) the plain FP32 pipeline, one session at a time (the server's reference point)
+        print("[bench] FP32 single session, serial", file=sys.stderr, flush=True)
+        fp32 = run_single(nuif, wl, frames, STORAGES["f32"], 0, args.warmup, args.steps, stage_timing=False)
+        # 3) the multi-session server
+        print(f"[bench] server: {args.sessions} sessions, storage {args.server_storage}", file=sys.stderr, flush=True)
+        server = run_server(nuif, wl, frames, args.sessions, STORAGES[args.server_storage], args.warmup, args.steps,
+                            args.budget_gb * 2 ** 30)
 
     cpu = None
     if not args.no_cpu:
         print(f"[bench] CPU baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
         cpu = cpu_baseline(wl, args.cpu_frames)
 
+    if server:
+        value, steps_ms = server["fps"], server["round_ms"]
+        workload = (f"{wl.name}: {server['sessions']} resident sessions, one fused frame each per round "
+                    f"(preprocess + ICP 10/5/4 + integrate every frame + raycast), {args.server_storage} volumes")
+    else:
+        value, steps_ms = single["fps"], single["ms_per_frame"]
+        workload = f"{wl.name}: 1 session, fused frame (preprocess + ICP 10/5/4 + integrate every frame + raycast)"
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": round(steps_ms, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{wl.name}: 1 session, fused frame (preprocess + ICP 10/5/4 + integrate every "
-                               f"frame + raycast)", "storage": args.storage, "graph": bool(args.graph),
-                   "resolution": [wl.width, wl.height], "volume": [wl.volume_res] * 3, "volume_m": wl.volume_dim,
-                   "tracking_ok": bool(ok)},
-        "stage_ms": {k: round(v, 4) for k, v in zip(names, stage_avg)},
-        "wall_s": round(wall, 4),
+        "config": {"workload": workload, "resolution": [wl.width, wl.height], "volume": [wl.volume_res] * 3,
+                   "volume_m": wl.volume_dim, "sessions": server["sessions"] if server else 1},
+        "p50_frame_latency_ms": round(server["p50_ms"], 3) if server else None,
+        "p99_frame_latency_ms": round(server["p99_ms"], 3) if server else None,
+        "server": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in server.items()} if server else None,
+        "single_session": {"storage": args.storage, "graph": bool(args.graph), "fps": round(single["fps"], 3),
+                           "ms_per_frame": round(single["ms_per_frame"], 4), "tracking_ok": single["ok"],
+                           "stage_ms": {k: round(v, 4) for k, v in zip(names, single["stage_avg"])}
+                           if single["stage_avg"] else None},
+        "fp32_serial_fps": round(fp32["fps"], 3) if fp32 else None,
+        "server_vs_fp32_serial": round(server["fps"] / fp32["fps"], 3) if server and fp32 else None,
         "roofline": roof,
         "cpu_baseline": cpu,
         "device": info["name"],

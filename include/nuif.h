@@ -131,8 +131,8 @@ int nuif_transform_maps(const float* vmap, const float* nmap, float* vmap_dst, f
 int nuif_scale_depths(float* depth, int width, int height, const NuiCLCameraParams* cam, nuif_stream_t s);
 
 /* Fused frame preprocessing: OCL/NuiKinfuOpenCLFrame.cpp:314-367 UpdateVertexBuffers
- * (H1 passing + H3 bilateral + H5 level 0) and OCL/NuiKinfuOpenCLDepthTracker.cpp:140-236
- * (H4 pyramid + H5 per level) in as few launches as the data flow allows.
+ * (passing filter + bilateral + level-0 vertices) and OCL/NuiKinfuOpenCLDepthTracker.cpp:140-236
+ * (pyramid + vertices per level) in as few launches as the data flow allows.
  *   depth_out      : passing-filter depth (W*H); if scale_for_tsdf, already scaled to ray
  *                    length (K/tsdfVolume.cl:8-29) for the dense scene's integration
  *   depth_pyr[l]   : filtered depth per level (level 0 = bilateral output)
@@ -172,17 +172,23 @@ typedef struct nuif_icp_config {
     int reduction;                         /* NUIF_ICP_REDUCE_* */
 } nuif_icp_config;
 
+/* State block: the tracking state plus the completion counters of the fused reductions
+ * (device memory, initialised by nuif_icp_state_init). */
 size_t nuif_icp_state_bytes(void);
+/* Workspace of nuif_icp_iteration for a width x height level-0 image. */
 size_t nuif_icp_partials_bytes(int width, int height);
 /* Reset tracking state: estimate = pose, last-integration pose = pose. */
 int nuif_icp_state_init(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s);
 int nuif_icp_state_read(const void* state, nuif_icp_result* out, nuif_stream_t s); /* synchronises s */
 /* Begin a frame: estimate <- pose, clear break/failure flags. */
 int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s);
-/* One iteration at `level`: association + 29-term reduction (partials) and, in the last
- * workgroup, the reference's LM accept/reject, FP64 6x6 solve and pose update.
+/* One iteration at `level`, ONE kernel launch: association + 29-term reduction and, in the
+ * workgroup that completes the reduction, the reference's LM accept/reject, FP64 6x6 solve
+ * and pose update (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-497).
  * vmap_level: current-frame vertices at the level; ref_vmap/ref_nmap: world maps at full
- * resolution; prev_pose: previous frame's transform. count_threshold per quirk Q10. */
+ * resolution; prev_pose: previous frame's transform; count_threshold: see
+ * nuif_icp_count_thresholds.  Reference order handles at most 2^22 pixels per level.
+ * Launches sharing one state block must not overlap (they share its counters). */
 int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* cam, const float* vmap_level,
                        int level, const float* ref_vmap, const float* ref_nmap,
                        const NuiCLRigidTransform* prev_pose, float dist_threshold, uint32_t count_threshold,
@@ -202,7 +208,7 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
  * tracking success of the last frame (raycast runs only if set). */
 const int32_t* nuif_icp_gate_flag(const void* state);
 const int32_t* nuif_icp_ok_flag(const void* state);
-/* Count thresholds of the reference's stale-global-size rule (quirk Q10). */
+/* Count thresholds of the reference's stale-global-size rule. */
 int nuif_icp_count_thresholds(int width, int height, const nuif_icp_config* cfg, uint32_t* thresholds);
 
 /* ---- dense TSDF scene (OCL/NuiKinfuOpenCLScene.cpp, K/tsdfVolume.cl, K/raycast.cl) ---- */
@@ -238,7 +244,7 @@ int nuif_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_
 int nuif_exclusive_scan_flags(const uint8_t* flags, uint32_t* out, size_t n, uint32_t* total, void* scratch,
                               nuif_stream_t s);
 /* ids[scan[i]] = i for every i with flags[i] > 0 (order preserving; the reference's
- * compactifyValidPrefixSum_kernel is racy, quirk Q2) */
+ * compactifyValidPrefixSum_kernel is racy) */
 int nuif_compact_flags(const uint8_t* flags, const uint32_t* scan, size_t n, int32_t* ids, nuif_stream_t s);
 
 /* ---- dense marching cubes (K/marchingCube.cl:100-256, OCL/NuiKinfuOpenCLScene.cpp:617-786) ---- */

@@ -1,0 +1,105 @@
+/*
+ * nuif_kinfu.h - C ABI of the tracking engine and the multi-session fusion
+ * server (libnuif_kinfu.so), for hosts that are not C++ (ctypes, cgo, JNI).
+ *
+ * The engine entry points drive NuiKinfuTrackingEngine::RunTracking with a
+ * dense scene as the reference's tracking manager does
+ * (src/SLAM/VisualOdometry/NuiKinfuTrackingManager.cpp:53-108 calling
+ * src/SLAM/VisualOdometry/NuiKinfuTrackingEngine.cpp:68-142).  The reference
+ * runs one session on one global in-order queue
+ * (src/OpenCLUtilities/NuiOpenCLGlobal.cpp:175); the server entry points keep
+ * many sessions resident on one GPU, each on its own stream, admitted against
+ * a device-memory budget.
+ *
+ * Every function returns 0 (or a documented non-negative value) on success and
+ * a negative status on failure; nuifk_last_error() gives the calling thread's
+ * last message.  No C++ exception crosses this boundary.
+ */
+#ifndef NUIF_KINFU_H
+#define NUIF_KINFU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "nuif.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One session: camera, tracker (VO/NuiTrackerConfig.cpp:5-24) and dense volume
+ * (VO/NuiKinfuVolumeConfig.cpp:5-15) settings. */
+typedef struct nuifk_desc {
+    int width, height;
+    float fx, fy, cx, cy;
+    float depth_min, depth_max;     /* metres */
+    int filter_radius;
+    float sigma_space2_inv_half, sigma_depth2_inv_half, depth_threshold, dist_threshold;
+    int levels;
+    int iterations[4];              /* level 0 = finest */
+    float integration_threshold;    /* 0 = integrate every frame */
+    int resolution[3];
+    float dimensions[3];
+    float tranc_dist;
+    int color;                      /* colour volume on / off */
+    int storage;                    /* NUIF_TSDF_* */
+    int use_graph;                  /* replay tracked frames from a captured hipGraph */
+    int icp_reduction;              /* NUIF_ICP_REDUCE_* */
+} nuifk_desc;
+
+const char* nuifk_last_error(void);
+
+/* ---- one session: NuiKinfuTrackingEngine + dense NuiKinfuScene ------------- */
+int nuifk_create(const nuifk_desc* d, void** out);
+int nuifk_destroy(void* h);
+/* One frame (VO/NuiKinfuTrackingEngine.h:36-42 RunTracking).  depth: W*H uint16 mm,
+ * a device pointer if on_device else a host pointer; bgra: W*H*4 bytes or NULL.
+ * sync != 0 waits and returns 1 (tracked) / 0 (tracking failure); sync == 0 returns 1. */
+int nuifk_run(void* h, const uint16_t* depth, int on_device, const uint8_t* bgra, int sync);
+int nuifk_synchronize(void* h);                    /* 1: no failure so far, 0: a failure */
+int nuifk_reset(void* h);
+/* Pose history rows (NUIF_POSE_ROW floats) of the first n frames; returns the count. */
+int nuifk_poses(void* h, float* out, int n);
+int nuifk_state(void* h, nuif_icp_result* out);
+int nuifk_read_volume(void* h, void* out, size_t bytes);
+int nuifk_read_maps(void* h, float* vmap, float* nmap);
+/* VO/NuiTrackerConfig.cpp:26-115 key=value files; -1 if the file cannot be opened. */
+int nuifk_load_tracker_config(const char* path, nuifk_desc* d);
+int nuifk_save_tracker_config(const char* path, const nuifk_desc* d);
+int nuifk_set_stage_timing(void* h, int on);
+/* Sums (ms) of preprocess / ICP / integrate / raycast over the timed frames; returns their number. */
+int nuifk_take_stage_times(void* h, double* ms4);
+int nuifk_updated_voxels(void* h, unsigned long long* out);
+void* nuifk_stream(void* h);
+void* nuifk_volume_ptr(void* h);
+size_t nuifk_volume_bytes(void* h);
+float nuifk_trunc(void* h);
+
+/* ---- multi-session fusion server ------------------------------------------
+ * Sessions stay resident (volume, pyramids, pose, tracking state).  A round
+ * submits one frame for each listed session; each session's pipeline is
+ * enqueued on its own stream (replayed from a captured graph when the session
+ * asks for it) so the GPU overlaps the sessions.  Latency of a frame = device
+ * time from the round's start event to the end of that frame's pipeline. */
+int nuifs_create(size_t hbm_budget_bytes, void** out);
+int nuifs_destroy(void* srv);
+/* Device bytes a session would hold (volume + frame / pyramid / map buffers). */
+size_t nuifs_session_bytes(const nuifk_desc* d);
+/* Admits a session; returns its id (>= 0) or NUIF_ERR_OOM when the budget is exceeded. */
+int nuifs_open(void* srv, const nuifk_desc* d);
+int nuifs_close(void* srv, int id);
+size_t nuifs_bytes_in_use(void* srv);
+int nuifs_session_count(void* srv);
+/* The engine handle of a session (for the nuifk_* read-back calls). */
+void* nuifs_engine(void* srv, int id);
+/* Enqueue one round: n sessions, device-resident depth frames (W*H uint16 each). */
+int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths);
+/* Wait for the last round; latency_ms[i] for the i-th submitted frame (may be NULL).
+ * Returns the number of frames whose tracking failed in the round. */
+int nuifs_wait(void* srv, float* latency_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUIF_KINFU_H */

@@ -1,19 +1,32 @@
-// Projective point-to-plane ICP (rows H8-H11).
+// Projective point-to-plane ICP.
 //
 // The reference runs per iteration: icp_block_kernel (128-wide LDS trees of
 // 29 floats), compute_sums, a blocking read-back, and the Levenberg-Marquardt
 // bookkeeping + Eigen FP64 6x6 solve on the host
-// (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504).  Here every iteration is two
-// device launches and the whole pyramid runs without a host round trip:
+// (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504).  Here every iteration is ONE
+// device launch and the whole pyramid runs without a host round trip:
 //
-//   icp_partials_kernel  grid-stride over the level's pixels; each thread keeps
-//                        Kahan-compensated FP32 sums of the 29 terms, the block
-//                        transposes them through LDS and reduces each term with
-//                        a compensated 32-wide pass, combined in FP64.
-//   icp_solve_kernel     one workgroup: FP64 sum over block partials, the
-//                        reference's accept/reject state machine (quirks
-//                        Q10-Q12 kept), FP64 LDL^T solve of the damped 6x6
-//                        system, Euler-angle increment composed in FP64.
+//   icp_ref_kernel   reference order: one 128-item block per 128 pixels and the
+//                    reference's tree pairing (item i takes item i + d for
+//                    d = 64..1) as one LDS step plus wave shuffles; the last
+//                    block of every 128 to finish runs compute_sums' tree over
+//                    their partials, and the last of those sums the group
+//                    results in group order, as the host does, then solves.
+//   icp_comp_kernel  compensated: grid-stride blocks, Kahan FP32 per thread,
+//                    FP64 across threads; the last block to finish sums the
+//                    block partials in block order in FP64, then solves.
+//
+// Completion is counted with device-scope atomics after a release fence, and
+// the counting block acquires before it reads other blocks' partials.  The
+// block that completes a counter re-arms it, so the next launch starts from
+// zero without a fill.  Every sum is taken in a fixed order: the result does
+// not depend on which block finishes last.
+//
+// The solve runs on one thread over an LDS copy of the state: the reference's
+// accept/reject state machine (count threshold left over from the previous
+// level, constant distance threshold, reject restores the last good pose),
+// the damped 6x6 system normalised by the count, solved in FP64 (LLT in
+// reference order, LDL^T compensated), and the Euler-angle increment.
 //
 // The tracking state (estimate, last good pose, lambda, break flags, last
 // integration pose, integration gate) lives in one device block so that
@@ -26,7 +39,8 @@ namespace nuif {
 
 constexpr int kTerms = NUIF_ICP_TERMS;
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 1024;
+constexpr int kMaxBlocks = 1024;   // compensated: blocks per launch
+constexpr int kMaxGroups2 = 256;   // reference: 128-block groups per launch (W*H <= 2^22)
 
 struct IcpState {
     float R[9], t[3];          // current estimate (Rcurr, tcurr)
@@ -41,6 +55,34 @@ struct IcpState {
     float goodf[27];       // reference mode: goodCores as the reference keeps them (float)
     float pad2;
 };
+static_assert(sizeof(IcpState) % 8 == 0, "state is copied as 32-bit words");
+
+// Completion counters, after the state block in the same allocation
+struct IcpSync {
+    uint32_t arrive[kMaxGroups2];  // reference: finished blocks of each 128-block group
+    uint32_t arrive_all;           // finished groups (reference) or blocks (compensated)
+    uint32_t pad[63];
+};
+constexpr size_t kStateBytes = (sizeof(IcpState) + 255) / 256 * 256;
+constexpr size_t kSyncBytes = (sizeof(IcpSync) + 255) / 256 * 256;
+
+__device__ __forceinline__ IcpSync* sync_of(IcpState* st) {
+    return reinterpret_cast<IcpSync*>(reinterpret_cast<char*>(st) + kStateBytes);
+}
+
+__device__ __forceinline__ void clear_sync(IcpSync* sy, int tid, int nthreads) {
+    for (int i = tid; i < kMaxGroups2; i += nthreads) sy->arrive[i] = 0u;
+    if (tid == 0) sy->arrive_all = 0u;
+}
+
+// The reference tree's steps d = 32..1 (item i takes item i + d) inside one
+// wave; lane 0 ends with the sum.  A lane whose source is past the wave keeps
+// its own value, which no lane below it reads afterwards.
+__device__ __forceinline__ float wave_tree64(float v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    return v;
+}
 
 // Reduction orders (nuif_icp_config::reduction)
 constexpr int kRefOrder = 0;      // the reference's float trees, bit for bit
@@ -54,7 +96,7 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 }
 
 // K/icp.cl:192-220 interpolateBilinear_withHoles.  Reads past the row end as
-// the reference does (quirk Q14); a read past the buffer end is a hole.
+// the reference does; a read past the buffer end is a hole.
 __device__ __forceinline__ f3 bilinear_holes(const float* __restrict__ src, float px, float py, int W, int H) {
     const int x = (int)floorf(px), y = (int)floorf(py);
     const float dx = px - (float)x, dy = py - (float)y;
@@ -74,14 +116,23 @@ __device__ __forceinline__ f3 bilinear_holes(const float* __restrict__ src, floa
 // LDS: 29 terms x 8 segments x 33 (padded) floats, reused as doubles later
 constexpr int kSeg = 8, kSegLen = 32, kSegPad = 33;
 
+template <int MODE>
+__device__ void finish_iteration(IcpState* st, IcpState& S, const double* sums, const float* sumsf, int level,
+                                 int iter, uint32_t count_threshold);
+
+// The state and the partials are written and read by different blocks of one
+// launch: no __restrict__ on them.
 __global__ void __launch_bounds__(kBlock)
-    icp_partials_kernel(const IcpState* __restrict__ st, const float* __restrict__ vcur, int npx,
-                        NuiCLCameraParams cam, const float* __restrict__ vref, const float* __restrict__ nref,
-                        const NuiCLRigidTransform* __restrict__ prev, float distThr, int level,
-                        double* __restrict__ partials) {
+    icp_comp_kernel(IcpState* st, const float* __restrict__ vcur, int npx, NuiCLCameraParams cam,
+                    const float* __restrict__ vref, const float* __restrict__ nref,
+                    const NuiCLRigidTransform* __restrict__ prev, float distThr, int level, int iter,
+                    uint32_t count_threshold, double* partials) {
     if (st->failed || st->brk[level]) return;
     __shared__ float lds[kTerms * kSeg * kSegPad];
     __shared__ double part[kTerms * kSeg];
+    __shared__ double sums[kTerms];
+    __shared__ IcpState S;
+    __shared__ int last;
 
     float R[9], t[3], Ri[9], tp[3];
 #pragma unroll
@@ -146,6 +197,30 @@ __global__ void __launch_bounds__(kBlock)
         for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
         partials[(size_t)blockIdx.x * kTerms + tid] = s;
     }
+
+    // the last block to finish reduces the block partials and solves
+    IcpSync* sy = sync_of(st);
+    __threadfence();  // release this block's partial (written by threads 0..28)
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(&sy->arrive_all, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // acquire the other blocks' partials
+    const int nblocks = (int)gridDim.x;
+    if (tid < kTerms * kSeg) {
+        const int k = tid / kSeg, j = tid % kSeg;
+        double s = 0.0;
+        for (int b = j; b < nblocks; b += kSeg) s += partials[(size_t)b * kTerms + k];
+        part[k * kSeg + j] = s;
+    }
+    __syncthreads();
+    if (tid < kTerms) {
+        double s = 0.0;
+        for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
+        sums[tid] = s;
+    }
+    if (tid == 0) sy->arrive_all = 0u;  // re-arm: every block has arrived
+    finish_iteration<kCompensated>(st, S, sums, nullptr, level, iter, count_threshold);
 }
 
 // One pixel's 29 terms of K/icp.cl:243-289 (zeros when rejected).
@@ -179,29 +254,21 @@ __device__ __forceinline__ void icp_terms(const f3 vc, const float* R, const flo
     out[28] = 1.0f;
 }
 
-// K/icp.cl:162-190 reduce(): in-place 128-wide tree over buf[128][29]
-__device__ __forceinline__ void tree128(float* buf, int lid) {
-    for (int d = 64; d > 0; d >>= 1) {
-        __syncthreads();
-        if (lid < d) {
-            float* a = buf + lid * kTerms;
-            const float* b = buf + (lid + d) * kTerms;
-            for (int i = 0; i < kTerms; ++i) a[i] += b[i];
-        }
-    }
-    __syncthreads();
-}
-
-// Reference-order partials: one 128-item work-group per 128 pixels, the
-// reference's LDS tree, 29 floats per group (K/icp.cl:222-292).  A partial
-// last group counts its missing items as zeros.
+// Reference order, one launch per iteration.  Block g (128 items) takes pixels
+// 128g..128g+127; a partial last block counts its missing items as zeros
+// (K/icp.cl:222-292).  The partials buffer holds ngroups x 29 group sums
+// followed by ceil(ngroups / 128) x 29 compute_sums results.
 __global__ void __launch_bounds__(128)
-    icp_ref_partials_kernel(const IcpState* __restrict__ st, const float* __restrict__ vcur, int npx,
-                            NuiCLCameraParams cam, const float* __restrict__ vref, const float* __restrict__ nref,
-                            const NuiCLRigidTransform* __restrict__ prev, float distThr, int level,
-                            float* __restrict__ partials) {
+    icp_ref_kernel(IcpState* st, const float* __restrict__ vcur, int npx, NuiCLCameraParams cam,
+                   const float* __restrict__ vref, const float* __restrict__ nref,
+                   const NuiCLRigidTransform* __restrict__ prev, float distThr, int level, int iter,
+                   uint32_t count_threshold, float* partials) {
     if (st->failed || st->brk[level]) return;
-    __shared__ float buf[128 * kTerms];
+    __shared__ float xfer[64 * kTerms];
+    __shared__ float sumsf[kTerms];
+    __shared__ double sums[kTerms];
+    __shared__ IcpState S;
+    __shared__ int last, last2;
     float R[9], t[3], Ri[9], tp[3];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -213,15 +280,68 @@ __global__ void __launch_bounds__(128)
         t[k] = st->t[k];
         tp[k] = prev->t[k];
     }
-    const int lid = threadIdx.x;
-    const int p = blockIdx.x * 128 + lid;
+    const int lid = threadIdx.x, lane = lid & 63, wave = lid >> 6;
+    const int g = blockIdx.x, ngroups = (int)gridDim.x;
+    const int p = g * 128 + lid;
     float v[kTerms];
     const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
     icp_terms(vc, R, t, Ri, tp, cam, vref, nref, distThr, v);
+
+    // K/icp.cl:162-190 reduce(): step d = 64 through LDS, steps 32..1 in wave 0
+    if (wave == 1) {
 #pragma unroll
-    for (int k = 0; k < kTerms; ++k) buf[lid * kTerms + k] = v[k];
-    tree128(buf, lid);
-    if (lid < kTerms) partials[(size_t)blockIdx.x * kTerms + lid] = buf[lid];
+        for (int k = 0; k < kTerms; ++k) xfer[lane * kTerms + k] = v[k];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int k = 0; k < kTerms; ++k) v[k] = wave_tree64(v[k] + xfer[lane * kTerms + k]);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < kTerms; ++k) partials[(size_t)g * kTerms + k] = v[k];
+        }
+    }
+
+    // the last block of each 128-block group runs compute_sums' tree over the group
+    IcpSync* sy = sync_of(st);
+    const int grp = g >> 7;
+    if (lid == 0) {
+        __threadfence();  // release this block's partial
+        const uint32_t members = (uint32_t)min(128, ngroups - (grp << 7));
+        last = atomicAdd(&sy->arrive[grp], 1u) == members - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // acquire the group's partials
+    float* tree = partials + (size_t)ngroups * kTerms;
+    for (int k = wave; k < kTerms; k += 2) {
+        const int g0 = (grp << 7) + lane, g1 = g0 + 64;
+        const float x = g0 < ngroups ? partials[(size_t)g0 * kTerms + k] : 0.f;
+        const float y = g1 < ngroups ? partials[(size_t)g1 * kTerms + k] : 0.f;
+        const float sum = wave_tree64(x + y);
+        if (lane == 0) tree[grp * kTerms + k] = sum;
+    }
+    __threadfence();  // release the group result (written by lanes 0 of both waves)
+    __syncthreads();
+    const int ngroups2 = (ngroups + 127) >> 7;
+    if (lid == 0) {
+        sy->arrive[grp] = 0u;  // re-arm: every block of the group has arrived
+        last2 = atomicAdd(&sy->arrive_all, 1u) == (uint32_t)(ngroups2 - 1);
+    }
+    __syncthreads();
+    if (!last2) return;
+    __threadfence();  // acquire every group result
+
+    // the host's sequential float sum over the compute_sums groups
+    // (OCL/NuiKinfuOpenCLDepthTracker.cpp:394-402, 428-440)
+    if (lid < kTerms) {
+        float sum = 0.f;
+        for (int c = 0; c < ngroups2; ++c) sum += tree[c * kTerms + lid];
+        sumsf[lid] = sum;
+        sums[lid] = (double)sum;
+    }
+    if (lid == 0) sy->arrive_all = 0u;
+    finish_iteration<kRefOrder>(st, S, sums, sumsf, level, iter, count_threshold);
 }
 
 // ---- small FP64 linear algebra for the solve ----------------------------------
@@ -351,104 +471,54 @@ __device__ void inverse3f(const float m[9], float out[9]) {
     out[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
 }
 
-// The host loop of OCL/NuiKinfuOpenCLDepthTracker.cpp:382-497 for one iteration,
-// on one workgroup.  Reference order: compute_sums' 128-wide trees over the group
-// partials, then the host's sequential float sum over the second-level blocks,
-// float goodCores, Eigen-like LLT in double and the float Euler increment.
-// Compensated order: FP64 sums, LDL^T in FP64 and the increment composed in FP64.
+// The host loop of OCL/NuiKinfuOpenCLDepthTracker.cpp:402-497 for one iteration,
+// given the iteration's sums.  Reference order: float sums and goodCores,
+// Eigen-like LLT in double and the float Euler increment.  Compensated order:
+// FP64 sums, LDL^T in FP64 and the increment composed in FP64.  Runs on one
+// thread over the LDS copy of the state.
 template <int MODE>
-__global__ void __launch_bounds__(kBlock)
-    icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials_v, int ngroups, int level,
-                     int iter, uint32_t count_threshold) {
-    if (st->failed || st->brk[level]) return;
-    __shared__ float buf[128 * kTerms];
-    __shared__ double part[kTerms * kSeg];
-    __shared__ double sums[kTerms];
-    __shared__ float sumsf[kTerms];
-    const int tid = threadIdx.x;
-    if (MODE == kRefOrder) {
-        const float* partials = static_cast<const float*>(partials_v);
-        const int nb2 = (ngroups + 127) / 128;  // compute_sums work-groups
-        if (tid < kTerms) sumsf[tid] = 0.f;
-        for (int b = 0; b < nb2; ++b) {
-            if (tid < 128) {
-                const int gi = b * 128 + tid;
-                for (int k = 0; k < kTerms; ++k)
-                    buf[tid * kTerms + k] = gi < ngroups ? partials[(size_t)gi * kTerms + k] : 0.f;
-            }
-            // tree128 with the whole 256-thread block taking part in the barriers
-            for (int d = 64; d > 0; d >>= 1) {
-                __syncthreads();
-                if (tid < d) {
-                    float* a = buf + tid * kTerms;
-                    const float* c = buf + (tid + d) * kTerms;
-                    for (int i = 0; i < kTerms; ++i) a[i] += c[i];
-                }
-            }
-            __syncthreads();
-            if (tid < kTerms) sumsf[tid] += buf[tid];  // host: sequential float sum over blocks
-            __syncthreads();
-        }
-        if (tid < kTerms) sums[tid] = (double)sumsf[tid];
-        __syncthreads();
-    } else {
-        const double* partials = static_cast<const double*>(partials_v);
-        if (tid < kTerms * kSeg) {
-            const int k = tid / kSeg, j = tid % kSeg;
-            double s = 0.0;
-            for (int b = j; b < ngroups; b += kSeg) s += partials[(size_t)b * kTerms + k];
-            part[k * kSeg + j] = s;
-        }
-        __syncthreads();
-        if (tid < kTerms) {
-            double s = 0.0;
-            for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
-            sums[tid] = s;
-        }
-        __syncthreads();
-    }
-    if (tid != 0) return;
-
-    // :395-458: error/count, accept or reject
-    for (int k = 0; k < kTerms; ++k) st->sums[k] = sums[k];
+__device__ void icp_update(IcpState& S, const double* sums, const float* sumsf, int level, int iter,
+                           uint32_t count_threshold) {
+    // :402-458: error/count, accept or reject
+    for (int k = 0; k < kTerms; ++k) S.sums[k] = sums[k];
     const float cnt = (float)sums[28];
     const float errsum = MODE == kRefOrder ? sumsf[27] : (float)sums[27];
     const float err = cnt > 0.f ? sqrtf(errsum) / cnt : 3.402823466e38f;
-    st->error = err;
-    st->count = cnt;
-    st->iterations += 1;
+    S.error = err;
+    S.count = cnt;
+    S.iterations += 1;
     if (iter == 0) {  // per-level initialisation (:299-306)
-        st->lambda = 1.0f;
-        st->err_old = 1e20f;
-        for (int k = 0; k < 9; ++k) st->Rg[k] = st->R[k];
-        for (int k = 0; k < 3; ++k) st->tg[k] = st->t[k];
+        S.lambda = 1.0f;
+        S.err_old = 1e20f;
+        for (int k = 0; k < 9; ++k) S.Rg[k] = S.R[k];
+        for (int k = 0; k < 3; ++k) S.tg[k] = S.t[k];
     }
-    float lambda = st->lambda;
-    if (cnt < (float)count_threshold || err > st->err_old) {
+    float lambda = S.lambda;
+    if (cnt < (float)count_threshold || err > S.err_old) {
         lambda *= 10.0f;
-        for (int k = 0; k < 9; ++k) st->R[k] = st->Rg[k];
-        for (int k = 0; k < 3; ++k) st->t[k] = st->tg[k];
-        st->lambda = lambda;
+        for (int k = 0; k < 9; ++k) S.R[k] = S.Rg[k];
+        for (int k = 0; k < 3; ++k) S.t[k] = S.tg[k];
+        S.lambda = lambda;
         if (iter == 0) {
-            st->brk[level] = 1;
+            S.brk[level] = 1;
             return;
         }
     } else {
         lambda /= 10.0f;
-        st->lambda = lambda;
-        for (int k = 0; k < 9; ++k) st->Rg[k] = st->R[k];
-        for (int k = 0; k < 3; ++k) st->tg[k] = st->t[k];
-        st->err_old = err;
+        S.lambda = lambda;
+        for (int k = 0; k < 9; ++k) S.Rg[k] = S.R[k];
+        for (int k = 0; k < 3; ++k) S.tg[k] = S.t[k];
+        S.err_old = err;
         for (int k = 0; k < 27; ++k) {
-            st->good[k] = sums[k];
-            st->goodf[k] = MODE == kRefOrder ? sumsf[k] : (float)sums[k];
+            S.good[k] = sums[k];
+            S.goodf[k] = MODE == kRefOrder ? sumsf[k] : (float)sums[k];
         }
     }
-    // damped normal equations, normalised by the current count (quirk Q12)
+    // damped normal equations, normalised by the current count
     double A[36], b[6];
     for (int i = 0, s = 0; i < 6; ++i) {
         for (int j = i; j < 7; ++j, ++s) {
-            const double v = MODE == kRefOrder ? (double)(st->goodf[s] / cnt) : st->good[s] / (double)cnt;
+            const double v = MODE == kRefOrder ? (double)(S.goodf[s] / cnt) : S.good[s] / (double)cnt;
             if (j == 6)
                 b[i] = v;
             else
@@ -459,7 +529,7 @@ __global__ void __launch_bounds__(kBlock)
     double det, x[6];
     const bool spd = MODE == kRefOrder ? llt6(A, &det) : ldlt6(A, &det);
     if (!spd || fabs(det) < 1e-15 || isnan(det)) {
-        st->failed = 1;
+        S.failed = 1;
         return;
     }
     if (MODE == kRefOrder)
@@ -468,7 +538,7 @@ __global__ void __launch_bounds__(kBlock)
         ldlt6_solve(A, b, x);
     const float al = (float)x[0], be = (float)x[1], ga = (float)x[2];
     if (al * al + be * be + ga * ga < 1e-6f) {  // converged: break before applying (:482-484)
-        st->brk[level] = 1;
+        S.brk[level] = 1;
         return;
     }
     // Rinc = Rz(gamma) Ry(beta) Rx(alpha); t <- Rinc^-1 t + t_inc; R <- Rinc^-1 R   (:486-497)
@@ -482,11 +552,11 @@ __global__ void __launch_bounds__(kBlock)
         matmul3f(Rzy, Rx, Rinc);
         inverse3f(Rinc, Ri);
         float tn[3];
-        for (int i = 0; i < 3; ++i) tn[i] = Ri[3 * i] * st->t[0] + Ri[3 * i + 1] * st->t[1] + Ri[3 * i + 2] * st->t[2];
-        for (int i = 0; i < 3; ++i) st->t[i] = tn[i] + tinc[i];
+        for (int i = 0; i < 3; ++i) tn[i] = Ri[3 * i] * S.t[0] + Ri[3 * i + 1] * S.t[1] + Ri[3 * i + 2] * S.t[2];
+        for (int i = 0; i < 3; ++i) S.t[i] = tn[i] + tinc[i];
         float Rn[9];
-        matmul3f(Ri, st->R, Rn);
-        for (int k = 0; k < 9; ++k) st->R[k] = Rn[k];
+        matmul3f(Ri, S.R, Rn);
+        for (int k = 0; k < 9; ++k) S.R[k] = Rn[k];
     } else {
         const double sa = sin((double)al), ca = cos((double)al);
         const double sb = sin((double)be), cb = cos((double)be);
@@ -501,17 +571,36 @@ __global__ void __launch_bounds__(kBlock)
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rinc[3 * j + i];
         double Rc[9], tc[3];
-        for (int k = 0; k < 9; ++k) Rc[k] = st->R[k];
-        for (int k = 0; k < 3; ++k) tc[k] = st->t[k];
+        for (int k = 0; k < 9; ++k) Rc[k] = S.R[k];
+        for (int k = 0; k < 3; ++k) tc[k] = S.t[k];
         double Rn[9];
         matmul3d(Ri, Rc, Rn);
         for (int i = 0; i < 3; ++i)
-            st->t[i] = (float)(Ri[3 * i] * tc[0] + Ri[3 * i + 1] * tc[1] + Ri[3 * i + 2] * tc[2] + (double)tinc[i]);
-        for (int k = 0; k < 9; ++k) st->R[k] = (float)Rn[k];
+            S.t[i] = (float)(Ri[3 * i] * tc[0] + Ri[3 * i + 1] * tc[1] + Ri[3 * i + 2] * tc[2] + (double)tinc[i]);
+        for (int k = 0; k < 9; ++k) S.R[k] = (float)Rn[k];
     }
 }
 
+// Run by the whole block that completes the reduction: copy the state into LDS,
+// update it on thread 0, write it back.  No other block reads the state after
+// it has arrived, so the write-back races with nothing.
+template <int MODE>
+__device__ void finish_iteration(IcpState* st, IcpState& S, const double* sums, const float* sumsf, int level,
+                                 int iter, uint32_t count_threshold) {
+    constexpr int kWords = (int)(sizeof(IcpState) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(st);
+    uint32_t* lds = reinterpret_cast<uint32_t*>(&S);
+    for (int i = threadIdx.x; i < kWords; i += blockDim.x) lds[i] = src[i];
+    __syncthreads();
+    if (threadIdx.x == 0) icp_update<MODE>(S, sums, sumsf, level, iter, count_threshold);
+    __syncthreads();
+    uint32_t* dst = reinterpret_cast<uint32_t*>(st);
+    for (int i = threadIdx.x; i < kWords; i += blockDim.x) dst[i] = lds[i];
+}
+
 __global__ void icp_begin_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
+    clear_sync(sync_of(st), threadIdx.x, blockDim.x);
+    if (threadIdx.x != 0) return;
     for (int k = 0; k < 9; ++k) st->R[k] = pose->R[k];
     for (int k = 0; k < 3; ++k) st->t[k] = pose->t[k];
     for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
@@ -521,6 +610,8 @@ __global__ void icp_begin_kernel(IcpState* st, const NuiCLRigidTransform* pose) 
 }
 
 __global__ void icp_state_init_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
+    clear_sync(sync_of(st), threadIdx.x, blockDim.x);
+    if (threadIdx.x != 0) return;
     for (int k = 0; k < 9; ++k) st->R[k] = st->Rg[k] = st->Rlast[k] = pose->R[k];
     for (int k = 0; k < 3; ++k) st->t[k] = st->tg[k] = st->tlast[k] = pose->t[k];
     for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
@@ -596,11 +687,12 @@ using namespace nuif;
 
 extern "C" {
 
-size_t nuif_icp_state_bytes(void) { return (sizeof(IcpState) + 255) / 256 * 256; }
+size_t nuif_icp_state_bytes(void) { return kStateBytes + kSyncBytes; }
 
 size_t nuif_icp_partials_bytes(int width, int height) {
     const size_t compensated = (size_t)kMaxBlocks * kTerms * sizeof(double);
-    const size_t reference = ((size_t)width * height + 127) / 128 * kTerms * sizeof(float);
+    const size_t groups = ((size_t)width * height + 127) / 128;
+    const size_t reference = (groups + (groups + 127) / 128) * kTerms * sizeof(float);
     return std::max(compensated, reference);
 }
 
@@ -615,7 +707,7 @@ const int32_t* nuif_icp_ok_flag(const void* state) {
 int nuif_icp_state_init(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(state && pose, "nuif_icp_state_init: null argument");
-    icp_state_init_kernel<<<1, 1, 0, stream_of(s)>>>((IcpState*)state, pose);
+    icp_state_init_kernel<<<1, 64, 0, stream_of(s)>>>((IcpState*)state, pose);
     NUIF_LAUNCHED("icp_state_init_kernel");
     return NUIF_OK;
 }
@@ -641,7 +733,7 @@ int nuif_icp_state_read(const void* state, nuif_icp_result* out, nuif_stream_t s
 int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(state && pose, "nuif_icp_begin: null argument");
-    icp_begin_kernel<<<1, 1, 0, stream_of(s)>>>((IcpState*)state, pose);
+    icp_begin_kernel<<<1, 64, 0, stream_of(s)>>>((IcpState*)state, pose);
     NUIF_LAUNCHED("icp_begin_kernel");
     return NUIF_OK;
 }
@@ -661,20 +753,19 @@ int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* ca
     hipStream_t st = stream_of(s);
     if (reduction == NUIF_ICP_REDUCE_REFERENCE) {
         const int groups = (int)div_up(npx, 128);
-        icp_ref_partials_kernel<<<groups, 128, 0, st>>>((const IcpState*)state, vmap_level, npx, *cam, ref_vmap,
-                                                        ref_nmap, prev_pose, dist_threshold, level, partials);
-        NUIF_LAUNCHED("icp_ref_partials_kernel");
-        icp_solve_kernel<kRefOrder><<<1, kBlock, 0, st>>>((IcpState*)state, partials, groups, level, iter,
-                                                           count_threshold);
+        NUIF_CHECK_ARG(div_up(groups, 128) <= kMaxGroups2,
+                       "nuif_icp_iteration: %d pixels per level exceed the reference-order limit %d", npx,
+                       kMaxGroups2 * 128 * 128);
+        icp_ref_kernel<<<groups, 128, 0, st>>>((IcpState*)state, vmap_level, npx, *cam, ref_vmap, ref_nmap,
+                                               prev_pose, dist_threshold, level, iter, count_threshold, partials);
+        NUIF_LAUNCHED("icp_ref_kernel");
     } else {
         const int nblocks = std::min<int>(div_up(npx, kBlock), kMaxBlocks);
-        icp_partials_kernel<<<nblocks, kBlock, 0, st>>>((const IcpState*)state, vmap_level, npx, *cam, ref_vmap,
-                                                         ref_nmap, prev_pose, dist_threshold, level, (double*)partials);
-        NUIF_LAUNCHED("icp_partials_kernel");
-        icp_solve_kernel<kCompensated><<<1, kBlock, 0, st>>>((IcpState*)state, partials, nblocks, level, iter,
-                                                              count_threshold);
+        icp_comp_kernel<<<nblocks, kBlock, 0, st>>>((IcpState*)state, vmap_level, npx, *cam, ref_vmap, ref_nmap,
+                                                    prev_pose, dist_threshold, level, iter, count_threshold,
+                                                    (double*)partials);
+        NUIF_LAUNCHED("icp_comp_kernel");
     }
-    NUIF_LAUNCHED("icp_solve_kernel");
     return NUIF_OK;
 }
 
@@ -692,7 +783,7 @@ int nuif_icp_count_thresholds(int W, int H, const nuif_icp_config* cfg, uint32_t
     NUIF_CHECK_ARG(cfg && thr && W > 0 && H > 0, "nuif_icp_count_thresholds: bad arguments");
     NUIF_CHECK_ARG(cfg->levels >= 1 && cfg->levels <= NUIF_ICP_MAX_LEVELS, "bad level count %d", cfg->levels);
     // OCL/NuiKinfuOpenCLDepthTracker.cpp:309: (kernelGlobalSize[0] >> 2l) * 0.1 where
-    // kernelGlobalSize[0] is left over from the previous level's compute_sums launch (Q10)
+    // kernelGlobalSize[0] is left over from the previous level's compute_sums launch
     size_t kgs = (size_t)W * H;
     for (int level = cfg->levels - 1; level >= 0; --level) {
         thr[level] = (uint32_t)((double)(kgs >> (2 * level)) * 0.1);
@@ -714,7 +805,7 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
     int rc = nuif_icp_count_thresholds((int)cam->depthImageWidth, (int)cam->depthImageHeight, cfg, thr);
     if (rc) return rc;
     const float L = (float)cfg->levels;
-    const float dist = cfg->dist_threshold + cfg->dist_threshold / L;  // :300-304 (Q11)
+    const float dist = cfg->dist_threshold + cfg->dist_threshold / L;  // :300-304
     rc = nuif_icp_begin(state, pose, s);
     if (rc) return rc;
     for (int level = cfg->levels - 1; level >= 0; --level) {

@@ -1,9 +1,9 @@
-// Dense marching cubes (row H16): K/marchingCube.cl:100-256 marchingCubeKernel
+// Dense marching cubes: K/marchingCube.cl:100-256 marchingCubeKernel
 // driven by OCL/NuiKinfuOpenCLScene.cpp:617-786 Volume2CLMesh.
 //
 // The reference runs one work-item per (x,y) column with a local size of 1x1
 // and appends triangles through a global atomic counter, so the vertex order
-// changes from run to run (quirk Q25).  Here the cubes are processed in
+// changes from run to run.  Here the cubes are processed in
 // 256-cube tiles (4 columns x 64 z, z fastest across the wavefront so that the
 // eight corner loads of a wavefront are contiguous 256-byte runs):
 //   pass 1  vertex count per tile (sum of the case table's counts);

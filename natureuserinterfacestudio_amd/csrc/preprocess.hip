@@ -1,4 +1,4 @@
-// Frame preprocessing kernels (rows H1-H7, H12): passing filter, bilateral
+// Frame preprocessing kernels: passing filter, bilateral
 // filter, depth pyramid, vertex and normal maps, map transform, depth scaling.
 //
 // Layout: images are row-major W*H float (depth) or packed float3 AoS
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) bilateral_kernel(const float* __restrict_
     if (x < W && y < H) out[(size_t)y * W + x] = bilateral_px(tile, tw, x, y, ox, oy, W, H, g, r, sigma_depth, thr);
 }
 
-// Fused H1 + H3 + H5(level 0) + H12: reads the raw depth once.
+// Fused passing filter + bilateral + level-0 vertices + ray-length depth: reads the raw depth once.
 __global__ void __launch_bounds__(256)
     preprocess_l0_kernel(const uint16_t* __restrict__ raw, float* __restrict__ depth_out,
                          float* __restrict__ depth_f, float* __restrict__ vmap, int W, int H, NuiCLCameraParams cam,

@@ -128,7 +128,10 @@ int nuif_malloc(void** dptr, size_t bytes) {
     if (bytes == 0) bytes = 256;
     void* p = nullptr;
     NUIF_HIP(hipMalloc(&p, bytes));
-    hipError_t e = hipMemset(p, 0, bytes);
+    // The fill runs on the null stream, which the library's non-blocking streams
+    // do not wait for: finish it before the memory is handed out.
+    hipError_t e = hipMemsetAsync(p, 0, bytes, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
         (void)hipFree(p);
         return hip_fail(e, "hipMemset");

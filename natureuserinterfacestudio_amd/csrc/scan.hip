@@ -1,8 +1,8 @@
-// Device-wide exclusive prefix sum and flag compaction (row H22).
+// Device-wide exclusive prefix sum and flag compaction.
 //
 // The reference scans uchar flags with a warp-synchronous LDS scan written for
 // 32-wide warps (K/prefixSum.cl:4-24, 163-232) and compacts with racy writes
-// (quirk Q2), then reads the count back to the host.  Here: a three-phase
+//, then reads the count back to the host.  Here: a three-phase
 // reduce / scan-of-partials / down-sweep over 1024-element tiles with 64-wide
 // wavefront scans, deterministic results, and the total left in device memory
 // so that later launches can size themselves without a host round trip.

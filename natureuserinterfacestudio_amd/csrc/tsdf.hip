@@ -1,4 +1,4 @@
-// Dense TSDF volume: reset (H14), integration (H13) and raycast (H15).
+// Dense TSDF volume: reset, integration and raycast.
 //
 // Volume layout is the reference's ((y*Nx + x)*Nz + z), z contiguous, so a
 // z-run of one (x,y) column is a contiguous byte range.
@@ -574,7 +574,7 @@ int nuif_tsdf_params(int nx, int ny, int nz, float dx, float dy, float dz, float
     out->cell_size[1] = dy / (float)ny;
     out->cell_size[2] = dz / (float)nz;
     const float c = std::max(out->cell_size[0], std::max(out->cell_size[1], out->cell_size[2]));
-    out->tranc_dist = std::max(trunc, 2.1f * c);  // OCL/NuiKinfuOpenCLScene.cpp:41 (Q16)
+    out->tranc_dist = std::max(trunc, 2.1f * c);  // OCL/NuiKinfuOpenCLScene.cpp:41
     return NUIF_OK;
 }
 

@@ -1,0 +1,148 @@
+#include "NuiKinfuSession.h"
+
+#include <stdexcept>
+
+using namespace NuiKinfuEngine;
+
+namespace nuif_kinfu {
+
+NuiTrackerConfig NuiKinfuSession::trackerConfig(const nuifk_desc& d) {
+    NuiTrackerConfig tc;
+    tc.filter_radius = (UINT)d.filter_radius;
+    tc.sigma_space2_inv_half = d.sigma_space2_inv_half;
+    tc.sigma_depth2_inv_half = d.sigma_depth2_inv_half;
+    tc.depth_threshold = d.depth_threshold;
+    tc.dist_threshold = d.dist_threshold;
+    tc.iterations.clear();
+    for (int l = 0; l < d.levels; ++l)
+        tc.iterations.push_back({(UINT)d.iterations[l], NuiTrackerConfig::eTracker_Iteration_Both});
+    return tc;
+}
+
+std::unique_ptr<NuiKinfuSession> NuiKinfuSession::create(const nuifk_desc& d) {
+    if (d.width <= 0 || d.height <= 0) throw std::runtime_error("session: empty image");
+    if (d.levels < 1 || d.levels > NUIF_ICP_MAX_LEVELS) throw std::runtime_error("session: 1..4 pyramid levels");
+    auto s = std::make_unique<NuiKinfuSession>();
+    NuiKinfuVolumeConfig vc;
+    vc.resolution = Vector3i(d.resolution[0], d.resolution[1], d.resolution[2]);
+    vc.dimensions = Vector3f(d.dimensions[0], d.dimensions[1], d.dimensions[2]);
+    vc.tranc_dist = d.tranc_dist;
+    vc.bHas_color_volume = d.color != 0;
+    s->engine = std::make_unique<NuiKinfuTrackingEngine>();
+    s->engine->setIntegrationMetricThreshold(d.integration_threshold);
+    s->engine->initialize(trackerConfig(d), false, (UINT)d.width, (UINT)d.height);
+    s->engine->setUseGraph(d.use_graph != 0);
+    s->engine->tracker()->SetReduction(d.icp_reduction);
+    s->scene = std::make_unique<NuiKinfuHIPScene>(vc, d.storage, s->engine->stream());
+    s->cam.m_intrinsics = NuiCameraIntrinsics(d.fx, d.fy, d.cx, d.cy);
+    s->cam.m_sensorDepthMin = d.depth_min;
+    s->cam.m_sensorDepthMax = d.depth_max;
+    s->W = (UINT)d.width;
+    s->H = (UINT)d.height;
+    return s;
+}
+
+size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
+    const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
+    const size_t vb = nuif_tsdf_voxel_bytes(d.storage);
+    size_t bytes = voxels * vb + (d.color ? voxels * 4 : 0) + 256;  // + update counter
+    const size_t n = (size_t)d.width * d.height;
+    bytes += n * (2 + 4 + 4);  // raw, passing-filter depth, colours
+    for (int l = 0; l < d.levels; ++l) bytes += (size_t)(d.width >> l) * (d.height >> l) * (4 + 12);
+    bytes += n * (12 + 12 + 4);  // feedback vertex / normal / colour maps
+    bytes += nuif_icp_state_bytes() + nuif_icp_partials_bytes(d.width, d.height);
+    bytes += (size_t)(1 << 16) * NUIF_POSE_ROW * sizeof(float) + 256;  // pose history, camera pose
+    return bytes;
+}
+
+NuiFusionServer::NuiFusionServer(size_t hbmBudget) : m_budget(hbmBudget) {
+    check(nuif_stream_create(&m_ctl), "server stream");
+    check(nuif_event_create(&m_start), "server event");
+}
+
+NuiFusionServer::~NuiFusionServer() {
+    if (m_ctl) nuif_stream_synchronize(m_ctl);
+    for (auto& s : m_sessions)
+        if (s) s->engine->synchronize();
+    m_sessions.clear();
+    for (nuif_event_t e : m_ends)
+        if (e) nuif_event_destroy(e);
+    if (m_start) nuif_event_destroy(m_start);
+    if (m_ctl) nuif_stream_destroy(m_ctl);
+}
+
+int NuiFusionServer::open(const nuifk_desc& d) {
+    const size_t need = NuiKinfuSession::deviceBytes(d);
+    if (m_inUse + need > m_budget) return NUIF_ERR_OOM;
+    auto s = NuiKinfuSession::create(d);
+    nuif_event_t e = nullptr;
+    check(nuif_event_create(&e), "session event");
+    int id = -1;
+    for (size_t i = 0; i < m_sessions.size(); ++i)
+        if (!m_sessions[i]) {
+            id = (int)i;
+            break;
+        }
+    if (id < 0) {
+        id = (int)m_sessions.size();
+        m_sessions.emplace_back();
+        m_bytes.push_back(0);
+        m_ends.push_back(nullptr);
+    }
+    m_sessions[id] = std::move(s);
+    m_bytes[id] = need;
+    m_ends[id] = e;
+    m_inUse += need;
+    return id;
+}
+
+void NuiFusionServer::close(int id) {
+    NuiKinfuSession* s = session(id);
+    s->engine->synchronize();
+    m_sessions[id].reset();
+    m_inUse -= m_bytes[id];
+    m_bytes[id] = 0;
+    nuif_event_destroy(m_ends[id]);
+    m_ends[id] = nullptr;
+}
+
+NuiKinfuSession* NuiFusionServer::session(int id) const {
+    if (id < 0 || id >= (int)m_sessions.size() || !m_sessions[id]) throw std::runtime_error("server: no such session");
+    return m_sessions[id].get();
+}
+
+int NuiFusionServer::sessionCount() const {
+    int n = 0;
+    for (auto& s : m_sessions) n += s ? 1 : 0;
+    return n;
+}
+
+void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depths) {
+    if (n < 0 || (n > 0 && (!ids || !depths))) throw std::runtime_error("server submit: bad arguments");
+    for (int i = 0; i < n; ++i) {
+        session(ids[i]);
+        if (!depths[i]) throw std::runtime_error("server submit: null frame");
+        for (int j = 0; j < i; ++j)
+            if (ids[j] == ids[i]) throw std::runtime_error("server submit: a session appears twice in a round");
+    }
+    m_round.assign(ids, ids + n);
+    check(nuif_event_record(m_start, m_ctl), "round start");
+    for (int i = 0; i < n; ++i) {
+        NuiKinfuSession* s = m_sessions[ids[i]].get();
+        const nuif_stream_t st = s->engine->stream();
+        check(nuif_stream_wait_event(st, m_start), "round wait");
+        s->engine->RunTrackingDevice(0, depths[i], nullptr, s->scene.get(), s->cam, false);
+        check(nuif_event_record(m_ends[ids[i]], st), "frame end");
+    }
+}
+
+int NuiFusionServer::wait(float* latencyMs) {
+    for (size_t i = 0; i < m_round.size(); ++i) {
+        const int id = m_round[i];
+        check(nuif_event_synchronize(m_ends[id]), "frame wait");
+        if (latencyMs) check(nuif_event_elapsed_ms(m_start, m_ends[id], &latencyMs[i]), "frame latency");
+    }
+    return 0;
+}
+
+}  // namespace nuif_kinfu

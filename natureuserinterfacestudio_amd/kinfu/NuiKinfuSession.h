@@ -1,0 +1,64 @@
+// One tracking session (engine + dense scene) and the multi-session fusion
+// server that keeps many of them resident on one GPU.
+//
+// The reference builds one engine and one scene per application
+// (src/SLAM/Backend/NuiSLAMController.cpp:33-69) and runs every OpenCL kernel
+// on one global in-order queue (CLU/NuiOpenCLGlobal.cpp:175).  Here a session
+// owns its stream, volume, pyramids, feedback maps and tracking state, and the
+// server admits sessions against a device-memory budget and enqueues one frame
+// per session per round on the sessions' own streams, so that the GPU overlaps
+// independent sessions.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "../../include/nuif_kinfu.h"
+#include "NuiKinfuHIP.h"
+
+namespace nuif_kinfu {
+
+struct NuiKinfuSession {
+    std::unique_ptr<NuiKinfuEngine::NuiKinfuTrackingEngine> engine;
+    std::unique_ptr<NuiKinfuHIPScene> scene;
+    NuiCameraParams cam;
+    UINT W = 0, H = 0;
+
+    static std::unique_ptr<NuiKinfuSession> create(const nuifk_desc& d);
+    // device bytes the session allocates (volume, colour volume, frame buffers,
+    // pyramids, feedback maps, ICP state and workspace, pose history)
+    static size_t deviceBytes(const nuifk_desc& d);
+    // converts the descriptor's tracker fields (VO/NuiTrackerConfig.cpp:5-24)
+    static NuiTrackerConfig trackerConfig(const nuifk_desc& d);
+};
+
+class NuiFusionServer {
+public:
+    explicit NuiFusionServer(size_t hbmBudget);
+    ~NuiFusionServer();
+    NuiFusionServer(const NuiFusionServer&) = delete;
+    NuiFusionServer& operator=(const NuiFusionServer&) = delete;
+
+    // admits a session; returns its id, or NUIF_ERR_OOM when it does not fit the budget
+    int open(const nuifk_desc& d);
+    void close(int id);
+    NuiKinfuSession* session(int id) const;
+    size_t bytesInUse() const { return m_inUse; }
+    int sessionCount() const;
+
+    // one round: a frame (device-resident uint16 depth) for each listed session
+    void submit(int n, const int* ids, const uint16_t* const* depths);
+    // waits for the round; latencyMs[i] = round start -> end of the i-th frame
+    int wait(float* latencyMs);
+
+private:
+    size_t m_budget;
+    size_t m_inUse = 0;
+    std::vector<std::unique_ptr<NuiKinfuSession>> m_sessions;  // nullptr: closed id
+    std::vector<size_t> m_bytes;
+    std::vector<nuif_event_t> m_ends;  // per id: end of its last submitted frame
+    std::vector<int> m_round;          // ids of the last round, in submission order
+    nuif_stream_t m_ctl = nullptr;     // carries the round start event
+    nuif_event_t m_start = nullptr;
+};
+
+}  // namespace nuif_kinfu

@@ -10,7 +10,8 @@
 #include <stdexcept>
 #include <string>
 
-#include "NuiKinfuHIP.h"
+#include "../../include/nuif_kinfu.h"
+#include "NuiKinfuSession.h"
 
 using namespace NuiKinfuEngine;
 using namespace nuif_kinfu;
@@ -18,12 +19,7 @@ using namespace nuif_kinfu;
 namespace {
 thread_local std::string t_err;
 
-struct Session {
-    std::unique_ptr<NuiKinfuTrackingEngine> engine;
-    std::unique_ptr<NuiKinfuHIPScene> scene;
-    NuiCameraParams cam;
-    UINT W = 0, H = 0;
-};
+using Session = NuiKinfuSession;
 
 template <class F> int guarded(F&& f) {
     try {
@@ -37,58 +33,12 @@ template <class F> int guarded(F&& f) {
 
 extern "C" {
 
-typedef struct nuifk_desc {
-    int width, height;
-    float fx, fy, cx, cy;
-    float depth_min, depth_max;     // metres
-    // tracker (NuiTrackerConfig)
-    int filter_radius;
-    float sigma_space2_inv_half, sigma_depth2_inv_half, depth_threshold, dist_threshold;
-    int levels;
-    int iterations[4];              // level 0 = finest
-    float integration_threshold;    // 0 = integrate every frame
-    // dense volume (NuiKinfuVolumeConfig)
-    int resolution[3];
-    float dimensions[3];
-    float tranc_dist;
-    int color;                      // colour volume on/off
-    int storage;                    // NUIF_TSDF_*
-    int use_graph;
-    int icp_reduction;              // NUIF_ICP_REDUCE_*
-} nuifk_desc;
-
 const char* nuifk_last_error(void) { return t_err.c_str(); }
 
 int nuifk_create(const nuifk_desc* d, void** out) {
     return guarded([&] {
         if (!d || !out) throw std::runtime_error("nuifk_create: null argument");
-        auto s = std::make_unique<Session>();
-        NuiTrackerConfig tc;
-        tc.filter_radius = (UINT)d->filter_radius;
-        tc.sigma_space2_inv_half = d->sigma_space2_inv_half;
-        tc.sigma_depth2_inv_half = d->sigma_depth2_inv_half;
-        tc.depth_threshold = d->depth_threshold;
-        tc.dist_threshold = d->dist_threshold;
-        tc.iterations.clear();
-        for (int l = 0; l < d->levels; ++l)
-            tc.iterations.push_back({(UINT)d->iterations[l], NuiTrackerConfig::eTracker_Iteration_Both});
-        NuiKinfuVolumeConfig vc;
-        vc.resolution = Vector3i(d->resolution[0], d->resolution[1], d->resolution[2]);
-        vc.dimensions = Vector3f(d->dimensions[0], d->dimensions[1], d->dimensions[2]);
-        vc.tranc_dist = d->tranc_dist;
-        vc.bHas_color_volume = d->color != 0;
-        s->engine = std::make_unique<NuiKinfuTrackingEngine>();
-        s->engine->setIntegrationMetricThreshold(d->integration_threshold);
-        s->engine->initialize(tc, false, (UINT)d->width, (UINT)d->height);
-        s->engine->setUseGraph(d->use_graph != 0);
-        s->engine->tracker()->SetReduction(d->icp_reduction);
-        s->scene = std::make_unique<NuiKinfuHIPScene>(vc, d->storage, s->engine->stream());
-        s->cam.m_intrinsics = NuiCameraIntrinsics(d->fx, d->fy, d->cx, d->cy);
-        s->cam.m_sensorDepthMin = d->depth_min;
-        s->cam.m_sensorDepthMax = d->depth_max;
-        s->W = (UINT)d->width;
-        s->H = (UINT)d->height;
-        *out = s.release();
+        *out = Session::create(*d).release();
         return 0;
     });
 }
@@ -204,15 +154,7 @@ int nuifk_load_tracker_config(const char* path, nuifk_desc* d) {
 int nuifk_save_tracker_config(const char* path, const nuifk_desc* d) {
     return guarded([&] {
         if (!path || !d) throw std::runtime_error("nuifk_save_tracker_config: null argument");
-        NuiTrackerConfig tc;
-        tc.filter_radius = (UINT)d->filter_radius;
-        tc.sigma_space2_inv_half = d->sigma_space2_inv_half;
-        tc.sigma_depth2_inv_half = d->sigma_depth2_inv_half;
-        tc.depth_threshold = d->depth_threshold;
-        tc.dist_threshold = d->dist_threshold;
-        tc.iterations.clear();
-        for (int l = 0; l < d->levels; ++l)
-            tc.iterations.push_back({(UINT)d->iterations[l], NuiTrackerConfig::eTracker_Iteration_Both});
+        NuiTrackerConfig tc = Session::trackerConfig(*d);
         std::remove(path);
         return tc.log(path) ? 0 : -1;
     });
@@ -243,5 +185,68 @@ void* nuifk_stream(void* h) { return h ? (void*)static_cast<Session*>(h)->engine
 void* nuifk_volume_ptr(void* h) { return h ? static_cast<Session*>(h)->scene->GetVolume() : nullptr; }
 size_t nuifk_volume_bytes(void* h) { return h ? static_cast<Session*>(h)->scene->VolumeBytes() : 0; }
 float nuifk_trunc(void* h) { return h ? static_cast<Session*>(h)->scene->GetParams().tranc_dist : 0.f; }
+
+// ---- multi-session server -------------------------------------------------------
+
+int nuifs_create(size_t hbm_budget_bytes, void** out) {
+    return guarded([&] {
+        if (!out) throw std::runtime_error("nuifs_create: null argument");
+        *out = new NuiFusionServer(hbm_budget_bytes);
+        return 0;
+    });
+}
+
+int nuifs_destroy(void* srv) {
+    return guarded([&] {
+        delete static_cast<NuiFusionServer*>(srv);
+        return 0;
+    });
+}
+
+size_t nuifs_session_bytes(const nuifk_desc* d) { return d ? Session::deviceBytes(*d) : 0; }
+
+int nuifs_open(void* srv, const nuifk_desc* d) {
+    return guarded([&] {
+        if (!srv || !d) throw std::runtime_error("nuifs_open: null argument");
+        const int id = static_cast<NuiFusionServer*>(srv)->open(*d);
+        if (id < 0) t_err = "nuifs_open: session does not fit the device-memory budget";
+        return id;
+    });
+}
+
+int nuifs_close(void* srv, int id) {
+    return guarded([&] {
+        static_cast<NuiFusionServer*>(srv)->close(id);
+        return 0;
+    });
+}
+
+size_t nuifs_bytes_in_use(void* srv) { return srv ? static_cast<NuiFusionServer*>(srv)->bytesInUse() : 0; }
+
+int nuifs_session_count(void* srv) { return srv ? static_cast<NuiFusionServer*>(srv)->sessionCount() : 0; }
+
+void* nuifs_engine(void* srv, int id) {
+    try {
+        return srv ? static_cast<NuiFusionServer*>(srv)->session(id) : nullptr;
+    } catch (const std::exception& e) {
+        t_err = e.what();
+        return nullptr;
+    }
+}
+
+int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths) {
+    return guarded([&] {
+        if (!srv) throw std::runtime_error("nuifs_submit: null server");
+        static_cast<NuiFusionServer*>(srv)->submit(n, ids, depths);
+        return 0;
+    });
+}
+
+int nuifs_wait(void* srv, float* latency_ms) {
+    return guarded([&] {
+        if (!srv) throw std::runtime_error("nuifs_wait: null server");
+        return static_cast<NuiFusionServer*>(srv)->wait(latency_ms);
+    });
+}
 
 }  // extern "C"

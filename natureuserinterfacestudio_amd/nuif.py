@@ -154,6 +154,16 @@ _KSIGS = {
     "nuifk_updated_voxels": (c_int, [c_void_p, C.POINTER(C.c_ulonglong)]),
     "nuifk_load_tracker_config": (c_int, [C.c_char_p, C.POINTER(KinfuDesc)]),
     "nuifk_save_tracker_config": (c_int, [C.c_char_p, C.POINTER(KinfuDesc)]),
+    "nuifs_create": (c_int, [c_size_t, C.POINTER(c_void_p)]),
+    "nuifs_destroy": (c_int, [c_void_p]),
+    "nuifs_session_bytes": (c_size_t, [C.POINTER(KinfuDesc)]),
+    "nuifs_open": (c_int, [c_void_p, C.POINTER(KinfuDesc)]),
+    "nuifs_close": (c_int, [c_void_p, c_int]),
+    "nuifs_bytes_in_use": (c_size_t, [c_void_p]),
+    "nuifs_session_count": (c_int, [c_void_p]),
+    "nuifs_engine": (c_void_p, [c_void_p, c_int]),
+    "nuifs_submit": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p)]),
+    "nuifs_wait": (c_int, [c_void_p, C.POINTER(c_float)]),
 }
 
 _bound = {}
@@ -186,6 +196,11 @@ def K():
 def declared_symbols():
     """Names of every function declared in include/nuif.h that this binding knows."""
     return sorted(_SIGS)
+
+
+def declared_kinfu_symbols():
+    """Names of every function declared in include/nuif_kinfu.h that this binding knows."""
+    return sorted(_KSIGS)
 
 
 class NuifError(RuntimeError):
@@ -345,14 +360,18 @@ class Event:
 class Engine:
     """One tracking session: NuiKinfuTrackingEngine + dense NuiKinfuHIPScene."""
 
-    def __init__(self, desc):
+    def __init__(self, desc, handle=None):
         init()
         K()
+        self.desc = desc
+        self._owned = handle is None
+        if handle is not None:  # a session owned by a Server
+            self._h = c_void_p(handle)
+            return
         self._h = c_void_p()
         rc = K().nuifk_create(C.byref(desc), C.byref(self._h))
         if rc != 0:
             raise NuifError("nuifk_create: " + (K().nuifk_last_error() or b"").decode())
-        self.desc = desc
 
     def _check(self, rc, what):
         if rc < 0:
@@ -417,9 +436,9 @@ class Engine:
         return K().nuifk_trunc(self._h)
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             K().nuifk_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -438,3 +457,70 @@ def decode_volume(raw, storage):
         return v[:, 0].view(np.float16).astype(np.float32), v[:, 1].astype(np.int32)
     v = raw.view(np.float32).reshape(-1, 2)
     return v[:, 0].copy(), v[:, 1].astype(np.int32)
+
+
+class Server:
+    """Multi-session fusion server (nuifs_*): sessions resident on the device,
+    one frame per session per round, each session on its own stream."""
+
+    def __init__(self, budget_bytes):
+        init()
+        self._h = c_void_p()
+        self._check(K().nuifs_create(c_size_t(int(budget_bytes)), C.byref(self._h)), "nuifs_create")
+        self.sessions = {}
+
+    @staticmethod
+    def _check(rc, what):
+        if rc < 0:
+            raise NuifError(f"{what}: status {rc}: " + (K().nuifk_last_error() or b"").decode())
+        return rc
+
+    @staticmethod
+    def session_bytes(desc):
+        return K().nuifs_session_bytes(C.byref(desc))
+
+    def open(self, desc):
+        """Admit a session; returns its id, or None when it does not fit the budget."""
+        sid = K().nuifs_open(self._h, C.byref(desc))
+        if sid == -3:
+            return None
+        self._check(sid, "nuifs_open")
+        self.sessions[sid] = Engine(desc, handle=K().nuifs_engine(self._h, sid))
+        return sid
+
+    def close_session(self, sid):
+        self.sessions.pop(sid)._h = None
+        self._check(K().nuifs_close(self._h, sid), "nuifs_close")
+
+    @property
+    def bytes_in_use(self):
+        return K().nuifs_bytes_in_use(self._h)
+
+    def submit(self, ids, frames):
+        """ids: session ids; frames: DeviceArray uint16 depth per id."""
+        n = len(ids)
+        self._ids = (c_int * n)(*ids)
+        self._ptrs = (c_void_p * n)(*[f.ptr for f in frames])
+        self._n = n
+        self._check(K().nuifs_submit(self._h, n, self._ids, self._ptrs), "nuifs_submit")
+
+    def wait(self):
+        """Waits for the round; returns per-frame latency (ms, round start -> frame end)."""
+        lat = (c_float * max(self._n, 1))()
+        self._check(K().nuifs_wait(self._h, lat), "nuifs_wait")
+        return np.array(lat[:self._n], np.float64)
+
+    def close(self):
+        if self._h:
+            for e in self.sessions.values():
+                e._h = None
+            self.sessions = {}
+            K().nuifs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -63,6 +63,15 @@ def lib():
             "or_mc_emit_s16": (cll, [i16p, vp, ci, cf, f32p, vp, cll]),
             "or_mc_mesh_dedup_s16": (cll, [i16p, ci, cf, f32p, cll, i32p, cll, C.POINTER(cll)]),
             "or_mc_table": (None, [np.ctypeslib.ndpointer(np.int8), np.ctypeslib.ndpointer(np.uint8)]),
+            "or_hash_touched": (ci, [f32p, ci, ci, f32p, f32p, f32p, cf, cf, vp, cll]),
+            "or_hash_index": (C.c_uint, [ci, ci, ci, C.c_uint]),
+            "or_hash_engine_create": (vp, [ci, ci, f32p, ci, cf, cf, cf, cf, ci, i32p, cf, cf, cf, cf, ci, ci]),
+            "or_hash_engine_destroy": (None, [vp]),
+            "or_hash_engine_run": (ci, [vp, u16p, vp, f64p, f64p]),
+            "or_hash_engine_blocks": (ci, [vp, vp, vp, vp, vp, cll]),
+            "or_hash_engine_visible": (ci, [vp, vp, cll]),
+            "or_hash_engine_refmaps": (None, [vp, vp, vp, vp]),
+            "or_hash_engine_mesh": (cll, [vp, i32p, i32p, vp, cll]),
             "or_expf_det": (cf, [cf]),
             "or_sincosf_det": (None, [cf, C.POINTER(cf), C.POINTER(cf)]),
         }
@@ -124,3 +133,85 @@ class OracleEngine:
             self.close()
         except Exception:
             pass
+
+
+def mc_tables():
+    """(edge_table int32[256], tri_table int32[256*16]) of the standard marching-cubes
+    tables; the edge mask of a cube is the set of edges its triangles use."""
+    tri = np.empty(256 * 16, np.int8)
+    nv = np.empty(256, np.uint8)
+    lib().or_mc_table(tri, nv)
+    tri = tri.astype(np.int32)
+    edge = np.zeros(256, np.int32)
+    for c in range(256):
+        for e in tri[c * 16:(c + 1) * 16]:
+            if e < 0:
+                break
+            edge[c] |= 1 << int(e)
+    return edge, tri
+
+
+class OracleHashEngine:
+    """RunTracking of the reference over a voxel-hashing scene, float arithmetic,
+    deterministic allocation (every touched block in the frame that touches it)."""
+
+    def __init__(self, wl, gate=0.0, voxel=0.005, trunc_scale=0.02, truncation=0.06, wmax=255, capacity=1 << 18,
+                 iterations=None):
+        its = list(iterations or wl.iterations)
+        self.wl = wl
+        self.h = lib().or_hash_engine_create(wl.width, wl.height, wl.cam6(), 3, 0.08, 50.0, 0.3, 0.1, len(its),
+                                             np.array(its + [0] * 8, np.int32)[:8], gate, voxel, trunc_scale,
+                                             truncation, wmax, capacity)
+
+    def run(self, depth, bgra=None):
+        pose = np.zeros(12, np.float64)
+        info = np.zeros(3, np.float64)
+        ok = lib().or_hash_engine_run(self.h, np.ascontiguousarray(depth, np.uint16),
+                                      None if bgra is None else np.ascontiguousarray(bgra).ctypes.data, pose, info)
+        return ok, pose, info
+
+    def blocks(self, voxels=True):
+        """(keys int16[n,3] sorted by (x,y,z), sdf int16[n,512], weight uint8[n,512], rgb uint8[n,512,3])."""
+        n = lib().or_hash_engine_blocks(self.h, None, None, None, None, 0)
+        keys = np.empty((n, 3), np.int16)
+        if not voxels:
+            lib().or_hash_engine_blocks(self.h, keys.ctypes.data, None, None, None, n)
+            return keys
+        sdf = np.empty((n, 512), np.int16)
+        w = np.empty((n, 512), np.uint8)
+        rgb = np.empty((n, 512, 3), np.uint8)
+        lib().or_hash_engine_blocks(self.h, keys.ctypes.data, sdf.ctypes.data, w.ctypes.data, rgb.ctypes.data, n)
+        return keys, sdf, w, rgb
+
+    def visible(self):
+        n = lib().or_hash_engine_visible(self.h, None, 0)
+        keys = np.empty((n, 3), np.int16)
+        lib().or_hash_engine_visible(self.h, keys.ctypes.data, n)
+        return keys
+
+    def refmaps(self):
+        n = self.wl.width * self.wl.height
+        v = np.empty((n, 3), np.float32)
+        nm = np.empty((n, 3), np.float32)
+        rg = np.empty((n, 2), np.float32)
+        lib().or_hash_engine_refmaps(self.h, v.ctypes.data, nm.ctypes.data, rg.ctypes.data)
+        return v, nm, rg
+
+    def mesh(self):
+        edge, tri = mc_tables()
+        n = lib().or_hash_engine_mesh(self.h, edge, tri, None, 0)
+        out = np.empty((max(n, 1), 3), np.float32)
+        lib().or_hash_engine_mesh(self.h, edge, tri, out.ctypes.data, n)
+        return out[:n]
+
+    def close(self):
+        if self.h:
+            lib().or_hash_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

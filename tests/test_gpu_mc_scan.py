@@ -3,7 +3,7 @@
 Scan values, counts and compacted ids are integers: exact.  Marching cubes:
 the triangle count is exact; the vertices are the oracle's vertices bit for
 bit as a multiset (the order is the tile order here and nondeterministic in
-the reference, quirk Q25)."""
+the reference)."""
 import ctypes as C
 import sys
 

@@ -30,9 +30,11 @@ def test_every_declared_symbol_is_exported():
 
 def test_engine_library_exports():
     lib = libkinfu()
-    for n in ["nuifk_create", "nuifk_destroy", "nuifk_run", "nuifk_synchronize", "nuifk_poses", "nuifk_state",
-              "nuifk_read_volume", "nuifk_read_maps", "nuifk_last_error"]:
-        assert hasattr(lib, n), n
+    names = declared("nuif_kinfu.h")
+    assert len(names) >= 29
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(nuif.declared_kinfu_symbols()), sorted(set(names) - set(nuif.declared_kinfu_symbols()))
 
 
 def test_python_binding_covers_header():

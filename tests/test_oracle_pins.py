@@ -20,7 +20,7 @@ def lib():
     return O.lib()
 
 
-# ---------------------------------------------------------------- H1 passing filter
+# ---------------------------------------------------------------- passing filter
 
 def test_passing_filter_strict_range():
     raw = np.array([0, 500, 501, 1234, 4499, 4500, 4501, 65535], np.uint16)
@@ -32,7 +32,7 @@ def test_passing_filter_strict_range():
     np.testing.assert_allclose(out[valid], raw[valid] / 1000.0, rtol=2e-7)
 
 
-# ---------------------------------------------------------------- H2/H3 bilateral
+# ---------------------------------------------------------------- bilateral
 
 def gauss(r=3, s=0.08):
     g = np.empty(2 * r + 1, np.float32)
@@ -98,7 +98,7 @@ def test_bilateral_brute_force_double():
     np.testing.assert_allclose(out, ref, rtol=1e-13, equal_nan=True)
 
 
-# ---------------------------------------------------------------- H4 pyramid
+# ---------------------------------------------------------------- pyramid
 
 def half(src, thr=0.3):
     Hs, Ws = src.shape
@@ -131,7 +131,7 @@ def test_half_sample_closed_forms():
     assert np.isnan(half(np.full((4, 4), np.nan, np.float32))).all()
 
 
-# ---------------------------------------------------------------- H5 / H6 / H7 / H12
+# ---------------------------------------------------------------- vertices, normals, transforms, ray length
 
 def test_depth2vertex_intrinsics_per_level():
     W_, H_ = 320, 240
@@ -184,7 +184,7 @@ def test_scale_depths_closed_form():
     np.testing.assert_allclose(d, 2.0 * np.sqrt(1 + xn ** 2 + yn ** 2), rtol=1e-6)
 
 
-# ---------------------------------------------------------------- H8-H11 ICP
+# ---------------------------------------------------------------- ICP
 
 def _plane_maps(W_=64, H_=48):
     cam = np.array([52.5, 52.5, 31.5, 23.5, 0.5, 4.5], np.float32)
@@ -201,7 +201,7 @@ def test_icp_sums_identity_closed_form():
     every vertex projects exactly onto its own pixel, so b = n.(q - p) = 0, r^2 = 0 and
     A = sum J^T J with J = [n x p, n], n = (0,0,1).  Pixels of the last row are
     rejected (the bilinear lookup reads row y+1), pixels of the last column are kept
-    (the x+1 read wraps into the next row, quirk Q14) except the one whose x+1, y+1
+    (the x+1 read wraps into the next row) except the one whose x+1, y+1
     corner lies past the end of the buffer."""
     W_, H_ = 64, 48
     cam = np.array([64.0, 64.0, 32.0, 24.0, 0.5, 4.5], np.float32)
@@ -300,7 +300,7 @@ def test_icp_recovers_known_motion():
 
 
 def test_icp_count_threshold_rule():
-    # K/icp thresholds of the stale-global-size rule (quirk Q10) for 640x480, 3 levels
+    # K/icp thresholds of the stale-global-size rule for 640x480, 3 levels
     import ctypes as C
     from natureuserinterfacestudio_amd import nuif
     cfg = nuif.IcpConfig()
@@ -312,7 +312,7 @@ def test_icp_count_threshold_rule():
     assert list(thr)[:3] == [64, 6, 1920]
 
 
-# ---------------------------------------------------------------- H12-H15 dense TSDF
+# ---------------------------------------------------------------- dense TSDF
 
 def _plane_volume(N=64, dim=2.0, trunc=0.05, D=1.2, frames=1):
     """Integrates a fronto-parallel plane at depth D seen from the identity pose."""
@@ -332,7 +332,7 @@ def _plane_volume(N=64, dim=2.0, trunc=0.05, D=1.2, frames=1):
 def test_integrate_plane_closed_form():
     N, dim, trunc, D = 64, 2.0, 0.05, 1.2
     cam, depth, vol, nupd, (W_, H_) = _plane_volume(N, dim, trunc, D)
-    trunc = lib().or_effective_trunc(N, dim, trunc)  # 2.1 cells here (quirk Q16)
+    trunc = lib().or_effective_trunc(N, dim, trunc)  # 2.1 cells here
     c = dim / N
     y, x, z = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
     X = (x - N // 2 + 0.5) * c
@@ -385,6 +385,6 @@ def test_raycast_plane_depth_and_normal():
 
 
 def test_effective_truncation_clamp():
-    # OCL/NuiKinfuOpenCLScene.cpp:41 (quirk Q16): 128^3 over 2 m -> 2.1 cells = 32.8 mm
+    # OCL/NuiKinfuOpenCLScene.cpp:41: 128^3 over 2 m -> 2.1 cells = 32.8 mm
     assert np.isclose(lib().or_effective_trunc(128, 2.0, 0.03), 2.1 * 2.0 / 128, rtol=1e-6)
     assert np.isclose(lib().or_effective_trunc(512, 3.0, 0.025), 0.025)
