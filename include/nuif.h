@@ -259,6 +259,59 @@ int nuif_mc_dense(const void* volume, const uint8_t* color_volume, const TsdfPar
                   float* vertices, float* colors, uint32_t capacity, uint32_t* total, void* scratch,
                   nuif_stream_t s);
 
+/* ---- voxel hashing (OCL/NuiKinfuOpenCLHashScene.cpp, K/KinfuHashScene.cl,
+ * K/KinfuHashRaycast.cl, K/KinfuHashVisualization.cl) ----------------------------
+ * One device allocation holds the whole scene: header, hash table (2^b buckets of
+ * 4 entries, a full bucket overflowing into the next ones), per-entry flags, the
+ * visible list and the brick pool (8^3 voxels per brick, voxel index x + 8y + 64z).
+ * Replaces NuiKinfuVoxelBlockHash (OCL/NuiKinfuVoxelBlockHash.cpp:24-213). */
+typedef struct nuif_hash_params {
+    float voxel_size;   /* m_virtualVoxelSize, metres (VO/NuiHashingSDFConfig.cpp:3-15) */
+    float trunc_scale;  /* m_truncScale: band half-width and sdf normalisation, metres */
+    float truncation;   /* m_truncation: raycast step scale, metres */
+    int weight_max;     /* m_integrationWeightMax, 1..255 */
+    int buckets_log2;   /* 2^b buckets of 4 entries, b in 4..24 */
+    int bricks;         /* brick pool capacity */
+    int storage;        /* NUIF_TSDF_S16: the reference's short sdf*32767 + weight;
+                           NUIF_TSDF_F16: FP16 sdf + uint16 weight */
+    int color;          /* colour per voxel (uchar4) */
+} nuif_hash_params;
+
+size_t nuif_hash_scene_bytes(const nuif_hash_params* p);
+/* Byte offsets inside the scene: header, entries (16 B: uint64 key, int32 brick, pad),
+ * flags, visible list (int32 entry ids), bricks (uint32 per voxel), colours (0 if none),
+ * then the entry count and the total size. */
+int nuif_hash_layout(const nuif_hash_params* p, size_t* offsets8);
+/* Empty table, empty pool (sdf 0, weight 0, colour 0): K/KinfuHashScene.cl:5-40 reset kernels. */
+int nuif_hash_reset(void* scene, const nuif_hash_params* p, nuif_stream_t s);
+/* OCL/NuiKinfuOpenCLHashScene.cpp:404-424 AllocateSceneFromDepth: every block that a pixel's
+ * truncation band touches is inserted (K/KinfuHashScene.cl:52-194); the visible list becomes
+ * the touched blocks plus the previously visible blocks with a corner inside the image
+ * (:197-285), compacted by the device-wide scan.  depth: passing-filter depth (metres, NaN
+ * invalid); gate: device flag, 0 skips the frame (NULL: always). */
+int nuif_hash_allocate(void* scene, const nuif_hash_params* p, const float* depth, const NuiCLCameraParams* cam,
+                       const NuiCLRigidTransform* pose, const int32_t* gate, nuif_stream_t s);
+/* K/KinfuHashScene.cl:387-470 integrateIntoScene_kernel over the visible list; bgra: W*H*4
+ * or NULL; updated: device counter of voxels written (NULL: none). */
+int nuif_hash_integrate(void* scene, const nuif_hash_params* p, const float* depth, const uint8_t* bgra,
+                        const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, const int32_t* gate,
+                        unsigned long long* updated, nuif_stream_t s);
+/* Ray ranges of the visible blocks (K/KinfuHashRaycast.cl:4-76, exact min / max) into
+ * range2 (W*H float2), then castRay (:155-448): world vertex map, normal map (may be NULL)
+ * and colour map (W*H uchar4, may be NULL).  enable: device flag, 0 skips (NULL: always). */
+int nuif_hash_raycast(void* scene, const nuif_hash_params* p, const NuiCLCameraParams* cam,
+                      const NuiCLRigidTransform* pose, float* range2, float* vmap, float* nmap, uint8_t* cmap,
+                      const int32_t* enable, nuif_stream_t s);
+size_t nuif_hash_mesh_scratch_bytes(const nuif_hash_params* p);
+/* K/KinfuHashVisualization.cl:88-271 meshHashScene_kernel over the visible list: count,
+ * scan, emit; vertices float3 in metres (3 per triangle), colours float4 (may be NULL);
+ * *total (device uint32) receives the vertex count before capping at `capacity`. */
+int nuif_hash_mesh(void* scene, const nuif_hash_params* p, float* vertices, float* colors, uint32_t capacity,
+                   uint32_t* total, void* scratch, nuif_stream_t s);
+/* Host copy of [bricks handed out, visible-list length, failed insertions, occupied
+ * entries]; synchronises s. */
+int nuif_hash_stats(const void* scene, uint32_t* out4, nuif_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,20 @@ class TsdfParams(C.Structure):
                 ("tranc_dist", c_float)]
 
 
+class HashParams(C.Structure):
+    _fields_ = [("voxel_size", c_float), ("trunc_scale", c_float), ("truncation", c_float), ("weight_max", c_int),
+                ("buckets_log2", c_int), ("bricks", c_int), ("storage", c_int), ("color", c_int)]
+
+
+def hash_params(voxel=0.005, trunc_scale=0.02, truncation=0.06, weight_max=255, buckets_log2=20, bricks=1 << 18,
+                storage=0, color=False):
+    """nuif_hash_params with the reference's defaults (VO/NuiHashingSDFConfig.cpp:3-15)."""
+    p = HashParams()
+    p.voxel_size, p.trunc_scale, p.truncation, p.weight_max = voxel, trunc_scale, truncation, weight_max
+    p.buckets_log2, p.bricks, p.storage, p.color = buckets_log2, bricks, storage, 1 if color else 0
+    return p
+
+
 ICP_TERMS = 29
 ICP_MAX_LEVELS = 4
 POSE_ROW = 16
@@ -132,6 +146,18 @@ _SIGS = {
     "nuif_mc_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
     "nuif_mc_dense": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p, C.c_uint32,
                               c_void_p, c_void_p, c_void_p]),
+    "nuif_hash_scene_bytes": (c_size_t, [C.POINTER(HashParams)]),
+    "nuif_hash_layout": (c_int, [C.POINTER(HashParams), C.POINTER(c_size_t)]),
+    "nuif_hash_reset": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p]),
+    "nuif_hash_allocate": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p,
+                                   c_void_p]),
+    "nuif_hash_integrate": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "nuif_hash_raycast": (c_int, [c_void_p, C.POINTER(HashParams), C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nuif_hash_mesh_scratch_bytes": (c_size_t, [C.POINTER(HashParams)]),
+    "nuif_hash_mesh": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, c_void_p, C.c_uint32, c_void_p, c_void_p, c_void_p]),
+    "nuif_hash_stats": (c_int, [c_void_p, C.POINTER(C.c_uint32), c_void_p]),
 }
 
 _KSIGS = {
@@ -524,3 +550,103 @@ class Server:
         except Exception:
             pass
 
+
+
+class HashScene:
+    """A voxel-hashing scene in device memory (nuif_hash_*), driven with explicit poses."""
+
+    ENTRY = np.dtype([("key", "<u8"), ("ptr", "<i4"), ("pad", "<u4")])
+
+    def __init__(self, params, stream=None):
+        init()
+        self.p = params
+        self.stream = stream
+        nb = L().nuif_hash_scene_bytes(C.byref(params))
+        if nb == 0:
+            raise NuifError("nuif_hash_scene_bytes: bad parameters: " + L().nuif_last_error().decode())
+        self.mem = DeviceArray((nb,), np.uint8)
+        off = (c_size_t * 8)()
+        check(L().nuif_hash_layout(C.byref(params), off), "nuif_hash_layout")
+        self.offsets = list(off)
+        check(L().nuif_hash_reset(self.mem.ptr, C.byref(params), stream), "nuif_hash_reset")
+
+    def allocate(self, depth, cam, pose, gate=None):
+        check(L().nuif_hash_allocate(self.mem.ptr, C.byref(self.p), depth.ptr, C.byref(cam), pose.ptr,
+                                     gate, self.stream), "nuif_hash_allocate")
+
+    def integrate(self, depth, cam, pose, bgra=None, gate=None, updated=None):
+        check(L().nuif_hash_integrate(self.mem.ptr, C.byref(self.p), depth.ptr, bgra.ptr if bgra else None,
+                                      C.byref(cam), pose.ptr, gate, updated, self.stream), "nuif_hash_integrate")
+
+    def raycast(self, cam, pose, color=False):
+        n = cam.depthImageWidth * cam.depthImageHeight
+        rg = DeviceArray((n, 2), np.float32)
+        v = DeviceArray((n, 3), np.float32)
+        nm = DeviceArray((n, 3), np.float32)
+        cm = DeviceArray((n, 4), np.uint8) if color else None
+        check(L().nuif_hash_raycast(self.mem.ptr, C.byref(self.p), C.byref(cam), pose.ptr, rg.ptr, v.ptr, nm.ptr,
+                                    cm.ptr if cm else None, None, self.stream), "nuif_hash_raycast")
+        return v.download(), nm.download(), rg.download(), cm.download() if cm else None
+
+    def mesh(self, capacity=None, colors=False):
+        scratch = DeviceArray((L().nuif_hash_mesh_scratch_bytes(C.byref(self.p)),), np.uint8)
+        tot = DeviceArray((1,), np.uint32)
+        check(L().nuif_hash_mesh(self.mem.ptr, C.byref(self.p), None, None, 0, tot.ptr, scratch.ptr, self.stream),
+              "nuif_hash_mesh")
+        n = int(tot.download()[0]) if capacity is None else capacity
+        v = DeviceArray((max(n, 1), 3), np.float32)
+        c = DeviceArray((max(n, 1), 4), np.float32) if colors else None
+        check(L().nuif_hash_mesh(self.mem.ptr, C.byref(self.p), v.ptr, c.ptr if c else None, n, tot.ptr,
+                                 scratch.ptr, self.stream), "nuif_hash_mesh")
+        total = int(tot.download()[0])
+        return v.download()[:min(n, total)], (c.download()[:min(n, total)] if c else None), total
+
+    def stats(self):
+        out = (C.c_uint32 * 4)()
+        check(L().nuif_hash_stats(self.mem.ptr, out, self.stream), "nuif_hash_stats")
+        return {"bricks": out[0], "visible": out[1], "failed": out[2], "entries": out[3]}
+
+    def _region(self, start, count, dtype):
+        raw = self.mem.download()
+        return raw[start:start + count * np.dtype(dtype).itemsize].view(dtype)
+
+    def blocks(self):
+        """(keys int16[n,3] sorted by (x,y,z), sdf float32[n,512] (decoded), weight[n,512], raw words[n,512],
+        rgb uint8[n,512,3] or None) of every entry that holds a brick."""
+        raw = self.mem.download()
+        ne = self.offsets[6]
+        ent = raw[self.offsets[1]:self.offsets[1] + ne * 16].view(self.ENTRY)
+        sel = np.nonzero((ent["key"] != 0) & (ent["ptr"] >= 0))[0]
+        k = ent["key"][sel]
+        keys = np.stack([((k >> 32) & 0xFFFF).astype(np.uint16).view(np.int16),
+                         ((k >> 16) & 0xFFFF).astype(np.uint16).view(np.int16),
+                         (k & 0xFFFF).astype(np.uint16).view(np.int16)], 1)
+        order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+        keys = keys[order]
+        ptr = ent["ptr"][sel][order]
+        nbr = self.p.bricks
+        words = raw[self.offsets[4]:self.offsets[4] + nbr * 512 * 4].view(np.uint32).reshape(nbr, 512)[ptr]
+        lo = (words & 0xFFFF).astype(np.uint16)
+        if self.p.storage == TSDF_S16:
+            sdf = lo.view(np.int16).astype(np.float32) / np.float32(32767.0)
+        else:
+            sdf = lo.view(np.float16).astype(np.float32)
+        weight = (words >> 16).astype(np.int32)
+        rgb = None
+        if self.p.color:
+            col = raw[self.offsets[5]:self.offsets[5] + nbr * 512 * 4].reshape(nbr, 512, 4)[ptr]
+            rgb = col[:, :, :3].copy()
+        return keys, sdf, weight, words, rgb
+
+    def visible_keys(self):
+        """Keys of the visible list (int16[n,3], sorted by (x,y,z))."""
+        raw = self.mem.download()
+        nvis = int(raw[4:8].view(np.uint32)[0])
+        ids = raw[self.offsets[3]:self.offsets[3] + nvis * 4].view(np.int32)
+        ne = self.offsets[6]
+        ent = raw[self.offsets[1]:self.offsets[1] + ne * 16].view(self.ENTRY)
+        k = ent["key"][ids]
+        keys = np.stack([((k >> 32) & 0xFFFF).astype(np.uint16).view(np.int16),
+                         ((k >> 16) & 0xFFFF).astype(np.uint16).view(np.int16),
+                         (k & 0xFFFF).astype(np.uint16).view(np.int16)], 1)
+        return keys[np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))]

@@ -124,6 +124,10 @@ HASH = Workload("hash-640x480-5mm", 640, 480, 525.0, 525.0, 319.5, 239.5, frames
                 volume_res=0, volume_dim=0.0, trunc=0.02, iterations=[10, 5, 4], seed_index=2)
 
 
+HASH_SMALL = Workload("hash-320x240-5mm", 320, 240, 262.5, 262.5, 159.5, 119.5, frames=30, scene_kind=1,
+                      volume_res=0, volume_dim=0.0, trunc=0.02, iterations=[10, 5, 4], seed_index=2)
+
+
 def tiny(width=64, height=48, res=32, frames=4):
     """A small workload for quick checks (same scene family as SMALL, scaled intrinsics)."""
     s = width / 320.0

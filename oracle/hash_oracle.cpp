@@ -704,16 +704,54 @@ OR_EXPORT int or_hash_engine_run(void* hp, const uint16_t* raw, const uint8_t* b
     return ok;
 }
 
+// Scene of a hash engine (for the scene-level calls below).
+OR_EXPORT void* or_hash_engine_scene(void* hp) { return &((HashEngine*)hp)->scene; }
+
+// Reference maps of the next ICP (world vertices / normals).
+OR_EXPORT void or_hash_engine_refmaps(void* hp, float* vmap, float* nmap) {
+    HashEngine* e = (HashEngine*)hp;
+    if (vmap) std::memcpy(vmap, e->vref.data(), e->vref.size() * sizeof(float));
+    if (nmap) std::memcpy(nmap, e->nref.data(), e->nref.size() * sizeof(float));
+}
+
+// ---- scene-level calls (explicit poses) ----
+
+OR_EXPORT void* or_hash_scene_create(float vs, float trunc_scale, float truncation, int wmax, int capacity) {
+    return new HashScene(HashCfg{vs, trunc_scale, truncation, wmax, capacity});
+}
+OR_EXPORT void or_hash_scene_destroy(void* sp) { delete (HashScene*)sp; }
+
+// AllocateSceneFromDepth + IntegrateIntoScene for one frame; returns voxels updated.
+OR_EXPORT long long or_hash_scene_integrate(void* sp, const float* depth, const uint8_t* bgra, int W, int H,
+                                            const float* cam6, const float* R9, const float* t3) {
+    HashScene* s = (HashScene*)sp;
+    const Cam<float> cam = cam_of(cam6, W, H);
+    const Rigid<float> g = make_rigid<float>(R9, t3);
+    s->allocate_and_select(depth, cam, g);
+    return s->integrate(depth, bgra, cam, g);
+}
+
+// Ray ranges + castRay at a pose: vmap / nmap (W*H*3), range2 (W*H*2), cmap (W*H*4 or NULL).
+OR_EXPORT void or_hash_scene_raycast(void* sp, int W, int H, const float* cam6, const float* R9, const float* t3,
+                                     float* vmap, float* nmap, float* range2, uint8_t* cmap) {
+    HashScene* s = (HashScene*)sp;
+    const Cam<float> cam = cam_of(cam6, W, H);
+    const Rigid<float> g = make_rigid<float>(R9, t3);
+    s->ray_ranges(cam, g);
+    s->raycast(cam, g, vmap, nmap, cmap);
+    if (range2) std::memcpy(range2, s->range.data(), s->range.size() * sizeof(float));
+}
+
 // Allocated blocks sorted by key: keys (int16 x3), sdf (int16 x512), weight (uint8 x512)
 // and colour (uint8 x512x3) per block, each optional; returns the block count.
-OR_EXPORT int or_hash_engine_blocks(void* hp, int16_t* keys, int16_t* sdf, uint8_t* w, uint8_t* rgb, long long cap) {
-    HashEngine* e = (HashEngine*)hp;
+OR_EXPORT int or_hash_scene_blocks(void* sp, int16_t* keys, int16_t* sdf, uint8_t* w, uint8_t* rgb, long long cap) {
+    HashScene* s = (HashScene*)sp;
     std::vector<uint64_t> ks;
-    ks.reserve(e->scene.index.size());
-    for (auto& kv : e->scene.index) ks.push_back(kv.first);
+    ks.reserve(s->index.size());
+    for (auto& kv : s->index) ks.push_back(kv.first);
     std::sort(ks.begin(), ks.end(), key_less);
     for (size_t i = 0; i < ks.size() && (long long)i < cap; ++i) {
-        const Brick& b = e->scene.bricks[e->scene.index.at(ks[i])];
+        const Brick& b = s->bricks[s->index.at(ks[i])];
         if (sdf) std::memcpy(sdf + i * kSB3, b.sdf, sizeof(b.sdf));
         if (w) std::memcpy(w + i * kSB3, b.w, sizeof(b.w));
         if (rgb) std::memcpy(rgb + i * kSB3 * 3, b.rgb, sizeof(b.rgb));
@@ -722,20 +760,12 @@ OR_EXPORT int or_hash_engine_blocks(void* hp, int16_t* keys, int16_t* sdf, uint8
 }
 
 // Visible list of the last integration, sorted by key.
-OR_EXPORT int or_hash_engine_visible(void* hp, int16_t* keys, long long cap) {
-    return export_keys(((HashEngine*)hp)->scene.visible, keys, cap);
-}
-
-// Reference maps of the next ICP (world vertices / normals) and the last ray-range image.
-OR_EXPORT void or_hash_engine_refmaps(void* hp, float* vmap, float* nmap, float* range2) {
-    HashEngine* e = (HashEngine*)hp;
-    if (vmap) std::memcpy(vmap, e->vref.data(), e->vref.size() * sizeof(float));
-    if (nmap) std::memcpy(nmap, e->nref.data(), e->nref.size() * sizeof(float));
-    if (range2 && !e->scene.range.empty()) std::memcpy(range2, e->scene.range.data(), e->scene.range.size() * sizeof(float));
+OR_EXPORT int or_hash_scene_visible(void* sp, int16_t* keys, long long cap) {
+    return export_keys(((HashScene*)sp)->visible, keys, cap);
 }
 
 // Marching cubes over the visible list; returns the vertex count (3 per triangle).
-OR_EXPORT long long or_hash_engine_mesh(void* hp, const int* edge_table, const int* tri_table, float* verts,
-                                        long long cap) {
-    return (long long)((HashEngine*)hp)->scene.mesh(edge_table, tri_table, verts, (size_t)cap);
+OR_EXPORT long long or_hash_scene_mesh(void* sp, const int* edge_table, const int* tri_table, float* verts,
+                                       long long cap) {
+    return (long long)((HashScene*)sp)->mesh(edge_table, tri_table, verts, (size_t)cap);
 }

@@ -68,10 +68,15 @@ def lib():
             "or_hash_engine_create": (vp, [ci, ci, f32p, ci, cf, cf, cf, cf, ci, i32p, cf, cf, cf, cf, ci, ci]),
             "or_hash_engine_destroy": (None, [vp]),
             "or_hash_engine_run": (ci, [vp, u16p, vp, f64p, f64p]),
-            "or_hash_engine_blocks": (ci, [vp, vp, vp, vp, vp, cll]),
-            "or_hash_engine_visible": (ci, [vp, vp, cll]),
-            "or_hash_engine_refmaps": (None, [vp, vp, vp, vp]),
-            "or_hash_engine_mesh": (cll, [vp, i32p, i32p, vp, cll]),
+            "or_hash_engine_scene": (vp, [vp]),
+            "or_hash_engine_refmaps": (None, [vp, vp, vp]),
+            "or_hash_scene_create": (vp, [cf, cf, cf, ci, ci]),
+            "or_hash_scene_destroy": (None, [vp]),
+            "or_hash_scene_integrate": (cll, [vp, f32p, vp, ci, ci, f32p, f32p, f32p]),
+            "or_hash_scene_raycast": (None, [vp, ci, ci, f32p, f32p, f32p, vp, vp, vp, vp]),
+            "or_hash_scene_blocks": (ci, [vp, vp, vp, vp, vp, cll]),
+            "or_hash_scene_visible": (ci, [vp, vp, cll]),
+            "or_hash_scene_mesh": (cll, [vp, i32p, i32p, vp, cll]),
             "or_expf_det": (cf, [cf]),
             "or_sincosf_det": (None, [cf, C.POINTER(cf), C.POINTER(cf)]),
         }
@@ -151,7 +156,83 @@ def mc_tables():
     return edge, tri
 
 
-class OracleHashEngine:
+class _HashSceneView:
+    """Read-back of a hash scene held by the oracle (blocks, visible list, mesh)."""
+
+    def _scene(self):
+        raise NotImplementedError
+
+    def blocks(self, voxels=True):
+        """(keys int16[n,3] sorted by (x,y,z), sdf int16[n,512], weight uint8[n,512], rgb uint8[n,512,3])."""
+        sp = self._scene()
+        n = lib().or_hash_scene_blocks(sp, None, None, None, None, 0)
+        keys = np.empty((n, 3), np.int16)
+        if not voxels:
+            lib().or_hash_scene_blocks(sp, keys.ctypes.data, None, None, None, n)
+            return keys
+        sdf = np.empty((n, 512), np.int16)
+        w = np.empty((n, 512), np.uint8)
+        rgb = np.empty((n, 512, 3), np.uint8)
+        lib().or_hash_scene_blocks(sp, keys.ctypes.data, sdf.ctypes.data, w.ctypes.data, rgb.ctypes.data, n)
+        return keys, sdf, w, rgb
+
+    def visible(self):
+        sp = self._scene()
+        n = lib().or_hash_scene_visible(sp, None, 0)
+        keys = np.empty((n, 3), np.int16)
+        lib().or_hash_scene_visible(sp, keys.ctypes.data, n)
+        return keys
+
+    def mesh(self):
+        edge, tri = mc_tables()
+        sp = self._scene()
+        n = lib().or_hash_scene_mesh(sp, edge, tri, None, 0)
+        out = np.empty((max(n, 1), 3), np.float32)
+        lib().or_hash_scene_mesh(sp, edge, tri, out.ctypes.data, n)
+        return out[:n]
+
+
+class OracleHashScene(_HashSceneView):
+    """The reference's voxel-hashing scene on the CPU, driven with explicit poses."""
+
+    def __init__(self, voxel=0.005, trunc_scale=0.02, truncation=0.06, wmax=255, capacity=1 << 18):
+        self.h = lib().or_hash_scene_create(voxel, trunc_scale, truncation, wmax, capacity)
+
+    def _scene(self):
+        return self.h
+
+    def integrate(self, depth, wl, R, t, bgra=None):
+        return lib().or_hash_scene_integrate(self.h, np.ascontiguousarray(depth, np.float32),
+                                             None if bgra is None else np.ascontiguousarray(bgra).ctypes.data,
+                                             wl.width, wl.height, wl.cam6(),
+                                             np.ascontiguousarray(R, np.float32).reshape(-1),
+                                             np.ascontiguousarray(t, np.float32))
+
+    def raycast(self, wl, R, t, color=False):
+        n = wl.width * wl.height
+        v = np.empty((n, 3), np.float32)
+        nm = np.empty((n, 3), np.float32)
+        rg = np.empty((n, 2), np.float32)
+        cm = np.empty((n, 4), np.uint8) if color else None
+        lib().or_hash_scene_raycast(self.h, wl.width, wl.height, wl.cam6(),
+                                    np.ascontiguousarray(R, np.float32).reshape(-1),
+                                    np.ascontiguousarray(t, np.float32), v.ctypes.data, nm.ctypes.data,
+                                    rg.ctypes.data, None if cm is None else cm.ctypes.data)
+        return v, nm, rg, cm
+
+    def close(self):
+        if self.h:
+            lib().or_hash_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class OracleHashEngine(_HashSceneView):
     """RunTracking of the reference over a voxel-hashing scene, float arithmetic,
     deterministic allocation (every touched block in the frame that touches it)."""
 
@@ -163,6 +244,9 @@ class OracleHashEngine:
                                              np.array(its + [0] * 8, np.int32)[:8], gate, voxel, trunc_scale,
                                              truncation, wmax, capacity)
 
+    def _scene(self):
+        return lib().or_hash_engine_scene(self.h)
+
     def run(self, depth, bgra=None):
         pose = np.zeros(12, np.float64)
         info = np.zeros(3, np.float64)
@@ -170,39 +254,12 @@ class OracleHashEngine:
                                       None if bgra is None else np.ascontiguousarray(bgra).ctypes.data, pose, info)
         return ok, pose, info
 
-    def blocks(self, voxels=True):
-        """(keys int16[n,3] sorted by (x,y,z), sdf int16[n,512], weight uint8[n,512], rgb uint8[n,512,3])."""
-        n = lib().or_hash_engine_blocks(self.h, None, None, None, None, 0)
-        keys = np.empty((n, 3), np.int16)
-        if not voxels:
-            lib().or_hash_engine_blocks(self.h, keys.ctypes.data, None, None, None, n)
-            return keys
-        sdf = np.empty((n, 512), np.int16)
-        w = np.empty((n, 512), np.uint8)
-        rgb = np.empty((n, 512, 3), np.uint8)
-        lib().or_hash_engine_blocks(self.h, keys.ctypes.data, sdf.ctypes.data, w.ctypes.data, rgb.ctypes.data, n)
-        return keys, sdf, w, rgb
-
-    def visible(self):
-        n = lib().or_hash_engine_visible(self.h, None, 0)
-        keys = np.empty((n, 3), np.int16)
-        lib().or_hash_engine_visible(self.h, keys.ctypes.data, n)
-        return keys
-
     def refmaps(self):
         n = self.wl.width * self.wl.height
         v = np.empty((n, 3), np.float32)
         nm = np.empty((n, 3), np.float32)
-        rg = np.empty((n, 2), np.float32)
-        lib().or_hash_engine_refmaps(self.h, v.ctypes.data, nm.ctypes.data, rg.ctypes.data)
-        return v, nm, rg
-
-    def mesh(self):
-        edge, tri = mc_tables()
-        n = lib().or_hash_engine_mesh(self.h, edge, tri, None, 0)
-        out = np.empty((max(n, 1), 3), np.float32)
-        lib().or_hash_engine_mesh(self.h, edge, tri, out.ctypes.data, n)
-        return out[:n]
+        lib().or_hash_engine_refmaps(self.h, v.ctypes.data, nm.ctypes.data)
+        return v, nm
 
     def close(self):
         if self.h:
@@ -214,4 +271,3 @@ class OracleHashEngine:
             self.close()
         except Exception:
             pass
-
