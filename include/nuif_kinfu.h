@@ -45,11 +45,19 @@ typedef struct nuifk_desc {
     int storage;                    /* NUIF_TSDF_* */
     int use_graph;                  /* replay tracked frames from a captured hipGraph */
     int icp_reduction;              /* NUIF_ICP_REDUCE_* */
+    int scene;                      /* NUIFK_SCENE_DENSE or NUIFK_SCENE_HASH */
+    /* voxel hashing (VO/NuiHashingSDFConfig.cpp:3-15); storage S16 or F16, color as above */
+    float hash_voxel_size, hash_trunc_scale, hash_truncation;
+    int hash_weight_max;
+    int hash_bricks;                /* brick pool (m_numSDFBlocks) */
 } nuifk_desc;
+
+#define NUIFK_SCENE_DENSE 0         /* OCL/NuiKinfuOpenCLScene: dense TSDF volume */
+#define NUIFK_SCENE_HASH 1          /* OCL/NuiKinfuOpenCLHashScene: voxel hashing */
 
 const char* nuifk_last_error(void);
 
-/* ---- one session: NuiKinfuTrackingEngine + dense NuiKinfuScene ------------- */
+/* ---- one session: NuiKinfuTrackingEngine + its dense or hash NuiKinfuScene ---- */
 int nuifk_create(const nuifk_desc* d, void** out);
 int nuifk_destroy(void* h);
 /* One frame (VO/NuiKinfuTrackingEngine.h:36-42 RunTracking).  depth: W*H uint16 mm,
@@ -61,6 +69,8 @@ int nuifk_reset(void* h);
 /* Pose history rows (NUIF_POSE_ROW floats) of the first n frames; returns the count. */
 int nuifk_poses(void* h, float* out, int n);
 int nuifk_state(void* h, nuif_icp_result* out);
+/* Copies the scene's device allocation: the dense volume, or the whole hash scene
+ * (layout of nuif_hash_layout); nuifk_volume_bytes gives its size. */
 int nuifk_read_volume(void* h, void* out, size_t bytes);
 int nuifk_read_maps(void* h, float* vmap, float* nmap);
 /* VO/NuiTrackerConfig.cpp:26-115 key=value files; -1 if the file cannot be opened. */
@@ -95,7 +105,7 @@ void* nuifs_engine(void* srv, int id);
 /* Enqueue one round: n sessions, device-resident depth frames (W*H uint16 each). */
 int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths);
 /* Wait for the last round; latency_ms[i] for the i-th submitted frame (may be NULL).
- * Returns the number of frames whose tracking failed in the round. */
+ * Tracking outcomes stay in each session's pose history (nuifk_poses). */
 int nuifs_wait(void* srv, float* latency_ms);
 
 #ifdef __cplusplus

@@ -108,3 +108,22 @@ bool NuiKinfuVolumeConfig::log(const std::string& fileName) const {
         << "\nDynamicVoxelShift=" << voxel_shift << "\n";
     return true;
 }
+
+NuiHashingSDFConfig::NuiHashingSDFConfig()
+    : m_hashNumBuckets(512 * 512),
+      m_numSDFBlocks(400000),
+      m_hashMaxCollisionLinkedListSize(7),
+      m_virtualVoxelSize(0.005f),
+      m_truncScale(0.02f),
+      m_truncation(0.06f),
+      m_integrationWeightSample(3),
+      m_integrationWeightMax(255),
+      m_bUseSwapping(false),
+      m_bUseForwardRender(false) {}
+
+int NuiHashingSDFConfig::bucketsLog2() const {
+    int b = 4;
+    while (b < 24 && ((size_t)4 << b) < (size_t)2 * m_numSDFBlocks) ++b;
+    return b;
+}
+

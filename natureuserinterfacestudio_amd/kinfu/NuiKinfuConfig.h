@@ -52,3 +52,24 @@ struct NuiKinfuVolumeConfig {
     bool load(const std::string& fileName);
     bool log(const std::string& fileName) const;
 };
+
+// VO/NuiHashingSDFConfig.cpp:3-15.  The table here has 2^b buckets of 4 entries
+// sized from m_numSDFBlocks (load <= 1/2); m_hashNumBuckets and
+// m_hashMaxCollisionLinkedListSize describe the reference's table and are kept
+// for the configuration interface only.
+struct NuiHashingSDFConfig {
+    UINT m_hashNumBuckets;
+    UINT m_numSDFBlocks;
+    UINT m_hashMaxCollisionLinkedListSize;
+    float m_virtualVoxelSize;  // metres
+    float m_truncScale;        // metres: band half-width and sdf normalisation
+    float m_truncation;        // metres: raycast step scale
+    UINT m_integrationWeightSample;
+    UINT m_integrationWeightMax;
+    bool m_bUseSwapping;
+    bool m_bUseForwardRender;
+
+    NuiHashingSDFConfig();
+    int bucketsLog2() const;
+};
+

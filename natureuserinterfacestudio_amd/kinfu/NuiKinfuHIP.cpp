@@ -424,7 +424,7 @@ int NuiKinfuTrackingEngine::takeStageTimes(double ms[4]) {
     return frames;
 }
 
-void NuiKinfuTrackingEngine::enqueueTrackedFrame(NuiKinfuHIPScene* scene) {
+void NuiKinfuTrackingEngine::enqueueTrackedFrame(NuiKinfuHIPSceneBase* scene) {
     recordStage(0);
     m_pFrame->UpdateVertexBuffersDevice(m_pFrame->GetRawBuffer(), m_pCameraState);
     recordStage(1);
@@ -449,10 +449,10 @@ bool NuiKinfuTrackingEngine::RunTrackingDevice(INT64, const UINT16* dDepths, con
                                                NuiKinfuScene* pScene, const NuiCameraParams& cameraParams, bool sync) {
     if (!m_pTracker || !m_pFrame || !m_pCameraState) return true;
     if (!dDepths) return true;
-    auto* scene = dynamic_cast<NuiKinfuHIPScene*>(pScene);
+    auto* scene = dynamic_cast<NuiKinfuHIPSceneBase*>(pScene);
     if (pScene && !scene) throw std::runtime_error("RunTracking: scene is not a GPU scene");
     m_pCameraState->UpdateCameraParams(cameraParams, m_pFrame->GetWidth(), m_pFrame->GetHeight());
-    m_pFrame->SetScaleDepthForTsdf(scene != nullptr);
+    m_pFrame->SetScaleDepthForTsdf(scene != nullptr && scene->NeedsRayLengthDepth());
     const size_t n = (size_t)m_pFrame->GetWidth() * m_pFrame->GetHeight();
     // inputs go to the frame's own buffers so that a captured graph can replay
     if (dDepths != m_pFrame->GetRawBuffer())

@@ -149,8 +149,33 @@ private:
     int m_historyCap = 0;
 };
 
+// What the tracking engine needs from a GPU scene beyond NuiKinfuScene: the
+// device predicates of the tracker (integration gate, tracking success) that
+// guard its launches, and whether it integrates ray-length depth.
+class NuiKinfuHIPSceneBase : public NuiKinfuScene {
+public:
+    void SetGates(const int32_t* integrate, const int32_t* raycast) {
+        m_gateIntegrate = integrate;
+        m_gateRaycast = raycast;
+    }
+    // the dense scene integrates depth scaled to ray length (K/tsdfVolume.cl:8-29);
+    // the hash scene integrates the passing-filter depth as it is
+    virtual bool NeedsRayLengthDepth() const = 0;
+    // the scene's main device allocation (dense volume / hash scene) and its size
+    virtual void* DevicePointer() const = 0;
+    virtual size_t DeviceBytes() const = 0;
+    // truncation distance (dense: clamped tranc_dist; hash: truncScale), metres
+    virtual float Truncation() const = 0;
+    // running count of voxels written by integration (synchronises)
+    virtual unsigned long long UpdatedVoxels() const = 0;
+
+protected:
+    const int32_t* m_gateIntegrate = nullptr;
+    const int32_t* m_gateRaycast = nullptr;
+};
+
 // OCL/NuiKinfuOpenCLScene.*: dense TSDF volume
-class NuiKinfuHIPScene : public NuiKinfuScene {
+class NuiKinfuHIPScene : public NuiKinfuHIPSceneBase {
 public:
     NuiKinfuHIPScene(const NuiKinfuVolumeConfig& config, int storage, nuif_stream_t s);
     bool log(const std::string& fileName) const override { return m_config.log(fileName); }
@@ -159,20 +184,19 @@ public:
     void reset() override;
     bool integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) override;
     void raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) override;
-    bool Volume2CLMesh(NuiKinfuMesh* pData) override { return false; }
-    bool Volume2Mesh(NuiKinfuMesh* pMesh) override { return false; }
-    // device predicates produced by the tracker (integration gate / tracking success)
-    void SetGates(const int32_t* integrate, const int32_t* raycast) {
-        m_gateIntegrate = integrate;
-        m_gateRaycast = raycast;
-    }
+    // K/marchingCube.cl via nuif_mc_dense: triangle soup (3 vertices per triangle)
+    bool Volume2CLMesh(NuiKinfuMesh* pData) override;
+    bool Volume2Mesh(NuiKinfuMesh* pMesh) override { return Volume2CLMesh(pMesh); }
+    bool NeedsRayLengthDepth() const override { return true; }
+    void* DevicePointer() const override { return m_volume.get(); }
+    size_t DeviceBytes() const override { return m_volume.bytes(); }
+    float Truncation() const override { return m_params.tranc_dist; }
     const TsdfParams& GetParams() const { return m_params; }
     int GetStorage() const { return m_storage; }
     void* GetVolume() const { return m_volume.get(); }
     uint8_t* GetColorVolume() const { return m_color.as<uint8_t>(); }
     size_t VolumeBytes() const { return m_volume.bytes(); }
-    // running count of voxels updated by integration since construction (synchronises)
-    unsigned long long UpdatedVoxels() const;
+    unsigned long long UpdatedVoxels() const override;
 
 private:
     NuiKinfuVolumeConfig m_config;
@@ -180,8 +204,42 @@ private:
     nuif_stream_t m_stream;
     TsdfParams m_params{};
     DeviceBuffer m_volume, m_color, m_counter;
-    const int32_t* m_gateIntegrate = nullptr;
-    const int32_t* m_gateRaycast = nullptr;
+};
+
+// OCL/NuiKinfuOpenCLHashScene.*: voxel-hashing scene.  The raycast is castRay
+// over ray ranges from the visible blocks (the reference's m_bUseForwardRender
+// == false path, OCL/NuiKinfuOpenCLHashScene.cpp:659-736).
+class NuiKinfuHIPHashScene : public NuiKinfuHIPSceneBase {
+public:
+    NuiKinfuHIPHashScene(const NuiHashingSDFConfig& config, int storage, bool color, nuif_stream_t s);
+    bool log(const std::string&) const override { return true; }
+    bool hasColorData() const override { return m_params.color != 0; }
+    bool needAcceleratedFeedback() const override { return true; }
+    float getVoxelLeafSize() const override { return m_config.m_virtualVoxelSize; }
+    void reset() override;
+    bool integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) override;
+    void raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) override;
+    // K/KinfuHashVisualization.cl via nuif_hash_mesh over the visible list
+    bool Volume2CLMesh(NuiKinfuMesh* pData) override;
+    bool Volume2Mesh(NuiKinfuMesh* pMesh) override { return Volume2CLMesh(pMesh); }
+    bool NeedsRayLengthDepth() const override { return false; }
+    void* DevicePointer() const override { return m_scene.get(); }
+    size_t DeviceBytes() const override { return m_scene.bytes(); }
+    float Truncation() const override { return m_params.trunc_scale; }
+    const nuif_hash_params& GetParams() const { return m_params; }
+    void* GetScene() const { return m_scene.get(); }
+    float* GetRangeImage() const { return m_range.as<float>(); }
+    // bricks, visible-list length, failed insertions, occupied entries (synchronises)
+    void Stats(uint32_t out[4]) const;
+    unsigned long long UpdatedVoxels() const override;
+    static nuif_hash_params ParamsOf(const NuiHashingSDFConfig& config, int storage, bool color);
+
+private:
+    NuiHashingSDFConfig m_config;
+    nuif_hash_params m_params{};
+    nuif_stream_t m_stream;
+    DeviceBuffer m_scene, m_range, m_counter;
+    UINT m_rangeW = 0, m_rangeH = 0;
 };
 
 }  // namespace nuif_kinfu
@@ -240,7 +298,7 @@ public:
     int takeStageTimes(double ms[4]);
 
 private:
-    void enqueueTrackedFrame(nuif_kinfu::NuiKinfuHIPScene* scene);
+    void enqueueTrackedFrame(nuif_kinfu::NuiKinfuHIPSceneBase* scene);
     void recordStage(int k);
     bool m_stageTiming = false;
     std::vector<nuif_event_t> m_events;  // 5 per timed frame
@@ -256,7 +314,7 @@ private:
     int m_frames = 0;
     bool m_useGraph = false;
     nuif_graph_t m_graph = nullptr;
-    nuif_kinfu::NuiKinfuHIPScene* m_graphScene = nullptr;
+    nuif_kinfu::NuiKinfuHIPSceneBase* m_graphScene = nullptr;
     nuif_kinfu::DeviceBuffer m_history;
     int m_historyCap = 1 << 16;
     mutable std::vector<NuiCameraPos> m_poses;

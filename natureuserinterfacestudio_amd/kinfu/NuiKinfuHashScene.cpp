@@ -1,0 +1,130 @@
+// Voxel-hashing scene (OCL/NuiKinfuOpenCLHashScene.cpp) on the nuif_hash_* entry
+// points, and the meshing of both GPU scenes.
+#include <algorithm>
+#include <stdexcept>
+
+#include "NuiKinfuHIP.h"
+
+namespace nuif_kinfu {
+
+nuif_hash_params NuiKinfuHIPHashScene::ParamsOf(const NuiHashingSDFConfig& c, int storage, bool color) {
+    nuif_hash_params p{};
+    p.voxel_size = c.m_virtualVoxelSize;
+    p.trunc_scale = c.m_truncScale;
+    p.truncation = c.m_truncation;
+    p.weight_max = (int)std::min<UINT>(c.m_integrationWeightMax, 255u);
+    p.buckets_log2 = c.bucketsLog2();
+    p.bricks = (int)c.m_numSDFBlocks;
+    p.storage = storage;
+    p.color = color ? 1 : 0;
+    return p;
+}
+
+NuiKinfuHIPHashScene::NuiKinfuHIPHashScene(const NuiHashingSDFConfig& config, int storage, bool color,
+                                           nuif_stream_t s)
+    : m_config(config), m_stream(s) {
+    m_params = ParamsOf(config, storage, color);
+    const size_t bytes = nuif_hash_scene_bytes(&m_params);
+    if (!bytes) throw std::runtime_error(std::string("hash scene: ") + nuif_last_error());
+    m_scene = DeviceBuffer(bytes);
+    m_counter = DeviceBuffer(sizeof(unsigned long long));
+    reset();
+}
+
+// OCL/NuiKinfuOpenCLHashScene.cpp:134-144 (NuiKinfuVoxelBlockHash::reset)
+void NuiKinfuHIPHashScene::reset() {
+    check(nuif_hash_reset(m_scene.get(), &m_params, m_stream), "nuif_hash_reset");
+    setDirty();
+}
+
+// OCL/NuiKinfuOpenCLHashScene.cpp:533-583 integrateVolume: AllocateSceneFromDepth +
+// IntegrateIntoScene, guarded by the tracker's integration gate
+bool NuiKinfuHIPHashScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) {
+    auto* f = dynamic_cast<NuiKinfuHIPFrame*>(pFrame);
+    auto* cam = dynamic_cast<NuiKinfuHIPCameraState*>(pCameraState);
+    if (!f || !cam) return false;
+    if (f->IsDepthScaled()) throw std::runtime_error("hash scene: the frame depth is scaled to ray length");
+    const NuiCLCameraParams& c = cam->GetCameraParams();
+    check(nuif_hash_allocate(m_scene.get(), &m_params, f->GetDepthBuffer(), &c, cam->GetCameraTransformBuffer(),
+                             m_gateIntegrate, m_stream),
+          "nuif_hash_allocate");
+    const uint8_t* colors = m_params.color ? f->GetColorBuffer() : nullptr;
+    check(nuif_hash_integrate(m_scene.get(), &m_params, f->GetDepthBuffer(), colors, &c,
+                              cam->GetCameraTransformBuffer(), m_gateIntegrate,
+                              m_counter.as<unsigned long long>(), m_stream),
+          "nuif_hash_integrate");
+    setDirty();
+    return true;
+}
+
+// OCL/NuiKinfuOpenCLHashScene.cpp:1014-1064 raycastRender: ray ranges of the
+// visible blocks, then castRay into the feedback frame's maps
+void NuiKinfuHIPHashScene::raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) {
+    auto* fb = dynamic_cast<NuiKinfuHIPFeedbackFrame*>(pFeedbackFrame);
+    auto* cam = dynamic_cast<NuiKinfuHIPCameraState*>(pCameraState);
+    if (!fb || !cam) return;
+    if (!m_range.get() || m_rangeW != fb->GetWidth() || m_rangeH != fb->GetHeight()) {
+        m_rangeW = fb->GetWidth();
+        m_rangeH = fb->GetHeight();
+        m_range = DeviceBuffer((size_t)m_rangeW * m_rangeH * 2 * sizeof(float));
+    }
+    const NuiCLCameraParams& c = cam->GetCameraParams();
+    check(nuif_hash_raycast(m_scene.get(), &m_params, &c, cam->GetCameraTransformBuffer(), m_range.as<float>(),
+                            fb->GetVertexBuffer(), fb->GetNormalBuffer(),
+                            m_params.color ? fb->GetColorBuffer() : nullptr, m_gateRaycast, m_stream),
+          "nuif_hash_raycast");
+}
+
+void NuiKinfuHIPHashScene::Stats(uint32_t out[4]) const {
+    check(nuif_hash_stats(m_scene.get(), out, m_stream), "nuif_hash_stats");
+}
+
+unsigned long long NuiKinfuHIPHashScene::UpdatedVoxels() const {
+    unsigned long long h = 0;
+    check(nuif_memcpy_d2h(&h, m_counter.get(), sizeof(h), m_stream), "counter read");
+    check(nuif_stream_synchronize(m_stream), "counter read");
+    return h;
+}
+
+namespace {
+// count pass, then the emit pass into device buffers of the exact size, then the
+// host copy (the reference maps the result into GL buffers instead)
+template <class CountEmit>
+bool mesh_to_host(NuiKinfuMesh* out, nuif_stream_t s, CountEmit&& run) {
+    if (!out) return false;
+    DeviceBuffer total(sizeof(uint32_t));
+    run(nullptr, nullptr, 0u, total.as<uint32_t>());
+    uint32_t n = 0;
+    check(nuif_memcpy_d2h(&n, total.get(), sizeof(n), s), "mesh count");
+    check(nuif_stream_synchronize(s), "mesh count");
+    out->positions.assign((size_t)n * 3, 0.f);
+    out->colors.assign((size_t)n * 4, 0.f);
+    out->indices.clear();
+    if (n == 0) return true;
+    DeviceBuffer v((size_t)n * 3 * sizeof(float)), c((size_t)n * 4 * sizeof(float));
+    run(v.as<float>(), c.as<float>(), n, total.as<uint32_t>());
+    check(nuif_memcpy_d2h(out->positions.data(), v.get(), v.bytes(), s), "mesh read");
+    check(nuif_memcpy_d2h(out->colors.data(), c.get(), c.bytes(), s), "mesh read");
+    check(nuif_stream_synchronize(s), "mesh read");
+    return true;
+}
+}  // namespace
+
+bool NuiKinfuHIPHashScene::Volume2CLMesh(NuiKinfuMesh* pData) {
+    DeviceBuffer scratch(nuif_hash_mesh_scratch_bytes(&m_params));
+    return mesh_to_host(pData, m_stream, [&](float* v, float* c, uint32_t cap, uint32_t* total) {
+        check(nuif_hash_mesh(m_scene.get(), &m_params, v, c, cap, total, scratch.get(), m_stream), "nuif_hash_mesh");
+    });
+}
+
+// OCL/NuiKinfuOpenCLScene.cpp:617-786 Volume2CLMesh
+bool NuiKinfuHIPScene::Volume2CLMesh(NuiKinfuMesh* pData) {
+    DeviceBuffer scratch(nuif_mc_scratch_bytes(&m_params));
+    return mesh_to_host(pData, m_stream, [&](float* v, float* c, uint32_t cap, uint32_t* total) {
+        check(nuif_mc_dense(m_volume.get(), m_color.as<uint8_t>(), &m_params, m_storage, v, c, cap, total,
+                            scratch.get(), m_stream),
+              "nuif_mc_dense");
+    });
+}
+
+}  // namespace nuif_kinfu

@@ -33,7 +33,12 @@ std::unique_ptr<NuiKinfuSession> NuiKinfuSession::create(const nuifk_desc& d) {
     s->engine->initialize(trackerConfig(d), false, (UINT)d.width, (UINT)d.height);
     s->engine->setUseGraph(d.use_graph != 0);
     s->engine->tracker()->SetReduction(d.icp_reduction);
-    s->scene = std::make_unique<NuiKinfuHIPScene>(vc, d.storage, s->engine->stream());
+    if (d.scene == NUIFK_SCENE_HASH)
+        s->scene = std::make_unique<NuiKinfuHIPHashScene>(hashConfig(d), d.storage, d.color != 0, s->engine->stream());
+    else if (d.scene == NUIFK_SCENE_DENSE)
+        s->scene = std::make_unique<NuiKinfuHIPScene>(vc, d.storage, s->engine->stream());
+    else
+        throw std::runtime_error("session: unknown scene kind");
     s->cam.m_intrinsics = NuiCameraIntrinsics(d.fx, d.fy, d.cx, d.cy);
     s->cam.m_sensorDepthMin = d.depth_min;
     s->cam.m_sensorDepthMax = d.depth_max;
@@ -42,11 +47,26 @@ std::unique_ptr<NuiKinfuSession> NuiKinfuSession::create(const nuifk_desc& d) {
     return s;
 }
 
+NuiHashingSDFConfig NuiKinfuSession::hashConfig(const nuifk_desc& d) {
+    NuiHashingSDFConfig c;
+    c.m_virtualVoxelSize = d.hash_voxel_size;
+    c.m_truncScale = d.hash_trunc_scale;
+    c.m_truncation = d.hash_truncation;
+    c.m_integrationWeightMax = (UINT)d.hash_weight_max;
+    c.m_numSDFBlocks = (UINT)d.hash_bricks;
+    return c;
+}
+
 size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
-    const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
-    const size_t vb = nuif_tsdf_voxel_bytes(d.storage);
-    size_t bytes = voxels * vb + (d.color ? voxels * 4 : 0) + 256;  // + update counter
     const size_t n = (size_t)d.width * d.height;
+    size_t bytes = 256;  // update counter
+    if (d.scene == NUIFK_SCENE_HASH) {
+        const nuif_hash_params hp = NuiKinfuHIPHashScene::ParamsOf(hashConfig(d), d.storage, d.color != 0);
+        bytes += nuif_hash_scene_bytes(&hp) + n * 8;  // + ray-range image
+    } else {
+        const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
+        bytes += voxels * nuif_tsdf_voxel_bytes(d.storage) + (d.color ? voxels * 4 : 0);
+    }
     bytes += n * (2 + 4 + 4);  // raw, passing-filter depth, colours
     for (int l = 0; l < d.levels; ++l) bytes += (size_t)(d.width >> l) * (d.height >> l) * (4 + 12);
     bytes += n * (12 + 12 + 4);  // feedback vertex / normal / colour maps

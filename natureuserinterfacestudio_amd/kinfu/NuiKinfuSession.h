@@ -19,7 +19,7 @@ namespace nuif_kinfu {
 
 struct NuiKinfuSession {
     std::unique_ptr<NuiKinfuEngine::NuiKinfuTrackingEngine> engine;
-    std::unique_ptr<NuiKinfuHIPScene> scene;
+    std::unique_ptr<NuiKinfuHIPSceneBase> scene;
     NuiCameraParams cam;
     UINT W = 0, H = 0;
 
@@ -29,6 +29,7 @@ struct NuiKinfuSession {
     static size_t deviceBytes(const nuifk_desc& d);
     // converts the descriptor's tracker fields (VO/NuiTrackerConfig.cpp:5-24)
     static NuiTrackerConfig trackerConfig(const nuifk_desc& d);
+    static NuiHashingSDFConfig hashConfig(const nuifk_desc& d);
 };
 
 class NuiFusionServer {

@@ -107,8 +107,8 @@ int nuifk_state(void* h, nuif_icp_result* out) {
 int nuifk_read_volume(void* h, void* out, size_t bytes) {
     return guarded([&] {
         Session* s = static_cast<Session*>(h);
-        if (bytes != s->scene->VolumeBytes()) throw std::runtime_error("nuifk_read_volume: size mismatch");
-        check(nuif_memcpy_d2h(out, s->scene->GetVolume(), bytes, s->engine->stream()), "volume read");
+        if (bytes != s->scene->DeviceBytes()) throw std::runtime_error("nuifk_read_volume: size mismatch");
+        check(nuif_memcpy_d2h(out, s->scene->DevicePointer(), bytes, s->engine->stream()), "volume read");
         check(nuif_stream_synchronize(s->engine->stream()), "volume read");
         return 0;
     });
@@ -182,9 +182,9 @@ int nuifk_updated_voxels(void* h, unsigned long long* out) {
 }
 
 void* nuifk_stream(void* h) { return h ? (void*)static_cast<Session*>(h)->engine->stream() : nullptr; }
-void* nuifk_volume_ptr(void* h) { return h ? static_cast<Session*>(h)->scene->GetVolume() : nullptr; }
-size_t nuifk_volume_bytes(void* h) { return h ? static_cast<Session*>(h)->scene->VolumeBytes() : 0; }
-float nuifk_trunc(void* h) { return h ? static_cast<Session*>(h)->scene->GetParams().tranc_dist : 0.f; }
+void* nuifk_volume_ptr(void* h) { return h ? static_cast<Session*>(h)->scene->DevicePointer() : nullptr; }
+size_t nuifk_volume_bytes(void* h) { return h ? static_cast<Session*>(h)->scene->DeviceBytes() : 0; }
+float nuifk_trunc(void* h) { return h ? static_cast<Session*>(h)->scene->Truncation() : 0.f; }
 
 // ---- multi-session server -------------------------------------------------------
 

@@ -71,7 +71,12 @@ class KinfuDesc(C.Structure):
                 ("depth_threshold", c_float), ("dist_threshold", c_float), ("levels", c_int),
                 ("iterations", c_int * 4), ("integration_threshold", c_float), ("resolution", c_int * 3),
                 ("dimensions", c_float * 3), ("tranc_dist", c_float), ("color", c_int), ("storage", c_int),
-                ("use_graph", c_int), ("icp_reduction", c_int)]
+                ("use_graph", c_int), ("icp_reduction", c_int), ("scene", c_int), ("hash_voxel_size", c_float),
+                ("hash_trunc_scale", c_float), ("hash_truncation", c_float), ("hash_weight_max", c_int),
+                ("hash_bricks", c_int)]
+
+
+SCENE_DENSE, SCENE_HASH = 0, 1
 
 
 _SIGS = {
@@ -613,30 +618,7 @@ class HashScene:
     def blocks(self):
         """(keys int16[n,3] sorted by (x,y,z), sdf float32[n,512] (decoded), weight[n,512], raw words[n,512],
         rgb uint8[n,512,3] or None) of every entry that holds a brick."""
-        raw = self.mem.download()
-        ne = self.offsets[6]
-        ent = raw[self.offsets[1]:self.offsets[1] + ne * 16].view(self.ENTRY)
-        sel = np.nonzero((ent["key"] != 0) & (ent["ptr"] >= 0))[0]
-        k = ent["key"][sel]
-        keys = np.stack([((k >> 32) & 0xFFFF).astype(np.uint16).view(np.int16),
-                         ((k >> 16) & 0xFFFF).astype(np.uint16).view(np.int16),
-                         (k & 0xFFFF).astype(np.uint16).view(np.int16)], 1)
-        order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
-        keys = keys[order]
-        ptr = ent["ptr"][sel][order]
-        nbr = self.p.bricks
-        words = raw[self.offsets[4]:self.offsets[4] + nbr * 512 * 4].view(np.uint32).reshape(nbr, 512)[ptr]
-        lo = (words & 0xFFFF).astype(np.uint16)
-        if self.p.storage == TSDF_S16:
-            sdf = lo.view(np.int16).astype(np.float32) / np.float32(32767.0)
-        else:
-            sdf = lo.view(np.float16).astype(np.float32)
-        weight = (words >> 16).astype(np.int32)
-        rgb = None
-        if self.p.color:
-            col = raw[self.offsets[5]:self.offsets[5] + nbr * 512 * 4].reshape(nbr, 512, 4)[ptr]
-            rgb = col[:, :, :3].copy()
-        return keys, sdf, weight, words, rgb
+        return decode_hash_blocks(self.mem.download(), self.p)
 
     def visible_keys(self):
         """Keys of the visible list (int16[n,3], sorted by (x,y,z))."""
@@ -650,3 +632,44 @@ class HashScene:
                          ((k >> 16) & 0xFFFF).astype(np.uint16).view(np.int16),
                          (k & 0xFFFF).astype(np.uint16).view(np.int16)], 1)
         return keys[np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))]
+
+
+def hash_params_of_desc(d):
+    """The nuif_hash_params a hash session builds from its descriptor (buckets: the
+    smallest 2^b with 4 * 2^b >= 2 * bricks)."""
+    b = 4
+    while b < 24 and (4 << b) < 2 * d.hash_bricks:
+        b += 1
+    return hash_params(voxel=d.hash_voxel_size, trunc_scale=d.hash_trunc_scale, truncation=d.hash_truncation,
+                       weight_max=d.hash_weight_max, buckets_log2=b, bricks=d.hash_bricks, storage=d.storage,
+                       color=bool(d.color))
+
+
+def decode_hash_blocks(raw, params):
+    """Blocks of a downloaded hash scene: (keys int16[n,3] sorted by (x,y,z), sdf float32[n,512],
+    weight int32[n,512], raw words uint32[n,512], rgb uint8[n,512,3] or None)."""
+    off = (c_size_t * 8)()
+    check(L().nuif_hash_layout(C.byref(params), off), "nuif_hash_layout")
+    ne = off[6]
+    ent = raw[off[1]:off[1] + ne * 16].view(HashScene.ENTRY)
+    sel = np.nonzero((ent["key"] != 0) & (ent["ptr"] >= 0))[0]
+    k = ent["key"][sel]
+    keys = np.stack([((k >> 32) & 0xFFFF).astype(np.uint16).view(np.int16),
+                     ((k >> 16) & 0xFFFF).astype(np.uint16).view(np.int16),
+                     (k & 0xFFFF).astype(np.uint16).view(np.int16)], 1)
+    order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+    keys = keys[order]
+    ptr = ent["ptr"][sel][order]
+    nbr = params.bricks
+    words = raw[off[4]:off[4] + nbr * 512 * 4].view(np.uint32).reshape(nbr, 512)[ptr]
+    lo = (words & 0xFFFF).astype(np.uint16)
+    if params.storage == TSDF_S16:
+        sdf = lo.view(np.int16).astype(np.float32) / np.float32(32767.0)
+    else:
+        sdf = lo.view(np.float16).astype(np.float32)
+    weight = (words >> 16).astype(np.int32)
+    rgb = None
+    if params.color:
+        rgb = raw[off[5]:off[5] + nbr * 512 * 4].reshape(nbr, 512, 4)[ptr][:, :, :3].copy()
+    return keys, sdf, weight, words, rgb
+

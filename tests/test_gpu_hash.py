@@ -104,3 +104,30 @@ def test_hash_f16_bricks_close_to_oracle(gpu):
     np.testing.assert_array_equal(w, ow.astype(np.int32))
     m = ow > 0
     assert np.abs(sdf[m] - osdf[m] / np.float32(32767.0)).max() <= 1e-3
+
+
+def test_hash_engine_matches_oracle_engine(gpu):
+    """RunTracking with the hash scene: preprocessing, ICP against the castRay maps,
+    allocation and integration -- poses and every brick equal the oracle's."""
+    announce("hash engine 320x240, 5 frames vs oracle hash engine")
+    n = 5
+    desc = WL.kinfu_desc(bricks=1 << 17)
+    eng = nuif.Engine(desc)
+    for f in range(n):
+        eng.run(nuif.DeviceArray.from_host(WL.render(f)[0]), sync=False)
+    eng.synchronize()
+    poses = eng.poses(n)
+    ref2 = O.OracleHashEngine(WL, capacity=1 << 17)
+    for f in range(n):
+        ok, pr, _ = ref2.run(WL.render(f)[0])
+        assert ok == 1 and poses[f, 12] == 1.0
+        np.testing.assert_array_equal(poses[f, :12], pr.astype(np.float32))
+    keys, _, w, words, _ = nuif.decode_hash_blocks(eng.volume_raw(), nuif.hash_params_of_desc(desc))
+    okeys, osdf, ow, _ = ref2.blocks()
+    np.testing.assert_array_equal(keys, okeys)
+    np.testing.assert_array_equal((words & 0xFFFF).astype(np.uint16).view(np.int16), osdf)
+    np.testing.assert_array_equal(w, ow.astype(np.int32))
+    v, nm = eng.maps()
+    ov, on = ref2.refmaps()
+    np.testing.assert_array_equal(v, ov)
+    np.testing.assert_array_equal(nm, on)
