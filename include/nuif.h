@@ -216,13 +216,21 @@ int nuif_icp_count_thresholds(int width, int height, const nuif_icp_config* cfg,
 size_t nuif_tsdf_voxel_bytes(int storage);
 /* K/tsdfVolume.cl:192-211 initializeVolume: (32767,0) / (0,0) and colour 0 */
 int nuif_tsdf_reset(void* volume, uint8_t* color_volume, const TsdfParams* params, int storage, nuif_stream_t s);
+/* Depth bounds of a frame for nuif_tsdf_integrate: the largest valid depth over square
+ * pixel cells of 8, 16, 32, ... pixels (device floats, nuif_tsdf_depth_bounds_bytes). */
+size_t nuif_tsdf_depth_bounds_bytes(int width, int height);
+int nuif_tsdf_depth_bounds(const float* depth, int width, int height, float* bounds, const int32_t* gate,
+                           nuif_stream_t s);
 /* K/tsdfVolume.cl:31-190 integrateTsdfVolumeKernel.  depth: ray-length depth (scaled);
  * bgra/color_volume may be NULL; gate: optional device int32, integrate only if non-zero;
- * updated: optional device counter, incremented by the number of voxels updated. */
+ * updated: optional device counter, incremented by the number of voxels updated;
+ * depth_bounds: optional nuif_tsdf_depth_bounds of the same depth -- 64-voxel runs of a
+ * column that lie more than the truncation behind every depth they can project to are
+ * skipped (the reference's kernel skips each of their voxels; results are identical). */
 int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                         const TsdfParams* params, const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose,
                         int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
-                        nuif_stream_t s);
+                        const float* depth_bounds, nuif_stream_t s);
 /* K/raycast.cl:128-277 raycastKernel: world vmap/nmap (float3 AoS) and optional cmap */
 int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
                       const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,

@@ -154,20 +154,30 @@ __device__ __forceinline__ void column_advance(ColumnWalk& c, float cz) {
     c.vy += c.incy;
 }
 
-// True when none of the next n steps of the walk can land inside the image, so
-// column_step would return NaN for all of them.  The camera-space point of the
-// column is affine in z, hence so are the projection's numerators and its depth;
-// with the depth of one sign over the segment the projected coordinates are
-// monotonic and the end points bound them.  The margins cover the rounding of
-// the serial accumulation (relative 1e-4 of the magnitudes, far above 64 ulps).
-__device__ __forceinline__ bool column_chunk_outside(const ColumnWalk& c, float R8, float cz, int n,
-                                                     const NuiCLCameraParams& cam) {
+// Where the next n steps of a column walk can project.  The camera-space point
+// of the column is affine in z, hence so are the projection's numerators and its
+// depth; with the depth of one sign over the segment the projected coordinates
+// are monotonic and the end points bound them.  The margins cover the rounding
+// of the serial accumulation (relative 1e-4 of the magnitudes, far above 64 ulps).
+//   outside: no step can land inside the image (column_step returns NaN for all)
+//   rect:    the pixel rectangle [x0, x1] x [y0, y1] holds every pixel a step can read
+struct ChunkFootprint {
+    bool outside, rect;
+    int x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ ChunkFootprint chunk_footprint(const ColumnWalk& c, float R8, float cz, int n,
+                                                          const NuiCLCameraParams& cam) {
+    ChunkFootprint f{false, false, 0, 0, 0, 0};
     const float steps = (float)(n - 1);
     const float z0 = c.vz + R8 * c.zs;
     const float z1 = c.vz + R8 * (c.zs + steps * cz);
     const float ez = 1e-4f * (fabsf(c.vz) + fabsf(R8 * c.zs) + fabsf(R8 * steps * cz)) + 1e-6f;
-    if (z0 < -ez && z1 < -ez) return true;  // behind the camera over the whole chunk
-    if (!(z0 > ez && z1 > ez)) return false;
+    if (z0 < -ez && z1 < -ez) {  // behind the camera over the whole chunk
+        f.outside = true;
+        return f;
+    }
+    if (!(z0 > ez && z1 > ez)) return f;
     const float W = (float)cam.depthImageWidth, H = (float)cam.depthImageHeight;
     const float x0 = c.vx, x1 = c.vx + steps * c.incx;
     const float y0 = c.vy, y1 = c.vy + steps * c.incy;
@@ -177,9 +187,108 @@ __device__ __forceinline__ bool column_chunk_outside(const ColumnWalk& c, float 
     // pixel u is hit iff round(u' + cx) in [0, W-1], i.e. u' + cx in [-0.5, W - 0.5)
     const float ulo = fminf(u0, u1) + cam.cx, uhi = fmaxf(u0, u1) + cam.cx;
     const float vlo = fminf(v0, v1) + cam.cy, vhi = fmaxf(v0, v1) + cam.cy;
-    if (uhi < -0.5f - mu || ulo > W - 0.5f + mu) return true;
-    if (vhi < -0.5f - mv || vlo > H - 0.5f + mv) return true;
-    return false;
+    if (uhi < -0.5f - mu || ulo > W - 0.5f + mu || vhi < -0.5f - mv || vlo > H - 0.5f + mv) {
+        f.outside = true;
+        return f;
+    }
+    f.rect = true;
+    f.x0 = (int)fmaxf(floorf(ulo - mu), 0.f);
+    f.x1 = (int)fminf(ceilf(uhi + mu), W - 1.f);
+    f.y0 = (int)fmaxf(floorf(vlo - mv), 0.f);
+    f.y1 = (int)fminf(ceilf(vhi + mv), H - 1.f);
+    return f;
+}
+
+// ---- depth bounds: max valid depth over square pixel cells --------------------
+// Level L has cells of 2^(L+3) pixels a side, stored level after level; a cell
+// without a valid pixel holds -1.  Levels go up to a single cell.
+
+constexpr int kBoundsBase = 3;
+
+__host__ __device__ __forceinline__ int bounds_levels(int W, int H) {
+    int L = 0;
+    while (((W - 1) >> (L + kBoundsBase)) > 0 || ((H - 1) >> (L + kBoundsBase)) > 0) ++L;
+    return L + 1;
+}
+__host__ __device__ __forceinline__ int bounds_dim(int n, int L) { return ((n - 1) >> (L + kBoundsBase)) + 1; }
+__host__ __device__ __forceinline__ size_t bounds_offset(int W, int H, int L) {
+    size_t o = 0;
+    for (int k = 0; k < L; ++k) o += (size_t)bounds_dim(W, k) * bounds_dim(H, k);
+    return o;
+}
+
+__global__ void depth_bounds_base_kernel(const float* __restrict__ depth, int W, int H, float* __restrict__ out,
+                                         const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int cw = bounds_dim(W, 0), ch = bounds_dim(H, 0);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cw * ch) return;
+    const int cx = i % cw, cy = i / cw;
+    float m = -1.f;
+    for (int y = cy * 8; y < min(H, cy * 8 + 8); ++y)
+        for (int x = cx * 8; x < min(W, cx * 8 + 8); ++x) {
+            const float d = depth[(size_t)y * W + x];
+            if (!isnan(d)) m = fmaxf(m, d);
+        }
+    out[i] = m;
+}
+
+__global__ void __launch_bounds__(1024)
+    depth_bounds_levels_kernel(int W, int H, float* __restrict__ out, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const int levels = bounds_levels(W, H);
+    for (int L = 1; L < levels; ++L) {
+        const float* src = out + bounds_offset(W, H, L - 1);
+        float* dst = out + bounds_offset(W, H, L);
+        const int sw = bounds_dim(W, L - 1), sh = bounds_dim(H, L - 1);
+        const int dw = bounds_dim(W, L), dh = bounds_dim(H, L);
+        for (int i = threadIdx.x; i < dw * dh; i += blockDim.x) {
+            const int x = (i % dw) * 2, y = (i / dw) * 2;
+            float m = src[y * sw + x];
+            if (x + 1 < sw) m = fmaxf(m, src[y * sw + x + 1]);
+            if (y + 1 < sh) {
+                m = fmaxf(m, src[(y + 1) * sw + x]);
+                if (x + 1 < sw) m = fmaxf(m, src[(y + 1) * sw + x + 1]);
+            }
+            dst[i] = m;
+        }
+        __syncthreads();
+    }
+}
+
+// Largest valid depth of the pixels in [x0, x1] x [y0, y1] (within the image), or
+// an upper bound of it: the max over the at most 2 x 2 cells of the first level
+// whose cells cover the rectangle that way.
+__device__ __forceinline__ float bounds_max(const float* __restrict__ b, int W, int H, int x0, int x1, int y0, int y1) {
+    const int levels = bounds_levels(W, H);
+    int L = 0;
+    while (L + 1 < levels && (((x1 >> (L + kBoundsBase)) - (x0 >> (L + kBoundsBase))) > 1 ||
+                              ((y1 >> (L + kBoundsBase)) - (y0 >> (L + kBoundsBase))) > 1))
+        ++L;
+    const float* lv = b + bounds_offset(W, H, L);
+    const int cw = bounds_dim(W, L);
+    const int cx0 = x0 >> (L + kBoundsBase), cx1 = x1 >> (L + kBoundsBase);
+    const int cy0 = y0 >> (L + kBoundsBase), cy1 = y1 >> (L + kBoundsBase);
+    float m = lv[cy0 * cw + cx0];
+    m = fmaxf(m, lv[cy0 * cw + cx1]);
+    m = fmaxf(m, lv[cy1 * cw + cx0]);
+    m = fmaxf(m, lv[cy1 * cw + cx1]);
+    return m;
+}
+
+// True when every voxel of the next n steps lies more than trunc behind the
+// largest depth its pixels can hold, so column_step skips all of them (sdf =
+// D - |v - c| < -trunc).  |v - c|^2 = vgz^2 + part with vgz affine in the step;
+// the margin (2e-4 m + 1e-5 relative) covers the serial accumulation of vgz.
+__device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float cz, int n, const ChunkFootprint& f,
+                                                     const float* __restrict__ bounds, int W, int H, float trunc) {
+    if (!bounds || !f.rect) return false;
+    const float a = c.vgz, e = c.vgz + (float)(n - 1) * cz;
+    const float lo = fminf(a, e), hi = fmaxf(a, e);
+    const float m2 = (lo <= 0.f && hi >= 0.f) ? 0.f : fminf(lo * lo, hi * hi);
+    const float dmin = sqrtf(m2 + c.part);
+    const float dmax = bounds_max(bounds, W, H, f.x0, f.x1, f.y0, f.y1);
+    return dmax + trunc < dmin * (1.0f - 1e-5f) - 2e-4f;
 }
 
 // Fast path: Nx, Ny multiples of 8, Nz multiple of 64.
@@ -188,7 +297,8 @@ __global__ void __launch_bounds__(64)
     integrate_tile_kernel(const float* __restrict__ depth, const uchar4* __restrict__ bgra,
                           typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
                           NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
-                          const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated) {
+                          const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated,
+                          const float* __restrict__ bounds) {
     if (gate && *gate == 0) return;
     using word = typename S::word;
     constexpr int VPL = 16 / S::kBytes;      // voxels per lane per 16-byte vector
@@ -211,7 +321,10 @@ __global__ void __launch_bounds__(64)
     for (int zc = 0; zc < Nz; zc += kZChunk) {
         // phase 1: serial walk of this lane's column; chunks that cannot project
         // into the image only advance the walk
-        const bool outside = column_chunk_outside(c, R8, cz, kZChunk, cam);
+        const ChunkFootprint fp = chunk_footprint(c, R8, cz, kZChunk, cam);
+        const bool outside = fp.outside || chunk_behind_surface(c, cz, kZChunk, fp, bounds,
+                                                                (int)cam.depthImageWidth,
+                                                                (int)cam.depthImageHeight, trunc);
         if (__all(outside)) {
             for (int k = 0; k < kZChunk; ++k) column_advance(c, cz);
             continue;
@@ -499,7 +612,7 @@ __global__ void __launch_bounds__(64)
 template <class S>
 int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                      const TsdfParams& p, const NuiCLCameraParams& cam, const NuiCLRigidTransform* pose, int maxcw,
-                     const int32_t* gate, unsigned long long* updated, hipStream_t st) {
+                     const int32_t* gate, unsigned long long* updated, const float* bounds, hipStream_t st) {
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const bool color = bgra && color_volume;
     auto* vol = reinterpret_cast<typename S::word*>(volume);
@@ -508,9 +621,11 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
     if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0) {
         dim3 grid(Nx / 8, Ny / 8);
         if (color)
-            integrate_tile_kernel<S, true><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated);
+            integrate_tile_kernel<S, true><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
+                                                                 bounds);
         else
-            integrate_tile_kernel<S, false><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated);
+            integrate_tile_kernel<S, false><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
+                                                                  bounds);
         NUIF_LAUNCHED("integrate_tile_kernel");
     } else {
         dim3 block(16, 16), grid(div_up(Nx, 16), div_up(Ny, 16));
@@ -600,7 +715,7 @@ int nuif_tsdf_reset(void* volume, uint8_t* color_volume, const TsdfParams* param
 int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                         const TsdfParams* params, const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose,
                         int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
-                        nuif_stream_t s) {
+                        const float* depth_bounds, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     int rc = check_params(params);
     if (rc) return rc;
@@ -609,13 +724,31 @@ int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, u
     hipStream_t st = stream_of(s);
     switch (storage) {
         case NUIF_TSDF_S16:
-            return launch_integrate<StS16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, st);
+            return launch_integrate<StS16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, depth_bounds, st);
         case NUIF_TSDF_F16:
-            return launch_integrate<StF16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, st);
+            return launch_integrate<StF16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, depth_bounds, st);
         case NUIF_TSDF_F32:
-            return launch_integrate<StF32>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, st);
+            return launch_integrate<StF32>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, depth_bounds, st);
         default: set_error("nuif_tsdf_integrate: unknown storage %d", storage); return NUIF_ERR_INVALID;
     }
+}
+
+size_t nuif_tsdf_depth_bounds_bytes(int width, int height) {
+    if (width <= 0 || height <= 0) return 0;
+    return bounds_offset(width, height, bounds_levels(width, height)) * sizeof(float);
+}
+
+int nuif_tsdf_depth_bounds(const float* depth, int width, int height, float* bounds, const int32_t* gate,
+                           nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(depth && bounds && width > 0 && height > 0, "nuif_tsdf_depth_bounds: bad argument");
+    hipStream_t st = stream_of(s);
+    const int cells = bounds_dim(width, 0) * bounds_dim(height, 0);
+    depth_bounds_base_kernel<<<div_up(cells, 256), 256, 0, st>>>(depth, width, height, bounds, gate);
+    NUIF_LAUNCHED("depth_bounds_base_kernel");
+    depth_bounds_levels_kernel<<<1, 1024, 0, st>>>(width, height, bounds, gate);
+    NUIF_LAUNCHED("depth_bounds_levels_kernel");
+    return NUIF_OK;
 }
 
 int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,

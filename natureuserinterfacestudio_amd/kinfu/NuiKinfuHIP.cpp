@@ -270,9 +270,15 @@ bool NuiKinfuHIPScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraStat
         check(nuif_scale_depths(f->GetDepthBuffer(), (int)f->GetWidth(), (int)f->GetHeight(), &c, m_stream),
               "nuif_scale_depths");
     const uint8_t* colors = m_config.bHas_color_volume ? f->GetColorBuffer() : nullptr;
+    // depth bounds of the frame: runs of voxels far behind every observed surface are skipped
+    const size_t nb = nuif_tsdf_depth_bounds_bytes((int)f->GetWidth(), (int)f->GetHeight());
+    if (m_bounds.bytes() != nb) m_bounds = DeviceBuffer(nb);
+    check(nuif_tsdf_depth_bounds(f->GetDepthBuffer(), (int)f->GetWidth(), (int)f->GetHeight(), m_bounds.as<float>(),
+                                 m_gateIntegrate, m_stream),
+          "nuif_tsdf_depth_bounds");
     check(nuif_tsdf_integrate(f->GetDepthBuffer(), colors, m_volume.get(), colors ? m_color.as<uint8_t>() : nullptr,
                               &m_params, &c, cam->GetCameraTransformBuffer(), m_storage, m_config.max_color_weight,
-                              m_gateIntegrate, m_counter.as<unsigned long long>(), m_stream),
+                              m_gateIntegrate, m_counter.as<unsigned long long>(), m_bounds.as<float>(), m_stream),
           "nuif_tsdf_integrate");
     setDirty();
     return true;

@@ -203,7 +203,7 @@ private:
     int m_storage;
     nuif_stream_t m_stream;
     TsdfParams m_params{};
-    DeviceBuffer m_volume, m_color, m_counter;
+    DeviceBuffer m_volume, m_color, m_counter, m_bounds;
 };
 
 // OCL/NuiKinfuOpenCLHashScene.*: voxel-hashing scene.  The raycast is castRay

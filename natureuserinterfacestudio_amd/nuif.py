@@ -139,7 +139,9 @@ _SIGS = {
     "nuif_tsdf_reset": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p]),
     "nuif_tsdf_integrate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(TsdfParams),
                                     C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_int, c_void_p, c_void_p,
-                                    c_void_p]),
+                                    c_void_p, c_void_p]),
+    "nuif_tsdf_depth_bounds_bytes": (c_size_t, [c_int, c_int]),
+    "nuif_tsdf_depth_bounds": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "nuif_tsdf_raycast": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "nuif_tsdf_raycast_gated": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),

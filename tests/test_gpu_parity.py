@@ -253,9 +253,11 @@ def _small_volume_inputs(wl, frame):
     return dep
 
 
-@pytest.mark.parametrize("res", [128, 96])  # 96^3 runs the generic column kernel (96 % 64 != 0)
-def test_integrate_bit_exact_s16(gpu, res):
-    announce(f"integrate S16 {res}^3 320x240, 3 frames with motion")
+@pytest.mark.parametrize("res,bounds", [(128, False), (128, True), (96, False)])  # 96^3: generic column kernel
+def test_integrate_bit_exact_s16(gpu, res, bounds):
+    """With depth bounds, runs of voxels behind every surface they project to are
+    skipped wholesale; the volume must not change by a single bit."""
+    announce(f"integrate S16 {res}^3 320x240, 3 frames with motion, depth bounds {bounds}")
     wl = W.SMALL
     N = res
     p = nuif.tsdf_params(N, wl.volume_dim, wl.trunc)
@@ -269,8 +271,15 @@ def test_integrate_bit_exact_s16(gpu, res):
         dep = _small_volume_inputs(wl, f)
         ddep = nuif.DeviceArray.from_host(dep)
         pose = nuif.upload_struct(nuif.rigid(R, t))
+        db = None
+        if bounds:
+            db = nuif.DeviceArray((gpu.L().nuif_tsdf_depth_bounds_bytes(wl.width, wl.height),), np.uint8)
+            nuif.check(gpu.L().nuif_tsdf_depth_bounds(ddep.ptr, wl.width, wl.height, db.ptr, None, None))
+            # the coarsest level is the frame's largest valid depth
+            mx = db.download().view(np.float32)[-1]
+            assert mx == np.nanmax(dep)
         nuif.check(gpu.L().nuif_tsdf_integrate(ddep.ptr, None, dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr,
-                                               0, 128, None, None, None))
+                                               0, 128, None, None, db.ptr if db else None, None))
         O.lib().or_integrate_s16_f(dep, None, vol.reshape(-1), None, N, wl.volume_dim, wl.trunc, wl.cam6(),
                                    wl.width, wl.height, R.reshape(-1).copy(), t, 128)
     got = dvol.download()
@@ -298,7 +307,7 @@ def test_integrate_f16_f32_close_to_fp32_math(gpu, storage):
     pose = nuif.upload_struct(nuif.rigid(R, t))
     for _ in range(3):
         nuif.check(gpu.L().nuif_tsdf_integrate(ddep.ptr, None, dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr,
-                                               storage, 128, None, None, None))
+                                               storage, 128, None, None, None, None))
         O.lib().or_integrate_s16_f(dep, None, vol.reshape(-1), None, N, wl.volume_dim, wl.trunc, wl.cam6(),
                                    wl.width, wl.height, R.reshape(-1).copy(), t, 128)
     tg, wg = nuif.decode_volume(dvol.download(), storage)

@@ -54,6 +54,18 @@ def test_mc_table_equals_reference_table():
     np.testing.assert_array_equal(nv, ref_nv)
 
 
+@pytest.mark.skipif(not os.path.exists(REF_TABLE), reason="reference sources not mounted")
+def test_edge_masks_equal_reference_edge_table():
+    """The hash mesher takes a cube's edge mask from the edges its triangles use;
+    that is the reference's sEdgeTable (OCL/NuiMarchingCubeTable.h:542) entry by entry."""
+    src = open(REF_TABLE).read()
+    blk = src[src.index("sEdgeTable"):]
+    ref = [int(v, 16) for v in re.findall(r"0x[0-9a-fA-F]+", blk[blk.index("{"):blk.index("}")])]
+    edge, _ = O.mc_tables()
+    assert len(ref) == 256
+    np.testing.assert_array_equal(edge, np.array(ref, np.int32))
+
+
 def sphere_volume(N=32, r=0.3, dim=1.0):
     c = dim / N
     y, x, z = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
