@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0, help="server device-memory budget")
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many MiB device to device and exit (scales the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the fusion sessions shard by session with no collective, "
@@ -128,8 +130,18 @@ def main():
     from natureuserinterfacestudio_amd import nuif
     from natureuserinterfacestudio_amd import workloads as W
 
-    wl = W.FULL
     nuif.init(0)
+    if args.copy_probe:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE per byte of this copy
+        # convert the counters of the measured kernels to bytes
+        n = args.copy_probe << 20
+        a, b = nuif.DeviceArray((n,), np.uint8), nuif.DeviceArray((n,), np.uint8)
+        for _ in range(3):
+            nuif.check(nuif.L().nuif_memcpy_d2d(b.ptr, a.ptr, n, None), "copy")
+        nuif.check(nuif.L().nuif_device_synchronize(), "sync")
+        print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
+        return
+    wl = W.FULL
     info = nuif.device_info()
     n = args.warmup + args.steps + max(args.sessions - 1, 0)
     if n > wl.frames:

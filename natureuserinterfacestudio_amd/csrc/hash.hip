@@ -463,10 +463,12 @@ __global__ void __launch_bounds__(64)
             cym = max(cym, __shfl_xor(cym, off, 64));
             z = fminf(z, __shfl_xor(z, off, 64));
         }
-        fx = __shfl(fx, 0, 64);
-        fy = __shfl(fy, 0, 64);
-        cxm = __shfl(cxm, 0, 64);
-        cym = __shfl(cym, 0, 64);
+        // the reference starts the rectangle at upperLeft = (W, H), lowerRight = (0, 0)
+        // (K/KinfuHashRaycast.cl:24-25): those start values take part in the min / max
+        fx = min(__shfl(fx, 0, 64), W);
+        fy = min(__shfl(fy, 0, 64), H);
+        cxm = max(__shfl(cxm, 0, 64), 0);
+        cym = max(__shfl(cym, 0, 64), 0);
         float znear = __shfl(z, 0, 64);
         if (fx < 0) fx = 0;
         if (fy < 0) fy = 0;

@@ -39,7 +39,7 @@ namespace nuif {
 
 constexpr int kTerms = NUIF_ICP_TERMS;
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 1024;   // compensated: blocks per launch
+constexpr int kMaxBlocks = 256;    // compensated: blocks per launch (grid-stride over the pixels)
 constexpr int kMaxGroups2 = 256;   // reference: 128-block groups per launch (W*H <= 2^22)
 
 struct IcpState {
@@ -208,9 +208,19 @@ __global__ void __launch_bounds__(kBlock)
     __threadfence();  // acquire the other blocks' partials
     const int nblocks = (int)gridDim.x;
     if (tid < kTerms * kSeg) {
+        // term k, segment j: blocks j, j + 8, ... in order; the loads of 8 blocks are
+        // issued together, the FP64 sum keeps the fixed order
         const int k = tid / kSeg, j = tid % kSeg;
         double s = 0.0;
-        for (int b = j; b < nblocks; b += kSeg) s += partials[(size_t)b * kTerms + k];
+        int b = j;
+        for (; b + 7 * kSeg < nblocks; b += 8 * kSeg) {
+            double v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = partials[(size_t)(b + q * kSeg) * kTerms + k];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s += v[q];
+        }
+        for (; b < nblocks; b += kSeg) s += partials[(size_t)b * kTerms + k];
         part[k * kSeg + j] = s;
     }
     __syncthreads();

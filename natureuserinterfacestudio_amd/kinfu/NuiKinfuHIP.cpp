@@ -132,6 +132,14 @@ void NuiKinfuHIPFrame::UpdateVertexBuffersDevice(const UINT16* dDepths, NuiKinfu
                                 m_levels, (int)m_W, (int)m_H, &c, m_gaussian.data(), (int)m_cfg.filter_radius,
                                 m_cfg.sigma_depth2_inv_half, m_cfg.depth_threshold, m_scaleForTsdf ? 1 : 0, m_stream),
           "nuif_frame_preprocess");
+    if (m_scaleForTsdf) {
+        // depth bounds of the ray-length depth, used by the dense integration to skip
+        // runs of voxels behind every observed surface
+        const size_t nb = nuif_tsdf_depth_bounds_bytes((int)m_W, (int)m_H);
+        if (m_bounds.bytes() != nb) m_bounds = DeviceBuffer(nb);
+        check(nuif_tsdf_depth_bounds(m_depth.as<float>(), (int)m_W, (int)m_H, m_bounds.as<float>(), nullptr, m_stream),
+              "nuif_tsdf_depth_bounds");
+    }
 }
 
 void NuiKinfuHIPFrame::UpdateColorBuffers(BGRQUAD* pColors, UINT nNum) {
@@ -270,15 +278,11 @@ bool NuiKinfuHIPScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraStat
         check(nuif_scale_depths(f->GetDepthBuffer(), (int)f->GetWidth(), (int)f->GetHeight(), &c, m_stream),
               "nuif_scale_depths");
     const uint8_t* colors = m_config.bHas_color_volume ? f->GetColorBuffer() : nullptr;
-    // depth bounds of the frame: runs of voxels far behind every observed surface are skipped
-    const size_t nb = nuif_tsdf_depth_bounds_bytes((int)f->GetWidth(), (int)f->GetHeight());
-    if (m_bounds.bytes() != nb) m_bounds = DeviceBuffer(nb);
-    check(nuif_tsdf_depth_bounds(f->GetDepthBuffer(), (int)f->GetWidth(), (int)f->GetHeight(), m_bounds.as<float>(),
-                                 m_gateIntegrate, m_stream),
-          "nuif_tsdf_depth_bounds");
+    // the frame's depth bounds exist when it was preprocessed for this scene
+    const float* bounds = f->IsDepthScaled() ? f->GetDepthBounds() : nullptr;
     check(nuif_tsdf_integrate(f->GetDepthBuffer(), colors, m_volume.get(), colors ? m_color.as<uint8_t>() : nullptr,
                               &m_params, &c, cam->GetCameraTransformBuffer(), m_storage, m_config.max_color_weight,
-                              m_gateIntegrate, m_counter.as<unsigned long long>(), m_bounds.as<float>(), m_stream),
+                              m_gateIntegrate, m_counter.as<unsigned long long>(), bounds, m_stream),
           "nuif_tsdf_integrate");
     setDirty();
     return true;

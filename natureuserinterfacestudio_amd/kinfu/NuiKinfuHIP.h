@@ -77,6 +77,8 @@ public:
     void SetScaleDepthForTsdf(bool v) { m_scaleForTsdf = v; }
     bool IsDepthScaled() const { return m_scaleForTsdf; }
     float* GetDepthBuffer() const { return m_depth.as<float>(); }
+    // max-depth pyramid of the ray-length depth (nuif_tsdf_depth_bounds), when scaled
+    const float* GetDepthBounds() const { return m_bounds.as<float>(); }
     float* GetFilteredDepthBuffer() const { return m_dpyr[0].as<float>(); }
     float* GetVertexBuffer(int level = 0) const { return m_vpyr[level].as<float>(); }
     const float* const* GetVertexPyramid() const { return m_vptrs.data(); }
@@ -92,7 +94,7 @@ private:
     nuif_stream_t m_stream;
     bool m_scaleForTsdf = false;
     bool m_hasColor = false;
-    DeviceBuffer m_raw, m_depth, m_colors;
+    DeviceBuffer m_raw, m_depth, m_colors, m_bounds;
     std::vector<DeviceBuffer> m_dpyr, m_vpyr;
     std::vector<float*> m_dptrs;
     std::vector<const float*> m_vptrs;
@@ -203,7 +205,7 @@ private:
     int m_storage;
     nuif_stream_t m_stream;
     TsdfParams m_params{};
-    DeviceBuffer m_volume, m_color, m_counter, m_bounds;
+    DeviceBuffer m_volume, m_color, m_counter;
 };
 
 // OCL/NuiKinfuOpenCLHashScene.*: voxel-hashing scene.  The raycast is castRay
