@@ -45,6 +45,18 @@ def cpu_baseline(wl, frames):
                       f"float oracle with the reference's short2 volume, OpenMP"}
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy of known size), or None when not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_bytes_full_640x480_512.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_{key}:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def run_single(nuif, wl, frames, storage, graph, warmup, steps, reduction=None, stage_timing=True):
     """One session, frames back to back on its stream.  Returns throughput, the
     per-stage device times (event-timed when graphs are off) and updated voxels."""
@@ -161,7 +173,8 @@ def main():
         t_int = single["stage_avg"][2] / 1e3
         achieved = integ_bytes / t_int / 1e9
         roof = {"kernel": f"integrate_tile_kernel<{args.storage}>", "bound": "hbm", "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(f"integrate_tile_kernel_{args.storage}"),
                 "algorithmic_bytes": int(integ_bytes), "avg_ms": round(single["stage_avg"][2], 4),
                 "updated_voxels_per_launch": int(single["updated_per_frame"])}
 
