@@ -267,6 +267,17 @@ int nuif_mc_dense(const void* volume, const uint8_t* color_volume, const TsdfPar
                   float* vertices, float* colors, uint32_t capacity, uint32_t* total, void* scratch,
                   nuif_stream_t s);
 
+/* OCL/NuiKinfuOpenCLScene.cpp:788-984 Volume2Mesh on the device: marching cubes in the
+ * reference's (y, x, z) cube order with its weight test, vertices keyed by the ordered voxel
+ * pair of their cube edge and numbered in first-use order, positions centred on all axes
+ * (getNodeCoo, :84-91).  points: float3 x max_vertices, indices: uint32 x max_vertices
+ * (device); *n_points / *n_indices are host counts.  Synchronous (the count decides the
+ * work); fails with NUIF_ERR_INVALID when the corners exceed max_vertices (*n_indices then
+ * holds the needed count). */
+size_t nuif_mesh_dense_scratch_bytes(const TsdfParams* params, uint32_t max_vertices);
+int nuif_mesh_dense(const void* volume, const TsdfParams* params, int storage, uint32_t max_vertices, float* points,
+                    uint32_t* indices, uint32_t* n_points, uint32_t* n_indices, void* scratch, nuif_stream_t s);
+
 /* ---- voxel hashing (OCL/NuiKinfuOpenCLHashScene.cpp, K/KinfuHashScene.cl,
  * K/KinfuHashRaycast.cl, K/KinfuHashVisualization.cl) ----------------------------
  * One device allocation holds the whole scene: header, hash table (2^b buckets of

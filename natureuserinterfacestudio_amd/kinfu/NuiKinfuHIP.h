@@ -188,7 +188,8 @@ public:
     void raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) override;
     // K/marchingCube.cl via nuif_mc_dense: triangle soup (3 vertices per triangle)
     bool Volume2CLMesh(NuiKinfuMesh* pData) override;
-    bool Volume2Mesh(NuiKinfuMesh* pMesh) override { return Volume2CLMesh(pMesh); }
+    // OCL/NuiKinfuOpenCLScene.cpp:788-984 via nuif_mesh_dense: shared vertices + indices
+    bool Volume2Mesh(NuiKinfuMesh* pMesh) override;
     bool NeedsRayLengthDepth() const override { return true; }
     void* DevicePointer() const override { return m_volume.get(); }
     size_t DeviceBytes() const override { return m_volume.bytes(); }

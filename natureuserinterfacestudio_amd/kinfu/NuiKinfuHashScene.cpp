@@ -127,4 +127,33 @@ bool NuiKinfuHIPScene::Volume2CLMesh(NuiKinfuMesh* pData) {
     });
 }
 
+bool NuiKinfuHIPScene::Volume2Mesh(NuiKinfuMesh* pMesh) {
+    if (!pMesh) return false;
+    // count first (a zero-capacity call reports the corner count), then build
+    DeviceBuffer total(sizeof(uint32_t));
+    DeviceBuffer scratch(nuif_mc_scratch_bytes(&m_params));
+    check(nuif_mc_dense(m_volume.get(), nullptr, &m_params, m_storage, nullptr, nullptr, 0, total.as<uint32_t>(),
+                        scratch.get(), m_stream),
+          "nuif_mc_dense");
+    uint32_t n = 0;
+    check(nuif_memcpy_d2h(&n, total.get(), sizeof(n), m_stream), "mesh count");
+    check(nuif_stream_synchronize(m_stream), "mesh count");
+    pMesh->positions.clear();
+    pMesh->colors.clear();
+    pMesh->indices.clear();
+    if (n == 0) return true;
+    DeviceBuffer pts((size_t)n * 3 * sizeof(float)), idx((size_t)n * sizeof(uint32_t));
+    DeviceBuffer work(nuif_mesh_dense_scratch_bytes(&m_params, n));
+    uint32_t np = 0, ni = 0;
+    check(nuif_mesh_dense(m_volume.get(), &m_params, m_storage, n, pts.as<float>(), idx.as<uint32_t>(), &np, &ni,
+                          work.get(), m_stream),
+          "nuif_mesh_dense");
+    pMesh->positions.resize((size_t)np * 3);
+    pMesh->indices.resize(ni);
+    check(nuif_memcpy_d2h(pMesh->positions.data(), pts.get(), (size_t)np * 3 * sizeof(float), m_stream), "mesh read");
+    check(nuif_memcpy_d2h(pMesh->indices.data(), idx.get(), (size_t)ni * sizeof(uint32_t), m_stream), "mesh read");
+    check(nuif_stream_synchronize(m_stream), "mesh read");
+    return true;
+}
+
 }  // namespace nuif_kinfu

@@ -153,6 +153,9 @@ _SIGS = {
     "nuif_mc_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
     "nuif_mc_dense": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p, C.c_uint32,
                               c_void_p, c_void_p, c_void_p]),
+    "nuif_mesh_dense_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams), C.c_uint32]),
+    "nuif_mesh_dense": (c_int, [c_void_p, C.POINTER(TsdfParams), c_int, C.c_uint32, c_void_p, c_void_p,
+                                C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), c_void_p, c_void_p]),
     "nuif_hash_scene_bytes": (c_size_t, [C.POINTER(HashParams)]),
     "nuif_hash_layout": (c_int, [C.POINTER(HashParams), C.POINTER(c_size_t)]),
     "nuif_hash_reset": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p]),

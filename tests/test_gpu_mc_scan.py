@@ -98,3 +98,38 @@ def test_marching_cubes_matches_oracle(gpu, res):
     # capped output writes only `capacity` vertices but reports the full count
     nuif.check(L.nuif_mc_dense(dvol.ptr, None, C.byref(p), 0, verts.ptr, None, 999, tot.ptr, scratch.ptr, None))
     assert int(tot.download()[0]) == total_ref
+
+
+@pytest.mark.parametrize("res", [128, 100])
+def test_indexed_mesh_matches_host_mesher(gpu, res):
+    """nuif_mesh_dense against the restated host mesher (Volume2Mesh): the same points
+    in the same first-use order and the same index list, bit for bit."""
+    import dataclasses
+    announce(f"indexed mesh {res}^3 after 3 frames vs host mesher")
+    L = sig(gpu)
+    wl = dataclasses.replace(W.SMALL, volume_res=res)
+    vol = _oracle_volume(wl, 3)
+    N = res
+    nidx_ref = C.c_longlong()
+    cap = 4_000_000
+    pts_ref = np.empty((cap, 3), np.float32)
+    idx_ref = np.empty(cap, np.int32)
+    npts_ref = O.lib().or_mc_mesh_dedup_s16(vol, N, wl.volume_dim, pts_ref, cap, idx_ref, cap, C.byref(nidx_ref))
+    nidx_ref = nidx_ref.value
+    assert 3000 < npts_ref < nidx_ref <= cap
+    p = nuif.tsdf_params(N, wl.volume_dim, wl.trunc)
+    dvol = nuif.DeviceArray.from_host(vol)
+    m = nidx_ref + 16
+    pts = nuif.DeviceArray((m, 3), np.float32)
+    idx = nuif.DeviceArray((m,), np.uint32)
+    scratch = nuif.DeviceArray((L.nuif_mesh_dense_scratch_bytes(C.byref(p), m),), np.uint8)
+    npts, nidx = C.c_uint32(), C.c_uint32()
+    nuif.check(L.nuif_mesh_dense(dvol.ptr, C.byref(p), 0, m, pts.ptr, idx.ptr, C.byref(npts), C.byref(nidx),
+                                 scratch.ptr, None))
+    assert nidx.value == nidx_ref and npts.value == npts_ref
+    np.testing.assert_array_equal(idx.download()[:nidx_ref], idx_ref[:nidx_ref].astype(np.uint32))
+    np.testing.assert_array_equal(pts.download()[:npts_ref], pts_ref[:npts_ref])
+    # too small a capacity: refused, with the needed count reported
+    rc = L.nuif_mesh_dense(dvol.ptr, C.byref(p), 0, 100, pts.ptr, idx.ptr, C.byref(npts), C.byref(nidx),
+                           scratch.ptr, None)
+    assert rc == -1 and nidx.value == nidx_ref
