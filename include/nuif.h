@@ -321,6 +321,21 @@ int nuif_hash_integrate(void* scene, const nuif_hash_params* p, const float* dep
 int nuif_hash_raycast(void* scene, const nuif_hash_params* p, const NuiCLCameraParams* cam,
                       const NuiCLRigidTransform* pose, float* range2, float* vmap, float* nmap, uint8_t* cmap,
                       const int32_t* enable, nuif_stream_t s);
+/* Workspace of nuif_hash_forward_render for a W x H image. */
+size_t nuif_hash_forward_render_bytes(int width, int height);
+/* ForwardRender (OCL/NuiKinfuOpenCLHashScene.cpp:991-1012) after the ray ranges of
+ * nuif_hash_raycast: the points of the previous world vertex map vmap (and colours cmap,
+ * may be NULL) move to the pixels they project to at `pose` (forwardProject_kernel,
+ * K/KinfuHashRaycast.cl:473-506; the nearest point wins a pixel); pixels that received
+ * none and have a non-empty ray range are raycast (findMissingPoints_kernel :508-529,
+ * genericRaycastMissingPoints_kernel :531-559); normals are recomputed from the vertex
+ * map with a 0.3 m threshold (Vertex2Normal, :943-989).  vmap and cmap are updated in
+ * place, nmap is overwritten.  reference_transform != 0 projects as the reference does
+ * (camera->world applied to the world points, scaled by the voxel size); 0 uses
+ * world->camera.  work: nuif_hash_forward_render_bytes(W, H) bytes of device memory. */
+int nuif_hash_forward_render(void* scene, const nuif_hash_params* p, const NuiCLCameraParams* cam,
+                             const NuiCLRigidTransform* pose, float* range2, float* vmap, float* nmap, uint8_t* cmap,
+                             void* work, int reference_transform, const int32_t* enable, nuif_stream_t s);
 size_t nuif_hash_mesh_scratch_bytes(const nuif_hash_params* p);
 /* K/KinfuHashVisualization.cl:88-271 meshHashScene_kernel over the visible list: count,
  * scan, emit; vertices float3 in metres (3 per triangle), colours float4 (may be NULL);

@@ -50,10 +50,14 @@ typedef struct nuifk_desc {
     float hash_voxel_size, hash_trunc_scale, hash_truncation;
     int hash_weight_max;
     int hash_bricks;                /* brick pool (m_numSDFBlocks) */
+    int hash_forward_render;        /* NUIFK_RENDER_*: how the hash scene renders the ICP maps */
 } nuifk_desc;
 
 #define NUIFK_SCENE_DENSE 0         /* OCL/NuiKinfuOpenCLScene: dense TSDF volume */
 #define NUIFK_SCENE_HASH 1          /* OCL/NuiKinfuOpenCLHashScene: voxel hashing */
+#define NUIFK_RENDER_CASTRAY 0      /* castRay at every pixel (m_bUseForwardRender false) */
+#define NUIFK_RENDER_FORWARD 1      /* ForwardRender, world->camera projection */
+#define NUIFK_RENDER_FORWARD_REFERENCE 2 /* ForwardRender with the reference's camera->world projection */
 
 const char* nuifk_last_error(void);
 

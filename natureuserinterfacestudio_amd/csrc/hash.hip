@@ -515,19 +515,13 @@ __device__ __forceinline__ float rd(const Scene& s, BrickCache& c, int x, int y,
     return v;
 }
 
+// castRay of pixel (x, y): vertex (metres, world), normal (nmap may be null) and the
+// nearest corner's colour (cmap may be null)
 template <class V>
-__global__ void __launch_bounds__(64)
-    hash_raycast_kernel(Scene s, const float2* __restrict__ range, NuiCLCameraParams cam,
-                        const NuiCLRigidTransform* __restrict__ pose, float vs, float truncation,
-                        float* __restrict__ vmap, float* __restrict__ nmap, uchar4* __restrict__ cmap,
-                        const int32_t* __restrict__ enable) {
-    if (enable && *enable == 0) return;
-    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
-    // 8x8 pixel tiles: neighbouring rays share bricks
-    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
-    if (x >= W || y >= H) return;
-    const NuiCLRigidTransform g = *pose;
-    const size_t pid = (size_t)y * W + x;
+__device__ void cast_ray(const Scene& s, const float2* __restrict__ range, const NuiCLCameraParams& cam,
+                         const NuiCLRigidTransform& g, float vs, float truncation, int x, int y,
+                         float* __restrict__ vmap, float* __restrict__ nmap, uchar4* __restrict__ cmap) {
+    const size_t pid = (size_t)y * cam.depthImageWidth + x;
     const float ivs = 1.0f / vs;
     const float step_scale = truncation * ivs * 0.6f;
     const float2 mm = range[pid];
@@ -631,6 +625,161 @@ __global__ void __launch_bounds__(64)
         st3(nmap, pid, normalize3(ret));
     }
     if (cmap) cmap[pid] = col;
+}
+
+// K/KinfuHashRaycast.cl:450-471 raycast_kernel: castRay at every pixel, 8x8 pixel tiles
+// per wave so that neighbouring rays share bricks
+template <class V>
+__global__ void __launch_bounds__(64)
+    hash_raycast_kernel(Scene s, const float2* __restrict__ range, NuiCLCameraParams cam,
+                        const NuiCLRigidTransform* __restrict__ pose, float vs, float truncation,
+                        float* __restrict__ vmap, float* __restrict__ nmap, uchar4* __restrict__ cmap,
+                        const int32_t* __restrict__ enable) {
+    if (enable && *enable == 0) return;
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    if (x >= W || y >= H) return;
+    cast_ray<V>(s, range, cam, *pose, vs, truncation, x, y, vmap, nmap, cmap);
+}
+
+// ---- forward render (OCL/NuiKinfuOpenCLHashScene.cpp:991-1012 ForwardRender) -------
+//
+// The reference moves each point of the previous vertex map to the pixel it projects
+// to, then casts rays only where no point landed and the ray range is not empty.  Here
+// the projection is a z-buffer: every point does one 64-bit atomicMin of
+// (ordered z << 32 | source pixel) on its target pixel, so the nearest point wins
+// (smallest source index on ties) whatever the order of the writes, which the
+// reference leaves to a race.  The pixels to raycast are appended per wave (ballot +
+// one atomic) to a list that a grid-stride castRay kernel consumes; each pixel's result
+// depends on that pixel only, so the list order does not matter.
+
+constexpr unsigned long long kNoPoint = ~0ull;
+
+// float -> uint32 key that orders as the float does (negative values included)
+__device__ __forceinline__ uint32_t ordered_z(float z) {
+    const uint32_t b = __float_as_uint(z);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+// float -> uint32 as the GPU converts: round toward zero, saturating, NaN -> 0
+__device__ __forceinline__ uint32_t cvt_u32_sat(float v) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)v;
+}
+
+__global__ void hash_fwd_clear_kernel(unsigned long long* __restrict__ keys, int n, uint32_t* __restrict__ count,
+                                      const int32_t* __restrict__ enable) {
+    if (enable && *enable == 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *count = 0;
+    if (i < n) keys[i] = kNoPoint;
+}
+
+// K/KinfuHashRaycast.cl:473-506 forwardProject_kernel.  reference_transform: the
+// reference's projection (camera->world applied to the world point, scaled by the voxel
+// size; uint conversion of p + 0.5); otherwise world->camera, z > 0 and p + 0.5 >= 0.
+__global__ void __launch_bounds__(256)
+    hash_fwd_splat_kernel(const float* __restrict__ vmap, NuiCLCameraParams cam,
+                          const NuiCLRigidTransform* __restrict__ pose, float vs, int reference_transform,
+                          unsigned long long* __restrict__ keys, const int32_t* __restrict__ enable) {
+    if (enable && *enable == 0) return;
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= W * H) return;
+    const f3 v = ld3(vmap, i);
+    if (nan3(v)) return;
+    const NuiCLRigidTransform g = *pose;
+    f3 q;
+    uint32_t ux, uy;
+    if (reference_transform) {
+        q = mul3(xform(g, v), vs);
+        ux = cvt_u32_sat(q.x * cam.fx / q.z + cam.cx + 0.5f);
+        uy = cvt_u32_sat(q.y * cam.fy / q.z + cam.cy + 0.5f);
+    } else {
+        q = xform_inv(g, v);
+        if (!(q.z > 0.0f)) return;
+        const float sx = q.x * cam.fx / q.z + cam.cx + 0.5f, sy = q.y * cam.fy / q.z + cam.cy + 0.5f;
+        if (!(sx >= 0.0f && sy >= 0.0f)) return;
+        ux = cvt_u32_sat(sx);
+        uy = cvt_u32_sat(sy);
+    }
+    if (ux >= (uint32_t)W || uy >= (uint32_t)H) return;
+    atomicMin(&keys[(size_t)uy * W + ux], ((unsigned long long)ordered_z(q.z) << 32) | (uint32_t)i);
+}
+
+// The forwarded point of each pixel (NaN where none landed) into fwd, and its colour
+// (the pixel's own where none landed) into the low half of the pixel's key
+__global__ void __launch_bounds__(256)
+    hash_fwd_gather_kernel(const float* __restrict__ vmap, const uchar4* __restrict__ cmap, float* __restrict__ fwd,
+                           unsigned long long* __restrict__ keys, int n, const int32_t* __restrict__ enable) {
+    if (enable && *enable == 0) return;
+    const int d = blockIdx.x * 256 + threadIdx.x;
+    if (d >= n) return;
+    const unsigned long long k = keys[d];
+    const bool hit = k != kNoPoint;
+    const uint32_t src = hit ? (uint32_t)k : (uint32_t)d;
+    st3(fwd, d, hit ? ld3(vmap, src) : mk3(kNaN, kNaN, kNaN));
+    if (cmap) {
+        const uchar4 c = cmap[src];
+        keys[d] = (unsigned long long)((uint32_t)c.x | ((uint32_t)c.y << 8) | ((uint32_t)c.z << 16) |
+                                       ((uint32_t)c.w << 24));
+    }
+}
+
+// K/KinfuHashRaycast.cl:508-529 findMissingPoints_kernel: keep the forwarded point, or
+// list the pixel for a ray when none landed and its ray range is not empty.  8x8 tiles
+// per wave keep the listed pixels of a wave close together.
+__global__ void __launch_bounds__(64)
+    hash_fwd_missing_kernel(const float* __restrict__ fwd, const float2* __restrict__ range, float* __restrict__ vmap,
+                            uchar4* __restrict__ cmap, const unsigned long long* __restrict__ keys, int W, int H,
+                            uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                            const int32_t* __restrict__ enable) {
+    if (enable && *enable == 0) return;
+    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const size_t d = (size_t)y * W + x;
+    bool needed = false;
+    if (x < W && y < H) {
+        const f3 f = ld3(fwd, d);
+        const float2 mm = range[d];
+        needed = nan3(f) && mm.x < mm.y;
+        if (!needed) st3(vmap, d, f);
+        if (cmap) {
+            const uint32_t c = (uint32_t)keys[d];
+            cmap[d] = make_uchar4(c & 255u, (c >> 8) & 255u, (c >> 16) & 255u, c >> 24);
+        }
+    }
+    const unsigned long long mask = __ballot(needed);
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0 && mask) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, 0, 64);
+    if (needed) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)d;
+}
+
+// K/KinfuHashRaycast.cl:531-559 genericRaycastMissingPoints_kernel: castRay (no normal)
+// at the listed pixels
+template <class V>
+__global__ void __launch_bounds__(64)
+    hash_fwd_raycast_kernel(Scene s, const float2* __restrict__ range, NuiCLCameraParams cam,
+                            const NuiCLRigidTransform* __restrict__ pose, float vs, float truncation,
+                            float* __restrict__ vmap, uchar4* __restrict__ cmap, const uint32_t* __restrict__ list,
+                            const uint32_t* __restrict__ count, const int32_t* __restrict__ enable) {
+    if (enable && *enable == 0) return;
+    const uint32_t n = *count, W = cam.depthImageWidth;
+    const NuiCLRigidTransform g = *pose;
+    for (uint32_t k = blockIdx.x * 64 + threadIdx.x; k < n; k += gridDim.x * 64) {
+        const uint32_t d = list[k];
+        cast_ray<V>(s, range, cam, g, vs, truncation, (int)(d % W), (int)(d / W), vmap, nullptr, cmap);
+    }
+}
+
+// OCL/NuiKinfuOpenCLHashScene.cpp:943-989 Vertex2Normal (threshold 0.3 m)
+__global__ void hash_fwd_normals_kernel(const float* __restrict__ vmap, float* __restrict__ nmap, int W, int H,
+                                        const int32_t* __restrict__ enable) {
+    if (enable && *enable == 0) return;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    st3(nmap, (size_t)y * W + x, normal_simple_at(vmap, x, y, W, H, 0.3f));
 }
 
 // ---- marching cubes (K/KinfuHashVisualization.cl:88-271) -------------------------
@@ -876,6 +1025,40 @@ int nuif_hash_integrate(void* scene, const nuif_hash_params* p, const float* dep
     return NUIF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// resetExpectedRange + CreateExpectedDepths (OCL/NuiKinfuOpenCLHashScene.cpp:1030-1040)
+int launch_ranges(const Scene& sc, const nuif_hash_params* p, const NuiCLCameraParams* cam,
+                  const NuiCLRigidTransform* pose, float2* r2, const int32_t* enable, hipStream_t st) {
+    const int W = (int)cam->depthImageWidth, H = (int)cam->depthImageHeight;
+    hash_range_init_kernel<<<div_up((size_t)W * H, 256), 256, 0, st>>>(r2, W * H, enable);
+    NUIF_LAUNCHED("hash_range_init_kernel");
+    hash_range_kernel<<<kWalkGrid, 64, 0, st>>>(sc, r2, *cam, pose, p->voxel_size, enable);
+    NUIF_LAUNCHED("hash_range_kernel");
+    return NUIF_OK;
+}
+
+struct FwdWork {
+    unsigned long long* keys;
+    uint32_t* list;
+    uint32_t* count;
+    size_t bytes;
+};
+FwdWork fwd_work(void* base, size_t n) {
+    FwdWork w;
+    char* b = static_cast<char*>(base);
+    const size_t keys = align256(n * 8), list = align256(n * 4);
+    w.keys = reinterpret_cast<unsigned long long*>(b);
+    w.list = reinterpret_cast<uint32_t*>(b + keys);
+    w.count = reinterpret_cast<uint32_t*>(b + keys + list);
+    w.bytes = keys + list + 256;
+    return w;
+}
+}  // namespace
+
+extern "C" {
+
 int nuif_hash_raycast(void* scene, const nuif_hash_params* p, const NuiCLCameraParams* cam,
                       const NuiCLRigidTransform* pose, float* range2, float* vmap, float* nmap, uint8_t* cmap,
                       const int32_t* enable, nuif_stream_t s) {
@@ -888,10 +1071,8 @@ int nuif_hash_raycast(void* scene, const nuif_hash_params* p, const NuiCLCameraP
     Scene sc = scene_of(scene, *p);
     hipStream_t st = stream_of(s);
     float2* r2 = reinterpret_cast<float2*>(range2);
-    hash_range_init_kernel<<<div_up((size_t)W * H, 256), 256, 0, st>>>(r2, W * H, enable);
-    NUIF_LAUNCHED("hash_range_init_kernel");
-    hash_range_kernel<<<kWalkGrid, 64, 0, st>>>(sc, r2, *cam, pose, p->voxel_size, enable);
-    NUIF_LAUNCHED("hash_range_kernel");
+    rc = launch_ranges(sc, p, cam, pose, r2, enable, st);
+    if (rc) return rc;
     const dim3 grid(div_up(W, 8), div_up(H, 8));
     uchar4* c4 = reinterpret_cast<uchar4*>(cmap);
     if (p->storage == NUIF_TSDF_S16)
@@ -901,6 +1082,55 @@ int nuif_hash_raycast(void* scene, const nuif_hash_params* p, const NuiCLCameraP
         hash_raycast_kernel<VF16><<<grid, 64, 0, st>>>(sc, r2, *cam, pose, p->voxel_size, p->truncation, vmap, nmap,
                                                         c4, enable);
     NUIF_LAUNCHED("hash_raycast_kernel");
+    return NUIF_OK;
+}
+
+size_t nuif_hash_forward_render_bytes(int width, int height) {
+    if (width <= 0 || height <= 0) return 0;
+    return fwd_work(nullptr, (size_t)width * height).bytes;
+}
+
+int nuif_hash_forward_render(void* scene, const nuif_hash_params* p, const NuiCLCameraParams* cam,
+                             const NuiCLRigidTransform* pose, float* range2, float* vmap, float* nmap, uint8_t* cmap,
+                             void* work, int reference_transform, const int32_t* enable, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(p);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(scene && cam && pose && range2 && vmap && nmap && work,
+                   "nuif_hash_forward_render: null argument");
+    NUIF_CHECK_ARG(vmap != nmap, "nuif_hash_forward_render: vmap and nmap alias");
+    const int W = (int)cam->depthImageWidth, H = (int)cam->depthImageHeight;
+    NUIF_CHECK_ARG(W > 0 && H > 0, "nuif_hash_forward_render: empty image");
+    const int n = W * H;
+    Scene sc = scene_of(scene, *p);
+    hipStream_t st = stream_of(s);
+    float2* r2 = reinterpret_cast<float2*>(range2);
+    uchar4* c4 = reinterpret_cast<uchar4*>(cmap);
+    const FwdWork w = fwd_work(work, (size_t)n);
+    rc = launch_ranges(sc, p, cam, pose, r2, enable, st);
+    if (rc) return rc;
+    hash_fwd_clear_kernel<<<div_up((size_t)n, 256), 256, 0, st>>>(w.keys, n, w.count, enable);
+    NUIF_LAUNCHED("hash_fwd_clear_kernel");
+    hash_fwd_splat_kernel<<<div_up((size_t)n, 256), 256, 0, st>>>(vmap, *cam, pose, p->voxel_size,
+                                                                  reference_transform, w.keys, enable);
+    NUIF_LAUNCHED("hash_fwd_splat_kernel");
+    // the forwarded points go to nmap, which the normals overwrite at the end
+    hash_fwd_gather_kernel<<<div_up((size_t)n, 256), 256, 0, st>>>(vmap, c4, nmap, w.keys, n, enable);
+    NUIF_LAUNCHED("hash_fwd_gather_kernel");
+    const dim3 tiles(div_up(W, 8), div_up(H, 8));
+    hash_fwd_missing_kernel<<<tiles, 64, 0, st>>>(nmap, r2, vmap, c4, w.keys, W, H, w.list, w.count, enable);
+    NUIF_LAUNCHED("hash_fwd_missing_kernel");
+    const int rgrid = (int)std::min<size_t>(div_up((size_t)n, 64), 2048);
+    if (p->storage == NUIF_TSDF_S16)
+        hash_fwd_raycast_kernel<VS16><<<rgrid, 64, 0, st>>>(sc, r2, *cam, pose, p->voxel_size, p->truncation, vmap,
+                                                             c4, w.list, w.count, enable);
+    else
+        hash_fwd_raycast_kernel<VF16><<<rgrid, 64, 0, st>>>(sc, r2, *cam, pose, p->voxel_size, p->truncation, vmap,
+                                                             c4, w.list, w.count, enable);
+    NUIF_LAUNCHED("hash_fwd_raycast_kernel");
+    const dim3 block(16, 16), grid(div_up(W, 16), div_up(H, 16));
+    hash_fwd_normals_kernel<<<grid, block, 0, st>>>(vmap, nmap, W, H, enable);
+    NUIF_LAUNCHED("hash_fwd_normals_kernel");
     return NUIF_OK;
 }
 

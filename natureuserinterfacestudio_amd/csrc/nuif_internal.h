@@ -93,4 +93,24 @@ __device__ __forceinline__ f3 rot3(const float* R, f3 v) {
 
 constexpr float kNaN = __builtin_nanf("");
 
+// K/normalEstimation.cl:770-822 estimate_normals_simple_kernel at pixel (x, y): the
+// normalised cross product of the right-left and down-up differences; a neighbour
+// outside the image, NaN or more than thr away in z is replaced by the centre; NaN
+// centre -> NaN normal.
+__device__ __forceinline__ f3 normal_simple_at(const float* __restrict__ vmap, int x, int y, int W, int H,
+                                               float thr) {
+    const size_t i = (size_t)y * W + x;
+    const f3 c = ld3(vmap, i);
+    if (nan3(c)) return mk3(kNaN, kNaN, kNaN);
+    auto pick = [&](bool ok, size_t j) {
+        if (!ok) return c;
+        const f3 n = ld3(vmap, j);
+        if (nan3(n) || fabsf(c.z - n.z) > thr) return c;
+        return n;
+    };
+    const f3 l = pick(x > 0, i - 1), rr = pick(x < W - 1, i + 1);
+    const f3 u = pick(y > 0, i - W), d = pick(y < H - 1, i + W);
+    return normalize3(cross3(sub3(rr, l), sub3(d, u)));
+}
+
 }  // namespace nuif

@@ -167,21 +167,7 @@ __global__ void normals_simple_kernel(const float* __restrict__ vmap, float* __r
                                       float thr) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= W || y >= H) return;
-    const size_t i = (size_t)y * W + x;
-    const f3 c = ld3(vmap, i);
-    if (nan3(c)) {
-        st3(nmap, i, mk3(kNaN, kNaN, kNaN));
-        return;
-    }
-    auto pick = [&](bool ok, size_t j) {
-        if (!ok) return c;
-        const f3 n = ld3(vmap, j);
-        if (nan3(n) || fabsf(c.z - n.z) > thr) return c;
-        return n;
-    };
-    const f3 l = pick(x > 0, i - 1), rr = pick(x < W - 1, i + 1);
-    const f3 u = pick(y > 0, i - W), d = pick(y < H - 1, i + W);
-    st3(nmap, i, normalize3(cross3(sub3(rr, l), sub3(d, u))));
+    st3(nmap, (size_t)y * W + x, normal_simple_at(vmap, x, y, W, H, thr));
 }
 
 // K/transformMaps.cl:91-127 transform_maps_kernel (in-place safe per pixel)

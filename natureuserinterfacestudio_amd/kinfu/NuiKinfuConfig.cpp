@@ -119,7 +119,8 @@ NuiHashingSDFConfig::NuiHashingSDFConfig()
       m_integrationWeightSample(3),
       m_integrationWeightMax(255),
       m_bUseSwapping(false),
-      m_bUseForwardRender(false) {}
+      m_bUseForwardRender(false),
+      m_bForwardReferenceProjection(false) {}
 
 int NuiHashingSDFConfig::bucketsLog2() const {
     int b = 4;

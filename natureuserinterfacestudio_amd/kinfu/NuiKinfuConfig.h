@@ -68,6 +68,9 @@ struct NuiHashingSDFConfig {
     UINT m_integrationWeightMax;
     bool m_bUseSwapping;
     bool m_bUseForwardRender;
+    // Not in the reference: true projects the forward render as forwardProject_kernel
+    // does (camera->world transform applied to world points); false uses world->camera.
+    bool m_bForwardReferenceProjection;
 
     NuiHashingSDFConfig();
     int bucketsLog2() const;

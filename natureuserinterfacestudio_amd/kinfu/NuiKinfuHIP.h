@@ -241,7 +241,7 @@ private:
     NuiHashingSDFConfig m_config;
     nuif_hash_params m_params{};
     nuif_stream_t m_stream;
-    DeviceBuffer m_scene, m_range, m_counter;
+    DeviceBuffer m_scene, m_range, m_counter, m_forwardWork;
     UINT m_rangeW = 0, m_rangeH = 0;
 };
 

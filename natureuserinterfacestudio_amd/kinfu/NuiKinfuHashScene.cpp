@@ -58,7 +58,8 @@ bool NuiKinfuHIPHashScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCamera
 }
 
 // OCL/NuiKinfuOpenCLHashScene.cpp:1014-1064 raycastRender: ray ranges of the
-// visible blocks, then castRay into the feedback frame's maps
+// visible blocks, then the forward render (m_bUseForwardRender) or castRay at every
+// pixel into the feedback frame's maps
 void NuiKinfuHIPHashScene::raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) {
     auto* fb = dynamic_cast<NuiKinfuHIPFeedbackFrame*>(pFeedbackFrame);
     auto* cam = dynamic_cast<NuiKinfuHIPCameraState*>(pCameraState);
@@ -67,8 +68,18 @@ void NuiKinfuHIPHashScene::raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, 
         m_rangeW = fb->GetWidth();
         m_rangeH = fb->GetHeight();
         m_range = DeviceBuffer((size_t)m_rangeW * m_rangeH * 2 * sizeof(float));
+        if (m_config.m_bUseForwardRender)
+            m_forwardWork = DeviceBuffer(nuif_hash_forward_render_bytes((int)m_rangeW, (int)m_rangeH));
     }
     const NuiCLCameraParams& c = cam->GetCameraParams();
+    if (m_config.m_bUseForwardRender) {
+        check(nuif_hash_forward_render(m_scene.get(), &m_params, &c, cam->GetCameraTransformBuffer(),
+                                       m_range.as<float>(), fb->GetVertexBuffer(), fb->GetNormalBuffer(),
+                                       m_params.color ? fb->GetColorBuffer() : nullptr, m_forwardWork.get(),
+                                       m_config.m_bForwardReferenceProjection ? 1 : 0, m_gateRaycast, m_stream),
+              "nuif_hash_forward_render");
+        return;
+    }
     check(nuif_hash_raycast(m_scene.get(), &m_params, &c, cam->GetCameraTransformBuffer(), m_range.as<float>(),
                             fb->GetVertexBuffer(), fb->GetNormalBuffer(),
                             m_params.color ? fb->GetColorBuffer() : nullptr, m_gateRaycast, m_stream),

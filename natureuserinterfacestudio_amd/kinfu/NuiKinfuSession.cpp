@@ -33,6 +33,8 @@ std::unique_ptr<NuiKinfuSession> NuiKinfuSession::create(const nuifk_desc& d) {
     s->engine->initialize(trackerConfig(d), false, (UINT)d.width, (UINT)d.height);
     s->engine->setUseGraph(d.use_graph != 0);
     s->engine->tracker()->SetReduction(d.icp_reduction);
+    if (d.hash_forward_render < NUIFK_RENDER_CASTRAY || d.hash_forward_render > NUIFK_RENDER_FORWARD_REFERENCE)
+        throw std::runtime_error("session: unknown hash_forward_render mode");
     if (d.scene == NUIFK_SCENE_HASH)
         s->scene = std::make_unique<NuiKinfuHIPHashScene>(hashConfig(d), d.storage, d.color != 0, s->engine->stream());
     else if (d.scene == NUIFK_SCENE_DENSE)
@@ -54,6 +56,8 @@ NuiHashingSDFConfig NuiKinfuSession::hashConfig(const nuifk_desc& d) {
     c.m_truncation = d.hash_truncation;
     c.m_integrationWeightMax = (UINT)d.hash_weight_max;
     c.m_numSDFBlocks = (UINT)d.hash_bricks;
+    c.m_bUseForwardRender = d.hash_forward_render != NUIFK_RENDER_CASTRAY;
+    c.m_bForwardReferenceProjection = d.hash_forward_render == NUIFK_RENDER_FORWARD_REFERENCE;
     return c;
 }
 
@@ -63,6 +67,7 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
     if (d.scene == NUIFK_SCENE_HASH) {
         const nuif_hash_params hp = NuiKinfuHIPHashScene::ParamsOf(hashConfig(d), d.storage, d.color != 0);
         bytes += nuif_hash_scene_bytes(&hp) + n * 8;  // + ray-range image
+        if (d.hash_forward_render != NUIFK_RENDER_CASTRAY) bytes += nuif_hash_forward_render_bytes(d.width, d.height);
     } else {
         const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
         bytes += voxels * nuif_tsdf_voxel_bytes(d.storage) + (d.color ? voxels * 4 : 0);

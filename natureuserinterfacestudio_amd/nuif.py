@@ -73,10 +73,11 @@ class KinfuDesc(C.Structure):
                 ("dimensions", c_float * 3), ("tranc_dist", c_float), ("color", c_int), ("storage", c_int),
                 ("use_graph", c_int), ("icp_reduction", c_int), ("scene", c_int), ("hash_voxel_size", c_float),
                 ("hash_trunc_scale", c_float), ("hash_truncation", c_float), ("hash_weight_max", c_int),
-                ("hash_bricks", c_int)]
+                ("hash_bricks", c_int), ("hash_forward_render", c_int)]
 
 
 SCENE_DENSE, SCENE_HASH = 0, 1
+RENDER_CASTRAY, RENDER_FORWARD, RENDER_FORWARD_REFERENCE = 0, 1, 2
 
 
 _SIGS = {
@@ -165,6 +166,10 @@ _SIGS = {
                                     c_void_p, c_void_p, c_void_p]),
     "nuif_hash_raycast": (c_int, [c_void_p, C.POINTER(HashParams), C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nuif_hash_forward_render_bytes": (c_size_t, [c_int, c_int]),
+    "nuif_hash_forward_render": (c_int, [c_void_p, C.POINTER(HashParams), C.POINTER(NuiCLCameraParams), c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                         c_void_p]),
     "nuif_hash_mesh_scratch_bytes": (c_size_t, [C.POINTER(HashParams)]),
     "nuif_hash_mesh": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, c_void_p, C.c_uint32, c_void_p, c_void_p, c_void_p]),
     "nuif_hash_stats": (c_int, [c_void_p, C.POINTER(C.c_uint32), c_void_p]),
@@ -596,6 +601,21 @@ class HashScene:
         cm = DeviceArray((n, 4), np.uint8) if color else None
         check(L().nuif_hash_raycast(self.mem.ptr, C.byref(self.p), C.byref(cam), pose.ptr, rg.ptr, v.ptr, nm.ptr,
                                     cm.ptr if cm else None, None, self.stream), "nuif_hash_raycast")
+        return v.download(), nm.download(), rg.download(), cm.download() if cm else None
+
+    def forward_render(self, cam, pose, vmap, cmap=None, reference_transform=False):
+        """Forward render from the previous world vertex map `vmap` (and colours `cmap`)
+        at `pose`: returns (vmap, nmap, range, cmap) as host arrays."""
+        W, H = cam.depthImageWidth, cam.depthImageHeight
+        n = W * H
+        rg = DeviceArray((n, 2), np.float32)
+        v = DeviceArray.from_host(np.ascontiguousarray(vmap, np.float32).reshape(n, 3))
+        nm = DeviceArray((n, 3), np.float32)
+        cm = DeviceArray.from_host(np.ascontiguousarray(cmap, np.uint8).reshape(n, 4)) if cmap is not None else None
+        work = DeviceArray((L().nuif_hash_forward_render_bytes(W, H),), np.uint8)
+        check(L().nuif_hash_forward_render(self.mem.ptr, C.byref(self.p), C.byref(cam), pose.ptr, rg.ptr, v.ptr,
+                                           nm.ptr, cm.ptr if cm else None, work.ptr, int(bool(reference_transform)),
+                                           None, self.stream), "nuif_hash_forward_render")
         return v.download(), nm.download(), rg.download(), cm.download() if cm else None
 
     def mesh(self, capacity=None, colors=False):

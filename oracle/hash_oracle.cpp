@@ -334,13 +334,20 @@ struct HashScene {
         }
     }
 
-    // K/KinfuHashRaycast.cl:155-448 castRay: world vertex / normal maps
+    // K/KinfuHashRaycast.cl:462-470 raycast_kernel: castRay at every pixel
     void raycast(const Cam<float>& cam, const Rigid<float>& g, float* vmap, float* nmap, uint8_t* cmap) const {
-        const float ivs = 1.0f / c.vs;
-        const float step_scale = c.truncation * ivs * 0.6f;
 #pragma omp parallel for schedule(dynamic, 4)
         for (int y = 0; y < cam.H; ++y)
-            for (int x = 0; x < cam.W; ++x) {
+            for (int x = 0; x < cam.W; ++x) cast_ray(cam, g, x, y, vmap, nmap, cmap);
+    }
+
+    // K/KinfuHashRaycast.cl:155-448 castRay: world vertex / normal / colour of pixel (x, y)
+    void cast_ray(const Cam<float>& cam, const Rigid<float>& g, int x, int y, float* vmap, float* nmap,
+                  uint8_t* cmap) const {
+        const float ivs = 1.0f / c.vs;
+        const float step_scale = c.truncation * ivs * 0.6f;
+        {
+            {
                 const size_t pid = (size_t)y * cam.W + x;
                 const float* mm = &range[2 * pid];
                 V3<float> ps = depth_to_skeleton(cam, x, y, mm[0]);
@@ -371,7 +378,7 @@ struct HashScene {
                     store3(vmap, pid, mk(qnan<float>(), qnan<float>(), qnan<float>()));
                     if (nmap) store3(nmap, pid, mk(qnan<float>(), qnan<float>(), qnan<float>()));
                     if (cmap) std::memset(cmap + 4 * pid, 0, 4);
-                    continue;
+                    return;
                 }
                 step = sdf * step_scale;
                 pt = pt + dir * step;
@@ -445,6 +452,85 @@ struct HashScene {
                 }
                 if (cmap) std::memcpy(cmap + 4 * pid, col, 4);
             }
+        }
+    }
+
+    // OCL/NuiKinfuOpenCLHashScene.cpp:991-1012 ForwardRender after the ray ranges:
+    //  * forwardProject_kernel (K/KinfuHashRaycast.cl:473-506) moves every valid point of
+    //    the previous vertex map (and its colour) to the pixel it projects to under the
+    //    current pose;
+    //  * findMissingPoints_kernel (:508-529) keeps the forwarded point where one landed and
+    //    flags the pixel for a ray where none did and its ray range is not empty;
+    //  * genericRaycastMissingPoints_kernel (:531-559) runs castRay (no normal) on the
+    //    flagged pixels;
+    //  * Vertex2Normal (:943-989) recomputes the normals from the vertex map (threshold 0.3).
+    // reference_transform selects the reference's projection (Q9): camera->world applied
+    // to world points, scaled by the voxel size, pixel = (uint)(p + 0.5) with negative and
+    // NaN values converting to 0 (the saturating float->uint conversion); otherwise the
+    // world->camera transform, z > 0 and p + 0.5 >= 0.  Where several points land on one
+    // pixel (a race in the reference) the nearest (smallest z, then smallest source index)
+    // wins.  Returns the number of flagged pixels.
+    static uint32_t zkey(float z) {
+        uint32_t b;
+        std::memcpy(&b, &z, 4);
+        return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    }
+    static uint32_t cvt_u32_sat(float v) {
+        if (!(v > 0.0f)) return 0u;
+        if (v >= 4294967296.0f) return 0xffffffffu;
+        return (uint32_t)v;
+    }
+    size_t forward_render(const Cam<float>& cam, const Rigid<float>& g, float* vmap, float* nmap, uint8_t* cmap,
+                          bool reference_transform) const {
+        const size_t n = (size_t)cam.W * cam.H;
+        std::vector<uint64_t> keys(n, ~0ull);
+        for (size_t i = 0; i < n; ++i) {
+            const V3<float> v = load3(vmap, i);
+            if (isnan3(v)) continue;
+            V3<float> q;
+            float sx, sy;
+            uint32_t ux, uy;
+            if (reference_transform) {
+                q = transform(g, v) * c.vs;
+                to_screen(cam, q, &sx, &sy);
+                ux = cvt_u32_sat(sx + 0.5f);
+                uy = cvt_u32_sat(sy + 0.5f);
+            } else {
+                q = transform_inverse(g, v);
+                if (!(q.z > 0.0f)) continue;
+                to_screen(cam, q, &sx, &sy);
+                sx += 0.5f;
+                sy += 0.5f;
+                if (!(sx >= 0.0f && sy >= 0.0f)) continue;
+                ux = cvt_u32_sat(sx);
+                uy = cvt_u32_sat(sy);
+            }
+            if (ux >= (uint32_t)cam.W || uy >= (uint32_t)cam.H) continue;
+            const size_t dst = (size_t)uy * cam.W + ux;
+            keys[dst] = std::min(keys[dst], ((uint64_t)zkey(q.z) << 32) | (uint64_t)i);
+        }
+        std::vector<float> fwd(3 * n);
+        std::vector<uint8_t> col(cmap ? 4 * n : 0);
+        for (size_t d = 0; d < n; ++d) {
+            const bool hit = keys[d] != ~0ull;
+            const size_t src = hit ? (size_t)(uint32_t)keys[d] : d;
+            store3(fwd.data(), d, hit ? load3(vmap, src) : mk(qnan<float>(), qnan<float>(), qnan<float>()));
+            if (cmap) std::memcpy(&col[4 * d], cmap + 4 * src, 4);
+        }
+        std::vector<size_t> needed;
+        for (size_t d = 0; d < n; ++d) {
+            const V3<float> f = load3(fwd.data(), d);
+            if (isnan3(f) && range[2 * d] < range[2 * d + 1])
+                needed.push_back(d);
+            else
+                store3(vmap, d, f);
+            if (cmap) std::memcpy(cmap + 4 * d, &col[4 * d], 4);
+        }
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long long k = 0; k < (long long)needed.size(); ++k)
+            cast_ray(cam, g, (int)(needed[k] % cam.W), (int)(needed[k] / cam.W), vmap, nullptr, cmap);
+        normals_simple<float>(vmap, nmap, cam.W, cam.H, 0.3f);
+        return needed.size();
     }
 
     // K/KinfuHashVisualization.cl:88-271 meshHashScene_kernel over the visible list
@@ -515,6 +601,7 @@ struct HashEngineCfg {
     int iters[8];
     float gate;
     HashCfg hc;
+    int forward;  // 0: castRay; 1: forward render; 2: forward render with the reference's transform
 };
 
 // src/SLAM/VisualOdometry/NuiKinfuTrackingEngine.cpp:68-142 with the hash scene:
@@ -611,7 +698,10 @@ struct HashEngine {
                 std::memcpy(tlast, t, sizeof(tlast));
             }
             scene.ray_ranges(cam, np);
-            scene.raycast(cam, np, vref.data(), nref.data(), nullptr);
+            if (c.forward)
+                scene.forward_render(cam, np, vref.data(), nref.data(), nullptr, c.forward == 2);
+            else
+                scene.raycast(cam, np, vref.data(), nref.data(), nullptr);
         }
         frames++;
         return 1;
@@ -669,7 +759,7 @@ OR_EXPORT unsigned or_hash_index(int x, int y, int z, unsigned mask) {
 OR_EXPORT void* or_hash_engine_create(int W, int H, const float* cam6, int filter_radius, float sigma_space,
                                       float sigma_depth, float depth_threshold, float dist_threshold, int levels,
                                       const int* iters, float gate, float vs, float trunc_scale, float truncation,
-                                      int wmax, int capacity) {
+                                      int wmax, int capacity, int forward) {
     HashEngineCfg c;
     c.W = W;
     c.H = H;
@@ -683,6 +773,7 @@ OR_EXPORT void* or_hash_engine_create(int W, int H, const float* cam6, int filte
     for (int l = 0; l < 8; ++l) c.iters[l] = l < levels ? iters[l] : 0;
     c.gate = gate;
     c.hc = HashCfg{vs, trunc_scale, truncation, wmax, capacity};
+    c.forward = forward;
     return new HashEngine(c);
 }
 
@@ -740,6 +831,21 @@ OR_EXPORT void or_hash_scene_raycast(void* sp, int W, int H, const float* cam6, 
     s->ray_ranges(cam, g);
     s->raycast(cam, g, vmap, nmap, cmap);
     if (range2) std::memcpy(range2, s->range.data(), s->range.size() * sizeof(float));
+}
+
+// Ray ranges + forward render at a pose.  vmap holds the previous world vertex map on
+// entry and the rendered one on return; nmap (W*H*3) receives the normals, range2 the
+// ray ranges, cmap (W*H*4 or NULL) is forwarded in place.  Returns the pixels raycast.
+OR_EXPORT long long or_hash_scene_forward_render(void* sp, int W, int H, const float* cam6, const float* R9,
+                                                 const float* t3, float* vmap, float* nmap, float* range2,
+                                                 uint8_t* cmap, int reference_transform) {
+    HashScene* s = (HashScene*)sp;
+    const Cam<float> cam = cam_of(cam6, W, H);
+    const Rigid<float> g = make_rigid<float>(R9, t3);
+    s->ray_ranges(cam, g);
+    const size_t n = s->forward_render(cam, g, vmap, nmap, cmap, reference_transform != 0);
+    if (range2) std::memcpy(range2, s->range.data(), s->range.size() * sizeof(float));
+    return (long long)n;
 }
 
 // Allocated blocks sorted by key: keys (int16 x3), sdf (int16 x512), weight (uint8 x512)

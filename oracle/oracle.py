@@ -65,7 +65,7 @@ def lib():
             "or_mc_table": (None, [np.ctypeslib.ndpointer(np.int8), np.ctypeslib.ndpointer(np.uint8)]),
             "or_hash_touched": (ci, [f32p, ci, ci, f32p, f32p, f32p, cf, cf, vp, cll]),
             "or_hash_index": (C.c_uint, [ci, ci, ci, C.c_uint]),
-            "or_hash_engine_create": (vp, [ci, ci, f32p, ci, cf, cf, cf, cf, ci, i32p, cf, cf, cf, cf, ci, ci]),
+            "or_hash_engine_create": (vp, [ci, ci, f32p, ci, cf, cf, cf, cf, ci, i32p, cf, cf, cf, cf, ci, ci, ci]),
             "or_hash_engine_destroy": (None, [vp]),
             "or_hash_engine_run": (ci, [vp, u16p, vp, f64p, f64p]),
             "or_hash_engine_scene": (vp, [vp]),
@@ -74,6 +74,7 @@ def lib():
             "or_hash_scene_destroy": (None, [vp]),
             "or_hash_scene_integrate": (cll, [vp, f32p, vp, ci, ci, f32p, f32p, f32p]),
             "or_hash_scene_raycast": (None, [vp, ci, ci, f32p, f32p, f32p, vp, vp, vp, vp]),
+            "or_hash_scene_forward_render": (cll, [vp, ci, ci, f32p, f32p, f32p, vp, vp, vp, vp, ci]),
             "or_hash_scene_blocks": (ci, [vp, vp, vp, vp, vp, cll]),
             "or_hash_scene_visible": (ci, [vp, vp, cll]),
             "or_hash_scene_mesh": (cll, [vp, i32p, i32p, vp, cll]),
@@ -220,6 +221,21 @@ class OracleHashScene(_HashSceneView):
                                     rg.ctypes.data, None if cm is None else cm.ctypes.data)
         return v, nm, rg, cm
 
+    def forward_render(self, wl, R, t, vmap, cmap=None, reference_transform=False):
+        """Forward render from the previous world vertex map `vmap` (and colours `cmap`)
+        at pose (R, t).  Returns (vmap, nmap, range, cmap, pixels raycast)."""
+        n = wl.width * wl.height
+        v = np.array(vmap, np.float32).reshape(n, 3)
+        nm = np.empty((n, 3), np.float32)
+        rg = np.empty((n, 2), np.float32)
+        cm = None if cmap is None else np.array(cmap, np.uint8).reshape(n, 4)
+        k = lib().or_hash_scene_forward_render(self.h, wl.width, wl.height, wl.cam6(),
+                                               np.ascontiguousarray(R, np.float32).reshape(-1),
+                                               np.ascontiguousarray(t, np.float32), v.ctypes.data, nm.ctypes.data,
+                                               rg.ctypes.data, None if cm is None else cm.ctypes.data,
+                                               int(bool(reference_transform)))
+        return v, nm, rg, cm, int(k)
+
     def close(self):
         if self.h:
             lib().or_hash_scene_destroy(self.h)
@@ -234,15 +250,17 @@ class OracleHashScene(_HashSceneView):
 
 class OracleHashEngine(_HashSceneView):
     """RunTracking of the reference over a voxel-hashing scene, float arithmetic,
-    deterministic allocation (every touched block in the frame that touches it)."""
+    deterministic allocation (every touched block in the frame that touches it).
+    forward: 0 renders with castRay, 1 with the forward render, 2 with the forward
+    render and the reference's camera->world projection."""
 
     def __init__(self, wl, gate=0.0, voxel=0.005, trunc_scale=0.02, truncation=0.06, wmax=255, capacity=1 << 18,
-                 iterations=None):
+                 iterations=None, forward=0):
         its = list(iterations or wl.iterations)
         self.wl = wl
         self.h = lib().or_hash_engine_create(wl.width, wl.height, wl.cam6(), 3, 0.08, 50.0, 0.3, 0.1, len(its),
                                              np.array(its + [0] * 8, np.int32)[:8], gate, voxel, trunc_scale,
-                                             truncation, wmax, capacity)
+                                             truncation, wmax, capacity, int(forward))
 
     def _scene(self):
         return lib().or_hash_engine_scene(self.h)

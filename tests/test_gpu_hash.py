@@ -5,7 +5,8 @@ Bars: allocated and visible block sets exact; voxel words (int16 sdf * 32767,
 weight) and colours exact; ray ranges exact; raycast vertex / normal maps bit
 for bit; marching-cubes vertex count exact and the triangles equal as a
 multiset (the emission order follows the table order of the entries).  FP16
-bricks: sdf within 1e-3 of the int16 oracle, weights exact."""
+bricks: sdf within 1e-3 of the int16 oracle, weights exact.  Forward render: vertex,
+normal and colour maps bit for bit in both projection modes."""
 import ctypes as C
 import sys
 
@@ -83,6 +84,30 @@ def test_hash_raycast_matches_oracle(gpu):
     np.testing.assert_array_equal(n[hit], on[hit])
 
 
+@pytest.mark.parametrize("reference_transform", [False, True])
+def test_hash_forward_render_matches_oracle(gpu, reference_transform):
+    """Forward render of the frame-2 castRay maps (vertices and colours) at the frame-3
+    pose: ranges, vertices, normals and colours equal the oracle's bit for bit."""
+    announce(f"hash forward render (reference transform {reference_transform}) vs oracle")
+    g, o = _build(color=True)
+    _, _, R2, t2 = _depth(2)
+    prev_v, _, _, prev_c = o.raycast(WL, R2, t2, color=True)
+    _, _, R3, t3 = _depth(3)
+    pose = nuif.upload_struct(nuif.rigid(R3, t3))
+    v, n, rg, c = g.forward_render(WL.camera_params(), pose, prev_v, prev_c, reference_transform)
+    ov, on, org, oc, k = o.forward_render(WL, R3, t3, prev_v, prev_c, reference_transform)
+    assert k > 100
+    np.testing.assert_array_equal(rg, org)
+    np.testing.assert_array_equal(np.isnan(v), np.isnan(ov))
+    hit = ~np.isnan(ov[:, 0])
+    assert hit.mean() > 0.5
+    np.testing.assert_array_equal(v[hit], ov[hit])
+    np.testing.assert_array_equal(np.isnan(n), np.isnan(on))
+    nh = ~np.isnan(on[:, 0])
+    np.testing.assert_array_equal(n[nh], on[nh])
+    np.testing.assert_array_equal(c, oc)
+
+
 def test_hash_mesh_matches_oracle(gpu):
     announce("hash marching cubes over the visible list vs oracle")
     g, o = _build(frames=[0, 1])
@@ -106,18 +131,20 @@ def test_hash_f16_bricks_close_to_oracle(gpu):
     assert np.abs(sdf[m] - osdf[m] / np.float32(32767.0)).max() <= 1e-3
 
 
-def test_hash_engine_matches_oracle_engine(gpu):
-    """RunTracking with the hash scene: preprocessing, ICP against the castRay maps,
-    allocation and integration -- poses and every brick equal the oracle's."""
-    announce("hash engine 320x240, 5 frames vs oracle hash engine")
+@pytest.mark.parametrize("forward", [nuif.RENDER_CASTRAY, nuif.RENDER_FORWARD, nuif.RENDER_FORWARD_REFERENCE])
+def test_hash_engine_matches_oracle_engine(gpu, forward):
+    """RunTracking with the hash scene: preprocessing, ICP against the rendered maps
+    (castRay, or the forward render in either projection), allocation and integration --
+    poses and every brick equal the oracle's."""
+    announce(f"hash engine 320x240, 5 frames, render mode {forward} vs oracle hash engine")
     n = 5
-    desc = WL.kinfu_desc(bricks=1 << 17)
+    desc = WL.kinfu_desc(bricks=1 << 17, forward=forward)
     eng = nuif.Engine(desc)
     for f in range(n):
         eng.run(nuif.DeviceArray.from_host(WL.render(f)[0]), sync=False)
     eng.synchronize()
     poses = eng.poses(n)
-    ref2 = O.OracleHashEngine(WL, capacity=1 << 17)
+    ref2 = O.OracleHashEngine(WL, capacity=1 << 17, forward=forward)
     for f in range(n):
         ok, pr, _ = ref2.run(WL.render(f)[0])
         assert ok == 1 and poses[f, 12] == 1.0

@@ -198,3 +198,117 @@ def test_mesh_count_matches_brute_force_and_vertices_on_edges():
     g = verts.astype(np.float64) / VS
     on_grid = np.abs(g - np.round(g)) < 1e-3
     assert (on_grid.sum(1) >= 2).all()
+
+
+# ---- forward render (OCL/NuiKinfuOpenCLHashScene.cpp:991-1012) ----------------------
+
+def _rot_y(a):
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]], np.float32)
+
+
+def _normals(v):
+    out = np.empty_like(v)
+    O.lib().or_normals_simple_f(np.ascontiguousarray(v).reshape(-1), out.reshape(-1), W_, H_, 0.3)
+    return out
+
+
+def _pixel_point(u, v, z):
+    """Camera-space point on the ray through pixel centre (u, v) at depth z."""
+    return np.array([(u - CAM6[2]) / CAM6[0] * z, (v - CAM6[3]) / CAM6[1] * z, z], np.float64)
+
+
+def test_forward_render_empty_scene_moves_points_to_their_pixels():
+    """Empty scene: every ray range is empty, so nothing is raycast and the output is the
+    forward projection alone.  Points built on the rays of known pixels (world = R c + t)
+    must land on those pixels; of two points on one pixel the nearer wins; points behind
+    the camera or left of the image are dropped; colours follow their points and pixels
+    that receive none keep their own colour; the normals are those of the new map."""
+    rng = np.random.default_rng(11)
+    R, t = _rot_y(0.1), np.array([0.1, -0.05, 0.2], np.float32)
+    n = W_ * H_
+    vmap = np.full((n, 3), np.nan, np.float32)
+    targets = {}  # dst pixel -> (depth, source)
+    src = rng.permutation(n)[:400]
+    for k, s in enumerate(src):
+        u, v = int(rng.integers(0, W_)), int(rng.integers(0, H_))
+        z = float(rng.uniform(0.8, 3.0))
+        if k % 50 == 0:
+            z = -1.0  # behind the camera
+        if k % 50 == 1:
+            u = -3    # left of the image
+        c = _pixel_point(u, v, z)
+        vmap[s] = (R.astype(np.float64) @ c + t).astype(np.float32)
+        if z > 0 and u >= 0:
+            d = v * W_ + u
+            if d not in targets or z < targets[d][0]:
+                targets[d] = (z, s)
+    colors = rng.integers(0, 256, (n, 4), dtype=np.uint8)
+    sc = O.OracleHashScene(voxel=VS, trunc_scale=TS)
+    out_v, out_n, rg, out_c, raycast = sc.forward_render(_Wl, R, t, vmap, colors)
+    assert raycast == 0 and (rg[:, 0] > rg[:, 1]).all()
+    exp_v = np.full((n, 3), np.nan, np.float32)
+    exp_c = colors.copy()
+    for d, (_, s) in targets.items():
+        exp_v[d] = vmap[s]
+        exp_c[d] = colors[s]
+    assert len(targets) > 300
+    np.testing.assert_array_equal(out_v, exp_v)
+    np.testing.assert_array_equal(out_c, exp_c)
+    np.testing.assert_array_equal(out_n, _normals(exp_v))
+
+
+def test_forward_render_reference_transform_is_camera_to_world():
+    """The reference's forwardProject_kernel projects transform(p) = R p + t of a world
+    point (scaled by the voxel size, a no-op at voxel size 1); with R = I that is the
+    world->camera projection at translation -t, point for point.  Its pixel is the
+    saturating uint conversion of p + 0.5: a point left of the image lands in column 0."""
+    rng = np.random.default_rng(5)
+    n = W_ * H_
+    t = np.array([0.05, 0.02, -0.1], np.float32)
+    vmap = np.full((n, 3), np.nan, np.float32)
+    idx = rng.permutation(n)[:300]
+    for s in idx:
+        c = _pixel_point(int(rng.integers(0, W_)), int(rng.integers(0, H_)), float(rng.uniform(1.0, 3.0)))
+        vmap[s] = (c - t).astype(np.float32)
+    sc = O.OracleHashScene(voxel=1.0, trunc_scale=TS)
+    ref_v, ref_n, _, _, _ = sc.forward_render(_Wl, I3, t, vmap, reference_transform=True)
+    own_v, own_n, _, _, _ = sc.forward_render(_Wl, I3, -t, vmap, reference_transform=False)
+    np.testing.assert_array_equal(ref_v, own_v)
+    np.testing.assert_array_equal(ref_n, own_n)
+    assert (~np.isnan(ref_v[:, 0])).sum() > 250
+    # saturation: a point 5 pixels left of the image, row 7
+    one = np.full((n, 3), np.nan, np.float32)
+    one[0] = _pixel_point(-5, 7, 2.0).astype(np.float32)
+    v2, _, _, _, _ = sc.forward_render(_Wl, I3, Z3, one, reference_transform=True)
+    assert np.flatnonzero(~np.isnan(v2[:, 0])).tolist() == [7 * W_]
+    v3, _, _, _, _ = sc.forward_render(_Wl, I3, Z3, one, reference_transform=False)
+    assert np.isnan(v3).all()
+
+
+def test_forward_render_fills_holes_with_castray():
+    """Nothing to forward (all-NaN map): every pixel with a non-empty ray range is
+    raycast, so vertices and colours equal castRay's there and stay NaN elsewhere."""
+    s = plane_scene(1.2, frames=2)
+    rv, _, rg, rc = s.raycast(_Wl, I3, Z3, color=True)
+    n = W_ * H_
+    fv, fn, frg, fc, k = s.forward_render(_Wl, I3, Z3, np.full((n, 3), np.nan, np.float32),
+                                          np.zeros((n, 4), np.uint8))
+    np.testing.assert_array_equal(frg, rg)
+    rays = rg[:, 0] < rg[:, 1]
+    assert k == rays.sum() and rays.mean() > 0.8
+    np.testing.assert_array_equal(fv[rays], rv[rays])
+    np.testing.assert_array_equal(fc[rays], rc[rays])
+    assert np.isnan(fv[~rays]).all()
+    np.testing.assert_array_equal(fn, _normals(fv))
+
+
+def test_forward_render_of_a_raycast_at_its_own_pose_is_a_fixed_point():
+    """Each raycast vertex lies on its own pixel's ray, so forwarding the map at the same
+    pose puts (almost) every point back on its pixel and nothing is raycast again there."""
+    s = plane_scene(1.2, frames=2)
+    rv, _, _, _ = s.raycast(_Wl, I3, Z3)
+    hit = ~np.isnan(rv[:, 0])
+    fv, _, _, _, _ = s.forward_render(_Wl, I3, Z3, rv)
+    same = (fv == rv).all(axis=1)
+    assert same[hit].mean() > 0.99
