@@ -346,6 +346,67 @@ int nuif_hash_mesh(void* scene, const nuif_hash_params* p, float* vertices, floa
  * entries]; synchronises s. */
 int nuif_hash_stats(const void* scene, uint32_t* out4, nuif_stream_t s);
 
+/* ---- photometric (colour) tracker (OCL/NuiKinfuOpenCLColorTracker.cpp) ---- */
+
+/* K/transformMaps.cl:405-446 half_sample_bgra_kernel: dst (dst_width x dst_height BGRA)
+ * = integer mean of the (2r+1)^2 window around (2x, 2y) of src (2*dst_width x
+ * 2*dst_height) clipped to the image, alpha 255. */
+int nuif_color_half_sample(const uint8_t* src, uint8_t* dst, int dst_width, int dst_height, int radius,
+                           nuif_stream_t s);
+/* K/transformMaps.cl:513-563 bgra_gradient_kernel: Sobel / 8 per channel, alpha 255
+ * inside and all zero on the border.  sign = 0: uint8 values (saturating, as the
+ * reference's conversion on the GPU); sign = 1: int8 values (the corrected tracker). */
+int nuif_color_gradient(const uint8_t* bgra, uint8_t* gx, uint8_t* gy, int width, int height, int sign,
+                        nuif_stream_t s);
+/* SubSampleColors + GradientBuffers (OCL/NuiKinfuOpenCLColorTracker.cpp:125-242): level 0
+ * = a copy of bgra, levels 1.. by nuif_color_half_sample (radius 1), then the gradients
+ * of every level.  colors / gx / gy: `levels` device pointers each (host arrays). */
+int nuif_color_pyramid(const uint8_t* bgra, uint8_t* const* colors, uint8_t* const* gx, uint8_t* const* gy,
+                       int width, int height, int levels, int sign, nuif_stream_t s);
+/* K/colorIcp.cl:7-57 color_difference_kernel (GetColorsDifference,
+ * OCL/NuiKinfuOpenCLColorTracker.cpp:244-340): out2 (device uint64[2]) = exact
+ * {sum over valid points of |obs - known|^2, valid count}, the model points
+ * transformed by `pose` (R p + t, as the reference passes the camera pose).  The
+ * reference's result is out2[0] * (W*H / out2[1]). */
+int nuif_color_difference(const float* vmap, const uint8_t* colors, const uint8_t* level_colors,
+                          const NuiCLCameraParams* cam, int level, const NuiCLRigidTransform* pose,
+                          unsigned long long* out2, nuif_stream_t s);
+
+/* Host view of the colour tracker state (nuif_color_icp_read). */
+typedef struct nuif_color_icp_result {
+    int32_t ok;          /* 0: the normal matrix was singular or not positive definite */
+    int32_t iterations;  /* iterations that ran */
+    float error;         /* sqrt(sum |diff|) / count of the last iteration */
+    float count;
+    float R[9], t[3];    /* the transform applied to the model points */
+    float sums[29];      /* the last iteration's sums in the kernel's slot order */
+} nuif_color_icp_result;
+
+size_t nuif_color_icp_state_bytes(void);
+size_t nuif_color_icp_partials_bytes(int width, int height);
+/* Starts a track from the camera pose.  corrected = 0: the reference's host decode
+ * (OCL/NuiKinfuOpenCLColorTracker.cpp:341-578 as written); 1: the sums decoded in the
+ * kernel's order, world -> camera transform, signed gradients (nuif_color_gradient
+ * sign = 1), Gauss-Newton step. */
+int nuif_color_icp_begin(void* state, const NuiCLRigidTransform* pose, int corrected, nuif_stream_t s);
+/* One iteration at `level` over all W*H model points (world vertices vmap, colours
+ * with alpha 255 where valid): color_icp_block_kernel + compute_color_sums + the host
+ * update, in one launch.  Skipped after a break at this level or a failure. */
+int nuif_color_icp_iteration(void* state, float* partials, const NuiCLCameraParams* cam, int level,
+                             int rotation_only, float dist_threshold, const float* vmap, const uint8_t* colors,
+                             const uint8_t* level_colors, const uint8_t* level_gx, const uint8_t* level_gy,
+                             nuif_stream_t s);
+/* UpdateCameraTransform: writes the estimate into pose (R, R_inv, t) unless the track failed. */
+int nuif_color_icp_end(const void* state, NuiCLRigidTransform* pose, nuif_stream_t s);
+/* ColorIterativeClosestPoint: begin, levels - 1 .. 0 x iterations[level], end; no host sync. */
+int nuif_color_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam, int levels,
+                            const int* iterations, const int* rotation_only, float dist_threshold, const float* vmap,
+                            const uint8_t* colors, const uint8_t* const* level_colors,
+                            const uint8_t* const* level_gx, const uint8_t* const* level_gy,
+                            NuiCLRigidTransform* pose, int corrected, nuif_stream_t s);
+/* Synchronises s and copies the state out. */
+int nuif_color_icp_read(const void* state, nuif_color_icp_result* out, nuif_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

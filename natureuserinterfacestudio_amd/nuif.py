@@ -64,6 +64,11 @@ class IcpConfig(C.Structure):
 ICP_REDUCE_REFERENCE, ICP_REDUCE_COMPENSATED = 0, 1
 
 
+class ColorIcpResult(C.Structure):
+    _fields_ = [("ok", C.c_int32), ("iterations", C.c_int32), ("error", c_float), ("count", c_float),
+                ("R", c_float * 9), ("t", c_float * 3), ("sums", c_float * 29)]
+
+
 class KinfuDesc(C.Structure):
     _fields_ = [("width", c_int), ("height", c_int), ("fx", c_float), ("fy", c_float), ("cx", c_float),
                 ("cy", c_float), ("depth_min", c_float), ("depth_max", c_float), ("filter_radius", c_int),
@@ -166,6 +171,21 @@ _SIGS = {
                                     c_void_p, c_void_p, c_void_p]),
     "nuif_hash_raycast": (c_int, [c_void_p, C.POINTER(HashParams), C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nuif_color_half_sample": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "nuif_color_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "nuif_color_pyramid": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
+    "nuif_color_difference": (c_int, [c_void_p, c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_int, c_void_p,
+                                      c_void_p, c_void_p]),
+    "nuif_color_icp_state_bytes": (c_size_t, []),
+    "nuif_color_icp_partials_bytes": (c_size_t, [c_int, c_int]),
+    "nuif_color_icp_begin": (c_int, [c_void_p, c_void_p, c_int, c_void_p]),
+    "nuif_color_icp_iteration": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_int, c_int, c_float,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nuif_color_icp_end": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nuif_color_icp_estimate": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_int, c_void_p, c_void_p,
+                                        c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                        c_void_p]),
+    "nuif_color_icp_read": (c_int, [c_void_p, C.POINTER(ColorIcpResult), c_void_p]),
     "nuif_hash_forward_render_bytes": (c_size_t, [c_int, c_int]),
     "nuif_hash_forward_render": (c_int, [c_void_p, C.POINTER(HashParams), C.POINTER(NuiCLCameraParams), c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
@@ -698,3 +718,48 @@ def decode_hash_blocks(raw, params):
         rgb = raw[off[5]:off[5] + nbr * 512 * 4].reshape(nbr, 512, 4)[ptr][:, :, :3].copy()
     return keys, sdf, weight, words, rgb
 
+
+
+class ColorTracker:
+    """The photometric tracker (OCL/NuiKinfuOpenCLColorTracker.cpp) on nuif_color_*:
+    the frame's colour pyramid and gradients, then ColorIterativeClosestPoint from a
+    camera pose against a model (world vertices + colours of the previous render)."""
+
+    def __init__(self, W, H, levels=3, corrected=False, stream=None):
+        self.W, self.H, self.levels, self.corrected, self.stream = W, H, levels, bool(corrected), stream
+        self.colors = [DeviceArray(((H >> l) * (W >> l) * 4,), np.uint8) for l in range(levels)]
+        self.gx = [DeviceArray(((H >> l) * (W >> l) * 4,), np.uint8) for l in range(levels)]
+        self.gy = [DeviceArray(((H >> l) * (W >> l) * 4,), np.uint8) for l in range(levels)]
+        self.state = DeviceArray((L().nuif_color_icp_state_bytes(),), np.uint8)
+        self.partials = DeviceArray((L().nuif_color_icp_partials_bytes(W, H),), np.uint8)
+
+    @staticmethod
+    def _ptrs(arrs):
+        return (c_void_p * len(arrs))(*[a.ptr for a in arrs])
+
+    def pyramid(self, frame):
+        """SubSampleColors + GradientBuffers of a device BGRA frame."""
+        check(L().nuif_color_pyramid(frame.ptr, self._ptrs(self.colors), self._ptrs(self.gx), self._ptrs(self.gy),
+                                     self.W, self.H, self.levels, int(self.corrected), self.stream),
+              "nuif_color_pyramid")
+
+    def estimate(self, cam, vmap, colors, pose, iterations, rotation_only=None, dist_threshold=0.1):
+        """Runs the levels from the device pose (updated in place when the track holds);
+        returns the host view of the state."""
+        its = (c_int * self.levels)(*iterations[:self.levels])
+        ro = (c_int * self.levels)(*(rotation_only or [0] * self.levels)[:self.levels])
+        check(L().nuif_color_icp_estimate(self.state.ptr, self.partials.ptr, C.byref(cam), self.levels, its, ro,
+                                          dist_threshold, vmap.ptr, colors.ptr, self._ptrs(self.colors),
+                                          self._ptrs(self.gx), self._ptrs(self.gy), pose.ptr, int(self.corrected),
+                                          self.stream), "nuif_color_icp_estimate")
+        r = ColorIcpResult()
+        check(L().nuif_color_icp_read(self.state.ptr, C.byref(r), self.stream), "nuif_color_icp_read")
+        return r
+
+    def difference(self, cam, level, vmap, colors, pose):
+        """GetColorsDifference's exact sums (sum |obs - known|^2, valid count)."""
+        out = DeviceArray((2,), np.uint64)
+        check(L().nuif_color_difference(vmap.ptr, colors.ptr, self.colors[level].ptr, C.byref(cam), level, pose.ptr,
+                                        out.ptr, self.stream), "nuif_color_difference")
+        d = out.download()
+        return int(d[0]), int(d[1])

@@ -78,6 +78,13 @@ def lib():
             "or_hash_scene_blocks": (ci, [vp, vp, vp, vp, vp, cll]),
             "or_hash_scene_visible": (ci, [vp, vp, cll]),
             "or_hash_scene_mesh": (cll, [vp, i32p, i32p, vp, cll]),
+            "or_color_half_sample": (None, [u8p, u8p, ci, ci, ci]),
+            "or_color_gradient": (None, [u8p, u8p, u8p, ci, ci, ci]),
+            "or_color_difference": (None, [ci, ci, f32p, ci, f32p, u8p, u8p, f32p, f32p,
+                                           np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
+            "or_color_terms": (None, [ci, ci, f32p, ci, f32p, u8p, u8p, u8p, u8p, f32p, f32p, ci, ci, f32p]),
+            "or_color_reduce": (ci, [ci, ci, f32p, ci, f32p, u8p, u8p, u8p, u8p, f32p, f32p, ci, ci, vp, ci]),
+            "or_color_icp": (ci, [ci, ci, f32p, ci, i32p, i32p, cf, f32p, u8p, u8p, f32p, f32p, ci, f64p]),
             "or_expf_det": (cf, [cf]),
             "or_sincosf_det": (None, [cf, C.POINTER(cf), C.POINTER(cf)]),
         }
@@ -289,3 +296,67 @@ class OracleHashEngine(_HashSceneView):
             self.close()
         except Exception:
             pass
+
+
+# ---- photometric (colour) tracker -------------------------------------------------
+
+def _u8(a):
+    return np.ascontiguousarray(a, np.uint8).reshape(-1)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32).reshape(-1)
+
+
+def color_half_sample(src, w_dst, h_dst, radius=1):
+    """half_sample_bgra_kernel: BGRA (2w x 2h) -> (w x h)."""
+    dst = np.empty(w_dst * h_dst * 4, np.uint8)
+    lib().or_color_half_sample(_u8(src), dst, w_dst, h_dst, radius)
+    return dst.reshape(h_dst, w_dst, 4)
+
+
+def color_gradient(bgra, w, h, signed=False):
+    """bgra_gradient_kernel: (gx, gy) as uint8 (or int8 values stored in uint8)."""
+    gx = np.empty(w * h * 4, np.uint8)
+    gy = np.empty(w * h * 4, np.uint8)
+    lib().or_color_gradient(_u8(bgra), gx, gy, w, h, int(bool(signed)))
+    return gx.reshape(h, w, 4), gy.reshape(h, w, 4)
+
+
+def color_difference(wl, level, vmap, colors, level_colors, R, t):
+    """GetColorsDifference's exact sums: (sum |obs - known|^2, valid count)."""
+    out = np.zeros(2, np.int64)
+    lib().or_color_difference(wl.width, wl.height, wl.cam6(), level, _f32(vmap), _u8(colors), _u8(level_colors),
+                              _f32(R), _f32(t), out)
+    return int(out[0]), int(out[1])
+
+
+def color_terms(wl, level, p, known, level_colors, gx, gy, R, t, rot_only=False, signed=False):
+    out = np.empty(29, np.float32)
+    lib().or_color_terms(wl.width, wl.height, wl.cam6(), level, _f32(p), _u8(known), _u8(level_colors), _u8(gx),
+                         _u8(gy), _f32(R), _f32(t), int(bool(rot_only)), int(bool(signed)), out)
+    return out
+
+
+def color_reduce(wl, level, vmap, colors, level_colors, gx, gy, R, t, rot_only=False, signed=False):
+    """One iteration's second-level block sums (nblocks x 29), in the reference's tree order."""
+    args = (wl.width, wl.height, wl.cam6(), level, _f32(vmap), _u8(colors), _u8(level_colors), _u8(gx), _u8(gy),
+            _f32(R), _f32(t), int(bool(rot_only)), int(bool(signed)))
+    nb = lib().or_color_reduce(*args, None, 0)
+    out = np.empty((nb, 29), np.float32)
+    lib().or_color_reduce(*args, out.ctypes.data, nb)
+    return out
+
+
+def color_icp(wl, vmap, colors, frame_bgra, R, t, iterations=None, rot_only=None, dist_threshold=0.1,
+              corrected=False):
+    """ColorIterativeClosestPoint from pose (R, t): returns (ok, R, t, iterations, error, count)."""
+    its = list(iterations or wl.iterations)
+    ro = list(rot_only or [0] * len(its))
+    Rn = np.array(R, np.float32).reshape(-1).copy()
+    tn = np.array(t, np.float32).reshape(-1).copy()
+    info = np.zeros(4, np.float64)
+    ok = lib().or_color_icp(wl.width, wl.height, wl.cam6(), len(its), np.array(its, np.int32),
+                            np.array(ro, np.int32), dist_threshold, _f32(vmap), _u8(colors), _u8(frame_bgra), Rn, tn,
+                            int(bool(corrected)), info)
+    return ok, Rn.reshape(3, 3), tn, int(info[1]), float(info[2]), float(info[3])
