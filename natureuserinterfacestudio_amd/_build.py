@@ -18,7 +18,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NUIF_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["runtime.hip", "preprocess.hip", "icp.hip", "tsdf.hip", "mc.hip", "mesh.hip", "hash.hip", "scan.hip", "color.hip"]
-KINFU_SOURCES = ["NuiKinfuConfig.cpp", "NuiKinfuHIP.cpp", "NuiKinfuSession.cpp", "NuiKinfuHashScene.cpp", "nuifk_capi.cpp"]
+KINFU_SOURCES = ["NuiKinfuConfig.cpp", "NuiKinfuHIP.cpp", "NuiKinfuSession.cpp", "NuiKinfuHashScene.cpp",
+                 "NuiKinfuColorTracker.cpp", "nuifk_capi.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

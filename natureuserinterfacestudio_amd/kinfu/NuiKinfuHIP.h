@@ -151,6 +151,36 @@ private:
     int m_historyCap = 0;
 };
 
+// OCL/NuiKinfuOpenCLColorTracker.*: photometric tracker against the feedback
+// frame's world vertices and colours.  The reference's factory never builds it
+// (DS/NuiKinfuTrackingFactory.h:36-43); here it is a tracker of its own that a
+// caller constructs directly.  corrected selects the decode of the sums
+// (nuif_color_icp_begin): false = the reference's host loop as written.
+class NuiKinfuHIPColorTracker : public NuiKinfuTracker {
+public:
+    NuiKinfuHIPColorTracker(const NuiTrackerConfig& config, UINT nWidth, UINT nHeight, bool corrected,
+                            nuif_stream_t s);
+    // OCL/NuiKinfuOpenCLColorTracker.cpp:580-605: SubSampleColors, GradientBuffers,
+    // ColorIterativeClosestPoint; synchronises the stream (as the reference's reads do)
+    bool EstimatePose(NuiKinfuFrame* pFrame, NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState,
+                      Affine3f* hint) override;
+    bool hasColorData() const override { return true; }
+    void log(const std::string& fileName) const override { m_cfg.log(fileName); }
+    float getError() const override { return m_error; }
+    int getCount() const override { return (int)m_count; }
+    const nuif_color_icp_result& LastResult() const { return m_last; }
+
+private:
+    NuiTrackerConfig m_cfg;
+    UINT m_W, m_H;
+    bool m_corrected;
+    nuif_stream_t m_stream;
+    std::vector<DeviceBuffer> m_colors, m_gx, m_gy;
+    DeviceBuffer m_state, m_partials, m_pose;
+    float m_error = 0.f, m_count = 0.f;
+    nuif_color_icp_result m_last{};
+};
+
 // What the tracking engine needs from a GPU scene beyond NuiKinfuScene: the
 // device predicates of the tracker (integration gate, tracking success) that
 // guard its launches, and whether it integrates ray-length depth.
@@ -209,9 +239,9 @@ private:
     DeviceBuffer m_volume, m_color, m_counter;
 };
 
-// OCL/NuiKinfuOpenCLHashScene.*: voxel-hashing scene.  The raycast is castRay
-// over ray ranges from the visible blocks (the reference's m_bUseForwardRender
-// == false path, OCL/NuiKinfuOpenCLHashScene.cpp:659-736).
+// OCL/NuiKinfuOpenCLHashScene.*: voxel-hashing scene.  The render is castRay over
+// ray ranges from the visible blocks (OCL/NuiKinfuOpenCLHashScene.cpp:659-736) or,
+// with m_bUseForwardRender, the forward render (:991-1012).
 class NuiKinfuHIPHashScene : public NuiKinfuHIPSceneBase {
 public:
     NuiKinfuHIPHashScene(const NuiHashingSDFConfig& config, int storage, bool color, nuif_stream_t s);
