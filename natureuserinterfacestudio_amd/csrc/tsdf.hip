@@ -102,7 +102,10 @@ __global__ void reset_kernel(typename S::word* __restrict__ vol, uint32_t* __res
 
 // ---- integration ---------------------------------------------------------------
 
-constexpr int kZChunk = 64;
+#ifndef NUIF_ZCHUNK
+#define NUIF_ZCHUNK 64
+#endif
+constexpr int kZChunk = NUIF_ZCHUNK;
 constexpr int kRayBatch = 8;  // raycast samples whose loads are in flight together
 constexpr int kLdsPitch = kZChunk + 1;
 
@@ -291,9 +294,18 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
     return dmax + trunc < dmin * (1.0f - 1e-5f) - 2e-4f;
 }
 
+#ifndef NUIF_INTEGRATE_BATCH
+#define NUIF_INTEGRATE_BATCH 8
+#endif
+#ifdef NUIF_INTEGRATE_WPE
+#define NUIF_INTEGRATE_ATTR __attribute__((amdgpu_waves_per_eu(NUIF_INTEGRATE_WPE)))
+#else
+#define NUIF_INTEGRATE_ATTR
+#endif
+
 // Fast path: Nx, Ny multiples of 8, Nz multiple of 64.
 template <class S, bool COLOR>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) NUIF_INTEGRATE_ATTR
     integrate_tile_kernel(const float* __restrict__ depth, const uchar4* __restrict__ bgra,
                           typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
                           NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
@@ -305,7 +317,8 @@ __global__ void __launch_bounds__(64)
     constexpr int LPC = kZChunk / VPL;        // lanes covering one column chunk
     constexpr int CPI = 64 / LPC;             // columns per vector instruction
     constexpr int NIT = 64 / CPI;             // instructions per chunk
-    constexpr int BATCH = 8;
+    constexpr int BATCH = NUIF_INTEGRATE_BATCH;
+    static_assert(NIT % BATCH == 0 && kZChunk % VPL == 0, "phase 2 covers the 64 columns exactly");
     __shared__ float ts[64 * kLdsPitch];
     __shared__ int cp[COLOR ? 64 * kLdsPitch : 1];
 
