@@ -230,7 +230,7 @@ int nuif_tsdf_depth_bounds(const float* depth, int width, int height, float* bou
 int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                         const TsdfParams* params, const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose,
                         int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
-                        const float* depth_bounds, nuif_stream_t s);
+                        const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s);
 /* K/raycast.cl:128-277 raycastKernel: world vmap/nmap (float3 AoS) and optional cmap */
 int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
                       const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
@@ -238,7 +238,15 @@ int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const Tsd
 /* Same, skipped entirely when the device int32 *enable is zero (tracking failure). */
 int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
                             const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap,
-                            float* nmap, uint8_t* cmap, int storage, const int32_t* enable, nuif_stream_t s);
+                            float* nmap, uint8_t* cmap, int storage, const int32_t* enable,
+                            const uint8_t* surface_blocks, nuif_stream_t s);
+/* Surface blocks of a volume: one byte per 8 x 8 x 8 voxel block (y-major, z fastest),
+ * set by nuif_tsdf_integrate (surface_blocks argument) when it stores a tsdf <= 0 and
+ * cleared only by nuif_tsdf_surface_blocks_clear (with the volume's reset).  The raycast
+ * does not read the voxels of blocks left at zero where its exit tests cannot need their
+ * values; its output is identical with or without them (NULL). */
+size_t nuif_tsdf_surface_blocks_bytes(const TsdfParams* params);
+int nuif_tsdf_surface_blocks_clear(uint8_t* surface_blocks, const TsdfParams* params, nuif_stream_t s);
 /* Effective truncation distance (OCL/NuiKinfuOpenCLScene.cpp:41: >= 2.1 * max cell). */
 int nuif_tsdf_params(int nx, int ny, int nz, float dx, float dy, float dz, float trunc, TsdfParams* out);
 

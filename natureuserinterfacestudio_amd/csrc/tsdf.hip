@@ -109,6 +109,21 @@ constexpr int kZChunk = NUIF_ZCHUNK;
 constexpr int kRayBatch = 8;  // raycast samples whose loads are in flight together
 constexpr int kLdsPitch = kZChunk + 1;
 
+// Surface blocks: one byte per 8 x 8 x 8 voxel block (same y-major, z-fastest order
+// as the voxels), set by the integration when it stores a tsdf <= 0 and never
+// cleared, so a zero byte guarantees that every voxel of the block reads positive
+// or unobserved.  The raycast reads the voxels of such blocks only where its exit
+// tests can need their values.
+constexpr int kSurfBlock = 8;
+struct SurfaceBlocks {
+    uint8_t* b;
+    int nbx, nbz;  // blocks along x and z
+    __device__ __forceinline__ size_t index(int x, int y, int z) const {
+        return ((size_t)(y >> 3) * nbx + (x >> 3)) * nbz + (z >> 3);
+    }
+};
+__host__ __device__ __forceinline__ int surf_dim(int n) { return (n + kSurfBlock - 1) / kSurfBlock; }
+
 struct ColumnWalk {  // the reference's per-column incremental state
     float vgz, part, vx, vy, vz, zs, incx, incy;
 };
@@ -129,25 +144,39 @@ __device__ __forceinline__ ColumnWalk column_setup(int x, int y, int Nx, int Ny,
     return c;
 }
 
+// The pixel that one step of K/tsdfVolume.cl:74-120 reads for the current z, or -1
+// where the reference skips the voxel before reading (behind the camera or
+// outside the image).
+__device__ __forceinline__ int column_pixel(const ColumnWalk& c, const NuiCLCameraParams& cam, float R8) {
+    const float inv_z = 1.0f / (c.vz + R8 * c.zs);
+    if (inv_z < 0.f) return -1;
+    const int u = to_int(roundf(c.vx * inv_z + cam.cx));
+    const int v = to_int(roundf(c.vy * inv_z + cam.cy));
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    if (!(u >= 0 && v >= 0 && u < W && v < H)) return -1;
+    return v * W + u;
+}
+
+// column_step's result from its pixel id, the depth read there and the distance
+// of the voxel from the camera
+__device__ __forceinline__ float column_finish(int id, float Dp, float dist, float trunc, float tr_inv, int* pix) {
+    *pix = -1;
+    if (id < 0) return kNaN;
+    const float sdf = Dp - dist;
+    if (isnan(Dp) || !(sdf >= -trunc)) return kNaN;
+    if (fabsf(sdf) < trunc) *pix = id;
+    return fminf(1.0f, sdf * tr_inv);
+}
+
 // One step of K/tsdfVolume.cl:74-120 for the current z; returns the new tsdf
 // sample or NaN when the reference skips the voxel; *pix receives the pixel
 // for colour integration (|sdf| < trunc) or -1.
 __device__ __forceinline__ float column_step(const ColumnWalk& c, const float* __restrict__ depth,
                                              const NuiCLCameraParams& cam, float R8, float trunc, float tr_inv,
                                              int* pix) {
-    *pix = -1;
-    const float inv_z = 1.0f / (c.vz + R8 * c.zs);
-    if (inv_z < 0.f) return kNaN;
-    const int u = to_int(roundf(c.vx * inv_z + cam.cx));
-    const int v = to_int(roundf(c.vy * inv_z + cam.cy));
-    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
-    if (!(u >= 0 && v >= 0 && u < W && v < H)) return kNaN;
-    const int id = v * W + u;
-    const float Dp = depth[id];
-    const float sdf = Dp - sqrtf(c.vgz * c.vgz + c.part);
-    if (isnan(Dp) || !(sdf >= -trunc)) return kNaN;
-    if (fabsf(sdf) < trunc) *pix = id;
-    return fminf(1.0f, sdf * tr_inv);
+    const int id = column_pixel(c, cam, R8);
+    const float Dp = id >= 0 ? depth[id] : 0.f;
+    return column_finish(id, Dp, sqrtf(c.vgz * c.vgz + c.part), trunc, tr_inv, pix);
 }
 
 __device__ __forceinline__ void column_advance(ColumnWalk& c, float cz) {
@@ -294,6 +323,9 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
     return dmax + trunc < dmin * (1.0f - 1e-5f) - 2e-4f;
 }
 
+#ifndef NUIF_WALK_BATCH
+#define NUIF_WALK_BATCH 1
+#endif
 #ifndef NUIF_INTEGRATE_BATCH
 #define NUIF_INTEGRATE_BATCH 8
 #endif
@@ -310,7 +342,7 @@ __global__ void __launch_bounds__(64) NUIF_INTEGRATE_ATTR
                           typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
                           NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
                           const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated,
-                          const float* __restrict__ bounds) {
+                          const float* __restrict__ bounds, SurfaceBlocks surf) {
     if (gate && *gate == 0) return;
     using word = typename S::word;
     constexpr int VPL = 16 / S::kBytes;      // voxels per lane per 16-byte vector
@@ -318,6 +350,8 @@ __global__ void __launch_bounds__(64) NUIF_INTEGRATE_ATTR
     constexpr int CPI = 64 / LPC;             // columns per vector instruction
     constexpr int NIT = 64 / CPI;             // instructions per chunk
     constexpr int BATCH = NUIF_INTEGRATE_BATCH;
+    constexpr int KB = NUIF_WALK_BATCH;
+    static_assert(kZChunk % KB == 0, "walk batches tile the chunk");
     static_assert(NIT % BATCH == 0 && kZChunk % VPL == 0, "phase 2 covers the 64 columns exactly");
     __shared__ float ts[64 * kLdsPitch];
     __shared__ int cp[COLOR ? 64 * kLdsPitch : 1];
@@ -350,14 +384,29 @@ __global__ void __launch_bounds__(64) NUIF_INTEGRATE_ATTR
                 column_advance(c, cz);
             }
         } else {
-            for (int k = 0; k < kZChunk; ++k) {
-                int pix;
-                const float v = column_step(c, depth, cam, R8, trunc, tr_inv, &pix);
-                ts[lane * kLdsPitch + k] = v;
-                if (COLOR) cp[lane * kLdsPitch + k] = pix;
-                any |= !isnan(v);
-                nupd += !isnan(v);
-                column_advance(c, cz);
+            // KB steps at a time: their pixels first, then their KB depth reads in
+            // flight together, then the samples (the same float operations as
+            // column_step, in the same order per voxel)
+            for (int k0 = 0; k0 < kZChunk; k0 += KB) {
+                int id[KB];
+                float dist[KB], Dp[KB];
+#pragma unroll
+                for (int j = 0; j < KB; ++j) {
+                    id[j] = column_pixel(c, cam, R8);
+                    dist[j] = sqrtf(c.vgz * c.vgz + c.part);
+                    column_advance(c, cz);
+                }
+#pragma unroll
+                for (int j = 0; j < KB; ++j) Dp[j] = id[j] >= 0 ? depth[id[j]] : 0.f;
+#pragma unroll
+                for (int j = 0; j < KB; ++j) {
+                    int pix;
+                    const float v = column_finish(id[j], Dp[j], dist[j], trunc, tr_inv, &pix);
+                    ts[lane * kLdsPitch + k0 + j] = v;
+                    if (COLOR) cp[lane * kLdsPitch + k0 + j] = pix;
+                    any |= !isnan(v);
+                    nupd += !isnan(v);
+                }
             }
         }
         if (!__any(any)) continue;  // the whole tile misses the frustum / band in this chunk
@@ -392,9 +441,15 @@ __global__ void __launch_bounds__(64) NUIF_INTEGRATE_ATTR
                 const size_t base = ((size_t)gy * Nx + gx) * Nz + zc + sub * VPL;
                 uint4 outv;
                 word* ov = reinterpret_cast<word*>(&outv);
+                bool neg = false;
 #pragma unroll
-                for (int q = 0; q < VPL; ++q) ov[q] = isnan(tv[q]) ? buf[j][q] : update_voxel<S>(buf[j][q], tv[q]);
+                for (int q = 0; q < VPL; ++q) {
+                    ov[q] = isnan(tv[q]) ? buf[j][q] : update_voxel<S>(buf[j][q], tv[q]);
+                    neg |= !isnan(tv[q]) && !(S::tsdf(ov[q]) > 0.f);
+                }
                 *reinterpret_cast<uint4*>(vol + base) = outv;
+                // the VPL voxels share one 8-voxel z block (VPL divides 8)
+                if (neg && surf.b) surf.b[surf.index(gx, gy, zc + sub * VPL)] = 1;
                 if (COLOR) {
                     const int* pv = &cp[col * kLdsPitch + sub * VPL];
 #pragma unroll
@@ -422,7 +477,7 @@ __global__ void integrate_column_kernel(const float* __restrict__ depth, const u
                                         typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
                                         NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
                                         int maxcw, const int32_t* __restrict__ gate,
-                                        unsigned long long* __restrict__ updated) {
+                                        unsigned long long* __restrict__ updated, SurfaceBlocks surf) {
     if (gate && *gate == 0) return;
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -437,7 +492,9 @@ __global__ void integrate_column_kernel(const float* __restrict__ depth, const u
         const float v = column_step(c, depth, cam, R8, trunc, tr_inv, &pix);
         if (isnan(v)) continue;
         ++nupd;
-        vol[base + z] = update_voxel<S>(vol[base + z], v);
+        const typename S::word nw = update_voxel<S>(vol[base + z], v);
+        vol[base + z] = nw;
+        if (surf.b && !(S::tsdf(nw) > 0.f)) surf.b[surf.index(x, y, z)] = 1;
         if (COLOR && pix >= 0) {
             const uchar4 nc = bgra[pix];
             if (nc.w != 0) cvol[base + z] = update_color(cvol[base + z], nc, maxcw);
@@ -520,7 +577,8 @@ template <class S>
 __global__ void __launch_bounds__(64)
     raycast_kernel(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol, TsdfParams p,
                    NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
-                   float* __restrict__ nmap, uchar4* __restrict__ cmap, const int32_t* __restrict__ enable) {
+                   float* __restrict__ nmap, uchar4* __restrict__ cmap, const int32_t* __restrict__ enable,
+                   SurfaceBlocks surf) {
     if (enable && *enable == 0) return;
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
     const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
@@ -553,11 +611,15 @@ __global__ void __launch_bounds__(64)
         // sample positions of the next steps do not depend on the samples, so their
         // loads are issued together and then consumed in order with the reference's
         // exit tests (loads past the exit are in bounds and simply unused).
-        bool marching = true, crossing = false;
+        // Surface blocks: a sample in a block without tsdf <= 0 (value > 0 or NaN)
+        // whose neighbouring samples are too can trigger neither exit test, at its
+        // own step or at the next one, whatever its exact value; such a sample is
+        // not read and stands in as +1.
+        bool marching = true, crossing = false, prev_safe = false;
         while (marching) {
             float tb[kRayBatch], sv[kRayBatch];
             long long ib[kRayBatch];
-            bool inb[kRayBatch];
+            bool inb[kRayBatch], safe[kRayBatch + 1];
             float tn = tcur;
 #pragma unroll
             for (int k = 0; k < kRayBatch; ++k) {
@@ -565,10 +627,20 @@ __global__ void __launch_bounds__(64)
                 get_voxel(add3(rs, mul3(dir, tn + step)), p, gv);
                 inb[k] = gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny && gv[2] < vv.Nz;
                 ib[k] = inb[k] ? ((long long)gv[1] * vv.Nx + gv[0]) * vv.Nz + gv[2] : 0;
+                safe[k] = surf.b && inb[k] && surf.b[surf.index(gv[0], gv[1], gv[2])] == 0;
                 tn += step;  // the reference's `time_curr += time_step`
             }
+            safe[kRayBatch] = false;  // the first sample of the next batch
+            if (surf.b) {
+                get_voxel(add3(rs, mul3(dir, tn + step)), p, gv);
+                safe[kRayBatch] = gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny &&
+                                  gv[2] < vv.Nz && surf.b[surf.index(gv[0], gv[1], gv[2])] == 0;
+            }
 #pragma unroll
-            for (int k = 0; k < kRayBatch; ++k) sv[k] = inb[k] ? vv.sample(ib[k]) : kNaN;
+            for (int k = 0; k < kRayBatch; ++k) {
+                const bool need = !safe[k] || !(k ? safe[k - 1] : prev_safe) || !safe[k + 1];
+                sv[k] = inb[k] ? (need ? vv.sample(ib[k]) : 1.0f) : kNaN;
+            }
             for (int k = 0; k < kRayBatch && marching; ++k) {
                 tcur = tb[k];
                 if (!(tcur < max_t) || !inb[k]) {  // loop guard / checkInds: leave the march
@@ -585,7 +657,10 @@ __global__ void __launch_bounds__(64)
                     crossing = true;
                 }
             }
-            if (marching) tcur = tn;
+            if (marching) {
+                tcur = tn;
+                prev_safe = safe[kRayBatch - 1];
+            }
         }
         if (crossing) {
             do {
@@ -625,8 +700,10 @@ __global__ void __launch_bounds__(64)
 template <class S>
 int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                      const TsdfParams& p, const NuiCLCameraParams& cam, const NuiCLRigidTransform* pose, int maxcw,
-                     const int32_t* gate, unsigned long long* updated, const float* bounds, hipStream_t st) {
+                     const int32_t* gate, unsigned long long* updated, const float* bounds, uint8_t* surface,
+                     hipStream_t st) {
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
+    const SurfaceBlocks surf{surface, surf_dim(Nx), surf_dim(Nz)};
     const bool color = bgra && color_volume;
     auto* vol = reinterpret_cast<typename S::word*>(volume);
     auto* cv = reinterpret_cast<uchar4*>(color_volume);
@@ -635,17 +712,19 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
         dim3 grid(Nx / 8, Ny / 8);
         if (color)
             integrate_tile_kernel<S, true><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                 bounds);
+                                                                 bounds, surf);
         else
             integrate_tile_kernel<S, false><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                  bounds);
+                                                                  bounds, surf);
         NUIF_LAUNCHED("integrate_tile_kernel");
     } else {
         dim3 block(16, 16), grid(div_up(Nx, 16), div_up(Ny, 16));
         if (color)
-            integrate_column_kernel<S, true><<<grid, block, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated);
+            integrate_column_kernel<S, true><<<grid, block, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
+                                                                     surf);
         else
-            integrate_column_kernel<S, false><<<grid, block, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated);
+            integrate_column_kernel<S, false><<<grid, block, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate,
+                                                                      updated, surf);
         NUIF_LAUNCHED("integrate_column_kernel");
     }
     return NUIF_OK;
@@ -654,12 +733,14 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
 template <class S>
 int launch_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams& p,
                    const NuiCLCameraParams& cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
-                   uint8_t* cmap, const int32_t* enable, hipStream_t st) {
+                   uint8_t* cmap, const int32_t* enable, const uint8_t* surface, hipStream_t st) {
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
     dim3 grid(div_up(W, 8), div_up(H, 8));
+    const SurfaceBlocks surf{const_cast<uint8_t*>(surface), surf_dim((int)p.resolution[0]),
+                             surf_dim((int)p.resolution[2])};
     raycast_kernel<S><<<grid, 64, 0, st>>>(reinterpret_cast<const typename S::word*>(volume),
                                            reinterpret_cast<const uchar4*>(color_volume), p, cam, pose, vmap, nmap,
-                                           reinterpret_cast<uchar4*>(cmap), enable);
+                                           reinterpret_cast<uchar4*>(cmap), enable, surf);
     NUIF_LAUNCHED("raycast_kernel");
     return NUIF_OK;
 }
@@ -728,7 +809,7 @@ int nuif_tsdf_reset(void* volume, uint8_t* color_volume, const TsdfParams* param
 int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                         const TsdfParams* params, const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose,
                         int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
-                        const float* depth_bounds, nuif_stream_t s) {
+                        const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     int rc = check_params(params);
     if (rc) return rc;
@@ -737,11 +818,14 @@ int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, u
     hipStream_t st = stream_of(s);
     switch (storage) {
         case NUIF_TSDF_S16:
-            return launch_integrate<StS16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, depth_bounds, st);
+            return launch_integrate<StS16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
+                                          gate, updated, depth_bounds, surface_blocks, st);
         case NUIF_TSDF_F16:
-            return launch_integrate<StF16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, depth_bounds, st);
+            return launch_integrate<StF16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
+                                          gate, updated, depth_bounds, surface_blocks, st);
         case NUIF_TSDF_F32:
-            return launch_integrate<StF32>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight, gate, updated, depth_bounds, st);
+            return launch_integrate<StF32>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
+                                          gate, updated, depth_bounds, surface_blocks, st);
         default: set_error("nuif_tsdf_integrate: unknown storage %d", storage); return NUIF_ERR_INVALID;
     }
 }
@@ -766,7 +850,8 @@ int nuif_tsdf_depth_bounds(const float* depth, int width, int height, float* bou
 
 int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
                             const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
-                            uint8_t* cmap, int storage, const int32_t* enable, nuif_stream_t s) {
+                            uint8_t* cmap, int storage, const int32_t* enable, const uint8_t* surface_blocks,
+                            nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     int rc = check_params(params);
     if (rc) return rc;
@@ -774,9 +859,12 @@ int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, con
     NUIF_CHECK_ARG(cam->depthImageWidth > 0 && cam->depthImageHeight > 0, "nuif_tsdf_raycast: empty image");
     hipStream_t st = stream_of(s);
     switch (storage) {
-        case NUIF_TSDF_S16: return launch_raycast<StS16>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable, st);
-        case NUIF_TSDF_F16: return launch_raycast<StF16>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable, st);
-        case NUIF_TSDF_F32: return launch_raycast<StF32>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable, st);
+        case NUIF_TSDF_S16: return launch_raycast<StS16>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable,
+                                                            surface_blocks, st);
+        case NUIF_TSDF_F16: return launch_raycast<StF16>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable,
+                                                            surface_blocks, st);
+        case NUIF_TSDF_F32: return launch_raycast<StF32>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable,
+                                                            surface_blocks, st);
         default: set_error("nuif_tsdf_raycast: unknown storage %d", storage); return NUIF_ERR_INVALID;
     }
 }
@@ -784,7 +872,22 @@ int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, con
 int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
                       const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
                       uint8_t* cmap, int storage, nuif_stream_t s) {
-    return nuif_tsdf_raycast_gated(volume, color_volume, params, cam, pose, vmap, nmap, cmap, storage, nullptr, s);
+    return nuif_tsdf_raycast_gated(volume, color_volume, params, cam, pose, vmap, nmap, cmap, storage, nullptr,
+                                   nullptr, s);
+}
+
+size_t nuif_tsdf_surface_blocks_bytes(const TsdfParams* params) {
+    if (!params || check_params(params)) return 0;
+    return (size_t)surf_dim((int)params->resolution[0]) * surf_dim((int)params->resolution[1]) *
+           surf_dim((int)params->resolution[2]);
+}
+
+int nuif_tsdf_surface_blocks_clear(uint8_t* surface_blocks, const TsdfParams* params, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(params);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(surface_blocks != nullptr, "nuif_tsdf_surface_blocks_clear: null buffer");
+    return nuif_fill_u8(surface_blocks, 0, nuif_tsdf_surface_blocks_bytes(params), s);
 }
 
 }  // extern "C"

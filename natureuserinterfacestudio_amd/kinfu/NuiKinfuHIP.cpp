@@ -248,6 +248,7 @@ NuiKinfuHIPScene::NuiKinfuHIPScene(const NuiKinfuVolumeConfig& config, int stora
     m_volume = DeviceBuffer(n * vb);
     if (config.bHas_color_volume) m_color = DeviceBuffer(n * 4);
     m_counter = DeviceBuffer(sizeof(unsigned long long));
+    m_surface = DeviceBuffer(nuif_tsdf_surface_blocks_bytes(&m_params));
     reset();
 }
 
@@ -265,6 +266,8 @@ float NuiKinfuHIPScene::getVoxelLeafSize() const {
 // OCL/NuiKinfuOpenCLScene.cpp:93-137
 void NuiKinfuHIPScene::reset() {
     check(nuif_tsdf_reset(m_volume.get(), m_color.as<uint8_t>(), &m_params, m_storage, m_stream), "nuif_tsdf_reset");
+    check(nuif_tsdf_surface_blocks_clear(m_surface.as<uint8_t>(), &m_params, m_stream),
+          "nuif_tsdf_surface_blocks_clear");
     setDirty();
 }
 
@@ -282,7 +285,8 @@ bool NuiKinfuHIPScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraStat
     const float* bounds = f->IsDepthScaled() ? f->GetDepthBounds() : nullptr;
     check(nuif_tsdf_integrate(f->GetDepthBuffer(), colors, m_volume.get(), colors ? m_color.as<uint8_t>() : nullptr,
                               &m_params, &c, cam->GetCameraTransformBuffer(), m_storage, m_config.max_color_weight,
-                              m_gateIntegrate, m_counter.as<unsigned long long>(), bounds, m_stream),
+                              m_gateIntegrate, m_counter.as<unsigned long long>(), bounds, m_surface.as<uint8_t>(),
+                              m_stream),
           "nuif_tsdf_integrate");
     setDirty();
     return true;
@@ -296,7 +300,8 @@ void NuiKinfuHIPScene::raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiK
     const NuiCLCameraParams& c = cam->GetCameraParams();
     check(nuif_tsdf_raycast_gated(m_volume.get(), m_color.as<uint8_t>(), &m_params, &c,
                                   cam->GetCameraTransformBuffer(), fb->GetVertexBuffer(), fb->GetNormalBuffer(),
-                                  m_color.get() ? fb->GetColorBuffer() : nullptr, m_storage, m_gateRaycast, m_stream),
+                                  m_color.get() ? fb->GetColorBuffer() : nullptr, m_storage, m_gateRaycast,
+                                  m_surface.as<uint8_t>(), m_stream),
           "nuif_tsdf_raycast");
 }
 

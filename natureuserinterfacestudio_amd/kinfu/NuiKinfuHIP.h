@@ -237,6 +237,7 @@ private:
     nuif_stream_t m_stream;
     TsdfParams m_params{};
     DeviceBuffer m_volume, m_color, m_counter;
+    DeviceBuffer m_surface;  // surface blocks of the volume (nuif_tsdf_surface_blocks_bytes)
 };
 
 // OCL/NuiKinfuOpenCLHashScene.*: voxel-hashing scene.  The render is castRay over

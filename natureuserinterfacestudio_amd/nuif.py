@@ -145,13 +145,16 @@ _SIGS = {
     "nuif_tsdf_reset": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p]),
     "nuif_tsdf_integrate": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(TsdfParams),
                                     C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_int, c_void_p, c_void_p,
-                                    c_void_p, c_void_p]),
+                                    c_void_p, c_void_p, c_void_p]),
     "nuif_tsdf_depth_bounds_bytes": (c_size_t, [c_int, c_int]),
     "nuif_tsdf_depth_bounds": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "nuif_tsdf_raycast": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "nuif_tsdf_raycast_gated": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),
-                                        c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                        c_void_p]),
+    "nuif_tsdf_surface_blocks_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
+    "nuif_tsdf_surface_blocks_clear": (c_int, [c_void_p, C.POINTER(TsdfParams), c_void_p]),
     "nuif_scan_scratch_bytes": (c_size_t, [c_size_t]),
     "nuif_exclusive_scan_u32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
     "nuif_exclusive_scan_flags": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),

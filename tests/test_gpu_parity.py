@@ -279,13 +279,51 @@ def test_integrate_bit_exact_s16(gpu, res, bounds):
             mx = db.download().view(np.float32)[-1]
             assert mx == np.nanmax(dep)
         nuif.check(gpu.L().nuif_tsdf_integrate(ddep.ptr, None, dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr,
-                                               0, 128, None, None, db.ptr if db else None, None))
+                                               0, 128, None, None, db.ptr if db else None, None, None))
         O.lib().or_integrate_s16_f(dep, None, vol.reshape(-1), None, N, wl.volume_dim, wl.trunc, wl.cam6(),
                                    wl.width, wl.height, R.reshape(-1).copy(), t, 128)
     got = dvol.download()
     same = np.all(got == vol, axis=1)
     assert (vol[:, 1] > 0).sum() > 10000
     assert same.mean() == 1.0, f"{(~same).sum()} voxels differ"
+
+
+def _surface_blocks_of(vol, N):
+    """Blocks of 8^3 voxels holding an observed voxel with tsdf <= 0 (y-major, z fastest)."""
+    v = vol.reshape(N, N, N, 2)  # (y, x, z)
+    neg = (v[..., 1] > 0) & (v[..., 0] <= 0)
+    nb = (N + 7) // 8
+    pad = np.zeros((nb * 8, nb * 8, nb * 8), bool)
+    pad[:N, :N, :N] = neg
+    return pad.reshape(nb, 8, nb, 8, nb, 8).any(axis=(1, 3, 5)).astype(np.uint8).reshape(-1)
+
+
+@pytest.mark.parametrize("res", [128, 96])  # 96^3: generic column kernel
+def test_surface_blocks_mark_stored_nonpositive_tsdf(gpu, res):
+    """Three integrations of one frame at one pose: the set of voxels holding tsdf <= 0
+    never changes, so the blocks the integration marks are exactly those holding one."""
+    announce(f"surface blocks {res}^3 320x240, 3 integrations of one frame")
+    wl = W.SMALL
+    N = res
+    p = nuif.tsdf_params(N, wl.volume_dim, wl.trunc)
+    dvol = nuif.DeviceArray((N ** 3 * 4,), np.uint8)
+    nuif.check(gpu.L().nuif_tsdf_reset(dvol.ptr, None, C.byref(p), 0, None))
+    nb = gpu.L().nuif_tsdf_surface_blocks_bytes(C.byref(p))
+    assert nb == ((N + 7) // 8) ** 3
+    surf = nuif.DeviceArray((nb,), np.uint8)
+    nuif.check(gpu.L().nuif_fill_u8(surf.ptr, 7, nb, None))
+    nuif.check(gpu.L().nuif_tsdf_surface_blocks_clear(surf.ptr, C.byref(p), None))
+    cam = wl.camera_params()
+    dep = _small_volume_inputs(wl, 2)
+    ddep = nuif.DeviceArray.from_host(dep)
+    pose = nuif.upload_struct(nuif.rigid(np.eye(3), np.zeros(3)))
+    for _ in range(3):
+        nuif.check(gpu.L().nuif_tsdf_integrate(ddep.ptr, None, dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr,
+                                               0, 128, None, None, None, surf.ptr, None))
+    expect = _surface_blocks_of(dvol.download().view(np.int16).reshape(-1, 2), N)
+    got = surf.download()
+    assert expect.sum() > 100
+    np.testing.assert_array_equal(got, expect)
 
 
 @pytest.mark.parametrize("storage", [1, 2])
@@ -307,7 +345,7 @@ def test_integrate_f16_f32_close_to_fp32_math(gpu, storage):
     pose = nuif.upload_struct(nuif.rigid(R, t))
     for _ in range(3):
         nuif.check(gpu.L().nuif_tsdf_integrate(ddep.ptr, None, dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr,
-                                               storage, 128, None, None, None, None))
+                                               storage, 128, None, None, None, None, None))
         O.lib().or_integrate_s16_f(dep, None, vol.reshape(-1), None, N, wl.volume_dim, wl.trunc, wl.cam6(),
                                    wl.width, wl.height, R.reshape(-1).copy(), t, 128)
     tg, wg = nuif.decode_volume(dvol.download(), storage)
@@ -349,6 +387,43 @@ def test_raycast_matches_oracle(gpu):
 
 
 # ------------------------------------------------------------------ end to end
+
+
+@pytest.mark.parametrize("extra", [0.0, 0.3])
+def test_raycast_surface_blocks_identical(gpu, extra):
+    """The raycast with surface blocks (exact, or with 30 % of the other blocks marked
+    too) gives the oracle's maps bit for bit."""
+    announce(f"raycast S16 128^3 with surface blocks (extra {extra})")
+    wl = W.SMALL
+    e = _oracle_state(wl, 4)
+    vol = e.volume_raw()
+    N = wl.volume_res
+    p = nuif.tsdf_params(N, wl.volume_dim, wl.trunc)
+    cam = wl.camera_params()
+    _, _, gt = wl.render(4)
+    R = gt[:, :3].astype(np.float32).copy()
+    t = gt[:, 3].astype(np.float32).copy()
+    blocks = _surface_blocks_of(vol.reshape(-1, 2), N)
+    rng = np.random.default_rng(4)
+    blocks = blocks | (rng.random(blocks.size) < extra).astype(np.uint8)
+    assert 0 < blocks.mean() < 0.5 or extra > 0
+    dvol = nuif.DeviceArray.from_host(vol)
+    dsurf = nuif.DeviceArray.from_host(blocks)
+    pose = nuif.upload_struct(nuif.rigid(R, t))
+    vm = nuif.DeviceArray((wl.width * wl.height, 3), np.float32)
+    nm = nuif.DeviceArray((wl.width * wl.height, 3), np.float32)
+    nuif.check(gpu.L().nuif_tsdf_raycast_gated(dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr, vm.ptr, nm.ptr,
+                                               None, 0, None, dsurf.ptr, None))
+    vr = np.empty(wl.width * wl.height * 3, np.float32)
+    nr = np.empty_like(vr)
+    O.lib().or_raycast_s16_f(vol, None, N, wl.volume_dim, wl.trunc, wl.cam6(), wl.width, wl.height,
+                             R.reshape(-1).copy(), t, vr, nr, None)
+    vg, ng = vm.download().reshape(-1), nm.download().reshape(-1)
+    assert (~np.isnan(vr)).sum() > 3 * 5000
+    np.testing.assert_array_equal(np.isnan(vg), np.isnan(vr))
+    np.testing.assert_array_equal(vg[~np.isnan(vr)], vr[~np.isnan(vr)])
+    np.testing.assert_array_equal(np.isnan(ng), np.isnan(nr))
+    np.testing.assert_array_equal(ng[~np.isnan(nr)], nr[~np.isnan(nr)])
 
 def _run_engine(wl, frames, storage=0, use_graph=False, gate=0.0):
     eng = nuif.Engine(wl.kinfu_desc(storage=storage, gate=gate, use_graph=use_graph))
