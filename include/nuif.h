@@ -224,7 +224,7 @@ int nuif_tsdf_depth_bounds(const float* depth, int width, int height, float* bou
 /* K/tsdfVolume.cl:31-190 integrateTsdfVolumeKernel.  depth: ray-length depth (scaled);
  * bgra/color_volume may be NULL; gate: optional device int32, integrate only if non-zero;
  * updated: optional device counter, incremented by the number of voxels updated;
- * depth_bounds: optional nuif_tsdf_depth_bounds of the same depth -- 64-voxel runs of a
+ * depth_bounds: optional nuif_tsdf_depth_bounds of the same depth -- 32-voxel runs of a
  * column that lie more than the truncation behind every depth they can project to are
  * skipped (the reference's kernel skips each of their voxels; results are identical). */
 int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,

@@ -102,10 +102,9 @@ __global__ void reset_kernel(typename S::word* __restrict__ vol, uint32_t* __res
 
 // ---- integration ---------------------------------------------------------------
 
-#ifndef NUIF_ZCHUNK
-#define NUIF_ZCHUNK 64
-#endif
-constexpr int kZChunk = NUIF_ZCHUNK;
+// Column chunk of the integration: 32 voxels keep the LDS transpose buffer at 8.3 KB
+// per 64-column wave (5 waves per SIMD; 64 voxels allowed 3 and ran 20 % slower).
+constexpr int kZChunk = 32;
 constexpr int kRayBatch = 8;  // raycast samples whose loads are in flight together
 constexpr int kLdsPitch = kZChunk + 1;
 
@@ -323,21 +322,9 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
     return dmax + trunc < dmin * (1.0f - 1e-5f) - 2e-4f;
 }
 
-#ifndef NUIF_WALK_BATCH
-#define NUIF_WALK_BATCH 1
-#endif
-#ifndef NUIF_INTEGRATE_BATCH
-#define NUIF_INTEGRATE_BATCH 8
-#endif
-#ifdef NUIF_INTEGRATE_WPE
-#define NUIF_INTEGRATE_ATTR __attribute__((amdgpu_waves_per_eu(NUIF_INTEGRATE_WPE)))
-#else
-#define NUIF_INTEGRATE_ATTR
-#endif
-
-// Fast path: Nx, Ny multiples of 8, Nz multiple of 64.
+// Fast path: Nx, Ny multiples of 8, Nz a multiple of the column chunk.
 template <class S, bool COLOR>
-__global__ void __launch_bounds__(64) NUIF_INTEGRATE_ATTR
+__global__ void __launch_bounds__(64)
     integrate_tile_kernel(const float* __restrict__ depth, const uchar4* __restrict__ bgra,
                           typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
                           NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
@@ -349,8 +336,8 @@ __global__ void __launch_bounds__(64) NUIF_INTEGRATE_ATTR
     constexpr int LPC = kZChunk / VPL;        // lanes covering one column chunk
     constexpr int CPI = 64 / LPC;             // columns per vector instruction
     constexpr int NIT = 64 / CPI;             // instructions per chunk
-    constexpr int BATCH = NUIF_INTEGRATE_BATCH;
-    constexpr int KB = NUIF_WALK_BATCH;
+    constexpr int BATCH = 2;  // column chunks of phase 2 whose loads are in flight together
+    constexpr int KB = 8;     // walk steps whose depth reads are in flight together
     static_assert(kZChunk % KB == 0, "walk batches tile the chunk");
     static_assert(NIT % BATCH == 0 && kZChunk % VPL == 0, "phase 2 covers the 64 columns exactly");
     __shared__ float ts[64 * kLdsPitch];
