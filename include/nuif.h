@@ -241,10 +241,12 @@ int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, con
                             float* nmap, uint8_t* cmap, int storage, const int32_t* enable,
                             const uint8_t* surface_blocks, nuif_stream_t s);
 /* Surface blocks of a volume: one byte per 8 x 8 x 8 voxel block (y-major, z fastest),
- * set by nuif_tsdf_integrate (surface_blocks argument) when it stores a tsdf <= 0 and
- * cleared only by nuif_tsdf_surface_blocks_clear (with the volume's reset).  The raycast
- * does not read the voxels of blocks left at zero where its exit tests cannot need their
- * values; its output is identical with or without them (NULL). */
+ * then one byte per 32 x 32 x 32 superblock (the OR of its blocks), set by
+ * nuif_tsdf_integrate (surface_blocks argument) when it stores a tsdf <= 0 and cleared
+ * only by nuif_tsdf_surface_blocks_clear (with the volume's reset).  The raycast does not
+ * read the voxels of blocks left at zero where its exit tests cannot need their values,
+ * and crosses clear superblocks without locating its samples; its output is identical
+ * with or without them (NULL). */
 size_t nuif_tsdf_surface_blocks_bytes(const TsdfParams* params);
 int nuif_tsdf_surface_blocks_clear(uint8_t* surface_blocks, const TsdfParams* params, nuif_stream_t s);
 /* Effective truncation distance (OCL/NuiKinfuOpenCLScene.cpp:41: >= 2.1 * max cell). */

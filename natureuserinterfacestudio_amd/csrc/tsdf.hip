@@ -37,6 +37,7 @@ struct StS16 {
     static constexpr int kBytes = 4;
     __device__ static word init() { return 0x00007fffu; }  // (SHORT_NAN 32767, 0)
     __device__ static float tsdf(word w) { return (float)(int16_t)(w & 0xffffu) / 32766.0f; }
+    __device__ static bool nonpositive(word w) { return (int16_t)(w & 0xffffu) <= 0; }  // tsdf(w) <= 0
     __device__ static int weight(word w) { return (int16_t)(w >> 16); }
     __device__ static word pack(float f, int wt) {
         const int16_t q = (int16_t)(int)(f * 32766.0f);
@@ -49,6 +50,8 @@ struct StF16 {
     static constexpr int kBytes = 4;
     __device__ static word init() { return (uint32_t)__half_as_ushort(__float2half_rn(1.0f)); }
     __device__ static float tsdf(word w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
+    // tsdf(w) <= 0: sign bit set or +0 (no NaN is ever stored)
+    __device__ static bool nonpositive(word w) { return (w & 0x8000u) || (w & 0x7fffu) == 0; }
     __device__ static int weight(word w) { return (int)(w >> 16); }
     __device__ static word pack(float f, int wt) {
         return (uint32_t)__half_as_ushort(__float2half_rn(f)) | ((uint32_t)(uint16_t)wt << 16);
@@ -60,6 +63,7 @@ struct StF32 {
     static constexpr int kBytes = 8;
     __device__ static word init() { return make_uint2(__float_as_uint(1.0f), 0u); }
     __device__ static float tsdf(word w) { return __uint_as_float(w.x); }
+    __device__ static bool nonpositive(word w) { return !(__uint_as_float(w.x) > 0.f); }
     __device__ static int weight(word w) { return (int)__uint_as_float(w.y); }
     __device__ static word pack(float f, int wt) { return make_uint2(__float_as_uint(f), __float_as_uint((float)wt)); }
 };
@@ -105,23 +109,43 @@ __global__ void reset_kernel(typename S::word* __restrict__ vol, uint32_t* __res
 // Column chunk of the integration: 32 voxels keep the LDS transpose buffer at 8.3 KB
 // per 64-column wave (5 waves per SIMD; 64 voxels allowed 3 and ran 20 % slower).
 constexpr int kZChunk = 32;
+// Columns are split into z segments walked by separate waves: tiles that look through
+// long runs of free space would otherwise set the kernel's duration alone.
+constexpr int kSegVoxels = 128;
+static_assert(kSegVoxels % kZChunk == 0, "segments hold whole chunks");
 constexpr int kRayBatch = 8;  // raycast samples whose loads are in flight together
 constexpr int kLdsPitch = kZChunk + 1;
 
 // Surface blocks: one byte per 8 x 8 x 8 voxel block (same y-major, z-fastest order
 // as the voxels), set by the integration when it stores a tsdf <= 0 and never
 // cleared, so a zero byte guarantees that every voxel of the block reads positive
-// or unobserved.  The raycast reads the voxels of such blocks only where its exit
-// tests can need their values.
-constexpr int kSurfBlock = 8;
+// or unobserved.  A second level holds one byte per 32 x 32 x 32 superblock (the OR
+// of its blocks).  The raycast reads the voxels of clear blocks only where its exit
+// tests can need their values, and crosses the inside of clear superblocks without
+// locating its samples.
+constexpr int kSurfBlock = 8, kSurfSuper = 32;
 struct SurfaceBlocks {
-    uint8_t* b;
-    int nbx, nbz;  // blocks along x and z
+    uint8_t* b;     // blocks, then superblocks at b + nb
+    int nbx, nbz;   // blocks along x and z
+    size_t nb;      // block count
+    int nsx, nsz;   // superblocks along x and z
     __device__ __forceinline__ size_t index(int x, int y, int z) const {
         return ((size_t)(y >> 3) * nbx + (x >> 3)) * nbz + (z >> 3);
     }
+    __device__ __forceinline__ size_t sindex(int x, int y, int z) const {
+        return nb + ((size_t)(y >> 5) * nsx + (x >> 5)) * nsz + (z >> 5);
+    }
+    __device__ __forceinline__ void mark(int x, int y, int z) const {
+        b[index(x, y, z)] = 1;
+        b[sindex(x, y, z)] = 1;
+    }
 };
 __host__ __device__ __forceinline__ int surf_dim(int n) { return (n + kSurfBlock - 1) / kSurfBlock; }
+__host__ __device__ __forceinline__ int super_dim(int n) { return (n + kSurfSuper - 1) / kSurfSuper; }
+__host__ __device__ __forceinline__ SurfaceBlocks surface_blocks_of(uint8_t* b, int Nx, int Ny, int Nz) {
+    return SurfaceBlocks{b, surf_dim(Nx), surf_dim(Nz), (size_t)surf_dim(Nx) * surf_dim(Ny) * surf_dim(Nz),
+                         super_dim(Nx), super_dim(Nz)};
+}
 
 struct ColumnWalk {  // the reference's per-column incremental state
     float vgz, part, vx, vy, vz, zs, incx, incy;
@@ -352,7 +376,12 @@ __global__ void __launch_bounds__(64)
 
     const int sub = lane % LPC;  // position inside a column chunk (phase 2)
     unsigned nupd = 0;
-    for (int zc = 0; zc < Nz; zc += kZChunk) {
+    // z segment of this wave (blockIdx.z): the walk state at its start is replayed
+    // with the same sequence of additions, so every voxel sees the same values
+    const int zbeg = blockIdx.z * kSegVoxels, zend = min(Nz, zbeg + kSegVoxels);
+#pragma unroll 1
+    for (int k = 0; k < zbeg; ++k) column_advance(c, cz);
+    for (int zc = zbeg; zc < zend; zc += kZChunk) {
         // phase 1: serial walk of this lane's column; chunks that cannot project
         // into the image only advance the walk
         const ChunkFootprint fp = chunk_footprint(c, R8, cz, kZChunk, cam);
@@ -432,11 +461,11 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
                 for (int q = 0; q < VPL; ++q) {
                     ov[q] = isnan(tv[q]) ? buf[j][q] : update_voxel<S>(buf[j][q], tv[q]);
-                    neg |= !isnan(tv[q]) && !(S::tsdf(ov[q]) > 0.f);
+                    neg |= !isnan(tv[q]) && S::nonpositive(ov[q]);
                 }
                 *reinterpret_cast<uint4*>(vol + base) = outv;
                 // the VPL voxels share one 8-voxel z block (VPL divides 8)
-                if (neg && surf.b) surf.b[surf.index(gx, gy, zc + sub * VPL)] = 1;
+                if (neg && surf.b) surf.mark(gx, gy, zc + sub * VPL);
                 if (COLOR) {
                     const int* pv = &cp[col * kLdsPitch + sub * VPL];
 #pragma unroll
@@ -481,7 +510,7 @@ __global__ void integrate_column_kernel(const float* __restrict__ depth, const u
         ++nupd;
         const typename S::word nw = update_voxel<S>(vol[base + z], v);
         vol[base + z] = nw;
-        if (surf.b && !(S::tsdf(nw) > 0.f)) surf.b[surf.index(x, y, z)] = 1;
+        if (surf.b && S::nonpositive(nw)) surf.mark(x, y, z);
         if (COLOR && pix >= 0) {
             const uchar4 nc = bgra[pix];
             if (nc.w != 0) cvol[base + z] = update_color(cvol[base + z], nc, maxcw);
@@ -507,6 +536,26 @@ __device__ __forceinline__ void get_voxel(f3 pt, const TsdfParams& p, int g[3]) 
     g[2] = to_int(roundf(pt.z / p.cell_size[2]));
 }
 
+// to_int(roundf(x / c)) without the division where its rounding is decided anyway:
+// a = x * rc (rc = 1 / c rounded) is within 2^-22 |x / c| of the rounded quotient, so
+// when a lies farther than 2^-20 max(|a|, 1) from every half-integer both round to the
+// same integer; otherwise (or for NaN / huge values) the quotient is computed.
+__device__ __forceinline__ int round_div(float x, float c, float rc) {
+    const float a = x * rc;
+    const float r = roundf(a);
+    const float d = fabsf(fabsf(a - r) - 0.5f);
+    if (d > 9.5367431640625e-7f * fmaxf(fabsf(a), 1.0f)) return to_int(r);
+    return to_int(roundf(x / c));
+}
+struct CellInv {
+    float x, y, z;
+};
+__device__ __forceinline__ void get_voxel_fast(f3 pt, const TsdfParams& p, const CellInv& ri, int g[3]) {
+    g[0] = round_div(pt.x, p.cell_size[0], ri.x);
+    g[1] = round_div(pt.y, p.cell_size[1], ri.y);
+    g[2] = round_div(pt.z, p.cell_size[2], ri.z);
+}
+
 // K/raycast.cl:36-120 interpolateTrilineary, corner weights exactly as written
 // (the x and y fractions land on swapped corners relative to the y-major layout).
 template <class S> __device__ float trilinear(f3 pt, const VolView<S>& vv, const TsdfParams& p) {
@@ -526,22 +575,15 @@ template <class S> __device__ float trilinear(f3 pt, const VolView<S>& vv, const
     const float c = (pt.z - ((float)g[2] + 0.5f) * p.cell_size[2]) / p.cell_size[2];
     const long long sx = vv.Nz, sy = (long long)vv.Nx * vv.Nz;
     const long long idx = ((long long)g[1] * vv.Nx + g[0]) * vv.Nz + g[2];
-    const float f000 = vv.sample(idx);
-    if (isnan(f000)) return kNaN;
-    const float f001 = vv.sample(idx + 1);
-    if (isnan(f001)) return kNaN;
-    const float f011 = vv.sample(idx + sx + 1);
-    if (isnan(f011)) return kNaN;
-    const float f010 = vv.sample(idx + sx);
-    if (isnan(f010)) return kNaN;
-    const float f110 = vv.sample(idx + sx + sy);
-    if (isnan(f110)) return kNaN;
-    const float f111 = vv.sample(idx + sx + sy + 1);
-    if (isnan(f111)) return kNaN;
-    const float f101 = vv.sample(idx + sy + 1);
-    if (isnan(f101)) return kNaN;
-    const float f100 = vv.sample(idx + sy);
-    if (isnan(f100)) return kNaN;
+    // the eight corners (all inside the volume) are read together; any unobserved
+    // corner makes the result NaN, as the reference's early returns do
+    const float f000 = vv.sample(idx), f001 = vv.sample(idx + 1);
+    const float f011 = vv.sample(idx + sx + 1), f010 = vv.sample(idx + sx);
+    const float f110 = vv.sample(idx + sx + sy), f111 = vv.sample(idx + sx + sy + 1);
+    const float f101 = vv.sample(idx + sy + 1), f100 = vv.sample(idx + sy);
+    if (isnan(f000) || isnan(f001) || isnan(f011) || isnan(f010) || isnan(f110) || isnan(f111) || isnan(f101) ||
+        isnan(f100))
+        return kNaN;
     return f000 * (1 - a) * (1 - b) * (1 - c) + f001 * (1 - a) * (1 - b) * c + f010 * (1 - a) * b * (1 - c) +
            f011 * (1 - a) * b * c + f100 * a * (1 - b) * (1 - c) + f101 * a * (1 - b) * c +
            f110 * a * b * (1 - c) + f111 * a * b * c;
@@ -558,6 +600,32 @@ __device__ __forceinline__ float max_time(const float* vmax, f3 o, f3 d) {
     const float ty = fabs((double)d.y) < 1e-14 ? 1e14f : ((d.y > 0 ? vmax[1] : 0.f) - o.y) / d.y;
     const float tz = fabs((double)d.z) < 1e-14 ? 1e14f : ((d.z > 0 ? vmax[2] : 0.f) - o.z) / d.z;
     return fminf(fminf(tx, ty), tz);
+}
+
+// Ray parameters u with rs + dir * u inside the superblock of voxel g shrunk by one
+// voxel on every side (voxel i spans [(i - 1/2) c, (i + 1/2) c)): [*u_in, *u_out].
+// The one-voxel margin dwarfs the rounding of the sample positions and of these
+// bounds, so every sample with u in the span lies in the superblock.
+__device__ __forceinline__ void superblock_span(const int g[3], const int N[3], const float c[3], f3 rs, f3 dir,
+                                                float* u_in, float* u_out) {
+    const float r[3] = {rs.x, rs.y, rs.z}, d[3] = {dir.x, dir.y, dir.z};
+    float lo_t = -3.402823466e38f, hi_t = 3.402823466e38f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int x0 = g[a] & ~(kSurfSuper - 1), x1 = min(x0 + kSurfSuper - 1, N[a] - 1);
+        const float lo = ((float)x0 + 0.5f) * c[a], hi = ((float)x1 - 0.5f) * c[a];
+        if (d[a] > 0.f) {
+            lo_t = fmaxf(lo_t, (lo - r[a]) / d[a]);
+            hi_t = fminf(hi_t, (hi - r[a]) / d[a]);
+        } else if (d[a] < 0.f) {
+            lo_t = fmaxf(lo_t, (hi - r[a]) / d[a]);
+            hi_t = fminf(hi_t, (lo - r[a]) / d[a]);
+        } else if (!(r[a] >= lo && r[a] <= hi)) {
+            hi_t = -3.402823466e38f;
+        }
+    }
+    *u_in = lo_t;
+    *u_out = hi_t;
 }
 
 template <class S>
@@ -579,6 +647,7 @@ __global__ void __launch_bounds__(64)
     f3 rn = mk3(((float)x - cam.cx) * cam.fx_inv, ((float)y - cam.cy) * cam.fy_inv, 1.f);
     rn = add3(add3(rot3(g.R, rn), mk3(g.t[0], g.t[1], g.t[2])), off);
     const f3 dir = normalize3(sub3(rn, rs));
+    const CellInv ri{1.0f / p.cell_size[0], 1.0f / p.cell_size[1], 1.0f / p.cell_size[2]};
     float tcur = min_time(p.dimension, rs, dir);
     const float texit = max_time(p.dimension, rs, dir);
     tcur = fmaxf(tcur, 0.f);
@@ -603,7 +672,37 @@ __global__ void __launch_bounds__(64)
         // own step or at the next one, whatever its exact value; such a sample is
         // not read and stands in as +1.
         bool marching = true, crossing = false, prev_safe = false;
+        const int Nv[3] = {vv.Nx, vv.Ny, vv.Nz};
         while (marching) {
+            // Inside clear superblocks no sample can trigger an exit test when the
+            // previous sample is not negative: only the times advance (the same float
+            // additions), and the last crossed sample is read for the next test.  A
+            // sample inside the current superblock's one-voxel-shrunk box is known to be
+            // in it; any other sample is located exactly and continues the crossing
+            // only if its superblock is clear too.
+            if (surf.b && !(tsdf < 0.f)) {
+                float tn = tcur, s_in = 1.f, s_out = 0.f;  // empty span
+                while (tn < max_t) {
+                    const float un = tn + step;
+                    if (!(un >= s_in && un <= s_out)) {
+                        int gq[3];
+                        get_voxel_fast(add3(rs, mul3(dir, un)), p, ri, gq);
+                        if (!(gq[0] >= 0 && gq[1] >= 0 && gq[2] >= 0 && gq[0] < vv.Nx && gq[1] < vv.Ny &&
+                              gq[2] < vv.Nz) ||
+                            surf.b[surf.sindex(gq[0], gq[1], gq[2])] != 0)
+                            break;
+                        superblock_span(gq, Nv, p.cell_size, rs, dir, &s_in, &s_out);
+                    }
+                    tn = un;
+                }
+                if (tn != tcur) {
+                    int gs[3];
+                    get_voxel_fast(add3(rs, mul3(dir, tn)), p, ri, gs);
+                    tsdf = vv.sample(((long long)gs[1] * vv.Nx + gs[0]) * vv.Nz + gs[2]);
+                    tcur = tn;
+                    prev_safe = true;
+                }
+            }
             float tb[kRayBatch], sv[kRayBatch];
             long long ib[kRayBatch];
             bool inb[kRayBatch], safe[kRayBatch + 1];
@@ -611,7 +710,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
             for (int k = 0; k < kRayBatch; ++k) {
                 tb[k] = tn;
-                get_voxel(add3(rs, mul3(dir, tn + step)), p, gv);
+                get_voxel_fast(add3(rs, mul3(dir, tn + step)), p, ri, gv);
                 inb[k] = gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny && gv[2] < vv.Nz;
                 ib[k] = inb[k] ? ((long long)gv[1] * vv.Nx + gv[0]) * vv.Nz + gv[2] : 0;
                 safe[k] = surf.b && inb[k] && surf.b[surf.index(gv[0], gv[1], gv[2])] == 0;
@@ -619,7 +718,7 @@ __global__ void __launch_bounds__(64)
             }
             safe[kRayBatch] = false;  // the first sample of the next batch
             if (surf.b) {
-                get_voxel(add3(rs, mul3(dir, tn + step)), p, gv);
+                get_voxel_fast(add3(rs, mul3(dir, tn + step)), p, ri, gv);
                 safe[kRayBatch] = gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny &&
                                   gv[2] < vv.Nz && surf.b[surf.index(gv[0], gv[1], gv[2])] == 0;
             }
@@ -651,10 +750,10 @@ __global__ void __launch_bounds__(64)
         }
         if (crossing) {
             do {
+                // both interpolations are evaluated (pure reads) so that their loads overlap
                 const float Ftdt = trilinear<S>(add3(rs, mul3(dir, tcur + step)), vv, p);
-                if (isnan(Ftdt)) break;
                 const float Ft = trilinear<S>(add3(rs, mul3(dir, tcur)), vv, p);
-                if (isnan(Ft)) break;
+                if (isnan(Ftdt) || isnan(Ft)) break;
                 const float Ts = tcur - step * Ft / (Ftdt - Ft);
                 const f3 vf = add3(rs, mul3(dir, Ts));
                 vout = sub3(vf, off);
@@ -690,13 +789,13 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
                      const int32_t* gate, unsigned long long* updated, const float* bounds, uint8_t* surface,
                      hipStream_t st) {
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
-    const SurfaceBlocks surf{surface, surf_dim(Nx), surf_dim(Nz)};
+    const SurfaceBlocks surf = surface_blocks_of(surface, Nx, Ny, Nz);
     const bool color = bgra && color_volume;
     auto* vol = reinterpret_cast<typename S::word*>(volume);
     auto* cv = reinterpret_cast<uchar4*>(color_volume);
     auto* bg = reinterpret_cast<const uchar4*>(bgra);
     if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0) {
-        dim3 grid(Nx / 8, Ny / 8);
+        dim3 grid(Nx / 8, Ny / 8, div_up(Nz, kSegVoxels));
         if (color)
             integrate_tile_kernel<S, true><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
                                                                  bounds, surf);
@@ -723,8 +822,8 @@ int launch_raycast(const void* volume, const uint8_t* color_volume, const TsdfPa
                    uint8_t* cmap, const int32_t* enable, const uint8_t* surface, hipStream_t st) {
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
     dim3 grid(div_up(W, 8), div_up(H, 8));
-    const SurfaceBlocks surf{const_cast<uint8_t*>(surface), surf_dim((int)p.resolution[0]),
-                             surf_dim((int)p.resolution[2])};
+    const SurfaceBlocks surf = surface_blocks_of(const_cast<uint8_t*>(surface), (int)p.resolution[0],
+                                                 (int)p.resolution[1], (int)p.resolution[2]);
     raycast_kernel<S><<<grid, 64, 0, st>>>(reinterpret_cast<const typename S::word*>(volume),
                                            reinterpret_cast<const uchar4*>(color_volume), p, cam, pose, vmap, nmap,
                                            reinterpret_cast<uchar4*>(cmap), enable, surf);
@@ -865,8 +964,8 @@ int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const Tsd
 
 size_t nuif_tsdf_surface_blocks_bytes(const TsdfParams* params) {
     if (!params || check_params(params)) return 0;
-    return (size_t)surf_dim((int)params->resolution[0]) * surf_dim((int)params->resolution[1]) *
-           surf_dim((int)params->resolution[2]);
+    const int nx = (int)params->resolution[0], ny = (int)params->resolution[1], nz = (int)params->resolution[2];
+    return (size_t)surf_dim(nx) * surf_dim(ny) * surf_dim(nz) + (size_t)super_dim(nx) * super_dim(ny) * super_dim(nz);
 }
 
 int nuif_tsdf_surface_blocks_clear(uint8_t* surface_blocks, const TsdfParams* params, nuif_stream_t s) {

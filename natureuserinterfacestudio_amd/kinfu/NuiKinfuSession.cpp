@@ -71,7 +71,8 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
     } else {
         const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
         bytes += voxels * nuif_tsdf_voxel_bytes(d.storage) + (d.color ? voxels * 4 : 0);
-        bytes += (size_t)((d.resolution[0] + 7) / 8) * ((d.resolution[1] + 7) / 8) * ((d.resolution[2] + 7) / 8);
+        bytes += (size_t)((d.resolution[0] + 7) / 8) * ((d.resolution[1] + 7) / 8) * ((d.resolution[2] + 7) / 8) +
+                 (size_t)((d.resolution[0] + 31) / 32) * ((d.resolution[1] + 31) / 32) * ((d.resolution[2] + 31) / 32);
     }
     bytes += n * (2 + 4 + 4);  // raw, passing-filter depth, colours
     for (int l = 0; l < d.levels; ++l) bytes += (size_t)(d.width >> l) * (d.height >> l) * (4 + 12);

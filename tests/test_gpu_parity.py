@@ -298,6 +298,15 @@ def _surface_blocks_of(vol, N):
     return pad.reshape(nb, 8, nb, 8, nb, 8).any(axis=(1, 3, 5)).astype(np.uint8).reshape(-1)
 
 
+def _with_superblocks(blocks, N):
+    """The two-level surface buffer: blocks, then 32^3 superblocks (OR of 4^3 blocks)."""
+    nb, ns = (N + 7) // 8, (N + 31) // 32
+    pad = np.zeros((ns * 4, ns * 4, ns * 4), np.uint8)
+    pad[:nb, :nb, :nb] = blocks.reshape(nb, nb, nb)
+    sup = pad.reshape(ns, 4, ns, 4, ns, 4).max(axis=(1, 3, 5)).reshape(-1)
+    return np.concatenate([blocks, sup]).astype(np.uint8)
+
+
 @pytest.mark.parametrize("res", [128, 96])  # 96^3: generic column kernel
 def test_surface_blocks_mark_stored_nonpositive_tsdf(gpu, res):
     """Three integrations of one frame at one pose: the set of voxels holding tsdf <= 0
@@ -309,7 +318,7 @@ def test_surface_blocks_mark_stored_nonpositive_tsdf(gpu, res):
     dvol = nuif.DeviceArray((N ** 3 * 4,), np.uint8)
     nuif.check(gpu.L().nuif_tsdf_reset(dvol.ptr, None, C.byref(p), 0, None))
     nb = gpu.L().nuif_tsdf_surface_blocks_bytes(C.byref(p))
-    assert nb == ((N + 7) // 8) ** 3
+    assert nb == ((N + 7) // 8) ** 3 + ((N + 31) // 32) ** 3
     surf = nuif.DeviceArray((nb,), np.uint8)
     nuif.check(gpu.L().nuif_fill_u8(surf.ptr, 7, nb, None))
     nuif.check(gpu.L().nuif_tsdf_surface_blocks_clear(surf.ptr, C.byref(p), None))
@@ -320,7 +329,7 @@ def test_surface_blocks_mark_stored_nonpositive_tsdf(gpu, res):
     for _ in range(3):
         nuif.check(gpu.L().nuif_tsdf_integrate(ddep.ptr, None, dvol.ptr, None, C.byref(p), C.byref(cam), pose.ptr,
                                                0, 128, None, None, None, surf.ptr, None))
-    expect = _surface_blocks_of(dvol.download().view(np.int16).reshape(-1, 2), N)
+    expect = _with_superblocks(_surface_blocks_of(dvol.download().view(np.int16).reshape(-1, 2), N), N)
     got = surf.download()
     assert expect.sum() > 100
     np.testing.assert_array_equal(got, expect)
@@ -391,8 +400,8 @@ def test_raycast_matches_oracle(gpu):
 
 @pytest.mark.parametrize("extra", [0.0, 0.3])
 def test_raycast_surface_blocks_identical(gpu, extra):
-    """The raycast with surface blocks (exact, or with 30 % of the other blocks marked
-    too) gives the oracle's maps bit for bit."""
+    """The raycast with surface blocks and superblocks (exact, or with 30 % of the other
+    blocks marked too) gives the oracle's maps bit for bit."""
     announce(f"raycast S16 128^3 with surface blocks (extra {extra})")
     wl = W.SMALL
     e = _oracle_state(wl, 4)
@@ -407,6 +416,8 @@ def test_raycast_surface_blocks_identical(gpu, extra):
     rng = np.random.default_rng(4)
     blocks = blocks | (rng.random(blocks.size) < extra).astype(np.uint8)
     assert 0 < blocks.mean() < 0.5 or extra > 0
+    blocks = _with_superblocks(blocks, N)
+    assert gpu.L().nuif_tsdf_surface_blocks_bytes(C.byref(p)) == blocks.size
     dvol = nuif.DeviceArray.from_host(vol)
     dsurf = nuif.DeviceArray.from_host(blocks)
     pose = nuif.upload_struct(nuif.rigid(R, t))
