@@ -142,33 +142,44 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < kTerms; ++k) acc[k] = cmp[k] = 0.f;
 
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
-    for (int p = blockIdx.x * kBlock + threadIdx.x; p < npx; p += gridDim.x * kBlock) {
-        const f3 vc = ld3(vcur, p);
-        if (nan3(vc)) continue;
-        // K/icp.cl:243-289
-        const f3 pv = add3(rot3(R, vc), mk3(t[0], t[1], t[2]));
-        const f3 pp = rot3(Ri, sub3(pv, mk3(tp[0], tp[1], tp[2])));
-        const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
-        if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) continue;
-        const f3 rn = bilinear_holes(nref, px, py, W, H);
-        if (nan3(rn)) continue;
-        const f3 rv = bilinear_holes(vref, px, py, W, H);
-        if (nan3(rv)) continue;
-        const f3 diff = sub3(rv, pv);
-        const float dist = sqrtf(dot3(diff, diff));
-        if (!(dist < distThr)) continue;
-        const f3 row0 = cross3(rn, pv);
-        const float c[7] = {row0.x, row0.y, row0.z, rn.x, rn.y, rn.z, dot3(rn, diff)};
-        int s = 0;
+    // kPB pixels of the grid-stride sequence at a time: their gathers are in flight
+    // together, their terms are accumulated in the sequence's order
+    constexpr int kPB = 4;
+    const int stride = gridDim.x * kBlock;
+    for (int p0 = blockIdx.x * kBlock + threadIdx.x; p0 < npx; p0 += kPB * stride) {
+        f3 pv[kPB], rn[kPB], rv[kPB];
+        bool ok[kPB];
 #pragma unroll
-        for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < kPB; ++j) {
+            const int p = p0 + j * stride;
+            const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
+            // K/icp.cl:243-289
+            pv[j] = add3(rot3(R, vc), mk3(t[0], t[1], t[2]));
+            const f3 pp = rot3(Ri, sub3(pv[j], mk3(tp[0], tp[1], tp[2])));
+            const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
+            ok[j] = !nan3(vc) && px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H;
+            rn[j] = ok[j] ? bilinear_holes(nref, px, py, W, H) : mk3(kNaN, kNaN, kNaN);
+            rv[j] = ok[j] ? bilinear_holes(vref, px, py, W, H) : mk3(kNaN, kNaN, kNaN);
+        }
 #pragma unroll
-            for (int j = i; j < 7; ++j) {
-                kahan_add(acc[s], cmp[s], c[i] * c[j]);
-                ++s;
-            }
-        kahan_add(acc[27], cmp[27], c[6] * c[6]);
-        acc[28] += 1.0f;  // counts are exact in FP32 up to 2^24
+        for (int j = 0; j < kPB; ++j) {
+            if (!ok[j] || nan3(rn[j]) || nan3(rv[j])) continue;
+            const f3 diff = sub3(rv[j], pv[j]);
+            const float dist = sqrtf(dot3(diff, diff));
+            if (!(dist < distThr)) continue;
+            const f3 row0 = cross3(rn[j], pv[j]);
+            const float c[7] = {row0.x, row0.y, row0.z, rn[j].x, rn[j].y, rn[j].z, dot3(rn[j], diff)};
+            int s = 0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+#pragma unroll
+                for (int q = i; q < 7; ++q) {
+                    kahan_add(acc[s], cmp[s], c[i] * c[q]);
+                    ++s;
+                }
+            kahan_add(acc[27], cmp[27], c[6] * c[6]);
+            acc[28] += 1.0f;  // counts are exact in FP32 up to 2^24
+        }
     }
 
     // transpose through LDS: term k, thread tid -> lds[k][tid / 32][tid % 32]
@@ -238,10 +249,10 @@ __device__ __forceinline__ void icp_terms(const f3 vc, const float* R, const flo
     const f3 pp = rot3(Ri, sub3(pv, mk3(tp[0], tp[1], tp[2])));
     const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
     if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) return;
+    // both lookups are issued before either is tested (pure reads)
     const f3 rn = bilinear_holes(nref, px, py, W, H);
-    if (nan3(rn)) return;
     const f3 rv = bilinear_holes(vref, px, py, W, H);
-    if (nan3(rv)) return;
+    if (nan3(rn) || nan3(rv)) return;
     const f3 diff = sub3(rv, pv);
     const float dist = sqrtf(dot3(diff, diff));
     if (!(dist < distThr)) return;
