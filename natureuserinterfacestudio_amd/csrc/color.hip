@@ -357,13 +357,13 @@ __global__ void __launch_bounds__(128)
     ColorIcpSync* sy = csync_of(st);
     const int grp = g >> 7;
     if (lid == 0) {
-        __threadfence();  // release this block's partial
+        fence_release_agent();  // release this block's partial (written by this thread)
         const uint32_t members = (uint32_t)min(128, ngroups - (grp << 7));
         last = atomicAdd(&sy->arrive[grp], 1u) == members - 1;
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();  // acquire the group's partials
+    fence_acquire_agent();  // acquire the group's partials
     float* tree = partials + (size_t)ngroups * kCTerms;
     for (int k = wave; k < kCTerms; k += 2) {
         const int g0 = (grp << 7) + lane, g1 = g0 + 64;
@@ -372,16 +372,16 @@ __global__ void __launch_bounds__(128)
         const float sum = wave_tree64(x + y);
         if (lane == 0) tree[grp * kCTerms + k] = sum;
     }
-    __threadfence();  // release the group result
-    __syncthreads();
+    __syncthreads();  // the group result is written
     const int ngroups2 = (ngroups + 127) >> 7;
     if (lid == 0) {
         sy->arrive[grp] = 0u;  // re-arm
+        fence_release_agent();  // release the group result
         last2 = atomicAdd(&sy->arrive_all, 1u) == (uint32_t)(ngroups2 - 1);
     }
     __syncthreads();
     if (!last2) return;
-    __threadfence();  // acquire every group result
+    fence_acquire_agent();  // acquire every group result
 
     // the host's sequential float sums over the compute_color_sums blocks
     if (lid < kCTerms) {

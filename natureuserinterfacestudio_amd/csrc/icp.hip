@@ -203,12 +203,14 @@ __global__ void __launch_bounds__(kBlock)
 
     // the last block to finish reduces the block partials and solves
     IcpSync* sy = sync_of(st);
-    __threadfence();  // release this block's partial (written by threads 0..28)
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(&sy->arrive_all, 1u) == gridDim.x - 1;
+    __syncthreads();  // the partial (threads 0..28) is written
+    if (tid == 0) {
+        fence_release_agent();  // release this block's partial
+        last = atomicAdd(&sy->arrive_all, 1u) == gridDim.x - 1;
+    }
     __syncthreads();
     if (!last) return;
-    __threadfence();  // acquire the other blocks' partials
+    fence_acquire_agent();  // acquire the other blocks' partials
     const int nblocks = (int)gridDim.x;
     if (tid < kTerms * kSeg) {
         // term k, segment j: blocks j, j + 8, ... in order; the loads of 8 blocks are
@@ -267,17 +269,22 @@ __device__ __forceinline__ void icp_terms(const f3 vc, const float* R, const flo
     out[28] = 1.0f;
 }
 
-// Reference order, one launch per iteration.  Block g (128 items) takes pixels
-// 128g..128g+127; a partial last block counts its missing items as zeros
-// (K/icp.cl:222-292).  The partials buffer holds ngroups x 29 group sums
-// followed by ceil(ngroups / 128) x 29 compute_sums results.
-__global__ void __launch_bounds__(128)
+// Reference order, one launch per iteration.  Group g (128 items) takes pixels
+// 128g..128g+127; a partial last group counts its missing items as zeros
+// (K/icp.cl:222-292).  A block holds kRefGroups groups, each reduced with the
+// reference's 128-wide tree on its own 128 threads.  The partials buffer holds
+// ngroups x 29 group sums followed by ceil(ngroups / 128) x 29 compute_sums
+// results; the last block of every 128 groups to finish runs compute_sums' tree
+// over them, the last of those sums the results in order and solves.
+constexpr int kRefGroups = 4;
+static_assert(128 % kRefGroups == 0, "a block's groups lie in one compute_sums chunk");
+__global__ void __launch_bounds__(128 * kRefGroups)
     icp_ref_kernel(IcpState* st, const float* __restrict__ vcur, int npx, NuiCLCameraParams cam,
                    const float* __restrict__ vref, const float* __restrict__ nref,
                    const NuiCLRigidTransform* __restrict__ prev, float distThr, int level, int iter,
                    uint32_t count_threshold, float* partials) {
     if (st->failed || st->brk[level]) return;
-    __shared__ float xfer[64 * kTerms];
+    __shared__ float xfer[kRefGroups][64 * kTerms];
     __shared__ float sumsf[kTerms];
     __shared__ double sums[kTerms];
     __shared__ IcpState S;
@@ -293,67 +300,71 @@ __global__ void __launch_bounds__(128)
         t[k] = st->t[k];
         tp[k] = prev->t[k];
     }
-    const int lid = threadIdx.x, lane = lid & 63, wave = lid >> 6;
-    const int g = blockIdx.x, ngroups = (int)gridDim.x;
-    const int p = g * 128 + lid;
+    const int tid = threadIdx.x, sub = tid >> 7, l = tid & 127, lane = l & 63, wave = l >> 6;
+    const int ngroups = (npx + 127) >> 7;
+    const int g = blockIdx.x * kRefGroups + sub;
+    const int p = g * 128 + l;
     float v[kTerms];
     const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
     icp_terms(vc, R, t, Ri, tp, cam, vref, nref, distThr, v);
 
-    // K/icp.cl:162-190 reduce(): step d = 64 through LDS, steps 32..1 in wave 0
+    // K/icp.cl:162-190 reduce(): step d = 64 through LDS, steps 32..1 in the group's wave 0
     if (wave == 1) {
 #pragma unroll
-        for (int k = 0; k < kTerms; ++k) xfer[lane * kTerms + k] = v[k];
+        for (int k = 0; k < kTerms; ++k) xfer[sub][lane * kTerms + k] = v[k];
     }
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
-        for (int k = 0; k < kTerms; ++k) v[k] = wave_tree64(v[k] + xfer[lane * kTerms + k]);
-        if (lane == 0) {
+        for (int k = 0; k < kTerms; ++k) v[k] = wave_tree64(v[k] + xfer[sub][lane * kTerms + k]);
+        if (lane == 0 && g < ngroups) {
 #pragma unroll
             for (int k = 0; k < kTerms; ++k) partials[(size_t)g * kTerms + k] = v[k];
         }
     }
+    __syncthreads();  // the block's group partials are written
 
-    // the last block of each 128-block group runs compute_sums' tree over the group
+    // the last block of each 128-group chunk runs compute_sums' tree over the chunk
     IcpSync* sy = sync_of(st);
-    const int grp = g >> 7;
-    if (lid == 0) {
-        __threadfence();  // release this block's partial
-        const uint32_t members = (uint32_t)min(128, ngroups - (grp << 7));
-        last = atomicAdd(&sy->arrive[grp], 1u) == members - 1;
+    const int chunk = (blockIdx.x * kRefGroups) >> 7;
+    if (tid == 0) {
+        fence_release_agent();  // release this block's partials
+        const int groups_in_chunk = min(128, ngroups - (chunk << 7));
+        const uint32_t members = (uint32_t)((groups_in_chunk + kRefGroups - 1) / kRefGroups);
+        last = atomicAdd(&sy->arrive[chunk], 1u) == members - 1;
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();  // acquire the group's partials
+    fence_acquire_agent();  // acquire the chunk's partials
     float* tree = partials + (size_t)ngroups * kTerms;
-    for (int k = wave; k < kTerms; k += 2) {
-        const int g0 = (grp << 7) + lane, g1 = g0 + 64;
+    const int wid = tid >> 6, nwaves = (128 * kRefGroups) >> 6;
+    for (int k = wid; k < kTerms; k += nwaves) {
+        const int g0 = (chunk << 7) + (tid & 63), g1 = g0 + 64;
         const float x = g0 < ngroups ? partials[(size_t)g0 * kTerms + k] : 0.f;
         const float y = g1 < ngroups ? partials[(size_t)g1 * kTerms + k] : 0.f;
         const float sum = wave_tree64(x + y);
-        if (lane == 0) tree[grp * kTerms + k] = sum;
+        if ((tid & 63) == 0) tree[chunk * kTerms + k] = sum;
     }
-    __threadfence();  // release the group result (written by lanes 0 of both waves)
-    __syncthreads();
-    const int ngroups2 = (ngroups + 127) >> 7;
-    if (lid == 0) {
-        sy->arrive[grp] = 0u;  // re-arm: every block of the group has arrived
-        last2 = atomicAdd(&sy->arrive_all, 1u) == (uint32_t)(ngroups2 - 1);
+    __syncthreads();  // the chunk result is written (lanes 0 of the waves)
+    const int nchunks = (ngroups + 127) >> 7;
+    if (tid == 0) {
+        sy->arrive[chunk] = 0u;  // re-arm: every block of the chunk has arrived
+        fence_release_agent();  // release the chunk result
+        last2 = atomicAdd(&sy->arrive_all, 1u) == (uint32_t)(nchunks - 1);
     }
     __syncthreads();
     if (!last2) return;
-    __threadfence();  // acquire every group result
+    fence_acquire_agent();  // acquire every chunk result
 
-    // the host's sequential float sum over the compute_sums groups
+    // the host's sequential float sum over the compute_sums results
     // (OCL/NuiKinfuOpenCLDepthTracker.cpp:394-402, 428-440)
-    if (lid < kTerms) {
+    if (tid < kTerms) {
         float sum = 0.f;
-        for (int c = 0; c < ngroups2; ++c) sum += tree[c * kTerms + lid];
-        sumsf[lid] = sum;
-        sums[lid] = (double)sum;
+        for (int c = 0; c < nchunks; ++c) sum += tree[c * kTerms + tid];
+        sumsf[tid] = sum;
+        sums[tid] = (double)sum;
     }
-    if (lid == 0) sy->arrive_all = 0u;
+    if (tid == 0) sy->arrive_all = 0u;
     finish_iteration<kRefOrder>(st, S, sums, sumsf, level, iter, count_threshold);
 }
 
@@ -688,8 +699,10 @@ int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* ca
         NUIF_CHECK_ARG(div_up(groups, 128) <= kMaxGroups2,
                        "nuif_icp_iteration: %d pixels per level exceed the reference-order limit %d", npx,
                        kMaxGroups2 * 128 * 128);
-        icp_ref_kernel<<<groups, 128, 0, st>>>((IcpState*)state, vmap_level, npx, *cam, ref_vmap, ref_nmap,
-                                               prev_pose, dist_threshold, level, iter, count_threshold, partials);
+        const int blocks = (int)div_up((size_t)groups, kRefGroups);
+        icp_ref_kernel<<<blocks, 128 * kRefGroups, 0, st>>>((IcpState*)state, vmap_level, npx, *cam, ref_vmap,
+                                                            ref_nmap, prev_pose, dist_threshold, level, iter,
+                                                            count_threshold, partials);
         NUIF_LAUNCHED("icp_ref_kernel");
     } else {
         const int nblocks = std::min<int>(div_up(npx, kBlock), kMaxBlocks);

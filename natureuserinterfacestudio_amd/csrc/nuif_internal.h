@@ -93,6 +93,13 @@ __device__ __forceinline__ f3 rot3(const float* R, f3 v) {
 
 constexpr float kNaN = __builtin_nanf("");
 
+// Device-scope release / acquire fences for the last-block pattern: every block
+// releases its results before counting itself in, only the block that goes on
+// acquires.  (A full fence in every block would also invalidate its L2 while the
+// other blocks are still reading through it.)
+__device__ __forceinline__ void fence_release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
 // K/normalEstimation.cl:770-822 estimate_normals_simple_kernel at pixel (x, y): the
 // normalised cross product of the right-left and down-up differences; a neighbour
 // outside the image, NaN or more than thr away in z is replaced by the centre; NaN
