@@ -417,6 +417,31 @@ int nuif_color_icp_estimate(void* state, float* partials, const NuiCLCameraParam
 /* Synchronises s and copies the state out. */
 int nuif_color_icp_read(const void* state, nuif_color_icp_result* out, nuif_stream_t s);
 
+/* ---- joint depth + intensity ICP terms (K/icp.cl:328-480; the reference has no host loop) ---- */
+
+/* K/transformMaps.cl:372-384 bgra_to_intensity_kernel: 0.299 R + 0.587 G + 0.114 B in [0, 1],
+ * NaN where alpha is 0. */
+int nuif_bgra_to_intensity(const uint8_t* bgra, float* intensities, size_t n, nuif_stream_t s);
+/* K/transformMaps.cl:448-511 intensity_derivatives_kernel: Sobel / 8 (float2 per pixel) with
+ * the neighbours one row apart, NaN on the border and next to NaN intensities (the
+ * reference steps by the image height and leaves those pixels unwritten). */
+int nuif_intensity_derivatives(const float* intensities, float* derivatives, int width, int height,
+                               nuif_stream_t s);
+size_t nuif_intensity_icp_workspace_bytes(int width, int height);
+/* K/icp.cl:352-480 intensity_icp_block_kernel + the 128-wide reduction trees + the
+ * sequential sum of the second-level results, at `level` (all maps W >> level by H >>
+ * level: current vertices / normals / intensities, previous vertices / normals /
+ * intensities / derivatives).  pose: current estimate applied to the current vertices;
+ * prev_pose: the previous camera (transformInverse); euler3 (host): the Euler angles of
+ * the evalR_d* derivatives; thresholds4 (host): distance, angle (sine), intensity
+ * difference, minimum gradient.  sums29: device float[29], the reference's slot order.
+ * workspace: nuif_intensity_icp_workspace_bytes(W >> level, H >> level) bytes. */
+int nuif_intensity_icp_sums(const NuiCLCameraParams* cam, int level, const float* vmap, const float* nmap,
+                            const float* intensities, const NuiCLRigidTransform* pose, const float* vmap_prev,
+                            const float* nmap_prev, const float* intensities_prev, const float* derivatives_prev,
+                            const NuiCLRigidTransform* prev_pose, const float* euler3, const float* thresholds4,
+                            void* workspace, float* sums29, nuif_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@
 // A normal matrix that is not positive definite fails the track in both (the
 // reference's Eigen LLT would return an unspecified solution).
 
+#include "det_math.h"
 #include "icp_math.h"
 #include "nuif_internal.h"
 
@@ -442,6 +443,193 @@ __global__ void color_icp_end_kernel(const ColorIcpState* st, NuiCLRigidTransfor
     inverse3f(R, pose->R_inv);
 }
 
+// ---- joint depth + intensity ICP (K/icp.cl:328-480; no host in the reference) ------
+
+// K/transformMaps.cl:372-384 bgra_to_intensity_kernel
+__global__ void bgra_to_intensity_kernel(const uchar4* __restrict__ bgra, float* __restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uchar4 c = bgra[i];
+    out[i] = c.w > 0 ? 0.299f * ((float)c.z / 255.0f) + 0.587f * ((float)c.y / 255.0f) + 0.114f * ((float)c.x / 255.0f)
+                     : kNaN;
+}
+
+// K/transformMaps.cl:448-511 intensity_derivatives_kernel: neighbours one row (the
+// width) apart, result at the centre; NaN where the reference writes nothing
+__global__ void intensity_derivatives_kernel(const float* __restrict__ I, float2* __restrict__ out, int W, int H) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t c = (size_t)y * W + x;
+    float2 r = make_float2(kNaN, kNaN);
+    if (!isnan(I[c]) && x > 0 && x < W - 1 && y > 0 && y < H - 1) {
+        const float i00 = I[c - W - 1], i10 = I[c - W], i20 = I[c - W + 1];
+        const float i01 = I[c - 1], i21 = I[c + 1];
+        const float i02 = I[c + W - 1], i12 = I[c + W], i22 = I[c + W + 1];
+        if (!(isnan(i00) || isnan(i10) || isnan(i20) || isnan(i01) || isnan(i21) || isnan(i02) || isnan(i12) ||
+              isnan(i22))) {
+            const float dx = (-1.0f) * i00 + (1.0f) * i20 + (-2.0f) * i01 + (2.0f) * i21 + (-1.0f) * i02 + (1.0f) * i22;
+            const float dy = (-1.0f) * i00 + (-2.0f) * i10 + (-1.0f) * i20 + (1.0f) * i02 + (2.0f) * i12 + (1.0f) * i22;
+            r = make_float2(dx / 8.0f, dy / 8.0f);
+        }
+    }
+    out[c] = r;
+}
+
+// K/icp.cl:328-350 evalR_dAlpha / evalR_dBeta / evalR_dGamma as written
+__device__ __forceinline__ void eval_phis(const float s[3], const float c[3], f3 p, f3* pa, f3* pb, f3* pg) {
+    const float sx = s[0], cx = c[0], sy = s[1], cy = c[1], sz = s[2], cz = c[2];
+    *pa = mk3(dot3(mk3(-sx * cy, -cx * cz - sx * sy * sz, cx * sz - sx * sy * cz), p),
+              dot3(mk3(cx * cy, -sx * cz + cx * sy * sz, sx * sz + cx * sy * cz), p), 0.0f);
+    *pb = mk3(dot3(mk3(-cx * sy, cx * cy * sz, cx * cy * cz), p), dot3(mk3(-sx * sy, sx * cy * sz, sx * cy * sz), p),
+              dot3(mk3(-cy, -sy * sz, -sy * cz), p));
+    *pg = mk3(dot3(mk3(0.0f, sx * sz + sx * cy * sz, sx * cz - cx * sy * sz), p),
+              dot3(mk3(0.0f, -cx * sz + sx * sy * cz, -cx * cz - sx * sy * sz), p), dot3(mk3(0.0f, cy * cz, -cy * sz), p));
+}
+
+struct IntensityIcpArgs {
+    const float *v, *n, *I, *vp, *np, *Ip;
+    const float2* dp;
+    float es[3], ec[3];  // sin, cos of the Euler angles
+    float distThr, angleThr, colorThr, gradMin;
+};
+
+// K/icp.cl:352-480 intensity_icp_block_kernel, one pixel (zeros when rejected)
+__device__ __forceinline__ void intensity_terms(const IntensityIcpArgs& a, const LevelCam& L, const float* R,
+                                                const float* t, const NuiCLRigidTransform& prev, int gid,
+                                                float out[kCTerms]) {
+#pragma unroll
+    for (int k = 0; k < kCTerms; ++k) out[k] = 0.f;
+    const f3 nc = ld3(a.n, gid);
+    if (nan3(nc)) return;
+    const f3 vc = ld3(a.v, gid);
+    if (nan3(vc)) return;
+    const f3 pv = add3(rot3(R, vc), mk3(t[0], t[1], t[2]));
+    const f3 pp = rot3(prev.R_inv, sub3(pv, mk3(prev.t[0], prev.t[1], prev.t[2])));
+    int ref;
+    if (!project_round(L, pp, &ref)) return;
+    const f3 rn = ld3(a.np, ref), rv = ld3(a.vp, ref);
+    const float ri = a.Ip[ref];
+    const float2 rd = a.dp[ref];
+    if (nan3(rn) || isnan(ri) || nan3(rv)) return;
+    const f3 diff = mk3(fabsf(rv.x - pv.x), fabsf(rv.y - pv.y), fabsf(rv.z - pv.z));
+    const float dist = sqrtf(diff.x * diff.x + diff.y * diff.y + diff.z * diff.z);
+    if (!(dist < a.distThr)) return;
+    f3 pa, pb, pg;
+    eval_phis(a.es, a.ec, pp, &pa, &pb, &pg);
+    const f3 cr = cross3(rot3(R, nc), rn);
+    const float sine = sqrtf(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+    if (!(sine < a.angleThr)) return;
+    const float wd = fmaxf(0.0f, 1.0f - dist / a.distThr);
+    const f3 mn = mk3(-rn.x, -rn.y, -rn.z);
+    float c[7] = {wd * dot3(mn, pa), wd * dot3(mn, pb), wd * dot3(mn, pg), wd * rn.x, wd * rn.y, wd * rn.z,
+                  wd * dot3(rn, diff)};
+    const float ic = a.I[gid];
+    if (!isnan(ic)) {
+        const float di = fabsf(ic - ri);
+        if (di < a.colorThr && !isnan(rd.x) && !isnan(rd.y) && sqrtf(rd.x * rd.x + rd.y * rd.y) > a.gradMin) {
+            const float wc = fmaxf(0.0f, 1.0f - di / a.colorThr);
+            const f3 t0 = mk3(rd.x * L.fx / pp.z, rd.y * L.fy / pp.z,
+                              (rd.x * L.cx + rd.y * L.cy) / pp.z - (rd.x * pp.x + rd.y * pp.y) / (pp.z * pp.z));
+            c[0] += wc * dot3(t0, pa);
+            c[1] += wc * dot3(t0, pb);
+            c[2] += wc * dot3(t0, pg);
+            c[3] += wc * t0.x;
+            c[4] += wc * t0.y;
+            c[5] += wc * t0.z;
+            c[6] += wc * di;
+        }
+    }
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 7; ++j) out[k++] = c[i] * c[j];
+    if (dist > a.distThr / 100.0f && sine > a.angleThr / 100.0f) {
+        out[27] = c[6] * c[6];
+        out[28] = 1.0f;
+    }
+}
+
+// The joint kernel's sums in one launch: kIntGroups 128-item groups per block, each
+// reduced with the reference's 128-wide tree; the last block of every 128 groups runs
+// compute_sums' tree, the last of those sums the results in order into out29.
+constexpr int kIntGroups = 4;
+__global__ void __launch_bounds__(128 * kIntGroups)
+    intensity_icp_kernel(IntensityIcpArgs a, NuiCLCameraParams cam, int level, const NuiCLRigidTransform* pose,
+                         const NuiCLRigidTransform* prev_pose, float* partials, ColorIcpSync* sy, float* out29) {
+    __shared__ float xfer[kIntGroups][64 * kCTerms];
+    __shared__ int last, last2;
+    const LevelCam L = level_cam(cam, 1 << level);
+    const int npx = L.W * L.H, ngroups = (npx + 127) >> 7;
+    float R[9], t[3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = pose->R[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[k] = pose->t[k];
+    const NuiCLRigidTransform prev = *prev_pose;
+    const int tid = threadIdx.x, sub = tid >> 7, l = tid & 127, lane = l & 63, wave = l >> 6;
+    const int g = blockIdx.x * kIntGroups + sub, p = g * 128 + l;
+    float v[kCTerms];
+    if (p < npx)
+        intensity_terms(a, L, R, t, prev, p, v);
+    else
+#pragma unroll
+        for (int k = 0; k < kCTerms; ++k) v[k] = 0.f;
+    if (wave == 1) {
+#pragma unroll
+        for (int k = 0; k < kCTerms; ++k) xfer[sub][lane * kCTerms + k] = v[k];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int k = 0; k < kCTerms; ++k) v[k] = wave_tree64(v[k] + xfer[sub][lane * kCTerms + k]);
+        if (lane == 0 && g < ngroups) {
+#pragma unroll
+            for (int k = 0; k < kCTerms; ++k) partials[(size_t)g * kCTerms + k] = v[k];
+        }
+    }
+    __syncthreads();
+    const int chunk = (blockIdx.x * kIntGroups) >> 7;
+    if (tid == 0) {
+        fence_release_agent();
+        const int in_chunk = min(128, ngroups - (chunk << 7));
+        last = atomicAdd(&sy->arrive[chunk], 1u) == (uint32_t)((in_chunk + kIntGroups - 1) / kIntGroups) - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    fence_acquire_agent();
+    float* tree = partials + (size_t)ngroups * kCTerms;
+    const int wid = tid >> 6, nwaves = (128 * kIntGroups) >> 6;
+    for (int k = wid; k < kCTerms; k += nwaves) {
+        const int g0 = (chunk << 7) + (tid & 63), g1 = g0 + 64;
+        const float x = g0 < ngroups ? partials[(size_t)g0 * kCTerms + k] : 0.f;
+        const float y = g1 < ngroups ? partials[(size_t)g1 * kCTerms + k] : 0.f;
+        const float sum = wave_tree64(x + y);
+        if ((tid & 63) == 0) tree[chunk * kCTerms + k] = sum;
+    }
+    __syncthreads();
+    const int nchunks = (ngroups + 127) >> 7;
+    if (tid == 0) {
+        sy->arrive[chunk] = 0u;
+        fence_release_agent();
+        last2 = atomicAdd(&sy->arrive_all, 1u) == (uint32_t)(nchunks - 1);
+    }
+    __syncthreads();
+    if (!last2) return;
+    fence_acquire_agent();
+    if (tid < kCTerms) {
+        float sum = 0.f;
+        for (int c = 0; c < nchunks; ++c) sum += tree[c * kCTerms + tid];
+        out29[tid] = sum;
+    }
+    if (tid == 0) sy->arrive_all = 0u;
+}
+
+__global__ void intensity_sync_clear_kernel(ColorIcpSync* sy) {
+    for (int i = threadIdx.x; i < kCMaxGroups2; i += blockDim.x) sy->arrive[i] = 0u;
+    if (threadIdx.x == 0) sy->arrive_all = 0u;
+}
+
 }  // namespace nuif
 
 using namespace nuif;
@@ -576,6 +764,64 @@ int nuif_color_icp_estimate(void* state, float* partials, const NuiCLCameraParam
                                           level_gy[level], s);
     if (!rc) rc = nuif_color_icp_end(state, pose, s);
     return rc;
+}
+
+int nuif_bgra_to_intensity(const uint8_t* bgra, float* intensities, size_t n, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(bgra && intensities && n > 0, "nuif_bgra_to_intensity: bad arguments");
+    bgra_to_intensity_kernel<<<div_up(n, 256), 256, 0, stream_of(s)>>>(reinterpret_cast<const uchar4*>(bgra),
+                                                                      intensities, n);
+    NUIF_LAUNCHED("bgra_to_intensity_kernel");
+    return NUIF_OK;
+}
+
+int nuif_intensity_derivatives(const float* intensities, float* derivatives, int width, int height,
+                               nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(intensities && derivatives && width > 0 && height > 0 && (const void*)intensities != derivatives,
+                   "nuif_intensity_derivatives: bad arguments");
+    const dim3 block(16, 16), grid(div_up(width, 16), div_up(height, 16));
+    intensity_derivatives_kernel<<<grid, block, 0, stream_of(s)>>>(intensities, reinterpret_cast<float2*>(derivatives),
+                                                                   width, height);
+    NUIF_LAUNCHED("intensity_derivatives_kernel");
+    return NUIF_OK;
+}
+
+size_t nuif_intensity_icp_workspace_bytes(int width, int height) {
+    if (width <= 0 || height <= 0) return 0;
+    const size_t groups = div_up((size_t)width * height, 128);
+    return kCSyncBytes + (groups + div_up(groups, 128)) * kCTerms * sizeof(float);
+}
+
+int nuif_intensity_icp_sums(const NuiCLCameraParams* cam, int level, const float* vmap, const float* nmap,
+                            const float* intensities, const NuiCLRigidTransform* pose, const float* vmap_prev,
+                            const float* nmap_prev, const float* intensities_prev, const float* derivatives_prev,
+                            const NuiCLRigidTransform* prev_pose, const float* euler3, const float* thresholds4,
+                            void* workspace, float* sums29, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(cam && vmap && nmap && intensities && pose && vmap_prev && nmap_prev && intensities_prev &&
+                       derivatives_prev && prev_pose && euler3 && thresholds4 && workspace && sums29,
+                   "nuif_intensity_icp_sums: null argument");
+    NUIF_CHECK_ARG(level >= 0 && level < 16, "nuif_intensity_icp_sums: bad level");
+    const int Wl = (int)(cam->depthImageWidth >> level), Hl = (int)(cam->depthImageHeight >> level);
+    NUIF_CHECK_ARG(Wl > 0 && Hl > 0, "nuif_intensity_icp_sums: empty level");
+    const size_t groups = div_up((size_t)Wl * Hl, 128);
+    NUIF_CHECK_ARG(div_up(groups, 128) <= (size_t)kCMaxGroups2, "nuif_intensity_icp_sums: image too large");
+    IntensityIcpArgs a;
+    a.v = vmap, a.n = nmap, a.I = intensities, a.vp = vmap_prev, a.np = nmap_prev, a.Ip = intensities_prev;
+    a.dp = reinterpret_cast<const float2*>(derivatives_prev);
+    for (int k = 0; k < 3; ++k) nuif_det::sincosf_det(euler3[k], &a.es[k], &a.ec[k]);
+    a.distThr = thresholds4[0], a.angleThr = thresholds4[1], a.colorThr = thresholds4[2], a.gradMin = thresholds4[3];
+    ColorIcpSync* sy = reinterpret_cast<ColorIcpSync*>(workspace);
+    float* partials = reinterpret_cast<float*>(static_cast<char*>(workspace) + kCSyncBytes);
+    hipStream_t st = stream_of(s);
+    intensity_sync_clear_kernel<<<1, 256, 0, st>>>(sy);
+    NUIF_LAUNCHED("intensity_sync_clear_kernel");
+    intensity_icp_kernel<<<(unsigned)div_up(groups, kIntGroups), 128 * kIntGroups, 0, st>>>(a, *cam, level, pose,
+                                                                                            prev_pose, partials, sy,
+                                                                                            sums29);
+    NUIF_LAUNCHED("intensity_icp_kernel");
+    return NUIF_OK;
 }
 
 int nuif_color_icp_read(const void* state, nuif_color_icp_result* out, nuif_stream_t s) {

@@ -189,6 +189,12 @@ _SIGS = {
                                         c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                         c_void_p]),
     "nuif_color_icp_read": (c_int, [c_void_p, C.POINTER(ColorIcpResult), c_void_p]),
+    "nuif_bgra_to_intensity": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
+    "nuif_intensity_derivatives": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "nuif_intensity_icp_workspace_bytes": (c_size_t, [c_int, c_int]),
+    "nuif_intensity_icp_sums": (c_int, [C.POINTER(NuiCLCameraParams), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(c_float),
+                                        C.POINTER(c_float), c_void_p, c_void_p, c_void_p]),
     "nuif_hash_forward_render_bytes": (c_size_t, [c_int, c_int]),
     "nuif_hash_forward_render": (c_int, [c_void_p, C.POINTER(HashParams), C.POINTER(NuiCLCameraParams), c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,

@@ -356,6 +356,154 @@ ColorIcpResult color_icp(const Cam<float>& cam, int levels, const int* iters, co
     return res;
 }
 
+// ---- joint depth + intensity ICP (K/icp.cl:328-480, no host in the reference) ----
+
+// K/transformMaps.cl:372-384 bgra_to_intensity_kernel
+void bgra_to_intensity(const uint8_t* bgra, float* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        const uint8_t* c = bgra + 4 * i;
+        out[i] = c[3] > 0 ? 0.299f * ((float)c[2] / 255.0f) + 0.587f * ((float)c[1] / 255.0f) +
+                                0.114f * ((float)c[0] / 255.0f)
+                          : qnan<float>();
+    }
+}
+
+// K/transformMaps.cl:448-511 intensity_derivatives_kernel with the neighbours one row
+// (the width) apart and the result at the centre; NaN where the reference writes
+// nothing (border, NaN centre or neighbour).  out: float2 per pixel.
+void intensity_derivatives(const float* I, float* out, int W, int H) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t c = (size_t)y * W + x;
+            out[2 * c] = out[2 * c + 1] = qnan<float>();
+            if (std::isnan(I[c]) || !(x > 0 && x < W - 1 && y > 0 && y < H - 1)) continue;
+            auto at = [&](int dx, int dy) { return I[(size_t)(y + dy) * W + x + dx]; };
+            const float i00 = at(-1, -1), i10 = at(0, -1), i20 = at(1, -1);
+            const float i01 = at(-1, 0), i21 = at(1, 0);
+            const float i02 = at(-1, 1), i12 = at(0, 1), i22 = at(1, 1);
+            if (std::isnan(i00) || std::isnan(i10) || std::isnan(i20) || std::isnan(i01) || std::isnan(i21) ||
+                std::isnan(i02) || std::isnan(i12) || std::isnan(i22))
+                continue;
+            float dx = (-1.0f) * i00 + (1.0f) * i20 + (-2.0f) * i01 + (2.0f) * i21 + (-1.0f) * i02 + (1.0f) * i22;
+            float dy = (-1.0f) * i00 + (-2.0f) * i10 + (-1.0f) * i20 + (1.0f) * i02 + (2.0f) * i12 + (1.0f) * i22;
+            out[2 * c] = dx / 8.0f;
+            out[2 * c + 1] = dy / 8.0f;
+        }
+}
+
+// K/icp.cl:328-350 evalR_dAlpha / evalR_dBeta / evalR_dGamma as written
+void eval_phis(const float e[3], V3<float> p, V3<float>* pa, V3<float>* pb, V3<float>* pg) {
+    float sx, cx, sy, cy, sz, cz;
+    sincos_t(e[0], &sx, &cx);
+    sincos_t(e[1], &sy, &cy);
+    sincos_t(e[2], &sz, &cz);
+    *pa = mk(dot(mk(-sx * cy, -cx * cz - sx * sy * sz, cx * sz - sx * sy * cz), p),
+             dot(mk(cx * cy, -sx * cz + cx * sy * sz, sx * sz + cx * sy * cz), p), 0.0f);
+    *pb = mk(dot(mk(-cx * sy, cx * cy * sz, cx * cy * cz), p), dot(mk(-sx * sy, sx * cy * sz, sx * cy * sz), p),
+             dot(mk(-cy, -sy * sz, -sy * cz), p));
+    *pg = mk(dot(mk(0.0f, sx * sz + sx * cy * sz, sx * cz - cx * sy * sz), p),
+             dot(mk(0.0f, -cx * sz + sx * sy * cz, -cx * cz - sx * sy * sz), p), dot(mk(0.0f, cy * cz, -cy * sz), p));
+}
+
+struct IntensityArgs {
+    Level L;
+    const float *v, *n, *I;              // current level maps
+    const float *vp, *np, *Ip, *dp;      // previous level maps (dp: float2)
+    Rigid<float> cur, prev;
+    float euler[3];
+    float distThr, angleThr, colorThr, gradMin;
+};
+
+// K/icp.cl:352-480 intensity_icp_block_kernel, one pixel
+void intensity_terms(const IntensityArgs& a, size_t gid, float out[kTerms]) {
+    for (int k = 0; k < kTerms; ++k) out[k] = 0.f;
+    const V3<float> nc = load3(a.n, gid);
+    if (isnan3(nc)) return;
+    const V3<float> vc = load3(a.v, gid);
+    if (isnan3(vc)) return;
+    const V3<float> pv = transform(a.cur, vc);
+    const V3<float> pp = transform_inverse(a.prev, pv);
+    int ref;
+    if (!project(a.L, pp, &ref)) return;
+    const V3<float> rn = load3(a.np, (size_t)ref);
+    const float ri = a.Ip[ref];
+    if (isnan3(rn) || std::isnan(ri)) return;
+    const V3<float> rv = load3(a.vp, (size_t)ref);
+    if (isnan3(rv)) return;
+    const V3<float> diff = mk(std::fabs(rv.x - pv.x), std::fabs(rv.y - pv.y), std::fabs(rv.z - pv.z));
+    const float dist = std::sqrt(diff.x * diff.x + diff.y * diff.y + diff.z * diff.z);
+    if (!(dist < a.distThr)) return;
+    V3<float> pa, pb, pg;
+    eval_phis(a.euler, pp, &pa, &pb, &pg);
+    const V3<float> pn = rotate(a.cur.R, nc);
+    const V3<float> cr = cross(pn, rn);
+    const float sine = std::sqrt(cr.x * cr.x + cr.y * cr.y + cr.z * cr.z);
+    if (!(sine < a.angleThr)) return;
+    const float wd = std::max(0.0f, 1.0f - dist / a.distThr);
+    const V3<float> mn = mk(-rn.x, -rn.y, -rn.z);
+    float c[7] = {wd * dot(mn, pa), wd * dot(mn, pb), wd * dot(mn, pg), wd * rn.x, wd * rn.y, wd * rn.z,
+                  wd * dot(rn, diff)};
+    const float ic = a.I[gid];
+    if (!std::isnan(ic)) {
+        const float di = std::fabs(ic - ri);
+        const float gx = a.dp[2 * (size_t)ref], gy = a.dp[2 * (size_t)ref + 1];
+        if (di < a.colorThr && !std::isnan(gx) && !std::isnan(gy) && std::sqrt(gx * gx + gy * gy) > a.gradMin) {
+            const float wc = std::max(0.0f, 1.0f - di / a.colorThr);
+            const V3<float> t0 = mk(gx * a.L.fx / pp.z, gy * a.L.fy / pp.z,
+                                    (gx * a.L.cx + gy * a.L.cy) / pp.z - (gx * pp.x + gy * pp.y) / (pp.z * pp.z));
+            c[0] += wc * dot(t0, pa);
+            c[1] += wc * dot(t0, pb);
+            c[2] += wc * dot(t0, pg);
+            c[3] += wc * t0.x;
+            c[4] += wc * t0.y;
+            c[5] += wc * t0.z;
+            c[6] += wc * di;
+        }
+    }
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j) out[k++] = c[i] * c[j];
+    if (dist > a.distThr / 100.0f && sine > a.angleThr / 100.0f) {
+        out[k++] = c[6] * c[6];
+        out[k++] = 1.0f;
+    }
+}
+
+// The 128-wide block trees, compute_sums' trees over 128 groups and the host's
+// sequential sum (the depth tracker's reduction, K/icp.cl:162-190, 294-326)
+void intensity_sums(const IntensityArgs& a, int npx, float out[kTerms]) {
+    const int groups = (npx + 127) / 128;
+    std::vector<float> gsum((size_t)groups * kTerms);
+#pragma omp parallel
+    {
+        std::vector<float> buf(128 * kTerms);
+#pragma omp for schedule(static)
+        for (int gi = 0; gi < groups; ++gi) {
+            for (int l = 0; l < 128; ++l) {
+                const int p = gi * 128 + l;
+                float* o = &buf[l * kTerms];
+                if (p < npx)
+                    intensity_terms(a, (size_t)p, o);
+                else
+                    for (int k = 0; k < kTerms; ++k) o[k] = 0.f;
+            }
+            tree128(buf.data());
+            std::memcpy(&gsum[(size_t)gi * kTerms], buf.data(), kTerms * sizeof(float));
+        }
+    }
+    const int nblocks = (groups + 127) / 128;
+    std::vector<float> buf(128 * kTerms);
+    for (int k = 0; k < kTerms; ++k) out[k] = 0.f;
+    for (int b = 0; b < nblocks; ++b) {
+        for (int l = 0; l < 128; ++l) {
+            const int gi = b * 128 + l;
+            for (int k = 0; k < kTerms; ++k) buf[l * kTerms + k] = gi < groups ? gsum[(size_t)gi * kTerms + k] : 0.f;
+        }
+        tree128(buf.data());
+        for (int k = 0; k < kTerms; ++k) out[k] += buf[k];
+    }
+}
+
 }  // namespace coloror
 
 using namespace coloror;
@@ -446,4 +594,37 @@ OR_EXPORT int or_color_icp(int W, int H, const float* cam6, int levels, const in
         info4[3] = r.count;
     }
     return r.ok;
+}
+
+OR_EXPORT void or_bgra_to_intensity(const uint8_t* bgra, float* out, long long n) {
+    bgra_to_intensity(bgra, out, (size_t)n);
+}
+
+OR_EXPORT void or_intensity_derivatives(const float* I, float* out, int W, int H) {
+    intensity_derivatives(I, out, W, H);
+}
+
+// evalR_dAlpha / dBeta / dGamma at (euler, p): out9 = the three vectors
+OR_EXPORT void or_eval_phis(const float* euler3, const float* p3, float* out9) {
+    V3<float> a, b, g;
+    eval_phis(euler3, mk(p3[0], p3[1], p3[2]), &a, &b, &g);
+    const float v[9] = {a.x, a.y, a.z, b.x, b.y, b.z, g.x, g.y, g.z};
+    std::memcpy(out9, v, sizeof(v));
+}
+
+// The joint kernel's 29 sums at `level` (maps of that level, W >> level by H >> level);
+// pose12 / prev12: R (row-major) + t of the current and previous camera poses;
+// thr4: dist, angle, colour thresholds and the gradient minimum.
+OR_EXPORT void or_intensity_icp_sums(int W, int H, const float* cam6, int level, const float* v, const float* n,
+                                     const float* I, const float* vp, const float* np, const float* Ip,
+                                     const float* dp, const float* pose12, const float* prev12, const float* euler3,
+                                     const float* thr4, float* out29) {
+    IntensityArgs a;
+    a.L = level_of(cam_of(cam6, W, H), 1 << level);
+    a.v = v, a.n = n, a.I = I, a.vp = vp, a.np = np, a.Ip = Ip, a.dp = dp;
+    a.cur = make_rigid<float>(pose12, pose12 + 9);
+    a.prev = make_rigid<float>(prev12, prev12 + 9);
+    std::memcpy(a.euler, euler3, sizeof(a.euler));
+    a.distThr = thr4[0], a.angleThr = thr4[1], a.colorThr = thr4[2], a.gradMin = thr4[3];
+    intensity_sums(a, a.L.W * a.L.H, out29);
 }

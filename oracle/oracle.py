@@ -85,6 +85,11 @@ def lib():
             "or_color_terms": (None, [ci, ci, f32p, ci, f32p, u8p, u8p, u8p, u8p, f32p, f32p, ci, ci, f32p]),
             "or_color_reduce": (ci, [ci, ci, f32p, ci, f32p, u8p, u8p, u8p, u8p, f32p, f32p, ci, ci, vp, ci]),
             "or_color_icp": (ci, [ci, ci, f32p, ci, i32p, i32p, cf, f32p, u8p, u8p, f32p, f32p, ci, f64p]),
+            "or_bgra_to_intensity": (None, [u8p, f32p, cll]),
+            "or_intensity_derivatives": (None, [f32p, f32p, ci, ci]),
+            "or_eval_phis": (None, [f32p, f32p, f32p]),
+            "or_intensity_icp_sums": (None, [ci, ci, f32p, ci, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p,
+                                             f32p, f32p, f32p]),
             "or_expf_det": (cf, [cf]),
             "or_sincosf_det": (None, [cf, C.POINTER(cf), C.POINTER(cf)]),
         }
@@ -360,3 +365,34 @@ def color_icp(wl, vmap, colors, frame_bgra, R, t, iterations=None, rot_only=None
                             np.array(ro, np.int32), dist_threshold, _f32(vmap), _u8(colors), _u8(frame_bgra), Rn, tn,
                             int(bool(corrected)), info)
     return ok, Rn.reshape(3, 3), tn, int(info[1]), float(info[2]), float(info[3])
+
+
+# ---- joint depth + intensity ICP terms ------------------------------------------
+
+def bgra_to_intensity(bgra):
+    b = _u8(bgra)
+    out = np.empty(b.size // 4, np.float32)
+    lib().or_bgra_to_intensity(b, out, out.size)
+    return out
+
+
+def intensity_derivatives(I, w, h):
+    out = np.empty(w * h * 2, np.float32)
+    lib().or_intensity_derivatives(_f32(I), out, w, h)
+    return out.reshape(h, w, 2)
+
+
+def eval_phis(euler, p):
+    out = np.empty(9, np.float32)
+    lib().or_eval_phis(_f32(euler), _f32(p), out)
+    return out.reshape(3, 3)
+
+
+def intensity_icp_sums(wl, level, v, n, I, vp, np_, Ip, dp, R, t, Rp, tp, euler, thr):
+    """The joint kernel's 29 sums (the reference's reduction order) at `level`."""
+    out = np.empty(29, np.float32)
+    pose = np.concatenate([np.asarray(R, np.float32).reshape(-1), np.asarray(t, np.float32).reshape(-1)])
+    prev = np.concatenate([np.asarray(Rp, np.float32).reshape(-1), np.asarray(tp, np.float32).reshape(-1)])
+    lib().or_intensity_icp_sums(wl.width, wl.height, wl.cam6(), level, _f32(v), _f32(n), _f32(I), _f32(vp),
+                                _f32(np_), _f32(Ip), _f32(dp), pose, prev, _f32(euler), _f32(thr), out)
+    return out

@@ -210,3 +210,83 @@ def test_corrected_tracker_halves_the_error_between_frames_and_reference_decode_
     ok_ref, Rr, tr, its, _, _ = O.color_icp(WL, vw, cols, frame, R0, t0, iterations=[10, 5, 4], corrected=False)
     assert ok_ref == 0 and its == 1
     np.testing.assert_array_equal(Rr, R0.astype(np.float32))
+
+
+# ---- joint depth + intensity ICP terms (K/icp.cl:328-480) ---------------------------
+
+def test_intensity_and_derivative_closed_forms():
+    rng = np.random.default_rng(2)
+    px = rng.integers(0, 256, (200, 4), dtype=np.uint8)
+    px[:20, 3] = 0
+    I = O.bgra_to_intensity(px)
+    assert np.isnan(I[:20]).all()
+    ref = 0.299 * px[20:, 2] / 255.0 + 0.587 * px[20:, 1] / 255.0 + 0.114 * px[20:, 0] / 255.0
+    assert np.abs(I[20:] - ref).max() < 3e-7
+    assert abs(O.bgra_to_intensity(np.array([[255, 255, 255, 255]], np.uint8))[0] - 1.0) < 1.2e-7
+    # a plane of intensity a x + b y (dyadic values: exact) has derivative (a, b) inside
+    w, h = 12, 9
+    ys, xs = np.mgrid[0:h, 0:w]
+    a, b = 0.0625, -0.125
+    plane = (0.25 + a * xs + b * ys).astype(np.float32)
+    d = O.intensity_derivatives(plane, w, h)
+    assert (d[1:-1, 1:-1, 0] == a).all() and (d[1:-1, 1:-1, 1] == b).all()
+    border = np.ones((h, w), bool)
+    border[1:-1, 1:-1] = False
+    assert np.isnan(d[border]).all()
+    holed = plane.copy()
+    holed[4, 5] = np.nan
+    dh = O.intensity_derivatives(holed, w, h)
+    near = np.zeros((h, w), bool)
+    near[3:6, 4:7] = True
+    assert np.isnan(dh[near]).all() and (dh[~near & ~border] == d[~near & ~border]).all()
+
+
+def test_euler_derivatives_at_zero_are_the_rotation_generators():
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        p = rng.normal(size=3).astype(np.float32)
+        phis = O.eval_phis(np.zeros(3, np.float32), p)
+        x, y, z = p
+        np.testing.assert_array_equal(phis[0], np.array([-y, x, 0], np.float32))   # about z
+        np.testing.assert_array_equal(phis[1], np.array([z, 0, -x], np.float32))   # about y
+        np.testing.assert_array_equal(phis[2], np.array([0, -z, y], np.float32))   # about x
+
+
+def _camera_maps(f):
+    d, bgra, gt = WL.render(f, color=True)
+    dep = np.empty(d.size, np.float32)
+    O.lib().or_passing_filter_f(d.reshape(-1), dep, d.size, 0.5, 4.5)
+    cam = WL.cam6()
+    ys, xs = np.divmod(np.arange(d.size), WL.width)
+    v = np.stack([dep * ((xs - cam[2]) / cam[0]), dep * ((ys - cam[3]) / cam[1]), dep], -1).astype(np.float32)
+    n = np.empty_like(v)
+    O.lib().or_normals_simple_f(v.reshape(-1), n.reshape(-1), WL.width, WL.height, 0.3)
+    return v, n, O.bgra_to_intensity(bgra), gt[:, :3].astype(np.float32), gt[:, 3].astype(np.float32)
+
+
+def test_intensity_icp_identical_frames_closed_form():
+    """Current = previous maps at the same pose, no intensity term: every valid point
+    matches itself (distance 0), so its terms are the products of (n x p, n, 0) (the
+    generators at zero Euler angles against the negated normal) and the count slot,
+    which needs a distance above 1 % of the threshold, stays 0."""
+    v, n, I, R, t = _camera_maps(3)
+    Rw, tw = np.eye(3, dtype=np.float32), np.zeros(3, np.float32)
+    d = O.intensity_derivatives(I, WL.width, WL.height)
+    thr = np.array([0.1, 0.34, 0.0, 0.0], np.float32)  # colour threshold 0: no intensity term
+    out = O.intensity_icp_sums(WL, 0, v, n, I, v, n, I, d, Rw, tw, Rw, tw, np.zeros(3, np.float32), thr)
+    assert out[27] == 0 and out[28] == 0
+    ok = ~np.isnan(v).any(1) & ~np.isnan(n).any(1)
+    p, nn = v[ok].astype(np.float64), n[ok].astype(np.float64)
+    # a normal's cross product with itself is zero: every valid point passes the angle test;
+    # c[:, 0:3] = -n . (generator) for the generators about z, y, x, then n, then n . 0
+    c = np.concatenate([np.zeros((len(p), 3)), nn, np.zeros((len(p), 1))], 1)
+    c[:, 0] = -(nn[:, 0] * -p[:, 1] + nn[:, 1] * p[:, 0])
+    c[:, 1] = -(nn[:, 0] * p[:, 2] - nn[:, 2] * p[:, 0])
+    c[:, 2] = -(-nn[:, 1] * p[:, 2] + nn[:, 2] * p[:, 1])
+    expect = []
+    for i in range(6):
+        for j in range(i, 7):
+            expect.append((c[:, i] * c[:, j]).sum())
+    expect = np.array(expect)
+    scale = np.abs(expect).max()
+    assert np.abs(out[:27] - expect).max() <= 1e-4 * scale

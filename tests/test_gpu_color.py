@@ -119,3 +119,59 @@ def test_color_icp_iteration_sums_match_reference_tree(gpu):
         host = (host + b).astype(np.float32)
     np.testing.assert_array_equal(np.array(r.sums[:], np.float32), host)
     assert host[28] > 100000
+
+
+def test_intensity_icp_sums_match_oracle(gpu):
+    """bgra_to_intensity, the derivatives and the joint kernel's 29 sums (levels 0 and 1,
+    frame 5 against the world maps of frame 4 at a perturbed pose) bit for bit."""
+    announce("joint depth + intensity ICP sums 320x240 vs oracle")
+    def maps(f):
+        d, bgra, gt = WL.render(f, color=True)
+        dep = np.empty(d.size, np.float32)
+        O.lib().or_passing_filter_f(d.reshape(-1), dep, d.size, 0.5, 4.5)
+        cam = WL.cam6()
+        ys, xs = np.divmod(np.arange(d.size), WL.width)
+        v = np.stack([dep * ((xs - cam[2]) / cam[0]), dep * ((ys - cam[3]) / cam[1]), dep], -1).astype(np.float32)
+        n = np.empty_like(v)
+        O.lib().or_normals_simple_f(v.reshape(-1), n.reshape(-1), WL.width, WL.height, 0.3)
+        return v, n, bgra, gt[:, :3].astype(np.float32), gt[:, 3].astype(np.float32)
+    v5, n5, bgra5, R5, t5 = maps(5)
+    v4, n4, bgra4, R4, t4 = maps(4)
+    vw4 = (v4 @ R4.T + t4).astype(np.float32)
+    nw4 = (n4 @ R4.T).astype(np.float32)
+    I5 = nuif.DeviceArray((WL.width * WL.height,), np.float32)
+    nuif.check(nuif.L().nuif_bgra_to_intensity(nuif.DeviceArray.from_host(bgra5).ptr, I5.ptr, WL.width * WL.height,
+                                               None))
+    np.testing.assert_array_equal(I5.download(), O.bgra_to_intensity(bgra5))
+    I4h = O.bgra_to_intensity(bgra4)
+    d4 = nuif.DeviceArray((WL.width * WL.height * 2,), np.float32)
+    nuif.check(nuif.L().nuif_intensity_derivatives(nuif.DeviceArray.from_host(I4h).ptr, d4.ptr, WL.width,
+                                                   WL.height, None))
+    d4h = O.intensity_derivatives(I4h, WL.width, WL.height)
+    np.testing.assert_array_equal(d4.download().reshape(d4h.shape), d4h)
+    I5h = O.bgra_to_intensity(bgra5)
+    Rp = (R5 @ np.array([[1, -0.01, 0], [0.01, 1, 0], [0, 0, 1]], np.float32)).astype(np.float32)
+    tp = (t5 + np.float32([0.01, -0.005, 0.0])).astype(np.float32)
+    euler = np.float32([0.01, -0.02, 0.005])
+    thr = np.float32([0.1, 0.34, 0.2, 0.002])
+    cam = WL.camera_params()
+    for level in (0, 1):
+        s = 1 << level
+        def lv(a, k):
+            return np.ascontiguousarray(a.reshape(WL.height, WL.width, k)[::s, ::s]).reshape(-1, k)
+        args_h = [lv(v5, 3), lv(n5, 3), lv(I5h, 1), lv(vw4, 3), lv(nw4, 3), lv(I4h, 1), lv(d4h, 2)]
+        exp = O.intensity_icp_sums(WL, level, *args_h, Rp, tp, R4, t4, euler, thr)
+        dev = [nuif.DeviceArray.from_host(a) for a in args_h]
+        ws = nuif.DeviceArray((nuif.L().nuif_intensity_icp_workspace_bytes(WL.width >> level, WL.height >> level),),
+                              np.uint8)
+        out = nuif.DeviceArray((29,), np.float32)
+        pose = nuif.upload_struct(nuif.rigid(Rp, tp))
+        prev = nuif.upload_struct(nuif.rigid(R4, t4))
+        e3 = (C.c_float * 3)(*euler)
+        t4c = (C.c_float * 4)(*thr)
+        for _ in range(2):  # a second call re-arms the counters
+            nuif.check(nuif.L().nuif_intensity_icp_sums(C.byref(cam), level, dev[0].ptr, dev[1].ptr, dev[2].ptr,
+                                                        pose.ptr, dev[3].ptr, dev[4].ptr, dev[5].ptr, dev[6].ptr,
+                                                        prev.ptr, e3, t4c, ws.ptr, out.ptr, None))
+            np.testing.assert_array_equal(out.download(), exp)
+        assert exp[28] > 1000
