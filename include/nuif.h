@@ -277,6 +277,21 @@ int nuif_mc_dense(const void* volume, const uint8_t* color_volume, const TsdfPar
                   float* vertices, float* colors, uint32_t capacity, uint32_t* total, void* scratch,
                   nuif_stream_t s);
 
+/* ---- dense point cloud (K/extract.cl:4-78, OCL/NuiKinfuOpenCLScene.cpp:436-610) ---- */
+
+size_t nuif_points_dense_scratch_bytes(const TsdfParams* params);
+/* fetchVolumeKernel (FetchVolume / Volume2CLVertices): per (x, y) column along z, a point at
+ * the voxel centre where the tsdf is exactly 0, else at the linear zero crossing with the
+ * voxel below where the sign changes (weights > 0 on both).  Points float3 (x and y centred,
+ * y flipped); colours float4 RGBA from the colour volume (bytes 0,1,2 for a zero, 2,1,0 for a
+ * crossing, as the reference reads them; (0,0,0,1) without one; colors may be NULL).  Written
+ * in (y, x, z) order; *total (device uint32) receives the count before capping at `capacity`.
+ * surface_blocks: optional flags maintained by nuif_tsdf_integrate for this volume -- steps
+ * whose voxels lie in clear blocks are skipped unread (same result). */
+int nuif_points_dense(const void* volume, const uint8_t* color_volume, const TsdfParams* params, int storage,
+                      const uint8_t* surface_blocks, float* points, float* colors, uint32_t capacity,
+                      uint32_t* total, void* scratch, nuif_stream_t s);
+
 /* OCL/NuiKinfuOpenCLScene.cpp:788-984 Volume2Mesh on the device: marching cubes in the
  * reference's (y, x, z) cube order with its weight test, vertices keyed by the ordered voxel
  * pair of their cube edge and numbered in first-use order, positions centred on all axes
@@ -352,6 +367,15 @@ size_t nuif_hash_mesh_scratch_bytes(const nuif_hash_params* p);
  * *total (device uint32) receives the vertex count before capping at `capacity`. */
 int nuif_hash_mesh(void* scene, const nuif_hash_params* p, float* vertices, float* colors, uint32_t capacity,
                    uint32_t* total, void* scratch, nuif_stream_t s);
+size_t nuif_hash_points_scratch_bytes(const nuif_hash_params* p);
+/* K/KinfuHashVisualization.cl:5-52 fetchHashScene_kernel (OCL/NuiKinfuOpenCLHashScene.cpp:1066-1190
+ * Volume2CLVertices): every voxel of a visible block with weight > 0 and |short sdf| <
+ * 32767 * trunc_scale (FP16 storage: |sdf| < trunc_scale) becomes a point at its centre
+ * ((8 * block + local) + 0.5) * voxel_size; colours float4 RGBA (voxel bytes 0..2 / 255, or
+ * (0,0,0,1) without colour; may be NULL).  Written in visible-list order, voxel index
+ * ascending inside a block; *total (device uint32) receives the count before the cap. */
+int nuif_hash_points(void* scene, const nuif_hash_params* p, float* points, float* colors, uint32_t capacity,
+                     uint32_t* total, void* scratch, nuif_stream_t s);
 /* Host copy of [bricks handed out, visible-list length, failed insertions, occupied
  * entries]; synchronises s. */
 int nuif_hash_stats(const void* scene, uint32_t* out4, nuif_stream_t s);

@@ -120,10 +120,13 @@ Scene scene_of(void* base, const nuif_hash_params& p) {
 struct VS16 {  // the reference's short sdf (x32767, round toward zero) + weight
     __device__ static float sdf(uint32_t w) { return (float)(int16_t)(w & 0xffffu) / 32767.0f; }
     __device__ static int weight(uint32_t w) { return (int)(w >> 16); }
+    // K/KinfuHashVisualization.cl:25 abs(sdf) < 32767 * truncScale on the short sdf
+    __device__ static bool in_band(uint32_t w, float ts) { return (float)abs((int)(int16_t)(w & 0xffffu)) < 32767 * ts; }
 };
 struct VF16 {  // FP16 sdf (round to nearest even) + uint16 weight
     __device__ static float sdf(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
     __device__ static int weight(uint32_t w) { return (int)(w >> 16); }
+    __device__ static bool in_band(uint32_t w, float ts) { return fabsf(sdf(w)) < ts; }
 };
 
 __device__ __forceinline__ int cvt_sat(float v) {
@@ -890,6 +893,55 @@ __global__ void __launch_bounds__(64)
     }
 }
 
+// K/KinfuHashVisualization.cl:5-52 fetchHashScene_kernel: every observed voxel of a
+// visible block inside the band becomes a point at its centre.  The reference
+// appends through an atomic counter; here one wavefront takes one visible block
+// (8 coalesced 256-byte loads), pass 1 counts its points, the scan turns the
+// counts into offsets and pass 2 writes them in visible-list order with the voxel
+// index (x + 8y + 64z) ascending inside a block.
+template <class V, bool EMIT>
+__global__ void __launch_bounds__(64)
+    hash_points_kernel(Scene s, float vs, float ts, uint32_t* __restrict__ counts, float* __restrict__ pts,
+                       float* __restrict__ cols, uint32_t capacity) {
+    const uint32_t n = s.hdr->nvisible;
+    const int lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int32_t id = s.visible[i];
+        const HashEntry e = id >= 0 ? s.entries[id] : HashEntry{0ull, -1, 0u};
+        uint32_t base = EMIT ? counts[i] : 0u;
+        if (e.ptr >= 0) {
+            int p[3];
+            key_pos(e.key, p);
+            for (int lz = 0; lz < kSB; ++lz) {
+                const uint32_t vi = (uint32_t)e.ptr * kSB3 + (uint32_t)(lane + 64 * lz);
+                const uint32_t w = s.bricks[vi];
+                const bool keep = V::weight(w) > 0 && V::in_band(w, ts);
+                const unsigned long long m = __ballot(keep);
+                if (EMIT && keep) {
+                    const uint32_t o = base + (uint32_t)__popcll(m & below);
+                    if (o < capacity) {
+                        const float pz = ((float)(p[2] * kSB) + ((float)lz + 0.5f)) * vs;
+                        const float py = ((float)(p[1] * kSB) + ((float)(lane >> 3) + 0.5f)) * vs;
+                        const float px = ((float)(p[0] * kSB) + ((float)(lane & 7) + 0.5f)) * vs;
+                        st3(pts, o, mk3(px, py, pz));
+                        if (cols) {
+                            float4 c = make_float4(0.f, 0.f, 0.f, 1.f);
+                            if (s.colors) {
+                                const uchar4 q = s.colors[vi];
+                                c = make_float4((float)q.x / 255.f, (float)q.y / 255.f, (float)q.z / 255.f, 1.f);
+                            }
+                            reinterpret_cast<float4*>(cols)[o] = c;
+                        }
+                    }
+                }
+                base += (uint32_t)__popcll(m);
+            }
+        }
+        if (!EMIT && lane == 0) counts[i] = base;
+    }
+}
+
 __global__ void hash_reset_kernel(Scene s, size_t nentries, size_t nwords) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1171,6 +1223,46 @@ int nuif_hash_mesh(void* scene, const nuif_hash_params* p, float* vertices, floa
         else
             hash_mc_emit_kernel<VF16><<<kWalkGrid, 64, 0, st>>>(sc, offsets, p->voxel_size, vertices, colors, capacity);
         NUIF_LAUNCHED("hash_mc_emit_kernel");
+    }
+    return NUIF_OK;
+}
+
+size_t nuif_hash_points_scratch_bytes(const nuif_hash_params* p) { return nuif_hash_mesh_scratch_bytes(p); }
+
+int nuif_hash_points(void* scene, const nuif_hash_params* p, float* points, float* colors, uint32_t capacity,
+                     uint32_t* total, void* scratch, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(p);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(scene && total && scratch, "nuif_hash_points: null argument");
+    NUIF_CHECK_ARG(capacity == 0 || points, "nuif_hash_points: point buffer missing");
+    const size_t n = layout_of(*p).nentries;
+    Scene sc = scene_of(scene, *p);
+    hipStream_t st = stream_of(s);
+    char* b = static_cast<char*>(scratch);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(b);
+    uint32_t* offsets = reinterpret_cast<uint32_t*>(b + align256(n * 4));
+    void* scan = b + 2 * align256(n * 4);
+    // counts past the visible list's length stay zero, so the scan's total is the point count
+    rc = nuif_fill_u32(counts, 0u, n, s);
+    if (rc) return rc;
+    if (p->storage == NUIF_TSDF_S16)
+        hash_points_kernel<VS16, false><<<kWalkGrid, 64, 0, st>>>(sc, p->voxel_size, p->trunc_scale, counts, nullptr,
+                                                                   nullptr, 0u);
+    else
+        hash_points_kernel<VF16, false><<<kWalkGrid, 64, 0, st>>>(sc, p->voxel_size, p->trunc_scale, counts, nullptr,
+                                                                   nullptr, 0u);
+    NUIF_LAUNCHED("hash_points_kernel");
+    rc = launch_scan(counts, false, offsets, n, total, scan, st);
+    if (rc) return rc;
+    if (capacity > 0) {
+        if (p->storage == NUIF_TSDF_S16)
+            hash_points_kernel<VS16, true><<<kWalkGrid, 64, 0, st>>>(sc, p->voxel_size, p->trunc_scale, offsets,
+                                                                      points, colors, capacity);
+        else
+            hash_points_kernel<VF16, true><<<kWalkGrid, 64, 0, st>>>(sc, p->voxel_size, p->trunc_scale, offsets,
+                                                                      points, colors, capacity);
+        NUIF_LAUNCHED("hash_points_kernel");
     }
     return NUIF_OK;
 }

@@ -216,6 +216,9 @@ public:
     void reset() override;
     bool integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) override;
     void raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) override;
+    // OCL/NuiKinfuOpenCLScene.cpp:572-615 via nuif_points_dense: the point cloud
+    // (positions and colours, no indices); false when the scene is unchanged or empty
+    bool Volume2CLVertices(NuiKinfuMesh* pData) override;
     // K/marchingCube.cl via nuif_mc_dense: triangle soup (3 vertices per triangle)
     bool Volume2CLMesh(NuiKinfuMesh* pData) override;
     // OCL/NuiKinfuOpenCLScene.cpp:788-984 via nuif_mesh_dense: shared vertices + indices
@@ -253,6 +256,9 @@ public:
     void reset() override;
     bool integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) override;
     void raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) override;
+    // OCL/NuiKinfuOpenCLHashScene.cpp:1066-1190 via nuif_hash_points over the visible
+    // list; false when the scene is unchanged or nothing is visible
+    bool Volume2CLVertices(NuiKinfuMesh* pData) override;
     // K/KinfuHashVisualization.cl via nuif_hash_mesh over the visible list
     bool Volume2CLMesh(NuiKinfuMesh* pData) override;
     bool Volume2Mesh(NuiKinfuMesh* pMesh) override { return Volume2CLMesh(pMesh); }

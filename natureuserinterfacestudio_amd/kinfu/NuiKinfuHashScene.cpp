@@ -99,7 +99,8 @@ unsigned long long NuiKinfuHIPHashScene::UpdatedVoxels() const {
 
 namespace {
 // count pass, then the emit pass into device buffers of the exact size, then the
-// host copy (the reference maps the result into GL buffers instead)
+// host copy (the reference maps the result into GL buffers instead); used for the
+// triangle soups and the point clouds alike
 template <class CountEmit>
 bool mesh_to_host(NuiKinfuMesh* out, nuif_stream_t s, CountEmit&& run) {
     if (!out) return false;
@@ -121,11 +122,34 @@ bool mesh_to_host(NuiKinfuMesh* out, nuif_stream_t s, CountEmit&& run) {
 }
 }  // namespace
 
+bool NuiKinfuHIPHashScene::Volume2CLVertices(NuiKinfuMesh* pData) {
+    if (!pData || !m_dirty) return false;
+    clearDirty();
+    DeviceBuffer scratch(nuif_hash_points_scratch_bytes(&m_params));
+    const bool ok = mesh_to_host(pData, m_stream, [&](float* v, float* c, uint32_t cap, uint32_t* total) {
+        check(nuif_hash_points(m_scene.get(), &m_params, v, c, cap, total, scratch.get(), m_stream),
+              "nuif_hash_points");
+    });
+    return ok && !pData->positions.empty();
+}
+
 bool NuiKinfuHIPHashScene::Volume2CLMesh(NuiKinfuMesh* pData) {
     DeviceBuffer scratch(nuif_hash_mesh_scratch_bytes(&m_params));
     return mesh_to_host(pData, m_stream, [&](float* v, float* c, uint32_t cap, uint32_t* total) {
         check(nuif_hash_mesh(m_scene.get(), &m_params, v, c, cap, total, scratch.get(), m_stream), "nuif_hash_mesh");
     });
+}
+
+bool NuiKinfuHIPScene::Volume2CLVertices(NuiKinfuMesh* pData) {
+    if (!pData || !m_dirty) return false;
+    clearDirty();
+    DeviceBuffer scratch(nuif_points_dense_scratch_bytes(&m_params));
+    const bool ok = mesh_to_host(pData, m_stream, [&](float* v, float* c, uint32_t cap, uint32_t* total) {
+        check(nuif_points_dense(m_volume.get(), m_color.as<uint8_t>(), &m_params, m_storage, m_surface.as<uint8_t>(),
+                                v, c, cap, total, scratch.get(), m_stream),
+              "nuif_points_dense");
+    });
+    return ok && !pData->positions.empty();
 }
 
 // OCL/NuiKinfuOpenCLScene.cpp:617-786 Volume2CLMesh

@@ -162,6 +162,9 @@ _SIGS = {
     "nuif_mc_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
     "nuif_mc_dense": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p, C.c_uint32,
                               c_void_p, c_void_p, c_void_p]),
+    "nuif_points_dense_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
+    "nuif_points_dense": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p, c_void_p,
+                                  C.c_uint32, c_void_p, c_void_p, c_void_p]),
     "nuif_mesh_dense_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams), C.c_uint32]),
     "nuif_mesh_dense": (c_int, [c_void_p, C.POINTER(TsdfParams), c_int, C.c_uint32, c_void_p, c_void_p,
                                 C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), c_void_p, c_void_p]),
@@ -201,6 +204,9 @@ _SIGS = {
                                          c_void_p]),
     "nuif_hash_mesh_scratch_bytes": (c_size_t, [C.POINTER(HashParams)]),
     "nuif_hash_mesh": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, c_void_p, C.c_uint32, c_void_p, c_void_p, c_void_p]),
+    "nuif_hash_points_scratch_bytes": (c_size_t, [C.POINTER(HashParams)]),
+    "nuif_hash_points": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, c_void_p, C.c_uint32, c_void_p, c_void_p,
+                                 c_void_p]),
     "nuif_hash_stats": (c_int, [c_void_p, C.POINTER(C.c_uint32), c_void_p]),
 }
 
@@ -406,6 +412,26 @@ def tsdf_params(N, dim, trunc):
     p = TsdfParams()
     check(L().nuif_tsdf_params(N, N, N, dim, dim, dim, trunc, C.byref(p)), "nuif_tsdf_params")
     return p
+
+
+def points_dense(volume, params, storage, color_volume=None, surface_blocks=None, capacity=None, stream=None):
+    """nuif_points_dense over device arrays: (points float32[n,3], colours float32[n,4], total)
+    in (y, x, z) column order; capacity None sizes the output from a count pass."""
+    scratch = DeviceArray((L().nuif_points_dense_scratch_bytes(C.byref(params)),), np.uint8)
+    tot = DeviceArray((1,), np.uint32)
+    cv = color_volume.ptr if color_volume is not None else None
+    sb = surface_blocks.ptr if surface_blocks is not None else None
+    if capacity is None:
+        check(L().nuif_points_dense(volume.ptr, cv, C.byref(params), storage, sb, None, None, 0, tot.ptr,
+                                    scratch.ptr, stream), "nuif_points_dense")
+        capacity = int(tot.download()[0])
+    v = DeviceArray((max(capacity, 1), 3), np.float32)
+    c = DeviceArray((max(capacity, 1), 4), np.float32)
+    check(L().nuif_points_dense(volume.ptr, cv, C.byref(params), storage, sb, v.ptr, c.ptr, capacity, tot.ptr,
+                                scratch.ptr, stream), "nuif_points_dense")
+    total = int(tot.download()[0])
+    n = min(capacity, total)
+    return v.download()[:n], c.download()[:n], total
 
 
 class Event:
@@ -659,6 +685,39 @@ class HashScene:
                                  scratch.ptr, self.stream), "nuif_hash_mesh")
         total = int(tot.download()[0])
         return v.download()[:min(n, total)], (c.download()[:min(n, total)] if c else None), total
+
+    def points(self, capacity=None):
+        """nuif_hash_points: (points float32[n,3], colours float32[n,4], total) in visible-list order."""
+        scratch = DeviceArray((L().nuif_hash_points_scratch_bytes(C.byref(self.p)),), np.uint8)
+        tot = DeviceArray((1,), np.uint32)
+        if capacity is None:
+            check(L().nuif_hash_points(self.mem.ptr, C.byref(self.p), None, None, 0, tot.ptr, scratch.ptr,
+                                       self.stream), "nuif_hash_points")
+            capacity = int(tot.download()[0])
+        v = DeviceArray((max(capacity, 1), 3), np.float32)
+        c = DeviceArray((max(capacity, 1), 4), np.float32)
+        check(L().nuif_hash_points(self.mem.ptr, C.byref(self.p), v.ptr, c.ptr, capacity, tot.ptr, scratch.ptr,
+                                   self.stream), "nuif_hash_points")
+        total = int(tot.download()[0])
+        n = min(capacity, total)
+        return v.download()[:n], c.download()[:n], total
+
+    def visible_blocks(self):
+        """(pos int32[n,3], brick ptr int32[n]) of the visible list in list order, the raw
+        brick words uint32[bricks*512] and the voxel colours uint8[bricks*512*4] or None."""
+        raw = self.mem.download()
+        nvis = int(raw[4:8].view(np.uint32)[0])
+        ids = raw[self.offsets[3]:self.offsets[3] + nvis * 4].view(np.int32)
+        ne = self.offsets[6]
+        ent = raw[self.offsets[1]:self.offsets[1] + ne * 16].view(self.ENTRY)[ids]
+        k = ent["key"]
+        pos = np.stack([((k >> 32) & 0xFFFF).astype(np.uint16).view(np.int16),
+                        ((k >> 16) & 0xFFFF).astype(np.uint16).view(np.int16),
+                        (k & 0xFFFF).astype(np.uint16).view(np.int16)], 1).astype(np.int32)
+        nbr = self.p.bricks
+        words = raw[self.offsets[4]:self.offsets[4] + nbr * 512 * 4].view(np.uint32).copy()
+        cols = raw[self.offsets[5]:self.offsets[5] + nbr * 512 * 4].copy() if self.p.color else None
+        return pos, ent["ptr"].astype(np.int32), words, cols
 
     def stats(self):
         out = (C.c_uint32 * 4)()

@@ -90,6 +90,9 @@ def lib():
             "or_eval_phis": (None, [f32p, f32p, f32p]),
             "or_intensity_icp_sums": (None, [ci, ci, f32p, ci, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p, f32p,
                                              f32p, f32p, f32p]),
+            "or_points_dense": (cll, [vp, vp, ci, ci, ci, ci, f32p, vp, vp, cll]),
+            "or_points_hash": (cll, [ci, i32p, i32p, vp, vp, ci, cf, cf, vp, vp, cll]),
+            "or_half_to_float": (cf, [C.c_uint16]),
             "or_expf_det": (cf, [cf]),
             "or_sincosf_det": (None, [cf, C.POINTER(cf), C.POINTER(cf)]),
         }
@@ -396,3 +399,40 @@ def intensity_icp_sums(wl, level, v, n, I, vp, np_, Ip, dp, R, t, Rp, tp, euler,
     lib().or_intensity_icp_sums(wl.width, wl.height, wl.cam6(), level, _f32(v), _f32(n), _f32(I), _f32(vp),
                                 _f32(np_), _f32(Ip), _f32(dp), pose, prev, _f32(euler), _f32(thr), out)
     return out
+
+
+# ---- point clouds -------------------------------------------------------------------
+
+def points_dense(words, color_volume, storage, res, cell, cap=None):
+    """fetchVolumeKernel: (points float32[n,3], colours float32[n,4], total) in (y, x, z)
+    column order.  words: the raw volume (uint32 per voxel for storages 0/1, uint32 pairs
+    for storage 2), res: (Nx, Ny, Nz), cell: (cx, cy, cz)."""
+    words = np.ascontiguousarray(words)
+    cv = None if color_volume is None else np.ascontiguousarray(color_volume, np.uint8)
+    Nx, Ny, Nz = (int(r) for r in res)
+    cell = np.asarray(cell, np.float32)
+    total = lib().or_points_dense(words.ctypes.data, None if cv is None else cv.ctypes.data, storage, Nx, Ny, Nz,
+                                  cell, None, None, 0)
+    n = total if cap is None else min(cap, total)
+    pts = np.empty((max(n, 1), 3), np.float32)
+    cols = np.empty((max(n, 1), 4), np.float32)
+    lib().or_points_dense(words.ctypes.data, None if cv is None else cv.ctypes.data, storage, Nx, Ny, Nz, cell,
+                          pts.ctypes.data, cols.ctypes.data, n)
+    return pts[:n], cols[:n], total
+
+
+def points_hash(pos, ptr, bricks, colors, storage, voxel_size, trunc_scale, cap=None):
+    """fetchHashScene_kernel over the visible blocks in list order: pos int32[n,3],
+    ptr int32[n], bricks uint32[nbricks*512], colors uint8[nbricks*512*4] or None."""
+    pos = np.ascontiguousarray(pos, np.int32).reshape(-1)
+    ptr = np.ascontiguousarray(ptr, np.int32).reshape(-1)
+    br = np.ascontiguousarray(bricks, np.uint32).reshape(-1)
+    cv = None if colors is None else np.ascontiguousarray(colors, np.uint8).reshape(-1)
+    args = (ptr.size, pos, ptr, br.ctypes.data, None if cv is None else cv.ctypes.data, storage, voxel_size,
+            trunc_scale)
+    total = lib().or_points_hash(*args, None, None, 0)
+    n = total if cap is None else min(cap, total)
+    pts = np.empty((max(n, 1), 3), np.float32)
+    cols = np.empty((max(n, 1), 4), np.float32)
+    lib().or_points_hash(*args, pts.ctypes.data, cols.ctypes.data, n)
+    return pts[:n], cols[:n], total
