@@ -19,7 +19,7 @@ ARCH = os.environ.get("NUIF_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["runtime.hip", "preprocess.hip", "icp.hip", "tsdf.hip", "mc.hip", "mesh.hip", "hash.hip", "scan.hip", "color.hip", "points.hip"]
 KINFU_SOURCES = ["NuiKinfuConfig.cpp", "NuiKinfuHIP.cpp", "NuiKinfuSession.cpp", "NuiKinfuHashScene.cpp",
-                 "NuiKinfuColorTracker.cpp", "nuifk_capi.cpp"]
+                 "NuiKinfuColorTracker.cpp", "NuiFrameIO.cpp", "nuifk_capi.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -66,7 +66,7 @@ def build_kinfu():
     srcs = [os.path.join(kdir, s) for s in _existing(kdir, KINFU_SOURCES)]
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
           "-I", os.path.join(ROOT, "include"), *srcs, "-L", LIB, "-lnuif",
-          "-Wl,-rpath,$ORIGIN", "-o", os.path.join(LIB, "libnuif_kinfu.so")])
+          "-lz", "-pthread", "-Wl,-rpath,$ORIGIN", "-o", os.path.join(LIB, "libnuif_kinfu.so")])
 
 
 def build_synth():

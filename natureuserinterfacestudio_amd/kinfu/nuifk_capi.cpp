@@ -11,6 +11,7 @@
 #include <string>
 
 #include "../../include/nuif_kinfu.h"
+#include "NuiFrameIO.h"
 #include "NuiKinfuSession.h"
 
 using namespace NuiKinfuEngine;
@@ -24,6 +25,12 @@ using Session = NuiKinfuSession;
 template <class F> int guarded(F&& f) {
     try {
         return f();
+    } catch (const IoError& e) {
+        t_err = e.what();
+        return NUIF_ERR_IO;
+    } catch (const std::invalid_argument& e) {
+        t_err = e.what();
+        return NUIF_ERR_INVALID;
     } catch (const std::exception& e) {
         t_err = e.what();
         return -2;
@@ -246,6 +253,97 @@ int nuifs_wait(void* srv, float* latency_ms) {
     return guarded([&] {
         if (!srv) throw std::runtime_error("nuifs_wait: null server");
         return static_cast<NuiFusionServer*>(srv)->wait(latency_ms);
+    });
+}
+
+// ---- frame ingest (include/nuif_io.h) ------------------------------------------
+
+int nuif_io_write_frame_image(const char* path, int compressed, int64_t timestamp, uint32_t width,
+                              uint32_t height, const void* buffer, uint32_t bytes) {
+    return guarded([&] {
+        if (!path || (bytes && !buffer)) throw std::invalid_argument("nuif_io_write_frame_image: null argument");
+        NuiFileIOUtilities::writeFrameImage(path, compressed != 0, timestamp, width, height, buffer, bytes);
+        return 0;
+    });
+}
+
+int nuif_io_read_frame_header(const char* path, int64_t* timestamp, uint32_t* width, uint32_t* height) {
+    return guarded([&] {
+        if (!path) throw std::invalid_argument("nuif_io_read_frame_header: null path");
+        if (!NuiFileIOUtilities::readFrameImageHeader(path, timestamp, width, height))
+            throw IoError(std::string(path) + ": could not open file");
+        return 0;
+    });
+}
+
+int nuif_io_read_frame_buffer(const char* path, void* buffer, uint32_t bytes) {
+    return guarded([&] {
+        if (!path || (bytes && !buffer)) throw std::invalid_argument("nuif_io_read_frame_buffer: null argument");
+        NuiFileIOUtilities::readFrameImageBuffer(path, buffer, bytes);
+        return 0;
+    });
+}
+
+int nuif_io_write_camera(const char* path, float fx, float fy, float cx, float cy) {
+    return guarded([&] {
+        if (!path) throw std::invalid_argument("nuif_io_write_camera: null path");
+        NuiFileIOUtilities::writeCamera(path, fx, fy, cx, cy);
+        return 0;
+    });
+}
+
+int nuif_io_read_camera(const char* path, float* fx, float* fy, float* cx, float* cy) {
+    return guarded([&] {
+        if (!path || !fx || !fy) throw std::invalid_argument("nuif_io_read_camera: null argument");
+        if (!NuiFileIOUtilities::readCamera(path, fx, fy, cx, cy))
+            throw IoError(std::string(path) + ": could not open file");
+        return 0;
+    });
+}
+
+int nuif_frame_loader_create(const char* dir, int slots, int threads, int upload, void** out) {
+    return guarded([&] {
+        if (!dir || !out) throw std::invalid_argument("nuif_frame_loader_create: null argument");
+        *out = new NuiKinfuFrameLoader(dir, slots, threads, upload != 0);
+        return 0;
+    });
+}
+
+int nuif_frame_loader_next(void* loader, nuif_loaded_frame* out) {
+    return guarded([&] {
+        if (!loader || !out) throw std::invalid_argument("nuif_frame_loader_next: null argument");
+        return static_cast<NuiKinfuFrameLoader*>(loader)->next(out);
+    });
+}
+
+int nuif_frame_loader_release(void* loader, int slot) {
+    return guarded([&] {
+        if (!loader) throw std::invalid_argument("nuif_frame_loader_release: null loader");
+        static_cast<NuiKinfuFrameLoader*>(loader)->release(slot);
+        return 0;
+    });
+}
+
+int nuif_frame_loader_reset(void* loader) {
+    return guarded([&] {
+        if (!loader) throw std::invalid_argument("nuif_frame_loader_reset: null loader");
+        static_cast<NuiKinfuFrameLoader*>(loader)->reset();
+        return 0;
+    });
+}
+
+int nuif_frame_loader_destroy(void* loader) {
+    return guarded([&] {
+        delete static_cast<NuiKinfuFrameLoader*>(loader);
+        return 0;
+    });
+}
+
+int nuif_frame_save(const char* dir, const nuif_loaded_frame* frame, int compressed) {
+    return guarded([&] {
+        if (!dir || !frame) throw std::invalid_argument("nuif_frame_save: null argument");
+        saveFrame(dir, *frame, compressed != 0);
+        return 0;
     });
 }
 

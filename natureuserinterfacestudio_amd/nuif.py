@@ -4,6 +4,7 @@ This is the Python counterpart of the binding a host application would add;
 the product computation all happens in libnuif.so.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -210,6 +211,18 @@ _SIGS = {
     "nuif_hash_stats": (c_int, [c_void_p, C.POINTER(C.c_uint32), c_void_p]),
 }
 
+class LoadedFrame(C.Structure):
+    """nuif_loaded_frame (include/nuif_io.h)."""
+    _fields_ = [("frame_id", C.c_uint32), ("parts", C.c_uint32), ("slot", C.c_int32), ("pad", C.c_uint32),
+                ("timestamp", C.c_int64 * 4), ("width", C.c_uint32 * 4), ("height", C.c_uint32 * 4),
+                ("depth", c_void_p), ("color", c_void_p), ("cmap", c_void_p), ("vertex", c_void_p),
+                ("device_depth", c_void_p), ("device_color", c_void_p),
+                ("fx", c_float), ("fy", c_float), ("cx", c_float), ("cy", c_float)]
+
+
+FRAME_DEPTH, FRAME_COLOR, FRAME_CMAP, FRAME_VERTEX, FRAME_CAMERA = 1, 2, 4, 8, 16
+ERR_IO = -6
+
 _KSIGS = {
     "nuifk_last_error": (C.c_char_p, []),
     "nuifk_create": (c_int, [C.POINTER(KinfuDesc), C.POINTER(c_void_p)]),
@@ -240,6 +253,20 @@ _KSIGS = {
     "nuifs_engine": (c_void_p, [c_void_p, c_int]),
     "nuifs_submit": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p)]),
     "nuifs_wait": (c_int, [c_void_p, C.POINTER(c_float)]),
+    "nuif_io_write_frame_image": (c_int, [C.c_char_p, c_int, C.c_int64, C.c_uint32, C.c_uint32, c_void_p,
+                                          C.c_uint32]),
+    "nuif_io_read_frame_header": (c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint32)]),
+    "nuif_io_read_frame_buffer": (c_int, [C.c_char_p, c_void_p, C.c_uint32]),
+    "nuif_io_write_camera": (c_int, [C.c_char_p, c_float, c_float, c_float, c_float]),
+    "nuif_io_read_camera": (c_int, [C.c_char_p, C.POINTER(c_float), C.POINTER(c_float), C.POINTER(c_float),
+                                    C.POINTER(c_float)]),
+    "nuif_frame_loader_create": (c_int, [C.c_char_p, c_int, c_int, c_int, C.POINTER(c_void_p)]),
+    "nuif_frame_loader_next": (c_int, [c_void_p, C.POINTER(LoadedFrame)]),
+    "nuif_frame_loader_release": (c_int, [c_void_p, c_int]),
+    "nuif_frame_loader_reset": (c_int, [c_void_p]),
+    "nuif_frame_loader_destroy": (c_int, [c_void_p]),
+    "nuif_frame_save": (c_int, [C.c_char_p, C.POINTER(LoadedFrame), c_int]),
 }
 
 _bound = {}
@@ -272,6 +299,14 @@ def K():
 def declared_symbols():
     """Names of every function declared in include/nuif.h that this binding knows."""
     return sorted(_SIGS)
+
+
+def kcheck(rc, what=""):
+    """check() for the libnuif_kinfu entry points (message from nuifk_last_error)."""
+    if rc < 0:
+        msg = K().nuifk_last_error()
+        raise NuifError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+    return rc
 
 
 def declared_kinfu_symbols():
@@ -831,3 +866,83 @@ class ColorTracker:
                                         out.ptr, self.stream), "nuif_color_difference")
         d = out.download()
         return int(d[0]), int(d[1])
+
+
+# ---- frame ingest (include/nuif_io.h) ---------------------------------------------
+
+_PART_TYPES = [(np.uint16, 1), (np.uint8, 4), (np.float32, 2), (np.float32, 3)]
+
+
+def write_frame_image(path, timestamp, buffer, compressed=True):
+    """nuif_io_write_frame_image of an array [h, w(, c)]."""
+    a = np.ascontiguousarray(buffer)
+    h, w = a.shape[0], a.shape[1]
+    kcheck(K().nuif_io_write_frame_image(os.fsencode(path), int(bool(compressed)), int(timestamp), w, h,
+                                         a.ctypes.data, a.nbytes), "nuif_io_write_frame_image")
+
+
+def read_frame_image(path, dtype, channels):
+    """(timestamp, array[h, w, channels]) through nuif_io_read_frame_header / _buffer."""
+    ts, w, h = C.c_int64(), C.c_uint32(), C.c_uint32()
+    kcheck(K().nuif_io_read_frame_header(os.fsencode(path), C.byref(ts), C.byref(w), C.byref(h)),
+           "nuif_io_read_frame_header")
+    out = np.empty((h.value, w.value, channels), dtype)
+    kcheck(K().nuif_io_read_frame_buffer(os.fsencode(path), out.ctypes.data, out.nbytes),
+           "nuif_io_read_frame_buffer")
+    return ts.value, out
+
+
+def _part_array(ptr, w, h, k):
+    dt, ch = _PART_TYPES[k]
+    n = w * h * ch
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), (n,)).reshape(h, w, ch)
+
+
+class FrameLoader:
+    """nuif_frame_loader_*: frames of a recording in id order, prefetched by `threads`
+    workers into `slots` buffers (uploaded to the device when upload is set)."""
+
+    def __init__(self, dirname, slots=4, threads=2, upload=False):
+        if upload:
+            init()
+        self.h = c_void_p()
+        kcheck(K().nuif_frame_loader_create(os.fsencode(dirname), slots, threads, int(bool(upload)),
+                                            C.byref(self.h)), "nuif_frame_loader_create")
+
+    def next_raw(self):
+        """The nuif_loaded_frame of the next frame (hold it until release) or None at the end."""
+        f = LoadedFrame()
+        rc = kcheck(K().nuif_frame_loader_next(self.h, C.byref(f)), "nuif_frame_loader_next")
+        return f if rc == 1 else None
+
+    def next(self):
+        """A dict copy of the next frame (its slot released), or None at the end."""
+        f = self.next_raw()
+        if f is None:
+            return None
+        out = {"id": f.frame_id, "parts": f.parts}
+        ptrs = [f.depth, f.color, f.cmap, f.vertex]
+        for k, name in enumerate(["depth", "color", "cmap", "vertex"]):
+            if f.parts & (1 << k):
+                out[name] = (f.timestamp[k], _part_array(ptrs[k], f.width[k], f.height[k], k).copy())
+        if f.parts & FRAME_CAMERA:
+            out["camera"] = (f.fx, f.fy, f.cx, f.cy)
+        self.release(f.slot)
+        return out
+
+    def release(self, slot):
+        kcheck(K().nuif_frame_loader_release(self.h, slot), "nuif_frame_loader_release")
+
+    def reset(self):
+        kcheck(K().nuif_frame_loader_reset(self.h), "nuif_frame_loader_reset")
+
+    def close(self):
+        if self.h:
+            kcheck(K().nuif_frame_loader_destroy(self.h), "nuif_frame_loader_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
