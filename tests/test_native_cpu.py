@@ -37,6 +37,15 @@ def test_engine_library_exports():
     assert set(names) <= set(nuif.declared_kinfu_symbols()), sorted(set(names) - set(nuif.declared_kinfu_symbols()))
 
 
+def test_ingest_library_exports():
+    lib = libkinfu()
+    names = declared("nuif_io.h")
+    assert len(names) == 11
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(nuif.declared_kinfu_symbols())
+
+
 def test_python_binding_covers_header():
     # every header function has a ctypes signature in nuif.py (so tests call them typed)
     names = set(declared("nuif.h"))
