@@ -240,6 +240,37 @@ int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, con
                             const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap,
                             float* nmap, uint8_t* cmap, int storage, const int32_t* enable,
                             const uint8_t* surface_blocks, nuif_stream_t s);
+/* ---- shifting volume (OCL/NuiKinfuOpenCLShiftScene.cpp) ----------------------------
+ * wrap: int32[3] voxel wrap (any integers, reduced modulo the resolution): the voxel at
+ * logical (x, y, z) is stored at ((x + w.x) % Nx, (y + w.y) % Ny, (z + w.z) % Nz)
+ * (K/tsdfVolume.cl:71,104, K/raycast.cl:62,184); NULL or 0 = the plain volume. */
+/* nuif_tsdf_integrate in a wrapped volume; the surface blocks are marked at stored positions */
+int nuif_tsdf_integrate_wrapped(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
+                                const TsdfParams* params, const NuiCLCameraParams* cam,
+                                const NuiCLRigidTransform* pose, int storage, int max_color_weight,
+                                const int32_t* wrap, const int32_t* gate, unsigned long long* updated,
+                                const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s);
+/* nuif_tsdf_raycast_gated in a wrapped volume.  Every trilinear corner is wrapped on its own
+ * (the reference wraps the first corner only); the surface-block skip is used only when the
+ * wrap is 0. */
+int nuif_tsdf_raycast_wrapped(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
+                              const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
+                              uint8_t* cmap, int storage, const int32_t* wrap, const int32_t* enable,
+                              const uint8_t* surface_blocks, nuif_stream_t s);
+/* K/tsdfVolume.cl:192-211 initializeVolume over range[3] voxels from offset `wrap`
+ * (clearSlice, OCL/NuiKinfuOpenCLShiftScene.cpp:72-117) */
+int nuif_tsdf_reset_slice(void* volume, uint8_t* color_volume, const TsdfParams* params, int storage,
+                          const int32_t* wrap, const int32_t* range, nuif_stream_t s);
+/* K/extract.cl:80-190 fetchShiftVolumeKernel (fetchSlice, OCL/NuiKinfuOpenCLShiftScene.cpp:119-245,
+ * and the shifting scene's Volume2CLVertices, :366-): range[0] x range[1] columns from stored
+ * column (x + wrap.x, y + wrap.y), walking voxel_z = wrap.z .. wrap.z + range[2] - 1 at stored
+ * z (voxel_z + wrap.z) % Nz as the reference does, points placed by the volume's voxel offset
+ * `offset` (int32[3]) as the reference computes it.  wrap in [0, N); output as
+ * nuif_points_dense, in (y, x, z) order; scratch: nuif_points_dense_scratch_bytes. */
+int nuif_points_dense_slice(const void* volume, const uint8_t* color_volume, const TsdfParams* params, int storage,
+                            const int32_t* wrap, const int32_t* offset, const int32_t* range, float* points,
+                            float* colors, uint32_t capacity, uint32_t* total, void* scratch, nuif_stream_t s);
+
 /* Surface blocks of a volume: one byte per 8 x 8 x 8 voxel block (y-major, z fastest),
  * then one byte per 32 x 32 x 32 superblock (the OR of its blocks), set by
  * nuif_tsdf_integrate (surface_blocks argument) when it stores a tsdf <= 0 and cleared

@@ -81,4 +81,20 @@ __host__ __device__ __forceinline__ SurfaceBlocks surface_blocks_of(uint8_t* b, 
                          super_dim(Nx), super_dim(Nz)};
 }
 
+// Cyclic offset of a shifting volume (K/tsdfVolume.cl:71,104, K/raycast.cl:62,184): the
+// voxel at logical (x, y, z) is stored at ((x + w.x) % Nx, (y + w.y) % Ny, (z + w.z) % Nz),
+// with 0 <= w < N on every axis (the reference's getVoxelWrap reduced modulo N).
+struct VoxelWrap {
+    int x, y, z;
+    __host__ __device__ __forceinline__ bool zero() const { return (x | y | z) == 0; }
+};
+__device__ __forceinline__ int wrap1(int g, int w, int n) {  // (g + w) % n for 0 <= g, w < n
+    const int s = g + w;
+    return s >= n ? s - n : s;
+}
+// first voxel of the stored column of logical column (x, y)
+__device__ __forceinline__ size_t wrapped_column(int x, int y, const VoxelWrap& w, int Nx, int Ny, int Nz) {
+    return ((size_t)wrap1(y, w.y, Ny) * Nx + wrap1(x, w.x, Nx)) * Nz;
+}
+
 }  // namespace nuif

@@ -63,6 +63,26 @@ __global__ void reset_kernel(typename S::word* __restrict__ vol, uint32_t* __res
     }
 }
 
+// K/tsdfVolume.cl:192-211 initializeVolume over a range with a voxel offset (clearSlice of
+// the shifting volume, OCL/NuiKinfuOpenCLShiftScene.cpp:72-117): the voxels stored at
+// ((x + w.x) % Nx, (y + w.y) % Ny, (z + w.z) % Nz) for x < rx, y < ry, z < rz.  One thread
+// per voxel, z fastest, so a wavefront clears 64 consecutive stored voxels of a column.
+template <class S>
+__global__ void reset_slice_kernel(typename S::word* __restrict__ vol, uint32_t* __restrict__ cvol, int Nx, int Ny,
+                                   int Nz, VoxelWrap w, int rx, int ry, int rz) {
+    const size_t n = (size_t)rx * ry * rz;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const typename S::word w0 = S::init();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % rz);
+        const size_t c = i / rz;
+        const int x = (int)(c % rx), y = (int)(c / rx);
+        const size_t idx = wrapped_column(x, y, w, Nx, Ny, Nz) + wrap1(z, w.z, Nz);
+        vol[idx] = w0;
+        if (cvol) cvol[idx] = 0u;
+    }
+}
+
 // ---- integration ---------------------------------------------------------------
 
 // Column chunk of the integration: 32 voxels keep the LDS transpose buffer at 8.3 KB
@@ -281,7 +301,7 @@ __global__ void __launch_bounds__(64)
                           typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
                           NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
                           const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated,
-                          const float* __restrict__ bounds, SurfaceBlocks surf) {
+                          const float* __restrict__ bounds, SurfaceBlocks surf, VoxelWrap wr) {
     if (gate && *gate == 0) return;
     using word = typename S::word;
     constexpr int VPL = 16 / S::kBytes;      // voxels per lane per 16-byte vector
@@ -369,7 +389,7 @@ __global__ void __launch_bounds__(64)
                 act[j] = a;
                 if (a) {
                     const int gx = blockIdx.x * 8 + (col & 7), gy = blockIdx.y * 8 + (col >> 3);
-                    const size_t base = ((size_t)gy * Nx + gx) * Nz + zc + sub * VPL;
+                    const size_t base = wrapped_column(gx, gy, wr, Nx, Ny, Nz) + wrap1(zc + sub * VPL, wr.z, Nz);
                     const uint4 raw = *reinterpret_cast<const uint4*>(vol + base);
                     const word* wv = reinterpret_cast<const word*>(&raw);
 #pragma unroll
@@ -382,7 +402,8 @@ __global__ void __launch_bounds__(64)
                 const int col = (it0 + j) * CPI + lane / LPC;
                 const float* tv = &ts[col * kLdsPitch + sub * VPL];
                 const int gx = blockIdx.x * 8 + (col & 7), gy = blockIdx.y * 8 + (col >> 3);
-                const size_t base = ((size_t)gy * Nx + gx) * Nz + zc + sub * VPL;
+                const int pz = wrap1(zc + sub * VPL, wr.z, Nz);  // stored z of the vector (w.z % VPL == 0)
+                const size_t base = wrapped_column(gx, gy, wr, Nx, Ny, Nz) + pz;
                 uint4 outv;
                 word* ov = reinterpret_cast<word*>(&outv);
                 bool neg = false;
@@ -392,8 +413,9 @@ __global__ void __launch_bounds__(64)
                     neg |= !isnan(tv[q]) && S::nonpositive(ov[q]);
                 }
                 *reinterpret_cast<uint4*>(vol + base) = outv;
-                // the VPL voxels share one 8-voxel z block (VPL divides 8)
-                if (neg && surf.b) surf.mark(gx, gy, zc + sub * VPL);
+                // the VPL voxels share one 8-voxel z block (VPL divides 8); the flags
+                // describe stored positions
+                if (neg && surf.b) surf.mark(wrap1(gx, wr.x, Nx), wrap1(gy, wr.y, Ny), pz);
                 if (COLOR) {
                     const int* pv = &cp[col * kLdsPitch + sub * VPL];
 #pragma unroll
@@ -421,7 +443,8 @@ __global__ void integrate_column_kernel(const float* __restrict__ depth, const u
                                         typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
                                         NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
                                         int maxcw, const int32_t* __restrict__ gate,
-                                        unsigned long long* __restrict__ updated, SurfaceBlocks surf) {
+                                        unsigned long long* __restrict__ updated, SurfaceBlocks surf,
+                                        VoxelWrap wr) {
     if (gate && *gate == 0) return;
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -429,19 +452,21 @@ __global__ void integrate_column_kernel(const float* __restrict__ depth, const u
     const NuiCLRigidTransform g = *pose;
     ColumnWalk c = column_setup(x, y, Nx, Ny, p, cam, g);
     const float trunc = p.tranc_dist, tr_inv = 1.f / p.tranc_dist, cz = p.cell_size[2], R8 = g.R_inv[8];
-    const size_t base = ((size_t)y * Nx + x) * Nz;
+    const size_t base = wrapped_column(x, y, wr, Nx, Ny, Nz);
+    const int px = wrap1(x, wr.x, Nx), py = wrap1(y, wr.y, Ny);
     unsigned nupd = 0;
     for (int z = 0; z < Nz; ++z, column_advance(c, cz)) {
         int pix;
         const float v = column_step(c, depth, cam, R8, trunc, tr_inv, &pix);
         if (isnan(v)) continue;
         ++nupd;
-        const typename S::word nw = update_voxel<S>(vol[base + z], v);
-        vol[base + z] = nw;
-        if (surf.b && S::nonpositive(nw)) surf.mark(x, y, z);
+        const int pz = wrap1(z, wr.z, Nz);
+        const typename S::word nw = update_voxel<S>(vol[base + pz], v);
+        vol[base + pz] = nw;
+        if (surf.b && S::nonpositive(nw)) surf.mark(px, py, pz);
         if (COLOR && pix >= 0) {
             const uchar4 nc = bgra[pix];
-            if (nc.w != 0) cvol[base + z] = update_color(cvol[base + z], nc, maxcw);
+            if (nc.w != 0) cvol[base + pz] = update_color(cvol[base + pz], nc, maxcw);
         }
     }
     if (updated && nupd) atomicAdd(updated, (unsigned long long)nupd);
@@ -449,12 +474,18 @@ __global__ void integrate_column_kernel(const float* __restrict__ depth, const u
 
 // ---- raycast -------------------------------------------------------------------
 
-template <class S> struct VolView {
+template <class S, bool WRAP> struct VolView {
     const typename S::word* __restrict__ v;
     int Nx, Ny, Nz;
+    VoxelWrap w;
     __device__ __forceinline__ float sample(long long idx) const {
         const typename S::word w = v[idx];
         return S::weight(w) > 0 ? S::tsdf(w) : kNaN;
+    }
+    // stored index of the voxel at logical g (every component inside the volume)
+    __device__ __forceinline__ long long index(int gx, int gy, int gz) const {
+        if (WRAP) return (long long)wrapped_column(gx, gy, w, Nx, Ny, Nz) + wrap1(gz, w.z, Nz);
+        return ((long long)gy * Nx + gx) * Nz + gz;
     }
 };
 
@@ -488,7 +519,10 @@ __device__ __forceinline__ void get_voxel_fast(f3 pt, const TsdfParams& p, const
 
 // K/raycast.cl:36-120 interpolateTrilineary, corner weights exactly as written
 // (the x and y fractions land on swapped corners relative to the y-major layout).
-template <class S> __device__ float trilinear(f3 pt, const VolView<S>& vv, const TsdfParams& p) {
+// In a shifting volume every corner is wrapped on its own; the reference wraps only
+// the first corner and steps to the others by the unwrapped strides, which reads the
+// wrong voxels (or past the end of the volume) where a cell straddles the wrap.
+template <class S, bool WRAP> __device__ float trilinear(f3 pt, const VolView<S, WRAP>& vv, const TsdfParams& p) {
     int g[3];
     get_voxel(pt, p, g);
     if (g[0] <= 0 || g[0] >= vv.Nx - 1) return kNaN;
@@ -503,14 +537,30 @@ template <class S> __device__ float trilinear(f3 pt, const VolView<S>& vv, const
     const float a = (pt.x - ((float)g[0] + 0.5f) * p.cell_size[0]) / p.cell_size[0];
     const float b = (pt.y - ((float)g[1] + 0.5f) * p.cell_size[1]) / p.cell_size[1];
     const float c = (pt.z - ((float)g[2] + 0.5f) * p.cell_size[2]) / p.cell_size[2];
-    const long long sx = vv.Nz, sy = (long long)vv.Nx * vv.Nz;
-    const long long idx = ((long long)g[1] * vv.Nx + g[0]) * vv.Nz + g[2];
     // the eight corners (all inside the volume) are read together; any unobserved
     // corner makes the result NaN, as the reference's early returns do
-    const float f000 = vv.sample(idx), f001 = vv.sample(idx + 1);
-    const float f011 = vv.sample(idx + sx + 1), f010 = vv.sample(idx + sx);
-    const float f110 = vv.sample(idx + sx + sy), f111 = vv.sample(idx + sx + sy + 1);
-    const float f101 = vv.sample(idx + sy + 1), f100 = vv.sample(idx + sy);
+    float f000, f001, f011, f010, f110, f111, f101, f100;
+    if (WRAP) {
+        f000 = vv.sample(vv.index(g[0], g[1], g[2]));
+        f001 = vv.sample(vv.index(g[0], g[1], g[2] + 1));
+        f011 = vv.sample(vv.index(g[0] + 1, g[1], g[2] + 1));
+        f010 = vv.sample(vv.index(g[0] + 1, g[1], g[2]));
+        f110 = vv.sample(vv.index(g[0] + 1, g[1] + 1, g[2]));
+        f111 = vv.sample(vv.index(g[0] + 1, g[1] + 1, g[2] + 1));
+        f101 = vv.sample(vv.index(g[0], g[1] + 1, g[2] + 1));
+        f100 = vv.sample(vv.index(g[0], g[1] + 1, g[2]));
+    } else {
+        const long long sx = vv.Nz, sy = (long long)vv.Nx * vv.Nz;
+        const long long idx = ((long long)g[1] * vv.Nx + g[0]) * vv.Nz + g[2];
+        f000 = vv.sample(idx);
+        f001 = vv.sample(idx + 1);
+        f011 = vv.sample(idx + sx + 1);
+        f010 = vv.sample(idx + sx);
+        f110 = vv.sample(idx + sx + sy);
+        f111 = vv.sample(idx + sx + sy + 1);
+        f101 = vv.sample(idx + sy + 1);
+        f100 = vv.sample(idx + sy);
+    }
     if (isnan(f000) || isnan(f001) || isnan(f011) || isnan(f010) || isnan(f110) || isnan(f111) || isnan(f101) ||
         isnan(f100))
         return kNaN;
@@ -564,12 +614,12 @@ __device__ __forceinline__ void superblock_span(const int g[3], const int N[3], 
 // superblock flags kept in LDS when the volume has at most this many superblocks
 constexpr int kSuperLds = 4096;
 
-template <class S>
+template <class S, bool WRAP>
 __global__ void __launch_bounds__(64)
     raycast_kernel(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol, TsdfParams p,
                    NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
                    float* __restrict__ nmap, uchar4* __restrict__ cmap, const int32_t* __restrict__ enable,
-                   SurfaceBlocks surf) {
+                   SurfaceBlocks surf, VoxelWrap wr) {
     if (enable && *enable == 0) return;
     // the superblock flags, read by every lane at every superblock it enters
     __shared__ __align__(16) uint8_t sflags[kSuperLds];
@@ -590,7 +640,7 @@ __global__ void __launch_bounds__(64)
     if (x >= W || y >= H) return;
     const size_t id = (size_t)y * W + x;
     const f3 NaN3 = mk3(kNaN, kNaN, kNaN);
-    VolView<S> vv{vol, (int)p.resolution[0], (int)p.resolution[1], (int)p.resolution[2]};
+    VolView<S, WRAP> vv{vol, (int)p.resolution[0], (int)p.resolution[1], (int)p.resolution[2], wr};
     const NuiCLRigidTransform g = *pose;
     const f3 off = mk3(p.dimension[0] / 2.0f, p.dimension[1] / 2.0f, 0.f);
     const f3 rs = add3(mk3(g.t[0], g.t[1], g.t[2]), off);
@@ -613,7 +663,7 @@ __global__ void __launch_bounds__(64)
         gv[0] = max(0, min(gv[0], vv.Nx - 1));
         gv[1] = max(0, min(gv[1], vv.Ny - 1));
         gv[2] = max(0, min(gv[2], vv.Nz - 1));
-        long long idx = ((long long)gv[1] * vv.Nx + gv[0]) * vv.Nz + gv[2];
+        long long idx = vv.index(gv[0], gv[1], gv[2]);
         float tsdf = vv.sample(idx);
         const float max_t = 3.f * (p.dimension[0] + p.dimension[1] + p.dimension[2]);
         const float step = p.tranc_dist * 0.8f;
@@ -657,7 +707,7 @@ __global__ void __launch_bounds__(64)
                 if (tn != tcur) {
                     int gs[3];
                     get_voxel_fast(add3(rs, mul3(dir, tn)), p, ri, gs);
-                    tsdf = vv.sample(((long long)gs[1] * vv.Nx + gs[0]) * vv.Nz + gs[2]);
+                    tsdf = vv.sample(vv.index(gs[0], gs[1], gs[2]));
                     tcur = tn;
                     prev_safe = true;
                 }
@@ -671,7 +721,7 @@ __global__ void __launch_bounds__(64)
                 tb[k] = tn;
                 get_voxel_fast(add3(rs, mul3(dir, tn + step)), p, ri, gv);
                 inb[k] = gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny && gv[2] < vv.Nz;
-                ib[k] = inb[k] ? ((long long)gv[1] * vv.Nx + gv[0]) * vv.Nz + gv[2] : 0;
+                ib[k] = inb[k] ? vv.index(gv[0], gv[1], gv[2]) : 0;
                 safe[k] = surf.b && inb[k] && surf.b[surf.index(gv[0], gv[1], gv[2])] == 0;
                 tn += step;  // the reference's `time_curr += time_step`
             }
@@ -710,8 +760,8 @@ __global__ void __launch_bounds__(64)
         if (crossing) {
             do {
                 // both interpolations are evaluated (pure reads) so that their loads overlap
-                const float Ftdt = trilinear<S>(add3(rs, mul3(dir, tcur + step)), vv, p);
-                const float Ft = trilinear<S>(add3(rs, mul3(dir, tcur)), vv, p);
+                const float Ftdt = trilinear<S, WRAP>(add3(rs, mul3(dir, tcur + step)), vv, p);
+                const float Ft = trilinear<S, WRAP>(add3(rs, mul3(dir, tcur)), vv, p);
                 if (isnan(Ftdt) || isnan(Ft)) break;
                 const float Ts = tcur - step * Ft / (Ftdt - Ft);
                 const f3 vf = add3(rs, mul3(dir, Ts));
@@ -721,14 +771,14 @@ __global__ void __launch_bounds__(64)
                 get_voxel(add3(rs, mul3(dir, tcur)), p, gg);
                 if (gg[0] > 1 && gg[1] > 1 && gg[2] > 1 && gg[0] < vv.Nx - 2 && gg[1] < vv.Ny - 2 && gg[2] < vv.Nz - 2) {
                     f3 n, q;
-                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S>(q, vv, p);
-                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S>(q, vv, p);
+                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S, WRAP>(q, vv, p);
+                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S, WRAP>(q, vv, p);
                     n.x = fx1 - fx2;
-                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S>(q, vv, p);
-                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S>(q, vv, p);
+                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S, WRAP>(q, vv, p);
+                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S, WRAP>(q, vv, p);
                     n.y = fy1 - fy2;
-                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S>(q, vv, p);
-                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S>(q, vv, p);
+                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S, WRAP>(q, vv, p);
+                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S, WRAP>(q, vv, p);
                     n.z = fz1 - fz2;
                     nout = normalize3(n);
                 }
@@ -746,30 +796,32 @@ template <class S>
 int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                      const TsdfParams& p, const NuiCLCameraParams& cam, const NuiCLRigidTransform* pose, int maxcw,
                      const int32_t* gate, unsigned long long* updated, const float* bounds, uint8_t* surface,
-                     hipStream_t st) {
+                     VoxelWrap wr, hipStream_t st) {
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const SurfaceBlocks surf = surface_blocks_of(surface, Nx, Ny, Nz);
     const bool color = bgra && color_volume;
     auto* vol = reinterpret_cast<typename S::word*>(volume);
     auto* cv = reinterpret_cast<uchar4*>(color_volume);
     auto* bg = reinterpret_cast<const uchar4*>(bgra);
-    if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0) {
+    // the tile kernel's 16-byte vectors stay aligned and unsplit when w.z is a multiple of
+    // the voxels per vector
+    if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0 && wr.z % (16 / S::kBytes) == 0) {
         dim3 grid(Nx / 8, Ny / 8, div_up(Nz, kSegVoxels));
         if (color)
             integrate_tile_kernel<S, true><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                 bounds, surf);
+                                                                 bounds, surf, wr);
         else
             integrate_tile_kernel<S, false><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                  bounds, surf);
+                                                                  bounds, surf, wr);
         NUIF_LAUNCHED("integrate_tile_kernel");
     } else {
         dim3 block(16, 16), grid(div_up(Nx, 16), div_up(Ny, 16));
         if (color)
             integrate_column_kernel<S, true><<<grid, block, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                     surf);
+                                                                     surf, wr);
         else
             integrate_column_kernel<S, false><<<grid, block, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate,
-                                                                      updated, surf);
+                                                                      updated, surf, wr);
         NUIF_LAUNCHED("integrate_column_kernel");
     }
     return NUIF_OK;
@@ -778,14 +830,22 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
 template <class S>
 int launch_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams& p,
                    const NuiCLCameraParams& cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
-                   uint8_t* cmap, const int32_t* enable, const uint8_t* surface, hipStream_t st) {
+                   uint8_t* cmap, const int32_t* enable, const uint8_t* surface, VoxelWrap wr, hipStream_t st) {
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
     dim3 grid(div_up(W, 8), div_up(H, 8));
-    const SurfaceBlocks surf = surface_blocks_of(const_cast<uint8_t*>(surface), (int)p.resolution[0],
-                                                 (int)p.resolution[1], (int)p.resolution[2]);
-    raycast_kernel<S><<<grid, 64, 0, st>>>(reinterpret_cast<const typename S::word*>(volume),
-                                           reinterpret_cast<const uchar4*>(color_volume), p, cam, pose, vmap, nmap,
-                                           reinterpret_cast<uchar4*>(cmap), enable, surf);
+    const auto* vol = reinterpret_cast<const typename S::word*>(volume);
+    const auto* cv = reinterpret_cast<const uchar4*>(color_volume);
+    auto* cm = reinterpret_cast<uchar4*>(cmap);
+    if (wr.zero()) {
+        const SurfaceBlocks surf = surface_blocks_of(const_cast<uint8_t*>(surface), (int)p.resolution[0],
+                                                     (int)p.resolution[1], (int)p.resolution[2]);
+        raycast_kernel<S, false><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, surf, wr);
+    } else {
+        // the surface-block skip works on logical positions: not used in a wrapped volume
+        const SurfaceBlocks none = surface_blocks_of(nullptr, (int)p.resolution[0], (int)p.resolution[1],
+                                                     (int)p.resolution[2]);
+        raycast_kernel<S, true><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, none, wr);
+    }
     NUIF_LAUNCHED("raycast_kernel");
     return NUIF_OK;
 }
@@ -851,28 +911,85 @@ int nuif_tsdf_reset(void* volume, uint8_t* color_volume, const TsdfParams* param
     return NUIF_OK;
 }
 
-int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
-                        const TsdfParams* params, const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose,
-                        int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
-                        const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s) {
+// host wrap (any integers) -> the reduced VoxelWrap of a volume
+static VoxelWrap wrap_of(const int32_t* w, const TsdfParams& p) {
+    VoxelWrap r{0, 0, 0};
+    if (!w) return r;
+    int* o[3] = {&r.x, &r.y, &r.z};
+    for (int k = 0; k < 3; ++k) {
+        const long long n = (long long)p.resolution[k];
+        *o[k] = (int)((((long long)w[k] % n) + n) % n);
+    }
+    return r;
+}
+
+int nuif_tsdf_integrate_wrapped(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
+                                const TsdfParams* params, const NuiCLCameraParams* cam,
+                                const NuiCLRigidTransform* pose, int storage, int max_color_weight,
+                                const int32_t* wrap, const int32_t* gate, unsigned long long* updated,
+                                const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     int rc = check_params(params);
     if (rc) return rc;
     NUIF_CHECK_ARG(depth && volume && cam && pose, "nuif_tsdf_integrate: null argument");
     NUIF_CHECK_ARG(cam->depthImageWidth > 0 && cam->depthImageHeight > 0, "nuif_tsdf_integrate: empty image");
     hipStream_t st = stream_of(s);
+    const VoxelWrap wr = wrap_of(wrap, *params);
     switch (storage) {
         case NUIF_TSDF_S16:
             return launch_integrate<StS16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
-                                          gate, updated, depth_bounds, surface_blocks, st);
+                                          gate, updated, depth_bounds, surface_blocks, wr, st);
         case NUIF_TSDF_F16:
             return launch_integrate<StF16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
-                                          gate, updated, depth_bounds, surface_blocks, st);
+                                          gate, updated, depth_bounds, surface_blocks, wr, st);
         case NUIF_TSDF_F32:
             return launch_integrate<StF32>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
-                                          gate, updated, depth_bounds, surface_blocks, st);
+                                          gate, updated, depth_bounds, surface_blocks, wr, st);
         default: set_error("nuif_tsdf_integrate: unknown storage %d", storage); return NUIF_ERR_INVALID;
     }
+}
+
+int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
+                        const TsdfParams* params, const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose,
+                        int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
+                        const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s) {
+    return nuif_tsdf_integrate_wrapped(depth, bgra, volume, color_volume, params, cam, pose, storage,
+                                       max_color_weight, nullptr, gate, updated, depth_bounds, surface_blocks, s);
+}
+
+int nuif_tsdf_reset_slice(void* volume, uint8_t* color_volume, const TsdfParams* params, int storage,
+                          const int32_t* wrap, const int32_t* range, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(params);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(volume && wrap && range, "nuif_tsdf_reset_slice: null argument");
+    const int Nx = (int)params->resolution[0], Ny = (int)params->resolution[1], Nz = (int)params->resolution[2];
+    NUIF_CHECK_ARG(range[0] >= 0 && range[0] <= Nx && range[1] >= 0 && range[1] <= Ny && range[2] >= 0 &&
+                       range[2] <= Nz,
+                   "nuif_tsdf_reset_slice: range outside the volume");
+    const size_t n = (size_t)range[0] * range[1] * range[2];
+    if (n == 0) return NUIF_OK;
+    const VoxelWrap w = wrap_of(wrap, *params);
+    const unsigned blocks = std::min<unsigned>(div_up(n, 256), 65536u);
+    hipStream_t st = stream_of(s);
+    uint32_t* cv = reinterpret_cast<uint32_t*>(color_volume);
+    switch (storage) {
+        case NUIF_TSDF_S16:
+            reset_slice_kernel<StS16><<<blocks, 256, 0, st>>>((uint32_t*)volume, cv, Nx, Ny, Nz, w, range[0], range[1],
+                                                               range[2]);
+            break;
+        case NUIF_TSDF_F16:
+            reset_slice_kernel<StF16><<<blocks, 256, 0, st>>>((uint32_t*)volume, cv, Nx, Ny, Nz, w, range[0], range[1],
+                                                               range[2]);
+            break;
+        case NUIF_TSDF_F32:
+            reset_slice_kernel<StF32><<<blocks, 256, 0, st>>>((uint2*)volume, cv, Nx, Ny, Nz, w, range[0], range[1],
+                                                               range[2]);
+            break;
+        default: set_error("nuif_tsdf_reset_slice: unknown storage %d", storage); return NUIF_ERR_INVALID;
+    }
+    NUIF_LAUNCHED("reset_slice_kernel");
+    return NUIF_OK;
 }
 
 size_t nuif_tsdf_depth_bounds_bytes(int width, int height) {
@@ -893,25 +1010,34 @@ int nuif_tsdf_depth_bounds(const float* depth, int width, int height, float* bou
     return NUIF_OK;
 }
 
-int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
-                            const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
-                            uint8_t* cmap, int storage, const int32_t* enable, const uint8_t* surface_blocks,
-                            nuif_stream_t s) {
+int nuif_tsdf_raycast_wrapped(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
+                              const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
+                              uint8_t* cmap, int storage, const int32_t* wrap, const int32_t* enable,
+                              const uint8_t* surface_blocks, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     int rc = check_params(params);
     if (rc) return rc;
     NUIF_CHECK_ARG(volume && cam && pose && vmap && nmap, "nuif_tsdf_raycast: null argument");
     NUIF_CHECK_ARG(cam->depthImageWidth > 0 && cam->depthImageHeight > 0, "nuif_tsdf_raycast: empty image");
     hipStream_t st = stream_of(s);
+    const VoxelWrap wr = wrap_of(wrap, *params);
     switch (storage) {
         case NUIF_TSDF_S16: return launch_raycast<StS16>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable,
-                                                            surface_blocks, st);
+                                                            surface_blocks, wr, st);
         case NUIF_TSDF_F16: return launch_raycast<StF16>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable,
-                                                            surface_blocks, st);
+                                                            surface_blocks, wr, st);
         case NUIF_TSDF_F32: return launch_raycast<StF32>(volume, color_volume, *params, *cam, pose, vmap, nmap, cmap, enable,
-                                                            surface_blocks, st);
+                                                            surface_blocks, wr, st);
         default: set_error("nuif_tsdf_raycast: unknown storage %d", storage); return NUIF_ERR_INVALID;
     }
+}
+
+int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
+                            const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
+                            uint8_t* cmap, int storage, const int32_t* enable, const uint8_t* surface_blocks,
+                            nuif_stream_t s) {
+    return nuif_tsdf_raycast_wrapped(volume, color_volume, params, cam, pose, vmap, nmap, cmap, storage, nullptr,
+                                     enable, surface_blocks, s);
 }
 
 int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams* params,

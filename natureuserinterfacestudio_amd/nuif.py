@@ -163,6 +163,16 @@ _SIGS = {
     "nuif_mc_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
     "nuif_mc_dense": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p, C.c_uint32,
                               c_void_p, c_void_p, c_void_p]),
+    "nuif_tsdf_integrate_wrapped": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(TsdfParams),
+                                            C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                            c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nuif_tsdf_raycast_wrapped": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                          c_void_p]),
+    "nuif_tsdf_reset_slice": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p,
+                                      c_void_p]),
+    "nuif_points_dense_slice": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, C.c_uint32, c_void_p, c_void_p, c_void_p]),
     "nuif_points_dense_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
     "nuif_points_dense": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p, c_void_p,
                                   C.c_uint32, c_void_p, c_void_p, c_void_p]),

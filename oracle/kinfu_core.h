@@ -472,6 +472,8 @@ struct VolS16 {
         v[2 * i] = (int16_t)(int)(f * (T)32766);  // convert_short: RTZ
         v[2 * i + 1] = (int16_t)w;
     }
+    void reset_voxel(size_t i) { v[2 * i] = 32767; v[2 * i + 1] = 0; }  // (SHORT_NAN, 0)
+    const void* words() const { return v.data(); }
 };
 struct VolF64 {
     std::vector<double> f;
@@ -480,14 +482,26 @@ struct VolF64 {
     template <class T> T tsdf(size_t i) const { return (T)f[i]; }
     int weight(size_t i) const { return w[i]; }
     template <class T> void set(size_t i, T v, int wt) { f[i] = (double)v; w[i] = wt; }
+    void reset_voxel(size_t i) { f[i] = 32767.0 / 32766.0; w[i] = 0; }
+    const void* words() const { return nullptr; }
 };
 
-// K/tsdfVolume.cl:31-190 integrateTsdfVolumeKernel (voxelWrap = 0), one
-// column per (x,y) walking z with the reference's incremental updates.
-// depth: ray-length depth (after scale_depths); bgra may be NULL.
+// Voxel wrap of a shifting volume (OCL/NuiKinfuOpenCLShiftScene.cpp:44-56 getVoxelWrap): the
+// voxel at logical (x, y, z) is stored at ((x + w0) % Nx, (y + w1) % Ny, (z + w2) % Nz)
+// (K/tsdfVolume.cl:71,104, K/raycast.cl:62,184).  w >= 0.
+struct Wrap3 {
+    int w[3] = {0, 0, 0};
+};
+inline size_t stored_index(int x, int y, int z, int Nx, int Ny, int Nz, const Wrap3& wr) {
+    return ((size_t)((y + wr.w[1]) % Ny) * Nx + (size_t)((x + wr.w[0]) % Nx)) * Nz + (size_t)((z + wr.w[2]) % Nz);
+}
+
+// K/tsdfVolume.cl:31-190 integrateTsdfVolumeKernel, one column per (x,y) walking z
+// with the reference's incremental updates.  depth: ray-length depth (after
+// scale_depths); bgra may be NULL.
 template <class T, class Vol>
 long long integrate(const T* depth, const uint8_t* bgra, Vol& vol, uint8_t* color_vol, const Tsdf<T>& p,
-                    const Cam<T>& cam, const Rigid<T>& g, int max_color_weight) {
+                    const Cam<T>& cam, const Rigid<T>& g, int max_color_weight, const Wrap3& wr = Wrap3()) {
     const int Nx = p.res[0], Ny = p.res[1], Nz = p.res[2];
     const T tr_inv = T(1) / p.trunc;
     long long nupd = 0;
@@ -504,7 +518,7 @@ long long integrate(const T* depth, const uint8_t* bgra, Vol& vol, uint8_t* colo
             T zs = 0;
             const T incx = g.R_inv[2] * p.cell[2] * cam.fx;
             const T incy = g.R_inv[5] * p.cell[2] * cam.fy;
-            const size_t base = ((size_t)y * Nx + x) * Nz;
+            const size_t base = stored_index(x, y, 0, Nx, Ny, Nz, Wrap3{{wr.w[0], wr.w[1], 0}});
             for (int z = 0; z < Nz; ++z, vgz += p.cell[2], zs += p.cell[2], vx += incx, vy += incy) {
                 const T inv_z = T(1) / (vz + g.R_inv[8] * zs);
                 if (inv_z < 0) continue;
@@ -516,7 +530,7 @@ long long integrate(const T* depth, const uint8_t* bgra, Vol& vol, uint8_t* colo
                 const T sdf = Dp - std::sqrt(vgz * vgz + part);
                 if (std::isnan(Dp) || !(sdf >= -p.trunc)) continue;
                 const T tsdf = std::min(T(1), sdf * tr_inv);
-                const size_t idx = base + z;
+                const size_t idx = base + (size_t)((z + wr.w[2]) % Nz);
                 const T prev = vol.template tsdf<T>(idx);
                 const int w = vol.weight(idx);
                 const T fnew = (prev * (T)w + T(1) * tsdf) / (T)(w + 1);
@@ -554,8 +568,11 @@ template <class T, class Vol> inline T sample(const Vol& vol, size_t idx) {
 // K/raycast.cl:36-120 interpolateTrilineary.  The reference names its eight
 // corners for an x-major layout while the volume is y-major
 // ((y*Nx + x)*Nz + z), so the x and y fractions weight the swapped corners;
-// this is reproduced as written.
-template <class T, class Vol> T trilinear(V3<T> pt, const Vol& vol, const Tsdf<T>& prm) {
+// this is reproduced as written.  In a wrapped volume every corner is wrapped
+// on its own (intended; the reference wraps the first corner only and steps to
+// the others by unwrapped strides, which leaves the cell where it straddles
+// the wrap and can read past the end of the volume).
+template <class T, class Vol> T trilinear(V3<T> pt, const Vol& vol, const Tsdf<T>& prm, const Wrap3& wr = Wrap3()) {
     int g[3];
     get_voxel(pt, prm, g);
     const int Nx = prm.res[0], Ny = prm.res[1], Nz = prm.res[2];
@@ -571,16 +588,19 @@ template <class T, class Vol> T trilinear(V3<T> pt, const Vol& vol, const Tsdf<T
     const T a = (pt.x - ((T)g[0] + T(0.5)) * prm.cell[0]) / prm.cell[0];
     const T b = (pt.y - ((T)g[1] + T(0.5)) * prm.cell[1]) / prm.cell[1];
     const T c = (pt.z - ((T)g[2] + T(0.5)) * prm.cell[2]) / prm.cell[2];
-    const long long sx = Nz, sy = (long long)Nx * Nz;
-    long long idx = ((long long)g[1] * Nx + g[0]) * Nz + g[2];
-    T f000 = sample<T>(vol, idx);             if (std::isnan(f000)) return qnan<T>();
-    T f001 = sample<T>(vol, idx + 1);         if (std::isnan(f001)) return qnan<T>();
-    T f011 = sample<T>(vol, idx + sx + 1);    if (std::isnan(f011)) return qnan<T>();
-    T f010 = sample<T>(vol, idx + sx);        if (std::isnan(f010)) return qnan<T>();
-    T f110 = sample<T>(vol, idx + sx + sy);   if (std::isnan(f110)) return qnan<T>();
-    T f111 = sample<T>(vol, idx + sx + sy + 1); if (std::isnan(f111)) return qnan<T>();
-    T f101 = sample<T>(vol, idx + sy + 1);    if (std::isnan(f101)) return qnan<T>();
-    T f100 = sample<T>(vol, idx + sy);        if (std::isnan(f100)) return qnan<T>();
+    // the corner (g0 + dx, g1 + dy, g2 + dz); the reference reaches it from idx by
+    // +1 (z), +Nz (x) and +Nx*Nz (y)
+    auto at = [&](int dx, int dy, int dz) {
+        return sample<T>(vol, stored_index(g[0] + dx, g[1] + dy, g[2] + dz, Nx, Ny, Nz, wr));
+    };
+    T f000 = at(0, 0, 0); if (std::isnan(f000)) return qnan<T>();
+    T f001 = at(0, 0, 1); if (std::isnan(f001)) return qnan<T>();
+    T f011 = at(1, 0, 1); if (std::isnan(f011)) return qnan<T>();
+    T f010 = at(1, 0, 0); if (std::isnan(f010)) return qnan<T>();
+    T f110 = at(1, 1, 0); if (std::isnan(f110)) return qnan<T>();
+    T f111 = at(1, 1, 1); if (std::isnan(f111)) return qnan<T>();
+    T f101 = at(0, 1, 1); if (std::isnan(f101)) return qnan<T>();
+    T f100 = at(0, 1, 0); if (std::isnan(f100)) return qnan<T>();
     return f000 * (1 - a) * (1 - b) * (1 - c) + f001 * (1 - a) * (1 - b) * c + f010 * (1 - a) * b * (1 - c) +
            f011 * (1 - a) * b * c + f100 * a * (1 - b) * (1 - c) + f101 * a * (1 - b) * c +
            f110 * a * b * (1 - c) + f111 * a * b * c;
@@ -599,11 +619,11 @@ template <class T> inline T max_time(const T* vmax, V3<T> o, V3<T> d) {
     return std::min(std::min(tx, ty), tz);
 }
 
-// K/raycast.cl:128-277 raycastKernel (voxelWrap = 0).  Outputs world-space
-// vmap/nmap (NaN where no surface) and, if both colour buffers are given, cmap.
+// K/raycast.cl:128-277 raycastKernel.  Outputs world-space vmap/nmap (NaN where
+// no surface) and, if both colour buffers are given, cmap.
 template <class T, class Vol>
 void raycast(const Vol& vol, const uint8_t* color_vol, const Tsdf<T>& prm, const Cam<T>& cam, const Rigid<T>& g,
-             T* vmap, T* nmap, uint8_t* cmap) {
+             T* vmap, T* nmap, uint8_t* cmap, const Wrap3& wr = Wrap3()) {
     const int W = cam.W, H = cam.H;
     const int Nx = prm.res[0], Ny = prm.res[1], Nz = prm.res[2];
 #pragma omp parallel for schedule(dynamic, 2)
@@ -630,7 +650,7 @@ void raycast(const Vol& vol, const uint8_t* color_vol, const Tsdf<T>& prm, const
             gv[0] = std::max(0, std::min(gv[0], Nx - 1));
             gv[1] = std::max(0, std::min(gv[1], Ny - 1));
             gv[2] = std::max(0, std::min(gv[2], Nz - 1));
-            size_t idx = ((size_t)gv[1] * Nx + gv[0]) * Nz + gv[2];
+            size_t idx = stored_index(gv[0], gv[1], gv[2], Nx, Ny, Nz, wr);
             T tsdf = sample<T>(vol, idx);
             const T max_t = T(3) * (prm.dim[0] + prm.dim[1] + prm.dim[2]);
             const T step = prm.trunc * T(0.8);
@@ -638,14 +658,14 @@ void raycast(const Vol& vol, const uint8_t* color_vol, const Tsdf<T>& prm, const
                 const T tprev = tsdf;
                 get_voxel(rs + dir * (tcur + step), prm, gv);
                 if (!(gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < Nx && gv[1] < Ny && gv[2] < Nz)) break;
-                idx = ((size_t)gv[1] * Nx + gv[0]) * Nz + gv[2];
+                idx = stored_index(gv[0], gv[1], gv[2], Nx, Ny, Nz, wr);
                 tsdf = sample<T>(vol, idx);
                 if (std::isnan(tsdf) || std::isnan(tprev)) continue;
                 if (tprev < 0 && tsdf > 0) break;
                 if (tprev >= 0 && tsdf <= 0) {
-                    const T Ftdt = trilinear(rs + dir * (tcur + step), vol, prm);
+                    const T Ftdt = trilinear(rs + dir * (tcur + step), vol, prm, wr);
                     if (std::isnan(Ftdt)) break;
-                    const T Ft = trilinear(rs + dir * tcur, vol, prm);
+                    const T Ft = trilinear(rs + dir * tcur, vol, prm, wr);
                     if (std::isnan(Ft)) break;
                     const T Ts = tcur - step * Ft / (Ftdt - Ft);
                     const V3<T> vf = rs + dir * Ts;
@@ -655,14 +675,14 @@ void raycast(const Vol& vol, const uint8_t* color_vol, const Tsdf<T>& prm, const
                     get_voxel(rs + dir * tcur, prm, gg);
                     if (gg[0] > 1 && gg[1] > 1 && gg[2] > 1 && gg[0] < Nx - 2 && gg[1] < Ny - 2 && gg[2] < Nz - 2) {
                         V3<T> n, q;
-                        q = vf; q.x += prm.cell[0]; T fx1 = trilinear(q, vol, prm);
-                        q = vf; q.x -= prm.cell[0]; T fx2 = trilinear(q, vol, prm);
+                        q = vf; q.x += prm.cell[0]; T fx1 = trilinear(q, vol, prm, wr);
+                        q = vf; q.x -= prm.cell[0]; T fx2 = trilinear(q, vol, prm, wr);
                         n.x = fx1 - fx2;
-                        q = vf; q.y += prm.cell[1]; T fy1 = trilinear(q, vol, prm);
-                        q = vf; q.y -= prm.cell[1]; T fy2 = trilinear(q, vol, prm);
+                        q = vf; q.y += prm.cell[1]; T fy1 = trilinear(q, vol, prm, wr);
+                        q = vf; q.y -= prm.cell[1]; T fy2 = trilinear(q, vol, prm, wr);
                         n.y = fy1 - fy2;
-                        q = vf; q.z += prm.cell[2]; T fz1 = trilinear(q, vol, prm);
-                        q = vf; q.z -= prm.cell[2]; T fz2 = trilinear(q, vol, prm);
+                        q = vf; q.z += prm.cell[2]; T fz1 = trilinear(q, vol, prm, wr);
+                        q = vf; q.z -= prm.cell[2]; T fz2 = trilinear(q, vol, prm, wr);
                         n.z = fz1 - fz2;
                         store3(nmap, id, normalize3(n));
                     }

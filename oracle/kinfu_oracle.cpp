@@ -7,6 +7,8 @@
 
 #include "kinfu_core.h"
 
+#include <type_traits>
+
 
 // ============================================================ C interface
 using namespace oracle;
@@ -141,7 +143,15 @@ struct EngineCfg {
     float trunc;           // metres (before the 2.1-cell clamp)
     float gate;            // integration metric threshold (0: integrate every frame)
     int max_color_weight;
+    int voxel_shift;       // > 0: shifting volume (VO/NuiKinfuVolumeConfig.cpp:13 voxel_shift)
+    float basis[3];        // translateBasis
 };
+
+// OCL/NuiKinfuOpenCLShiftScene.cpp:119-245 fetchSlice reads the slab with
+// K/extract.cl:80-190; the restatement lives in points_oracle.cpp.
+extern "C" long long or_points_dense_slice(const void* vol, const uint8_t* cvol, int storage, int Nx, int Ny, int Nz,
+                                           const float* cell, const int* wrap, const int* offset, const int* range,
+                                           float* pts, float* cols, long long cap);
 
 // Per-frame orchestration of src/SLAM/VisualOdometry/NuiKinfuTrackingEngine.cpp:68-142
 // with the OpenCL frame / feedback frame / depth tracker / dense scene.
@@ -159,6 +169,9 @@ template <class T, class Vol> struct Engine {
     IcpResult last{1, 0, 0, 0};
     long long nupd_last = 0;
     bool integrated_last = false;
+    // shifting volume: m_voxel_offset and the evicted points (NuiKinfuVertexCache)
+    int off[3] = {0, 0, 0};
+    std::vector<float> cache_pts, cache_cols;
 
     explicit Engine(const EngineCfg& cfg) : c(cfg) {
         cam = make_cam<T>(c.cam, c.W, c.H);
@@ -190,6 +203,75 @@ template <class T, class Vol> struct Engine {
         tlast[0] = tlast[1] = tlast[2] = 0;
     }
 
+    // OCL/NuiKinfuOpenCLShiftScene.cpp:44-56 getVoxelWrap
+    Wrap3 wrap() const {
+        Wrap3 w;
+        for (int k = 0; k < 3; ++k) w.w[k] = off[k] < 0 ? c.N - ((-off[k]) % c.N) : off[k];
+        return w;
+    }
+    // getVoxelOffsetSize: voxel offset in metres
+    T offset_size(int k) const { return (T)off[k] * prm.cell[k]; }
+
+    // OCL/NuiKinfuOpenCLShiftScene.cpp:247-287 fetchAndClear{X,Y,Z}
+    void fetch_and_clear(int axis, int trans) {
+        int o = off[axis];
+        if (trans < 0) o += trans;
+        o = o >= 0 ? o % c.N : c.N - (-o) % c.N;
+        int w[3] = {0, 0, 0}, r[3] = {c.N, c.N, c.N};
+        w[axis] = o;
+        r[axis] = std::abs(trans);
+        // fetchSlice: the slab's points into the cache (float volume of the compat engine only)
+        if (std::is_same<Vol, VolS16>::value) {
+            const float cell[3] = {(float)prm.cell[0], (float)prm.cell[1], (float)prm.cell[2]};
+            const void* vp = static_cast<const void*>(volume_words());
+            const long long n = or_points_dense_slice(vp, cvol.empty() ? nullptr : cvol.data(), 0, c.N, c.N, c.N, cell,
+                                                      w, off, r, nullptr, nullptr, 0);
+            const size_t at = cache_pts.size() / 3;
+            cache_pts.resize(3 * (at + (size_t)n));
+            cache_cols.resize(4 * (at + (size_t)n));
+            or_points_dense_slice(vp, cvol.empty() ? nullptr : cvol.data(), 0, c.N, c.N, c.N, cell, w, off, r,
+                                  cache_pts.data() + 3 * at, cache_cols.data() + 4 * at, n);
+        }
+        // clearSlice: initializeVolume over the slab (K/tsdfVolume.cl:192-211)
+        Wrap3 cw;
+        for (int k = 0; k < 3; ++k) cw.w[k] = w[k];
+        for (int y = 0; y < r[1]; ++y)
+            for (int x = 0; x < r[0]; ++x)
+                for (int z = 0; z < r[2]; ++z) {
+                    const size_t i = stored_index(x, y, z, c.N, c.N, c.N, cw);
+                    vol.reset_voxel(i);
+                    if (!cvol.empty()) std::memset(&cvol[4 * i], 0, 4);
+                }
+        off[axis] += trans;
+    }
+    const void* volume_words() const { return vol.words(); }
+
+    // OCL/NuiKinfuOpenCLShiftScene.cpp:289-351 shiftVolume, applied to the local translation t
+    void shift_volume(T* tr) {
+        T sh[3];
+        int vt[3];
+        for (int k = 0; k < 3; ++k) {
+            sh[k] = tr[k] - (T)c.basis[k];
+            const int f = (int)std::floor(sh[k] / prm.cell[k]);
+            vt[k] = sh[k] < 0 ? std::max(-c.voxel_shift, f) : std::min(c.voxel_shift, f);
+        }
+        for (int k = 0; k < 3; ++k)
+            if (std::abs(vt[k]) >= c.voxel_shift) {
+                fetch_and_clear(k, vt[k]);
+                sh[k] -= (T)vt[k] * prm.cell[k];
+            }
+        for (int k = 0; k < 3; ++k) tr[k] = sh[k] + (T)c.basis[k];
+    }
+
+    // NuiKinfuOpenCLShiftScene::integrateVolume (:353-364) around the dense integration
+    void integrate_frame(const uint8_t* bgra) {
+        if (c.voxel_shift > 0) shift_volume(t);
+        scale_depths<T>(depth.data(), c.W, c.H, cam);
+        nupd_last = integrate<T, Vol>(depth.data(), bgra, vol, bgra ? cvol.data() : nullptr, prm, cam,
+                                      make_rigid<T>(R, t), c.max_color_weight, wrap());
+        integrated_last = true;
+    }
+
     // returns 1 ok, 0 tracking failure
     int run(const uint16_t* raw, const uint8_t* bgra) {
         const int W = c.W, H = c.H;
@@ -206,10 +288,7 @@ template <class T, class Vol> struct Engine {
             std::vector<T> nm(3 * n);
             normals_simple<T>(vpyr[0].data(), nm.data(), W, H, (T)c.depth_threshold);
             transform_maps<T>(vpyr[0].data(), nm.data(), vref.data(), nref.data(), n, pose);
-            scale_depths<T>(depth.data(), W, H, cam);
-            nupd_last = integrate<T, Vol>(depth.data(), bgra, vol, bgra ? cvol.data() : nullptr, prm, cam, pose,
-                                          c.max_color_weight);
-            integrated_last = true;
+            integrate_frame(bgra);
         } else {
             // NuiKinfuOpenCLDepthTracker::EstimatePose: pyramid, vertices, ICP
             for (int l = 1; l < c.levels; ++l) {
@@ -229,23 +308,23 @@ template <class T, class Vol> struct Engine {
             if (!last.ok) return 0;
             std::memcpy(R, Rn, sizeof(Rn));
             std::memcpy(t, tn, sizeof(tn));
-            const Rigid<T> np = make_rigid<T>(R, t);
-            // integration gate (VO/NuiKinfuTrackingEngine.cpp:113-127)
+            // integration gate (VO/NuiKinfuTrackingEngine.cpp:113-127) on global translations
+            // (local + the shifting volume's offset; NuiCameraPos::getGlobalTranslation)
             T Rinv[9], Rrel[9];
             inverse3(R, Rinv);
             matmul3(Rinv, Rlast, Rrel);
             T rnorm = rodrigues_norm<T>(Rrel);
-            T d0 = t[0] - tlast[0], d1 = t[1] - tlast[1], d2 = t[2] - tlast[2];
-            T tnorm = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            T d[3];
+            for (int k = 0; k < 3; ++k) d[k] = (t[k] + offset_size(k)) - (tlast[k] + offset_size(k));
+            T tnorm = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
             if ((rnorm + tnorm) / T(2) >= (T)c.gate) {
-                scale_depths<T>(depth.data(), W, H, cam);
-                nupd_last = integrate<T, Vol>(depth.data(), bgra, vol, bgra ? cvol.data() : nullptr, prm, cam, np,
-                                              c.max_color_weight);
-                integrated_last = true;
+                integrate_frame(bgra);  // may shift the volume and move t (local)
                 std::memcpy(Rlast, R, sizeof(Rlast));
                 std::memcpy(tlast, t, sizeof(tlast));
             }
-            raycast<T, Vol>(vol, bgra ? cvol.data() : nullptr, prm, cam, np, vref.data(), nref.data(), nullptr);
+            const Rigid<T> np = make_rigid<T>(R, t);
+            raycast<T, Vol>(vol, bgra ? cvol.data() : nullptr, prm, cam, np, vref.data(), nref.data(), nullptr,
+                            wrap());
         }
         frames++;
         return 1;
@@ -284,6 +363,8 @@ OR_EXPORT void* or_engine_create(int prec, int W, int H, const float* cam6, int 
     c.trunc = trunc;
     c.gate = gate;
     c.max_color_weight = 128;
+    c.voxel_shift = 0;
+    c.basis[0] = c.basis[1] = c.basis[2] = 0.f;
     OrEngine* h = new OrEngine;
     h->prec = prec;
     if (prec == 0) {
@@ -333,6 +414,44 @@ OR_EXPORT int or_engine_run(void* hp, const uint16_t* raw, const uint8_t* bgra, 
         fill(e);
     }
     return ok;
+}
+
+// Shifting volume (OCL/NuiKinfuOpenCLShiftScene.cpp): set before the first frame.
+OR_EXPORT void or_engine_set_shift(void* hp, int voxel_shift, const float* basis) {
+    OrEngine* h = (OrEngine*)hp;
+    auto set = [&](auto& e) {
+        e.c.voxel_shift = voxel_shift;
+        for (int k = 0; k < 3; ++k) e.c.basis[k] = basis ? basis[k] : 0.f;
+    };
+    if (h->prec == 0)
+        set(((EngineBox<float, VolS16>*)h->p)->e);
+    else
+        set(((EngineBox<double, VolF64>*)h->p)->e);
+}
+// voxel offset (int[3]) and its size in metres (double[3], the engine's arithmetic type)
+OR_EXPORT void or_engine_offset(void* hp, int* off3, double* size3) {
+    OrEngine* h = (OrEngine*)hp;
+    auto rd = [&](auto& e) {
+        for (int k = 0; k < 3; ++k) {
+            if (off3) off3[k] = e.off[k];
+            if (size3) size3[k] = (double)e.offset_size(k);
+        }
+    };
+    if (h->prec == 0)
+        rd(((EngineBox<float, VolS16>*)h->p)->e);
+    else
+        rd(((EngineBox<double, VolF64>*)h->p)->e);
+}
+// the evicted points (compat engine): count, and up to cap points / colours
+OR_EXPORT long long or_engine_cache(void* hp, float* pts, float* cols, long long cap) {
+    OrEngine* h = (OrEngine*)hp;
+    if (h->prec != 0) return 0;
+    auto& e = ((EngineBox<float, VolS16>*)h->p)->e;
+    const long long n = (long long)(e.cache_pts.size() / 3);
+    const long long m = std::min(n, cap);
+    if (pts && m > 0) std::memcpy(pts, e.cache_pts.data(), (size_t)m * 3 * sizeof(float));
+    if (cols && m > 0) std::memcpy(cols, e.cache_cols.data(), (size_t)m * 4 * sizeof(float));
+    return n;
 }
 
 // Volume readback: tsdf (float, normalised) and weight per voxel in the
@@ -395,6 +514,31 @@ OR_EXPORT void or_raycast_s16_f(const int16_t* vol2, const uint8_t* color_vol, i
     v.v.assign(vol2, vol2 + 2 * (size_t)N * N * N);
     raycast<float, VolS16>(v, color_vol, make_tsdf<float>(N, dim, trunc), make_cam<float>(cam6, W, H),
                            make_rigid<float>(R9, t3), vmap, nmap, cmap);
+}
+// the same in a wrapped volume (wrap: int[3], >= 0)
+OR_EXPORT long long or_integrate_s16_wrap_f(const float* depth_scaled, const uint8_t* bgra, int16_t* vol2,
+                                            uint8_t* color_vol, int N, float dim, float trunc, const float* cam6,
+                                            int W, int H, const float* R9, const float* t3, int max_color_weight,
+                                            const int* wrap) {
+    VolS16 v;
+    v.v.assign(vol2, vol2 + 2 * (size_t)N * N * N);
+    Wrap3 wr;
+    for (int k = 0; k < 3; ++k) wr.w[k] = wrap[k];
+    long long nupd = integrate<float, VolS16>(depth_scaled, bgra, v, color_vol, make_tsdf<float>(N, dim, trunc),
+                                              make_cam<float>(cam6, W, H), make_rigid<float>(R9, t3),
+                                              max_color_weight, wr);
+    std::memcpy(vol2, v.v.data(), v.v.size() * sizeof(int16_t));
+    return nupd;
+}
+OR_EXPORT void or_raycast_s16_wrap_f(const int16_t* vol2, const uint8_t* color_vol, int N, float dim, float trunc,
+                                     const float* cam6, int W, int H, const float* R9, const float* t3, float* vmap,
+                                     float* nmap, uint8_t* cmap, const int* wrap) {
+    VolS16 v;
+    v.v.assign(vol2, vol2 + 2 * (size_t)N * N * N);
+    Wrap3 wr;
+    for (int k = 0; k < 3; ++k) wr.w[k] = wrap[k];
+    raycast<float, VolS16>(v, color_vol, make_tsdf<float>(N, dim, trunc), make_cam<float>(cam6, W, H),
+                           make_rigid<float>(R9, t3), vmap, nmap, cmap, wr);
 }
 OR_EXPORT float or_effective_trunc(int N, float dim, float trunc) { return make_tsdf<float>(N, dim, trunc).trunc; }
 

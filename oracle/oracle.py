@@ -59,6 +59,12 @@ def lib():
             "or_integrate_s16_f": (cll, [f32p, vp, i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, ci]),
             "or_raycast_s16_f": (None, [i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, f32p, f32p, vp]),
             "or_effective_trunc": (cf, [ci, cf, cf]),
+            "or_integrate_s16_wrap_f": (cll, [f32p, vp, i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, ci, i32p]),
+            "or_raycast_s16_wrap_f": (None, [i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, f32p, f32p, vp, i32p]),
+            "or_points_dense_slice": (cll, [vp, vp, ci, ci, ci, ci, f32p, i32p, i32p, i32p, vp, vp, cll]),
+            "or_engine_set_shift": (None, [vp, ci, f32p]),
+            "or_engine_offset": (None, [vp, i32p, f64p]),
+            "or_engine_cache": (cll, [vp, vp, vp, cll]),
             "or_mc_count_s16": (cll, [i16p, ci]),
             "or_mc_emit_s16": (cll, [i16p, vp, ci, cf, f32p, vp, cll]),
             "or_mc_mesh_dedup_s16": (cll, [i16p, ci, cf, f32p, cll, i32p, cll, C.POINTER(cll)]),
@@ -136,6 +142,25 @@ class OracleEngine:
         out = np.empty(2 * n, np.int16)
         lib().or_engine_volume_raw(self.h, out)
         return out
+
+    def set_shift(self, voxel_shift, basis=(0.0, 0.0, 0.0)):
+        """Shifting volume (OCL/NuiKinfuOpenCLShiftScene.cpp); before the first frame."""
+        lib().or_engine_set_shift(self.h, voxel_shift, np.asarray(basis, np.float32))
+
+    def offset(self):
+        """(voxel offset int[3], its size in metres float64[3])."""
+        o = np.zeros(3, np.int32)
+        s = np.zeros(3, np.float64)
+        lib().or_engine_offset(self.h, o, s)
+        return o, s
+
+    def cache(self):
+        """The points evicted by the shifts so far: (points float32[n,3], colours float32[n,4])."""
+        n = lib().or_engine_cache(self.h, None, None, 0)
+        p = np.empty((max(n, 1), 3), np.float32)
+        c = np.empty((max(n, 1), 4), np.float32)
+        lib().or_engine_cache(self.h, p.ctypes.data, c.ctypes.data, n)
+        return p[:n], c[:n]
 
     def refmaps(self):
         n = self.wl.width * self.wl.height * 3
@@ -436,3 +461,18 @@ def points_hash(pos, ptr, bricks, colors, storage, voxel_size, trunc_scale, cap=
     cols = np.empty((max(n, 1), 4), np.float32)
     lib().or_points_hash(*args, pts.ctypes.data, cols.ctypes.data, n)
     return pts[:n], cols[:n], total
+
+
+def points_dense_slice(words, color_volume, storage, res, cell, wrap, offset, rng):
+    """fetchShiftVolumeKernel (K/extract.cl:80-190): (points, colours, total) in (y, x, z) order."""
+    words = np.ascontiguousarray(words)
+    cv = None if color_volume is None else np.ascontiguousarray(color_volume, np.uint8)
+    Nx, Ny, Nz = (int(r) for r in res)
+    args = (words.ctypes.data, None if cv is None else cv.ctypes.data, storage, Nx, Ny, Nz,
+            np.asarray(cell, np.float32), np.asarray(wrap, np.int32), np.asarray(offset, np.int32),
+            np.asarray(rng, np.int32))
+    n = lib().or_points_dense_slice(*args, None, None, 0)
+    pts = np.empty((max(n, 1), 3), np.float32)
+    cols = np.empty((max(n, 1), 4), np.float32)
+    lib().or_points_dense_slice(*args, pts.ctypes.data, cols.ctypes.data, n)
+    return pts[:n], cols[:n], n

@@ -5,7 +5,9 @@
 //          resolution y) columns with range_z = resolution z;
 //   hash   K/KinfuHashVisualization.cl:5-52 fetchHashScene_kernel, launched by
 //          OCL/NuiKinfuOpenCLHashScene.cpp:1066-1190 Volume2CLVertices over the
-//          visible list with truncScale = m_truncScale.
+//          visible list with truncScale = m_truncScale;
+//   shift  K/extract.cl:80-190 fetchShiftVolumeKernel, the evicted slab of the
+//          shifting volume (OCL/NuiKinfuOpenCLShiftScene.cpp:119-245).
 // The reference appends through an atomic counter (its order is a race); the
 // oracle writes in the order the device extraction defines: columns in (y, x)
 // order with z ascending, and visible-list order with the voxel index ascending.
@@ -161,6 +163,80 @@ OR_EXPORT long long or_points_hash(int nvis, const int32_t* pos, const int32_t* 
                 c[2] = (float)q[2] / 255.f;
             }
             put(pts, cols, n++, cap, p, c);
+        }
+    }
+    return n;
+}
+
+// K/extract.cl:80-190 fetchShiftVolumeKernel over range[0] x range[1] work-items with
+// range_z = range[2], voxelWrap = wrap (>= 0) and voxelOffset = offset, restated as
+// written: the walk runs voxel_z = wrap.z .. wrap.z + range_z - 1 and reads stored z
+// (voxel_z + wrap.z) % Nz (wrap.z added twice), and the position of a voxel adds the
+// multiple of N that the reference derives from the offset and the work-item index.
+OR_EXPORT long long or_points_dense_slice(const void* vol, const uint8_t* cvol, int storage, int Nx, int Ny, int Nz,
+                                          const float* cell, const int* wrap, const int* offset, const int* range,
+                                          float* pts, float* cols, long long cap) {
+    const float half_x = (float)(Nx >> 1), half_y = (float)(Ny >> 1);
+    long long n = 0;
+    for (int voxel_y = 0; voxel_y < range[1]; ++voxel_y) {
+        for (int voxel_x = 0; voxel_x < range[0]; ++voxel_x) {
+            const size_t voxel_id = ((size_t)((voxel_y + wrap[1]) % Ny) * Nx + (size_t)((voxel_x + wrap[0]) % Nx)) * Nz;
+            int ob[3] = {(offset[0] / Nx) * Nx, (offset[1] / Ny) * Ny, (offset[2] / Nz) * Nz};
+            if (offset[0] > 0) {
+                if (voxel_x < offset[0] % Nx) ob[0] += Nx;
+            } else {
+                if (voxel_x > Nx + offset[0] % Nx) ob[0] -= Nx;
+            }
+            if (offset[1] > 0) {
+                if (voxel_y < offset[1] % Ny) ob[1] += Ny;
+            } else {
+                if (voxel_y > Ny + offset[1] % Ny) ob[1] -= Ny;
+            }
+            const int range_z = range[2] + wrap[2];
+            float tsdf = NAN;
+            for (int voxel_z = wrap[2]; voxel_z < range_z; ++voxel_z) {
+                const float tsdf_prev = tsdf;
+                const size_t idx = voxel_id + (size_t)((voxel_z + wrap[2]) % Nz);
+                const Vox v = dense_voxel(vol, storage, idx);
+                if (!v.valid) {
+                    tsdf = NAN;
+                    continue;
+                }
+                int oz = ob[2];
+                if (offset[2] > 0) {
+                    if (voxel_z < offset[2] % Nz) oz += Nz;
+                } else {
+                    if (voxel_z > Nz + offset[2] % Nz) oz -= Nz;
+                }
+                tsdf = v.t;
+                const float px = ((float)(voxel_x + ob[0]) + 0.5f - half_x) * cell[0];
+                const float py = (-(float)(voxel_y + ob[1]) - 0.5f + half_y) * cell[1];
+                if (0.f == tsdf) {
+                    const float p[3] = {px, py, ((float)(voxel_z + oz) + 0.5f) * cell[2]};
+                    float c[4] = {0.f, 0.f, 0.f, 1.f};
+                    if (cvol) {
+                        const uint8_t* q = cvol + 4 * idx;
+                        c[0] = (float)q[0] / 255.f;
+                        c[1] = (float)q[1] / 255.f;
+                        c[2] = (float)q[2] / 255.f;
+                    }
+                    put(pts, cols, n++, cap, p, c);
+                } else if ((tsdf_prev < 0.f && tsdf > 0.f) || (tsdf_prev > 0.f && tsdf < 0.f)) {
+                    const float tsdf_inv = 1.f / std::fabs(tsdf);
+                    const float tsdf_prev_inv = 1.f / std::fabs(tsdf_prev);
+                    const float dp = (tsdf_inv * (float)(voxel_z + oz) + tsdf_prev_inv * (float)(voxel_z + oz - 1)) /
+                                     (tsdf_inv + tsdf_prev_inv);
+                    const float p[3] = {px, py, (dp + 0.5f) * cell[2]};
+                    float c[4] = {0.f, 0.f, 0.f, 1.f};
+                    if (cvol) {
+                        const uint8_t* q = cvol + 4 * idx;
+                        c[0] = (float)q[2] / 255.f;
+                        c[1] = (float)q[1] / 255.f;
+                        c[2] = (float)q[0] / 255.f;
+                    }
+                    put(pts, cols, n++, cap, p, c);
+                }
+            }
         }
     }
     return n;
