@@ -265,7 +265,7 @@ int nuif_tsdf_reset_slice(void* volume, uint8_t* color_volume, const TsdfParams*
  * and the shifting scene's Volume2CLVertices, :366-): range[0] x range[1] columns from stored
  * column (x + wrap.x, y + wrap.y), walking voxel_z = wrap.z .. wrap.z + range[2] - 1 at stored
  * z (voxel_z + wrap.z) % Nz as the reference does, points placed by the volume's voxel offset
- * `offset` (int32[3]) as the reference computes it.  wrap in [0, N); output as
+ * `offset` (int32[3]) as the reference computes it.  wrap in [0, N]; output as
  * nuif_points_dense, in (y, x, z) order; scratch: nuif_points_dense_scratch_bytes. */
 int nuif_points_dense_slice(const void* volume, const uint8_t* color_volume, const TsdfParams* params, int storage,
                             const int32_t* wrap, const int32_t* offset, const int32_t* range, float* points,

@@ -43,7 +43,7 @@ __device__ __forceinline__ float observed_tsdf(typename S::word w) {
 // w.z twice), and the points are placed at voxel coordinates shifted by the volume offset o
 // as the reference computes it (the global position of the stored voxel).
 struct SliceArgs {
-    VoxelWrap w;     // as passed, 0 <= w < N
+    VoxelWrap w;     // as passed, 0 <= w <= N
     int ox, oy, oz;  // the volume's voxel offset
     int rx, ry, rz;  // work-item grid and walk length
 };
@@ -206,7 +206,7 @@ int nuif_points_dense_slice(const void* volume, const uint8_t* color_volume, con
     for (int k = 0; k < 3; ++k) {
         NUIF_CHECK_ARG(p->resolution[k] >= 1.f && p->resolution[k] <= 65536.f, "nuif_points_dense_slice: bad resolution");
         N[k] = (int)p->resolution[k];
-        NUIF_CHECK_ARG(wrap[k] >= 0 && wrap[k] < N[k], "nuif_points_dense_slice: wrap outside [0, N)");
+        NUIF_CHECK_ARG(wrap[k] >= 0 && wrap[k] <= N[k], "nuif_points_dense_slice: wrap outside [0, N]");
         NUIF_CHECK_ARG(range[k] >= 0 && range[k] <= N[k], "nuif_points_dense_slice: range outside [0, N]");
     }
     const SliceArgs a{{wrap[0], wrap[1], wrap[2]}, offset[0], offset[1], offset[2], range[0], range[1], range[2]};
