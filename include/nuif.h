@@ -200,6 +200,12 @@ int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* ca
 #define NUIF_POSE_ROW 16
 int nuif_icp_end(void* state, NuiCLRigidTransform* pose, float gate_threshold, float* history, int capacity,
                  nuif_stream_t s);
+/* Shifting volume (OCL/NuiKinfuOpenCLShiftScene.cpp:353-364): after a shift of the local frame,
+ * t3 (host float[3]) becomes the pose translation, the last integration translation and the
+ * translation of the last history row; offset3 (host float[3], metres) is added to both
+ * translations of the next integration gates (global = local + offset). */
+int nuif_icp_set_translation(void* state, NuiCLRigidTransform* pose, float* history, int capacity, const float* t3,
+                             const float* offset3, nuif_stream_t s);
 /* All levels and iterations of one frame, no host synchronisation. */
 int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
                       const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,

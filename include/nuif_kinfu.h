@@ -51,10 +51,16 @@ typedef struct nuifk_desc {
     int hash_weight_max;
     int hash_bricks;                /* brick pool (m_numSDFBlocks) */
     int hash_forward_render;        /* NUIFK_RENDER_*: how the hash scene renders the ICP maps */
+    /* shifting volume (VO/NuiKinfuVolumeConfig.cpp:13-14): shift threshold in voxels (> 0) and
+     * translateBasis (metres) */
+    int voxel_shift;
+    float volume_basis[3];
 } nuifk_desc;
 
 #define NUIFK_SCENE_DENSE 0         /* OCL/NuiKinfuOpenCLScene: dense TSDF volume */
 #define NUIFK_SCENE_HASH 1          /* OCL/NuiKinfuOpenCLHashScene: voxel hashing */
+#define NUIFK_SCENE_SHIFT 2         /* OCL/NuiKinfuOpenCLShiftScene: dense volume that follows the camera
+                                       (no graph replay: each integrated frame waits for its pose) */
 #define NUIFK_RENDER_CASTRAY 0      /* castRay at every pixel (m_bUseForwardRender false) */
 #define NUIFK_RENDER_FORWARD 1      /* ForwardRender, world->camera projection */
 #define NUIFK_RENDER_FORWARD_REFERENCE 2 /* ForwardRender with the reference's camera->world projection */
@@ -70,13 +76,19 @@ int nuifk_destroy(void* h);
 int nuifk_run(void* h, const uint16_t* depth, int on_device, const uint8_t* bgra, int sync);
 int nuifk_synchronize(void* h);                    /* 1: no failure so far, 0: a failure */
 int nuifk_reset(void* h);
-/* Pose history rows (NUIF_POSE_ROW floats) of the first n frames; returns the count. */
+/* Pose history rows (NUIF_POSE_ROW floats) of the first n frames; returns the count.  The
+ * translation is global (local + the shifting volume's offset, NuiCameraPos::getGlobalTranslation). */
 int nuifk_poses(void* h, float* out, int n);
 int nuifk_state(void* h, nuif_icp_result* out);
 /* Copies the scene's device allocation: the dense volume, or the whole hash scene
  * (layout of nuif_hash_layout); nuifk_volume_bytes gives its size. */
 int nuifk_read_volume(void* h, void* out, size_t bytes);
 int nuifk_read_maps(void* h, float* vmap, float* nmap);
+/* Shifting volume: voxel offset (int32[3]) and its size in metres (float[3]). */
+int nuifk_shift_state(void* h, int32_t* offset, float* offset_size);
+/* Shifting volume: points of the evicted slabs (NuiKinfuVertexCache); returns their number
+ * and copies up to cap points (float3) and colours (float4, may be NULL). */
+int nuifk_shift_cache(void* h, float* points, float* colors, size_t cap);
 /* VO/NuiTrackerConfig.cpp:26-115 key=value files; -1 if the file cannot be opened. */
 int nuifk_load_tracker_config(const char* path, nuifk_desc* d);
 int nuifk_save_tracker_config(const char* path, const nuifk_desc* d);

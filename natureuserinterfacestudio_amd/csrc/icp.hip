@@ -55,6 +55,8 @@ struct IcpState {
     double sums[kTerms];   // sums of the last executed iteration (both modes)
     float goodf[27];       // reference mode: goodCores as the reference keeps them (float)
     float pad2;
+    float off[3];          // shifting volume: offset of the local frame (metres), 0 otherwise
+    float pad3;
 };
 static_assert(sizeof(IcpState) % 8 == 0, "state is copied as 32-bit words");
 
@@ -557,6 +559,7 @@ __global__ void icp_state_init_kernel(IcpState* st, const NuiCLRigidTransform* p
     if (threadIdx.x != 0) return;
     for (int k = 0; k < 9; ++k) st->R[k] = st->Rg[k] = st->Rlast[k] = pose->R[k];
     for (int k = 0; k < 3; ++k) st->t[k] = st->tg[k] = st->tlast[k] = pose->t[k];
+    for (int k = 0; k < 3; ++k) st->off[k] = 0.f;
     for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
     st->failed = 0;
     st->iterations = 0;
@@ -613,7 +616,10 @@ __global__ void icp_end_kernel(IcpState* st, NuiCLRigidTransform* pose, float ga
                 Rrel[3 * i + j] = Ri[3 * i] * st->Rlast[j] + Ri[3 * i + 1] * st->Rlast[3 + j] + Ri[3 * i + 2] * st->Rlast[6 + j];
     }
     const float rnorm = rodrigues_norm(Rrel);
-    const float d0 = st->t[0] - st->tlast[0], d1 = st->t[1] - st->tlast[1], d2 = st->t[2] - st->tlast[2];
+    // global translations (local + the shifting volume's offset; NuiCameraPos::getGlobalTranslation)
+    const float d0 = (st->t[0] + st->off[0]) - (st->tlast[0] + st->off[0]);
+    const float d1 = (st->t[1] + st->off[1]) - (st->tlast[1] + st->off[1]);
+    const float d2 = (st->t[2] + st->off[2]) - (st->tlast[2] + st->off[2]);
     const float tnorm = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
     const int gate = (rnorm + 1.f * tnorm) / 2 >= gate_threshold;
     st->gate = gate;
@@ -624,11 +630,40 @@ __global__ void icp_end_kernel(IcpState* st, NuiCLRigidTransform* pose, float ga
     }
 }
 
+// A shifting volume moved the local frame after this frame's tracking
+// (OCL/NuiKinfuOpenCLShiftScene.cpp:353-364): the new local translation becomes the
+// camera pose, the last integration pose (taken after the shift, as
+// VO/NuiKinfuTrackingEngine.cpp:126 does) and the frame's history row; `off` is the
+// frame's offset for the following integration gates.
+__global__ void icp_set_translation_kernel(IcpState* st, NuiCLRigidTransform* pose, float* history, int capacity,
+                                           float tx, float ty, float tz, float ox, float oy, float oz) {
+    const float t[3] = {tx, ty, tz}, o[3] = {ox, oy, oz};
+    for (int k = 0; k < 3; ++k) {
+        st->t[k] = st->tlast[k] = pose->t[k] = t[k];
+        st->off[k] = o[k];
+    }
+    if (history && st->frame > 0) {
+        float* row = history + (size_t)((st->frame - 1) % capacity) * NUIF_POSE_ROW;
+        for (int k = 0; k < 3; ++k) row[9 + k] = t[k];
+    }
+}
+
 }  // namespace nuif
 
 using namespace nuif;
 
 extern "C" {
+
+int nuif_icp_set_translation(void* state, NuiCLRigidTransform* pose, float* history, int capacity, const float* t3,
+                             const float* offset3, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && pose && t3 && offset3, "nuif_icp_set_translation: null argument");
+    NUIF_CHECK_ARG(!history || capacity > 0, "nuif_icp_set_translation: bad history capacity");
+    icp_set_translation_kernel<<<1, 1, 0, stream_of(s)>>>((IcpState*)state, pose, history, capacity, t3[0], t3[1],
+                                                         t3[2], offset3[0], offset3[1], offset3[2]);
+    NUIF_LAUNCHED("icp_set_translation_kernel");
+    return NUIF_OK;
+}
 
 size_t nuif_icp_state_bytes(void) { return kStateBytes + kSyncBytes; }
 

@@ -92,6 +92,9 @@ bool NuiKinfuVolumeConfig::load(const std::string& fileName) {
         else if (key == "MaxColorWeight") max_color_weight = std::atoi(val);
         else if (key == "IsDynamicVolume") bIsDynamic = std::atoi(val) != 0;
         else if (key == "DynamicVoxelShift") voxel_shift = std::atoi(val);
+        else if (key == "VolumeBasisX") translateBasis[0] = (float)std::atof(val);
+        else if (key == "VolumeBasisY") translateBasis[1] = (float)std::atof(val);
+        else if (key == "VolumeBasisZ") translateBasis[2] = (float)std::atof(val);
     }
     return true;
 }
@@ -105,7 +108,8 @@ bool NuiKinfuVolumeConfig::log(const std::string& fileName) const {
         << "\nVolumeResolutionZ=" << resolution[2] << "\n";
     out << "VolumeTrancDist=" << tranc_dist << "\nVolumeHadColor=" << (bHas_color_volume ? 1 : 0)
         << "\nMaxColorWeight=" << max_color_weight << "\nIsDynamicVolume=" << (bIsDynamic ? 1 : 0)
-        << "\nDynamicVoxelShift=" << voxel_shift << "\n";
+        << "\nDynamicVoxelShift=" << voxel_shift << "\nVolumeBasisX=" << translateBasis[0]
+        << "\nVolumeBasisY=" << translateBasis[1] << "\nVolumeBasisZ=" << translateBasis[2] << "\n";
     return true;
 }
 

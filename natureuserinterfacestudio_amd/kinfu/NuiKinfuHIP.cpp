@@ -200,6 +200,12 @@ void NuiKinfuHIPDepthTracker::ResetState(NuiKinfuHIPCameraState* cam) {
     check(nuif_icp_state_init(m_state.get(), cam->GetCameraTransformBuffer(), m_stream), "nuif_icp_state_init");
 }
 
+void NuiKinfuHIPDepthTracker::SetTranslation(NuiKinfuHIPCameraState* cam, const float t[3], const float offset[3]) {
+    check(nuif_icp_set_translation(m_state.get(), cam->GetCameraTransformBuffer(), m_history, m_historyCap, t, offset,
+                                   m_stream),
+          "nuif_icp_set_translation");
+}
+
 void NuiKinfuHIPDepthTracker::RecordPose(NuiKinfuHIPCameraState* cam) {
     check(nuif_icp_begin(m_state.get(), cam->GetCameraTransformBuffer(), m_stream), "nuif_icp_begin");
     // a frame without tracking: the pose is unchanged and the frame always integrates
@@ -283,10 +289,11 @@ bool NuiKinfuHIPScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraStat
     const uint8_t* colors = m_config.bHas_color_volume ? f->GetColorBuffer() : nullptr;
     // the frame's depth bounds exist when it was preprocessed for this scene
     const float* bounds = f->IsDepthScaled() ? f->GetDepthBounds() : nullptr;
-    check(nuif_tsdf_integrate(f->GetDepthBuffer(), colors, m_volume.get(), colors ? m_color.as<uint8_t>() : nullptr,
-                              &m_params, &c, cam->GetCameraTransformBuffer(), m_storage, m_config.max_color_weight,
-                              m_gateIntegrate, m_counter.as<unsigned long long>(), bounds, m_surface.as<uint8_t>(),
-                              m_stream),
+    check(nuif_tsdf_integrate_wrapped(f->GetDepthBuffer(), colors, m_volume.get(),
+                                      colors ? m_color.as<uint8_t>() : nullptr, &m_params, &c,
+                                      cam->GetCameraTransformBuffer(), m_storage, m_config.max_color_weight, m_wrap,
+                                      m_gateIntegrate, m_counter.as<unsigned long long>(), bounds,
+                                      m_surface.as<uint8_t>(), m_stream),
           "nuif_tsdf_integrate");
     setDirty();
     return true;
@@ -298,11 +305,147 @@ void NuiKinfuHIPScene::raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiK
     auto* cam = dynamic_cast<NuiKinfuHIPCameraState*>(pCameraState);
     if (!fb || !cam) return;
     const NuiCLCameraParams& c = cam->GetCameraParams();
-    check(nuif_tsdf_raycast_gated(m_volume.get(), m_color.as<uint8_t>(), &m_params, &c,
-                                  cam->GetCameraTransformBuffer(), fb->GetVertexBuffer(), fb->GetNormalBuffer(),
-                                  m_color.get() ? fb->GetColorBuffer() : nullptr, m_storage, m_gateRaycast,
-                                  m_surface.as<uint8_t>(), m_stream),
+    check(nuif_tsdf_raycast_wrapped(m_volume.get(), m_color.as<uint8_t>(), &m_params, &c,
+                                    cam->GetCameraTransformBuffer(), fb->GetVertexBuffer(), fb->GetNormalBuffer(),
+                                    m_color.get() ? fb->GetColorBuffer() : nullptr, m_storage, m_wrap, m_gateRaycast,
+                                    m_surface.as<uint8_t>(), m_stream),
           "nuif_tsdf_raycast");
+}
+
+// ------------------------------------------------------------ shifting volume
+
+NuiKinfuHIPShiftScene::NuiKinfuHIPShiftScene(const NuiKinfuVolumeConfig& config, int storage, nuif_stream_t s)
+    : NuiKinfuHIPScene(config, storage, s) {
+    m_scratch = DeviceBuffer(nuif_points_dense_scratch_bytes(&m_params));
+}
+
+// OCL/NuiKinfuOpenCLShiftScene.cpp:35-41
+void NuiKinfuHIPShiftScene::reset() {
+    for (int k = 0; k < 3; ++k) m_offset[k] = 0;
+    updateWrap();
+    m_cachePos.clear();
+    m_cacheCol.clear();
+    NuiKinfuHIPScene::reset();
+}
+
+// :44-56 getVoxelWrap
+void NuiKinfuHIPShiftScene::updateWrap() {
+    for (int k = 0; k < 3; ++k) {
+        const int n = m_config.resolution[k];
+        m_wrap[k] = m_offset[k] < 0 ? n - ((-m_offset[k]) % n) : m_offset[k];
+    }
+}
+
+void NuiKinfuHIPShiftScene::OffsetSize(float out[3]) const {
+    for (int k = 0; k < 3; ++k) out[k] = (float)m_offset[k] * m_params.cell_size[k];
+}
+
+// :247-287 fetchAndClear{X,Y,Z}: fetchSlice (:119-245) then clearSlice (:72-117)
+void NuiKinfuHIPShiftScene::fetchAndClear(int axis, int trans) {
+    const int n = m_config.resolution[axis];
+    int o = m_offset[axis];
+    if (trans < 0) o += trans;
+    o = o >= 0 ? o % n : n - (-o) % n;
+    int32_t wrap[3] = {0, 0, 0};
+    int32_t range[3] = {m_config.resolution[0], m_config.resolution[1], m_config.resolution[2]};
+    wrap[axis] = o;
+    range[axis] = std::abs(trans);
+    DeviceBuffer total(sizeof(uint32_t));
+    const uint8_t* cv = m_color.as<uint8_t>();
+    check(nuif_points_dense_slice(m_volume.get(), cv, &m_params, m_storage, wrap, m_offset, range, nullptr, nullptr, 0,
+                                  total.as<uint32_t>(), m_scratch.get(), m_stream),
+          "nuif_points_dense_slice");
+    uint32_t cnt = 0;
+    check(nuif_memcpy_d2h(&cnt, total.get(), sizeof(cnt), m_stream), "slab count");
+    check(nuif_stream_synchronize(m_stream), "slab count");
+    if (cnt) {
+        DeviceBuffer pts((size_t)cnt * 3 * sizeof(float)), cols((size_t)cnt * 4 * sizeof(float));
+        check(nuif_points_dense_slice(m_volume.get(), cv, &m_params, m_storage, wrap, m_offset, range,
+                                      pts.as<float>(), cols.as<float>(), cnt, total.as<uint32_t>(), m_scratch.get(),
+                                      m_stream),
+              "nuif_points_dense_slice");
+        const size_t at = m_cachePos.size() / 3;
+        m_cachePos.resize((at + cnt) * 3);
+        m_cacheCol.resize((at + cnt) * 4);
+        check(nuif_memcpy_d2h(m_cachePos.data() + at * 3, pts.get(), pts.bytes(), m_stream), "slab points");
+        check(nuif_memcpy_d2h(m_cacheCol.data() + at * 4, cols.get(), cols.bytes(), m_stream), "slab colours");
+        check(nuif_stream_synchronize(m_stream), "slab points");
+    }
+    check(nuif_tsdf_reset_slice(m_volume.get(), m_color.as<uint8_t>(), &m_params, m_storage, wrap, range, m_stream),
+          "nuif_tsdf_reset_slice");
+    m_offset[axis] += trans;
+}
+
+// :289-351 shiftVolume on the local translation t (float, as the reference computes it)
+void NuiKinfuHIPShiftScene::shiftVolume(float t[3]) {
+    float sh[3];
+    int vt[3];
+    for (int k = 0; k < 3; ++k) {
+        sh[k] = t[k] - m_config.translateBasis[k];
+        const int f = (int)std::floor(sh[k] / m_params.cell_size[k]);
+        vt[k] = sh[k] < 0 ? std::max(-m_config.voxel_shift, f) : std::min(m_config.voxel_shift, f);
+    }
+    for (int k = 0; k < 3; ++k)
+        if (std::abs(vt[k]) >= m_config.voxel_shift) {
+            fetchAndClear(k, vt[k]);
+            sh[k] -= (float)vt[k] * m_params.cell_size[k];
+        }
+    for (int k = 0; k < 3; ++k) t[k] = sh[k] + m_config.translateBasis[k];
+}
+
+// :353-364 integrateVolume: shift on the tracked pose, then the dense integration
+bool NuiKinfuHIPShiftScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) {
+    auto* cam = dynamic_cast<NuiKinfuHIPCameraState*>(pCameraState);
+    if (!cam) return false;
+    if (m_gateIntegrate) {  // the frame integrates only when the device gate says so
+        int32_t gate = 0;
+        check(nuif_memcpy_d2h(&gate, m_gateIntegrate, sizeof(gate), m_stream), "gate read");
+        check(nuif_stream_synchronize(m_stream), "gate read");
+        if (!gate) return true;
+    }
+    cam->SyncFromDevice();
+    const NuiCameraPos& pos = cam->GetCameraPos();
+    float t[3] = {pos.getLocalTranslation()[0], pos.getLocalTranslation()[1], pos.getLocalTranslation()[2]};
+    const int before[3] = {m_offset[0], m_offset[1], m_offset[2]};
+    const float t0[3] = {t[0], t[1], t[2]};
+    shiftVolume(t);
+    updateWrap();
+    float off[3];
+    OffsetSize(off);
+    const bool moved = std::memcmp(before, m_offset, sizeof(before)) != 0 || std::memcmp(t0, t, sizeof(t0)) != 0;
+    if (moved && m_tracker) m_tracker->SetTranslation(cam, t, off);
+    cam->NuiKinfuCameraState::UpdateCameraTransform(pos.getRotation(), Vector3f(t[0], t[1], t[2]),
+                                                    Vector3f(off[0], off[1], off[2]));
+    return NuiKinfuHIPScene::integrateVolume(pFrame, pCameraState);
+}
+
+// :366-450 Volume2CLVertices: fetchShiftVolumeKernel over the whole volume, wrap 0,
+// placed by the voxel offset
+bool NuiKinfuHIPShiftScene::Volume2CLVertices(NuiKinfuMesh* pData) {
+    if (!pData || !m_dirty) return false;
+    clearDirty();
+    const int32_t wrap[3] = {0, 0, 0};
+    const int32_t range[3] = {m_config.resolution[0], m_config.resolution[1], m_config.resolution[2]};
+    DeviceBuffer total(sizeof(uint32_t));
+    uint32_t n = 0;
+    check(nuif_points_dense_slice(m_volume.get(), m_color.as<uint8_t>(), &m_params, m_storage, wrap, m_offset, range,
+                                  nullptr, nullptr, 0, total.as<uint32_t>(), m_scratch.get(), m_stream),
+          "nuif_points_dense_slice");
+    check(nuif_memcpy_d2h(&n, total.get(), sizeof(n), m_stream), "point count");
+    check(nuif_stream_synchronize(m_stream), "point count");
+    pData->positions.assign((size_t)n * 3, 0.f);
+    pData->colors.assign((size_t)n * 4, 0.f);
+    pData->indices.clear();
+    if (n == 0) return false;
+    DeviceBuffer pts((size_t)n * 3 * sizeof(float)), cols((size_t)n * 4 * sizeof(float));
+    check(nuif_points_dense_slice(m_volume.get(), m_color.as<uint8_t>(), &m_params, m_storage, wrap, m_offset, range,
+                                  pts.as<float>(), cols.as<float>(), n, total.as<uint32_t>(), m_scratch.get(),
+                                  m_stream),
+          "nuif_points_dense_slice");
+    check(nuif_memcpy_d2h(pData->positions.data(), pts.get(), pts.bytes(), m_stream), "points");
+    check(nuif_memcpy_d2h(pData->colors.data(), cols.get(), cols.bytes(), m_stream), "points");
+    check(nuif_stream_synchronize(m_stream), "points");
+    return true;
 }
 
 }  // namespace nuif_kinfu

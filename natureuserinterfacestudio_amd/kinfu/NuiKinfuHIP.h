@@ -140,6 +140,8 @@ public:
     void SetReduction(int mode) { m_icfg.reduction = mode; }
     void SetHistory(float* dHistory, int capacity) { m_history = dHistory; m_historyCap = capacity; }
     nuif_icp_result ReadState() const;
+    // a shifting volume moved the local frame: new local translation and frame offset
+    void SetTranslation(NuiKinfuHIPCameraState* cam, const float t[3], const float offset[3]);
 
 private:
     NuiTrackerConfig m_cfg;
@@ -234,13 +236,46 @@ public:
     size_t VolumeBytes() const { return m_volume.bytes(); }
     unsigned long long UpdatedVoxels() const override;
 
-private:
+protected:
     NuiKinfuVolumeConfig m_config;
     int m_storage;
     nuif_stream_t m_stream;
     TsdfParams m_params{};
     DeviceBuffer m_volume, m_color, m_counter;
     DeviceBuffer m_surface;  // surface blocks of the volume (nuif_tsdf_surface_blocks_bytes)
+    int32_t m_wrap[3] = {0, 0, 0};  // voxel wrap (non-zero only in a shifting volume)
+};
+
+// OCL/NuiKinfuOpenCLShiftScene.*: the dense volume as a cyclic buffer that follows the
+// camera.  Before an integration the camera's local translation is compared with the
+// volume basis; a move of voxel_shift voxels or more along an axis evicts the slab the
+// volume leaves (its points go to the vertex cache), clears it, and advances the voxel
+// offset; integration and raycast then address the volume through the wrap.  The
+// decision is the reference's host-side one: an integrated frame waits for its tracked
+// pose (no graph capture).
+class NuiKinfuHIPShiftScene : public NuiKinfuHIPScene {
+public:
+    NuiKinfuHIPShiftScene(const NuiKinfuVolumeConfig& config, int storage, nuif_stream_t s);
+    void reset() override;
+    bool integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) override;
+    // the whole volume placed at its offset (the reference's fetchShiftVolumeKernel with wrap 0)
+    bool Volume2CLVertices(NuiKinfuMesh* pData) override;
+    void SetTracker(NuiKinfuHIPDepthTracker* tracker) { m_tracker = tracker; }
+    const int* VoxelOffset() const { return m_offset; }
+    // getVoxelOffsetSize (metres)
+    void OffsetSize(float out[3]) const;
+    // NuiKinfuVertexCache: points of the evicted slabs
+    const std::vector<float>& CachedPositions() const { return m_cachePos; }
+    const std::vector<float>& CachedColors() const { return m_cacheCol; }
+
+private:
+    void fetchAndClear(int axis, int trans);
+    void shiftVolume(float t[3]);
+    void updateWrap();
+    int m_offset[3] = {0, 0, 0};
+    NuiKinfuHIPDepthTracker* m_tracker = nullptr;
+    std::vector<float> m_cachePos, m_cacheCol;
+    DeviceBuffer m_scratch;
 };
 
 // OCL/NuiKinfuOpenCLHashScene.*: voxel-hashing scene.  The render is castRay over

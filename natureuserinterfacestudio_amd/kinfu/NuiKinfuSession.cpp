@@ -35,18 +35,36 @@ std::unique_ptr<NuiKinfuSession> NuiKinfuSession::create(const nuifk_desc& d) {
     s->engine->tracker()->SetReduction(d.icp_reduction);
     if (d.hash_forward_render < NUIFK_RENDER_CASTRAY || d.hash_forward_render > NUIFK_RENDER_FORWARD_REFERENCE)
         throw std::runtime_error("session: unknown hash_forward_render mode");
-    if (d.scene == NUIFK_SCENE_HASH)
+    if (d.scene == NUIFK_SCENE_HASH) {
         s->scene = std::make_unique<NuiKinfuHIPHashScene>(hashConfig(d), d.storage, d.color != 0, s->engine->stream());
-    else if (d.scene == NUIFK_SCENE_DENSE)
+    } else if (d.scene == NUIFK_SCENE_DENSE) {
         s->scene = std::make_unique<NuiKinfuHIPScene>(vc, d.storage, s->engine->stream());
-    else
+    } else if (d.scene == NUIFK_SCENE_SHIFT) {
+        if (d.voxel_shift <= 0) throw std::runtime_error("session: the shifting volume needs voxel_shift > 0");
+        vc.bIsDynamic = true;
+        vc.voxel_shift = d.voxel_shift;
+        vc.translateBasis = Vector3f(d.volume_basis[0], d.volume_basis[1], d.volume_basis[2]);
+        auto sh = std::make_unique<NuiKinfuHIPShiftScene>(vc, d.storage, s->engine->stream());
+        sh->SetTracker(s->engine->tracker());
+        s->scene = std::move(sh);
+        s->engine->setUseGraph(false);  // the shift decision waits for each integrated frame's pose
+    } else {
         throw std::runtime_error("session: unknown scene kind");
+    }
     s->cam.m_intrinsics = NuiCameraIntrinsics(d.fx, d.fy, d.cx, d.cy);
     s->cam.m_sensorDepthMin = d.depth_min;
     s->cam.m_sensorDepthMax = d.depth_max;
     s->W = (UINT)d.width;
     s->H = (UINT)d.height;
     return s;
+}
+
+void NuiKinfuSession::afterFrame() {
+    auto* sh = dynamic_cast<NuiKinfuHIPShiftScene*>(scene.get());
+    if (!sh) return;
+    float o[3];
+    sh->OffsetSize(o);
+    offsets.insert(offsets.end(), o, o + 3);
 }
 
 NuiHashingSDFConfig NuiKinfuSession::hashConfig(const nuifk_desc& d) {
@@ -68,8 +86,9 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
         const nuif_hash_params hp = NuiKinfuHIPHashScene::ParamsOf(hashConfig(d), d.storage, d.color != 0);
         bytes += nuif_hash_scene_bytes(&hp) + n * 8;  // + ray-range image
         if (d.hash_forward_render != NUIFK_RENDER_CASTRAY) bytes += nuif_hash_forward_render_bytes(d.width, d.height);
-    } else {
+    } else {  // dense or shifting volume
         const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
+        if (d.scene == NUIFK_SCENE_SHIFT) bytes += ((size_t)d.resolution[0] * d.resolution[1] + 64) * 4 + 65536;
         bytes += voxels * nuif_tsdf_voxel_bytes(d.storage) + (d.color ? voxels * 4 : 0);
         bytes += (size_t)((d.resolution[0] + 7) / 8) * ((d.resolution[1] + 7) / 8) * ((d.resolution[2] + 7) / 8) +
                  (size_t)((d.resolution[0] + 31) / 32) * ((d.resolution[1] + 31) / 32) * ((d.resolution[2] + 31) / 32);
@@ -159,6 +178,7 @@ void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depth
         const nuif_stream_t st = s->engine->stream();
         check(nuif_stream_wait_event(st, m_start), "round wait");
         s->engine->RunTrackingDevice(0, depths[i], nullptr, s->scene.get(), s->cam, false);
+        s->afterFrame();
         check(nuif_event_record(m_ends[ids[i]], st), "frame end");
     }
 }

@@ -22,6 +22,9 @@ struct NuiKinfuSession {
     std::unique_ptr<NuiKinfuHIPSceneBase> scene;
     NuiCameraParams cam;
     UINT W = 0, H = 0;
+    std::vector<float> offsets;  // shifting volume: offset size after each frame (3 per frame)
+    // after each frame: keeps the shifting volume's offset for the pose history
+    void afterFrame();
 
     static std::unique_ptr<NuiKinfuSession> create(const nuifk_desc& d);
     // device bytes the session allocates (volume, colour volume, frame buffers,

@@ -70,6 +70,7 @@ int nuifk_run(void* h, const uint16_t* depth, int on_device, const uint8_t* bgra
         else
             ok = s->engine->RunTracking(0, const_cast<uint16_t*>(depth), (BGRQUAD*)const_cast<uint8_t*>(bgra),
                                         s->W * s->H, s->scene.get(), s->cam);
+        s->afterFrame();
         return ok ? 1 : 0;
     });
 }
@@ -83,6 +84,7 @@ int nuifk_reset(void* h) {
         Session* s = static_cast<Session*>(h);
         s->engine->reset();
         s->scene->reset();
+        s->offsets.clear();
         return 0;
     });
 }
@@ -98,8 +100,34 @@ int nuifk_poses(void* h, float* out, int n) {
                                   s->engine->stream()),
                   "pose history");
             check(nuif_stream_synchronize(s->engine->stream()), "pose history");
+            // global translation = local + the shifting volume's offset after that frame
+            if (s->offsets.size() >= (size_t)m * 3)
+                for (int i = 0; i < m; ++i)
+                    for (int k = 0; k < 3; ++k) out[(size_t)i * NUIF_POSE_ROW + 9 + k] += s->offsets[(size_t)i * 3 + k];
         }
         return m;
+    });
+}
+
+int nuifk_shift_state(void* h, int32_t* offset, float* offset_size) {
+    return guarded([&] {
+        auto* sh = dynamic_cast<NuiKinfuHIPShiftScene*>(static_cast<Session*>(h)->scene.get());
+        if (!sh) throw std::invalid_argument("nuifk_shift_state: not a shifting volume");
+        for (int k = 0; k < 3; ++k)
+            if (offset) offset[k] = sh->VoxelOffset()[k];
+        if (offset_size) sh->OffsetSize(offset_size);
+        return 0;
+    });
+}
+
+int nuifk_shift_cache(void* h, float* points, float* colors, size_t cap) {
+    return guarded([&] {
+        auto* sh = dynamic_cast<NuiKinfuHIPShiftScene*>(static_cast<Session*>(h)->scene.get());
+        if (!sh) throw std::invalid_argument("nuifk_shift_cache: not a shifting volume");
+        const size_t n = sh->CachedPositions().size() / 3, m = std::min(n, cap);
+        if (points && m) std::memcpy(points, sh->CachedPositions().data(), m * 3 * sizeof(float));
+        if (colors && m) std::memcpy(colors, sh->CachedColors().data(), m * 4 * sizeof(float));
+        return (int)n;
     });
 }
 

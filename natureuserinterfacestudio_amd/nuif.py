@@ -79,10 +79,11 @@ class KinfuDesc(C.Structure):
                 ("dimensions", c_float * 3), ("tranc_dist", c_float), ("color", c_int), ("storage", c_int),
                 ("use_graph", c_int), ("icp_reduction", c_int), ("scene", c_int), ("hash_voxel_size", c_float),
                 ("hash_trunc_scale", c_float), ("hash_truncation", c_float), ("hash_weight_max", c_int),
-                ("hash_bricks", c_int), ("hash_forward_render", c_int)]
+                ("hash_bricks", c_int), ("hash_forward_render", c_int), ("voxel_shift", c_int),
+                ("volume_basis", c_float * 3)]
 
 
-SCENE_DENSE, SCENE_HASH = 0, 1
+SCENE_DENSE, SCENE_HASH, SCENE_SHIFT = 0, 1, 2
 RENDER_CASTRAY, RENDER_FORWARD, RENDER_FORWARD_REFERENCE = 0, 1, 2
 
 
@@ -135,6 +136,7 @@ _SIGS = {
     "nuif_icp_begin": (c_int, [c_void_p, c_void_p, c_void_p]),
     "nuif_icp_iteration": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_void_p,
                                    c_void_p, c_void_p, c_float, C.c_uint32, c_int, c_int, c_void_p]),
+    "nuif_icp_set_translation": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "nuif_icp_end": (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_int, c_void_p]),
     "nuif_icp_estimate": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                   c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
@@ -253,6 +255,8 @@ _KSIGS = {
     "nuifk_updated_voxels": (c_int, [c_void_p, C.POINTER(C.c_ulonglong)]),
     "nuifk_load_tracker_config": (c_int, [C.c_char_p, C.POINTER(KinfuDesc)]),
     "nuifk_save_tracker_config": (c_int, [C.c_char_p, C.POINTER(KinfuDesc)]),
+    "nuifk_shift_state": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nuifk_shift_cache": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "nuifs_create": (c_int, [c_size_t, C.POINTER(c_void_p)]),
     "nuifs_destroy": (c_int, [c_void_p]),
     "nuifs_session_bytes": (c_size_t, [C.POINTER(KinfuDesc)]),
@@ -530,6 +534,21 @@ class Engine:
 
     def synchronize(self):
         return self._check(K().nuifk_synchronize(self._h), "synchronize")
+
+    def shift_state(self):
+        """Shifting volume: (voxel offset int32[3], its size in metres float32[3])."""
+        o = np.zeros(3, np.int32)
+        m = np.zeros(3, np.float32)
+        self._check(K().nuifk_shift_state(self._h, o.ctypes.data, m.ctypes.data), "shift state")
+        return o, m
+
+    def shift_cache(self):
+        """Shifting volume: points (float32[n,3]) and colours (float32[n,4]) of the evicted slabs."""
+        n = self._check(K().nuifk_shift_cache(self._h, None, None, 0), "shift cache")
+        p = np.empty((max(n, 1), 3), np.float32)
+        c = np.empty((max(n, 1), 4), np.float32)
+        self._check(K().nuifk_shift_cache(self._h, p.ctypes.data, c.ctypes.data, n), "shift cache")
+        return p[:n], c[:n]
 
     def poses(self, n):
         out = np.zeros((n, POSE_ROW), np.float32)

@@ -170,3 +170,38 @@ def test_slab_reset_and_fetch_match_definition_and_oracle(gpu):
     exp_v[:, xs, :, 0], exp_v[:, xs, :, 1], exp_c[:, xs] = 32767, 0, 0
     np.testing.assert_array_equal(dv.download().view(np.int16).reshape(N, N, N, 2), exp_v)
     np.testing.assert_array_equal(dc.download().reshape(N, N, N, 4), exp_c)
+
+
+@pytest.mark.parametrize("shift,color", [(3, False), (4, True)])
+def test_shift_engine_matches_oracle_engine(gpu, shift, color):
+    """RunTracking with the shifting volume (OCL/NuiKinfuOpenCLShiftScene.cpp:289-364): global
+    poses, voxel offsets, the stored volume and the vertex cache bit for bit against the
+    oracle engine over 12 frames that shift the volume several times."""
+    wl = W.SMALL
+    n = 12
+    announce(f"shift engine 320x240 128^3 voxel_shift {shift} colour {color}, {n} frames vs oracle")
+    eng = nuif.Engine(wl.kinfu_desc(shift=shift, color=color))
+    ref = O.OracleEngine(wl, prec=0, gate=0.0, color=color)
+    ref.set_shift(shift)
+    shifted = False
+    for f in range(n):
+        d, bgra, _ = wl.render(f, color=True)
+        ok = eng.run(d, bgra if color else None)
+        rok, pr, _ = ref.run(d, bgra if color else None)
+        assert ok == 1 and rok == 1, f
+        off, size = eng.shift_state()
+        roff, rsize = ref.offset()
+        np.testing.assert_array_equal(off, roff)
+        np.testing.assert_array_equal(size, rsize.astype(np.float32))
+        shifted |= bool(np.any(off != 0))
+        poses = eng.poses(f + 1)
+        expect = pr.astype(np.float32)
+        expect[9:12] = expect[9:12] + rsize.astype(np.float32)  # global = local + offset (float)
+        np.testing.assert_array_equal(poses[f, :12], expect, err_msg=f"frame {f}")
+    assert shifted
+    np.testing.assert_array_equal(eng.volume_raw().view(np.int16), ref.volume_raw())
+    gp, gc = eng.shift_cache()
+    rp, rc = ref.cache()
+    assert len(rp) > 100
+    np.testing.assert_array_equal(gp.view(np.uint32), rp.view(np.uint32))
+    np.testing.assert_array_equal(gc.view(np.uint32), rc.view(np.uint32))
