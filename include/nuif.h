@@ -404,6 +404,38 @@ size_t nuif_hash_mesh_scratch_bytes(const nuif_hash_params* p);
  * *total (device uint32) receives the vertex count before capping at `capacity`. */
 int nuif_hash_mesh(void* scene, const nuif_hash_params* p, float* vertices, float* colors, uint32_t capacity,
                    uint32_t* total, void* scratch, nuif_stream_t s);
+/* ---- host swapping of hash bricks (OCL/NuiKinfuOpenCLHashGlobalCache.cpp:121-538,
+ * K/KinfuHashSwapping.cl).  Every entry has a swap state (0: nothing to do, 1: visible and
+ * waiting for its stored data, 2: synchronised); bricks come from a free-brick stack.  Per
+ * integrated frame: nuif_hash_swap_prepare after nuif_hash_allocate (visible entries not in
+ * state 2 go to 1; visible entries without a brick take one), then after nuif_hash_integrate
+ * the swap-in list (state 1 with a brick, visible-list order) receives the host's stored
+ * blocks through nuif_hash_swap_in (weight-averaged into the brick, state 2), and the
+ * swap-out list (state 2 with a brick, not visible this frame) leaves through
+ * nuif_hash_swap_out (brick copied out, zeroed and freed, ptr -1, state 0).  The reference
+ * moves at most SDF_TRANSFER_BLOCK_NUM blocks per frame; here every listed entry moves. */
+#define NUIF_HASH_SWAP_IN 0
+#define NUIF_HASH_SWAP_OUT 1
+/* Byte offsets of the swap states (uint8 per entry) and the free-brick stack (int32 per brick). */
+int nuif_hash_swap_layout(const nuif_hash_params* p, size_t* offsets2);
+/* K/KinfuHashScene.cl:383-384 (swap states of the visible list) and K/KinfuHashSwapping.cl:6-21
+ * reAllocateSwappedOutVoxelBlocks_kernel; gate: device flag (NULL: always). */
+int nuif_hash_swap_prepare(void* scene, const nuif_hash_params* p, const int32_t* gate, nuif_stream_t s);
+size_t nuif_hash_swap_scratch_bytes(const nuif_hash_params* p);
+/* which = NUIF_HASH_SWAP_IN / _OUT: entry ids (device int32, capacity = entry count) and their
+ * number (*count, device uint32). */
+int nuif_hash_swap_list(void* scene, const nuif_hash_params* p, int which, int32_t* ids, uint32_t* count,
+                        void* scratch, nuif_stream_t s);
+/* K/KinfuHashSwapping.cl:57-118 integrateOldIntoActiveData_kernel for n listed entries: words
+ * (device, 512 per entry, the scene's voxel words; zeros where nothing was stored) and colours
+ * (uchar4 per voxel or NULL) are the stored blocks. */
+int nuif_hash_swap_in(void* scene, const nuif_hash_params* p, const int32_t* ids, uint32_t n, const uint32_t* words,
+                      const uint8_t* colors, nuif_stream_t s);
+/* K/KinfuHashSwapping.cl:135-180 moveActiveDataToTransferBuffer + cleanVoxelMemory for n listed
+ * entries: their bricks to words / colours (device, 512 per entry; colours may be NULL). */
+int nuif_hash_swap_out(void* scene, const nuif_hash_params* p, const int32_t* ids, uint32_t n, uint32_t* words,
+                       uint8_t* colors, nuif_stream_t s);
+
 size_t nuif_hash_points_scratch_bytes(const nuif_hash_params* p);
 /* K/KinfuHashVisualization.cl:5-52 fetchHashScene_kernel (OCL/NuiKinfuOpenCLHashScene.cpp:1066-1190
  * Volume2CLVertices): every voxel of a visible block with weight > 0 and |short sdf| <

@@ -46,7 +46,7 @@ struct alignas(16) HashEntry {
 };
 
 struct HashHeader {
-    uint32_t nbricks;   // bricks handed out (bump allocator)
+    uint32_t nbricks;   // bricks in use (depth of the free-brick stack)
     uint32_t nvisible;  // length of the visible list
     uint32_t failed;    // touched blocks without an entry or a brick
     uint32_t nentries;  // occupied entries
@@ -57,7 +57,7 @@ static_assert(sizeof(HashHeader) == 256, "header is 256 bytes");
 __constant__ McTable kMcHash = mc_decode();
 
 struct Layout {
-    size_t header, entries, flags, offsets, scan, visible, bricks, colors, total;
+    size_t header, entries, flags, offsets, scan, visible, bricks, colors, swap, freelist, total;
     size_t nentries;
 };
 
@@ -83,6 +83,10 @@ Layout layout_of(const nuif_hash_params& p) {
     o = align256(o + (size_t)p.bricks * kSB3 * 4);
     L.colors = o;
     if (p.color) o = align256(o + (size_t)p.bricks * kSB3 * 4);
+    L.swap = o;  // swap state per entry (host swapping)
+    o = align256(o + L.nentries);
+    L.freelist = o;  // free-brick stack: entries [nbricks, bricks) are free
+    o = align256(o + (size_t)p.bricks * 4);
     L.total = o;
     return L;
 }
@@ -96,6 +100,8 @@ struct Scene {
     int32_t* visible;
     uint32_t* bricks;
     uchar4* colors;
+    uint8_t* swap;       // 0: nothing to do, 1: wants its stored data, 2: synchronised
+    int32_t* freelist;   // brick ids; [nbricks, capacity) are free
     uint32_t mask;  // bucket mask
 };
 
@@ -111,6 +117,8 @@ Scene scene_of(void* base, const nuif_hash_params& p) {
     s.visible = reinterpret_cast<int32_t*>(b + L.visible);
     s.bricks = reinterpret_cast<uint32_t*>(b + L.bricks);
     s.colors = p.color ? reinterpret_cast<uchar4*>(b + L.colors) : nullptr;
+    s.swap = reinterpret_cast<uint8_t*>(b + L.swap);
+    s.freelist = reinterpret_cast<int32_t*>(b + L.freelist);
     s.mask = (1u << p.buckets_log2) - 1u;
     return s;
 }
@@ -160,6 +168,15 @@ __device__ __forceinline__ uint32_t bucket_of(int x, int y, int z, uint32_t mask
             ((uint32_t)(int16_t)z * 83492791u)) & mask;
 }
 
+// A brick from the free-brick stack, -1 when the pool is exhausted (the count is
+// restored, so it never exceeds the capacity).
+__device__ __forceinline__ int pop_brick(const Scene& s, uint32_t capacity) {
+    const uint32_t k = atomicAdd(&s.hdr->nbricks, 1u);
+    if (k < capacity) return s.freelist[k];
+    atomicSub(&s.hdr->nbricks, 1u);
+    return -1;
+}
+
 // Entry of a block, inserting it (and taking a brick) when absent; -1 when the
 // probe limit is reached.
 __device__ int find_or_insert(const Scene& s, int bx, int by, int bz, uint32_t capacity) {
@@ -176,12 +193,8 @@ __device__ int find_or_insert(const Scene& s, int bx, int by, int bz, uint32_t c
             if (cur != 0ull) continue;
             const unsigned long long old = atomicCAS(&e->key, 0ull, key);
             if (old == 0ull) {
-                const uint32_t ptr = atomicAdd(&s.hdr->nbricks, 1u);
-                int v = (int)ptr;
-                if (ptr >= capacity) {
-                    v = -1;
-                    atomicAdd(&s.hdr->failed, 1u);
-                }
+                const int v = pop_brick(s, capacity);
+                if (v < 0) atomicAdd(&s.hdr->failed, 1u);
                 e->ptr = v;
                 atomicAdd(&s.hdr->nentries, 1u);
                 return id;
@@ -942,7 +955,139 @@ __global__ void __launch_bounds__(64)
     }
 }
 
-__global__ void hash_reset_kernel(Scene s, size_t nentries, size_t nwords) {
+// ---- host swapping (OCL/NuiKinfuOpenCLHashGlobalCache.cpp, K/KinfuHashSwapping.cl) ----
+
+// K/KinfuHashScene.cl:383-384 and K/KinfuHashSwapping.cl:6-21 reAllocateSwappedOutVoxelBlocks:
+// a visible entry not yet synchronised asks for its stored data (state 1); a visible
+// entry without a brick (swapped out, or left without one by a full pool) takes one
+// from the free-brick stack (bricks come back zeroed).
+__global__ void hash_swap_mark_kernel(Scene s, uint32_t capacity, const int32_t* __restrict__ gate) {
+    if (gate && *gate == 0) return;
+    const uint32_t n = s.hdr->nvisible;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int id = s.visible[i];
+        if (s.swap[id] != 2) s.swap[id] = 1;
+        if (s.entries[id].ptr == -1) {
+            const int v = pop_brick(s, capacity);
+            if (v >= 0) s.entries[id].ptr = v;
+        }
+    }
+}
+
+// List flags: swap-in = visible entries in state 1 holding a brick (visible-list
+// order); swap-out = entries in state 2 holding a brick that this frame does not see
+// (K/KinfuHashSwapping.cl:120-133 buildSwapOutList_kernel; its test ptr > 0 keeps
+// whichever block holds brick 0 resident, an accident of allocation order, so here
+// every brick can go).
+__global__ void hash_swap_in_compact_kernel(Scene s, const uint32_t* __restrict__ flag,
+                                            const uint32_t* __restrict__ pos, uint32_t n, int32_t* __restrict__ ids) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (flag[i]) ids[pos[i]] = s.visible[i];
+}
+__global__ void hash_swap_out_flags_kernel(Scene s, uint32_t* __restrict__ flag, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        flag[i] = (s.swap[i] == 2 && s.entries[i].ptr >= 0 && s.flags[i] == 0) ? 1u : 0u;
+}
+__global__ void hash_swap_out_compact_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                             size_t n, int32_t* __restrict__ ids) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        if (flag[i]) ids[pos[i]] = (int32_t)i;
+}
+
+// the visible list's length as a kernel argument needs the host; the flags kernel
+// reads it from the header instead and flags the rest of the capacity 0
+__global__ void hash_swap_in_flags_all_kernel(Scene s, uint32_t* __restrict__ flag, size_t cap) {
+    const uint32_t n = s.hdr->nvisible;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t f = 0u;
+        if (i < n) {
+            const int id = s.visible[i];
+            f = (s.swap[id] == 1 && s.entries[id].ptr >= 0) ? 1u : 0u;
+        }
+        flag[i] = f;
+    }
+}
+
+// float -> voxel word of the two storages, NaN -> 0 (a block merged with nothing at a
+// voxel neither side observed)
+template <class V> __device__ __forceinline__ uint32_t sdf_bits(float f);
+template <> __device__ __forceinline__ uint32_t sdf_bits<VS16>(float f) {
+    return (uint32_t)(uint16_t)(int16_t)cvt_sat(f * 32767.0f);
+}
+template <> __device__ __forceinline__ uint32_t sdf_bits<VF16>(float f) {
+    return f != f ? 0u : (uint32_t)__half_as_ushort(__float2half_rn(f));
+}
+
+// K/KinfuHashSwapping.cl:57-118 integrateOldIntoActiveData_kernel: the stored block
+// (zeros when nothing was stored) and the active brick, weight-averaged into the
+// brick; the entry is synchronised (state 2).  One wavefront per listed entry.
+template <class V>
+__global__ void __launch_bounds__(64)
+    hash_swap_in_kernel(Scene s, const int32_t* __restrict__ ids, uint32_t n, const uint32_t* __restrict__ words,
+                        const uchar4* __restrict__ cols, int wmax) {
+    const int lane = threadIdx.x;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int id = ids[i];
+        const int ptr = s.entries[id].ptr;
+        for (int k = 0; k < 8; ++k) {
+            const int li = lane + 64 * k;
+            const size_t vi = (size_t)ptr * kSB3 + li;
+            const uint32_t old = words[(size_t)i * kSB3 + li];
+            const uint32_t cur = s.bricks[vi];
+            const float oldF = V::sdf(old), oldW = (float)V::weight(old);
+            const float newF = V::sdf(cur), newW = (float)V::weight(cur);
+            const float combinedW = oldW + newW;
+            float f = oldW * oldF + newW * newF;
+            f /= combinedW;
+            // the reference's convert_uchar of a weight sum above 255 is undefined; it saturates here
+            const int w = min(cvt_sat(combinedW), wmax);
+            s.bricks[vi] = sdf_bits<V>(f) | ((uint32_t)w << 16);
+            if (s.colors) {
+                const uchar4 oc = cols ? cols[(size_t)i * kSB3 + li] : make_uchar4(0, 0, 0, 0);
+                uchar4 nc = s.colors[vi];
+                const float r = (oldW * (float)oc.x + newW * (float)nc.x) / combinedW;
+                const float g = (oldW * (float)oc.y + newW * (float)nc.y) / combinedW;
+                const float b = (oldW * (float)oc.z + newW * (float)nc.z) / combinedW;
+                nc.x = (unsigned char)cvt_sat(fminf(fmaxf(r, 0.0f), 255.f));
+                nc.y = (unsigned char)cvt_sat(fminf(fmaxf(g, 0.0f), 255.f));
+                nc.z = (unsigned char)cvt_sat(fminf(fmaxf(b, 0.0f), 255.f));
+                s.colors[vi] = nc;
+            }
+        }
+        if (lane == 0) s.swap[id] = 2;
+    }
+}
+
+// K/KinfuHashSwapping.cl:135-180 moveActiveDataToTransferBuffer_kernel + cleanVoxelMemory_kernel:
+// the brick goes to the transfer buffer and back to the free-brick stack zeroed; the
+// entry keeps its key without a brick (ptr -1, state 0).
+__global__ void __launch_bounds__(64)
+    hash_swap_out_kernel(Scene s, const int32_t* __restrict__ ids, uint32_t n, uint32_t* __restrict__ words,
+                         uchar4* __restrict__ cols) {
+    const int lane = threadIdx.x;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int id = ids[i];
+        const int ptr = s.entries[id].ptr;
+        for (int k = 0; k < 8; ++k) {
+            const int li = lane + 64 * k;
+            const size_t vi = (size_t)ptr * kSB3 + li;
+            words[(size_t)i * kSB3 + li] = s.bricks[vi];
+            s.bricks[vi] = 0u;
+            if (s.colors) {
+                if (cols) cols[(size_t)i * kSB3 + li] = s.colors[vi];
+                s.colors[vi] = make_uchar4(0, 0, 0, 0);
+            }
+        }
+        if (lane == 0) {
+            const uint32_t top = atomicSub(&s.hdr->nbricks, 1u) - 1u;
+            s.freelist[top] = ptr;
+            s.entries[id].ptr = -1;
+            s.swap[id] = 0;
+        }
+    }
+}
+
+__global__ void hash_reset_kernel(Scene s, size_t nentries, size_t nwords, size_t nbricks) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i0 == 0) {
@@ -955,7 +1100,9 @@ __global__ void hash_reset_kernel(Scene s, size_t nentries, size_t nwords) {
         s.entries[i].key = 0ull;
         s.entries[i].ptr = -1;
         s.entries[i].pad = 0u;
+        s.swap[i] = 0;
     }
+    for (size_t i = i0; i < nbricks; i += stride) s.freelist[i] = (int32_t)i;
     for (size_t i = i0; i < nwords; i += stride) {
         s.bricks[i] = 0u;
         if (s.colors) s.colors[i] = make_uchar4(0, 0, 0, 0);
@@ -1009,7 +1156,8 @@ int nuif_hash_reset(void* scene, const nuif_hash_params* p, nuif_stream_t s) {
     if (rc) return rc;
     NUIF_CHECK_ARG(scene != nullptr, "nuif_hash_reset: null scene");
     const Layout L = layout_of(*p);
-    hash_reset_kernel<<<4096, 256, 0, stream_of(s)>>>(scene_of(scene, *p), L.nentries, (size_t)p->bricks * kSB3);
+    hash_reset_kernel<<<4096, 256, 0, stream_of(s)>>>(scene_of(scene, *p), L.nentries, (size_t)p->bricks * kSB3,
+                                                       (size_t)p->bricks);
     NUIF_LAUNCHED("hash_reset_kernel");
     return NUIF_OK;
 }
@@ -1264,6 +1412,95 @@ int nuif_hash_points(void* scene, const nuif_hash_params* p, float* points, floa
                                                                       points, colors, capacity);
         NUIF_LAUNCHED("hash_points_kernel");
     }
+    return NUIF_OK;
+}
+
+int nuif_hash_swap_layout(const nuif_hash_params* p, size_t* offsets2) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(offsets2 != nullptr, "nuif_hash_swap_layout: null output");
+    const Layout L = layout_of(*p);
+    offsets2[0] = L.swap;
+    offsets2[1] = L.freelist;
+    return NUIF_OK;
+}
+
+int nuif_hash_swap_prepare(void* scene, const nuif_hash_params* p, const int32_t* gate, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(p);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(scene != nullptr, "nuif_hash_swap_prepare: null scene");
+    hash_swap_mark_kernel<<<kWalkGrid / 4, 256, 0, stream_of(s)>>>(scene_of(scene, *p), (uint32_t)p->bricks, gate);
+    NUIF_LAUNCHED("hash_swap_mark_kernel");
+    return NUIF_OK;
+}
+
+size_t nuif_hash_swap_scratch_bytes(const nuif_hash_params* p) {
+    if (!p || check_params(p)) return 0;
+    const size_t n = layout_of(*p).nentries;
+    return 2 * align256(n * 4) + align256(nuif_scan_scratch_bytes(n)) + 256;
+}
+
+int nuif_hash_swap_list(void* scene, const nuif_hash_params* p, int which, int32_t* ids, uint32_t* count,
+                        void* scratch, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(p);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(scene && ids && count && scratch, "nuif_hash_swap_list: null argument");
+    NUIF_CHECK_ARG(which == NUIF_HASH_SWAP_IN || which == NUIF_HASH_SWAP_OUT, "nuif_hash_swap_list: bad list %d",
+                   which);
+    const size_t n = layout_of(*p).nentries;
+    Scene sc = scene_of(scene, *p);
+    hipStream_t st = stream_of(s);
+    char* b = static_cast<char*>(scratch);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(b);
+    uint32_t* pos = reinterpret_cast<uint32_t*>(b + align256(n * 4));
+    void* scan = b + 2 * align256(n * 4);
+    const unsigned grid = (unsigned)std::min<size_t>(div_up(n, 256), 4096);
+    if (which == NUIF_HASH_SWAP_IN)
+        hash_swap_in_flags_all_kernel<<<grid, 256, 0, st>>>(sc, flag, n);
+    else
+        hash_swap_out_flags_kernel<<<grid, 256, 0, st>>>(sc, flag, n);
+    NUIF_LAUNCHED("hash_swap_flags_kernel");
+    rc = launch_scan(flag, false, pos, n, count, scan, st);
+    if (rc) return rc;
+    if (which == NUIF_HASH_SWAP_IN)
+        hash_swap_in_compact_kernel<<<grid, 256, 0, st>>>(sc, flag, pos, (uint32_t)n, ids);
+    else
+        hash_swap_out_compact_kernel<<<grid, 256, 0, st>>>(flag, pos, n, ids);
+    NUIF_LAUNCHED("hash_swap_compact_kernel");
+    return NUIF_OK;
+}
+
+int nuif_hash_swap_in(void* scene, const nuif_hash_params* p, const int32_t* ids, uint32_t n, const uint32_t* words,
+                      const uint8_t* colors, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (n == 0) return NUIF_OK;
+    NUIF_CHECK_ARG(scene && ids && words, "nuif_hash_swap_in: null argument");
+    Scene sc = scene_of(scene, *p);
+    const unsigned grid = std::min<unsigned>(n, (unsigned)kWalkGrid);
+    const auto* c4 = reinterpret_cast<const uchar4*>(colors);
+    if (p->storage == NUIF_TSDF_S16)
+        hash_swap_in_kernel<VS16><<<grid, 64, 0, stream_of(s)>>>(sc, ids, n, words, c4, p->weight_max);
+    else
+        hash_swap_in_kernel<VF16><<<grid, 64, 0, stream_of(s)>>>(sc, ids, n, words, c4, p->weight_max);
+    NUIF_LAUNCHED("hash_swap_in_kernel");
+    return NUIF_OK;
+}
+
+int nuif_hash_swap_out(void* scene, const nuif_hash_params* p, const int32_t* ids, uint32_t n, uint32_t* words,
+                       uint8_t* colors, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (n == 0) return NUIF_OK;
+    NUIF_CHECK_ARG(scene && ids && words, "nuif_hash_swap_out: null argument");
+    Scene sc = scene_of(scene, *p);
+    const unsigned grid = std::min<unsigned>(n, (unsigned)kWalkGrid);
+    hash_swap_out_kernel<<<grid, 64, 0, stream_of(s)>>>(sc, ids, n, words, reinterpret_cast<uchar4*>(colors));
+    NUIF_LAUNCHED("hash_swap_out_kernel");
     return NUIF_OK;
 }
 

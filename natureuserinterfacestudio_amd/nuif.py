@@ -217,6 +217,14 @@ _SIGS = {
                                          c_void_p]),
     "nuif_hash_mesh_scratch_bytes": (c_size_t, [C.POINTER(HashParams)]),
     "nuif_hash_mesh": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, c_void_p, C.c_uint32, c_void_p, c_void_p, c_void_p]),
+    "nuif_hash_swap_layout": (c_int, [C.POINTER(HashParams), C.POINTER(c_size_t)]),
+    "nuif_hash_swap_prepare": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, c_void_p]),
+    "nuif_hash_swap_scratch_bytes": (c_size_t, [C.POINTER(HashParams)]),
+    "nuif_hash_swap_list": (c_int, [c_void_p, C.POINTER(HashParams), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nuif_hash_swap_in": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, C.c_uint32, c_void_p, c_void_p,
+                                  c_void_p]),
+    "nuif_hash_swap_out": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, C.c_uint32, c_void_p, c_void_p,
+                                   c_void_p]),
     "nuif_hash_points_scratch_bytes": (c_size_t, [C.POINTER(HashParams)]),
     "nuif_hash_points": (c_int, [c_void_p, C.POINTER(HashParams), c_void_p, c_void_p, C.c_uint32, c_void_p, c_void_p,
                                  c_void_p]),
@@ -782,6 +790,74 @@ class HashScene:
         words = raw[self.offsets[4]:self.offsets[4] + nbr * 512 * 4].view(np.uint32).copy()
         cols = raw[self.offsets[5]:self.offsets[5] + nbr * 512 * 4].copy() if self.p.color else None
         return pos, ent["ptr"].astype(np.int32), words, cols
+
+    def swap_prepare(self, gate=None):
+        check(L().nuif_hash_swap_prepare(self.mem.ptr, C.byref(self.p), gate, self.stream), "nuif_hash_swap_prepare")
+
+    def swap_cycle(self, store):
+        """The host side of one frame's swapping after integration (the role of
+        NuiKinfuOpenCLHashGlobalCache): stored blocks in, unseen blocks out.  store: dict entry
+        id -> (words uint32[512], colours uint8[512,4] or None), updated in place.  Returns the
+        (swapped in, swapped out) entry ids."""
+        ne = self.offsets[6]
+        if not hasattr(self, "_swap_scratch"):
+            self._swap_scratch = DeviceArray((L().nuif_hash_swap_scratch_bytes(C.byref(self.p)),), np.uint8)
+            self._swap_ids = DeviceArray((ne,), np.int32)
+            self._swap_cnt = DeviceArray((1,), np.uint32)
+        moved = []
+        for which in (0, 1):
+            check(L().nuif_hash_swap_list(self.mem.ptr, C.byref(self.p), which, self._swap_ids.ptr,
+                                          self._swap_cnt.ptr, self._swap_scratch.ptr, self.stream), "swap list")
+            n = int(self._swap_cnt.download()[0])
+            ids = self._swap_ids.download()[:n].copy()
+            moved.append(ids)
+            if n == 0:
+                continue
+            words = np.zeros((n, 512), np.uint32)
+            cols = np.zeros((n, 512, 4), np.uint8) if self.p.color else None
+            if which == 0:
+                for i, e in enumerate(ids):
+                    if int(e) in store:
+                        words[i] = store[int(e)][0]
+                        if cols is not None:
+                            cols[i] = store[int(e)][1]
+                dw, dc = DeviceArray.from_host(words), (DeviceArray.from_host(cols) if cols is not None else None)
+                check(L().nuif_hash_swap_in(self.mem.ptr, C.byref(self.p), self._swap_ids.ptr, n, dw.ptr,
+                                            dc.ptr if dc else None, self.stream), "nuif_hash_swap_in")
+            else:
+                dw = DeviceArray((n, 512), np.uint32)
+                dc = DeviceArray((n, 512, 4), np.uint8) if cols is not None else None
+                check(L().nuif_hash_swap_out(self.mem.ptr, C.byref(self.p), self._swap_ids.ptr, n, dw.ptr,
+                                             dc.ptr if dc else None, self.stream), "nuif_hash_swap_out")
+                w, c = dw.download(), (dc.download() if dc else None)
+                for i, e in enumerate(ids):
+                    store[int(e)] = (w[i].copy(), None if c is None else c[i].copy())
+        return moved[0], moved[1]
+
+    def swap_states(self):
+        """(keys int16[n,3] sorted by (x,y,z), state uint8[n]) of every occupied entry."""
+        raw = self.mem.download()
+        off = (c_size_t * 2)()
+        check(L().nuif_hash_swap_layout(C.byref(self.p), off), "nuif_hash_swap_layout")
+        ne = self.offsets[6]
+        ent = raw[self.offsets[1]:self.offsets[1] + ne * 16].view(self.ENTRY)
+        st = raw[off[0]:off[0] + ne]
+        sel = np.nonzero(ent["key"] != 0)[0]
+        k = ent["key"][sel]
+        keys = np.stack([((k >> 32) & 0xFFFF).astype(np.uint16).view(np.int16),
+                         ((k >> 16) & 0xFFFF).astype(np.uint16).view(np.int16),
+                         (k & 0xFFFF).astype(np.uint16).view(np.int16)], 1)
+        order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+        return keys[order], st[sel][order]
+
+    def entry_keys(self, ids):
+        """int16[n,3] keys of entry ids."""
+        raw = self.mem.download()
+        ne = self.offsets[6]
+        k = raw[self.offsets[1]:self.offsets[1] + ne * 16].view(self.ENTRY)["key"][np.asarray(ids, np.int64)]
+        return np.stack([((k >> 32) & 0xFFFF).astype(np.uint16).view(np.int16),
+                         ((k >> 16) & 0xFFFF).astype(np.uint16).view(np.int16),
+                         (k & 0xFFFF).astype(np.uint16).view(np.int16)], 1)
 
     def stats(self):
         out = (C.c_uint32 * 4)()

@@ -97,8 +97,14 @@ struct HashCfg {
 
 struct HashScene {
     HashCfg c;
-    std::unordered_map<uint64_t, int> index;  // block key -> brick id
+    std::unordered_map<uint64_t, int> index;  // block key -> brick id (-1: swapped out, no brick)
     std::vector<Brick> bricks;
+    // host swapping (OCL/NuiKinfuOpenCLHashGlobalCache.cpp): swap state per key (0 / 1 / 2),
+    // the host store of swapped-out blocks, recycled brick ids
+    bool swapping = false;
+    std::unordered_map<uint64_t, uint8_t> swap_state;
+    std::unordered_map<uint64_t, Brick> store;
+    std::vector<int> free_ids;
     std::vector<uint64_t> visible;  // visible list of the last integration (sorted by key)
     std::vector<float> range;       // ray range image of the last raycast (near, far)
 
@@ -178,10 +184,9 @@ struct HashScene {
         const std::vector<uint64_t> t = touched(depth, cam, g);
         for (uint64_t k : t) {
             if (index.count(k)) continue;
-            if ((int)bricks.size() >= c.capacity) continue;
-            index.emplace(k, (int)bricks.size());
-            bricks.emplace_back();
-            std::memset(&bricks.back(), 0, sizeof(Brick));
+            const int b = take_brick();
+            if (b < 0) continue;
+            index.emplace(k, b);
         }
         std::vector<uint64_t> vis;
         for (uint64_t k : t)
@@ -195,6 +200,78 @@ struct HashScene {
         return t.size();
     }
 
+    // a zeroed brick: a recycled one, else a new one while the pool has room; -1 when full
+    int take_brick() {
+        if (!free_ids.empty()) {
+            const int b = free_ids.back();
+            free_ids.pop_back();
+            return b;
+        }
+        if ((int)bricks.size() >= c.capacity) return -1;
+        bricks.emplace_back();
+        std::memset(&bricks.back(), 0, sizeof(Brick));
+        return (int)bricks.size() - 1;
+    }
+
+    // K/KinfuHashScene.cl:383-384: visible blocks not yet synchronised want their stored
+    // data; K/KinfuHashSwapping.cl:6-21: visible blocks without a brick take one
+    void swap_prepare() {
+        for (uint64_t k : visible) {
+            uint8_t& st = swap_state[k];
+            if (st != 2) st = 1;
+            int& b = index.at(k);
+            if (b == -1) b = take_brick();
+        }
+    }
+
+    // K/KinfuHashSwapping.cl:57-118 integrateOldIntoActiveData_kernel: the stored block
+    // (zeros when none) weight-averaged into the brick of every visible block in state 1
+    void swap_in() {
+        for (uint64_t k : visible) {
+            uint8_t& st = swap_state[k];
+            const int id = index.at(k);
+            if (st != 1 || id < 0) continue;
+            Brick old;
+            std::memset(&old, 0, sizeof(old));
+            auto it = store.find(k);
+            if (it != store.end()) old = it->second;
+            Brick& cur = bricks[id];
+            for (int v = 0; v < kSB3; ++v) {
+                const float oldF = (float)old.sdf[v] / 32767.0f, oldW = (float)old.w[v];
+                const float newF = (float)cur.sdf[v] / 32767.0f, newW = (float)cur.w[v];
+                const float combinedW = oldW + newW;
+                float f = oldW * oldF + newW * newF;
+                f /= combinedW;
+                cur.sdf[v] = (int16_t)cvt_sat(f * 32767.0f);
+                // convert_uchar of a weight sum above 255 is undefined in the reference: saturated
+                cur.w[v] = (uint8_t)std::min(cvt_sat(combinedW), c.wmax);
+                for (int ch = 0; ch < 3; ++ch) {
+                    const float x = (oldW * (float)old.rgb[v][ch] + newW * (float)cur.rgb[v][ch]) / combinedW;
+                    cur.rgb[v][ch] = (uint8_t)cvt_sat(std::min(std::max(x, 0.0f), 255.f));
+                }
+            }
+            st = 2;
+        }
+    }
+
+    // K/KinfuHashSwapping.cl:120-180 buildSwapOutList / moveActiveDataToTransferBuffer /
+    // cleanVoxelMemory: synchronised blocks with a brick that this frame does not see go to
+    // the store; their bricks are zeroed and recycled (every brick; the reference keeps
+    // the holder of brick 0)
+    void swap_out() {
+        std::vector<uint64_t> vis(visible);
+        std::sort(vis.begin(), vis.end());
+        for (auto& kv : index) {
+            uint8_t& st = swap_state[kv.first];
+            if (st != 2 || kv.second < 0 || std::binary_search(vis.begin(), vis.end(), kv.first)) continue;
+            store[kv.first] = bricks[kv.second];
+            std::memset(&bricks[kv.second], 0, sizeof(Brick));
+            free_ids.push_back(kv.second);
+            kv.second = -1;
+            st = 0;
+        }
+    }
+
     // K/KinfuHashScene.cl:387-470 integrateIntoScene_kernel over the visible list
     long long integrate(const float* depth, const uint8_t* bgra, const Cam<float>& cam, const Rigid<float>& g) {
         long long nupd = 0;
@@ -202,7 +279,9 @@ struct HashScene {
 #pragma omp parallel for reduction(+ : nupd) schedule(dynamic, 16)
         for (int i = 0; i < n; ++i) {
             const uint64_t key = visible[i];
-            Brick& b = bricks[index.at(key)];
+            const int bid = index.at(key);
+            if (bid < 0) continue;  // no brick (pool full)
+            Brick& b = bricks[bid];
             int p[3];
             unpack_key(key, p);
             for (int lid = 0; lid < kSB3; ++lid) {
@@ -250,7 +329,7 @@ struct HashScene {
     // ---- lookups (K/KinfuHashRaycast.cl:78-124)
     const Brick* brick_of(int bx, int by, int bz) const {
         auto it = index.find(pack_key(bx, by, bz));
-        return it == index.end() ? nullptr : &bricks[it->second];
+        return it == index.end() || it->second < 0 ? nullptr : &bricks[it->second];
     }
     static int floor_div8(int v) { return v < 0 ? (v - 7) / 8 : v / 8; }  // virtualPosToVoxelBlock
     // sdf of a voxel (world voxel units), +1 when its block is missing
@@ -819,7 +898,41 @@ OR_EXPORT long long or_hash_scene_integrate(void* sp, const float* depth, const 
     const Cam<float> cam = cam_of(cam6, W, H);
     const Rigid<float> g = make_rigid<float>(R9, t3);
     s->allocate_and_select(depth, cam, g);
-    return s->integrate(depth, bgra, cam, g);
+    if (s->swapping) s->swap_prepare();
+    const long long n = s->integrate(depth, bgra, cam, g);
+    if (s->swapping) {  // OCL/NuiKinfuOpenCLHashScene.cpp:576-580
+        s->swap_in();
+        s->swap_out();
+    }
+    return n;
+}
+
+OR_EXPORT void or_hash_scene_set_swapping(void* sp, int on) { ((HashScene*)sp)->swapping = on != 0; }
+
+// Swap states of the blocks with an entry (sorted by key) and the stored blocks (sorted by key).
+OR_EXPORT int or_hash_scene_swap_states(void* sp, int16_t* keys, uint8_t* states, long long cap) {
+    HashScene* s = (HashScene*)sp;
+    std::vector<uint64_t> ks;
+    for (auto& kv : s->index) ks.push_back(kv.first);
+    std::sort(ks.begin(), ks.end(), key_less);
+    for (size_t i = 0; i < ks.size() && (long long)i < cap; ++i) {
+        auto it = s->swap_state.find(ks[i]);
+        if (states) states[i] = it == s->swap_state.end() ? 0 : it->second;
+    }
+    return export_keys(ks, keys, cap);
+}
+OR_EXPORT int or_hash_scene_store(void* sp, int16_t* keys, int16_t* sdf, uint8_t* w, uint8_t* rgb, long long cap) {
+    HashScene* s = (HashScene*)sp;
+    std::vector<uint64_t> ks;
+    for (auto& kv : s->store) ks.push_back(kv.first);
+    std::sort(ks.begin(), ks.end(), key_less);
+    for (size_t i = 0; i < ks.size() && (long long)i < cap; ++i) {
+        const Brick& b = s->store.at(ks[i]);
+        if (sdf) std::memcpy(sdf + i * kSB3, b.sdf, sizeof(b.sdf));
+        if (w) std::memcpy(w + i * kSB3, b.w, sizeof(b.w));
+        if (rgb) std::memcpy(rgb + i * kSB3 * 3, b.rgb, sizeof(b.rgb));
+    }
+    return export_keys(ks, keys, cap);
 }
 
 // Ray ranges + castRay at a pose: vmap / nmap (W*H*3), range2 (W*H*2), cmap (W*H*4 or NULL).
@@ -854,7 +967,8 @@ OR_EXPORT int or_hash_scene_blocks(void* sp, int16_t* keys, int16_t* sdf, uint8_
     HashScene* s = (HashScene*)sp;
     std::vector<uint64_t> ks;
     ks.reserve(s->index.size());
-    for (auto& kv : s->index) ks.push_back(kv.first);
+    for (auto& kv : s->index)
+        if (kv.second >= 0) ks.push_back(kv.first);  // blocks holding a brick
     std::sort(ks.begin(), ks.end(), key_less);
     for (size_t i = 0; i < ks.size() && (long long)i < cap; ++i) {
         const Brick& b = s->bricks[s->index.at(ks[i])];

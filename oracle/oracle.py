@@ -84,6 +84,9 @@ def lib():
             "or_hash_scene_blocks": (ci, [vp, vp, vp, vp, vp, cll]),
             "or_hash_scene_visible": (ci, [vp, vp, cll]),
             "or_hash_scene_mesh": (cll, [vp, i32p, i32p, vp, cll]),
+            "or_hash_scene_set_swapping": (None, [vp, ci]),
+            "or_hash_scene_swap_states": (ci, [vp, vp, vp, cll]),
+            "or_hash_scene_store": (ci, [vp, vp, vp, vp, vp, cll]),
             "or_color_half_sample": (None, [u8p, u8p, ci, ci, ci]),
             "or_color_gradient": (None, [u8p, u8p, u8p, ci, ci, ci]),
             "or_color_difference": (None, [ci, ci, f32p, ci, f32p, u8p, u8p, f32p, f32p,
@@ -217,6 +220,26 @@ class _HashSceneView:
         lib().or_hash_scene_blocks(sp, keys.ctypes.data, sdf.ctypes.data, w.ctypes.data, rgb.ctypes.data, n)
         return keys, sdf, w, rgb
 
+    def swap_states(self):
+        """(keys int16[n,3] sorted by (x,y,z), swap state uint8[n]) of every block with an entry."""
+        sp = self._scene()
+        n = lib().or_hash_scene_swap_states(sp, None, None, 0)
+        keys = np.empty((n, 3), np.int16)
+        st = np.empty(n, np.uint8)
+        lib().or_hash_scene_swap_states(sp, keys.ctypes.data, st.ctypes.data, n)
+        return keys, st
+
+    def store(self):
+        """The host store of swapped-out blocks: (keys, sdf int16[n,512], weight uint8[n,512], rgb)."""
+        sp = self._scene()
+        n = lib().or_hash_scene_store(sp, None, None, None, None, 0)
+        keys = np.empty((n, 3), np.int16)
+        sdf = np.empty((n, 512), np.int16)
+        w = np.empty((n, 512), np.uint8)
+        rgb = np.empty((n, 512, 3), np.uint8)
+        lib().or_hash_scene_store(sp, keys.ctypes.data, sdf.ctypes.data, w.ctypes.data, rgb.ctypes.data, n)
+        return keys, sdf, w, rgb
+
     def visible(self):
         sp = self._scene()
         n = lib().or_hash_scene_visible(sp, None, 0)
@@ -236,8 +259,10 @@ class _HashSceneView:
 class OracleHashScene(_HashSceneView):
     """The reference's voxel-hashing scene on the CPU, driven with explicit poses."""
 
-    def __init__(self, voxel=0.005, trunc_scale=0.02, truncation=0.06, wmax=255, capacity=1 << 18):
+    def __init__(self, voxel=0.005, trunc_scale=0.02, truncation=0.06, wmax=255, capacity=1 << 18, swapping=False):
         self.h = lib().or_hash_scene_create(voxel, trunc_scale, truncation, wmax, capacity)
+        if swapping:
+            lib().or_hash_scene_set_swapping(self.h, 1)
 
     def _scene(self):
         return self.h

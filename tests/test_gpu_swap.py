@@ -1,0 +1,95 @@
+"""GPU parity of the host swapping of hash bricks (OCL/NuiKinfuOpenCLHashGlobalCache.cpp,
+K/KinfuHashSwapping.cl) against the oracle, by block key.
+
+A camera that integrates a view, looks away, comes back and moves on makes blocks leave
+(swap-out to the host store), come back (fresh brick, re-integration, weight-averaged
+with the stored copy) and stay (synchronised).  After every frame: the blocks holding a
+brick, their voxel words and colours, the swap state of every block and the host store
+equal the oracle's bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from natureuserinterfacestudio_amd import nuif
+from natureuserinterfacestudio_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+WL = W.HASH_SMALL
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _frame(f):
+    d, bgra, gt = WL.render(f, color=True)
+    dep = np.empty(d.size, np.float32)
+    O.lib().or_passing_filter_f(d.reshape(-1), dep, d.size, 0.5, 4.5)
+    return dep, bgra, gt[:, :3].astype(np.float32).copy(), gt[:, 3].astype(np.float32).copy()
+
+
+def _away(R):
+    a = np.deg2rad(150.0)
+    Ry = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]], np.float32)
+    return (R @ Ry).astype(np.float32)
+
+
+def _sequence():
+    d0, c0, R0, t0 = _frame(0)
+    d2, c2, R2, t2 = _frame(2)
+    d4, c4, R4, t4 = _frame(4)
+    return [(d0, c0, R0, t0), (d2, c2, R2, t2), (d2, c2, _away(R2), t2), (d2, c2, R2, t2), (d4, c4, R4, t4),
+            (d0, c0, _away(R0), t0), (d4, c4, R4, t4)]
+
+
+def _to_words(sdf, w):
+    return sdf.astype(np.int16).view(np.uint16).astype(np.uint32) | (w.astype(np.uint32) << 16)
+
+
+def test_swapping_scene_matches_oracle(gpu):
+    announce("hash swapping 320x240: 7 frames away and back, per-frame blocks / states / store vs oracle")
+    p = nuif.hash_params(buckets_log2=16, bricks=1 << 17, storage=0, color=True)
+    g = nuif.HashScene(p)
+    o = O.OracleHashScene(capacity=1 << 17, swapping=True)
+    cam = WL.camera_params()
+    store = {}
+    swapped_out = swapped_back = 0
+    for f, (dep, bgra, R, t) in enumerate(_sequence()):
+        dd, db = nuif.DeviceArray.from_host(dep), nuif.DeviceArray.from_host(bgra)
+        pose = nuif.upload_struct(nuif.rigid(R, t))
+        g.allocate(dd, cam, pose)
+        g.swap_prepare()
+        g.integrate(dd, cam, pose, bgra=db)
+        ins, outs = g.swap_cycle(store)
+        o.integrate(dep, WL, R, t, bgra=bgra)
+        swapped_out += len(outs)
+        swapped_back += sum(1 for e in ins if int(e) in store)
+        # blocks holding a brick
+        gk, gsdf, gw, gwords, grgb = g.blocks()
+        ok_, osdf, ow, orgb = o.blocks()
+        np.testing.assert_array_equal(gk, ok_, err_msg=f"frame {f}: block keys")
+        np.testing.assert_array_equal(gwords, _to_words(osdf, ow), err_msg=f"frame {f}: voxel words")
+        np.testing.assert_array_equal(grgb, orgb, err_msg=f"frame {f}: colours")
+        # swap states of every block
+        sk, sst = g.swap_states()
+        rk, rst = o.swap_states()
+        np.testing.assert_array_equal(sk, rk, err_msg=f"frame {f}: entries")
+        np.testing.assert_array_equal(sst, rst, err_msg=f"frame {f}: swap states")
+        # host store (entry ids -> keys)
+        if store:
+            ids = np.array(sorted(store), np.int64)
+            keys = g.entry_keys(ids)
+            order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+            words = np.stack([store[int(i)][0] for i in ids])[order]
+            cols = np.stack([store[int(i)][1] for i in ids])[order]
+            tk, tsdf, tw, trgb = o.store()
+            np.testing.assert_array_equal(keys[order], tk, err_msg=f"frame {f}: store keys")
+            np.testing.assert_array_equal(words, _to_words(tsdf, tw), err_msg=f"frame {f}: store words")
+            np.testing.assert_array_equal(cols[:, :, :3], trgb, err_msg=f"frame {f}: store colours")
+        else:
+            assert len(o.store()[0]) == 0
+    assert swapped_out > 100 and swapped_back > 100, (swapped_out, swapped_back)
