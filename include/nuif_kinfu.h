@@ -55,6 +55,7 @@ typedef struct nuifk_desc {
      * translateBasis (metres) */
     int voxel_shift;
     float volume_basis[3];
+    int hash_swapping;              /* m_bUseSwapping: host swapping of hash bricks (no graph replay) */
 } nuifk_desc;
 
 #define NUIFK_SCENE_DENSE 0         /* OCL/NuiKinfuOpenCLScene: dense TSDF volume */
@@ -89,6 +90,8 @@ int nuifk_shift_state(void* h, int32_t* offset, float* offset_size);
 /* Shifting volume: points of the evicted slabs (NuiKinfuVertexCache); returns their number
  * and copies up to cap points (float3) and colours (float4, may be NULL). */
 int nuifk_shift_cache(void* h, float* points, float* colors, size_t cap);
+/* Hash scene with swapping: blocks held by the host store (NuiKinfuOpenCLHashGlobalCache). */
+int nuifk_hash_stored_blocks(void* h, size_t* out);
 /* VO/NuiTrackerConfig.cpp:26-115 key=value files; -1 if the file cannot be opened. */
 int nuifk_load_tracker_config(const char* path, nuifk_desc* d);
 int nuifk_save_tracker_config(const char* path, const nuifk_desc* d);

@@ -7,6 +7,7 @@
 
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/nuif.h"
@@ -308,6 +309,8 @@ public:
     void Stats(uint32_t out[4]) const;
     unsigned long long UpdatedVoxels() const override;
     static nuif_hash_params ParamsOf(const NuiHashingSDFConfig& config, int storage, bool color);
+    // host swapping (m_bUseSwapping): blocks held by the host store
+    size_t StoredBlocks() const;
 
 private:
     NuiHashingSDFConfig m_config;
@@ -315,6 +318,36 @@ private:
     nuif_stream_t m_stream;
     DeviceBuffer m_scene, m_range, m_counter, m_forwardWork;
     UINT m_rangeW = 0, m_rangeH = 0;
+    std::unique_ptr<class NuiKinfuHIPHashGlobalCache> m_cache;
+};
+
+// OCL/NuiKinfuOpenCLHashGlobalCache.*: the host store of swapped-out bricks (keyed by
+// hash entry) and the transfers of one frame: the swap-in list receives its stored
+// blocks (zeros when none) and merges them into the active bricks, the swap-out list
+// leaves the device into the store.  Synchronous: each list length is read back.
+class NuiKinfuHIPHashGlobalCache {
+public:
+    NuiKinfuHIPHashGlobalCache(const nuif_hash_params& p, nuif_stream_t s);
+    void reset() { m_store.clear(); }
+    // integrateGlobalIntoLocal + saveToGlobalMemory (OCL/NuiKinfuOpenCLHashScene.cpp:576-580)
+    void update(void* scene);
+    size_t StoredBlocks() const { return m_store.size(); }
+
+private:
+    struct Stored {
+        std::vector<uint32_t> words;
+        std::vector<uint8_t> colors;
+    };
+    uint32_t list(void* scene, int which);
+    void grow(uint32_t n);
+    nuif_hash_params m_p;
+    nuif_stream_t m_stream;
+    DeviceBuffer m_scratch, m_ids, m_count, m_words, m_colors;
+    uint32_t m_cap = 0;
+    std::vector<int32_t> m_hostIds;
+    std::vector<uint32_t> m_hostWords;
+    std::vector<uint8_t> m_hostColors;
+    std::unordered_map<int32_t, Stored> m_store;
 };
 
 }  // namespace nuif_kinfu

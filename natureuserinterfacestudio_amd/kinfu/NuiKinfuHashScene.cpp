@@ -28,13 +28,97 @@ NuiKinfuHIPHashScene::NuiKinfuHIPHashScene(const NuiHashingSDFConfig& config, in
     if (!bytes) throw std::runtime_error(std::string("hash scene: ") + nuif_last_error());
     m_scene = DeviceBuffer(bytes);
     m_counter = DeviceBuffer(sizeof(unsigned long long));
+    if (config.m_bUseSwapping) m_cache = std::make_unique<NuiKinfuHIPHashGlobalCache>(m_params, m_stream);
     reset();
 }
 
-// OCL/NuiKinfuOpenCLHashScene.cpp:134-144 (NuiKinfuVoxelBlockHash::reset)
+// OCL/NuiKinfuOpenCLHashScene.cpp:134-144 (NuiKinfuVoxelBlockHash::reset, the global cache's reset)
 void NuiKinfuHIPHashScene::reset() {
     check(nuif_hash_reset(m_scene.get(), &m_params, m_stream), "nuif_hash_reset");
+    if (m_cache) m_cache->reset();
     setDirty();
+}
+
+size_t NuiKinfuHIPHashScene::StoredBlocks() const { return m_cache ? m_cache->StoredBlocks() : 0; }
+
+// ------------------------------------------------------------ global cache
+
+NuiKinfuHIPHashGlobalCache::NuiKinfuHIPHashGlobalCache(const nuif_hash_params& p, nuif_stream_t s)
+    : m_p(p), m_stream(s) {
+    size_t off[8];
+    check(nuif_hash_layout(&m_p, off), "nuif_hash_layout");
+    m_scratch = DeviceBuffer(nuif_hash_swap_scratch_bytes(&m_p));
+    m_ids = DeviceBuffer(off[6] * sizeof(int32_t));
+    m_count = DeviceBuffer(sizeof(uint32_t));
+}
+
+void NuiKinfuHIPHashGlobalCache::grow(uint32_t n) {
+    if (n <= m_cap) return;
+    m_cap = std::max(n, 2 * m_cap);
+    m_words = DeviceBuffer((size_t)m_cap * 512 * sizeof(uint32_t));
+    if (m_p.color) m_colors = DeviceBuffer((size_t)m_cap * 512 * 4);
+}
+
+uint32_t NuiKinfuHIPHashGlobalCache::list(void* scene, int which) {
+    check(nuif_hash_swap_list(scene, &m_p, which, m_ids.as<int32_t>(), m_count.as<uint32_t>(), m_scratch.get(),
+                              m_stream),
+          "nuif_hash_swap_list");
+    uint32_t n = 0;
+    check(nuif_memcpy_d2h(&n, m_count.get(), sizeof(n), m_stream), "swap list");
+    check(nuif_stream_synchronize(m_stream), "swap list");
+    m_hostIds.resize(n);
+    if (n) {
+        check(nuif_memcpy_d2h(m_hostIds.data(), m_ids.get(), (size_t)n * sizeof(int32_t), m_stream), "swap list");
+        check(nuif_stream_synchronize(m_stream), "swap list");
+    }
+    return n;
+}
+
+void NuiKinfuHIPHashGlobalCache::update(void* scene) {
+    // OCL/NuiKinfuOpenCLHashGlobalCache.cpp:212-319 LoadFromGlobalMemory + integrateGlobalIntoLocal
+    uint32_t n = list(scene, NUIF_HASH_SWAP_IN);
+    if (n) {
+        grow(n);
+        m_hostWords.assign((size_t)n * 512, 0u);
+        m_hostColors.assign(m_p.color ? (size_t)n * 512 * 4 : 0, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            auto it = m_store.find(m_hostIds[i]);
+            if (it == m_store.end()) continue;
+            std::copy(it->second.words.begin(), it->second.words.end(), m_hostWords.begin() + (size_t)i * 512);
+            if (m_p.color)
+                std::copy(it->second.colors.begin(), it->second.colors.end(),
+                          m_hostColors.begin() + (size_t)i * 512 * 4);
+        }
+        check(nuif_memcpy_h2d(m_words.get(), m_hostWords.data(), m_hostWords.size() * sizeof(uint32_t), m_stream),
+              "swap-in upload");
+        if (m_p.color)
+            check(nuif_memcpy_h2d(m_colors.get(), m_hostColors.data(), m_hostColors.size(), m_stream),
+                  "swap-in upload");
+        check(nuif_hash_swap_in(scene, &m_p, m_ids.as<int32_t>(), n, m_words.as<uint32_t>(),
+                                m_p.color ? m_colors.as<uint8_t>() : nullptr, m_stream),
+              "nuif_hash_swap_in");
+    }
+    // :321-538 BuildSwapOutList + MoveActiveDataToTransferBuffer + CleanVoxelMemory + saveToGlobalMemory
+    const uint32_t m = list(scene, NUIF_HASH_SWAP_OUT);
+    if (!m) return;
+    grow(m);
+    check(nuif_hash_swap_out(scene, &m_p, m_ids.as<int32_t>(), m, m_words.as<uint32_t>(),
+                             m_p.color ? m_colors.as<uint8_t>() : nullptr, m_stream),
+          "nuif_hash_swap_out");
+    m_hostWords.resize((size_t)m * 512);
+    m_hostColors.resize(m_p.color ? (size_t)m * 512 * 4 : 0);
+    check(nuif_memcpy_d2h(m_hostWords.data(), m_words.get(), m_hostWords.size() * sizeof(uint32_t), m_stream),
+          "swap-out read");
+    if (m_p.color)
+        check(nuif_memcpy_d2h(m_hostColors.data(), m_colors.get(), m_hostColors.size(), m_stream), "swap-out read");
+    check(nuif_stream_synchronize(m_stream), "swap-out read");
+    for (uint32_t i = 0; i < m; ++i) {
+        Stored& st = m_store[m_hostIds[i]];
+        st.words.assign(m_hostWords.begin() + (size_t)i * 512, m_hostWords.begin() + (size_t)(i + 1) * 512);
+        if (m_p.color)
+            st.colors.assign(m_hostColors.begin() + (size_t)i * 512 * 4,
+                             m_hostColors.begin() + (size_t)(i + 1) * 512 * 4);
+    }
 }
 
 // OCL/NuiKinfuOpenCLHashScene.cpp:533-583 integrateVolume: AllocateSceneFromDepth +
@@ -45,14 +129,23 @@ bool NuiKinfuHIPHashScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCamera
     if (!f || !cam) return false;
     if (f->IsDepthScaled()) throw std::runtime_error("hash scene: the frame depth is scaled to ray length");
     const NuiCLCameraParams& c = cam->GetCameraParams();
+    if (m_cache && m_gateIntegrate) {  // the host side of swapping runs only for integrated frames
+        int32_t gate = 0;
+        check(nuif_memcpy_d2h(&gate, m_gateIntegrate, sizeof(gate), m_stream), "gate read");
+        check(nuif_stream_synchronize(m_stream), "gate read");
+        if (!gate) return true;
+    }
     check(nuif_hash_allocate(m_scene.get(), &m_params, f->GetDepthBuffer(), &c, cam->GetCameraTransformBuffer(),
                              m_gateIntegrate, m_stream),
           "nuif_hash_allocate");
+    if (m_cache)
+        check(nuif_hash_swap_prepare(m_scene.get(), &m_params, m_gateIntegrate, m_stream), "nuif_hash_swap_prepare");
     const uint8_t* colors = m_params.color ? f->GetColorBuffer() : nullptr;
     check(nuif_hash_integrate(m_scene.get(), &m_params, f->GetDepthBuffer(), colors, &c,
                               cam->GetCameraTransformBuffer(), m_gateIntegrate,
                               m_counter.as<unsigned long long>(), m_stream),
           "nuif_hash_integrate");
+    if (m_cache) m_cache->update(m_scene.get());  // OCL/NuiKinfuOpenCLHashScene.cpp:576-580
     setDirty();
     return true;
 }

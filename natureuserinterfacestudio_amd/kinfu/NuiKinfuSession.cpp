@@ -37,6 +37,7 @@ std::unique_ptr<NuiKinfuSession> NuiKinfuSession::create(const nuifk_desc& d) {
         throw std::runtime_error("session: unknown hash_forward_render mode");
     if (d.scene == NUIFK_SCENE_HASH) {
         s->scene = std::make_unique<NuiKinfuHIPHashScene>(hashConfig(d), d.storage, d.color != 0, s->engine->stream());
+        if (d.hash_swapping) s->engine->setUseGraph(false);  // the host store waits for each frame's lists
     } else if (d.scene == NUIFK_SCENE_DENSE) {
         s->scene = std::make_unique<NuiKinfuHIPScene>(vc, d.storage, s->engine->stream());
     } else if (d.scene == NUIFK_SCENE_SHIFT) {
@@ -76,6 +77,7 @@ NuiHashingSDFConfig NuiKinfuSession::hashConfig(const nuifk_desc& d) {
     c.m_numSDFBlocks = (UINT)d.hash_bricks;
     c.m_bUseForwardRender = d.hash_forward_render != NUIFK_RENDER_CASTRAY;
     c.m_bForwardReferenceProjection = d.hash_forward_render == NUIFK_RENDER_FORWARD_REFERENCE;
+    c.m_bUseSwapping = d.hash_swapping != 0;
     return c;
 }
 
@@ -86,6 +88,7 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
         const nuif_hash_params hp = NuiKinfuHIPHashScene::ParamsOf(hashConfig(d), d.storage, d.color != 0);
         bytes += nuif_hash_scene_bytes(&hp) + n * 8;  // + ray-range image
         if (d.hash_forward_render != NUIFK_RENDER_CASTRAY) bytes += nuif_hash_forward_render_bytes(d.width, d.height);
+        if (d.hash_swapping) bytes += nuif_hash_swap_scratch_bytes(&hp) + (1u << 20) * 4;
     } else {  // dense or shifting volume
         const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
         if (d.scene == NUIFK_SCENE_SHIFT) bytes += ((size_t)d.resolution[0] * d.resolution[1] + 64) * 4 + 65536;

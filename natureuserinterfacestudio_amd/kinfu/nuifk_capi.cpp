@@ -131,6 +131,15 @@ int nuifk_shift_cache(void* h, float* points, float* colors, size_t cap) {
     });
 }
 
+int nuifk_hash_stored_blocks(void* h, size_t* out) {
+    return guarded([&] {
+        auto* hs = dynamic_cast<NuiKinfuHIPHashScene*>(static_cast<Session*>(h)->scene.get());
+        if (!hs || !out) throw std::invalid_argument("nuifk_hash_stored_blocks: not a hash scene");
+        *out = hs->StoredBlocks();
+        return 0;
+    });
+}
+
 int nuifk_state(void* h, nuif_icp_result* out) {
     return guarded([&] {
         *out = static_cast<Session*>(h)->engine->tracker()->ReadState();

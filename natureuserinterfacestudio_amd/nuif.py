@@ -80,7 +80,7 @@ class KinfuDesc(C.Structure):
                 ("use_graph", c_int), ("icp_reduction", c_int), ("scene", c_int), ("hash_voxel_size", c_float),
                 ("hash_trunc_scale", c_float), ("hash_truncation", c_float), ("hash_weight_max", c_int),
                 ("hash_bricks", c_int), ("hash_forward_render", c_int), ("voxel_shift", c_int),
-                ("volume_basis", c_float * 3)]
+                ("volume_basis", c_float * 3), ("hash_swapping", c_int)]
 
 
 SCENE_DENSE, SCENE_HASH, SCENE_SHIFT = 0, 1, 2
@@ -265,6 +265,7 @@ _KSIGS = {
     "nuifk_save_tracker_config": (c_int, [C.c_char_p, C.POINTER(KinfuDesc)]),
     "nuifk_shift_state": (c_int, [c_void_p, c_void_p, c_void_p]),
     "nuifk_shift_cache": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "nuifk_hash_stored_blocks": (c_int, [c_void_p, C.POINTER(c_size_t)]),
     "nuifs_create": (c_int, [c_size_t, C.POINTER(c_void_p)]),
     "nuifs_destroy": (c_int, [c_void_p]),
     "nuifs_session_bytes": (c_size_t, [C.POINTER(KinfuDesc)]),

@@ -86,7 +86,7 @@ class Workload:
         return nuif.camera_params(self.width, self.height, self.fx, self.fy, self.cx, self.cy, 0.5, 4.5)
 
     def kinfu_desc(self, storage=nuif.TSDF_S16, gate=0.0, color=False, use_graph=False, iterations=None,
-                   reduction=None, bricks=1 << 18, forward=0, shift=0, basis=(0.0, 0.0, 0.0)):
+                   reduction=None, bricks=1 << 18, forward=0, shift=0, basis=(0.0, 0.0, 0.0), swapping=False):
         d = nuif.KinfuDesc()
         d.width, d.height = self.width, self.height
         d.fx, d.fy, d.cx, d.cy = self.fx, self.fy, self.cx, self.cy
@@ -115,6 +115,7 @@ class Workload:
         d.icp_reduction = reduction
         # voxel hashing (scene_kind 1 workloads): VO/NuiHashingSDFConfig.cpp:3-15 values
         d.scene = nuif.SCENE_HASH if self.volume_res == 0 else nuif.SCENE_DENSE
+        d.hash_swapping = 1 if swapping else 0
         if shift:  # the shifting dense volume (OCL/NuiKinfuOpenCLShiftScene)
             d.scene = nuif.SCENE_SHIFT
             d.voxel_shift = shift

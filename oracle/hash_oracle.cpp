@@ -272,6 +272,19 @@ struct HashScene {
         }
     }
 
+    // OCL/NuiKinfuOpenCLHashScene.cpp:533-583 integrateVolume: allocation and visible list,
+    // integration, and with swapping the global cache's swap-in and swap-out (:576-580)
+    long long integrate_frame(const float* depth, const uint8_t* bgra, const Cam<float>& cam, const Rigid<float>& g) {
+        allocate_and_select(depth, cam, g);
+        if (swapping) swap_prepare();
+        const long long n = integrate(depth, bgra, cam, g);
+        if (swapping) {
+            swap_in();
+            swap_out();
+        }
+        return n;
+    }
+
     // K/KinfuHashScene.cl:387-470 integrateIntoScene_kernel over the visible list
     long long integrate(const float* depth, const uint8_t* bgra, const Cam<float>& cam, const Rigid<float>& g) {
         long long nupd = 0;
@@ -728,8 +741,7 @@ struct HashEngine {
     }
 
     void integrate(const uint8_t* bgra, const Rigid<float>& pose) {
-        scene.allocate_and_select(depth.data(), cam, pose);
-        nupd_last = scene.integrate(depth.data(), bgra, cam, pose);
+        nupd_last = scene.integrate_frame(depth.data(), bgra, cam, pose);
     }
 
     int run(const uint16_t* raw, const uint8_t* bgra) {
@@ -897,17 +909,11 @@ OR_EXPORT long long or_hash_scene_integrate(void* sp, const float* depth, const 
     HashScene* s = (HashScene*)sp;
     const Cam<float> cam = cam_of(cam6, W, H);
     const Rigid<float> g = make_rigid<float>(R9, t3);
-    s->allocate_and_select(depth, cam, g);
-    if (s->swapping) s->swap_prepare();
-    const long long n = s->integrate(depth, bgra, cam, g);
-    if (s->swapping) {  // OCL/NuiKinfuOpenCLHashScene.cpp:576-580
-        s->swap_in();
-        s->swap_out();
-    }
-    return n;
+    return s->integrate_frame(depth, bgra, cam, g);
 }
 
 OR_EXPORT void or_hash_scene_set_swapping(void* sp, int on) { ((HashScene*)sp)->swapping = on != 0; }
+OR_EXPORT void or_hash_engine_set_swapping(void* hp, int on) { ((HashEngine*)hp)->scene.swapping = on != 0; }
 
 // Swap states of the blocks with an entry (sorted by key) and the stored blocks (sorted by key).
 OR_EXPORT int or_hash_scene_swap_states(void* sp, int16_t* keys, uint8_t* states, long long cap) {

@@ -85,6 +85,7 @@ def lib():
             "or_hash_scene_visible": (ci, [vp, vp, cll]),
             "or_hash_scene_mesh": (cll, [vp, i32p, i32p, vp, cll]),
             "or_hash_scene_set_swapping": (None, [vp, ci]),
+            "or_hash_engine_set_swapping": (None, [vp, ci]),
             "or_hash_scene_swap_states": (ci, [vp, vp, vp, cll]),
             "or_hash_scene_store": (ci, [vp, vp, vp, vp, vp, cll]),
             "or_color_half_sample": (None, [u8p, u8p, ci, ci, ci]),
@@ -320,12 +321,14 @@ class OracleHashEngine(_HashSceneView):
     render and the reference's camera->world projection."""
 
     def __init__(self, wl, gate=0.0, voxel=0.005, trunc_scale=0.02, truncation=0.06, wmax=255, capacity=1 << 18,
-                 iterations=None, forward=0):
+                 iterations=None, forward=0, swapping=False):
         its = list(iterations or wl.iterations)
         self.wl = wl
         self.h = lib().or_hash_engine_create(wl.width, wl.height, wl.cam6(), 3, 0.08, 50.0, 0.3, 0.1, len(its),
                                              np.array(its + [0] * 8, np.int32)[:8], gate, voxel, trunc_scale,
                                              truncation, wmax, capacity, int(forward))
+        if swapping:
+            lib().or_hash_engine_set_swapping(self.h, 1)
 
     def _scene(self):
         return lib().or_hash_engine_scene(self.h)

@@ -93,3 +93,29 @@ def test_swapping_scene_matches_oracle(gpu):
         else:
             assert len(o.store()[0]) == 0
     assert swapped_out > 100 and swapped_back > 100, (swapped_out, swapped_back)
+
+
+def test_swapping_engine_matches_oracle_engine(gpu):
+    """RunTracking over a hash scene with swapping (m_bUseSwapping): poses bit for bit every
+    frame, and at the end the blocks holding bricks and the size of the host store."""
+    import ctypes as C
+    n = 12
+    announce(f"hash engine with swapping 320x240, {n} frames vs oracle hash engine")
+    desc = WL.kinfu_desc(bricks=1 << 17, swapping=True)
+    eng = nuif.Engine(desc)
+    ref = O.OracleHashEngine(WL, capacity=1 << 17, swapping=True)
+    for f in range(n):
+        d = WL.render(f)[0]
+        assert eng.run(d) == 1
+        ok, pr, _ = ref.run(d)
+        assert ok == 1
+        np.testing.assert_array_equal(eng.poses(f + 1)[f, :12], pr.astype(np.float32), err_msg=f"frame {f}")
+    keys, _, w, words, _ = nuif.decode_hash_blocks(eng.volume_raw(), nuif.hash_params_of_desc(desc))
+    okeys, osdf, ow, _ = ref.blocks()
+    np.testing.assert_array_equal(keys, okeys)
+    np.testing.assert_array_equal((words & 0xFFFF).astype(np.uint16).view(np.int16), osdf)
+    np.testing.assert_array_equal(w, ow.astype(np.int32))
+    stored = C.c_size_t()
+    assert nuif.K().nuifk_hash_stored_blocks(eng._h, C.byref(stored)) == 0
+    assert stored.value == len(ref.store()[0])
+    announce(f"stored blocks after {n} frames: {stored.value}")
