@@ -556,7 +556,7 @@ struct HashScene {
     //  * genericRaycastMissingPoints_kernel (:531-559) runs castRay (no normal) on the
     //    flagged pixels;
     //  * Vertex2Normal (:943-989) recomputes the normals from the vertex map (threshold 0.3).
-    // reference_transform selects the reference's projection (Q9): camera->world applied
+    // reference_transform selects the reference's projection (K/KinfuHashRaycast.cl:473-506): camera->world applied
     // to world points, scaled by the voxel size, pixel = (uint)(p + 0.5) with negative and
     // NaN values converting to 0 (the saturating float->uint conversion); otherwise the
     // world->camera transform, z > 0 and p + 0.5 >= 0.  Where several points land on one
