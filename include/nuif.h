@@ -250,12 +250,19 @@ int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, con
  * wrap: int32[3] voxel wrap (any integers, reduced modulo the resolution): the voxel at
  * logical (x, y, z) is stored at ((x + w.x) % Nx, (y + w.y) % Ny, (z + w.z) % Nz)
  * (K/tsdfVolume.cl:71,104, K/raycast.cl:62,184); NULL or 0 = the plain volume. */
-/* nuif_tsdf_integrate in a wrapped volume; the surface blocks are marked at stored positions */
+/* Device bytes of the walk-state workspace of nuif_tsdf_integrate_wrapped (16 B per column
+ * and z segment after the first). */
+size_t nuif_tsdf_walk_states_bytes(const TsdfParams* params);
+/* nuif_tsdf_integrate in a wrapped volume; the surface blocks are marked at stored positions.
+ * walk_states: optional workspace (nuif_tsdf_walk_states_bytes) -- the integration then
+ * computes every column's incremental walk state at the start of each z segment once, instead
+ * of replaying it per segment (same values). */
 int nuif_tsdf_integrate_wrapped(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                                 const TsdfParams* params, const NuiCLCameraParams* cam,
                                 const NuiCLRigidTransform* pose, int storage, int max_color_weight,
                                 const int32_t* wrap, const int32_t* gate, unsigned long long* updated,
-                                const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s);
+                                const float* depth_bounds, uint8_t* surface_blocks, float* walk_states,
+                                nuif_stream_t s);
 /* nuif_tsdf_raycast_gated in a wrapped volume.  Every trilinear corner is wrapped on its own
  * (the reference wraps the first corner only); the surface-block skip is used only when the
  * wrap is 0. */

@@ -294,6 +294,28 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
     return dmax + trunc < dmin * (1.0f - 1e-5f) - 2e-4f;
 }
 
+// The walk state of every column at the start of z segments 1, 2, ...: one thread
+// per column runs the additions of K/tsdfVolume.cl:74-189 once, instead of every
+// segment's wavefront replaying them from z = 0.
+__global__ void __launch_bounds__(256)
+    column_states_kernel(TsdfParams p, NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
+                         const int32_t* __restrict__ gate, float4* __restrict__ walk_states) {
+    if (gate && *gate == 0) return;
+    const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nx * Ny) return;
+    const int x = i % Nx, y = i / Nx;
+    const NuiCLRigidTransform g = *pose;
+    ColumnWalk c = column_setup(x, y, Nx, Ny, p, cam, g);
+    const float cz = p.cell_size[2];
+    const int nseg = (Nz + kSegVoxels - 1) / kSegVoxels;
+    for (int k = 1; k < nseg; ++k) {
+#pragma unroll 1
+        for (int j = 0; j < kSegVoxels; ++j) column_advance(c, cz);
+        walk_states[(size_t)(k - 1) * Nx * Ny + i] = make_float4(c.vgz, c.zs, c.vx, c.vy);
+    }
+}
+
 // Fast path: Nx, Ny multiples of 8, Nz a multiple of the column chunk.
 template <class S, bool COLOR>
 __global__ void __launch_bounds__(64)
@@ -301,7 +323,8 @@ __global__ void __launch_bounds__(64)
                           typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
                           NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
                           const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated,
-                          const float* __restrict__ bounds, SurfaceBlocks surf, VoxelWrap wr) {
+                          const float* __restrict__ bounds, SurfaceBlocks surf, VoxelWrap wr,
+                          const float4* __restrict__ walk_states) {
     if (gate && *gate == 0) return;
     using word = typename S::word;
     constexpr int VPL = 16 / S::kBytes;      // voxels per lane per 16-byte vector
@@ -324,11 +347,20 @@ __global__ void __launch_bounds__(64)
 
     const int sub = lane % LPC;  // position inside a column chunk (phase 2)
     unsigned nupd = 0;
-    // z segment of this wave (blockIdx.z): the walk state at its start is replayed
-    // with the same sequence of additions, so every voxel sees the same values
+    // z segment of this wave (blockIdx.z): the walk state at its start is the one the
+    // same sequence of additions reaches, read from the per-column states of
+    // column_states_kernel or replayed here
     const int zbeg = blockIdx.z * kSegVoxels, zend = min(Nz, zbeg + kSegVoxels);
+    if (zbeg > 0 && walk_states) {
+        const float4 w = walk_states[(size_t)(blockIdx.z - 1) * Nx * Ny + (size_t)y * Nx + x];
+        c.vgz = w.x;
+        c.zs = w.y;
+        c.vx = w.z;
+        c.vy = w.w;
+    } else {
 #pragma unroll 1
-    for (int k = 0; k < zbeg; ++k) column_advance(c, cz);
+        for (int k = 0; k < zbeg; ++k) column_advance(c, cz);
+    }
     for (int zc = zbeg; zc < zend; zc += kZChunk) {
         // phase 1: serial walk of this lane's column; chunks that cannot project
         // into the image only advance the walk
@@ -500,10 +532,11 @@ __device__ __forceinline__ void get_voxel(f3 pt, const TsdfParams& p, int g[3]) 
 // to_int(roundf(x / c)) without the division where its rounding is decided anyway:
 // a = x * rc (rc = 1 / c rounded) is within 2^-22 |x / c| of the rounded quotient, so
 // when a lies farther than 2^-20 max(|a|, 1) from every half-integer both round to the
-// same integer; otherwise (or for NaN / huge values) the quotient is computed.
+// same integer; otherwise (or for NaN / huge values) the quotient is computed.  Away
+// from half-integers round-half-to-even (one instruction) and round-half-away agree.
 __device__ __forceinline__ int round_div(float x, float c, float rc) {
     const float a = x * rc;
-    const float r = roundf(a);
+    const float r = rintf(a);
     const float d = fabsf(fabsf(a - r) - 0.5f);
     if (d > 9.5367431640625e-7f * fmaxf(fabsf(a), 1.0f)) return to_int(r);
     return to_int(roundf(x / c));
@@ -796,7 +829,7 @@ template <class S>
 int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                      const TsdfParams& p, const NuiCLCameraParams& cam, const NuiCLRigidTransform* pose, int maxcw,
                      const int32_t* gate, unsigned long long* updated, const float* bounds, uint8_t* surface,
-                     VoxelWrap wr, hipStream_t st) {
+                     VoxelWrap wr, float* walk_states, hipStream_t st) {
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const SurfaceBlocks surf = surface_blocks_of(surface, Nx, Ny, Nz);
     const bool color = bgra && color_volume;
@@ -807,12 +840,17 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
     // the voxels per vector
     if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0 && wr.z % (16 / S::kBytes) == 0) {
         dim3 grid(Nx / 8, Ny / 8, div_up(Nz, kSegVoxels));
+        float4* ws = grid.z > 1 ? reinterpret_cast<float4*>(walk_states) : nullptr;
+        if (ws) {
+            column_states_kernel<<<div_up(Nx * Ny, 256), 256, 0, st>>>(p, cam, pose, gate, ws);
+            NUIF_LAUNCHED("column_states_kernel");
+        }
         if (color)
             integrate_tile_kernel<S, true><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                 bounds, surf, wr);
+                                                                 bounds, surf, wr, ws);
         else
             integrate_tile_kernel<S, false><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                  bounds, surf, wr);
+                                                                  bounds, surf, wr, ws);
         NUIF_LAUNCHED("integrate_tile_kernel");
     } else {
         dim3 block(16, 16), grid(div_up(Nx, 16), div_up(Ny, 16));
@@ -923,11 +961,19 @@ static VoxelWrap wrap_of(const int32_t* w, const TsdfParams& p) {
     return r;
 }
 
+size_t nuif_tsdf_walk_states_bytes(const TsdfParams* params) {
+    if (!params || check_params(params)) return 0;
+    const size_t nseg = (size_t)div_up((int)params->resolution[2], kSegVoxels);
+    return nseg > 1 ? (nseg - 1) * (size_t)params->resolution[0] * (size_t)params->resolution[1] * sizeof(float4)
+                    : 16;
+}
+
 int nuif_tsdf_integrate_wrapped(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                                 const TsdfParams* params, const NuiCLCameraParams* cam,
                                 const NuiCLRigidTransform* pose, int storage, int max_color_weight,
                                 const int32_t* wrap, const int32_t* gate, unsigned long long* updated,
-                                const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s) {
+                                const float* depth_bounds, uint8_t* surface_blocks, float* walk_states,
+                                nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     int rc = check_params(params);
     if (rc) return rc;
@@ -938,13 +984,13 @@ int nuif_tsdf_integrate_wrapped(const float* depth, const uint8_t* bgra, void* v
     switch (storage) {
         case NUIF_TSDF_S16:
             return launch_integrate<StS16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
-                                          gate, updated, depth_bounds, surface_blocks, wr, st);
+                                          gate, updated, depth_bounds, surface_blocks, wr, walk_states, st);
         case NUIF_TSDF_F16:
             return launch_integrate<StF16>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
-                                          gate, updated, depth_bounds, surface_blocks, wr, st);
+                                          gate, updated, depth_bounds, surface_blocks, wr, walk_states, st);
         case NUIF_TSDF_F32:
             return launch_integrate<StF32>(depth, bgra, volume, color_volume, *params, *cam, pose, max_color_weight,
-                                          gate, updated, depth_bounds, surface_blocks, wr, st);
+                                          gate, updated, depth_bounds, surface_blocks, wr, walk_states, st);
         default: set_error("nuif_tsdf_integrate: unknown storage %d", storage); return NUIF_ERR_INVALID;
     }
 }
@@ -954,7 +1000,8 @@ int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, u
                         int storage, int max_color_weight, const int32_t* gate, unsigned long long* updated,
                         const float* depth_bounds, uint8_t* surface_blocks, nuif_stream_t s) {
     return nuif_tsdf_integrate_wrapped(depth, bgra, volume, color_volume, params, cam, pose, storage,
-                                       max_color_weight, nullptr, gate, updated, depth_bounds, surface_blocks, s);
+                                       max_color_weight, nullptr, gate, updated, depth_bounds, surface_blocks, nullptr,
+                                       s);
 }
 
 int nuif_tsdf_reset_slice(void* volume, uint8_t* color_volume, const TsdfParams* params, int storage,

@@ -255,6 +255,7 @@ NuiKinfuHIPScene::NuiKinfuHIPScene(const NuiKinfuVolumeConfig& config, int stora
     if (config.bHas_color_volume) m_color = DeviceBuffer(n * 4);
     m_counter = DeviceBuffer(sizeof(unsigned long long));
     m_surface = DeviceBuffer(nuif_tsdf_surface_blocks_bytes(&m_params));
+    m_walk = DeviceBuffer(nuif_tsdf_walk_states_bytes(&m_params));
     reset();
 }
 
@@ -293,7 +294,7 @@ bool NuiKinfuHIPScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraStat
                                       colors ? m_color.as<uint8_t>() : nullptr, &m_params, &c,
                                       cam->GetCameraTransformBuffer(), m_storage, m_config.max_color_weight, m_wrap,
                                       m_gateIntegrate, m_counter.as<unsigned long long>(), bounds,
-                                      m_surface.as<uint8_t>(), m_stream),
+                                      m_surface.as<uint8_t>(), m_walk.as<float>(), m_stream),
           "nuif_tsdf_integrate");
     setDirty();
     return true;

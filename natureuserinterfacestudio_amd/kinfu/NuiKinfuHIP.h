@@ -245,6 +245,7 @@ protected:
     DeviceBuffer m_volume, m_color, m_counter;
     DeviceBuffer m_surface;  // surface blocks of the volume (nuif_tsdf_surface_blocks_bytes)
     int32_t m_wrap[3] = {0, 0, 0};  // voxel wrap (non-zero only in a shifting volume)
+    DeviceBuffer m_walk;            // per-column walk states of the integration (nuif_tsdf_walk_states_bytes)
 };
 
 // OCL/NuiKinfuOpenCLShiftScene.*: the dense volume as a cyclic buffer that follows the

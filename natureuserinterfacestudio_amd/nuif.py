@@ -165,9 +165,10 @@ _SIGS = {
     "nuif_mc_scratch_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
     "nuif_mc_dense": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), c_int, c_void_p, c_void_p, C.c_uint32,
                               c_void_p, c_void_p, c_void_p]),
+    "nuif_tsdf_walk_states_bytes": (c_size_t, [C.POINTER(TsdfParams)]),
     "nuif_tsdf_integrate_wrapped": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(TsdfParams),
                                             C.POINTER(NuiCLCameraParams), c_void_p, c_int, c_int, c_void_p, c_void_p,
-                                            c_void_p, c_void_p, c_void_p, c_void_p]),
+                                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nuif_tsdf_raycast_wrapped": (c_int, [c_void_p, c_void_p, C.POINTER(TsdfParams), C.POINTER(NuiCLCameraParams),
                                           c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                           c_void_p]),

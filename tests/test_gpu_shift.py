@@ -77,7 +77,7 @@ def test_wrapped_integration_and_raycast_match_the_rolled_volume(gpu, wrap):
         for dv, dc, ds, ww in zip(vols, cols, surfs, (None, w)):
             nuif.check(L.nuif_tsdf_integrate_wrapped(ddep.ptr, dbg.ptr, dv.ptr, dc.ptr, C.byref(p), C.byref(cam),
                                                      pose.ptr, 0, 128, ww.ctypes.data if ww is not None else None,
-                                                     None, None, None, ds.ptr, None))
+                                                     None, None, None, ds.ptr, None, None))
     plain, wrapped = vols[0].download().view(np.int16), vols[1].download().view(np.int16)
     assert (plain[1::2] > 0).sum() > 10000
     np.testing.assert_array_equal(wrapped, _roll(plain, N, wrap, 2))
@@ -120,7 +120,7 @@ def test_wrapped_f32_tile_path_matches_rolled(gpu):
     for dv, ww in zip(dvs, (None, w)):
         nuif.check(L.nuif_tsdf_integrate_wrapped(ddep.ptr, None, dv.ptr, None, C.byref(p), C.byref(cam), pose.ptr, 2,
                                                  128, ww.ctypes.data if ww is not None else None, None, None, None,
-                                                 None, None))
+                                                 None, None, None))
     plain, wrapped = dvs[0].download().view(np.uint32), dvs[1].download().view(np.uint32)
     np.testing.assert_array_equal(wrapped, _roll(plain, N, wrap, 2))
 
@@ -205,3 +205,43 @@ def test_shift_engine_matches_oracle_engine(gpu, shift, color):
     assert len(rp) > 100
     np.testing.assert_array_equal(gp.view(np.uint32), rp.view(np.uint32))
     np.testing.assert_array_equal(gc.view(np.uint32), rc.view(np.uint32))
+
+
+@pytest.mark.parametrize("N,storage,wrap", [(256, 0, (0, 0, 0)), (384, 0, (5, 7, 16)), (256, 2, (3, 250, 2))])
+def test_walk_state_workspace_matches_the_replayed_walk(gpu, N, storage, wrap):
+    """The per-column walk states computed once per z segment give the same volume,
+    colour volume and updated-voxel count as every segment replaying the walk."""
+    wl = W.SMALL
+    announce(f"walk-state workspace vs replay: storage {storage}, {N}^3, wrap {wrap}, 3 frames")
+    L = gpu.L()
+    p = nuif.tsdf_params(N, wl.volume_dim * N / 128, wl.trunc)
+    cam = wl.camera_params()
+    w = np.array(wrap, np.int32)
+    ws_bytes = L.nuif_tsdf_walk_states_bytes(C.byref(p))
+    assert ws_bytes == ((N + 127) // 128 - 1) * N * N * 16
+    ws = nuif.DeviceArray((ws_bytes,), np.uint8)
+    vb = N ** 3 * L.nuif_tsdf_voxel_bytes(storage)
+    color = storage == 0
+    out = []
+    for walk in (None, ws.ptr):
+        dv = nuif.DeviceArray((vb,), np.uint8)
+        dc = nuif.DeviceArray((N ** 3 * 4,), np.uint8) if color else None
+        cnt = nuif.DeviceArray.from_host(np.zeros(1, np.uint64))
+        nuif.check(L.nuif_tsdf_reset(dv.ptr, dc.ptr if color else None, C.byref(p), storage, None))
+        for k, f in enumerate([0, 5, 11]):
+            dep, bgra = _depth(wl, f)
+            a = np.deg2rad(-1.5 * f)
+            R = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]], np.float32).T.copy()
+            t = np.array([0.03 * k, -0.01 * k, -0.6 + 0.02 * k], np.float32)
+            pose = nuif.upload_struct(nuif.rigid(R, t))
+            ddep, dbg = nuif.DeviceArray.from_host(dep), nuif.DeviceArray.from_host(bgra)
+            nuif.check(L.nuif_tsdf_integrate_wrapped(ddep.ptr, dbg.ptr if color else None, dv.ptr,
+                                                     dc.ptr if color else None, C.byref(p), C.byref(cam), pose.ptr,
+                                                     storage, 128, w.ctypes.data, None, cnt.ptr, None, None, walk,
+                                                     None))
+        nuif.check(L.nuif_stream_synchronize(None))
+        out.append((dv.download(), dc.download() if color else None, int(cnt.download()[0])))
+    assert out[0][2] == out[1][2] > 20000
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    if color:
+        np.testing.assert_array_equal(out[0][1], out[1][1])
