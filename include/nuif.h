@@ -250,8 +250,8 @@ int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, con
  * wrap: int32[3] voxel wrap (any integers, reduced modulo the resolution): the voxel at
  * logical (x, y, z) is stored at ((x + w.x) % Nx, (y + w.y) % Ny, (z + w.z) % Nz)
  * (K/tsdfVolume.cl:71,104, K/raycast.cl:62,184); NULL or 0 = the plain volume. */
-/* Device bytes of the walk-state workspace of nuif_tsdf_integrate_wrapped (16 B per column
- * and z segment after the first). */
+/* Device bytes of the walk-state workspace of nuif_tsdf_integrate_wrapped (8 B per column
+ * and z segment after the first, plus 8 B per such segment). */
 size_t nuif_tsdf_walk_states_bytes(const TsdfParams* params);
 /* nuif_tsdf_integrate in a wrapped volume; the surface blocks are marked at stored positions.
  * walk_states: optional workspace (nuif_tsdf_walk_states_bytes) -- the integration then

@@ -294,25 +294,43 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
     return dmax + trunc < dmin * (1.0f - 1e-5f) - 2e-4f;
 }
 
-// The walk state of every column at the start of z segments 1, 2, ...: one thread
-// per column runs the additions of K/tsdfVolume.cl:74-189 once, instead of every
-// segment's wavefront replaying them from z = 0.
+// The walk state of every column at the start of z segments 1, 2, ...: the
+// additions of K/tsdfVolume.cl:74-189 run once here, instead of every segment's
+// wavefront replaying them from z = 0.  The projection numerators (vx, vy) are
+// per column, float2[(k - 1) * Nx * Ny + column]; the z terms (vgz, zs) are the
+// same for every column and follow as float2[(nseg - 1) * Nx * Ny + k - 1],
+// written by column 0.
 __global__ void __launch_bounds__(256)
     column_states_kernel(TsdfParams p, NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
-                         const int32_t* __restrict__ gate, float4* __restrict__ walk_states) {
+                         const int32_t* __restrict__ gate, float2* __restrict__ walk_states) {
     if (gate && *gate == 0) return;
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nx * Ny) return;
     const int x = i % Nx, y = i / Nx;
     const NuiCLRigidTransform g = *pose;
-    ColumnWalk c = column_setup(x, y, Nx, Ny, p, cam, g);
+    const ColumnWalk c0 = column_setup(x, y, Nx, Ny, p, cam, g);
     const float cz = p.cell_size[2];
     const int nseg = (Nz + kSegVoxels - 1) / kSegVoxels;
+    float vx = c0.vx, vy = c0.vy;
     for (int k = 1; k < nseg; ++k) {
-#pragma unroll 1
-        for (int j = 0; j < kSegVoxels; ++j) column_advance(c, cz);
-        walk_states[(size_t)(k - 1) * Nx * Ny + i] = make_float4(c.vgz, c.zs, c.vx, c.vy);
+#pragma unroll 16
+        for (int j = 0; j < kSegVoxels; ++j) {
+            vx += c0.incx;
+            vy += c0.incy;
+        }
+        walk_states[(size_t)(k - 1) * Nx * Ny + i] = make_float2(vx, vy);
+    }
+    if (i == 0) {
+        float vgz = c0.vgz, zs = c0.zs;
+        for (int k = 1; k < nseg; ++k) {
+#pragma unroll 16
+            for (int j = 0; j < kSegVoxels; ++j) {
+                vgz += cz;
+                zs += cz;
+            }
+            walk_states[(size_t)(nseg - 1) * Nx * Ny + k - 1] = make_float2(vgz, zs);
+        }
     }
 }
 
@@ -324,7 +342,7 @@ __global__ void __launch_bounds__(64)
                           NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
                           const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated,
                           const float* __restrict__ bounds, SurfaceBlocks surf, VoxelWrap wr,
-                          const float4* __restrict__ walk_states) {
+                          const float2* __restrict__ walk_states) {
     if (gate && *gate == 0) return;
     using word = typename S::word;
     constexpr int VPL = 16 / S::kBytes;      // voxels per lane per 16-byte vector
@@ -352,11 +370,12 @@ __global__ void __launch_bounds__(64)
     // column_states_kernel or replayed here
     const int zbeg = blockIdx.z * kSegVoxels, zend = min(Nz, zbeg + kSegVoxels);
     if (zbeg > 0 && walk_states) {
-        const float4 w = walk_states[(size_t)(blockIdx.z - 1) * Nx * Ny + (size_t)y * Nx + x];
-        c.vgz = w.x;
-        c.zs = w.y;
-        c.vx = w.z;
-        c.vy = w.w;
+        const float2 w = walk_states[(size_t)(blockIdx.z - 1) * Nx * Ny + (size_t)y * Nx + x];
+        const float2 z = walk_states[(size_t)(gridDim.z - 1) * Nx * Ny + blockIdx.z - 1];
+        c.vx = w.x;
+        c.vy = w.y;
+        c.vgz = z.x;
+        c.zs = z.y;
     } else {
 #pragma unroll 1
         for (int k = 0; k < zbeg; ++k) column_advance(c, cz);
@@ -840,7 +859,7 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
     // the voxels per vector
     if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0 && wr.z % (16 / S::kBytes) == 0) {
         dim3 grid(Nx / 8, Ny / 8, div_up(Nz, kSegVoxels));
-        float4* ws = grid.z > 1 ? reinterpret_cast<float4*>(walk_states) : nullptr;
+        float2* ws = grid.z > 1 ? reinterpret_cast<float2*>(walk_states) : nullptr;
         if (ws) {
             column_states_kernel<<<div_up(Nx * Ny, 256), 256, 0, st>>>(p, cam, pose, gate, ws);
             NUIF_LAUNCHED("column_states_kernel");
@@ -964,7 +983,7 @@ static VoxelWrap wrap_of(const int32_t* w, const TsdfParams& p) {
 size_t nuif_tsdf_walk_states_bytes(const TsdfParams* params) {
     if (!params || check_params(params)) return 0;
     const size_t nseg = (size_t)div_up((int)params->resolution[2], kSegVoxels);
-    return nseg > 1 ? (nseg - 1) * (size_t)params->resolution[0] * (size_t)params->resolution[1] * sizeof(float4)
+    return nseg > 1 ? (nseg - 1) * ((size_t)params->resolution[0] * (size_t)params->resolution[1] + 1) * sizeof(float2)
                     : 16;
 }
 

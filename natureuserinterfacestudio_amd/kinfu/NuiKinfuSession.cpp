@@ -93,7 +93,7 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
     } else {  // dense or shifting volume
         const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
         if (d.scene == NUIFK_SCENE_SHIFT) bytes += ((size_t)d.resolution[0] * d.resolution[1] + 64) * 4 + 65536;
-        bytes += (size_t)std::max(0, (d.resolution[2] + 127) / 128 - 1) * d.resolution[0] * d.resolution[1] * 16 + 16;
+        bytes += (size_t)std::max(0, (d.resolution[2] + 127) / 128 - 1) * ((size_t)d.resolution[0] * d.resolution[1] + 1) * 8 + 16;
         bytes += voxels * nuif_tsdf_voxel_bytes(d.storage) + (d.color ? voxels * 4 : 0);
         bytes += (size_t)((d.resolution[0] + 7) / 8) * ((d.resolution[1] + 7) / 8) * ((d.resolution[2] + 7) / 8) +
                  (size_t)((d.resolution[0] + 31) / 32) * ((d.resolution[1] + 31) / 32) * ((d.resolution[2] + 31) / 32);
