@@ -297,9 +297,11 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
 // The walk state of every column at the start of z segments 1, 2, ...: the
 // additions of K/tsdfVolume.cl:74-189 run once here, instead of every segment's
 // wavefront replaying them from z = 0.  The projection numerators (vx, vy) are
-// per column, float2[(k - 1) * Nx * Ny + column]; the z terms (vgz, zs) are the
-// same for every column and follow as float2[(nseg - 1) * Nx * Ny + k - 1],
-// written by column 0.
+// per column, float2[(k - 1) * Nx * Ny + tile * 64 + lane] in the order of the
+// integration's 8x8-column tiles (one wave reads 512 contiguous bytes); the z
+// terms (vgz, zs) are the same for every column and follow as
+// float2[(nseg - 1) * Nx * Ny + k - 1], written by column 0.  Nx, Ny are
+// multiples of 8 (the tile kernel's condition).
 __global__ void __launch_bounds__(256)
     column_states_kernel(TsdfParams p, NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
                          const int32_t* __restrict__ gate, float2* __restrict__ walk_states) {
@@ -307,7 +309,8 @@ __global__ void __launch_bounds__(256)
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nx * Ny) return;
-    const int x = i % Nx, y = i / Nx;
+    const int tile = i >> 6, lane = i & 63, tx = Nx / 8;
+    const int x = (tile % tx) * 8 + (lane & 7), y = (tile / tx) * 8 + (lane >> 3);
     const NuiCLRigidTransform g = *pose;
     const ColumnWalk c0 = column_setup(x, y, Nx, Ny, p, cam, g);
     const float cz = p.cell_size[2];
@@ -370,7 +373,8 @@ __global__ void __launch_bounds__(64)
     // column_states_kernel or replayed here
     const int zbeg = blockIdx.z * kSegVoxels, zend = min(Nz, zbeg + kSegVoxels);
     if (zbeg > 0 && walk_states) {
-        const float2 w = walk_states[(size_t)(blockIdx.z - 1) * Nx * Ny + (size_t)y * Nx + x];
+        const float2 w = walk_states[(size_t)(blockIdx.z - 1) * Nx * Ny +
+                                     ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 64 + lane];
         const float2 z = walk_states[(size_t)(gridDim.z - 1) * Nx * Ny + blockIdx.z - 1];
         c.vx = w.x;
         c.vy = w.y;
