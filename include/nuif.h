@@ -256,7 +256,8 @@ size_t nuif_tsdf_walk_states_bytes(const TsdfParams* params);
 /* nuif_tsdf_integrate in a wrapped volume; the surface blocks are marked at stored positions.
  * walk_states: optional workspace (nuif_tsdf_walk_states_bytes) -- the integration then
  * computes every column's incremental walk state at the start of each z segment once, instead
- * of replaying it per segment (same values). */
+ * of replaying it per segment (same values).  The workspace is overwritten by every call that
+ * gets it: calls that may overlap (different streams) need workspaces of their own. */
 int nuif_tsdf_integrate_wrapped(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,
                                 const TsdfParams* params, const NuiCLCameraParams* cam,
                                 const NuiCLRigidTransform* pose, int storage, int max_color_weight,
