@@ -207,7 +207,7 @@ def test_shift_engine_matches_oracle_engine(gpu, shift, color):
     np.testing.assert_array_equal(gc.view(np.uint32), rc.view(np.uint32))
 
 
-@pytest.mark.parametrize("N,storage,wrap", [(256, 0, (0, 0, 0)), (384, 0, (5, 7, 16)), (256, 2, (3, 250, 2))])
+@pytest.mark.parametrize("N,storage,wrap", [(256, 0, (0, 0, 0)), (384, 0, (5, 7, 16)), (320, 2, (3, 250, 2))])
 def test_walk_state_workspace_matches_the_replayed_walk(gpu, N, storage, wrap):
     """The per-column walk states computed once per z segment give the same volume,
     colour volume and updated-voxel count as every segment replaying the walk."""
