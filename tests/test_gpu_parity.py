@@ -545,3 +545,43 @@ def test_server_sessions_match_standalone_engine(gpu):
     assert srv.bytes_in_use == 2 * per
     assert srv.open(desc) == ids[1]  # the freed id is reused
     srv.close()
+
+
+def test_server_mixed_sessions_match_standalone_engines(gpu):
+    """One server round after another over sessions of every kind side by side
+    (320x240 / 640x480 / 1280x720 frames; dense S16, F16 and F32 volumes of 128^3 and
+    256^3, a hash scene and a shifting volume): each session's poses and maps, and
+    the dense volumes, equal those of the same session run alone, bit for bit."""
+    import dataclasses
+    announce("server: 5 mixed sessions (320x240..1280x720, S16/F16/F32, hash, shift), 4 rounds vs standalone")
+    full256 = dataclasses.replace(W.FULL, volume_res=256)
+    hd = dataclasses.replace(W.FULL, name="hd-1280x720-256^3", width=1280, height=720, fx=1050.0, fy=1050.0,
+                             cx=639.5, cy=359.5, volume_res=256, seed_index=3)
+    cases = [(W.SMALL, W.SMALL.kinfu_desc(use_graph=True), True),
+             (full256, full256.kinfu_desc(storage=nuif.TSDF_F16, use_graph=True), True),
+             (hd, hd.kinfu_desc(storage=nuif.TSDF_F32, use_graph=True), True),
+             (W.HASH_SMALL, W.HASH_SMALL.kinfu_desc(bricks=1 << 17), False),
+             (W.SMALL, W.SMALL.kinfu_desc(shift=3), True)]
+    n = 4
+    frames = [[nuif.DeviceArray.from_host(wl.render(f)[0]) for f in range(n)] for wl, _, _ in cases]
+    srv = nuif.Server(sum(nuif.Server.session_bytes(d) for _, d, _ in cases) + (1 << 30))
+    ids = [srv.open(d) for _, d, _ in cases]
+    assert None not in ids
+    for r in range(n):
+        srv.submit(ids, [frames[k][r] for k in range(len(cases))])
+        lat = srv.wait()
+        assert (lat > 0).all()
+    for k, ((wl, d, dense), sid) in enumerate(zip(cases, ids)):
+        ref = nuif.Engine(d)
+        for r in range(n):
+            ref.run(frames[k][r], sync=False)
+        ref.synchronize()
+        s = srv.sessions[sid]
+        assert s.synchronize() == 1, wl.name
+        np.testing.assert_array_equal(s.poses(n).view(np.uint32), ref.poses(n).view(np.uint32), err_msg=wl.name)
+        for a, b in zip(s.maps(), ref.maps()):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=wl.name)
+        if dense:
+            np.testing.assert_array_equal(s.volume_raw(), ref.volume_raw(), err_msg=wl.name)
+        ref.close()
+    srv.close()
