@@ -78,6 +78,18 @@ def test_voxel_bytes_and_params():
     assert lib.nuif_tsdf_params(0, 1, 1, 1.0, 1.0, 1.0, 0.03, C.byref(p)) == -1
 
 
+@pytest.mark.parametrize("shape,expect", [((128, 128, 128), 16), ((512, 512, 512), 3 * (512 * 512 + 1) * 8),
+                                          ((64, 96, 320), 2 * (64 * 96 + 1) * 8), ((8, 8, 129), (64 + 1) * 8)])
+def test_walk_state_workspace_bytes(shape, expect):
+    """8 B per column and z segment of 128 voxels after the first, plus 8 B per such
+    segment; a single segment needs none (16 B placeholder)."""
+    lib = nuif.L()
+    p = nuif.TsdfParams()
+    assert lib.nuif_tsdf_params(*shape, 1.0, 1.0, 1.0, 0.03, C.byref(p)) == 0
+    assert lib.nuif_tsdf_walk_states_bytes(C.byref(p)) == expect
+    assert lib.nuif_tsdf_walk_states_bytes(None) == 0
+
+
 def test_synthetic_frames_are_deterministic():
     wl = W.tiny()
     d1, c1, p1 = wl.render(3, color=True)
