@@ -419,12 +419,16 @@ __device__ void matmul3d(const double a[9], const double b[9], double c[9]) {
 // The host loop of OCL/NuiKinfuOpenCLDepthTracker.cpp:402-497 for one iteration,
 // given the iteration's sums.  Reference order: float sums and goodCores,
 // Eigen-like LLT in double and the float Euler increment.  Compensated order:
-// FP64 sums, LDL^T in FP64 and the increment composed in FP64.  Runs on one
-// thread over the LDS copy of the state.
+// FP64 sums, LDL^T in FP64 and the increment composed in FP64.  The serial steps
+// run on one thread over the LDS copy of the state; the 27 normalised entries of
+// the system and (compensated order) the six sines and cosines of the increment
+// are computed one per lane in between, the same operations on the same operands.
+
+// :402-458: error/count, accept or reject (thread 0).  Returns whether a system
+// is solved in this iteration.
 template <int MODE>
-__device__ void icp_update(IcpState& S, const double* sums, const float* sumsf, int level, int iter,
+__device__ bool icp_accept(IcpState& S, const double* sums, const float* sumsf, int level, int iter,
                            uint32_t count_threshold) {
-    // :402-458: error/count, accept or reject
     for (int k = 0; k < kTerms; ++k) S.sums[k] = sums[k];
     const float cnt = (float)sums[28];
     const float errsum = MODE == kRefOrder ? sumsf[27] : (float)sums[27];
@@ -446,7 +450,7 @@ __device__ void icp_update(IcpState& S, const double* sums, const float* sumsf, 
         S.lambda = lambda;
         if (iter == 0) {
             S.brk[level] = 1;
-            return;
+            return false;
         }
     } else {
         lambda /= 10.0f;
@@ -459,11 +463,25 @@ __device__ void icp_update(IcpState& S, const double* sums, const float* sumsf, 
             S.goodf[k] = MODE == kRefOrder ? sumsf[k] : (float)sums[k];
         }
     }
-    // damped normal equations, normalised by the current count
+    return true;
+}
+
+// Entry s of the system normalised by the current count (one lane each).
+template <int MODE>
+__device__ __forceinline__ double icp_entry(const IcpState& S, int s) {
+    const float cnt = S.count;
+    return MODE == kRefOrder ? (double)(S.goodf[s] / cnt) : S.good[s] / (double)cnt;
+}
+
+// The damped system, its factorisation and solve, the convergence test (thread 0).
+// Returns whether the increment (angles al, be, ga, translation tinc) is applied.
+template <int MODE>
+__device__ bool icp_solve(IcpState& S, const double* Av, int level, float ang[3], float tinc[3]) {
+    const float lambda = S.lambda;
     double A[36], b[6];
     for (int i = 0, s = 0; i < 6; ++i) {
         for (int j = i; j < 7; ++j, ++s) {
-            const double v = MODE == kRefOrder ? (double)(S.goodf[s] / cnt) : S.good[s] / (double)cnt;
+            const double v = Av[s];
             if (j == 6)
                 b[i] = v;
             else
@@ -475,7 +493,7 @@ __device__ void icp_update(IcpState& S, const double* sums, const float* sumsf, 
     const bool spd = MODE == kRefOrder ? llt6(A, &det) : ldlt6(A, &det);
     if (!spd || fabs(det) < 1e-15 || isnan(det)) {
         S.failed = 1;
-        return;
+        return false;
     }
     if (MODE == kRefOrder)
         llt6_solve(A, b, x);
@@ -484,63 +502,94 @@ __device__ void icp_update(IcpState& S, const double* sums, const float* sumsf, 
     const float al = (float)x[0], be = (float)x[1], ga = (float)x[2];
     if (al * al + be * be + ga * ga < 1e-6f) {  // converged: break before applying (:482-484)
         S.brk[level] = 1;
-        return;
+        return false;
     }
-    // Rinc = Rz(gamma) Ry(beta) Rx(alpha); t <- Rinc^-1 t + t_inc; R <- Rinc^-1 R   (:486-497)
-    const float tinc[3] = {(float)x[3], (float)x[4], (float)x[5]};
-    if (MODE == kRefOrder) {
-        float Rz[9], Ry[9], Rx[9], Rzy[9], Rinc[9], Ri[9];
-        axis_rotf(ga, 2, Rz);
-        axis_rotf(be, 1, Ry);
-        axis_rotf(al, 0, Rx);
-        matmul3f(Rz, Ry, Rzy);
-        matmul3f(Rzy, Rx, Rinc);
-        inverse3f(Rinc, Ri);
-        float tn[3];
-        for (int i = 0; i < 3; ++i) tn[i] = Ri[3 * i] * S.t[0] + Ri[3 * i + 1] * S.t[1] + Ri[3 * i + 2] * S.t[2];
-        for (int i = 0; i < 3; ++i) S.t[i] = tn[i] + tinc[i];
-        float Rn[9];
-        matmul3f(Ri, S.R, Rn);
-        for (int k = 0; k < 9; ++k) S.R[k] = Rn[k];
-    } else {
-        const double sa = sin((double)al), ca = cos((double)al);
-        const double sb = sin((double)be), cb = cos((double)be);
-        const double sg = sin((double)ga), cg = cos((double)ga);
-        const double Rz[9] = {cg, -sg, 0, sg, cg, 0, 0, 0, 1};
-        const double Ry[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
-        const double Rx[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca};
-        double Rzy[9], Rinc[9];
-        matmul3d(Rz, Ry, Rzy);
-        matmul3d(Rzy, Rx, Rinc);
-        double Ri[9];  // inverse of a rotation: its transpose
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rinc[3 * j + i];
-        double Rc[9], tc[3];
-        for (int k = 0; k < 9; ++k) Rc[k] = S.R[k];
-        for (int k = 0; k < 3; ++k) tc[k] = S.t[k];
-        double Rn[9];
-        matmul3d(Ri, Rc, Rn);
-        for (int i = 0; i < 3; ++i)
-            S.t[i] = (float)(Ri[3 * i] * tc[0] + Ri[3 * i + 1] * tc[1] + Ri[3 * i + 2] * tc[2] + (double)tinc[i]);
-        for (int k = 0; k < 9; ++k) S.R[k] = (float)Rn[k];
-    }
+    ang[0] = al;
+    ang[1] = be;
+    ang[2] = ga;
+    for (int k = 0; k < 3; ++k) tinc[k] = (float)x[3 + k];
+    return true;
+}
+
+// Rinc = Rz(gamma) Ry(beta) Rx(alpha); t <- Rinc^-1 t + t_inc; R <- Rinc^-1 R   (:486-497)
+__device__ void icp_compose_ref(IcpState& S, const float ang[3], const float tinc[3]) {
+    float Rz[9], Ry[9], Rx[9], Rzy[9], Rinc[9], Ri[9];
+    axis_rotf(ang[2], 2, Rz);
+    axis_rotf(ang[1], 1, Ry);
+    axis_rotf(ang[0], 0, Rx);
+    matmul3f(Rz, Ry, Rzy);
+    matmul3f(Rzy, Rx, Rinc);
+    inverse3f(Rinc, Ri);
+    float tn[3];
+    for (int i = 0; i < 3; ++i) tn[i] = Ri[3 * i] * S.t[0] + Ri[3 * i + 1] * S.t[1] + Ri[3 * i + 2] * S.t[2];
+    for (int i = 0; i < 3; ++i) S.t[i] = tn[i] + tinc[i];
+    float Rn[9];
+    matmul3f(Ri, S.R, Rn);
+    for (int k = 0; k < 9; ++k) S.R[k] = Rn[k];
+}
+
+// The same composition in FP64; trig = sin, cos of alpha, beta, gamma (as doubles).
+__device__ void icp_compose_comp(IcpState& S, const double trig[6], const float tinc[3]) {
+    const double sa = trig[0], ca = trig[1], sb = trig[2], cb = trig[3], sg = trig[4], cg = trig[5];
+    const double Rz[9] = {cg, -sg, 0, sg, cg, 0, 0, 0, 1};
+    const double Ry[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
+    const double Rx[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca};
+    double Rzy[9], Rinc[9];
+    matmul3d(Rz, Ry, Rzy);
+    matmul3d(Rzy, Rx, Rinc);
+    double Ri[9];  // inverse of a rotation: its transpose
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rinc[3 * j + i];
+    double Rc[9], tc[3];
+    for (int k = 0; k < 9; ++k) Rc[k] = S.R[k];
+    for (int k = 0; k < 3; ++k) tc[k] = S.t[k];
+    double Rn[9];
+    matmul3d(Ri, Rc, Rn);
+    for (int i = 0; i < 3; ++i)
+        S.t[i] = (float)(Ri[3 * i] * tc[0] + Ri[3 * i + 1] * tc[1] + Ri[3 * i + 2] * tc[2] + (double)tinc[i]);
+    for (int k = 0; k < 9; ++k) S.R[k] = (float)Rn[k];
 }
 
 // Run by the whole block that completes the reduction: copy the state into LDS,
-// update it on thread 0, write it back.  No other block reads the state after
-// it has arrived, so the write-back races with nothing.
+// update it, write it back.  No other block reads the state after it has
+// arrived, so the write-back races with nothing.  Every branch between the
+// barriers reads a flag written before the previous barrier: it is uniform.
 template <int MODE>
 __device__ void finish_iteration(IcpState* st, IcpState& S, const double* sums, const float* sumsf, int level,
                                  int iter, uint32_t count_threshold) {
+    __shared__ double Av[27], trig[6];
+    __shared__ float ang[3], tinc[3];
+    __shared__ int go;
     constexpr int kWords = (int)(sizeof(IcpState) / 4);
+    const int tid = threadIdx.x;
     const uint32_t* src = reinterpret_cast<const uint32_t*>(st);
     uint32_t* lds = reinterpret_cast<uint32_t*>(&S);
-    for (int i = threadIdx.x; i < kWords; i += blockDim.x) lds[i] = src[i];
+    for (int i = tid; i < kWords; i += blockDim.x) lds[i] = src[i];
     __syncthreads();
-    if (threadIdx.x == 0) icp_update<MODE>(S, sums, sumsf, level, iter, count_threshold);
+    if (tid == 0) go = icp_accept<MODE>(S, sums, sumsf, level, iter, count_threshold);
+    __syncthreads();
+    if (go) {
+        if (tid < 27) Av[tid] = icp_entry<MODE>(S, tid);
+        __syncthreads();
+        if (tid == 0) go = icp_solve<MODE>(S, Av, level, ang, tinc);
+        __syncthreads();
+        if (go) {
+            if (MODE == kCompensated && tid < 6) {
+                const double a = (double)ang[tid >> 1];
+                trig[tid] = (tid & 1) ? cos(a) : sin(a);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                if (MODE == kRefOrder)
+                    icp_compose_ref(S, ang, tinc);
+                else
+                    icp_compose_comp(S, trig, tinc);
+            }
+        }
+    }
     __syncthreads();
     uint32_t* dst = reinterpret_cast<uint32_t*>(st);
-    for (int i = threadIdx.x; i < kWords; i += blockDim.x) dst[i] = lds[i];
+    for (int i = tid; i < kWords; i += blockDim.x) dst[i] = lds[i];
 }
 
 __global__ void icp_begin_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
