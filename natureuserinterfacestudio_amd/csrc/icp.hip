@@ -109,6 +109,7 @@ __device__ __forceinline__ f3 bilinear_holes(const float* __restrict__ src, floa
 
 // LDS: 29 terms x 8 segments x 33 (padded) floats, reused as doubles later
 constexpr int kSeg = 8, kSegLen = 32, kSegPad = 33;
+static_assert(kMaxBlocks % kSeg == 0, "the last block reads kMaxBlocks / kSeg partials per segment");
 
 template <int MODE>
 __device__ void finish_iteration(IcpState* st, IcpState& S, const double* sums, const float* sumsf, int level,
@@ -215,19 +216,21 @@ __global__ void __launch_bounds__(kBlock)
     fence_acquire_agent();  // acquire the other blocks' partials
     const int nblocks = (int)gridDim.x;
     if (tid < kTerms * kSeg) {
-        // term k, segment j: blocks j, j + 8, ... in order; the loads of 8 blocks are
-        // issued together, the FP64 sum keeps the fixed order
+        // term k, segment j: blocks j, j + 8, ... in order; the loads of all of them
+        // (at most kMaxBlocks / kSeg) are issued together, the FP64 sum keeps the
+        // fixed order and adds only blocks that exist
         const int k = tid / kSeg, j = tid % kSeg;
-        double s = 0.0;
-        int b = j;
-        for (; b + 7 * kSeg < nblocks; b += 8 * kSeg) {
-            double v[8];
+        constexpr int kPer = kMaxBlocks / kSeg;
+        double v[kPer];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = partials[(size_t)(b + q * kSeg) * kTerms + k];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) s += v[q];
+        for (int q = 0; q < kPer; ++q) {
+            const int b = j + q * kSeg;
+            v[q] = b < nblocks ? partials[(size_t)b * kTerms + k] : 0.0;
         }
-        for (; b < nblocks; b += kSeg) s += partials[(size_t)b * kTerms + k];
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q)
+            if (j + q * kSeg < nblocks) s += v[q];
         part[k * kSeg + j] = s;
     }
     __syncthreads();
