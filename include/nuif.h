@@ -153,16 +153,22 @@ typedef struct nuif_icp_result {
     float t[3];
     float error;         /* m_error of the last executed iteration */
     float count;         /* m_count of the last executed iteration */
-    int32_t status;      /* 0 ok, 1 tracking failure (singular system) */
+    int32_t status;      /* 0 ok, 1 tracking failure (singular system), 2 the persistent kernel gave up */
     int32_t iterations;  /* iterations that ran (not skipped by a break) */
     int32_t integrate;   /* integration gate decision of the last frame */
     int32_t frame;       /* frames finalised */
     double sums[NUIF_ICP_TERMS]; /* 29 sums of the last executed iteration */
+    double Rd[9], td[3]; /* compensated order: the pose in FP64 */
 } nuif_icp_result;
 
 /* Summation order of the 29 ICP sums */
-#define NUIF_ICP_REDUCE_REFERENCE 0    /* the reference's float trees and host loop, bit for bit */
-#define NUIF_ICP_REDUCE_COMPENSATED 1  /* Kahan FP32 per lane, FP64 across blocks, FP64 LDL^T */
+#define NUIF_ICP_REDUCE_REFERENCE 0    /* the reference's float trees and LM host loop, bit for bit */
+/* Kahan FP32 per lane, FP64 across lanes and tiles; Gauss-Newton on the undamped normal
+ * equations (FP64 LDL^T), the increment applied as an SE(3) exponential to an FP64 pose,
+ * a step below 1e-6 ends a level.  It replaces the reference's LM loop, whose accept /
+ * reject decisions at the noise floor of the residual make float and FP64 runs diverge
+ * by millimetres; this loop stays within hundredths of a millimetre of its FP64 run. */
+#define NUIF_ICP_REDUCE_COMPENSATED 1
 
 typedef struct nuif_icp_config {
     int levels;                            /* pyramid levels (<= 4) */
@@ -172,8 +178,8 @@ typedef struct nuif_icp_config {
     int reduction;                         /* NUIF_ICP_REDUCE_* */
 } nuif_icp_config;
 
-/* State block: the tracking state plus the completion counters of the fused reductions
- * (device memory, initialised by nuif_icp_state_init). */
+/* State block: the tracking state plus the work-queue ticket and completion counters of
+ * the persistent ICP kernel (device memory, initialised by nuif_icp_state_init). */
 size_t nuif_icp_state_bytes(void);
 /* Workspace of nuif_icp_iteration for a width x height level-0 image. */
 size_t nuif_icp_partials_bytes(int width, int height);
@@ -182,17 +188,21 @@ int nuif_icp_state_init(void* state, const NuiCLRigidTransform* pose, nuif_strea
 int nuif_icp_state_read(const void* state, nuif_icp_result* out, nuif_stream_t s); /* synchronises s */
 /* Begin a frame: estimate <- pose, clear break/failure flags. */
 int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s);
-/* One iteration at `level`, ONE kernel launch: association + 29-term reduction and, in the
- * workgroup that completes the reduction, the reference's LM accept/reject, FP64 6x6 solve
- * and pose update (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-497).
- * vmap_level: current-frame vertices at the level; ref_vmap/ref_nmap: world maps at full
- * resolution; prev_pose: previous frame's transform; count_threshold: see
- * nuif_icp_count_thresholds.  Reference order handles at most 2^22 pixels per level.
- * Launches sharing one state block must not overlap (they share its counters). */
+/* One iteration at `level` (the persistent kernel run for a single epoch): association +
+ * 29-term reduction and, in the workgroup that completes the reduction, the update of the
+ * chosen order (reference: the LM accept/reject, FP64 LLT solve and Euler increment of
+ * OCL/NuiKinfuOpenCLDepthTracker.cpp:240-497).  Skipped after a break at this level or a
+ * failure.  vmap_level: current-frame vertices at the level; ref_vmap/ref_nmap: world maps
+ * at full resolution; prev_pose: previous frame's transform; count_threshold: see
+ * nuif_icp_count_thresholds.  At most 2^22 pixels per level.  Launches sharing one state
+ * block must not overlap (they share its counters). */
 int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* cam, const float* vmap_level,
                        int level, const float* ref_vmap, const float* ref_nmap,
                        const NuiCLRigidTransform* prev_pose, float dist_threshold, uint32_t count_threshold,
                        int iter, int reduction, nuif_stream_t s);
+/* A frame without tracking (the first frame, VO/NuiKinfuTrackingEngine.cpp:93-104): the
+ * pose is kept, a history row is written and the frame integrates. */
+int nuif_icp_record(void* state, NuiCLRigidTransform* pose, float* history, int capacity, nuif_stream_t s);
 /* Finish the frame: on success pose <- estimate (with R_inv), and the integration gate
  * (VO/NuiKinfuTrackingEngine.cpp:113-127) is evaluated into the state.  If history is not
  * NULL, row (frame % capacity) of NUIF_POSE_ROW floats receives
@@ -206,10 +216,33 @@ int nuif_icp_end(void* state, NuiCLRigidTransform* pose, float gate_threshold, f
  * translations of the next integration gates (global = local + offset). */
 int nuif_icp_set_translation(void* state, NuiCLRigidTransform* pose, float* history, int capacity, const float* t3,
                              const float* offset3, nuif_stream_t s);
-/* All levels and iterations of one frame, no host synchronisation. */
+/* All levels and iterations of one frame (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504
+ * IterativeClosestPoint), no host synchronisation: begin, the persistent kernel, end. */
 int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
                       const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
                       const nuif_icp_config* cfg, float* history, int capacity, nuif_stream_t s);
+
+/* One session of nuif_icp_estimate_batch (the arguments of nuif_icp_estimate). */
+typedef struct nuif_icp_session {
+    void* state;                               /* nuif_icp_state_bytes() */
+    void* partials;                            /* nuif_icp_partials_bytes(W, H) */
+    NuiCLCameraParams cam;
+    const float* vmap_pyr[NUIF_ICP_MAX_LEVELS];
+    const float* ref_vmap;
+    const float* ref_nmap;
+    NuiCLRigidTransform* pose;                 /* device camera pose (previous frame in, new out) */
+    nuif_icp_config cfg;
+    float* history;
+    int capacity;
+} nuif_icp_session;
+/* Device bytes of the session table of nuif_icp_estimate_batch. */
+size_t nuif_icp_batch_table_bytes(int n);
+/* The frame's ICP of n sessions (any image sizes, either order) in one persistent kernel:
+ * blocks take 1024-pixel tiles of any session whose current iteration is open; each
+ * session moves through its levels and iterations on its own and leaves early on a break
+ * or failure.  No host synchronisation.  table: nuif_icp_batch_table_bytes(n) of device
+ * memory, written on the stream (calls on one stream may reuse it). */
+int nuif_icp_estimate_batch(const nuif_icp_session* sessions, int n, void* table, nuif_stream_t s);
 /* Device int32 flags inside a state block: integration gate of the last frame, and
  * tracking success of the last frame (raycast runs only if set). */
 const int32_t* nuif_icp_gate_flag(const void* state);

@@ -1,70 +1,94 @@
-// Projective point-to-plane ICP.
+// Projective point-to-plane ICP: every iteration of every session of a frame in
+// ONE persistent kernel.
 //
-// The reference runs per iteration: icp_block_kernel (128-wide LDS trees of
-// 29 floats), compute_sums, a blocking read-back, and the Levenberg-Marquardt
+// The reference runs per iteration: icp_block_kernel (128-wide LDS trees of 29
+// floats), compute_sums, a blocking read-back, and the Levenberg-Marquardt
 // bookkeeping + Eigen FP64 6x6 solve on the host
-// (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504).  Here every iteration is ONE
-// device launch and the whole pyramid runs without a host round trip:
+// (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504), for one session on one queue.
+// Here a frame's ICP for any number of sessions is three launches: begin (one
+// block per session), the persistent kernel, end (one block per session).
 //
-//   icp_ref_kernel   reference order: one 128-item block per 128 pixels and the
-//                    reference's tree pairing (item i takes item i + d for
-//                    d = 64..1) as one LDS step plus wave shuffles; the last
-//                    block of every 128 to finish runs compute_sums' tree over
-//                    their partials, and the last of those sums the group
-//                    results in group order, as the host does, then solves.
-//   icp_comp_kernel  compensated: grid-stride blocks, Kahan FP32 per thread,
-//                    FP64 across threads; the last block to finish sums the
-//                    block partials in block order in FP64, then solves.
+// Work queue.  Each session's state carries a 64-bit ticket: the current epoch
+// (level * 64 + iteration, or "finished") in the high word and the next
+// unclaimed tile of 1024 pixels in the low word.  A block claims a tile of any
+// session with an atomic increment, evaluates the 29 terms of its pixels and
+// writes the tile's partial sums; the block that completes the epoch's last
+// tile reduces the partials in a fixed order, runs the update (the reference's
+// LM step, or the compensated Gauss-Newton step) and publishes the next epoch
+// (the next iteration, the next level after a break or the last iteration, or
+// "finished") by exchanging the ticket.  A block holding a claim never waits,
+// so progress does not depend on how many blocks are resident; blocks without
+// work sleep and poll, and leave when every session is finished.  A poll loop
+// that runs longer than a bound set by the host leaves too, and the session's
+// end kernel then reports the frame as failed with a timeout status.
 //
-// Completion is counted with device-scope atomics after a release fence, and
-// the counting block acquires before it reads other blocks' partials.  The
-// block that completes a counter re-arms it, so the next launch starts from
-// zero without a fill.  Every sum is taken in a fixed order: the result does
-// not depend on which block finishes last.
+// Reduction orders (nuif_icp_config::reduction):
+//   reference    a tile is eight 128-pixel groups, each reduced with the
+//                reference's tree (item i takes item i + d for d = 64..1); the
+//                last tile of every 128 groups to finish runs compute_sums' tree
+//                over them, and the last of those sums the group results in
+//                order, as the host does, then runs the LM step: float sums,
+//                Eigen-like LLT in FP64, float Euler increment.  Bit for bit the
+//                reference's arithmetic.
+//   compensated  thread j of tile k takes pixels 1024k + j + 256i (i < 4) with
+//                Kahan-compensated FP32 sums, FP64 across threads, tiles and
+//                segments in a fixed order; Gauss-Newton on the undamped normal
+//                equations (FP64 LDL^T), increment applied as an SE(3)
+//                exponential to an FP64 pose (se3_math.h).
+// Every sum is taken in a fixed order: results do not depend on which block
+// processed which tile.
 //
-// The solve runs on one thread over an LDS copy of the state: the reference's
-// accept/reject state machine (count threshold left over from the previous
-// level, constant distance threshold, reject restores the last good pose),
-// the damped 6x6 system normalised by the count, solved in FP64 (LLT in
-// reference order, LDL^T compensated), and the Euler-angle increment.
-//
-// The tracking state (estimate, last good pose, lambda, break flags, last
-// integration pose, integration gate) lives in one device block so that
-// integrate/raycast launches can consume the ICP outcome without the host.
+// Memory ordering.  Writers of partials release (agent scope) before counting
+// themselves in; the block that goes on acquires.  Claimers read the epoch from
+// the ticket and the current estimate with agent-scope atomic loads, so their
+// caches are never invalidated wholesale.
 
 #include "det_math.h"
 #include "icp_math.h"
 #include "nuif_internal.h"
+#include "se3_math.h"
+
+#include <algorithm>
+#include <mutex>
 
 namespace nuif {
 
 constexpr int kTerms = NUIF_ICP_TERMS;
 constexpr int kBlock = 256;
-constexpr int kMaxBlocks = 256;    // compensated: blocks per launch (grid-stride over the pixels)
-constexpr int kMaxGroups2 = 256;   // reference: 128-block groups per launch (W*H <= 2^22)
+constexpr int kTilePx = 1024;                // pixels per work item
+constexpr int kTileGroups = kTilePx / 128;   // reference order: 128-item groups per tile
+constexpr int kMaxChunks = 256;              // reference order: 128-group chunks per level (W*H <= 2^22)
+constexpr int kEpochIters = 64;              // iterations per level are below this
+constexpr uint32_t kFinished = 0xffffffffu;  // epoch of a session whose frame is over
+static_assert(128 % kTileGroups == 0, "a tile lies in one compute_sums chunk");
+
+// Reduction orders (nuif_icp_config::reduction)
+constexpr int kRefOrder = 0;      // the reference's float trees and LM loop, bit for bit
+constexpr int kCompensated = 1;   // Kahan FP32 + FP64, Gauss-Newton with an FP64 pose
 
 struct IcpState {
-    float R[9], t[3];          // current estimate (Rcurr, tcurr)
-    float Rg[9], tg[3];        // last accepted (Rgood, tgood)
+    float R[9], t[3];          // current estimate (compensated: the FP64 pose rounded)
+    float Rg[9], tg[3];        // reference order: last accepted (Rgood, tgood)
     float Rlast[9], tlast[3];  // pose of the last integration
     float lambda, err_old;
     float error, count;
     int32_t brk[NUIF_ICP_MAX_LEVELS];
-    int32_t failed, iterations, gate, frame, ok, pad;
-    double good[27];       // compensated mode: accepted sums
-    double sums[kTerms];   // sums of the last executed iteration (both modes)
-    float goodf[27];       // reference mode: goodCores as the reference keeps them (float)
+    int32_t failed, iterations, gate, frame, ok, timeout;
+    double good[27];       // unused by both orders today; kept for the layout
+    double sums[kTerms];   // sums of the last executed iteration (both orders)
+    float goodf[27];       // reference order: goodCores as the reference keeps them (float)
     float pad2;
     float off[3];          // shifting volume: offset of the local frame (metres), 0 otherwise
     float pad3;
+    double Rd[9], td[3];   // compensated order: the pose in FP64
 };
 static_assert(sizeof(IcpState) % 8 == 0, "state is copied as 32-bit words");
 
-// Completion counters, after the state block in the same allocation
 struct IcpSync {
-    uint32_t arrive[kMaxGroups2];  // reference: finished blocks of each 128-block group
-    uint32_t arrive_all;           // finished groups (reference) or blocks (compensated)
-    uint32_t pad[63];
+    unsigned long long ticket;    // (epoch << 32) | next tile
+    uint32_t done;                // compensated: tiles finished in the epoch
+    uint32_t chunks_done;         // reference: chunks reduced in the epoch
+    uint32_t arrive[kMaxChunks];  // reference: tiles finished per 128-group chunk
 };
 constexpr size_t kStateBytes = (sizeof(IcpState) + 255) / 256 * 256;
 constexpr size_t kSyncBytes = (sizeof(IcpSync) + 255) / 256 * 256;
@@ -73,20 +97,43 @@ __device__ __forceinline__ IcpSync* sync_of(IcpState* st) {
     return reinterpret_cast<IcpSync*>(reinterpret_cast<char*>(st) + kStateBytes);
 }
 
-__device__ __forceinline__ void clear_sync(IcpSync* sy, int tid, int nthreads) {
-    for (int i = tid; i < kMaxGroups2; i += nthreads) sy->arrive[i] = 0u;
-    if (tid == 0) sy->arrive_all = 0u;
-}
+// One session of a launch (device copy of nuif_icp_session).
+struct IcpJob {
+    IcpState* st;
+    void* partials;
+    const float* vmap[NUIF_ICP_MAX_LEVELS];
+    const float* vref;
+    const float* nref;
+    NuiCLRigidTransform* pose;  // the camera pose: the previous frame's during the iterations
+    float* history;
+    int capacity;
+    NuiCLCameraParams cam;
+    float dist;                 // constant distance threshold (:300-304)
+    float gate_threshold;
+    uint32_t thr[NUIF_ICP_MAX_LEVELS];
+    int32_t iters[NUIF_ICP_MAX_LEVELS];
+    int32_t levels;
+    int32_t mode;
+    int32_t single;             // one epoch only (nuif_icp_iteration): finished after it
+    int32_t first_epoch;        // single: the epoch to run
+};
 
-// Reduction orders (nuif_icp_config::reduction)
-constexpr int kRefOrder = 0;      // the reference's float trees, bit for bit
-constexpr int kCompensated = 1;   // Kahan FP32 per thread + FP64 across blocks
+__host__ __device__ __forceinline__ int level_pixels(const NuiCLCameraParams& c, int level) {
+    return ((int)c.depthImageWidth >> level) * ((int)c.depthImageHeight >> level);
+}
+__host__ __device__ __forceinline__ int level_tiles(const NuiCLCameraParams& c, int level) {
+    return (level_pixels(c, level) + kTilePx - 1) / kTilePx;
+}
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     const float y = v - c;
     const float t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+__device__ __forceinline__ unsigned long long load_ticket(const IcpSync* sy) {
+    return __hip_atomic_load(&sy->ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // K/icp.cl:192-220 interpolateBilinear_withHoles.  Reads past the row end as
@@ -107,164 +154,49 @@ __device__ __forceinline__ f3 bilinear_holes(const float* __restrict__ src, floa
     return r;
 }
 
-// LDS: 29 terms x 8 segments x 33 (padded) floats, reused as doubles later
-constexpr int kSeg = 8, kSegLen = 32, kSegPad = 33;
-static_assert(kMaxBlocks % kSeg == 0, "the last block reads kMaxBlocks / kSeg partials per segment");
-
-template <int MODE>
-__device__ void finish_iteration(IcpState* st, IcpState& S, const double* sums, const float* sumsf, int level,
-                                 int iter, uint32_t count_threshold);
-
-// The state and the partials are written and read by different blocks of one
-// launch: no __restrict__ on them.
-__global__ void __launch_bounds__(kBlock)
-    icp_comp_kernel(IcpState* st, const float* __restrict__ vcur, int npx, NuiCLCameraParams cam,
-                    const float* __restrict__ vref, const float* __restrict__ nref,
-                    const NuiCLRigidTransform* __restrict__ prev, float distThr, int level, int iter,
-                    uint32_t count_threshold, double* partials) {
-    if (st->failed || st->brk[level]) return;
-    __shared__ float lds[kTerms * kSeg * kSegPad];
-    __shared__ double part[kTerms * kSeg];
-    __shared__ double sums[kTerms];
-    __shared__ IcpState S;
-    __shared__ int last;
-
+// The pose terms of a tile: current estimate (R, t) and the previous camera
+// (R_inv, t) of transformInverse (K/utils.cl:62-67).
+struct TilePose {
     float R[9], t[3], Ri[9], tp[3];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        R[k] = st->R[k];
-        Ri[k] = prev->R_inv[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        t[k] = st->t[k];
-        tp[k] = prev->t[k];
-    }
-    float acc[kTerms], cmp[kTerms];
-#pragma unroll
-    for (int k = 0; k < kTerms; ++k) acc[k] = cmp[k] = 0.f;
+};
 
-    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
-    // kPB pixels of the grid-stride sequence at a time: their gathers are in flight
-    // together, their terms are accumulated in the sequence's order
-    constexpr int kPB = 4;
-    const int stride = gridDim.x * kBlock;
-    for (int p0 = blockIdx.x * kBlock + threadIdx.x; p0 < npx; p0 += kPB * stride) {
-        f3 pv[kPB], rn[kPB], rv[kPB];
-        bool ok[kPB];
-#pragma unroll
-        for (int j = 0; j < kPB; ++j) {
-            const int p = p0 + j * stride;
-            const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
-            // K/icp.cl:243-289
-            pv[j] = add3(rot3(R, vc), mk3(t[0], t[1], t[2]));
-            const f3 pp = rot3(Ri, sub3(pv[j], mk3(tp[0], tp[1], tp[2])));
-            const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
-            ok[j] = !nan3(vc) && px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H;
-            rn[j] = ok[j] ? bilinear_holes(nref, px, py, W, H) : mk3(kNaN, kNaN, kNaN);
-            rv[j] = ok[j] ? bilinear_holes(vref, px, py, W, H) : mk3(kNaN, kNaN, kNaN);
-        }
-#pragma unroll
-        for (int j = 0; j < kPB; ++j) {
-            if (!ok[j] || nan3(rn[j]) || nan3(rv[j])) continue;
-            const f3 diff = sub3(rv[j], pv[j]);
-            const float dist = sqrtf(dot3(diff, diff));
-            if (!(dist < distThr)) continue;
-            const f3 row0 = cross3(rn[j], pv[j]);
-            const float c[7] = {row0.x, row0.y, row0.z, rn[j].x, rn[j].y, rn[j].z, dot3(rn[j], diff)};
-            int s = 0;
-#pragma unroll
-            for (int i = 0; i < 6; ++i)
-#pragma unroll
-                for (int q = i; q < 7; ++q) {
-                    kahan_add(acc[s], cmp[s], c[i] * c[q]);
-                    ++s;
-                }
-            kahan_add(acc[27], cmp[27], c[6] * c[6]);
-            acc[28] += 1.0f;  // counts are exact in FP32 up to 2^24
-        }
-    }
-
-    // transpose through LDS: term k, thread tid -> lds[k][tid / 32][tid % 32]
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kTerms; ++k) lds[k * kSeg * kSegPad + (tid / kSegLen) * kSegPad + (tid % kSegLen)] = acc[k];
-    __syncthreads();
-    if (tid < kTerms * kSeg) {
-        const int k = tid / kSeg, sg = tid % kSeg;
-        const float* src = &lds[k * kSeg * kSegPad + sg * kSegPad];
-        float s = 0.f, c = 0.f;
-        for (int j = 0; j < kSegLen; ++j) kahan_add(s, c, src[j]);
-        part[k * kSeg + sg] = (double)s - (double)c;
-    }
-    __syncthreads();
-    if (tid < kTerms) {
-        double s = 0.0;
-        for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
-        partials[(size_t)blockIdx.x * kTerms + tid] = s;
-    }
-
-    // the last block to finish reduces the block partials and solves
-    IcpSync* sy = sync_of(st);
-    __syncthreads();  // the partial (threads 0..28) is written
-    if (tid == 0) {
-        fence_release_agent();  // release this block's partial
-        last = atomicAdd(&sy->arrive_all, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    fence_acquire_agent();  // acquire the other blocks' partials
-    const int nblocks = (int)gridDim.x;
-    if (tid < kTerms * kSeg) {
-        // term k, segment j: blocks j, j + 8, ... in order; the loads of all of them
-        // (at most kMaxBlocks / kSeg) are issued together, the FP64 sum keeps the
-        // fixed order and adds only blocks that exist
-        const int k = tid / kSeg, j = tid % kSeg;
-        constexpr int kPer = kMaxBlocks / kSeg;
-        double v[kPer];
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int b = j + q * kSeg;
-            v[q] = b < nblocks ? partials[(size_t)b * kTerms + k] : 0.0;
-        }
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < kPer; ++q)
-            if (j + q * kSeg < nblocks) s += v[q];
-        part[k * kSeg + j] = s;
-    }
-    __syncthreads();
-    if (tid < kTerms) {
-        double s = 0.0;
-        for (int j = 0; j < kSeg; ++j) s += part[tid * kSeg + j];
-        sums[tid] = s;
-    }
-    if (tid == 0) sy->arrive_all = 0u;  // re-arm: every block has arrived
-    finish_iteration<kCompensated>(st, S, sums, nullptr, level, iter, count_threshold);
-}
-
-// One pixel's 29 terms of K/icp.cl:243-289 (zeros when rejected).
-__device__ __forceinline__ void icp_terms(const f3 vc, const float* R, const float* t, const float* Ri,
-                                          const float* tp, const NuiCLCameraParams& cam,
+// One pixel of K/icp.cl:243-289: its association and its 7 coefficients
+// c = (n x p, n, n . (q - p)); false when rejected.
+__device__ __forceinline__ bool icp_assoc(const f3 vc, const TilePose& P, const NuiCLCameraParams& cam,
                                           const float* __restrict__ vref, const float* __restrict__ nref,
-                                          float distThr, float out[kTerms]) {
-#pragma unroll
-    for (int k = 0; k < kTerms; ++k) out[k] = 0.f;
-    if (nan3(vc)) return;
+                                          float distThr, float c[7]) {
+    if (nan3(vc)) return false;
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
-    const f3 pv = add3(rot3(R, vc), mk3(t[0], t[1], t[2]));
-    const f3 pp = rot3(Ri, sub3(pv, mk3(tp[0], tp[1], tp[2])));
+    const f3 pv = add3(rot3(P.R, vc), mk3(P.t[0], P.t[1], P.t[2]));
+    const f3 pp = rot3(P.Ri, sub3(pv, mk3(P.tp[0], P.tp[1], P.tp[2])));
     const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
-    if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) return;
+    if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) return false;
     // both lookups are issued before either is tested (pure reads)
     const f3 rn = bilinear_holes(nref, px, py, W, H);
     const f3 rv = bilinear_holes(vref, px, py, W, H);
-    if (nan3(rn) || nan3(rv)) return;
+    if (nan3(rn) || nan3(rv)) return false;
     const f3 diff = sub3(rv, pv);
     const float dist = sqrtf(dot3(diff, diff));
-    if (!(dist < distThr)) return;
+    if (!(dist < distThr)) return false;
     const f3 row0 = cross3(rn, pv);
-    const float c[7] = {row0.x, row0.y, row0.z, rn.x, rn.y, rn.z, dot3(rn, diff)};
+    c[0] = row0.x;
+    c[1] = row0.y;
+    c[2] = row0.z;
+    c[3] = rn.x;
+    c[4] = rn.y;
+    c[5] = rn.z;
+    c[6] = dot3(rn, diff);
+    return true;
+}
+
+// The 29 terms of a pixel (zeros when rejected)
+__device__ __forceinline__ void icp_terms(const f3 vc, const TilePose& P, const NuiCLCameraParams& cam,
+                                          const float* __restrict__ vref, const float* __restrict__ nref,
+                                          float distThr, float out[kTerms]) {
+    float c[7];
+#pragma unroll
+    for (int k = 0; k < kTerms; ++k) out[k] = 0.f;
+    if (!icp_assoc(vc, P, cam, vref, nref, distThr, c)) return;
     int s = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i)
@@ -274,143 +206,13 @@ __device__ __forceinline__ void icp_terms(const f3 vc, const float* R, const flo
     out[28] = 1.0f;
 }
 
-// Reference order, one launch per iteration.  Group g (128 items) takes pixels
-// 128g..128g+127; a partial last group counts its missing items as zeros
-// (K/icp.cl:222-292).  A block holds kRefGroups groups, each reduced with the
-// reference's 128-wide tree on its own 128 threads.  The partials buffer holds
-// ngroups x 29 group sums followed by ceil(ngroups / 128) x 29 compute_sums
-// results; the last block of every 128 groups to finish runs compute_sums' tree
-// over them, the last of those sums the results in order and solves.
-constexpr int kRefGroups = 4;
-static_assert(128 % kRefGroups == 0, "a block's groups lie in one compute_sums chunk");
-__global__ void __launch_bounds__(128 * kRefGroups)
-    icp_ref_kernel(IcpState* st, const float* __restrict__ vcur, int npx, NuiCLCameraParams cam,
-                   const float* __restrict__ vref, const float* __restrict__ nref,
-                   const NuiCLRigidTransform* __restrict__ prev, float distThr, int level, int iter,
-                   uint32_t count_threshold, float* partials) {
-    if (st->failed || st->brk[level]) return;
-    __shared__ float xfer[kRefGroups][64 * kTerms];
-    __shared__ float sumsf[kTerms];
-    __shared__ double sums[kTerms];
-    __shared__ IcpState S;
-    __shared__ int last, last2;
-    float R[9], t[3], Ri[9], tp[3];
+__device__ __forceinline__ double wave_tree64d(double v) {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        R[k] = st->R[k];
-        Ri[k] = prev->R_inv[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        t[k] = st->t[k];
-        tp[k] = prev->t[k];
-    }
-    const int tid = threadIdx.x, sub = tid >> 7, l = tid & 127, lane = l & 63, wave = l >> 6;
-    const int ngroups = (npx + 127) >> 7;
-    const int g = blockIdx.x * kRefGroups + sub;
-    const int p = g * 128 + l;
-    float v[kTerms];
-    const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
-    icp_terms(vc, R, t, Ri, tp, cam, vref, nref, distThr, v);
-
-    // K/icp.cl:162-190 reduce(): step d = 64 through LDS, steps 32..1 in the group's wave 0
-    if (wave == 1) {
-#pragma unroll
-        for (int k = 0; k < kTerms; ++k) xfer[sub][lane * kTerms + k] = v[k];
-    }
-    __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-        for (int k = 0; k < kTerms; ++k) v[k] = wave_tree64(v[k] + xfer[sub][lane * kTerms + k]);
-        if (lane == 0 && g < ngroups) {
-#pragma unroll
-            for (int k = 0; k < kTerms; ++k) partials[(size_t)g * kTerms + k] = v[k];
-        }
-    }
-    __syncthreads();  // the block's group partials are written
-
-    // the last block of each 128-group chunk runs compute_sums' tree over the chunk
-    IcpSync* sy = sync_of(st);
-    const int chunk = (blockIdx.x * kRefGroups) >> 7;
-    if (tid == 0) {
-        fence_release_agent();  // release this block's partials
-        const int groups_in_chunk = min(128, ngroups - (chunk << 7));
-        const uint32_t members = (uint32_t)((groups_in_chunk + kRefGroups - 1) / kRefGroups);
-        last = atomicAdd(&sy->arrive[chunk], 1u) == members - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    fence_acquire_agent();  // acquire the chunk's partials
-    float* tree = partials + (size_t)ngroups * kTerms;
-    const int wid = tid >> 6, nwaves = (128 * kRefGroups) >> 6;
-    for (int k = wid; k < kTerms; k += nwaves) {
-        const int g0 = (chunk << 7) + (tid & 63), g1 = g0 + 64;
-        const float x = g0 < ngroups ? partials[(size_t)g0 * kTerms + k] : 0.f;
-        const float y = g1 < ngroups ? partials[(size_t)g1 * kTerms + k] : 0.f;
-        const float sum = wave_tree64(x + y);
-        if ((tid & 63) == 0) tree[chunk * kTerms + k] = sum;
-    }
-    __syncthreads();  // the chunk result is written (lanes 0 of the waves)
-    const int nchunks = (ngroups + 127) >> 7;
-    if (tid == 0) {
-        sy->arrive[chunk] = 0u;  // re-arm: every block of the chunk has arrived
-        fence_release_agent();  // release the chunk result
-        last2 = atomicAdd(&sy->arrive_all, 1u) == (uint32_t)(nchunks - 1);
-    }
-    __syncthreads();
-    if (!last2) return;
-    fence_acquire_agent();  // acquire every chunk result
-
-    // the host's sequential float sum over the compute_sums results
-    // (OCL/NuiKinfuOpenCLDepthTracker.cpp:394-402, 428-440)
-    if (tid < kTerms) {
-        float sum = 0.f;
-        for (int c = 0; c < nchunks; ++c) sum += tree[c * kTerms + tid];
-        sumsf[tid] = sum;
-        sums[tid] = (double)sum;
-    }
-    if (tid == 0) sy->arrive_all = 0u;
-    finish_iteration<kRefOrder>(st, S, sums, sumsf, level, iter, count_threshold);
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    return v;
 }
 
-// ---- small FP64 linear algebra for the solve ----------------------------------
-
-// A = L D L^T (unit lower L), in place; returns false if a pivot is not positive.
-__device__ bool ldlt6(double A[36], double* det) {
-    double d = 1.0;
-    for (int j = 0; j < 6; ++j) {
-        double Dj = A[j * 6 + j];
-        for (int k = 0; k < j; ++k) Dj -= A[j * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
-        A[j * 6 + j] = Dj;
-        d *= Dj;
-        if (!(Dj > 0.0)) {
-            *det = d;
-            return false;
-        }
-        for (int i = j + 1; i < 6; ++i) {
-            double v = A[i * 6 + j];
-            for (int k = 0; k < j; ++k) v -= A[i * 6 + k] * A[j * 6 + k] * A[k * 6 + k];
-            A[i * 6 + j] = v / Dj;
-        }
-    }
-    *det = d;
-    return true;
-}
-
-__device__ void ldlt6_solve(const double F[36], const double b[6], double x[6]) {
-    double y[6];
-    for (int i = 0; i < 6; ++i) {
-        double s = b[i];
-        for (int k = 0; k < i; ++k) s -= F[i * 6 + k] * y[k];
-        y[i] = s;
-    }
-    for (int i = 0; i < 6; ++i) y[i] /= F[i * 6 + i];
-    for (int i = 5; i >= 0; --i) {
-        double s = y[i];
-        for (int k = i + 1; k < 6; ++k) s -= F[k * 6 + i] * x[k];
-        x[i] = s;
-    }
-}
+// ---- the updates ------------------------------------------------------------------
 
 __device__ void matmul3d(const double a[9], const double b[9], double c[9]) {
     double r[9];
@@ -419,23 +221,12 @@ __device__ void matmul3d(const double a[9], const double b[9], double c[9]) {
     for (int k = 0; k < 9; ++k) c[k] = r[k];
 }
 
-// The host loop of OCL/NuiKinfuOpenCLDepthTracker.cpp:402-497 for one iteration,
-// given the iteration's sums.  Reference order: float sums and goodCores,
-// Eigen-like LLT in double and the float Euler increment.  Compensated order:
-// FP64 sums, LDL^T in FP64 and the increment composed in FP64.  The serial steps
-// run on one thread over the LDS copy of the state; the 27 normalised entries of
-// the system and (compensated order) the six sines and cosines of the increment
-// are computed one per lane in between, the same operations on the same operands.
-
-// :402-458: error/count, accept or reject (thread 0).  Returns whether a system
-// is solved in this iteration.
-template <int MODE>
-__device__ bool icp_accept(IcpState& S, const double* sums, const float* sumsf, int level, int iter,
-                           uint32_t count_threshold) {
-    for (int k = 0; k < kTerms; ++k) S.sums[k] = sums[k];
-    const float cnt = (float)sums[28];
-    const float errsum = MODE == kRefOrder ? sumsf[27] : (float)sums[27];
-    const float err = cnt > 0.f ? sqrtf(errsum) / cnt : 3.402823466e38f;
+// OCL/NuiKinfuOpenCLDepthTracker.cpp:402-458 for one iteration: error / count,
+// accept or reject.  Returns whether a system is solved in this iteration.
+__device__ bool ref_accept(IcpState& S, const float* sumsf, int level, int iter, uint32_t count_threshold) {
+    for (int k = 0; k < kTerms; ++k) S.sums[k] = (double)sumsf[k];
+    const float cnt = sumsf[28];
+    const float err = cnt > 0.f ? sqrtf(sumsf[27]) / cnt : 3.402823466e38f;
     S.error = err;
     S.count = cnt;
     S.iterations += 1;
@@ -461,25 +252,15 @@ __device__ bool icp_accept(IcpState& S, const double* sums, const float* sumsf, 
         for (int k = 0; k < 9; ++k) S.Rg[k] = S.R[k];
         for (int k = 0; k < 3; ++k) S.tg[k] = S.t[k];
         S.err_old = err;
-        for (int k = 0; k < 27; ++k) {
-            S.good[k] = sums[k];
-            S.goodf[k] = MODE == kRefOrder ? sumsf[k] : (float)sums[k];
-        }
+        for (int k = 0; k < 27; ++k) S.goodf[k] = sumsf[k];
     }
     return true;
 }
 
-// Entry s of the system normalised by the current count (one lane each).
-template <int MODE>
-__device__ __forceinline__ double icp_entry(const IcpState& S, int s) {
-    const float cnt = S.count;
-    return MODE == kRefOrder ? (double)(S.goodf[s] / cnt) : S.good[s] / (double)cnt;
-}
-
-// The damped system, its factorisation and solve, the convergence test (thread 0).
-// Returns whether the increment (angles al, be, ga, translation tinc) is applied.
-template <int MODE>
-__device__ bool icp_solve(IcpState& S, const double* Av, int level, float ang[3], float tinc[3]) {
+// :459-484 the damped system normalised by the current count (Q12), LLT in
+// double, the rotation-only convergence test.  Returns whether the increment
+// (angles, translation) is applied.
+__device__ bool ref_solve(IcpState& S, const double* Av, int level, float ang[3], float tinc[3]) {
     const float lambda = S.lambda;
     double A[36], b[6];
     for (int i = 0, s = 0; i < 6; ++i) {
@@ -493,15 +274,11 @@ __device__ bool icp_solve(IcpState& S, const double* Av, int level, float ang[3]
         A[i * 6 + i] *= (double)(1.0f + lambda);
     }
     double det, x[6];
-    const bool spd = MODE == kRefOrder ? llt6(A, &det) : ldlt6(A, &det);
-    if (!spd || fabs(det) < 1e-15 || isnan(det)) {
+    if (!llt6(A, &det) || fabs(det) < 1e-15 || isnan(det)) {
         S.failed = 1;
         return false;
     }
-    if (MODE == kRefOrder)
-        llt6_solve(A, b, x);
-    else
-        ldlt6_solve(A, b, x);
+    llt6_solve(A, b, x);
     const float al = (float)x[0], be = (float)x[1], ga = (float)x[2];
     if (al * al + be * be + ga * ga < 1e-6f) {  // converged: break before applying (:482-484)
         S.brk[level] = 1;
@@ -515,7 +292,7 @@ __device__ bool icp_solve(IcpState& S, const double* Av, int level, float ang[3]
 }
 
 // Rinc = Rz(gamma) Ry(beta) Rx(alpha); t <- Rinc^-1 t + t_inc; R <- Rinc^-1 R   (:486-497)
-__device__ void icp_compose_ref(IcpState& S, const float ang[3], const float tinc[3]) {
+__device__ void ref_compose(IcpState& S, const float ang[3], const float tinc[3]) {
     float Rz[9], Ry[9], Rx[9], Rzy[9], Rinc[9], Ri[9];
     axis_rotf(ang[2], 2, Rz);
     axis_rotf(ang[1], 1, Ry);
@@ -531,97 +308,371 @@ __device__ void icp_compose_ref(IcpState& S, const float ang[3], const float tin
     for (int k = 0; k < 9; ++k) S.R[k] = Rn[k];
 }
 
-// The same composition in FP64; trig = sin, cos of alpha, beta, gamma (as doubles).
-__device__ void icp_compose_comp(IcpState& S, const double trig[6], const float tinc[3]) {
-    const double sa = trig[0], ca = trig[1], sb = trig[2], cb = trig[3], sg = trig[4], cg = trig[5];
-    const double Rz[9] = {cg, -sg, 0, sg, cg, 0, 0, 0, 1};
-    const double Ry[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
-    const double Rx[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca};
-    double Rzy[9], Rinc[9];
-    matmul3d(Rz, Ry, Rzy);
-    matmul3d(Rzy, Rx, Rinc);
-    double Ri[9];  // inverse of a rotation: its transpose
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) Ri[3 * i + j] = Rinc[3 * j + i];
-    double Rc[9], tc[3];
-    for (int k = 0; k < 9; ++k) Rc[k] = S.R[k];
-    for (int k = 0; k < 3; ++k) tc[k] = S.t[k];
-    double Rn[9];
-    matmul3d(Ri, Rc, Rn);
-    for (int i = 0; i < 3; ++i)
-        S.t[i] = (float)(Ri[3 * i] * tc[0] + Ri[3 * i + 1] * tc[1] + Ri[3 * i + 2] * tc[2] + (double)tinc[i]);
-    for (int k = 0; k < 9; ++k) S.R[k] = (float)Rn[k];
+// The compensated order's iteration from its FP64 sums (thread 0): the count
+// threshold, the Gauss-Newton step on the FP64 pose (nuif_se3::gn_step).
+__device__ void comp_update(IcpState& S, const double* sums, int level, uint32_t count_threshold, double* ws) {
+    for (int k = 0; k < kTerms; ++k) S.sums[k] = sums[k];
+    const double cnt = sums[28];
+    S.count = (float)cnt;
+    S.error = cnt > 0.0 ? (float)(sqrt(sums[27]) / cnt) : 3.402823466e38f;
+    S.iterations += 1;
+    if (cnt < (double)count_threshold) {
+        S.brk[level] = 1;
+        return;
+    }
+    const int r = nuif_se3::gn_step(sums, nuif_se3::kGnStep2, S.Rd, S.td, ws);
+    if (r == 2) {
+        S.failed = 1;
+    } else if (r == 1) {
+        S.brk[level] = 1;
+    } else {
+        for (int k = 0; k < 9; ++k) S.R[k] = (float)S.Rd[k];
+        for (int k = 0; k < 3; ++k) S.t[k] = (float)S.td[k];
+    }
 }
 
-// Run by the whole block that completes the reduction: copy the state into LDS,
-// update it, write it back.  No other block reads the state after it has
-// arrived, so the write-back races with nothing.  Every branch between the
-// barriers reads a flag written before the previous barrier: it is uniform.
+// The epoch that follows (level, iter) given the updated state.
+__device__ uint32_t next_epoch(const IcpState& S, const IcpJob& J, int level, int iter) {
+    if (S.failed || J.single) return kFinished;
+    if (!S.brk[level] && iter + 1 < J.iters[level]) return (uint32_t)(level * kEpochIters + iter + 1);
+    for (int l = level - 1; l >= 0; --l)
+        if (J.iters[l] > 0 && !S.brk[l]) return (uint32_t)(l * kEpochIters);
+    return kFinished;
+}
+
+// The block that completes an epoch: state into LDS, update, write back,
+// publish the next epoch.  sums (FP64) or sumsf (reference, float) are in LDS.
 template <int MODE>
-__device__ void finish_iteration(IcpState* st, IcpState& S, const double* sums, const float* sumsf, int level,
-                                 int iter, uint32_t count_threshold) {
-    __shared__ double Av[27], trig[6];
+__device__ void finish_epoch(const IcpJob& J, IcpState& S, const double* sums, const float* sumsf, int level,
+                             int iter) {
+    __shared__ double Av[27];
+    __shared__ double ws[60];
     __shared__ float ang[3], tinc[3];
     __shared__ int go;
     constexpr int kWords = (int)(sizeof(IcpState) / 4);
     const int tid = threadIdx.x;
+    IcpState* st = J.st;
     const uint32_t* src = reinterpret_cast<const uint32_t*>(st);
     uint32_t* lds = reinterpret_cast<uint32_t*>(&S);
     for (int i = tid; i < kWords; i += blockDim.x) lds[i] = src[i];
     __syncthreads();
-    if (tid == 0) go = icp_accept<MODE>(S, sums, sumsf, level, iter, count_threshold);
-    __syncthreads();
-    if (go) {
-        if (tid < 27) Av[tid] = icp_entry<MODE>(S, tid);
-        __syncthreads();
-        if (tid == 0) go = icp_solve<MODE>(S, Av, level, ang, tinc);
+    if (MODE == kRefOrder) {
+        if (tid == 0) go = ref_accept(S, sumsf, level, iter, J.thr[level]);
         __syncthreads();
         if (go) {
-            if (MODE == kCompensated && tid < 6) {
-                const double a = (double)ang[tid >> 1];
-                trig[tid] = (tid & 1) ? cos(a) : sin(a);
-            }
+            if (tid < 27) Av[tid] = (double)(S.goodf[tid] / S.count);
             __syncthreads();
             if (tid == 0) {
-                if (MODE == kRefOrder)
-                    icp_compose_ref(S, ang, tinc);
-                else
-                    icp_compose_comp(S, trig, tinc);
+                go = ref_solve(S, Av, level, ang, tinc);
+                if (go) ref_compose(S, ang, tinc);
             }
         }
+    } else if (tid == 0) {
+        comp_update(S, sums, level, J.thr[level], ws);
     }
     __syncthreads();
     uint32_t* dst = reinterpret_cast<uint32_t*>(st);
     for (int i = tid; i < kWords; i += blockDim.x) dst[i] = lds[i];
+    __syncthreads();  // the state is written
+    if (tid == 0) {
+        const uint32_t e = next_epoch(S, J, level, iter);
+        IcpSync* sy = sync_of(st);
+        __hip_atomic_store(&sy->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&sy->chunks_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fence_release_agent();  // the state before the ticket
+        __hip_atomic_exchange(&sy->ticket, (unsigned long long)e << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
-__global__ void icp_begin_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
-    clear_sync(sync_of(st), threadIdx.x, blockDim.x);
-    if (threadIdx.x != 0) return;
-    for (int k = 0; k < 9; ++k) st->R[k] = pose->R[k];
-    for (int k = 0; k < 3; ++k) st->t[k] = pose->t[k];
-    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
-    st->failed = 0;
-    st->iterations = 0;
-    st->ok = 0;
+// ---- the persistent kernel ------------------------------------------------------
+
+struct Claim {
+    int s;          // session (-1: none, -2: all finished)
+    uint32_t epoch;
+    uint32_t tile;
+};
+
+// Thread 0: one pass over the sessions from `start`, claiming the first open tile.
+__device__ Claim claim_tile(const IcpJob* jobs, int n, int start) {
+    int nfin = 0;
+    for (int k = 0; k < n; ++k) {
+        const int s = (start + k) % n;
+        const IcpJob& J = jobs[s];
+        IcpSync* sy = sync_of(J.st);
+        const unsigned long long t = load_ticket(sy);
+        const uint32_t e = (uint32_t)(t >> 32);
+        if (e == kFinished) {
+            ++nfin;
+            continue;
+        }
+        if ((uint32_t)t >= (uint32_t)level_tiles(J.cam, (int)(e / kEpochIters))) continue;
+        const unsigned long long old = atomicAdd(&sy->ticket, 1ull);
+        const uint32_t e2 = (uint32_t)(old >> 32), t2 = (uint32_t)old;
+        // a claim that lands in an open tile of whatever epoch the ticket holds now is kept
+        if (e2 != kFinished && t2 < (uint32_t)level_tiles(J.cam, (int)(e2 / kEpochIters))) return Claim{s, e2, t2};
+    }
+    return Claim{nfin == n ? -2 : -1, 0u, 0u};
 }
 
-__global__ void icp_state_init_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
-    clear_sync(sync_of(st), threadIdx.x, blockDim.x);
+// LDS of the kernel (one block: 256 threads)
+struct IcpLds {
+    IcpJob single;                // the job of a one-session launch (passed by value)
+    IcpJob job;
+    TilePose pose;
+    Claim claim;
+    int last, last2;
+    float xfer[2][64 * kTerms];   // reference order: the tree's d = 64 step of two groups
+    double wsum[4][kTerms];       // compensated: wave sums
+    double seg[kTerms * 8];       // compensated: segment sums
+    double sums[kTerms];
+    float sumsf[kTerms];
+    IcpState S;
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    icp_persistent_kernel(const IcpJob* __restrict__ jobs, int n, IcpJob single, long long max_ticks) {
+    __shared__ IcpLds L;
+    const int tid = threadIdx.x;
+    if (tid == 0 && !jobs) L.single = single;
+    __syncthreads();
+    const IcpJob* table = jobs ? jobs : &L.single;
+    const long long t_start = wall_clock64();
+    int polls = 0;
+    for (;;) {
+        if (tid == 0) {
+            const int start = (int)((blockIdx.x + (unsigned)polls) % (unsigned)n);
+            L.claim = claim_tile(table, n, start);
+            if (L.claim.s >= 0) {
+                L.job = table[L.claim.s];
+                // the estimate of this epoch (published before the ticket moved)
+                const uint32_t* sp = reinterpret_cast<const uint32_t*>(L.job.st);
+                uint32_t* dp = reinterpret_cast<uint32_t*>(&L.pose);
+                for (int k = 0; k < 12; ++k)
+                    dp[k] = __hip_atomic_load(sp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const NuiCLRigidTransform* prev = L.job.pose;
+                for (int k = 0; k < 9; ++k) L.pose.Ri[k] = prev->R_inv[k];
+                for (int k = 0; k < 3; ++k) L.pose.tp[k] = prev->t[k];
+            } else if (L.claim.s == -1 && wall_clock64() - t_start > max_ticks) {
+                L.claim.s = -2;  // polled past the host's bound: leave (unfinished sessions fail)
+            }
+        }
+        __syncthreads();
+        const Claim c = L.claim;
+        if (c.s == -2) return;
+        if (c.s == -1) {
+            // nothing open: sleep, then poll again
+            ++polls;
+            __builtin_amdgcn_s_sleep(8);
+            __syncthreads();  // L.claim is rewritten by thread 0 only after every thread read it
+            continue;
+        }
+        const IcpJob& J = L.job;
+        const int level = (int)(c.epoch / kEpochIters), iter = (int)(c.epoch % kEpochIters);
+        const int npx = level_pixels(J.cam, level);
+        const float* __restrict__ vcur = J.vmap[level];
+        IcpSync* sy = sync_of(J.st);
+
+        if (MODE == kRefOrder) {
+            // eight 128-pixel groups, two at a time (one per 128 threads)
+            const int sub = tid >> 7, l = tid & 127, lane = l & 63, wave = l >> 6;
+            const int ngroups = (npx + 127) >> 7;
+            float* part = reinterpret_cast<float*>(J.partials);
+            for (int r = 0; r < kTileGroups / 2; ++r) {
+                const int g = (int)c.tile * kTileGroups + 2 * r + sub;
+                const int p = g * 128 + l;
+                float v[kTerms];
+                const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
+                icp_terms(vc, L.pose, J.cam, J.vref, J.nref, J.dist, v);
+                // K/icp.cl:162-190 reduce(): step d = 64 through LDS, steps 32..1 in the group's wave 0
+                if (wave == 1) {
+#pragma unroll
+                    for (int k = 0; k < kTerms; ++k) L.xfer[sub][lane * kTerms + k] = v[k];
+                }
+                __syncthreads();
+                if (wave == 0) {
+#pragma unroll
+                    for (int k = 0; k < kTerms; ++k) v[k] = wave_tree64(v[k] + L.xfer[sub][lane * kTerms + k]);
+                    if (lane == 0 && g < ngroups) {
+#pragma unroll
+                        for (int k = 0; k < kTerms; ++k) part[(size_t)g * kTerms + k] = v[k];
+                    }
+                }
+                __syncthreads();
+            }
+            // the last tile of each 128-group chunk runs compute_sums' tree over the chunk
+            const int chunk = ((int)c.tile * kTileGroups) >> 7;
+            if (tid == 0) {
+                fence_release_agent();  // this tile's group sums
+                const int groups_in_chunk = min(128, ngroups - (chunk << 7));
+                const uint32_t members = (uint32_t)((groups_in_chunk + kTileGroups - 1) / kTileGroups);
+                L.last = atomicAdd(&sy->arrive[chunk], 1u) == members - 1;
+            }
+            __syncthreads();
+            if (L.last) {
+                fence_acquire_agent();  // the chunk's group sums
+                float* tree = part + (size_t)ngroups * kTerms;
+                const int wid = tid >> 6;
+                for (int k = wid; k < kTerms; k += kBlock / 64) {
+                    const int g0 = (chunk << 7) + (tid & 63), g1 = g0 + 64;
+                    const float x = g0 < ngroups ? part[(size_t)g0 * kTerms + k] : 0.f;
+                    const float y = g1 < ngroups ? part[(size_t)g1 * kTerms + k] : 0.f;
+                    const float sum = wave_tree64(x + y);
+                    if ((tid & 63) == 0) tree[chunk * kTerms + k] = sum;
+                }
+                __syncthreads();  // the chunk result is written
+                const int nchunks = (ngroups + 127) >> 7;
+                if (tid == 0) {
+                    // re-arm: every tile of the chunk has arrived
+                    __hip_atomic_store(&sy->arrive[chunk], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    fence_release_agent();
+                    L.last2 = atomicAdd(&sy->chunks_done, 1u) == (uint32_t)(nchunks - 1);
+                }
+                __syncthreads();
+                if (L.last2) {
+                    fence_acquire_agent();  // every chunk result
+                    // the host's sequential float sum over the compute_sums results
+                    // (OCL/NuiKinfuOpenCLDepthTracker.cpp:394-402, 428-440)
+                    if (tid < kTerms) {
+                        float sum = 0.f;
+                        for (int q = 0; q < nchunks; ++q) sum += tree[q * kTerms + tid];
+                        L.sumsf[tid] = sum;
+                    }
+                    __syncthreads();
+                    finish_epoch<MODE>(J, L.S, nullptr, L.sumsf, level, iter);
+                }
+            }
+        } else {
+            // compensated: four pixels per thread, their gathers in flight together
+            float acc[kTerms], cmp[kTerms];
+#pragma unroll
+            for (int k = 0; k < kTerms; ++k) acc[k] = cmp[k] = 0.f;
+            const int p0 = (int)c.tile * kTilePx + tid;
+            // pixels i = 0..3 in order, two at a time: their gathers in flight together
+#pragma unroll 1
+            for (int i0 = 0; i0 < 4; i0 += 2) {
+                float cf[2][7];
+                bool ok[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int p = p0 + (i0 + i) * kBlock;
+                    const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
+                    ok[i] = icp_assoc(vc, L.pose, J.cam, J.vref, J.nref, J.dist, cf[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    if (!ok[i]) continue;
+                    int s = 0;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a)
+#pragma unroll
+                        for (int b = a; b < 7; ++b) {
+                            kahan_add(acc[s], cmp[s], cf[i][a] * cf[i][b]);
+                            ++s;
+                        }
+                    kahan_add(acc[27], cmp[27], cf[i][6] * cf[i][6]);
+                    acc[28] += 1.0f;
+                }
+            }
+            const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+            for (int k = 0; k < kTerms; ++k) {
+                const double v = wave_tree64d((double)acc[k] - (double)cmp[k]);
+                if (lane == 0) L.wsum[wv][k] = v;
+            }
+            __syncthreads();
+            double* part = reinterpret_cast<double*>(J.partials);
+            if (tid < kTerms)
+                part[(size_t)c.tile * kTerms + tid] = ((L.wsum[0][tid] + L.wsum[1][tid]) + L.wsum[2][tid]) + L.wsum[3][tid];
+            __syncthreads();
+            const int ntiles = level_tiles(J.cam, level);
+            if (tid == 0) {
+                fence_release_agent();  // this tile's partial
+                L.last = atomicAdd(&sy->done, 1u) == (uint32_t)(ntiles - 1);
+            }
+            __syncthreads();
+            if (L.last) {
+                fence_acquire_agent();  // every tile's partial
+                if (tid < kTerms * 8) {
+                    // term k, segment j: tiles j, j + 8, ... in order
+                    const int k = tid >> 3, j = tid & 7;
+                    double s = 0.0;
+                    for (int q = j; q < ntiles; q += 8) s += part[(size_t)q * kTerms + k];
+                    L.seg[k * 8 + j] = s;
+                }
+                __syncthreads();
+                if (tid < kTerms) {
+                    double s = 0.0;
+                    for (int j = 0; j < 8; ++j) s += L.seg[tid * 8 + j];
+                    L.sums[tid] = s;
+                }
+                __syncthreads();
+                finish_epoch<MODE>(J, L.S, L.sums, nullptr, level, iter);
+            }
+        }
+        polls = 0;
+        __syncthreads();  // LDS is reused by the next claim
+    }
+}
+
+// ---- per-frame begin / end (one block per session) ----------------------------------
+
+// First epoch of a frame: the coarsest level with iterations.
+__device__ uint32_t first_epoch(const IcpJob& J, const IcpState* st) {
+    // (called after the frame's flags are set; a single epoch is skipped after a break or failure)
+    if (J.single) {
+        const int level = (int)(J.first_epoch / kEpochIters);
+        return (st->failed || st->brk[level]) ? kFinished : (uint32_t)J.first_epoch;
+    }
+    for (int l = J.levels - 1; l >= 0; --l)
+        if (J.iters[l] > 0) return (uint32_t)(l * kEpochIters);
+    return kFinished;
+}
+
+__device__ __forceinline__ void clear_sync(IcpSync* sy, int tid, int nthreads) {
+    for (int i = tid; i < kMaxChunks; i += nthreads) sy->arrive[i] = 0u;
+    if (tid == 0) {
+        sy->done = 0u;
+        sy->chunks_done = 0u;
+    }
+}
+
+// Begin a frame: estimate <- the camera pose, flags cleared, the first epoch
+// opened.  Compensated: the FP64 pose carries over from the previous frame unless
+// the camera pose was changed from outside (reset, hint, volume shift), in which
+// case it restarts from that pose.
+__global__ void icp_begin_kernel(const IcpJob* __restrict__ jobs, IcpJob single) {
+    __shared__ IcpJob J;
+    if (threadIdx.x == 0) J = jobs ? jobs[blockIdx.x] : single;
+    __syncthreads();
+    IcpState* st = J.st;
+    IcpSync* sy = sync_of(st);
+    clear_sync(sy, threadIdx.x, blockDim.x);
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int k = 0; k < 9; ++k) st->R[k] = st->Rg[k] = st->Rlast[k] = pose->R[k];
-    for (int k = 0; k < 3; ++k) st->t[k] = st->tg[k] = st->tlast[k] = pose->t[k];
-    for (int k = 0; k < 3; ++k) st->off[k] = 0.f;
-    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
-    st->failed = 0;
-    st->iterations = 0;
-    st->gate = 1;
-    st->ok = 1;
-    st->frame = 0;
-    st->lambda = 1.0f;
-    st->err_old = 1e20f;
-    st->error = 0.f;
-    st->count = 0.f;
+    const NuiCLRigidTransform* pose = J.pose;
+    if (!J.single) {
+        if (J.mode == kCompensated) {
+            bool same = true;
+            for (int k = 0; k < 9; ++k) same = same && (float)st->Rd[k] == pose->R[k];
+            for (int k = 0; k < 3; ++k) same = same && (float)st->td[k] == pose->t[k];
+            if (!same) {
+                for (int k = 0; k < 9; ++k) st->Rd[k] = (double)pose->R[k];
+                for (int k = 0; k < 3; ++k) st->td[k] = (double)pose->t[k];
+            }
+            for (int k = 0; k < 9; ++k) st->R[k] = (float)st->Rd[k];
+            for (int k = 0; k < 3; ++k) st->t[k] = (float)st->td[k];
+        } else {
+            for (int k = 0; k < 9; ++k) st->R[k] = pose->R[k];
+            for (int k = 0; k < 3; ++k) st->t[k] = pose->t[k];
+        }
+        for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
+        st->failed = 0;
+        st->iterations = 0;
+        st->ok = 0;
+        st->timeout = 0;
+    }
+    sy->ticket = (unsigned long long)first_epoch(J, st) << 32;
 }
 
 // src/Foundation/NuiMatrixUtilities.cpp:7-52 rodrigues2(R).norm() for a rotation
@@ -635,8 +686,16 @@ __device__ float rodrigues_norm(const float Rm[9]) {
     return (float)theta;
 }
 
-__global__ void icp_end_kernel(IcpState* st, NuiCLRigidTransform* pose, float gate_threshold, float* history,
-                               int capacity) {
+// Finish the frame (one thread per session): tracking outcome, pose, the
+// integration gate (VO/NuiKinfuTrackingEngine.cpp:113-127), the history row.
+// A session whose epochs did not finish (the persistent kernel gave up) fails
+// with the timeout flag set.
+__device__ void end_frame(IcpState* st, NuiCLRigidTransform* pose, float gate_threshold, float* history,
+                          int capacity) {
+    if ((uint32_t)(sync_of(st)->ticket >> 32) != kFinished) {
+        st->failed = 1;
+        st->timeout = 1;
+    }
     const int frame = st->frame;
     st->frame = frame + 1;
     float* row = history ? history + (size_t)(frame % capacity) * NUIF_POSE_ROW : nullptr;
@@ -659,7 +718,6 @@ __global__ void icp_end_kernel(IcpState* st, NuiCLRigidTransform* pose, float ga
     for (int k = 0; k < 9; ++k) pose->R[k] = st->R[k];
     for (int k = 0; k < 3; ++k) pose->t[k] = st->t[k];
     inverse3f(st->R, pose->R_inv);
-    // VO/NuiKinfuTrackingEngine.cpp:113-127 integration gate
     float Rrel[9];
     {
         const float* Ri = pose->R_inv;
@@ -682,22 +740,161 @@ __global__ void icp_end_kernel(IcpState* st, NuiCLRigidTransform* pose, float ga
     }
 }
 
+__global__ void icp_end_kernel(const IcpJob* __restrict__ jobs, IcpJob single) {
+    if (threadIdx.x != 0) return;
+    __shared__ IcpJob J;
+    J = jobs ? jobs[blockIdx.x] : single;
+    end_frame(J.st, J.pose, J.gate_threshold, J.history, J.capacity);
+}
+
+// A frame without tracking (the first frame): the pose is kept and the frame
+// integrates; estimate <- pose.
+__global__ void icp_record_kernel(IcpState* st, NuiCLRigidTransform* pose, float* history, int capacity) {
+    for (int k = 0; k < 9; ++k) st->R[k] = pose->R[k];
+    for (int k = 0; k < 3; ++k) st->t[k] = pose->t[k];
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
+    st->failed = 0;
+    st->iterations = 0;
+    st->timeout = 0;
+    sync_of(st)->ticket = (unsigned long long)kFinished << 32;
+    end_frame(st, pose, -1.0f, history, capacity);
+}
+
+__global__ void icp_state_init_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
+    clear_sync(sync_of(st), threadIdx.x, blockDim.x);
+    if (threadIdx.x != 0) return;
+    sync_of(st)->ticket = (unsigned long long)kFinished << 32;
+    for (int k = 0; k < 9; ++k) st->R[k] = st->Rg[k] = st->Rlast[k] = pose->R[k];
+    for (int k = 0; k < 3; ++k) st->t[k] = st->tg[k] = st->tlast[k] = pose->t[k];
+    for (int k = 0; k < 9; ++k) st->Rd[k] = (double)pose->R[k];
+    for (int k = 0; k < 3; ++k) st->td[k] = (double)pose->t[k];
+    for (int k = 0; k < 3; ++k) st->off[k] = 0.f;
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
+    st->failed = 0;
+    st->iterations = 0;
+    st->gate = 1;
+    st->ok = 1;
+    st->timeout = 0;
+    st->frame = 0;
+    st->lambda = 1.0f;
+    st->err_old = 1e20f;
+    st->error = 0.f;
+    st->count = 0.f;
+}
+
 // A shifting volume moved the local frame after this frame's tracking
 // (OCL/NuiKinfuOpenCLShiftScene.cpp:353-364): the new local translation becomes the
-// camera pose, the last integration pose (taken after the shift, as
-// VO/NuiKinfuTrackingEngine.cpp:126 does) and the frame's history row; `off` is the
-// frame's offset for the following integration gates.
+// camera pose, the estimate (FP64 pose included), the last integration pose (taken
+// after the shift, as VO/NuiKinfuTrackingEngine.cpp:126 does) and the frame's history
+// row; `off` is the frame's offset for the following integration gates.
 __global__ void icp_set_translation_kernel(IcpState* st, NuiCLRigidTransform* pose, float* history, int capacity,
                                            float tx, float ty, float tz, float ox, float oy, float oz) {
     const float t[3] = {tx, ty, tz}, o[3] = {ox, oy, oz};
     for (int k = 0; k < 3; ++k) {
         st->t[k] = st->tlast[k] = pose->t[k] = t[k];
+        st->td[k] = (double)t[k];
         st->off[k] = o[k];
     }
     if (history && st->frame > 0) {
         float* row = history + (size_t)((st->frame - 1) % capacity) * NUIF_POSE_ROW;
         for (int k = 0; k < 3; ++k) row[9 + k] = t[k];
     }
+}
+
+// ---- host side ---------------------------------------------------------------------
+
+long long poll_ticks() {
+    // the poll bound of the persistent kernel: 2 s of the device's constant-rate clock
+    static std::once_flag once;
+    static long long ticks = 200000000LL;
+    std::call_once(once, [] {
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+            ticks = (long long)khz * 2000LL;
+    });
+    return ticks;
+}
+
+int device_cus() {
+    static std::once_flag once;
+    static int cus = 256;
+    std::call_once(once, [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+    });
+    return cus;
+}
+
+int make_job(const nuif_icp_session& in, IcpJob* j) {
+    NUIF_CHECK_ARG(in.state && in.partials && in.ref_vmap && in.ref_nmap && in.pose, "ICP session: null argument");
+    const nuif_icp_config& cfg = in.cfg;
+    NUIF_CHECK_ARG(cfg.levels >= 1 && cfg.levels <= NUIF_ICP_MAX_LEVELS, "ICP session: bad level count %d",
+                   cfg.levels);
+    NUIF_CHECK_ARG(cfg.reduction == NUIF_ICP_REDUCE_REFERENCE || cfg.reduction == NUIF_ICP_REDUCE_COMPENSATED,
+                   "ICP session: unknown reduction %d", cfg.reduction);
+    NUIF_CHECK_ARG(!in.history || in.capacity > 0, "ICP session: history capacity must be positive");
+    const int W = (int)in.cam.depthImageWidth, H = (int)in.cam.depthImageHeight;
+    NUIF_CHECK_ARG(W > 0 && H > 0, "ICP session: empty image");
+    NUIF_CHECK_ARG(((size_t)W * H + 127) / 128 <= (size_t)kMaxChunks * 128,
+                   "ICP session: %d x %d pixels exceed the limit of %d", W, H, kMaxChunks * 128 * 128);
+    IcpJob& J = *j;
+    J = IcpJob{};
+    J.st = (IcpState*)in.state;
+    J.partials = in.partials;
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) {
+        J.vmap[l] = l < cfg.levels ? in.vmap_pyr[l] : nullptr;
+        J.iters[l] = l < cfg.levels ? cfg.iterations[l] : 0;
+        NUIF_CHECK_ARG(J.iters[l] >= 0 && J.iters[l] < kEpochIters, "ICP session: %d iterations at level %d",
+                       J.iters[l], l);
+        NUIF_CHECK_ARG(J.iters[l] == 0 || (J.vmap[l] && (W >> l) > 0 && (H >> l) > 0),
+                       "ICP session: level %d has iterations but no vertices", l);
+    }
+    J.vref = in.ref_vmap;
+    J.nref = in.ref_nmap;
+    J.pose = in.pose;
+    J.history = in.history;
+    J.capacity = in.capacity;
+    J.cam = in.cam;
+    const float L = (float)cfg.levels;
+    J.dist = cfg.dist_threshold + cfg.dist_threshold / L;  // :300-304
+    J.gate_threshold = cfg.gate_threshold;
+    int rc = nuif_icp_count_thresholds(W, H, &cfg, J.thr);
+    if (rc) return rc;
+    J.levels = cfg.levels;
+    J.mode = cfg.reduction;
+    return NUIF_OK;
+}
+
+// blocks of the persistent launch: enough for the largest level's tiles, at most 8 per CU
+unsigned persistent_grid(const IcpJob* jobs, int n) {
+    size_t most = 1;
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) {
+        size_t tiles = 0;
+        for (int s = 0; s < n; ++s)
+            if (jobs[s].iters[l] > 0 || (jobs[s].single && (int)(jobs[s].first_epoch / kEpochIters) == l))
+                tiles += (size_t)level_tiles(jobs[s].cam, l);
+        most = std::max(most, tiles);
+    }
+    return (unsigned)std::min<size_t>(most, (size_t)device_cus() * 8);
+}
+
+// one session: begin, persistent, end with the job as a kernel argument
+int run_single(const IcpJob& J, hipStream_t st, bool end) {
+    icp_begin_kernel<<<1, 64, 0, st>>>(nullptr, J);
+    NUIF_LAUNCHED("icp_begin_kernel");
+    if (J.mode == kRefOrder)
+        icp_persistent_kernel<kRefOrder><<<persistent_grid(&J, 1), kBlock, 0, st>>>(nullptr, 1, J, poll_ticks());
+    else
+        icp_persistent_kernel<kCompensated><<<persistent_grid(&J, 1), kBlock, 0, st>>>(nullptr, 1, J, poll_ticks());
+    NUIF_LAUNCHED("icp_persistent_kernel");
+    if (end) {
+        icp_end_kernel<<<1, 64, 0, st>>>(nullptr, J);
+        NUIF_LAUNCHED("icp_end_kernel");
+    }
+    return NUIF_OK;
 }
 
 }  // namespace nuif
@@ -720,9 +917,13 @@ int nuif_icp_set_translation(void* state, NuiCLRigidTransform* pose, float* hist
 size_t nuif_icp_state_bytes(void) { return kStateBytes + kSyncBytes; }
 
 size_t nuif_icp_partials_bytes(int width, int height) {
-    const size_t compensated = (size_t)kMaxBlocks * kTerms * sizeof(double);
-    const size_t groups = ((size_t)width * height + 127) / 128;
-    const size_t reference = (groups + (groups + 127) / 128) * kTerms * sizeof(float);
+    if (width <= 0 || height <= 0) return 0;
+    const size_t npx = (size_t)width * height;
+    const size_t compensated = (npx + kTilePx - 1) / kTilePx * kTerms * sizeof(double);
+    const size_t groups = (npx + 127) / 128;
+    // reference order: whole tiles of groups, then the chunk results
+    const size_t tgroups = (groups + kTileGroups - 1) / kTileGroups * kTileGroups;
+    const size_t reference = (tgroups + (groups + 127) / 128) * kTerms * sizeof(float);
     return std::max(compensated, reference);
 }
 
@@ -752,18 +953,26 @@ int nuif_icp_state_read(const void* state, nuif_icp_result* out, nuif_stream_t s
     for (int k = 0; k < 3; ++k) out->t[k] = h.t[k];
     out->error = h.error;
     out->count = h.count;
-    out->status = h.failed ? 1 : 0;
+    out->status = h.timeout ? 2 : h.failed ? 1 : 0;
     out->iterations = h.iterations;
     out->integrate = h.gate;
     out->frame = h.frame;
     for (int k = 0; k < kTerms; ++k) out->sums[k] = h.sums[k];
+    for (int k = 0; k < 9; ++k) out->Rd[k] = h.Rd[k];
+    for (int k = 0; k < 3; ++k) out->td[k] = h.td[k];
     return NUIF_OK;
 }
 
 int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(state && pose, "nuif_icp_begin: null argument");
-    icp_begin_kernel<<<1, 64, 0, stream_of(s)>>>((IcpState*)state, pose);
+    // a frame with no iterations: estimate <- pose, flags cleared
+    IcpJob J{};
+    J.st = (IcpState*)state;
+    J.pose = const_cast<NuiCLRigidTransform*>(pose);
+    J.levels = 1;
+    J.mode = kRefOrder;
+    icp_begin_kernel<<<1, 64, 0, stream_of(s)>>>(nullptr, J);
     NUIF_LAUNCHED("icp_begin_kernel");
     return NUIF_OK;
 }
@@ -774,31 +983,30 @@ int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* ca
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(state && partials && cam && vmap_level && ref_vmap && ref_nmap && prev_pose,
                    "nuif_icp_iteration: null argument");
-    NUIF_CHECK_ARG(level >= 0 && level < NUIF_ICP_MAX_LEVELS && iter >= 0, "nuif_icp_iteration: bad level/iter");
+    NUIF_CHECK_ARG(level >= 0 && level < NUIF_ICP_MAX_LEVELS && iter >= 0 && iter < kEpochIters,
+                   "nuif_icp_iteration: bad level/iter");
     NUIF_CHECK_ARG(reduction == NUIF_ICP_REDUCE_REFERENCE || reduction == NUIF_ICP_REDUCE_COMPENSATED,
                    "nuif_icp_iteration: unknown reduction %d", reduction);
     const int Wl = (int)cam->depthImageWidth >> level, Hl = (int)cam->depthImageHeight >> level;
     NUIF_CHECK_ARG(Wl > 0 && Hl > 0, "nuif_icp_iteration: level %d too small", level);
-    const int npx = Wl * Hl;
-    hipStream_t st = stream_of(s);
-    if (reduction == NUIF_ICP_REDUCE_REFERENCE) {
-        const int groups = (int)div_up(npx, 128);
-        NUIF_CHECK_ARG(div_up(groups, 128) <= kMaxGroups2,
-                       "nuif_icp_iteration: %d pixels per level exceed the reference-order limit %d", npx,
-                       kMaxGroups2 * 128 * 128);
-        const int blocks = (int)div_up((size_t)groups, kRefGroups);
-        icp_ref_kernel<<<blocks, 128 * kRefGroups, 0, st>>>((IcpState*)state, vmap_level, npx, *cam, ref_vmap,
-                                                            ref_nmap, prev_pose, dist_threshold, level, iter,
-                                                            count_threshold, partials);
-        NUIF_LAUNCHED("icp_ref_kernel");
-    } else {
-        const int nblocks = std::min<int>(div_up(npx, kBlock), kMaxBlocks);
-        icp_comp_kernel<<<nblocks, kBlock, 0, st>>>((IcpState*)state, vmap_level, npx, *cam, ref_vmap, ref_nmap,
-                                                    prev_pose, dist_threshold, level, iter, count_threshold,
-                                                    (double*)partials);
-        NUIF_LAUNCHED("icp_comp_kernel");
-    }
-    return NUIF_OK;
+    NUIF_CHECK_ARG(((size_t)Wl * Hl + 127) / 128 <= (size_t)kMaxChunks * 128,
+                   "nuif_icp_iteration: %d pixels per level exceed the limit %d", Wl * Hl, kMaxChunks * 128 * 128);
+    IcpJob J{};
+    J.st = (IcpState*)state;
+    J.partials = partials;
+    J.vmap[level] = vmap_level;
+    J.vref = ref_vmap;
+    J.nref = ref_nmap;
+    J.pose = const_cast<NuiCLRigidTransform*>(prev_pose);
+    J.cam = *cam;
+    J.dist = dist_threshold;
+    J.thr[level] = count_threshold;
+    J.iters[level] = iter + 1;
+    J.levels = level + 1;
+    J.mode = reduction;
+    J.single = 1;
+    J.first_epoch = level * kEpochIters + iter;
+    return run_single(J, stream_of(s), false);
 }
 
 int nuif_icp_end(void* state, NuiCLRigidTransform* pose, float gate_threshold, float* history, int capacity,
@@ -806,8 +1014,23 @@ int nuif_icp_end(void* state, NuiCLRigidTransform* pose, float gate_threshold, f
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(state && pose, "nuif_icp_end: null argument");
     NUIF_CHECK_ARG(!history || capacity > 0, "nuif_icp_end: history capacity must be positive");
-    icp_end_kernel<<<1, 1, 0, stream_of(s)>>>((IcpState*)state, pose, gate_threshold, history, capacity);
+    IcpJob J{};
+    J.st = (IcpState*)state;
+    J.pose = pose;
+    J.gate_threshold = gate_threshold;
+    J.history = history;
+    J.capacity = capacity;
+    icp_end_kernel<<<1, 64, 0, stream_of(s)>>>(nullptr, J);
     NUIF_LAUNCHED("icp_end_kernel");
+    return NUIF_OK;
+}
+
+int nuif_icp_record(void* state, NuiCLRigidTransform* pose, float* history, int capacity, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && pose, "nuif_icp_record: null argument");
+    NUIF_CHECK_ARG(!history || capacity > 0, "nuif_icp_record: history capacity must be positive");
+    icp_record_kernel<<<1, 1, 0, stream_of(s)>>>((IcpState*)state, pose, history, capacity);
+    NUIF_LAUNCHED("icp_record_kernel");
     return NUIF_OK;
 }
 
@@ -824,6 +1047,7 @@ int nuif_icp_count_thresholds(int W, int H, const nuif_icp_config* cfg, uint32_t
             kgs = (groups + 127) / 128 * 128;
         }
     }
+    for (int level = cfg->levels; level < NUIF_ICP_MAX_LEVELS; ++level) thr[level] = 0;
     return NUIF_OK;
 }
 
@@ -833,22 +1057,58 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(state && partials && cam && vmap_pyr && ref_vmap && ref_nmap && pose && cfg,
                    "nuif_icp_estimate: null argument");
-    uint32_t thr[NUIF_ICP_MAX_LEVELS];
-    int rc = nuif_icp_count_thresholds((int)cam->depthImageWidth, (int)cam->depthImageHeight, cfg, thr);
+    nuif_icp_session in{};
+    in.state = state;
+    in.partials = partials;
+    in.cam = *cam;
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) in.vmap_pyr[l] = l < cfg->levels ? vmap_pyr[l] : nullptr;
+    in.ref_vmap = ref_vmap;
+    in.ref_nmap = ref_nmap;
+    in.pose = pose;
+    in.cfg = *cfg;
+    in.history = history;
+    in.capacity = capacity;
+    IcpJob J;
+    const int rc = make_job(in, &J);
     if (rc) return rc;
-    const float L = (float)cfg->levels;
-    const float dist = cfg->dist_threshold + cfg->dist_threshold / L;  // :300-304
-    rc = nuif_icp_begin(state, pose, s);
-    if (rc) return rc;
-    for (int level = cfg->levels - 1; level >= 0; --level) {
-        NUIF_CHECK_ARG(vmap_pyr[level], "nuif_icp_estimate: vmap level %d is NULL", level);
-        for (int it = 0; it < cfg->iterations[level]; ++it) {
-            rc = nuif_icp_iteration(state, partials, cam, vmap_pyr[level], level, ref_vmap, ref_nmap, pose, dist,
-                                    thr[level], it, cfg->reduction, s);
-            if (rc) return rc;
-        }
+    return run_single(J, stream_of(s), true);
+}
+
+size_t nuif_icp_batch_table_bytes(int n) { return n > 0 ? ((size_t)n * sizeof(IcpJob) + 255) / 256 * 256 : 0; }
+
+int nuif_icp_estimate_batch(const nuif_icp_session* sessions, int n, void* table, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(n >= 0 && (n == 0 || (sessions && table)), "nuif_icp_estimate_batch: bad arguments");
+    if (n == 0) return NUIF_OK;
+    std::vector<IcpJob> jobs((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const int rc = make_job(sessions[i], &jobs[i]);
+        if (rc) return rc;
+        for (int j = 0; j < i; ++j)
+            NUIF_CHECK_ARG(jobs[j].st != jobs[i].st, "nuif_icp_estimate_batch: a state appears twice");
     }
-    return nuif_icp_end(state, pose, cfg->gate_threshold, history, capacity, s);
+    // sessions of the reference order first, then the compensated ones: one persistent
+    // launch per order over its part of the table
+    std::stable_partition(jobs.begin(), jobs.end(), [](const IcpJob& j) { return j.mode == kRefOrder; });
+    const int nref = (int)std::count_if(jobs.begin(), jobs.end(), [](const IcpJob& j) { return j.mode == kRefOrder; });
+    hipStream_t st = stream_of(s);
+    NUIF_HIP(hipMemcpyAsync(table, jobs.data(), (size_t)n * sizeof(IcpJob), hipMemcpyHostToDevice, st));
+    const IcpJob* dj = (const IcpJob*)table;
+    icp_begin_kernel<<<n, 64, 0, st>>>(dj, IcpJob{});
+    NUIF_LAUNCHED("icp_begin_kernel");
+    if (nref > 0) {
+        icp_persistent_kernel<kRefOrder>
+            <<<persistent_grid(jobs.data(), nref), kBlock, 0, st>>>(dj, nref, IcpJob{}, poll_ticks());
+        NUIF_LAUNCHED("icp_persistent_kernel");
+    }
+    if (nref < n) {
+        icp_persistent_kernel<kCompensated><<<persistent_grid(jobs.data() + nref, n - nref), kBlock, 0, st>>>(
+            dj + nref, n - nref, IcpJob{}, poll_ticks());
+        NUIF_LAUNCHED("icp_persistent_kernel");
+    }
+    icp_end_kernel<<<n, 64, 0, st>>>(dj, IcpJob{});
+    NUIF_LAUNCHED("icp_end_kernel");
+    return NUIF_OK;
 }
 
 }  // extern "C"

@@ -207,10 +207,9 @@ void NuiKinfuHIPDepthTracker::SetTranslation(NuiKinfuHIPCameraState* cam, const 
 }
 
 void NuiKinfuHIPDepthTracker::RecordPose(NuiKinfuHIPCameraState* cam) {
-    check(nuif_icp_begin(m_state.get(), cam->GetCameraTransformBuffer(), m_stream), "nuif_icp_begin");
     // a frame without tracking: the pose is unchanged and the frame always integrates
-    check(nuif_icp_end(m_state.get(), cam->GetCameraTransformBuffer(), -1.0f, m_history, m_historyCap, m_stream),
-          "nuif_icp_end");
+    check(nuif_icp_record(m_state.get(), cam->GetCameraTransformBuffer(), m_history, m_historyCap, m_stream),
+          "nuif_icp_record");
 }
 
 // OCL/NuiKinfuOpenCLDepthTracker.cpp:103-138: pyramid (already built by the

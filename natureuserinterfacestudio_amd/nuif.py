@@ -54,7 +54,7 @@ TSDF_S16, TSDF_F16, TSDF_F32 = 0, 1, 2
 class IcpResult(C.Structure):
     _fields_ = [("R", c_float * 9), ("t", c_float * 3), ("error", c_float), ("count", c_float),
                 ("status", C.c_int32), ("iterations", C.c_int32), ("integrate", C.c_int32), ("frame", C.c_int32),
-                ("sums", C.c_double * ICP_TERMS)]
+                ("sums", C.c_double * ICP_TERMS), ("Rd", C.c_double * 9), ("td", C.c_double * 3)]
 
 
 class IcpConfig(C.Structure):
@@ -138,6 +138,9 @@ _SIGS = {
                                    c_void_p, c_void_p, c_float, C.c_uint32, c_int, c_int, c_void_p]),
     "nuif_icp_set_translation": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "nuif_icp_end": (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_int, c_void_p]),
+    "nuif_icp_record": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "nuif_icp_batch_table_bytes": (c_size_t, [c_int]),
+    "nuif_icp_estimate_batch": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "nuif_icp_estimate": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                   c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
     "nuif_icp_gate_flag": (c_void_p, [c_void_p]),

@@ -11,6 +11,7 @@
 #include <cstdio>
 
 #include "../natureuserinterfacestudio_amd/csrc/det_math.h"
+#include "../natureuserinterfacestudio_amd/csrc/se3_math.h"
 
 namespace oracle {
 
@@ -446,6 +447,128 @@ IcpResult icp(const Cam<T>& cam, const IcpConfig& cfg, const T* const* vpyr, con
     return res;
 }
 
+// ---------------------------------------------------------------- ICP, compensated mode
+//
+// The GPU's compensated mode (parity unpinned against the reference, which has no
+// such mode; its parts are pinned by closed forms in the tests) replaces the
+// reference's Levenberg-Marquardt host loop (OCL/NuiKinfuOpenCLDepthTracker.cpp:
+// 240-504) by Gauss-Newton with an FP64 pose:
+//   * the same association and 29 terms per pixel (icp_pixel), evaluated in T at
+//     the pose rounded to T, against the same maps and distance threshold;
+//   * the sums accumulated with compensation (float instance: exactly the GPU's
+//     order, icp_sums_compensated; FP64 instance: in double, pixel order);
+//   * per level, from the coarsest: an iteration whose count is below the
+//     reference's (stale) count threshold ends the level without a step; else the
+//     undamped normal equations are solved in FP64 (LDL^T) and the increment
+//     applied as an SE(3) exponential in FP64 (nuif_se3::gn_step); a step below
+//     1e-6 ends the level; a singular system is a tracking failure.
+
+// The GPU's compensated reduction of one iteration (nuif ICP kernel): tiles of 1024
+// pixels; in tile k thread j (256 threads) takes pixels 1024k + j + 256i, i = 0..3,
+// with Kahan-compensated FP32 sums (the count plainly); each thread's value s - c
+// in double; FP64 tree over each 64-lane wave (lane l takes lane l + d for d = 32,
+// 16, .., 1), the four wave sums added in order; then term by term the tile sums
+// k = j, j + 8, j + 16, .. in order for j = 0..7, and those eight in order.
+inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int npx, const float R[9], const float t[3],
+                                 const float* vref, const float* nref, const Rigid<float>& prev, float distThr,
+                                 double out[kTerms]) {
+    const int ntiles = (npx + 1023) / 1024;
+    std::vector<double> tiles((size_t)ntiles * kTerms);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < ntiles; ++k) {
+        std::vector<double> v(256 * kTerms);
+        float o[kTerms];
+        for (int j = 0; j < 256; ++j) {
+            float sm[kTerms] = {0}, cp[kTerms] = {0};
+            for (int i = 0; i < 4; ++i) {
+                const int p = 1024 * k + j + 256 * i;
+                if (p >= npx) continue;
+                icp_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, o);
+                if (o[kTerms - 1] == 0.f) continue;  // rejected: every term is zero
+                for (int q = 0; q < kTerms - 1; ++q) {
+                    const float y = o[q] - cp[q];
+                    const float tt = sm[q] + y;
+                    cp[q] = (tt - sm[q]) - y;
+                    sm[q] = tt;
+                }
+                sm[kTerms - 1] += 1.0f;
+            }
+            for (int q = 0; q < kTerms; ++q) v[(size_t)j * kTerms + q] = (double)sm[q] - (double)cp[q];
+        }
+        for (int q = 0; q < kTerms; ++q) {
+            double w[4];
+            for (int wv = 0; wv < 4; ++wv) {
+                double lane[64];
+                for (int l = 0; l < 64; ++l) lane[l] = v[(size_t)(64 * wv + l) * kTerms + q];
+                for (int d = 32; d > 0; d >>= 1)
+                    for (int l = 0; l < d; ++l) lane[l] = lane[l] + lane[l + d];
+                w[wv] = lane[0];
+            }
+            tiles[(size_t)k * kTerms + q] = ((w[0] + w[1]) + w[2]) + w[3];
+        }
+    }
+    for (int q = 0; q < kTerms; ++q) {
+        double seg[8];
+        for (int j = 0; j < 8; ++j) {
+            double sacc = 0.0;
+            for (int k = j; k < ntiles; k += 8) sacc += tiles[(size_t)k * kTerms + q];
+            seg[j] = sacc;
+        }
+        double tot = 0.0;
+        for (int j = 0; j < 8; ++j) tot += seg[j];
+        out[q] = tot;
+    }
+}
+
+inline void icp_sums_compensated(const Cam<double>& cam, const double* vcur, int npx, const double R[9],
+                                 const double t[3], const double* vref, const double* nref, const Rigid<double>& prev,
+                                 double distThr, double out[kTerms]) {
+    for (int q = 0; q < kTerms; ++q) out[q] = 0.0;
+    double o[kTerms];
+    for (int p = 0; p < npx; ++p) {
+        icp_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, o);
+        for (int q = 0; q < kTerms; ++q) out[q] += o[q];
+    }
+}
+
+// The compensated mode's iteration loop; (Rd, td) in/out, the pose in FP64.
+template <class T>
+IcpResult icp_compensated(const Cam<T>& cam, const IcpConfig& cfg, const T* const* vpyr, const T* vref,
+                          const T* nref, const Rigid<T>& prev, double Rd[9], double td[3]) {
+    IcpResult res{1, 0, 0, 0};
+    const int L = cfg.levels;
+    const T step = (T)cfg.dist_threshold / (T)L;
+    const T distThr = (T)cfg.dist_threshold + step;
+    size_t kgs = (size_t)cam.W * cam.H;
+    for (int level = L - 1; level >= 0; --level) {
+        const unsigned thr = (unsigned)((double)(kgs >> (2 * level)) * 0.1);
+        const int Wl = cam.W >> level, Hl = cam.H >> level, npx = Wl * Hl;
+        for (int it = 0; it < cfg.iters[level]; ++it) {
+            T R[9], t[3];
+            for (int k = 0; k < 9; ++k) R[k] = (T)Rd[k];
+            for (int k = 0; k < 3; ++k) t[k] = (T)td[k];
+            double sums[kTerms];
+            icp_sums_compensated(cam, vpyr[level], npx, R, t, vref, nref, prev, distThr, sums);
+            res.total_iters++;
+            const int groups = (npx + 127) / 128;
+            kgs = (size_t)((groups + 127) / 128) * 128;
+            const double cnt = sums[kTerms - 1];
+            res.count = (float)cnt;
+            res.error = cnt > 0 ? (float)(std::sqrt(sums[kTerms - 2]) / cnt) : 3.402823466e38f;
+            if (std::getenv("OR_ICP_TRACE"))
+                std::fprintf(stderr, "  L%d it%d err %.9g cnt %.0f\n", level, it, (double)res.error, cnt);
+            if (cnt < (double)thr) break;
+            const int r = nuif_se3::gn_step(sums, nuif_se3::kGnStep2, Rd, td);
+            if (r == 2) {
+                res.ok = 0;
+                return res;
+            }
+            if (r == 1) break;
+        }
+    }
+    return res;
+}
+
 // src/Foundation/NuiMatrixUtilities.cpp:7-52 rodrigues2(...).norm(), for an
 // (already orthonormal) rotation: 0 for tiny rotations, else the angle.
 template <class T> T rodrigues_norm(const T Rm[9]) {
@@ -473,6 +596,38 @@ struct VolS16 {
         v[2 * i + 1] = (int16_t)w;
     }
     void reset_voxel(size_t i) { v[2 * i] = 32767; v[2 * i + 1] = 0; }  // (SHORT_NAN, 0)
+    const void* words() const { return v.data(); }
+};
+// FP16 tsdf (round to nearest even) + uint16 weight, in the GPU's 32-bit word
+// (half bits | weight << 16); FP32 arithmetic.
+struct VolF16 {
+    std::vector<uint32_t> v;
+    void init(size_t n) { v.assign(n, (uint32_t)f16_from_float(1.0f)); }
+    template <class T> T tsdf(size_t i) const { return (T)f16_to_float((uint16_t)(v[i] & 0xffffu)); }
+    int weight(size_t i) const { return (int)(v[i] >> 16); }
+    template <class T> void set(size_t i, T f, int w) {
+        v[i] = (uint32_t)f16_from_float((float)f) | ((uint32_t)(uint16_t)w << 16);
+    }
+    void reset_voxel(size_t i) { v[i] = (uint32_t)f16_from_float(1.0f); }
+    const void* words() const { return v.data(); }
+};
+// FP32 tsdf + FP32 weight (the GPU's float pair).
+struct VolF32 {
+    std::vector<float> v;  // 2 per voxel
+    void init(size_t n) {
+        v.assign(2 * n, 0.f);
+        for (size_t i = 0; i < n; ++i) v[2 * i] = 1.0f;
+    }
+    template <class T> T tsdf(size_t i) const { return (T)v[2 * i]; }
+    int weight(size_t i) const { return (int)v[2 * i + 1]; }
+    template <class T> void set(size_t i, T f, int w) {
+        v[2 * i] = (float)f;
+        v[2 * i + 1] = (float)w;
+    }
+    void reset_voxel(size_t i) {
+        v[2 * i] = 1.0f;
+        v[2 * i + 1] = 0.f;
+    }
     const void* words() const { return v.data(); }
 };
 struct VolF64 {

@@ -145,6 +145,7 @@ struct EngineCfg {
     int max_color_weight;
     int voxel_shift;       // > 0: shifting volume (VO/NuiKinfuVolumeConfig.cpp:13 voxel_shift)
     float basis[3];        // translateBasis
+    int icp_mode = 0;      // 0: the reference's ICP loop (icp), 1: the compensated mode (icp_compensated)
 };
 
 // OCL/NuiKinfuOpenCLShiftScene.cpp:119-245 fetchSlice reads the slab with
@@ -165,6 +166,7 @@ template <class T, class Vol> struct Engine {
     std::vector<std::vector<T>> dpyr, vpyr;
     T R[9], t[3];
     T Rlast[9], tlast[3];
+    double Rd[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, td[3] = {0, 0, 0};  // icp_mode 1: the pose in FP64
     int frames = 0;
     IcpResult last{1, 0, 0, 0};
     long long nupd_last = 0;
@@ -301,13 +303,25 @@ template <class T, class Vol> struct Engine {
             ic.dist_threshold = c.dist_threshold;
             std::vector<const T*> vp(c.levels);
             for (int l = 0; l < c.levels; ++l) vp[l] = vpyr[l].data();
-            T Rn[9], tn[3];
-            std::memcpy(Rn, R, sizeof(Rn));
-            std::memcpy(tn, t, sizeof(tn));
-            last = icp<T>(cam, ic, vp.data(), vref.data(), nref.data(), pose, Rn, tn);
-            if (!last.ok) return 0;
-            std::memcpy(R, Rn, sizeof(Rn));
-            std::memcpy(t, tn, sizeof(tn));
+            if (c.icp_mode == 1) {
+                double Rn[9], tn[3];
+                std::memcpy(Rn, Rd, sizeof(Rn));
+                std::memcpy(tn, td, sizeof(tn));
+                last = icp_compensated<T>(cam, ic, vp.data(), vref.data(), nref.data(), pose, Rn, tn);
+                if (!last.ok) return 0;
+                std::memcpy(Rd, Rn, sizeof(Rn));
+                std::memcpy(td, tn, sizeof(tn));
+                for (int k = 0; k < 9; ++k) R[k] = (T)Rd[k];
+                for (int k = 0; k < 3; ++k) t[k] = (T)td[k];
+            } else {
+                T Rn[9], tn[3];
+                std::memcpy(Rn, R, sizeof(Rn));
+                std::memcpy(tn, t, sizeof(tn));
+                last = icp<T>(cam, ic, vp.data(), vref.data(), nref.data(), pose, Rn, tn);
+                if (!last.ok) return 0;
+                std::memcpy(R, Rn, sizeof(Rn));
+                std::memcpy(t, tn, sizeof(tn));
+            }
             // integration gate (VO/NuiKinfuTrackingEngine.cpp:113-127) on global translations
             // (local + the shifting volume's offset; NuiCameraPos::getGlobalTranslation)
             T Rinv[9], Rrel[9];
@@ -319,6 +333,8 @@ template <class T, class Vol> struct Engine {
             T tnorm = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
             if ((rnorm + tnorm) / T(2) >= (T)c.gate) {
                 integrate_frame(bgra);  // may shift the volume and move t (local)
+                if (c.voxel_shift > 0)
+                    for (int k = 0; k < 3; ++k) td[k] = (double)t[k];  // the shift moved the local frame
                 std::memcpy(Rlast, R, sizeof(Rlast));
                 std::memcpy(tlast, t, sizeof(tlast));
             }
@@ -338,11 +354,22 @@ template <class T, class Vol> struct EngineBox {
 
 }  // namespace
 
-// Engine handle API.  prec: 0 = float compat (short2 volume), 1 = FP64 oracle.
+// Engine handle API.  prec: 0 = float compat with the reference's short2 volume,
+// 1 = FP64 oracle, 2 = float with FP16 tsdf + uint16 weight, 3 = float with FP32
+// tsdf + FP32 weight (the GPU's storages).
 struct OrEngine {
     int prec;
     void* p;
 };
+
+template <class F> auto with_engine(OrEngine* h, F&& f) {
+    switch (h->prec) {
+        case 0: return f(((EngineBox<float, VolS16>*)h->p)->e);
+        case 2: return f(((EngineBox<float, VolF16>*)h->p)->e);
+        case 3: return f(((EngineBox<float, VolF32>*)h->p)->e);
+        default: return f(((EngineBox<double, VolF64>*)h->p)->e);
+    }
+}
 
 OR_EXPORT void* or_engine_create(int prec, int W, int H, const float* cam6, int filter_radius, float sigma_space,
                                  float sigma_depth, float depth_threshold, float dist_threshold, int levels,
@@ -367,80 +394,74 @@ OR_EXPORT void* or_engine_create(int prec, int W, int H, const float* cam6, int 
     c.basis[0] = c.basis[1] = c.basis[2] = 0.f;
     OrEngine* h = new OrEngine;
     h->prec = prec;
-    if (prec == 0) {
-        auto* b = new EngineBox<float, VolS16>(c);
-        if (color) b->e.cvol.assign((size_t)N * N * N * 4, 0);
-        h->p = b;
-    } else {
-        auto* b = new EngineBox<double, VolF64>(c);
-        if (color) b->e.cvol.assign((size_t)N * N * N * 4, 0);
-        h->p = b;
+    switch (prec) {
+        case 0: h->p = new EngineBox<float, VolS16>(c); break;
+        case 2: h->p = new EngineBox<float, VolF16>(c); break;
+        case 3: h->p = new EngineBox<float, VolF32>(c); break;
+        default: h->prec = 1; h->p = new EngineBox<double, VolF64>(c); break;
     }
+    with_engine(h, [&](auto& e) {
+        if (color) e.cvol.assign((size_t)N * N * N * 4, 0);
+        return 0;
+    });
     return h;
 }
 
 OR_EXPORT void or_engine_destroy(void* hp) {
     OrEngine* h = (OrEngine*)hp;
     if (!h) return;
-    if (h->prec == 0)
-        delete (EngineBox<float, VolS16>*)h->p;
-    else
-        delete (EngineBox<double, VolF64>*)h->p;
+    switch (h->prec) {
+        case 0: delete (EngineBox<float, VolS16>*)h->p; break;
+        case 2: delete (EngineBox<float, VolF16>*)h->p; break;
+        case 3: delete (EngineBox<float, VolF32>*)h->p; break;
+        default: delete (EngineBox<double, VolF64>*)h->p; break;
+    }
     delete h;
 }
 
-// Runs one frame. pose12 out: R (row-major) + t as double. Returns 1 ok / 0 failure.
+// Runs one frame. pose12 out: R (row-major) + t as double (the FP64 pose in the
+// compensated mode). Returns 1 ok / 0 failure.
 OR_EXPORT int or_engine_run(void* hp, const uint16_t* raw, const uint8_t* bgra, double* pose12, double* icp_info3) {
-    OrEngine* h = (OrEngine*)hp;
-    int ok;
-    auto fill = [&](auto& e) {
+    return with_engine((OrEngine*)hp, [&](auto& e) {
+        const int ok = e.run(raw, bgra);
         if (pose12) {
-            for (int k = 0; k < 9; ++k) pose12[k] = (double)e.R[k];
-            for (int k = 0; k < 3; ++k) pose12[9 + k] = (double)e.t[k];
+            for (int k = 0; k < 9; ++k) pose12[k] = e.c.icp_mode == 1 ? e.Rd[k] : (double)e.R[k];
+            for (int k = 0; k < 3; ++k) pose12[9 + k] = e.c.icp_mode == 1 ? e.td[k] : (double)e.t[k];
         }
         if (icp_info3) {
             icp_info3[0] = e.last.error;
             icp_info3[1] = e.last.count;
             icp_info3[2] = (double)e.nupd_last;
         }
-    };
-    if (h->prec == 0) {
-        auto& e = ((EngineBox<float, VolS16>*)h->p)->e;
-        ok = e.run(raw, bgra);
-        fill(e);
-    } else {
-        auto& e = ((EngineBox<double, VolF64>*)h->p)->e;
-        ok = e.run(raw, bgra);
-        fill(e);
-    }
-    return ok;
+        return ok;
+    });
+}
+
+// ICP loop: 0 the reference's (icp), 1 the compensated mode (icp_compensated); before the first frame.
+OR_EXPORT void or_engine_set_icp_mode(void* hp, int mode) {
+    with_engine((OrEngine*)hp, [&](auto& e) {
+        e.c.icp_mode = mode;
+        return 0;
+    });
 }
 
 // Shifting volume (OCL/NuiKinfuOpenCLShiftScene.cpp): set before the first frame.
 OR_EXPORT void or_engine_set_shift(void* hp, int voxel_shift, const float* basis) {
-    OrEngine* h = (OrEngine*)hp;
-    auto set = [&](auto& e) {
+    with_engine((OrEngine*)hp, [&](auto& e) {
         e.c.voxel_shift = voxel_shift;
         for (int k = 0; k < 3; ++k) e.c.basis[k] = basis ? basis[k] : 0.f;
-    };
-    if (h->prec == 0)
-        set(((EngineBox<float, VolS16>*)h->p)->e);
-    else
-        set(((EngineBox<double, VolF64>*)h->p)->e);
+        return 0;
+    });
 }
 // voxel offset (int[3]) and its size in metres (double[3], the engine's arithmetic type)
 OR_EXPORT void or_engine_offset(void* hp, int* off3, double* size3) {
-    OrEngine* h = (OrEngine*)hp;
-    auto rd = [&](auto& e) {
+    with_engine((OrEngine*)hp, [&](auto& e) {
         for (int k = 0; k < 3; ++k) {
             if (off3) off3[k] = e.off[k];
             if (size3) size3[k] = (double)e.offset_size(k);
         }
-    };
-    if (h->prec == 0)
-        rd(((EngineBox<float, VolS16>*)h->p)->e);
-    else
-        rd(((EngineBox<double, VolF64>*)h->p)->e);
+        return 0;
+    });
 }
 // the evicted points (compat engine): count, and up to cap points / colours
 OR_EXPORT long long or_engine_cache(void* hp, float* pts, float* cols, long long cap) {
@@ -457,18 +478,28 @@ OR_EXPORT long long or_engine_cache(void* hp, float* pts, float* cols, long long
 // Volume readback: tsdf (float, normalised) and weight per voxel in the
 // reference layout ((y*N + x)*N + z).
 OR_EXPORT void or_engine_volume(void* hp, float* tsdf, int* weight) {
-    OrEngine* h = (OrEngine*)hp;
-    auto rd = [&](auto& e) {
+    with_engine((OrEngine*)hp, [&](auto& e) {
         size_t n = (size_t)e.c.N * e.c.N * e.c.N;
         for (size_t i = 0; i < n; ++i) {
             if (tsdf) tsdf[i] = e.vol.template tsdf<float>(i);
             if (weight) weight[i] = e.vol.weight(i);
         }
-    };
-    if (h->prec == 0)
-        rd(((EngineBox<float, VolS16>*)h->p)->e);
-    else
-        rd(((EngineBox<double, VolF64>*)h->p)->e);
+        return 0;
+    });
+}
+
+// Raw volume words of a float engine, in the GPU's storage layout (S16: 2 x int16,
+// F16: one uint32, F32: 2 x float per voxel); returns the byte count (-1 for FP64).
+OR_EXPORT long long or_engine_volume_words(void* hp, void* out) {
+    OrEngine* h = (OrEngine*)hp;
+    if (h->prec == 1) return -1;
+    const size_t n = with_engine(h, [&](auto& e) { return (size_t)e.c.N * e.c.N * e.c.N; });
+    const size_t bytes = n * (h->prec == 3 ? 8 : 4);
+    if (out) with_engine(h, [&](auto& e) {
+        std::memcpy(out, e.vol.words(), bytes);
+        return 0;
+    });
+    return (long long)bytes;
 }
 
 // Raw short2 volume of the compat engine (2 x int16 per voxel).
@@ -482,18 +513,97 @@ OR_EXPORT int or_engine_volume_raw(void* hp, int16_t* out) {
 
 // Reference maps (world vmap/nmap, float) after the last frame.
 OR_EXPORT void or_engine_refmaps(void* hp, float* vmap, float* nmap) {
-    OrEngine* h = (OrEngine*)hp;
-    auto rd = [&](auto& e) {
+    with_engine((OrEngine*)hp, [&](auto& e) {
         for (size_t i = 0; i < e.vref.size(); ++i) {
             if (vmap) vmap[i] = (float)e.vref[i];
             if (nmap) nmap[i] = (float)e.nref[i];
         }
-    };
-    if (h->prec == 0)
-        rd(((EngineBox<float, VolS16>*)h->p)->e);
-    else
-        rd(((EngineBox<double, VolF64>*)h->p)->e);
+        return 0;
+    });
 }
+
+// The same maps in the engine's arithmetic (FP64 engine: doubles), for error reports.
+OR_EXPORT void or_engine_refmaps_d(void* hp, double* vmap, double* nmap) {
+    with_engine((OrEngine*)hp, [&](auto& e) {
+        for (size_t i = 0; i < e.vref.size(); ++i) {
+            if (vmap) vmap[i] = (double)e.vref[i];
+            if (nmap) nmap[i] = (double)e.nref[i];
+        }
+        return 0;
+    });
+}
+
+// ---- an FP64 dense volume driven with given poses: the precision reference for the
+// TSDF and raycast of a GPU session (same frames, same poses, every operation in double)
+struct OrVolD {
+    Tsdf<double> prm;
+    VolF64 vol;
+};
+OR_EXPORT void* or_vold_create(int N, float dim, float trunc) {
+    OrVolD* v = new OrVolD;
+    v->prm = make_tsdf<double>(N, dim, trunc);
+    v->vol.init((size_t)N * N * N);
+    return v;
+}
+OR_EXPORT void or_vold_destroy(void* h) { delete (OrVolD*)h; }
+// one frame: raw depth -> passing filter -> ray-length scaling -> integration at (R, t), in double
+OR_EXPORT long long or_vold_integrate(void* h, const uint16_t* raw, int W, int H, const float* cam6, const float* R9,
+                                      const float* t3) {
+    OrVolD* v = (OrVolD*)h;
+    const Cam<double> cam = make_cam<double>(cam6, W, H);
+    std::vector<double> depth((size_t)W * H);
+    passing_filter<double>(raw, depth.data(), depth.size(), cam.dmin, cam.dmax);
+    scale_depths<double>(depth.data(), W, H, cam);
+    double R[9], t[3];
+    for (int k = 0; k < 9; ++k) R[k] = R9[k];
+    for (int k = 0; k < 3; ++k) t[k] = t3[k];
+    return integrate<double, VolF64>(depth.data(), nullptr, v->vol, nullptr, v->prm, cam, make_rigid<double>(R, t), 128);
+}
+OR_EXPORT void or_vold_volume(void* h, double* tsdf, int* weight) {
+    OrVolD* v = (OrVolD*)h;
+    const size_t n = v->vol.f.size();
+    for (size_t i = 0; i < n; ++i) {
+        if (tsdf) tsdf[i] = v->vol.f[i];
+        if (weight) weight[i] = v->vol.w[i];
+    }
+}
+OR_EXPORT void or_vold_raycast(void* h, int W, int H, const float* cam6, const float* R9, const float* t3, double* vmap,
+                               double* nmap) {
+    OrVolD* v = (OrVolD*)h;
+    double R[9], t[3];
+    for (int k = 0; k < 9; ++k) R[k] = R9[k];
+    for (int k = 0; k < 3; ++k) t[k] = t3[k];
+    raycast<double, VolF64>(v->vol, nullptr, v->prm, make_cam<double>(cam6, W, H), make_rigid<double>(R, t), vmap, nmap,
+                            nullptr);
+}
+
+// FP64 algebra of the compensated ICP mode (shared with the GPU, se3_math.h)
+OR_EXPORT int or_se3_exp(const double* w3, const double* u3, double* R9, double* t3) {
+    return nuif_se3::se3_exp(w3, u3, R9, t3) ? 1 : 0;
+}
+OR_EXPORT void or_se3_coefficients(double th2, double* abc) { nuif_se3::se3_coefficients(th2, abc, abc + 1, abc + 2); }
+OR_EXPORT int or_ldlt6_solve(const double* A36, const double* b6, double* x6, double* det) {
+    double F[36];
+    std::memcpy(F, A36, sizeof(F));
+    if (!nuif_se3::ldlt6(F, det)) return 0;
+    nuif_se3::ldlt6_solve(F, b6, x6);
+    return 1;
+}
+OR_EXPORT int or_gn_step(const double* sums29, double step2, double* R9, double* t3) {
+    return nuif_se3::gn_step(sums29, step2, R9, t3);
+}
+// One iteration's sums of the compensated order (float instance: the GPU's order)
+OR_EXPORT void or_icp_sums_comp_f(const float* cam6, int W, int H, const float* vcur, int level, const float* R9,
+                                  const float* t3, const float* vref, const float* nref, const float* prevR9,
+                                  const float* prevt3, float distThr, double* sums29) {
+    Cam<float> cam = make_cam<float>(cam6, W, H);
+    icp_sums_compensated(cam, vcur, (W >> level) * (H >> level), R9, t3, vref, nref, make_rigid<float>(prevR9, prevt3),
+                         distThr, sums29);
+}
+
+// IEEE half conversions of the F16 storage (round to nearest even; exact back)
+OR_EXPORT uint16_t or_f16_from_float(float f) { return f16_from_float(f); }
+OR_EXPORT float or_f16_to_float(uint16_t h) { return f16_to_float(h); }
 
 // ---- stateless dense stages on caller-provided short2 volumes (float compat)
 OR_EXPORT long long or_integrate_s16_f(const float* depth_scaled, const uint8_t* bgra, int16_t* vol2,

@@ -63,6 +63,21 @@ def lib():
             "or_raycast_s16_wrap_f": (None, [i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, f32p, f32p, vp, i32p]),
             "or_points_dense_slice": (cll, [vp, vp, ci, ci, ci, ci, f32p, i32p, i32p, i32p, vp, vp, cll]),
             "or_engine_set_shift": (None, [vp, ci, f32p]),
+            "or_engine_set_icp_mode": (None, [vp, ci]),
+            "or_engine_volume_words": (cll, [vp, vp]),
+            "or_engine_refmaps_d": (None, [vp, f64p, f64p]),
+            "or_f16_from_float": (C.c_uint16, [cf]),
+            "or_se3_exp": (ci, [f64p, f64p, f64p, f64p]),
+            "or_vold_create": (vp, [ci, cf, cf]),
+            "or_vold_destroy": (None, [vp]),
+            "or_vold_integrate": (cll, [vp, u16p, ci, ci, f32p, f32p, f32p]),
+            "or_vold_volume": (None, [vp, f64p, i32p]),
+            "or_vold_raycast": (None, [vp, ci, ci, f32p, f32p, f32p, f64p, f64p]),
+            "or_se3_coefficients": (None, [cd, f64p]),
+            "or_ldlt6_solve": (ci, [f64p, f64p, f64p, C.POINTER(cd)]),
+            "or_gn_step": (ci, [f64p, cd, f64p, f64p]),
+            "or_icp_sums_comp_f": (None, [f32p, ci, ci, f32p, ci, f32p, f32p, f32p, f32p, f32p, f32p, cf, f64p]),
+            "or_f16_to_float": (cf, [C.c_uint16]),
             "or_engine_offset": (None, [vp, i32p, f64p]),
             "or_engine_cache": (cll, [vp, vp, vp, cll]),
             "or_mc_count_s16": (cll, [i16p, ci]),
@@ -116,16 +131,23 @@ def lib():
 class OracleEngine:
     """RunTracking of the reference over a dense scene, on the CPU.
 
-    prec 0: float arithmetic of the reference with short2 volume; prec 1: FP64 oracle.
+    prec 0: float arithmetic with the volume storage `storage` (0: the reference's
+    short2, 1: FP16 tsdf + uint16 weight, 2: FP32 tsdf + FP32 weight); prec 1: FP64 oracle.
+    compensated: the compensated ICP mode (Gauss-Newton, FP64 pose) instead of the
+    reference's loop.
     """
 
-    def __init__(self, wl, prec=0, gate=0.0, color=False, iterations=None):
+    def __init__(self, wl, prec=0, gate=0.0, color=False, iterations=None, compensated=False, storage=0):
         its = np.array(list(iterations or wl.iterations) + [0] * 8, np.int32)[:8]
         self.wl = wl
         self.prec = prec
-        self.h = lib().or_engine_create(prec, wl.width, wl.height, wl.cam6(), 3, 0.08, 50.0, 0.3, 0.1,
+        self.storage = storage if prec == 0 else None
+        code = 1 if prec == 1 else {0: 0, 1: 2, 2: 3}[storage]
+        self.h = lib().or_engine_create(code, wl.width, wl.height, wl.cam6(), 3, 0.08, 50.0, 0.3, 0.1,
                                         len(iterations or wl.iterations), its, wl.volume_res, wl.volume_dim,
                                         wl.trunc, gate, 1 if color else 0)
+        if compensated:
+            lib().or_engine_set_icp_mode(self.h, 1)
 
     def run(self, depth, bgra=None):
         pose = np.zeros(12, np.float64)
@@ -140,6 +162,21 @@ class OracleEngine:
         w = np.empty(n, np.int32)
         lib().or_engine_volume(self.h, t, w)
         return t, w
+
+    def volume_words(self):
+        """The float engine's volume in the GPU's storage words (uint8 bytes)."""
+        n = lib().or_engine_volume_words(self.h, None)
+        out = np.empty(max(n, 0), np.uint8)
+        if n > 0:
+            lib().or_engine_volume_words(self.h, out.ctypes.data)
+        return out
+
+    def refmaps_d(self):
+        n = self.wl.width * self.wl.height * 3
+        v = np.empty(n, np.float64)
+        nm = np.empty(n, np.float64)
+        lib().or_engine_refmaps_d(self.h, v, nm)
+        return v.reshape(-1, 3), nm.reshape(-1, 3)
 
     def volume_raw(self):
         n = self.wl.volume_res ** 3
@@ -176,6 +213,47 @@ class OracleEngine:
     def close(self):
         if self.h:
             lib().or_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class OracleVolumeD:
+    """An FP64 dense volume integrated and raycast at given (float) poses: the precision
+    reference of a GPU session's TSDF and raycast (the same frames and poses in double)."""
+
+    def __init__(self, wl):
+        self.wl = wl
+        self.h = lib().or_vold_create(wl.volume_res, wl.volume_dim, wl.trunc)
+
+    def integrate(self, depth, R, t):
+        return lib().or_vold_integrate(self.h, np.ascontiguousarray(depth, np.uint16).reshape(-1), self.wl.width,
+                                       self.wl.height, self.wl.cam6(), np.asarray(R, np.float32).reshape(-1).copy(),
+                                       np.asarray(t, np.float32).reshape(-1).copy())
+
+    def volume(self):
+        n = self.wl.volume_res ** 3
+        t = np.empty(n, np.float64)
+        w = np.empty(n, np.int32)
+        lib().or_vold_volume(self.h, t, w)
+        return t, w
+
+    def raycast(self, R, t):
+        n = self.wl.width * self.wl.height * 3
+        v = np.empty(n, np.float64)
+        nm = np.empty(n, np.float64)
+        lib().or_vold_raycast(self.h, self.wl.width, self.wl.height, self.wl.cam6(),
+                              np.asarray(R, np.float32).reshape(-1).copy(), np.asarray(t, np.float32).reshape(-1).copy(),
+                              v, nm)
+        return v.reshape(-1, 3), nm.reshape(-1, 3)
+
+    def close(self):
+        if self.h:
+            lib().or_vold_destroy(self.h)
             self.h = None
 
     def __del__(self):

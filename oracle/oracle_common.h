@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -53,6 +54,53 @@ template <class T> inline T cl_round(T v) { return std::round(v); }
 template <class T> inline int to_int(T v) {
     if (!(v > T(-2147483648.0) && v < T(2147483647.0))) return INT32_MIN;
     return (int)v;
+}
+
+// IEEE binary16 <-> binary32.  f16_to_float is exact; f16_from_float rounds to
+// nearest, ties to even (as the GPU's __float2half_rn), overflow to infinity.
+inline float f16_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    uint32_t bits;
+    if (e == 0) {
+        if (m == 0) {
+            bits = sign;
+        } else {
+            const float v = (float)m * 0x1p-24f;
+            std::memcpy(&bits, &v, 4);
+            bits |= sign;
+        }
+    } else if (e == 31) {
+        bits = sign | 0x7f800000u | (m << 13);
+    } else {
+        bits = sign | ((e + 112u) << 23) | (m << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+inline uint16_t f16_from_float(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint16_t sign = (uint16_t)((x >> 16) & 0x8000u);
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((ax >> 13) & 0x1ffu));  // NaN (quiet)
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // rounds to >= 65520: infinity
+    if (ax < 0x38800000u) {  // below the smallest normal half (2^-14): subnormal or zero
+        // value = ax as float; half subnormal unit 2^-24: q = round_even(value * 2^24)
+        float v;
+        std::memcpy(&v, &ax, 4);
+        const float scaled = v * 0x1p24f;  // exact (power of two)
+        float q = std::nearbyint(scaled);  // default rounding mode: to nearest even
+        return (uint16_t)(sign | (uint16_t)q);
+    }
+    // normal: keep 10 mantissa bits, round to nearest even on the 13 dropped bits
+    const uint32_t e = (ax >> 23) - 112u;  // half exponent
+    uint32_t m = ax & 0x7fffffu;
+    uint32_t h = (e << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1u;  // carry may move into the exponent
+    return (uint16_t)(sign | h);
 }
 
 // K/gpu_def.h:13-27 (NuiCLCameraParams), 29-34 (NuiCLRigidTransform), 5-11 (TsdfParams)
