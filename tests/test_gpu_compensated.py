@@ -34,10 +34,24 @@ def announce(msg):
 
 
 def pose_err(Ra, ta, Rb, tb):
+    """Translation distance (m) and rotation angle (deg) between two poses.  The angle
+    is atan2(|vee(skew part)|, (tr - 1) / 2): well conditioned near zero, where
+    arccos of the trace turns the float32 rounding of R into ~0.01 deg."""
     dt = np.linalg.norm(np.asarray(ta, np.float64) - np.asarray(tb, np.float64))
     Rr = np.asarray(Ra, np.float64).T @ np.asarray(Rb, np.float64)
-    ang = np.degrees(np.arccos(np.clip((np.trace(Rr) - 1) / 2, -1, 1)))
+    w = 0.5 * np.array([Rr[2, 1] - Rr[1, 2], Rr[0, 2] - Rr[2, 0], Rr[1, 0] - Rr[0, 1]])
+    ang = np.degrees(np.arctan2(np.linalg.norm(w), (np.trace(Rr) - 1) / 2))
     return dt, ang
+
+
+def same_bits_or_nan(a, b, msg=""):
+    """Identical NaN positions and identical bits elsewhere (a computed NaN's sign bit
+    differs between the host's and the GPU's arithmetic)."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=msg)
+    m = ~np.isnan(a)
+    np.testing.assert_array_equal(a[m].view(np.uint32), b[m].view(np.uint32), err_msg=msg)
 
 
 def _gpu_run(wl, storage, frames):
@@ -151,8 +165,8 @@ def test_compensated_engine_matches_oracle(gpu, size, storage):
     vg, ng = eng.maps()
     vr, nr = ref.refmaps()
     assert (~np.isnan(vr[:, 2])).sum() > wl.width * wl.height // 4
-    np.testing.assert_array_equal(vg.view(np.uint32), vr.view(np.uint32))
-    np.testing.assert_array_equal(ng.view(np.uint32), nr.view(np.uint32))
+    same_bits_or_nan(vg, vr, "vmap")
+    same_bits_or_nan(ng, nr, "nmap")
     eng.close()
 
 

@@ -48,7 +48,9 @@ def rot_y(deg):
 def pose_err(Ra, ta, Rb, tb):
     dt = np.linalg.norm(np.asarray(ta, np.float64) - np.asarray(tb, np.float64))
     Rr = np.asarray(Ra, np.float64).T @ np.asarray(Rb, np.float64)
-    ang = np.degrees(np.arccos(np.clip((np.trace(Rr) - 1) / 2, -1, 1)))
+    # atan2 of the skew part and the trace: well conditioned near zero angle
+    w = 0.5 * np.array([Rr[2, 1] - Rr[1, 2], Rr[0, 2] - Rr[2, 0], Rr[1, 0] - Rr[0, 1]])
+    ang = np.degrees(np.arctan2(np.linalg.norm(w), (np.trace(Rr) - 1) / 2))
     return dt, ang
 
 
