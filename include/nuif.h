@@ -222,6 +222,18 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
                       const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
                       const nuif_icp_config* cfg, float* history, int capacity, nuif_stream_t s);
 
+/* nuif_icp_estimate without the partials workspace, for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference.  The partial sums live in a static
+ * device array of nuif_icp_static_scratch_bytes() (1 MiB) per GPU, held until the
+ * process ends; an image whose nuif_icp_partials_bytes exceed it is refused with
+ * NUIF_ERR_INVALID (1280 x 720 fits).  Nothing is allocated or freed.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (different streams) race. */
+size_t nuif_icp_static_scratch_bytes(void);
+int nuif_icp_estimate_static(void* state, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
+                             const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
+                             const nuif_icp_config* cfg, float* history, int capacity, nuif_stream_t s);
+
 /* One session of nuif_icp_estimate_batch (the arguments of nuif_icp_estimate). */
 typedef struct nuif_icp_session {
     void* state;                               /* nuif_icp_state_bytes() */

@@ -801,6 +801,34 @@ __global__ void icp_set_translation_kernel(IcpState* st, NuiCLRigidTransform* po
     }
 }
 
+// ---- the allocation-free variant's scratch ------------------------------------------
+// Partial sums of nuif_icp_estimate_static: a file-scope device array, one copy per GPU
+// for the life of the process.  Host code never passes `scratch` itself (its name is a
+// host-side stand-in, not a device address): the address comes from
+// hipGetSymbolAddress, resolved once per device with that device current.
+constexpr size_t kStaticScratchBytes = 1u << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch` (resolved on first use for the device).
+int static_scratch(void** out) {
+    int dev = 0;
+    NUIF_HIP(hipGetDevice(&dev));
+    NUIF_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        NUIF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        NUIF_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the ICP scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return NUIF_OK;
+}
+
 // ---- host side ---------------------------------------------------------------------
 
 long long poll_ticks() {
@@ -1072,6 +1100,24 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
     const int rc = make_job(in, &J);
     if (rc) return rc;
     return run_single(J, stream_of(s), true);
+}
+
+size_t nuif_icp_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+int nuif_icp_estimate_static(void* state, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
+                             const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
+                             const nuif_icp_config* cfg, float* history, int capacity, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(cam, "nuif_icp_estimate_static: null argument");
+    const size_t need = nuif_icp_partials_bytes((int)cam->depthImageWidth, (int)cam->depthImageHeight);
+    NUIF_CHECK_ARG(need <= kStaticScratchBytes,
+                   "nuif_icp_estimate_static: a %u x %u image needs %zu bytes of partial sums, above the %zu-byte "
+                   "static scratch (use nuif_icp_estimate with a workspace)",
+                   cam->depthImageWidth, cam->depthImageHeight, need, kStaticScratchBytes);
+    void* ptr = nullptr;
+    const int rc = static_scratch(&ptr);
+    if (rc) return rc;
+    return nuif_icp_estimate(state, (float*)ptr, cam, vmap_pyr, ref_vmap, ref_nmap, pose, cfg, history, capacity, s);
 }
 
 size_t nuif_icp_batch_table_bytes(int n) { return n > 0 ? ((size_t)n * sizeof(IcpJob) + 255) / 256 * 256 : 0; }

@@ -48,6 +48,10 @@ bool initialised();
 
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// device address of the allocation-free ICP entry point's static scratch on the
+// current device (resolved once per device)
+int static_scratch(void** out);
+
 // device-wide exclusive scan of n uint32 values (flags=false) or of (uint8 > 0)
 // (flags=true); scratch of nuif_scan_scratch_bytes(n); in == out is allowed
 int launch_scan(const void* in, bool flags, uint32_t* out, size_t n, uint32_t* total, void* scratch,

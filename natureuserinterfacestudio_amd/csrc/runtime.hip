@@ -107,7 +107,9 @@ int nuif_init(int device) {
     g_default_stream = s;
     g_device.store(device);
     g_generation.fetch_add(1);
-    return NUIF_OK;
+    // the allocation-free ICP entry point's scratch address on this device
+    void* p = nullptr;
+    return static_scratch(&p);
 }
 
 int nuif_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes, int* clock_khz) {

@@ -140,6 +140,9 @@ _SIGS = {
     "nuif_icp_end": (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_int, c_void_p]),
     "nuif_icp_record": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "nuif_icp_batch_table_bytes": (c_size_t, [c_int]),
+    "nuif_icp_static_scratch_bytes": (c_size_t, []),
+    "nuif_icp_estimate_static": (c_int, [c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
+                                         c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
     "nuif_icp_estimate_batch": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "nuif_icp_estimate": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                   c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
