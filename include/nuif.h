@@ -153,7 +153,8 @@ typedef struct nuif_icp_result {
     float t[3];
     float error;         /* m_error of the last executed iteration */
     float count;         /* m_count of the last executed iteration */
-    int32_t status;      /* 0 ok, 1 tracking failure (singular system), 2 the persistent kernel gave up */
+    int32_t status;      /* 0 ok, 1 tracking failure (singular system), 2 iterations left unrun
+                            (fewer launched than the frame's schedule needs) */
     int32_t iterations;  /* iterations that ran (not skipped by a break) */
     int32_t integrate;   /* integration gate decision of the last frame */
     int32_t frame;       /* frames finalised */
@@ -178,8 +179,8 @@ typedef struct nuif_icp_config {
     int reduction;                         /* NUIF_ICP_REDUCE_* */
 } nuif_icp_config;
 
-/* State block: the tracking state plus the work-queue ticket and completion counters of
- * the persistent ICP kernel (device memory, initialised by nuif_icp_state_init). */
+/* State block: the tracking state plus the current iteration ("epoch") of the frame
+ * (device memory, initialised by nuif_icp_state_init). */
 size_t nuif_icp_state_bytes(void);
 /* Workspace of nuif_icp_iteration for a width x height level-0 image. */
 size_t nuif_icp_partials_bytes(int width, int height);
@@ -188,14 +189,14 @@ int nuif_icp_state_init(void* state, const NuiCLRigidTransform* pose, nuif_strea
 int nuif_icp_state_read(const void* state, nuif_icp_result* out, nuif_stream_t s); /* synchronises s */
 /* Begin a frame: estimate <- pose, clear break/failure flags. */
 int nuif_icp_begin(void* state, const NuiCLRigidTransform* pose, nuif_stream_t s);
-/* One iteration at `level` (the persistent kernel run for a single epoch): association +
- * 29-term reduction and, in the workgroup that completes the reduction, the update of the
- * chosen order (reference: the LM accept/reject, FP64 LLT solve and Euler increment of
- * OCL/NuiKinfuOpenCLDepthTracker.cpp:240-497).  Skipped after a break at this level or a
+/* One iteration at `level` (two launches: association + the tiles' partial sums, then one
+ * block that reduces them in a fixed order and runs the update of the chosen order
+ * (reference: the LM accept/reject, FP64 LLT solve and Euler increment of
+ * OCL/NuiKinfuOpenCLDepthTracker.cpp:240-497)).  Skipped after a break at this level or a
  * failure.  vmap_level: current-frame vertices at the level; ref_vmap/ref_nmap: world maps
  * at full resolution; prev_pose: previous frame's transform; count_threshold: see
  * nuif_icp_count_thresholds.  At most 2^22 pixels per level.  Launches sharing one state
- * block must not overlap (they share its counters). */
+ * block or one partials workspace must not overlap. */
 int nuif_icp_iteration(void* state, float* partials, const NuiCLCameraParams* cam, const float* vmap_level,
                        int level, const float* ref_vmap, const float* ref_nmap,
                        const NuiCLRigidTransform* prev_pose, float dist_threshold, uint32_t count_threshold,
@@ -217,7 +218,8 @@ int nuif_icp_end(void* state, NuiCLRigidTransform* pose, float gate_threshold, f
 int nuif_icp_set_translation(void* state, NuiCLRigidTransform* pose, float* history, int capacity, const float* t3,
                              const float* offset3, nuif_stream_t s);
 /* All levels and iterations of one frame (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504
- * IterativeClosestPoint), no host synchronisation: begin, the persistent kernel, end. */
+ * IterativeClosestPoint), no host synchronisation: begin, two launches per iteration of the
+ * frame's schedule (an iteration after a break or a failure returns at once), end. */
 int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
                       const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
                       const nuif_icp_config* cfg, float* history, int capacity, nuif_stream_t s);
@@ -249,11 +251,15 @@ typedef struct nuif_icp_session {
 } nuif_icp_session;
 /* Device bytes of the session table of nuif_icp_estimate_batch. */
 size_t nuif_icp_batch_table_bytes(int n);
-/* The frame's ICP of n sessions (any image sizes, either order) in one persistent kernel:
- * blocks take 1024-pixel tiles of any session whose current iteration is open; each
- * session moves through its levels and iterations on its own and leaves early on a break
- * or failure.  No host synchronisation.  table: nuif_icp_batch_table_bytes(n) of device
- * memory, written on the stream (calls on one stream may reuse it). */
+/* The frame's ICP of n <= 4096 sessions (any image sizes, either order) in lockstep steps:
+ * step k is two launches for all sessions of an order, the 512-pixel tiles of every
+ * session's k-th iteration (whatever its level: each session moves through its levels on
+ * its own and is done after a break at its finest level or a failure) spread over the
+ * grid, then one block per session for the reduction and the update.  The steps are as
+ * many as the most iterations a session is configured for.  Results are bit for bit
+ * those of nuif_icp_estimate.  No host synchronisation.  table:
+ * nuif_icp_batch_table_bytes(n) of device memory, written on the stream (calls on one
+ * stream may reuse it). */
 int nuif_icp_estimate_batch(const nuif_icp_session* sessions, int n, void* table, nuif_stream_t s);
 /* Device int32 flags inside a state block: integration gate of the last frame, and
  * tracking success of the last frame (raycast runs only if set). */
@@ -263,6 +269,7 @@ const int32_t* nuif_icp_ok_flag(const void* state);
 int nuif_icp_count_thresholds(int width, int height, const nuif_icp_config* cfg, uint32_t* thresholds);
 
 /* ---- dense TSDF scene (OCL/NuiKinfuOpenCLScene.cpp, K/tsdfVolume.cl, K/raycast.cl) ---- */
+/* (nuif_fusion_round, the batched frame of many dense sessions, follows the scene entry points) */
 
 size_t nuif_tsdf_voxel_bytes(int storage);
 /* K/tsdfVolume.cl:192-211 initializeVolume: (32767,0) / (0,0) and colour 0 */
@@ -341,6 +348,47 @@ size_t nuif_tsdf_surface_blocks_bytes(const TsdfParams* params);
 int nuif_tsdf_surface_blocks_clear(uint8_t* surface_blocks, const TsdfParams* params, nuif_stream_t s);
 /* Effective truncation distance (OCL/NuiKinfuOpenCLScene.cpp:41: >= 2.1 * max cell). */
 int nuif_tsdf_params(int nx, int ny, int nz, float dx, float dy, float dz, float trunc, TsdfParams* out);
+
+/* ---- the batched frame of many dense sessions (the fusion server's round) ---------------
+ * The reference runs one session's frame on one queue (VO/NuiKinfuTrackingEngine.cpp:68-142
+ * over CLU/NuiOpenCLGlobal.cpp:175).  nuif_fusion_round runs one tracked frame of each of n
+ * dense sessions with one launch per stage for all of them: preprocessing (passing +
+ * bilateral filter, ray-length depth, vertex pyramid), the ICP of every session in
+ * lockstep ICP steps (nuif_icp_estimate_batch), depth bounds, column walk states,
+ * integration (gated by each session's integration flag) and raycast into the feedback maps
+ * (gated by its tracking flag).  Every session's results equal those of its own
+ * nuif_frame_preprocess / nuif_icp_estimate / nuif_tsdf_depth_bounds /
+ * nuif_tsdf_integrate_wrapped / nuif_tsdf_raycast_gated sequence, bit for bit.  Volumes need
+ * Nx, Ny multiples of 8 and Nz a multiple of 32, no wrap. */
+typedef struct nuif_round_session {
+    const uint16_t* depth;                     /* W*H raw depth, mm (device) */
+    float* depth_out;                          /* W*H passing-filter depth scaled to ray length */
+    float* depth_pyr[NUIF_ICP_MAX_LEVELS];     /* filtered depth per level */
+    float* bounds;                             /* nuif_tsdf_depth_bounds_bytes(W, H) */
+    const float* gaussian;                     /* 2*radius+1 host floats */
+    int radius;
+    float sigma_depth2_inv_half, depth_threshold;
+    nuif_icp_session icp;                      /* vmap_pyr: written by the preprocessing; ref maps:
+                                                  the feedback maps below (previous frame) */
+    void* volume;
+    uint8_t* color_volume;                     /* NULL: no colour integration */
+    const uint8_t* bgra;                       /* W*H BGRA frame or NULL */
+    TsdfParams params;
+    int storage;                               /* NUIF_TSDF_* */
+    int max_color_weight;
+    unsigned long long* updated;               /* device counter or NULL */
+    uint8_t* surface_blocks;                   /* nuif_tsdf_surface_blocks_bytes or NULL */
+    float* walk_states;                        /* nuif_tsdf_walk_states_bytes or NULL */
+    float* vmap;                               /* feedback maps: raycast output, W*H float3 */
+    float* nmap;
+    uint8_t* cmap;                             /* W*H uchar4 or NULL */
+} nuif_round_session;
+/* Bytes of the round's job table (device memory) and of its host staging copy (pinned host
+ * memory, nuif_host_alloc) for n sessions. */
+size_t nuif_fusion_round_table_bytes(int n);
+/* table / staging: nuif_fusion_round_table_bytes(n) each; the staging copy may be reused once
+ * the stream has passed this call's upload (e.g. after waiting for the round). */
+int nuif_fusion_round(const nuif_round_session* sessions, int n, void* table, void* staging, nuif_stream_t s);
 
 /* ---- prefix sum and compaction (OCL/NuiOpenCLPrefixSum.cpp:75-300, K/prefixSum.cl:163-232) ---- */
 

@@ -106,10 +106,9 @@ float nuifk_trunc(void* h);
 
 /* ---- multi-session fusion server ------------------------------------------
  * Sessions stay resident (volume, pyramids, pose, tracking state).  A round
- * submits one frame for each listed session; each session's pipeline is
- * enqueued on its own stream (replayed from a captured graph when the session
- * asks for it) so the GPU overlaps the sessions.  Latency of a frame = device
- * time from the round's start event to the end of that frame's pipeline. */
+ * submits one frame for each listed session (see nuifs_submit).  Latency of a
+ * frame = device time from the round's start event to the end of that frame's
+ * pipeline. */
 int nuifs_create(size_t hbm_budget_bytes, void** out);
 int nuifs_destroy(void* srv);
 /* Device bytes a session would hold (volume + frame / pyramid / map buffers). */
@@ -121,8 +120,16 @@ size_t nuifs_bytes_in_use(void* srv);
 int nuifs_session_count(void* srv);
 /* The engine handle of a session (for the nuifk_* read-back calls). */
 void* nuifs_engine(void* srv, int id);
-/* Enqueue one round: n sessions, device-resident depth frames (W*H uint16 each). */
+/* Enqueue one round: n sessions, device-resident depth frames (W*H uint16 each).  The
+ * tracked frames of plain dense sessions run as one batched round (nuif_fusion_round: one
+ * launch per stage for all of them, the ICP of all of them in lockstep steps) on
+ * the server's stream; every other frame (a session's first frame, hash and shifting
+ * scenes) runs on its session's own stream, overlapping the batch. */
 int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths);
+/* Batched rounds on (default, non-zero) or off: every frame on its session's stream. */
+int nuifs_set_batching(void* srv, int on);
+/* Frames of the last round that ran in the batch. */
+int nuifs_last_batch(void* srv);
 /* Wait for the last round; latency_ms[i] for the i-th submitted frame (may be NULL).
  * Tracking outcomes stay in each session's pose history (nuifk_poses). */
 int nuifs_wait(void* srv, float* latency_ms);

@@ -17,7 +17,7 @@ OBJ = os.path.join(ROOT, "_build", "obj")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NUIF_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["runtime.hip", "preprocess.hip", "icp.hip", "tsdf.hip", "mc.hip", "mesh.hip", "hash.hip", "scan.hip", "color.hip", "points.hip"]
+HIP_SOURCES = ["runtime.hip", "preprocess.hip", "icp.hip", "tsdf.hip", "mc.hip", "mesh.hip", "hash.hip", "scan.hip", "color.hip", "points.hip", "fusion.hip"]
 KINFU_SOURCES = ["NuiKinfuConfig.cpp", "NuiKinfuHIP.cpp", "NuiKinfuSession.cpp", "NuiKinfuHashScene.cpp",
                  "NuiKinfuColorTracker.cpp", "NuiFrameIO.cpp", "nuifk_capi.cpp"]
 

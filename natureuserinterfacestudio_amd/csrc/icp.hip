@@ -1,48 +1,43 @@
-// Projective point-to-plane ICP: every iteration of every session of a frame in
-// ONE persistent kernel.
+// Projective point-to-plane ICP: every iteration of every session of a frame,
+// without host synchronisation.
 //
 // The reference runs per iteration: icp_block_kernel (128-wide LDS trees of 29
 // floats), compute_sums, a blocking read-back, and the Levenberg-Marquardt
 // bookkeeping + Eigen FP64 6x6 solve on the host
 // (OCL/NuiKinfuOpenCLDepthTracker.cpp:240-504), for one session on one queue.
-// Here a frame's ICP for any number of sessions is three launches: begin (one
-// block per session), the persistent kernel, end (one block per session).
-//
-// Work queue.  Each session's state carries a 64-bit ticket: the current epoch
-// (level * 64 + iteration, or "finished") in the high word and the next
-// unclaimed tile of 1024 pixels in the low word.  A block claims a tile of any
-// session with an atomic increment, evaluates the 29 terms of its pixels and
-// writes the tile's partial sums; the block that completes the epoch's last
-// tile reduces the partials in a fixed order, runs the update (the reference's
-// LM step, or the compensated Gauss-Newton step) and publishes the next epoch
-// (the next iteration, the next level after a break or the last iteration, or
-// "finished") by exchanging the ticket.  A block holding a claim never waits,
-// so progress does not depend on how many blocks are resident; blocks without
-// work sleep and poll, and leave when every session is finished.  A poll loop
-// that runs longer than a bound set by the host leaves too, and the session's
-// end kernel then reports the frame as failed with a timeout status.
+// Here an iteration ("epoch": level * 64 + iteration) is two launches for any
+// number of sessions:
+//   tiles   every session's tiles of 512 pixels at its current epoch, spread over
+//           the grid: the 29 terms of each pixel and the tile's partial sums
+//           (plain stores: no atomics, no fences; the launch boundary orders them);
+//   finish  one block per session: the partials reduced in a fixed order, the
+//           update (the reference's LM step, or the compensated Gauss-Newton step)
+//           and the next epoch (the next iteration, the next level after a break or
+//           the last iteration, or "finished").
+// A frame is begin, the pairs of launches, end.  One session tracked alone runs its
+// planned epochs in order, each pair launched with exactly its level's tiles, and a
+// pair whose epoch is not the session's current one (after a break or a failure)
+// returns at once.  A batch runs in lockstep steps: step k carries every session's
+// k-th epoch, whatever its level, so the steps are as many as the most iterations
+// any session is configured for, and the tile launch is sized for the most tiles a
+// session can have at that step.
 //
 // Reduction orders (nuif_icp_config::reduction):
-//   reference    a tile is eight 128-pixel groups, each reduced with the
+//   reference    a tile is four 128-pixel groups, each reduced with the
 //                reference's tree (item i takes item i + d for d = 64..1); the
-//                last tile of every 128 groups to finish runs compute_sums' tree
-//                over them, and the last of those sums the group results in
-//                order, as the host does, then runs the LM step: float sums,
-//                Eigen-like LLT in FP64, float Euler increment.  Bit for bit the
-//                reference's arithmetic.
-//   compensated  thread j of tile k takes pixels 1024k + j + 256i (i < 4) with
+//                finish block runs compute_sums' tree over each chunk of 128
+//                group sums and sums the chunk results in order, as the host does,
+//                then the LM step: float sums, Eigen-like LLT in FP64, float Euler
+//                increment.  Bit for bit the reference's arithmetic.
+//   compensated  thread j of tile k takes pixels 512k + j + 256i (i < 2) with
 //                Kahan-compensated FP32 sums, FP64 across threads, tiles and
 //                segments in a fixed order; Gauss-Newton on the undamped normal
 //                equations (FP64 LDL^T), increment applied as an SE(3)
 //                exponential to an FP64 pose (se3_math.h).
 // Every sum is taken in a fixed order: results do not depend on which block
-// processed which tile.
-//
-// Memory ordering.  Writers of partials release (agent scope) before counting
-// themselves in; the block that goes on acquires.  Claimers read the epoch from
-// the ticket and the current estimate with agent-scope atomic loads, so their
-// caches are never invalidated wholesale.
+// processed which tile, nor on how many sessions share the launches.
 
+#include "batch_jobs.h"
 #include "det_math.h"
 #include "icp_math.h"
 #include "nuif_internal.h"
@@ -55,11 +50,15 @@ namespace nuif {
 
 constexpr int kTerms = NUIF_ICP_TERMS;
 constexpr int kBlock = 256;
-constexpr int kTilePx = 1024;                // pixels per work item
+constexpr int kFinBlock = 1024;              // threads of a finish block
+constexpr int kTilePx = nuif_se3::kCompTilePixels;  // pixels per tile (512)
+constexpr int kPxPerThread = kTilePx / kBlock;      // compensated: pixels per thread of a tile
 constexpr int kTileGroups = kTilePx / 128;   // reference order: 128-item groups per tile
 constexpr int kMaxChunks = 256;              // reference order: 128-group chunks per level (W*H <= 2^22)
 constexpr int kEpochIters = 64;              // iterations per level are below this
+constexpr int kMaxBatch = 4096;              // sessions per batch
 constexpr uint32_t kFinished = 0xffffffffu;  // epoch of a session whose frame is over
+constexpr uint32_t kAnyEpoch = 0xfffffffeu;  // batch launches: each session's current epoch
 static_assert(128 % kTileGroups == 0, "a tile lies in one compute_sums chunk");
 
 // Reduction orders (nuif_icp_config::reduction)
@@ -85,10 +84,7 @@ struct IcpState {
 static_assert(sizeof(IcpState) % 8 == 0, "state is copied as 32-bit words");
 
 struct IcpSync {
-    unsigned long long ticket;    // (epoch << 32) | next tile
-    uint32_t done;                // compensated: tiles finished in the epoch
-    uint32_t chunks_done;         // reference: chunks reduced in the epoch
-    uint32_t arrive[kMaxChunks];  // reference: tiles finished per 128-group chunk
+    uint32_t epoch;  // the current epoch (kFinished: the frame's iterations are over)
 };
 constexpr size_t kStateBytes = (sizeof(IcpState) + 255) / 256 * 256;
 constexpr size_t kSyncBytes = (sizeof(IcpSync) + 255) / 256 * 256;
@@ -132,9 +128,6 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
-__device__ __forceinline__ unsigned long long load_ticket(const IcpSync* sy) {
-    return __hip_atomic_load(&sy->ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // K/icp.cl:192-220 interpolateBilinear_withHoles.  Reads past the row end as
 // the reference does; a read past the buffer end is a hole.
@@ -206,11 +199,7 @@ __device__ __forceinline__ void icp_terms(const f3 vc, const TilePose& P, const 
     out[28] = 1.0f;
 }
 
-__device__ __forceinline__ double wave_tree64d(double v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
-    return v;
-}
+__device__ __forceinline__ double wave_tree64d(double v) { return wave_tree64_t(v); }
 
 // ---- the updates ------------------------------------------------------------------
 
@@ -340,10 +329,10 @@ __device__ uint32_t next_epoch(const IcpState& S, const IcpJob& J, int level, in
     return kFinished;
 }
 
-// The block that completes an epoch: state into LDS, update, write back,
-// publish the next epoch.  sums (FP64) or sumsf (reference, float) are in LDS.
+// The finish block of an epoch: state into LDS, update, write back, the next
+// epoch.  sums (FP64) or sumsf (reference, float) are in LDS.
 template <int MODE>
-__device__ void finish_epoch(const IcpJob& J, IcpState& S, const double* sums, const float* sumsf, int level,
+__device__ __attribute__((noinline)) void finish_epoch(const IcpJob& J, IcpState& S, const double* sums, const float* sumsf, int level,
                              int iter) {
     __shared__ double Av[27];
     __shared__ double ws[60];
@@ -373,245 +362,277 @@ __device__ void finish_epoch(const IcpJob& J, IcpState& S, const double* sums, c
     __syncthreads();
     uint32_t* dst = reinterpret_cast<uint32_t*>(st);
     for (int i = tid; i < kWords; i += blockDim.x) dst[i] = lds[i];
-    __syncthreads();  // the state is written
-    if (tid == 0) {
-        const uint32_t e = next_epoch(S, J, level, iter);
-        IcpSync* sy = sync_of(st);
-        __hip_atomic_store(&sy->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&sy->chunks_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        fence_release_agent();  // the state before the ticket
-        __hip_atomic_exchange(&sy->ticket, (unsigned long long)e << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (tid == 0) sync_of(st)->epoch = next_epoch(S, J, level, iter);
 }
 
-// ---- the persistent kernel ------------------------------------------------------
+// ---- the tile launch --------------------------------------------------------------
 
-struct Claim {
-    int s;          // session (-1: none, -2: all finished)
-    uint32_t epoch;
-    uint32_t tile;
+// LDS of a tile block (256 threads)
+struct IcpLdsBase {
+    IcpJob single;  // the job of a one-session launch (passed by value)
+    IcpJob job;     // the job of the session being processed
+    TilePose pose;
+    int level;
+};
+template <int MODE> struct IcpLds;
+template <> struct IcpLds<kRefOrder> : IcpLdsBase {
+    float xfer[2][64 * kTerms];  // the tree's d = 64 step of two groups
+};
+template <> struct IcpLds<kCompensated> : IcpLdsBase {
+    double wsum[4][kTerms];  // wave sums
 };
 
-// Thread 0: one pass over the sessions from `start`, claiming the first open tile.
-__device__ Claim claim_tile(const IcpJob* jobs, int n, int start) {
-    int nfin = 0;
-    for (int k = 0; k < n; ++k) {
-        const int s = (start + k) % n;
-        const IcpJob& J = jobs[s];
-        IcpSync* sy = sync_of(J.st);
-        const unsigned long long t = load_ticket(sy);
-        const uint32_t e = (uint32_t)(t >> 32);
-        if (e == kFinished) {
-            ++nfin;
-            continue;
+// Tile `tile` of level L.level of job L.job (with L.pose loaded): the 29 terms of its
+// pixels and its partial sums.
+template <int MODE>
+__device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
+    const int tid = threadIdx.x;
+    const IcpJob& J = L.job;
+    const int level = L.level;
+    const int npx = level_pixels(J.cam, level);
+    const float* __restrict__ vcur = J.vmap[level];
+
+    if constexpr (MODE == kRefOrder) {
+        // four 128-pixel groups, two at a time (one per 128 threads)
+        const int sub = tid >> 7, l = tid & 127, lane = l & 63, wave = l >> 6;
+        const int ngroups = (npx + 127) >> 7;
+        float* part = reinterpret_cast<float*>(J.partials);
+        for (int r = 0; r < kTileGroups / 2; ++r) {
+            const int g = (int)tile * kTileGroups + 2 * r + sub;
+            const int p = g * 128 + l;
+            float v[kTerms];
+            const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
+            icp_terms(vc, L.pose, J.cam, J.vref, J.nref, J.dist, v);
+            // K/icp.cl:162-190 reduce(): step d = 64 through LDS, steps 32..1 in the group's wave 0
+            if (wave == 1) {
+#pragma unroll
+                for (int k = 0; k < kTerms; ++k) L.xfer[sub][lane * kTerms + k] = v[k];
+            }
+            __syncthreads();
+            if (wave == 0) {
+#pragma unroll
+                for (int k = 0; k < kTerms; ++k) v[k] = wave_tree64(v[k] + L.xfer[sub][lane * kTerms + k]);
+                if (lane == 0 && g < ngroups) {
+#pragma unroll
+                    for (int k = 0; k < kTerms; ++k) part[(size_t)g * kTerms + k] = v[k];
+                }
+            }
+            __syncthreads();
         }
-        if ((uint32_t)t >= (uint32_t)level_tiles(J.cam, (int)(e / kEpochIters))) continue;
-        const unsigned long long old = atomicAdd(&sy->ticket, 1ull);
-        const uint32_t e2 = (uint32_t)(old >> 32), t2 = (uint32_t)old;
-        // a claim that lands in an open tile of whatever epoch the ticket holds now is kept
-        if (e2 != kFinished && t2 < (uint32_t)level_tiles(J.cam, (int)(e2 / kEpochIters))) return Claim{s, e2, t2};
+    } else {
+        // compensated: the thread's pixels in order, their gathers in flight together
+        float acc[kTerms], cmp[kTerms];
+#pragma unroll
+        for (int k = 0; k < kTerms; ++k) acc[k] = cmp[k] = 0.f;
+        const int p0 = (int)tile * kTilePx + tid;
+        float cf[kPxPerThread][7];
+        bool ok[kPxPerThread];
+#pragma unroll
+        for (int i = 0; i < kPxPerThread; ++i) {
+            const int p = p0 + i * kBlock;
+            const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
+            ok[i] = icp_assoc(vc, L.pose, J.cam, J.vref, J.nref, J.dist, cf[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < kPxPerThread; ++i) {
+            if (!ok[i]) continue;
+            int s = 0;
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int b = a; b < 7; ++b) {
+                    kahan_add(acc[s], cmp[s], cf[i][a] * cf[i][b]);
+                    ++s;
+                }
+            kahan_add(acc[27], cmp[27], cf[i][6] * cf[i][6]);
+            acc[28] += 1.0f;
+        }
+        const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+        for (int k = 0; k < kTerms; ++k) {
+            const double v = wave_tree64d((double)acc[k] - (double)cmp[k]);
+            if (lane == 0) L.wsum[wv][k] = v;
+        }
+        __syncthreads();
+        double* part = reinterpret_cast<double*>(J.partials);
+        if (tid < kTerms)
+            part[(size_t)tile * kTerms + tid] = ((L.wsum[0][tid] + L.wsum[1][tid]) + L.wsum[2][tid]) + L.wsum[3][tid];
+        __syncthreads();  // wsum is reused by the next tile
     }
-    return Claim{nfin == n ? -2 : -1, 0u, 0u};
 }
 
-// LDS of the kernel (one block: 256 threads)
-struct IcpLds {
-    IcpJob single;                // the job of a one-session launch (passed by value)
+// The tiles a session works on now: those of its epoch's level, none when its frame
+// is over or (planned launches) when its epoch is not `planned`.
+__device__ __forceinline__ uint32_t open_tiles(const IcpJob& J, uint32_t e, uint32_t planned) {
+    if (e == kFinished || (planned != kAnyEpoch && e != planned)) return 0u;
+    return (uint32_t)level_tiles(J.cam, (int)(e / kEpochIters));
+}
+
+// Every open tile of n sessions (jobs, or `single` when jobs is null).  The sessions'
+// open tiles are numbered one after another (the prefix of their counts in dynamic
+// LDS: n + 1 words) and block b takes the b-th of gridDim.x equal runs of them.
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+    icp_tiles_kernel(const IcpJob* __restrict__ jobs, int n, IcpJob single, uint32_t planned) {
+    __shared__ IcpLds<MODE> L;
+    extern __shared__ uint32_t pre[];
+    const int tid = threadIdx.x;
+    if (!jobs) {
+        if (tid == 0) L.single = single;
+        __syncthreads();
+    }
+    const IcpJob* table = jobs ? jobs : &L.single;
+    // exclusive prefix of the open tiles, 256 sessions at a time
+    __shared__ uint32_t carry, wtot[kBlock / 64];
+    if (tid == 0) carry = 0u;
+    __syncthreads();
+    for (int c0 = 0; c0 < n; c0 += kBlock) {
+        const int s = c0 + tid;
+        uint32_t v = 0u;
+        if (s < n) v = open_tiles(table[s], sync_of(table[s].st)->epoch, planned);
+        // inclusive scan in the wave, then over the waves
+        const int lane = tid & 63, wv = tid >> 6;
+        uint32_t x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wtot[wv] = x;
+        __syncthreads();
+        uint32_t before = carry;
+        for (int w = 0; w < wv; ++w) before += wtot[w];
+        if (s < n) pre[s] = before + x - v;
+        __syncthreads();
+        if (tid == kBlock - 1) carry = before + x;
+        __syncthreads();
+    }
+    if (tid == 0) pre[n] = carry;
+    __syncthreads();
+    const uint32_t total = pre[n];
+    const uint32_t f0 = (uint32_t)((unsigned long long)blockIdx.x * total / gridDim.x);
+    const uint32_t f1 = (uint32_t)((unsigned long long)(blockIdx.x + 1) * total / gridDim.x);
+    int cur = -1;
+    for (uint32_t f = f0; f < f1; ++f) {
+        // the session of tile f: the last with pre[s] <= f
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= f)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        if (lo != cur) {
+            // the job, word by word; the estimate of this epoch and the previous camera pose
+            constexpr int kJobWords = (int)(sizeof(IcpJob) / 4);
+            const uint32_t* js = reinterpret_cast<const uint32_t*>(&table[lo]);
+            uint32_t* jd = reinterpret_cast<uint32_t*>(&L.job);
+            for (int w = tid; w < kJobWords; w += kBlock) jd[w] = js[w];
+            const IcpJob& J = table[lo];
+            float* dp = reinterpret_cast<float*>(&L.pose);
+            if (tid < 12) {
+                dp[tid] = reinterpret_cast<const float*>(J.st)[tid];
+            } else if (tid < 21) {
+                L.pose.Ri[tid - 12] = J.pose->R_inv[tid - 12];
+            } else if (tid < 24) {
+                L.pose.tp[tid - 21] = J.pose->t[tid - 21];
+            } else if (tid == 24) {
+                L.level = (int)(sync_of(J.st)->epoch / kEpochIters);
+            }
+            cur = lo;
+            __syncthreads();
+        }
+        process_tile<MODE>(L, f - pre[lo]);
+    }
+}
+
+// ---- the finish launch -------------------------------------------------------------
+
+struct FinLds {
     IcpJob job;
-    TilePose pose;
-    Claim claim;
-    int last, last2;
-    float xfer[2][64 * kTerms];   // reference order: the tree's d = 64 step of two groups
-    double wsum[4][kTerms];       // compensated: wave sums
-    double seg[kTerms * 8];       // compensated: segment sums
+    uint32_t epoch;
+    double seg[kTerms * 8];                // compensated: segment sums
     double sums[kTerms];
     float sumsf[kTerms];
+    float chunk[kMaxChunks * kTerms];      // reference: compute_sums results
     IcpState S;
 };
 
+// One block per session: the epoch's partials reduced in a fixed order, the update,
+// the next epoch.
 template <int MODE>
-__global__ void __launch_bounds__(kBlock)
-    icp_persistent_kernel(const IcpJob* __restrict__ jobs, int n, IcpJob single, long long max_ticks) {
-    __shared__ IcpLds L;
+__global__ void __launch_bounds__(kFinBlock)
+    icp_finish_kernel(const IcpJob* __restrict__ jobs, IcpJob single, uint32_t planned) {
+    __shared__ FinLds F;
     const int tid = threadIdx.x;
-    if (tid == 0 && !jobs) L.single = single;
+    if (tid == 0) {
+        F.job = jobs ? jobs[blockIdx.x] : single;
+        F.epoch = sync_of(F.job.st)->epoch;
+    }
     __syncthreads();
-    const IcpJob* table = jobs ? jobs : &L.single;
-    const long long t_start = wall_clock64();
-    int polls = 0;
-    for (;;) {
-        if (tid == 0) {
-            const int start = (int)((blockIdx.x + (unsigned)polls) % (unsigned)n);
-            L.claim = claim_tile(table, n, start);
-            if (L.claim.s >= 0) {
-                L.job = table[L.claim.s];
-                // the estimate of this epoch (published before the ticket moved)
-                const uint32_t* sp = reinterpret_cast<const uint32_t*>(L.job.st);
-                uint32_t* dp = reinterpret_cast<uint32_t*>(&L.pose);
-                for (int k = 0; k < 12; ++k)
-                    dp[k] = __hip_atomic_load(sp + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const NuiCLRigidTransform* prev = L.job.pose;
-                for (int k = 0; k < 9; ++k) L.pose.Ri[k] = prev->R_inv[k];
-                for (int k = 0; k < 3; ++k) L.pose.tp[k] = prev->t[k];
-            } else if (L.claim.s == -1 && wall_clock64() - t_start > max_ticks) {
-                L.claim.s = -2;  // polled past the host's bound: leave (unfinished sessions fail)
+    const IcpJob& J = F.job;
+    const uint32_t e = F.epoch;
+    if (e == kFinished || (planned != kAnyEpoch && e != planned)) return;
+    const int level = (int)(e / kEpochIters), iter = (int)(e % kEpochIters);
+    const int npx = level_pixels(J.cam, level);
+    if constexpr (MODE == kRefOrder) {
+        // compute_sums' tree over each chunk of 128 group sums (one wave per chunk, its
+        // 29 terms together), then the host's sequential float sum over the chunk
+        // results (OCL/NuiKinfuOpenCLDepthTracker.cpp:394-402, 428-440)
+        const float* part = reinterpret_cast<const float*>(J.partials);
+        const int ngroups = (npx + 127) >> 7, nchunks = (ngroups + 127) >> 7;
+        const int lane = tid & 63, wv = tid >> 6;
+        for (int q = wv; q < nchunks; q += kFinBlock / 64) {
+            const int g0 = (q << 7) + lane, g1 = g0 + 64;
+            float x[kTerms];
+#pragma unroll
+            for (int k = 0; k < kTerms; ++k) {
+                const float a = g0 < ngroups ? part[(size_t)g0 * kTerms + k] : 0.f;
+                const float b = g1 < ngroups ? part[(size_t)g1 * kTerms + k] : 0.f;
+                x[k] = a + b;
+            }
+#pragma unroll
+            for (int k = 0; k < kTerms; ++k) {
+                const float sum = wave_tree64(x[k]);
+                if (lane == 0) F.chunk[q * kTerms + k] = sum;
             }
         }
         __syncthreads();
-        const Claim c = L.claim;
-        if (c.s == -2) return;
-        if (c.s == -1) {
-            // nothing open: sleep, then poll again
-            ++polls;
-            __builtin_amdgcn_s_sleep(8);
-            __syncthreads();  // L.claim is rewritten by thread 0 only after every thread read it
-            continue;
+        if (tid < kTerms) {
+            float sum = 0.f;
+            for (int q = 0; q < nchunks; ++q) sum += F.chunk[q * kTerms + tid];
+            F.sumsf[tid] = sum;
         }
-        const IcpJob& J = L.job;
-        const int level = (int)(c.epoch / kEpochIters), iter = (int)(c.epoch % kEpochIters);
-        const int npx = level_pixels(J.cam, level);
-        const float* __restrict__ vcur = J.vmap[level];
-        IcpSync* sy = sync_of(J.st);
-
-        if (MODE == kRefOrder) {
-            // eight 128-pixel groups, two at a time (one per 128 threads)
-            const int sub = tid >> 7, l = tid & 127, lane = l & 63, wave = l >> 6;
-            const int ngroups = (npx + 127) >> 7;
-            float* part = reinterpret_cast<float*>(J.partials);
-            for (int r = 0; r < kTileGroups / 2; ++r) {
-                const int g = (int)c.tile * kTileGroups + 2 * r + sub;
-                const int p = g * 128 + l;
-                float v[kTerms];
-                const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
-                icp_terms(vc, L.pose, J.cam, J.vref, J.nref, J.dist, v);
-                // K/icp.cl:162-190 reduce(): step d = 64 through LDS, steps 32..1 in the group's wave 0
-                if (wave == 1) {
+        __syncthreads();
+        finish_epoch<MODE>(J, F.S, nullptr, F.sumsf, level, iter);
+    } else {
+        const double* part = reinterpret_cast<const double*>(J.partials);
+        const int ntiles = level_tiles(J.cam, level);
+        if (tid < kTerms * 8) {
+            // term k, segment j: tiles j, j + 8, ... in order (loads 16 at a time)
+            const int k = tid >> 3, j = tid & 7;
+            const double* pk = part + k;
+            double s = 0.0;
+            int q = j;
+            for (; q + 8 * 15 < ntiles; q += 8 * 16) {
+                double v[16];
 #pragma unroll
-                    for (int k = 0; k < kTerms; ++k) L.xfer[sub][lane * kTerms + k] = v[k];
-                }
-                __syncthreads();
-                if (wave == 0) {
+                for (int u = 0; u < 16; ++u) v[u] = pk[(size_t)(q + 8 * u) * kTerms];
 #pragma unroll
-                    for (int k = 0; k < kTerms; ++k) v[k] = wave_tree64(v[k] + L.xfer[sub][lane * kTerms + k]);
-                    if (lane == 0 && g < ngroups) {
-#pragma unroll
-                        for (int k = 0; k < kTerms; ++k) part[(size_t)g * kTerms + k] = v[k];
-                    }
-                }
-                __syncthreads();
+                for (int u = 0; u < 16; ++u) s += v[u];
             }
-            // the last tile of each 128-group chunk runs compute_sums' tree over the chunk
-            const int chunk = ((int)c.tile * kTileGroups) >> 7;
-            if (tid == 0) {
-                fence_release_agent();  // this tile's group sums
-                const int groups_in_chunk = min(128, ngroups - (chunk << 7));
-                const uint32_t members = (uint32_t)((groups_in_chunk + kTileGroups - 1) / kTileGroups);
-                L.last = atomicAdd(&sy->arrive[chunk], 1u) == members - 1;
-            }
-            __syncthreads();
-            if (L.last) {
-                fence_acquire_agent();  // the chunk's group sums
-                float* tree = part + (size_t)ngroups * kTerms;
-                const int wid = tid >> 6;
-                for (int k = wid; k < kTerms; k += kBlock / 64) {
-                    const int g0 = (chunk << 7) + (tid & 63), g1 = g0 + 64;
-                    const float x = g0 < ngroups ? part[(size_t)g0 * kTerms + k] : 0.f;
-                    const float y = g1 < ngroups ? part[(size_t)g1 * kTerms + k] : 0.f;
-                    const float sum = wave_tree64(x + y);
-                    if ((tid & 63) == 0) tree[chunk * kTerms + k] = sum;
-                }
-                __syncthreads();  // the chunk result is written
-                const int nchunks = (ngroups + 127) >> 7;
-                if (tid == 0) {
-                    // re-arm: every tile of the chunk has arrived
-                    __hip_atomic_store(&sy->arrive[chunk], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    fence_release_agent();
-                    L.last2 = atomicAdd(&sy->chunks_done, 1u) == (uint32_t)(nchunks - 1);
-                }
-                __syncthreads();
-                if (L.last2) {
-                    fence_acquire_agent();  // every chunk result
-                    // the host's sequential float sum over the compute_sums results
-                    // (OCL/NuiKinfuOpenCLDepthTracker.cpp:394-402, 428-440)
-                    if (tid < kTerms) {
-                        float sum = 0.f;
-                        for (int q = 0; q < nchunks; ++q) sum += tree[q * kTerms + tid];
-                        L.sumsf[tid] = sum;
-                    }
-                    __syncthreads();
-                    finish_epoch<MODE>(J, L.S, nullptr, L.sumsf, level, iter);
-                }
-            }
-        } else {
-            // compensated: four pixels per thread, their gathers in flight together
-            float acc[kTerms], cmp[kTerms];
-#pragma unroll
-            for (int k = 0; k < kTerms; ++k) acc[k] = cmp[k] = 0.f;
-            const int p0 = (int)c.tile * kTilePx + tid;
-            // pixels i = 0..3 in order, two at a time: their gathers in flight together
-#pragma unroll 1
-            for (int i0 = 0; i0 < 4; i0 += 2) {
-                float cf[2][7];
-                bool ok[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    const int p = p0 + (i0 + i) * kBlock;
-                    const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
-                    ok[i] = icp_assoc(vc, L.pose, J.cam, J.vref, J.nref, J.dist, cf[i]);
-                }
-#pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    if (!ok[i]) continue;
-                    int s = 0;
-#pragma unroll
-                    for (int a = 0; a < 6; ++a)
-#pragma unroll
-                        for (int b = a; b < 7; ++b) {
-                            kahan_add(acc[s], cmp[s], cf[i][a] * cf[i][b]);
-                            ++s;
-                        }
-                    kahan_add(acc[27], cmp[27], cf[i][6] * cf[i][6]);
-                    acc[28] += 1.0f;
-                }
-            }
-            const int lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-            for (int k = 0; k < kTerms; ++k) {
-                const double v = wave_tree64d((double)acc[k] - (double)cmp[k]);
-                if (lane == 0) L.wsum[wv][k] = v;
-            }
-            __syncthreads();
-            double* part = reinterpret_cast<double*>(J.partials);
-            if (tid < kTerms)
-                part[(size_t)c.tile * kTerms + tid] = ((L.wsum[0][tid] + L.wsum[1][tid]) + L.wsum[2][tid]) + L.wsum[3][tid];
-            __syncthreads();
-            const int ntiles = level_tiles(J.cam, level);
-            if (tid == 0) {
-                fence_release_agent();  // this tile's partial
-                L.last = atomicAdd(&sy->done, 1u) == (uint32_t)(ntiles - 1);
-            }
-            __syncthreads();
-            if (L.last) {
-                fence_acquire_agent();  // every tile's partial
-                if (tid < kTerms * 8) {
-                    // term k, segment j: tiles j, j + 8, ... in order
-                    const int k = tid >> 3, j = tid & 7;
-                    double s = 0.0;
-                    for (int q = j; q < ntiles; q += 8) s += part[(size_t)q * kTerms + k];
-                    L.seg[k * 8 + j] = s;
-                }
-                __syncthreads();
-                if (tid < kTerms) {
-                    double s = 0.0;
-                    for (int j = 0; j < 8; ++j) s += L.seg[tid * 8 + j];
-                    L.sums[tid] = s;
-                }
-                __syncthreads();
-                finish_epoch<MODE>(J, L.S, L.sums, nullptr, level, iter);
-            }
+            for (; q < ntiles; q += 8) s += pk[(size_t)q * kTerms];
+            F.seg[k * 8 + j] = s;
         }
-        polls = 0;
-        __syncthreads();  // LDS is reused by the next claim
+        __syncthreads();
+        if (tid < kTerms) {
+            double s = 0.0;
+            for (int j = 0; j < 8; ++j) s += F.seg[tid * 8 + j];
+            F.sums[tid] = s;
+        }
+        __syncthreads();
+        finish_epoch<MODE>(J, F.S, F.sums, nullptr, level, iter);
     }
 }
 
@@ -629,14 +650,6 @@ __device__ uint32_t first_epoch(const IcpJob& J, const IcpState* st) {
     return kFinished;
 }
 
-__device__ __forceinline__ void clear_sync(IcpSync* sy, int tid, int nthreads) {
-    for (int i = tid; i < kMaxChunks; i += nthreads) sy->arrive[i] = 0u;
-    if (tid == 0) {
-        sy->done = 0u;
-        sy->chunks_done = 0u;
-    }
-}
-
 // Begin a frame: estimate <- the camera pose, flags cleared, the first epoch
 // opened.  Compensated: the FP64 pose carries over from the previous frame unless
 // the camera pose was changed from outside (reset, hint, volume shift), in which
@@ -645,11 +658,9 @@ __global__ void icp_begin_kernel(const IcpJob* __restrict__ jobs, IcpJob single)
     __shared__ IcpJob J;
     if (threadIdx.x == 0) J = jobs ? jobs[blockIdx.x] : single;
     __syncthreads();
+    if (threadIdx.x != 0) return;
     IcpState* st = J.st;
     IcpSync* sy = sync_of(st);
-    clear_sync(sy, threadIdx.x, blockDim.x);
-    __syncthreads();
-    if (threadIdx.x != 0) return;
     const NuiCLRigidTransform* pose = J.pose;
     if (!J.single) {
         if (J.mode == kCompensated) {
@@ -672,7 +683,7 @@ __global__ void icp_begin_kernel(const IcpJob* __restrict__ jobs, IcpJob single)
         st->ok = 0;
         st->timeout = 0;
     }
-    sy->ticket = (unsigned long long)first_epoch(J, st) << 32;
+    sy->epoch = first_epoch(J, st);
 }
 
 // src/Foundation/NuiMatrixUtilities.cpp:7-52 rodrigues2(R).norm() for a rotation
@@ -688,11 +699,11 @@ __device__ float rodrigues_norm(const float Rm[9]) {
 
 // Finish the frame (one thread per session): tracking outcome, pose, the
 // integration gate (VO/NuiKinfuTrackingEngine.cpp:113-127), the history row.
-// A session whose epochs did not finish (the persistent kernel gave up) fails
-// with the timeout flag set.
+// A session whose epochs did not all run (fewer launched than its schedule needs)
+// fails with the incomplete flag set.
 __device__ void end_frame(IcpState* st, NuiCLRigidTransform* pose, float gate_threshold, float* history,
                           int capacity) {
-    if ((uint32_t)(sync_of(st)->ticket >> 32) != kFinished) {
+    if (sync_of(st)->epoch != kFinished) {
         st->failed = 1;
         st->timeout = 1;
     }
@@ -756,14 +767,13 @@ __global__ void icp_record_kernel(IcpState* st, NuiCLRigidTransform* pose, float
     st->failed = 0;
     st->iterations = 0;
     st->timeout = 0;
-    sync_of(st)->ticket = (unsigned long long)kFinished << 32;
+    sync_of(st)->epoch = kFinished;
     end_frame(st, pose, -1.0f, history, capacity);
 }
 
 __global__ void icp_state_init_kernel(IcpState* st, const NuiCLRigidTransform* pose) {
-    clear_sync(sync_of(st), threadIdx.x, blockDim.x);
     if (threadIdx.x != 0) return;
-    sync_of(st)->ticket = (unsigned long long)kFinished << 32;
+    sync_of(st)->epoch = kFinished;
     for (int k = 0; k < 9; ++k) st->R[k] = st->Rg[k] = st->Rlast[k] = pose->R[k];
     for (int k = 0; k < 3; ++k) st->t[k] = st->tg[k] = st->tlast[k] = pose->t[k];
     for (int k = 0; k < 9; ++k) st->Rd[k] = (double)pose->R[k];
@@ -831,19 +841,6 @@ int static_scratch(void** out) {
 
 // ---- host side ---------------------------------------------------------------------
 
-long long poll_ticks() {
-    // the poll bound of the persistent kernel: 2 s of the device's constant-rate clock
-    static std::once_flag once;
-    static long long ticks = 200000000LL;
-    std::call_once(once, [] {
-        int dev = 0, khz = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
-            ticks = (long long)khz * 2000LL;
-    });
-    return ticks;
-}
-
 int device_cus() {
     static std::once_flag once;
     static int cus = 256;
@@ -896,28 +893,118 @@ int make_job(const nuif_icp_session& in, IcpJob* j) {
     return NUIF_OK;
 }
 
-// blocks of the persistent launch: enough for the largest level's tiles, at most 8 per CU
-unsigned persistent_grid(const IcpJob* jobs, int n) {
-    size_t most = 1;
-    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) {
-        size_t tiles = 0;
-        for (int s = 0; s < n; ++s)
-            if (jobs[s].iters[l] > 0 || (jobs[s].single && (int)(jobs[s].first_epoch / kEpochIters) == l))
-                tiles += (size_t)level_tiles(jobs[s].cam, l);
-        most = std::max(most, tiles);
-    }
-    return (unsigned)std::min<size_t>(most, (size_t)device_cus() * 8);
+// tile blocks the device holds at once (the cap of a batch step's grid)
+template <int MODE> int resident_tile_blocks() {
+    static std::once_flag once;
+    static int blocks = 0;
+    std::call_once(once, [] {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, icp_tiles_kernel<MODE>, kBlock, 0) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 1;
+        blocks = per_cu * device_cus();
+    });
+    return blocks;
 }
 
-// one session: begin, persistent, end with the job as a kernel argument
+// Epochs a session runs at most (every iteration of every level).
+int max_epochs(const IcpJob& J) {
+    int n = 0;
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) n += J.iters[l];
+    return n;
+}
+
+// The most tiles a session can have open at step k: a level is reachable at step k if
+// the coarser levels with iterations can take k steps between them, at least one and
+// at most all of their iterations each.
+uint32_t step_tiles(const IcpJob& J, int k) {
+    uint32_t most = 0;
+    int least = 0, longest = 0;  // steps the coarser levels take at least / at most
+    for (int l = J.levels - 1; l >= 0; --l) {
+        if (J.iters[l] <= 0) continue;
+        if (k >= least && k < longest + J.iters[l]) most = std::max(most, (uint32_t)level_tiles(J.cam, l));
+        least += 1;
+        longest += J.iters[l];
+    }
+    return most;
+}
+
+size_t icp_job_bytes() { return sizeof(IcpJob); }
+
+// The jobs of n sessions, those of the reference order first (one series of steps per
+// order); *nref receives their number.
+int make_icp_jobs(const nuif_icp_session* sessions, int n, void* host_out, int* nref) {
+    IcpJob* jobs = static_cast<IcpJob*>(host_out);
+    for (int i = 0; i < n; ++i) {
+        const int rc = make_job(sessions[i], &jobs[i]);
+        if (rc) return rc;
+        for (int j = 0; j < i; ++j)
+            NUIF_CHECK_ARG(jobs[j].st != jobs[i].st, "ICP batch: a state appears twice");
+    }
+    std::stable_partition(jobs, jobs + n, [](const IcpJob& j) { return j.mode == kRefOrder; });
+    *nref = (int)std::count_if(jobs, jobs + n, [](const IcpJob& j) { return j.mode == kRefOrder; });
+    return NUIF_OK;
+}
+
+// The lockstep steps of n sessions of one order: per step, the tile launch (grid: the
+// most tiles the sessions can have open, capped at what the device holds at once)
+// and the finish launch.
+template <int MODE>
+void launch_steps(const IcpJob* dj, const IcpJob* hj, int n, hipStream_t st) {
+    int steps = 0;
+    for (int s = 0; s < n; ++s) steps = std::max(steps, max_epochs(hj[s]));
+    const size_t cap = (size_t)resident_tile_blocks<MODE>();
+    const size_t lds = (size_t)(n + 1) * sizeof(uint32_t);
+    for (int k = 0; k < steps; ++k) {
+        size_t tiles = 0;
+        for (int s = 0; s < n; ++s) tiles += step_tiles(hj[s], k);
+        const unsigned grid = (unsigned)std::max<size_t>(1, std::min(tiles, cap));
+        icp_tiles_kernel<MODE><<<grid, kBlock, lds, st>>>(dj, n, IcpJob{}, kAnyEpoch);
+        icp_finish_kernel<MODE><<<n, kFinBlock, 0, st>>>(dj, IcpJob{}, kAnyEpoch);
+    }
+}
+
+// begin (all sessions), the steps of each order over its part of the table, end (all
+// sessions); dtable: device copy of hjobs
+int launch_icp_jobs(const void* dtable, const void* hjobs, int n, int nref, hipStream_t st) {
+    if (n <= 0) return NUIF_OK;
+    NUIF_CHECK_ARG(n <= kMaxBatch, "ICP batch: %d sessions exceed the limit of %d", n, kMaxBatch);
+    const IcpJob* dj = static_cast<const IcpJob*>(dtable);
+    const IcpJob* hj = static_cast<const IcpJob*>(hjobs);
+    icp_begin_kernel<<<n, 64, 0, st>>>(dj, IcpJob{});
+    NUIF_LAUNCHED("icp_begin_kernel");
+    if (nref > 0) launch_steps<kRefOrder>(dj, hj, nref, st);
+    if (nref < n) launch_steps<kCompensated>(dj + nref, hj + nref, n - nref, st);
+    NUIF_LAUNCHED("icp_tiles_kernel / icp_finish_kernel");
+    icp_end_kernel<<<n, 64, 0, st>>>(dj, IcpJob{});
+    NUIF_LAUNCHED("icp_end_kernel");
+    return NUIF_OK;
+}
+
+// one session: begin, the pair of launches of each planned epoch in the frame's order,
+// end (the job as a kernel argument)
 int run_single(const IcpJob& J, hipStream_t st, bool end) {
     icp_begin_kernel<<<1, 64, 0, st>>>(nullptr, J);
     NUIF_LAUNCHED("icp_begin_kernel");
-    if (J.mode == kRefOrder)
-        icp_persistent_kernel<kRefOrder><<<persistent_grid(&J, 1), kBlock, 0, st>>>(nullptr, 1, J, poll_ticks());
-    else
-        icp_persistent_kernel<kCompensated><<<persistent_grid(&J, 1), kBlock, 0, st>>>(nullptr, 1, J, poll_ticks());
-    NUIF_LAUNCHED("icp_persistent_kernel");
+    auto launch = [&](int level, int iter) {
+        const uint32_t e = (uint32_t)(level * kEpochIters + iter);
+        const unsigned grid = (unsigned)level_tiles(J.cam, level);
+        const size_t lds = 2 * sizeof(uint32_t);
+        if (J.mode == kRefOrder) {
+            icp_tiles_kernel<kRefOrder><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e);
+            icp_finish_kernel<kRefOrder><<<1, kFinBlock, 0, st>>>(nullptr, J, e);
+        } else {
+            icp_tiles_kernel<kCompensated><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e);
+            icp_finish_kernel<kCompensated><<<1, kFinBlock, 0, st>>>(nullptr, J, e);
+        }
+    };
+    if (J.single) {
+        launch((int)(J.first_epoch / kEpochIters), (int)(J.first_epoch % kEpochIters));
+    } else {
+        for (int level = J.levels - 1; level >= 0; --level)
+            for (int it = 0; it < J.iters[level]; ++it) launch(level, it);
+    }
+    NUIF_LAUNCHED("icp_tiles_kernel / icp_finish_kernel");
     if (end) {
         icp_end_kernel<<<1, 64, 0, st>>>(nullptr, J);
         NUIF_LAUNCHED("icp_end_kernel");
@@ -949,9 +1036,9 @@ size_t nuif_icp_partials_bytes(int width, int height) {
     const size_t npx = (size_t)width * height;
     const size_t compensated = (npx + kTilePx - 1) / kTilePx * kTerms * sizeof(double);
     const size_t groups = (npx + 127) / 128;
-    // reference order: whole tiles of groups, then the chunk results
+    // reference order: the group sums of whole tiles
     const size_t tgroups = (groups + kTileGroups - 1) / kTileGroups * kTileGroups;
-    const size_t reference = (tgroups + (groups + 127) / 128) * kTerms * sizeof(float);
+    const size_t reference = tgroups * kTerms * sizeof(float);
     return std::max(compensated, reference);
 }
 
@@ -1127,34 +1214,12 @@ int nuif_icp_estimate_batch(const nuif_icp_session* sessions, int n, void* table
     NUIF_CHECK_ARG(n >= 0 && (n == 0 || (sessions && table)), "nuif_icp_estimate_batch: bad arguments");
     if (n == 0) return NUIF_OK;
     std::vector<IcpJob> jobs((size_t)n);
-    for (int i = 0; i < n; ++i) {
-        const int rc = make_job(sessions[i], &jobs[i]);
-        if (rc) return rc;
-        for (int j = 0; j < i; ++j)
-            NUIF_CHECK_ARG(jobs[j].st != jobs[i].st, "nuif_icp_estimate_batch: a state appears twice");
-    }
-    // sessions of the reference order first, then the compensated ones: one persistent
-    // launch per order over its part of the table
-    std::stable_partition(jobs.begin(), jobs.end(), [](const IcpJob& j) { return j.mode == kRefOrder; });
-    const int nref = (int)std::count_if(jobs.begin(), jobs.end(), [](const IcpJob& j) { return j.mode == kRefOrder; });
+    int nref = 0;
+    int rc = make_icp_jobs(sessions, n, jobs.data(), &nref);
+    if (rc) return rc;
     hipStream_t st = stream_of(s);
     NUIF_HIP(hipMemcpyAsync(table, jobs.data(), (size_t)n * sizeof(IcpJob), hipMemcpyHostToDevice, st));
-    const IcpJob* dj = (const IcpJob*)table;
-    icp_begin_kernel<<<n, 64, 0, st>>>(dj, IcpJob{});
-    NUIF_LAUNCHED("icp_begin_kernel");
-    if (nref > 0) {
-        icp_persistent_kernel<kRefOrder>
-            <<<persistent_grid(jobs.data(), nref), kBlock, 0, st>>>(dj, nref, IcpJob{}, poll_ticks());
-        NUIF_LAUNCHED("icp_persistent_kernel");
-    }
-    if (nref < n) {
-        icp_persistent_kernel<kCompensated><<<persistent_grid(jobs.data() + nref, n - nref), kBlock, 0, st>>>(
-            dj + nref, n - nref, IcpJob{}, poll_ticks());
-        NUIF_LAUNCHED("icp_persistent_kernel");
-    }
-    icp_end_kernel<<<n, 64, 0, st>>>(dj, IcpJob{});
-    NUIF_LAUNCHED("icp_end_kernel");
-    return NUIF_OK;
+    return launch_icp_jobs(table, jobs.data(), n, nref, st);
 }
 
 }  // extern "C"

@@ -8,14 +8,46 @@
 
 namespace nuif {
 
+// Lane l (l < d) receives the 32-bit value of lane l + d, without the LDS crossbar:
+// d = 32 and 16 by the lane swaps between the halves of the wave and between pairs of
+// 16-lane rows (the second result of v_permlane32/16_swap with both operands x holds, in
+// its low half / even rows, the high half / odd rows of x), d < 16 by a DPP row shift
+// to the left (lanes past the row's end keep their own value).  Every lane of the wave
+// must be active.
+template <int D>
+__device__ __forceinline__ uint32_t lane_from_above(uint32_t x) {
+    if constexpr (D == 32)
+        return __builtin_amdgcn_permlane32_swap(x, x, false, false)[1];
+    else if constexpr (D == 16)
+        return __builtin_amdgcn_permlane16_swap(x, x, false, false)[1];
+    else
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x100 + D, 0xf, 0xf, false);
+}
+template <int D>
+__device__ __forceinline__ float lane_from_above(float v) {
+    return __uint_as_float(lane_from_above<D>(__float_as_uint(v)));
+}
+template <int D>
+__device__ __forceinline__ double lane_from_above(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = lane_from_above<D>((uint32_t)u), hi = lane_from_above<D>((uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // The reference tree's steps d = 32..1 (item i takes item i + d) inside one
-// wave; lane 0 ends with the sum.  A lane whose source is past the wave keeps
-// its own value, which no lane below it reads afterwards.
-__device__ __forceinline__ float wave_tree64(float v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+// wave; lane 0 ends with the sum (the other lanes' results are not defined).
+// Every lane of the wave must be active.
+template <class T>
+__device__ __forceinline__ T wave_tree64_t(T v) {
+    v += lane_from_above<32>(v);
+    v += lane_from_above<16>(v);
+    v += lane_from_above<8>(v);
+    v += lane_from_above<4>(v);
+    v += lane_from_above<2>(v);
+    v += lane_from_above<1>(v);
     return v;
 }
+__device__ __forceinline__ float wave_tree64(float v) { return wave_tree64_t(v); }
 
 // Cholesky A = L L^T in place (lower), as Eigen's LLT; returns false if not SPD.
 __device__ inline bool llt6(double A[36], double* det) {

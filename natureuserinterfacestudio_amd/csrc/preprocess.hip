@@ -7,14 +7,11 @@
 // are bandwidth-trivial (a few MB per frame); the fused preprocessing kernel
 // exists to cut launches and to read the raw depth once.
 
+#include "batch_jobs.h"
 #include "det_math.h"
 #include "nuif_internal.h"
 
 namespace nuif {
-
-struct Gauss {
-    float g[15];  // 2*radius+1 <= 15
-};
 
 constexpr int kTileW = 32, kTileH = 8, kMaxR = 7;
 
@@ -90,34 +87,42 @@ __global__ void __launch_bounds__(256) bilateral_kernel(const float* __restrict_
     if (x < W && y < H) out[(size_t)y * W + x] = bilateral_px(tile, tw, x, y, ox, oy, W, H, g, r, sigma_depth, thr);
 }
 
-// Fused passing filter + bilateral + level-0 vertices + ray-length depth: reads the raw depth once.
-__global__ void __launch_bounds__(256)
-    preprocess_l0_kernel(const uint16_t* __restrict__ raw, float* __restrict__ depth_out,
-                         float* __restrict__ depth_f, float* __restrict__ vmap, int W, int H, NuiCLCameraParams cam,
-                         Gauss g, int r, float sigma_depth, float thr, int scale) {
+// Fused passing filter + bilateral + level-0 vertices + ray-length depth: reads the raw
+// depth once.  Body of one 32 x 8 tile (bx, by) of job J.
+__device__ __forceinline__ void preprocess_l0_body(const PreJob& J, int bx, int by) {
     __shared__ float tile[(kTileW + 2 * kMaxR) * (kTileH + 2 * kMaxR)];
     const int tw = kTileW + 2 * kMaxR, th = kTileH + 2 * kMaxR;
-    const int ox = blockIdx.x * kTileW, oy = blockIdx.y * kTileH;
-    load_tile<true>(tile, tw, th, raw, W, H, ox, oy, cam.sensorDepthWorldMin, cam.sensorDepthWorldMax);
+    const int W = J.W, H = J.H;
+    const NuiCLCameraParams cam = J.cam;
+    const int ox = bx * kTileW, oy = by * kTileH;
+    load_tile<true>(tile, tw, th, J.raw, W, H, ox, oy, cam.sensorDepthWorldMin, cam.sensorDepthWorldMax);
     __syncthreads();
     const int x = ox + threadIdx.x, y = oy + threadIdx.y;
     if (x >= W || y >= H) return;
     const size_t i = (size_t)y * W + x;
     float d = tile[(y - oy + kMaxR) * tw + (x - ox + kMaxR)];
-    if (scale && !isnan(d)) {
+    if (J.scale && !isnan(d)) {
         // K/tsdfVolume.cl:8-29 scaleDepthsKernel
         const float xl = ((float)x - cam.cx) * cam.fx_inv;
         const float yl = ((float)y - cam.cy) * cam.fy_inv;
         const float lambda = sqrtf(xl * xl + yl * yl + 1.f);
         d = d * lambda;
     }
-    depth_out[i] = d;
-    const float f = bilateral_px(tile, tw, x, y, ox, oy, W, H, g, r, sigma_depth, thr);
-    depth_f[i] = f;
+    J.depth_out[i] = d;
+    const float f = bilateral_px(tile, tw, x, y, ox, oy, W, H, J.g, J.r, J.sigma_depth, J.thr);
+    J.depth_f[i] = f;
     // K/transformMaps.cl:18-42 with div = 1
     f3 v = mk3(kNaN, kNaN, kNaN);
     if (!isnan(f)) v = mk3(f * (((float)x - cam.cx) * cam.fx_inv), f * (((float)y - cam.cy) * cam.fy_inv), f * 1.0f);
-    st3(vmap, i, v);
+    st3(J.vmap, i, v);
+}
+
+__global__ void __launch_bounds__(256) preprocess_l0_kernel(PreJob J) { preprocess_l0_body(J, blockIdx.x, blockIdx.y); }
+
+__global__ void __launch_bounds__(256) preprocess_l0_batch_kernel(const PreJob* __restrict__ jobs, int n) {
+    const PreJob& J = jobs[job_of(jobs, n, blockIdx.x)];
+    const unsigned b = blockIdx.x - J.block0;
+    preprocess_l0_body(J, (int)(b % J.gx), (int)(b / J.gx));
 }
 
 // K/transformMaps.cl:18-42 depth2vertex_kernel
@@ -136,11 +141,14 @@ __global__ void depth2vertex_kernel(const float* __restrict__ depth, float* __re
     st3(vmap, i, vertex_at(depth[i], x, y, cam, div));
 }
 
-// K/transformMaps.cl:44-88 half_sample_kernel (+ optional depth2vertex of the result)
-__global__ void half_sample_kernel(const float* __restrict__ src, float* __restrict__ dst, float* __restrict__ vmap,
-                                   int Wd, int Hd, int r, float thr, NuiCLCameraParams cam, int div) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+// K/transformMaps.cl:44-88 half_sample_kernel (+ optional depth2vertex of the result),
+// one 16 x 16 block (bx, by) of job J
+__device__ __forceinline__ void half_sample_body(const HalfJob& J, int bx, int by) {
+    const int Wd = J.Wd, Hd = J.Hd, r = J.r;
+    const int x = bx * 16 + threadIdx.x, y = by * 16 + threadIdx.y;
     if (x >= Wd || y >= Hd) return;
+    const float* __restrict__ src = J.src;
+    const float thr = J.thr;
     const int Ws = Wd << 1, Hs = Hd << 1;
     const int sx = x << 1, sy = y << 1;
     const float c = src[(size_t)sy * Ws + sx];
@@ -158,8 +166,16 @@ __global__ void half_sample_kernel(const float* __restrict__ src, float* __restr
         }
     const float o = cnt > 0 ? s / (float)cnt : kNaN;
     const size_t i = (size_t)y * Wd + x;
-    dst[i] = o;
-    if (vmap) st3(vmap, i, vertex_at(o, x, y, cam, div));
+    J.dst[i] = o;
+    if (J.vmap) st3(J.vmap, i, vertex_at(o, x, y, J.cam, J.div));
+}
+
+__global__ void half_sample_kernel(HalfJob J) { half_sample_body(J, blockIdx.x, blockIdx.y); }
+
+__global__ void half_sample_batch_kernel(const HalfJob* __restrict__ jobs, int n) {
+    const HalfJob& J = jobs[job_of(jobs, n, blockIdx.x)];
+    const unsigned b = blockIdx.x - J.block0;
+    half_sample_body(J, (int)(b % J.gx), (int)(b / J.gx));
 }
 
 // K/normalEstimation.cl:770-822 estimate_normals_simple_kernel
@@ -201,10 +217,26 @@ __global__ void scale_depths_kernel(float* depth, int W, int H, NuiCLCameraParam
     depth[i] = d * lambda;
 }
 
-static int make_gauss(const float* gaussian, int radius, Gauss* g) {
+int make_gauss(const float* gaussian, int radius, Gauss* g) {
     NUIF_CHECK_ARG(gaussian != nullptr, "gaussian table is NULL");
     NUIF_CHECK_ARG(radius >= 0 && radius <= kMaxR, "bilateral radius %d outside [0, %d]", radius, kMaxR);
     for (int i = 0; i < 15; ++i) g->g[i] = i < 2 * radius + 1 ? gaussian[i] : 0.f;
+    return NUIF_OK;
+}
+
+int launch_preprocess_batch(const PreJob* jobs, const PreJob* hjobs, int n, hipStream_t st) {
+    if (n <= 0) return NUIF_OK;
+    const unsigned blocks = hjobs[n - 1].block0 + hjobs[n - 1].gx * div_up(hjobs[n - 1].H, kTileH);
+    preprocess_l0_batch_kernel<<<blocks, dim3(kTileW, kTileH), 0, st>>>(jobs, n);
+    NUIF_LAUNCHED("preprocess_l0_batch_kernel");
+    return NUIF_OK;
+}
+
+int launch_half_batch(const HalfJob* jobs, const HalfJob* hjobs, int n, hipStream_t st) {
+    if (n <= 0) return NUIF_OK;
+    const unsigned blocks = hjobs[n - 1].block0 + hjobs[n - 1].gx * div_up(hjobs[n - 1].Hd, 16);
+    half_sample_batch_kernel<<<blocks, dim3(16, 16), 0, st>>>(jobs, n);
+    NUIF_LAUNCHED("half_sample_batch_kernel");
     return NUIF_OK;
 }
 
@@ -244,7 +276,8 @@ int nuif_half_sample(const float* src, float* dst, int Wd, int Hd, int radius, f
     NUIF_CHECK_ARG(src && dst && Wd > 0 && Hd > 0 && radius >= 0, "nuif_half_sample: bad arguments");
     NuiCLCameraParams cam{};
     dim3 block(16, 16), grid(div_up(Wd, 16), div_up(Hd, 16));
-    half_sample_kernel<<<grid, block, 0, stream_of(s)>>>(src, dst, nullptr, Wd, Hd, radius, depth_threshold, cam, 1);
+    const HalfJob hj{src, dst, nullptr, Wd, Hd, radius, depth_threshold, cam, 1, 0u, 0u};
+    half_sample_kernel<<<grid, block, 0, stream_of(s)>>>(hj);
     NUIF_LAUNCHED("half_sample_kernel");
     return NUIF_OK;
 }
@@ -303,15 +336,16 @@ int nuif_frame_preprocess(const uint16_t* raw, float* depth_out, float* const* d
     if (rc) return rc;
     hipStream_t st = stream_of(s);
     dim3 grid(div_up(W, kTileW), div_up(H, kTileH)), block(kTileW, kTileH);
-    preprocess_l0_kernel<<<grid, block, 0, st>>>(raw, depth_out, depth_pyr[0], vmap_pyr[0], W, H, *cam, g, radius,
-                                                 sigma_depth2_inv_half, depth_threshold, scale_for_tsdf);
+    const PreJob pj{raw, depth_out, depth_pyr[0], vmap_pyr[0], W, H, *cam, g, radius, sigma_depth2_inv_half,
+                    depth_threshold, scale_for_tsdf, 0u, 0u};
+    preprocess_l0_kernel<<<grid, block, 0, st>>>(pj);
     NUIF_LAUNCHED("preprocess_l0_kernel");
     for (int l = 1; l < levels; ++l) {
         const int Wd = W >> l, Hd = H >> l;
         dim3 b2(16, 16), g2(div_up(Wd, 16), div_up(Hd, 16));
         // OCL/NuiKinfuOpenCLDepthTracker.cpp:158 subSampleRadius = 1
-        half_sample_kernel<<<g2, b2, 0, st>>>(depth_pyr[l - 1], depth_pyr[l], vmap_pyr[l], Wd, Hd, 1,
-                                              depth_threshold, *cam, 1 << l);
+        const HalfJob hj{depth_pyr[l - 1], depth_pyr[l], vmap_pyr[l], Wd, Hd, 1, depth_threshold, *cam, 1 << l, 0u, 0u};
+        half_sample_kernel<<<g2, b2, 0, st>>>(hj);
         NUIF_LAUNCHED("half_sample_kernel");
     }
     return NUIF_OK;

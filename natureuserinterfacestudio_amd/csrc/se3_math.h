@@ -163,4 +163,8 @@ NUIF_HD int gn_step(const double sums[29], double step2_stop, double R[9], doubl
 // Convergence threshold of gn_step: a step below 1e-6 (radians and metres) stops the level.
 constexpr double kGnStep2 = 1e-12;
 
+// Pixels per work item of the compensated ICP order (256 lanes, 2 pixels each): the
+// order of its sums depends on it.
+constexpr int kCompTilePixels = 512;
+
 }  // namespace nuif_se3

@@ -23,6 +23,7 @@
 // test, trilinear refinement and gradient normal follow K/raycast.cl:128-277
 // operation by operation.
 
+#include "batch_jobs.h"
 #include "dense_volume.h"
 
 namespace nuif {
@@ -220,11 +221,12 @@ __host__ __device__ __forceinline__ size_t bounds_offset(int W, int H, int L) {
     return o;
 }
 
-__global__ void depth_bounds_base_kernel(const float* __restrict__ depth, int W, int H, float* __restrict__ out,
-                                         const int32_t* __restrict__ gate) {
-    if (gate && *gate == 0) return;
+__device__ __forceinline__ void depth_bounds_base_body(const BoundsJob& J, int b) {
+    if (J.gate && *J.gate == 0) return;
+    const int W = J.W, H = J.H;
+    const float* __restrict__ depth = J.depth;
     const int cw = bounds_dim(W, 0), ch = bounds_dim(H, 0);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = b * blockDim.x + threadIdx.x;
     if (i >= cw * ch) return;
     const int cx = i % cw, cy = i / cw;
     float m = -1.f;
@@ -233,12 +235,13 @@ __global__ void depth_bounds_base_kernel(const float* __restrict__ depth, int W,
             const float d = depth[(size_t)y * W + x];
             if (!isnan(d)) m = fmaxf(m, d);
         }
-    out[i] = m;
+    J.out[i] = m;
 }
 
-__global__ void __launch_bounds__(1024)
-    depth_bounds_levels_kernel(int W, int H, float* __restrict__ out, const int32_t* __restrict__ gate) {
-    if (gate && *gate == 0) return;
+__device__ __forceinline__ void depth_bounds_levels_body(const BoundsJob& J) {
+    if (J.gate && *J.gate == 0) return;
+    const int W = J.W, H = J.H;
+    float* __restrict__ out = J.out;
     const int levels = bounds_levels(W, H);
     for (int L = 1; L < levels; ++L) {
         const float* src = out + bounds_offset(W, H, L - 1);
@@ -257,6 +260,18 @@ __global__ void __launch_bounds__(1024)
         }
         __syncthreads();
     }
+}
+
+__global__ void depth_bounds_base_kernel(BoundsJob J) { depth_bounds_base_body(J, blockIdx.x); }
+__global__ void __launch_bounds__(1024) depth_bounds_levels_kernel(BoundsJob J) { depth_bounds_levels_body(J); }
+
+__global__ void depth_bounds_base_batch_kernel(const BoundsJob* __restrict__ jobs, int n) {
+    const BoundsJob& J = jobs[job_of(jobs, n, blockIdx.x)];
+    depth_bounds_base_body(J, (int)(blockIdx.x - J.block0));
+}
+// one block per job
+__global__ void __launch_bounds__(1024) depth_bounds_levels_batch_kernel(const BoundsJob* __restrict__ jobs) {
+    depth_bounds_levels_body(jobs[blockIdx.x]);
 }
 
 // Largest valid depth of the pixels in [x0, x1] x [y0, y1] (within the image), or
@@ -302,12 +317,13 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
 // terms (vgz, zs) are the same for every column and follow as
 // float2[(nseg - 1) * Nx * Ny + k - 1], written by column 0.  Nx, Ny are
 // multiples of 8 (the tile kernel's condition).
-__global__ void __launch_bounds__(256)
-    column_states_kernel(TsdfParams p, NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
-                         const int32_t* __restrict__ gate, float2* __restrict__ walk_states) {
+__device__ __forceinline__ void column_states_body(const TsdfParams& p, const NuiCLCameraParams& cam,
+                                                   const NuiCLRigidTransform* __restrict__ pose,
+                                                   const int32_t* __restrict__ gate, float2* __restrict__ walk_states,
+                                                   int b) {
     if (gate && *gate == 0) return;
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = b * blockDim.x + threadIdx.x;
     if (i >= Nx * Ny) return;
     const int tile = i >> 6, lane = i & 63, tx = Nx / 8;
     const int x = (tile % tx) * 8 + (lane & 7), y = (tile / tx) * 8 + (lane >> 3);
@@ -337,15 +353,23 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+__global__ void __launch_bounds__(256)
+    column_states_kernel(TsdfParams p, NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
+                         const int32_t* __restrict__ gate, float2* __restrict__ walk_states) {
+    column_states_body(p, cam, pose, gate, walk_states, blockIdx.x);
+}
+
 // Fast path: Nx, Ny multiples of 8, Nz a multiple of the column chunk.
 template <class S, bool COLOR>
-__global__ void __launch_bounds__(64)
-    integrate_tile_kernel(const float* __restrict__ depth, const uchar4* __restrict__ bgra,
-                          typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
-                          NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
-                          const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated,
-                          const float* __restrict__ bounds, SurfaceBlocks surf, VoxelWrap wr,
-                          const float2* __restrict__ walk_states) {
+__device__ __forceinline__ void integrate_tile_body(const float* __restrict__ depth, const uchar4* __restrict__ bgra,
+                                                    typename S::word* __restrict__ vol, uchar4* __restrict__ cvol,
+                                                    const TsdfParams& p, const NuiCLCameraParams& cam,
+                                                    const NuiCLRigidTransform* __restrict__ pose, int maxcw,
+                                                    const int32_t* __restrict__ gate,
+                                                    unsigned long long* __restrict__ updated,
+                                                    const float* __restrict__ bounds, const SurfaceBlocks& surf,
+                                                    VoxelWrap wr, const float2* __restrict__ walk_states, int bx,
+                                                    int by, int bz, int gdx, int gdz) {
     if (gate && *gate == 0) return;
     using word = typename S::word;
     constexpr int VPL = 16 / S::kBytes;      // voxels per lane per 16-byte vector
@@ -361,21 +385,20 @@ __global__ void __launch_bounds__(64)
 
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const int lane = threadIdx.x;
-    const int x = blockIdx.x * 8 + (lane & 7), y = blockIdx.y * 8 + (lane >> 3);
+    const int x = bx * 8 + (lane & 7), y = by * 8 + (lane >> 3);
     const NuiCLRigidTransform g = *pose;
     ColumnWalk c = column_setup(x, y, Nx, Ny, p, cam, g);
     const float trunc = p.tranc_dist, tr_inv = 1.f / p.tranc_dist, cz = p.cell_size[2], R8 = g.R_inv[8];
 
     const int sub = lane % LPC;  // position inside a column chunk (phase 2)
     unsigned nupd = 0;
-    // z segment of this wave (blockIdx.z): the walk state at its start is the one the
+    // z segment of this wave (bz): the walk state at its start is the one the
     // same sequence of additions reaches, read from the per-column states of
     // column_states_kernel or replayed here
-    const int zbeg = blockIdx.z * kSegVoxels, zend = min(Nz, zbeg + kSegVoxels);
+    const int zbeg = bz * kSegVoxels, zend = min(Nz, zbeg + kSegVoxels);
     if (zbeg > 0 && walk_states) {
-        const float2 w = walk_states[(size_t)(blockIdx.z - 1) * Nx * Ny +
-                                     ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 64 + lane];
-        const float2 z = walk_states[(size_t)(gridDim.z - 1) * Nx * Ny + blockIdx.z - 1];
+        const float2 w = walk_states[(size_t)(bz - 1) * Nx * Ny + ((size_t)by * gdx + bx) * 64 + lane];
+        const float2 z = walk_states[(size_t)(gdz - 1) * Nx * Ny + bz - 1];
         c.vx = w.x;
         c.vy = w.y;
         c.vgz = z.x;
@@ -443,7 +466,7 @@ __global__ void __launch_bounds__(64)
                 for (int q = 0; q < VPL; ++q) a |= !isnan(tv[q]);
                 act[j] = a;
                 if (a) {
-                    const int gx = blockIdx.x * 8 + (col & 7), gy = blockIdx.y * 8 + (col >> 3);
+                    const int gx = bx * 8 + (col & 7), gy = by * 8 + (col >> 3);
                     const size_t base = wrapped_column(gx, gy, wr, Nx, Ny, Nz) + wrap1(zc + sub * VPL, wr.z, Nz);
                     const uint4 raw = *reinterpret_cast<const uint4*>(vol + base);
                     const word* wv = reinterpret_cast<const word*>(&raw);
@@ -456,7 +479,7 @@ __global__ void __launch_bounds__(64)
                 if (!act[j]) continue;
                 const int col = (it0 + j) * CPI + lane / LPC;
                 const float* tv = &ts[col * kLdsPitch + sub * VPL];
-                const int gx = blockIdx.x * 8 + (col & 7), gy = blockIdx.y * 8 + (col >> 3);
+                const int gx = bx * 8 + (col & 7), gy = by * 8 + (col >> 3);
                 const int pz = wrap1(zc + sub * VPL, wr.z, Nz);  // stored z of the vector (w.z % VPL == 0)
                 const size_t base = wrapped_column(gx, gy, wr, Nx, Ny, Nz) + pz;
                 uint4 outv;
@@ -490,6 +513,46 @@ __global__ void __launch_bounds__(64)
         for (int off = 32; off > 0; off >>= 1) nupd += __shfl_xor(nupd, off);
         if (lane == 0 && nupd) atomicAdd(updated, (unsigned long long)nupd);
     }
+}
+
+template <class S, bool COLOR>
+__global__ void __launch_bounds__(64)
+    integrate_tile_kernel(const float* __restrict__ depth, const uchar4* __restrict__ bgra,
+                          typename S::word* __restrict__ vol, uchar4* __restrict__ cvol, TsdfParams p,
+                          NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
+                          const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated,
+                          const float* __restrict__ bounds, SurfaceBlocks surf, VoxelWrap wr,
+                          const float2* __restrict__ walk_states) {
+    integrate_tile_body<S, COLOR>(depth, bgra, vol, cvol, p, cam, pose, maxcw, gate, updated, bounds, surf, wr,
+                                  walk_states, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.z);
+}
+
+// all sessions' tiles of one storage: block -> (session, tile x, tile y, z segment)
+template <class S, bool COLOR>
+__global__ void __launch_bounds__(64) integrate_tile_batch_kernel(const IntJob* __restrict__ jobs, int n) {
+    const IntJob& J = jobs[job_of(jobs, n, blockIdx.x)];
+    unsigned b = blockIdx.x - J.block0;
+    const int bx = (int)(b % J.gx);
+    b /= J.gx;
+    const int by = (int)(b % J.gy), bz = (int)(b / J.gy);
+    integrate_tile_body<S, COLOR>(J.depth, J.bgra, reinterpret_cast<typename S::word*>(J.vol), J.cvol, J.p, J.cam,
+                                  J.pose, J.maxcw, J.gate, J.updated, J.bounds, J.surf, VoxelWrap{0, 0, 0}, J.walk,
+                                  bx, by, bz, (int)J.gx, (int)J.gz);
+}
+
+// column walk states of all sessions: block -> session by cblock0
+__global__ void __launch_bounds__(256) column_states_batch_kernel(const IntJob* __restrict__ jobs, int n) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].cblock0 <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const IntJob& J = jobs[lo];
+    if (!J.walk) return;
+    column_states_body(J.p, J.cam, J.pose, J.gate, J.walk, (int)(blockIdx.x - J.cblock0));
 }
 
 // Generic path (any resolution): the reference's one-column-per-thread walk.
@@ -671,11 +734,12 @@ __device__ __forceinline__ void superblock_span(const int g[3], const int N[3], 
 constexpr int kSuperLds = 4096;
 
 template <class S, bool WRAP>
-__global__ void __launch_bounds__(64)
-    raycast_kernel(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol, TsdfParams p,
-                   NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
-                   float* __restrict__ nmap, uchar4* __restrict__ cmap, const int32_t* __restrict__ enable,
-                   SurfaceBlocks surf, VoxelWrap wr) {
+__device__ __forceinline__ void raycast_body(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol,
+                                             const TsdfParams& p, const NuiCLCameraParams& cam,
+                                             const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
+                                             float* __restrict__ nmap, uchar4* __restrict__ cmap,
+                                             const int32_t* __restrict__ enable, const SurfaceBlocks& surf,
+                                             VoxelWrap wr, int bx, int by) {
     if (enable && *enable == 0) return;
     // the superblock flags, read by every lane at every superblock it enters
     __shared__ __align__(16) uint8_t sflags[kSuperLds];
@@ -692,7 +756,7 @@ __global__ void __launch_bounds__(64)
         __syncthreads();
     }
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
-    const int x = blockIdx.x * 8 + (threadIdx.x & 7), y = blockIdx.y * 8 + (threadIdx.x >> 3);
+    const int x = bx * 8 + (threadIdx.x & 7), y = by * 8 + (threadIdx.x >> 3);
     if (x >= W || y >= H) return;
     const size_t id = (size_t)y * W + x;
     const f3 NaN3 = mk3(kNaN, kNaN, kNaN);
@@ -846,6 +910,24 @@ __global__ void __launch_bounds__(64)
     if (cmap) cmap[id] = cout;
 }
 
+template <class S, bool WRAP>
+__global__ void __launch_bounds__(64)
+    raycast_kernel(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol, TsdfParams p,
+                   NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
+                   float* __restrict__ nmap, uchar4* __restrict__ cmap, const int32_t* __restrict__ enable,
+                   SurfaceBlocks surf, VoxelWrap wr) {
+    raycast_body<S, WRAP>(vol, cvol, p, cam, pose, vmap, nmap, cmap, enable, surf, wr, blockIdx.x, blockIdx.y);
+}
+
+// all sessions' 8 x 8 pixel tiles of one storage (volumes without wrap)
+template <class S>
+__global__ void __launch_bounds__(64) raycast_batch_kernel(const RayJob* __restrict__ jobs, int n) {
+    const RayJob& J = jobs[job_of(jobs, n, blockIdx.x)];
+    const unsigned b = blockIdx.x - J.block0;
+    raycast_body<S, false>(reinterpret_cast<const typename S::word*>(J.vol), J.cvol, J.p, J.cam, J.pose, J.vmap,
+                           J.nmap, J.cmap, J.enable, J.surf, VoxelWrap{0, 0, 0}, (int)(b % J.gx), (int)(b / J.gx));
+}
+
 // ---- host launchers ------------------------------------------------------------
 
 template <class S>
@@ -908,6 +990,58 @@ int launch_raycast(const void* volume, const uint8_t* color_volume, const TsdfPa
         raycast_kernel<S, true><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, none, wr);
     }
     NUIF_LAUNCHED("raycast_kernel");
+    return NUIF_OK;
+}
+
+int launch_bounds_batch(const BoundsJob* jobs, const BoundsJob* hjobs, int n, hipStream_t st) {
+    if (n <= 0) return NUIF_OK;
+    const BoundsJob& last = hjobs[n - 1];
+    const unsigned blocks = last.block0 + div_up((size_t)bounds_dim(last.W, 0) * bounds_dim(last.H, 0), 256);
+    depth_bounds_base_batch_kernel<<<blocks, 256, 0, st>>>(jobs, n);
+    NUIF_LAUNCHED("depth_bounds_base_batch_kernel");
+    depth_bounds_levels_batch_kernel<<<n, 1024, 0, st>>>(jobs);
+    NUIF_LAUNCHED("depth_bounds_levels_batch_kernel");
+    return NUIF_OK;
+}
+
+int launch_integrate_batch(const IntJob* jobs, const IntJob* hjobs, int n, int storage, bool color, hipStream_t st) {
+    if (n <= 0) return NUIF_OK;
+    const IntJob& last = hjobs[n - 1];
+    const unsigned cblocks =
+        last.cblock0 + (last.walk ? div_up((size_t)last.p.resolution[0] * last.p.resolution[1], 256) : 0u);
+    if (cblocks > 0) {
+        column_states_batch_kernel<<<cblocks, 256, 0, st>>>(jobs, n);
+        NUIF_LAUNCHED("column_states_batch_kernel");
+    }
+    const unsigned blocks = last.block0 + last.gx * last.gy * last.gz;
+    switch (storage) {
+        case NUIF_TSDF_S16:
+            if (color) integrate_tile_batch_kernel<StS16, true><<<blocks, 64, 0, st>>>(jobs, n);
+            else integrate_tile_batch_kernel<StS16, false><<<blocks, 64, 0, st>>>(jobs, n);
+            break;
+        case NUIF_TSDF_F16:
+            if (color) integrate_tile_batch_kernel<StF16, true><<<blocks, 64, 0, st>>>(jobs, n);
+            else integrate_tile_batch_kernel<StF16, false><<<blocks, 64, 0, st>>>(jobs, n);
+            break;
+        default:
+            if (color) integrate_tile_batch_kernel<StF32, true><<<blocks, 64, 0, st>>>(jobs, n);
+            else integrate_tile_batch_kernel<StF32, false><<<blocks, 64, 0, st>>>(jobs, n);
+            break;
+    }
+    NUIF_LAUNCHED("integrate_tile_batch_kernel");
+    return NUIF_OK;
+}
+
+int launch_raycast_batch(const RayJob* jobs, const RayJob* hjobs, int n, int storage, hipStream_t st) {
+    if (n <= 0) return NUIF_OK;
+    const RayJob& last = hjobs[n - 1];
+    const unsigned blocks = last.block0 + last.gx * div_up(last.cam.depthImageHeight, 8);
+    switch (storage) {
+        case NUIF_TSDF_S16: raycast_batch_kernel<StS16><<<blocks, 64, 0, st>>>(jobs, n); break;
+        case NUIF_TSDF_F16: raycast_batch_kernel<StF16><<<blocks, 64, 0, st>>>(jobs, n); break;
+        default: raycast_batch_kernel<StF32><<<blocks, 64, 0, st>>>(jobs, n); break;
+    }
+    NUIF_LAUNCHED("raycast_batch_kernel");
     return NUIF_OK;
 }
 
@@ -1073,9 +1207,10 @@ int nuif_tsdf_depth_bounds(const float* depth, int width, int height, float* bou
     NUIF_CHECK_ARG(depth && bounds && width > 0 && height > 0, "nuif_tsdf_depth_bounds: bad argument");
     hipStream_t st = stream_of(s);
     const int cells = bounds_dim(width, 0) * bounds_dim(height, 0);
-    depth_bounds_base_kernel<<<div_up(cells, 256), 256, 0, st>>>(depth, width, height, bounds, gate);
+    const BoundsJob bj{depth, width, height, bounds, gate, 0u};
+    depth_bounds_base_kernel<<<div_up(cells, 256), 256, 0, st>>>(bj);
     NUIF_LAUNCHED("depth_bounds_base_kernel");
-    depth_bounds_levels_kernel<<<1, 1024, 0, st>>>(width, height, bounds, gate);
+    depth_bounds_levels_kernel<<<1, 1024, 0, st>>>(bj);
     NUIF_LAUNCHED("depth_bounds_levels_kernel");
     return NUIF_OK;
 }

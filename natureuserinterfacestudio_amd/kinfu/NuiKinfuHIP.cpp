@@ -116,6 +116,12 @@ NuiKinfuHIPFrame::NuiKinfuHIPFrame(const NuiTrackerConfig& config, UINT nWidth, 
 
 NuiKinfuHIPFrame::~NuiKinfuHIPFrame() { nuif_stream_synchronize(m_stream); }
 
+float* NuiKinfuHIPFrame::EnsureDepthBounds() {
+    const size_t nb = nuif_tsdf_depth_bounds_bytes((int)m_W, (int)m_H);
+    if (m_bounds.bytes() != nb) m_bounds = DeviceBuffer(nb);
+    return m_bounds.as<float>();
+}
+
 // OCL/NuiKinfuOpenCLFrame.cpp:314-367 (+ the tracker's pyramid)
 void NuiKinfuHIPFrame::UpdateVertexBuffers(UINT16* pDepths, UINT nNum, NuiKinfuCameraState* pCameraState) {
     if (!pDepths) return;
@@ -135,8 +141,7 @@ void NuiKinfuHIPFrame::UpdateVertexBuffersDevice(const UINT16* dDepths, NuiKinfu
     if (m_scaleForTsdf) {
         // depth bounds of the ray-length depth, used by the dense integration to skip
         // runs of voxels behind every observed surface
-        const size_t nb = nuif_tsdf_depth_bounds_bytes((int)m_W, (int)m_H);
-        if (m_bounds.bytes() != nb) m_bounds = DeviceBuffer(nb);
+        EnsureDepthBounds();
         check(nuif_tsdf_depth_bounds(m_depth.as<float>(), (int)m_W, (int)m_H, m_bounds.as<float>(), nullptr, m_stream),
               "nuif_tsdf_depth_bounds");
     }
@@ -231,6 +236,21 @@ bool NuiKinfuHIPDepthTracker::EstimatePose(NuiKinfuFrame* pFrame, NuiKinfuFeedba
     return true;
 }
 
+void NuiKinfuHIPDepthTracker::FillIcpSession(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb,
+                                             NuiKinfuHIPCameraState* cam, nuif_icp_session* out) const {
+    *out = nuif_icp_session{};
+    out->state = m_state.get();
+    out->partials = m_partials.get();
+    out->cam = cam->GetCameraParams();
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) out->vmap_pyr[l] = l < m_icfg.levels ? f->GetVertexBuffer(l) : nullptr;
+    out->ref_vmap = fb->GetVertexBuffer();
+    out->ref_nmap = fb->GetNormalBuffer();
+    out->pose = cam->GetCameraTransformBuffer();
+    out->cfg = m_icfg;
+    out->history = m_history;
+    out->capacity = m_historyCap;
+}
+
 nuif_icp_result NuiKinfuHIPDepthTracker::ReadState() const {
     nuif_icp_result r;
     check(nuif_icp_state_read(m_state.get(), &r, m_stream), "nuif_icp_state_read");
@@ -295,6 +315,22 @@ bool NuiKinfuHIPScene::integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraStat
                                       m_gateIntegrate, m_counter.as<unsigned long long>(), bounds,
                                       m_surface.as<uint8_t>(), m_walk.as<float>(), m_stream),
           "nuif_tsdf_integrate");
+    setDirty();
+    return true;
+}
+
+bool NuiKinfuHIPScene::FillRoundScene(nuif_round_session* out) {
+    const int Nx = (int)m_params.resolution[0], Ny = (int)m_params.resolution[1], Nz = (int)m_params.resolution[2];
+    if (Nx % 8 || Ny % 8 || Nz % 32 || m_wrap[0] || m_wrap[1] || m_wrap[2]) return false;
+    out->volume = m_volume.get();
+    out->color_volume = m_color.as<uint8_t>();
+    out->bgra = nullptr;
+    out->params = m_params;
+    out->storage = m_storage;
+    out->max_color_weight = m_config.max_color_weight;
+    out->updated = m_counter.as<unsigned long long>();
+    out->surface_blocks = m_surface.as<uint8_t>();
+    out->walk_states = m_walk.as<float>();
     setDirty();
     return true;
 }
@@ -647,6 +683,33 @@ bool NuiKinfuTrackingEngine::RunTrackingDevice(INT64, const UINT16* dDepths, con
     const nuif_icp_result r = m_pTracker->ReadState();
     m_pCameraState->SyncFromDevice();
     return r.status == 0;
+}
+
+bool NuiKinfuTrackingEngine::PrepareRoundFrame(const UINT16* dDepths, NuiKinfuScene* pScene,
+                                               const NuiCameraParams& cameraParams, nuif_round_session* out) {
+    if (!m_pTracker || !m_pFrame || !m_pCameraState || !dDepths || m_frames == 0 || m_stageTiming) return false;
+    auto* scene = dynamic_cast<NuiKinfuHIPScene*>(pScene);
+    if (!scene || dynamic_cast<NuiKinfuHIPShiftScene*>(pScene)) return false;
+    *out = nuif_round_session{};
+    if (!scene->FillRoundScene(out)) return false;
+    m_pCameraState->UpdateCameraParams(cameraParams, m_pFrame->GetWidth(), m_pFrame->GetHeight());
+    m_pFrame->SetScaleDepthForTsdf(true);
+    out->depth = dDepths;
+    out->depth_out = m_pFrame->GetDepthBuffer();
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l)
+        out->depth_pyr[l] = l < m_pFrame->GetLevels() ? m_pFrame->GetDepthPyramidLevel(l) : nullptr;
+    out->bounds = m_pFrame->EnsureDepthBounds();
+    out->gaussian = m_pFrame->GetGaussian();
+    out->radius = (int)m_pFrame->GetConfig().filter_radius;
+    out->sigma_depth2_inv_half = m_pFrame->GetConfig().sigma_depth2_inv_half;
+    out->depth_threshold = m_pFrame->GetConfig().depth_threshold;
+    m_pTracker->FillIcpSession(m_pFrame, m_pFeedbackFrame, m_pCameraState, &out->icp);
+    out->vmap = m_pFeedbackFrame->GetVertexBuffer();
+    out->nmap = m_pFeedbackFrame->GetNormalBuffer();
+    out->cmap = out->color_volume ? m_pFeedbackFrame->GetColorBuffer() : nullptr;
+    m_frames++;
+    m_posesDirty = true;
+    return true;
 }
 
 bool NuiKinfuTrackingEngine::synchronize() {

@@ -81,6 +81,11 @@ public:
     // max-depth pyramid of the ray-length depth (nuif_tsdf_depth_bounds), when scaled
     const float* GetDepthBounds() const { return m_bounds.as<float>(); }
     float* GetFilteredDepthBuffer() const { return m_dpyr[0].as<float>(); }
+    float* GetDepthPyramidLevel(int level) const { return m_dpyr[level].as<float>(); }
+    const float* GetGaussian() const { return m_gaussian.data(); }
+    const NuiTrackerConfig& GetConfig() const { return m_cfg; }
+    // the depth-bounds buffer of the ray-length depth (allocated on first use)
+    float* EnsureDepthBounds();
     float* GetVertexBuffer(int level = 0) const { return m_vpyr[level].as<float>(); }
     const float* const* GetVertexPyramid() const { return m_vptrs.data(); }
     uint8_t* GetColorBuffer() const { return m_hasColor ? m_colors.as<uint8_t>() : nullptr; }
@@ -141,6 +146,9 @@ public:
     void SetReduction(int mode) { m_icfg.reduction = mode; }
     void SetHistory(float* dHistory, int capacity) { m_history = dHistory; m_historyCap = capacity; }
     nuif_icp_result ReadState() const;
+    // the arguments of this tracker's nuif_icp_estimate for a frame (a batched round's entry)
+    void FillIcpSession(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb, NuiKinfuHIPCameraState* cam,
+                        nuif_icp_session* out) const;
     // a shifting volume moved the local frame: new local translation and frame offset
     void SetTranslation(NuiKinfuHIPCameraState* cam, const float t[3], const float offset[3]);
 
@@ -236,6 +244,10 @@ public:
     uint8_t* GetColorVolume() const { return m_color.as<uint8_t>(); }
     size_t VolumeBytes() const { return m_volume.bytes(); }
     unsigned long long UpdatedVoxels() const override;
+    // the scene's part of a batched round's entry (volume, surface blocks, walk states);
+    // false when the volume cannot be batched (a shifting volume, or dimensions the tile
+    // integration does not take)
+    virtual bool FillRoundScene(nuif_round_session* out);
 
 protected:
     NuiKinfuVolumeConfig m_config;
@@ -399,6 +411,12 @@ public:
     nuif_kinfu::NuiKinfuHIPDepthTracker* tracker() const { return m_pTracker; }
     const float* poseHistoryDevice() const { return m_history.as<float>(); }
     int poseHistoryCapacity() const { return m_historyCap; }
+    // A tracked frame of a plain dense scene as one session of a batched round
+    // (nuif_fusion_round): fills *out from the session's buffers and counts the frame.
+    // Returns false, doing nothing, when the frame must run alone: the first frame, a
+    // scene other than a plain dense volume of batchable size, or stage timing on.
+    bool PrepareRoundFrame(const UINT16* dDepths, NuiKinfuScene* pScene, const NuiCameraParams& cameraParams,
+                           nuif_round_session* out);
     // Per-stage device timing of tracked frames (events around preprocessing, ICP,
     // integration and raycast on the session stream; frames then run un-captured).
     void setStageTiming(bool on);

@@ -109,6 +109,7 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
 NuiFusionServer::NuiFusionServer(size_t hbmBudget) : m_budget(hbmBudget) {
     check(nuif_stream_create(&m_ctl), "server stream");
     check(nuif_event_create(&m_start), "server event");
+    check(nuif_event_create(&m_batchEnd), "server event");
 }
 
 NuiFusionServer::~NuiFusionServer() {
@@ -119,7 +120,21 @@ NuiFusionServer::~NuiFusionServer() {
     for (nuif_event_t e : m_ends)
         if (e) nuif_event_destroy(e);
     if (m_start) nuif_event_destroy(m_start);
+    if (m_batchEnd) nuif_event_destroy(m_batchEnd);
+    if (m_staging) nuif_host_free(m_staging);
     if (m_ctl) nuif_stream_destroy(m_ctl);
+}
+
+void NuiFusionServer::ensureTable(int n) {
+    if (n <= m_tableCap) return;
+    const int cap = std::max(n, 2 * m_tableCap);
+    if (m_batchPending) check(nuif_event_synchronize(m_batchEnd), "batched round");
+    m_batchPending = false;
+    m_table = DeviceBuffer(nuif_fusion_round_table_bytes(cap));
+    if (m_staging) nuif_host_free(m_staging);
+    m_staging = nullptr;
+    check(nuif_host_alloc(&m_staging, nuif_fusion_round_table_bytes(cap)), "round staging");
+    m_tableCap = cap;
 }
 
 int NuiFusionServer::open(const nuifk_desc& d) {
@@ -178,13 +193,39 @@ void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depth
     }
     m_round.assign(ids, ids + n);
     check(nuif_event_record(m_start, m_ctl), "round start");
+    // the staging copy of the previous batched round must have been uploaded
+    if (m_batchPending) check(nuif_event_synchronize(m_batchEnd), "batched round");
+    m_batchPending = false;
+    std::vector<nuif_round_session> batch;
+    std::vector<int> batched;
     for (int i = 0; i < n; ++i) {
         NuiKinfuSession* s = m_sessions[ids[i]].get();
+        nuif_round_session rs;
+        if (m_batching && s->engine->PrepareRoundFrame(depths[i], s->scene.get(), s->cam, &rs)) {
+            batch.push_back(rs);
+            batched.push_back(ids[i]);
+            continue;
+        }
         const nuif_stream_t st = s->engine->stream();
         check(nuif_stream_wait_event(st, m_start), "round wait");
         s->engine->RunTrackingDevice(0, depths[i], nullptr, s->scene.get(), s->cam, false);
         s->afterFrame();
         check(nuif_event_record(m_ends[ids[i]], st), "frame end");
+    }
+    m_lastBatch = (int)batch.size();
+    if (batch.empty()) return;
+    // the batch follows each session's previous frame; each session's stream follows the
+    // batch (its later work and read-backs see the round's results)
+    for (int id : batched) check(nuif_stream_wait_event(m_ctl, m_ends[id]), "batch wait");
+    ensureTable((int)batch.size());
+    check(nuif_fusion_round(batch.data(), (int)batch.size(), m_table.get(), m_staging, m_ctl), "nuif_fusion_round");
+    check(nuif_event_record(m_batchEnd, m_ctl), "batch end");
+    m_batchPending = true;
+    for (int id : batched) {
+        NuiKinfuSession* s = m_sessions[id].get();
+        check(nuif_stream_wait_event(s->engine->stream(), m_batchEnd), "session wait");
+        check(nuif_event_record(m_ends[id], m_ctl), "frame end");
+        s->afterFrame();
     }
 }
 

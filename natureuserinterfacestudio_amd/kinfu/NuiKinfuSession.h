@@ -49,20 +49,35 @@ public:
     size_t bytesInUse() const { return m_inUse; }
     int sessionCount() const;
 
-    // one round: a frame (device-resident uint16 depth) for each listed session
+    // one round: a frame (device-resident uint16 depth) for each listed session.  The
+    // tracked frames of plain dense sessions run together as one batched round
+    // (nuif_fusion_round) on the server's stream; every other frame (a session's first,
+    // hash and shifting scenes) runs on its session's own stream, overlapping the batch.
     void submit(int n, const int* ids, const uint16_t* const* depths);
     // waits for the round; latencyMs[i] = round start -> end of the i-th frame
     int wait(float* latencyMs);
+    // batched rounds on (default) or off (every frame on its session's stream)
+    void setBatching(bool on) { m_batching = on; }
+    // frames of the last round that ran in the batch
+    int lastBatchSize() const { return m_lastBatch; }
 
 private:
+    void ensureTable(int n);
     size_t m_budget;
     size_t m_inUse = 0;
     std::vector<std::unique_ptr<NuiKinfuSession>> m_sessions;  // nullptr: closed id
     std::vector<size_t> m_bytes;
     std::vector<nuif_event_t> m_ends;  // per id: end of its last submitted frame
     std::vector<int> m_round;          // ids of the last round, in submission order
-    nuif_stream_t m_ctl = nullptr;     // carries the round start event
+    nuif_stream_t m_ctl = nullptr;     // carries the round start event and the batched round
     nuif_event_t m_start = nullptr;
+    nuif_event_t m_batchEnd = nullptr; // end of the last batched round
+    bool m_batching = true;
+    bool m_batchPending = false;       // a batched round's upload may still read the staging copy
+    int m_lastBatch = 0;
+    DeviceBuffer m_table;              // job table of the batched round (device)
+    void* m_staging = nullptr;         // its pinned host copy
+    int m_tableCap = 0;
 };
 
 }  // namespace nuif_kinfu

@@ -286,6 +286,16 @@ int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths
     });
 }
 
+int nuifs_set_batching(void* srv, int on) {
+    return guarded([&] {
+        if (!srv) throw std::runtime_error("nuifs_set_batching: null server");
+        static_cast<NuiFusionServer*>(srv)->setBatching(on != 0);
+        return 0;
+    });
+}
+
+int nuifs_last_batch(void* srv) { return srv ? static_cast<NuiFusionServer*>(srv)->lastBatchSize() : 0; }
+
 int nuifs_wait(void* srv, float* latency_ms) {
     return guarded([&] {
         if (!srv) throw std::runtime_error("nuifs_wait: null server");

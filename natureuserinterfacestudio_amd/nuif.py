@@ -141,6 +141,8 @@ _SIGS = {
     "nuif_icp_record": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "nuif_icp_batch_table_bytes": (c_size_t, [c_int]),
     "nuif_icp_static_scratch_bytes": (c_size_t, []),
+    "nuif_fusion_round_table_bytes": (c_size_t, [c_int]),
+    "nuif_fusion_round": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "nuif_icp_estimate_static": (c_int, [c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                          c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
     "nuif_icp_estimate_batch": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
@@ -283,6 +285,8 @@ _KSIGS = {
     "nuifs_engine": (c_void_p, [c_void_p, c_int]),
     "nuifs_submit": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p)]),
     "nuifs_wait": (c_int, [c_void_p, C.POINTER(c_float)]),
+    "nuifs_set_batching": (c_int, [c_void_p, c_int]),
+    "nuifs_last_batch": (c_int, [c_void_p]),
     "nuif_io_write_frame_image": (c_int, [C.c_char_p, c_int, C.c_int64, C.c_uint32, C.c_uint32, c_void_p,
                                           C.c_uint32]),
     "nuif_io_read_frame_header": (c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_uint32),
@@ -679,6 +683,14 @@ class Server:
         self._ptrs = (c_void_p * n)(*[f.ptr for f in frames])
         self._n = n
         self._check(K().nuifs_submit(self._h, n, self._ids, self._ptrs), "nuifs_submit")
+
+    def set_batching(self, on):
+        """Batched rounds (nuif_fusion_round) on or off."""
+        self._check(K().nuifs_set_batching(self._h, 1 if on else 0), "nuifs_set_batching")
+
+    @property
+    def last_batch(self):
+        return K().nuifs_last_batch(self._h)
 
     def wait(self):
         """Waits for the round; returns per-frame latency (ms, round start -> frame end)."""

@@ -463,16 +463,17 @@ IcpResult icp(const Cam<T>& cam, const IcpConfig& cfg, const T* const* vpyr, con
 //     applied as an SE(3) exponential in FP64 (nuif_se3::gn_step); a step below
 //     1e-6 ends the level; a singular system is a tracking failure.
 
-// The GPU's compensated reduction of one iteration (nuif ICP kernel): tiles of 1024
-// pixels; in tile k thread j (256 threads) takes pixels 1024k + j + 256i, i = 0..3,
-// with Kahan-compensated FP32 sums (the count plainly); each thread's value s - c
+// The GPU's compensated reduction of one iteration (nuif ICP kernel): tiles of T =
+// kCompTilePixels pixels; in tile k thread j (256 threads) takes pixels Tk + j + 256i,
+// i = 0..T/256-1, with Kahan-compensated FP32 sums (the count plainly); each thread's value s - c
 // in double; FP64 tree over each 64-lane wave (lane l takes lane l + d for d = 32,
 // 16, .., 1), the four wave sums added in order; then term by term the tile sums
 // k = j, j + 8, j + 16, .. in order for j = 0..7, and those eight in order.
 inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int npx, const float R[9], const float t[3],
                                  const float* vref, const float* nref, const Rigid<float>& prev, float distThr,
                                  double out[kTerms]) {
-    const int ntiles = (npx + 1023) / 1024;
+    constexpr int T = nuif_se3::kCompTilePixels, PER = T / 256;
+    const int ntiles = (npx + T - 1) / T;
     std::vector<double> tiles((size_t)ntiles * kTerms);
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < ntiles; ++k) {
@@ -480,8 +481,8 @@ inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int n
         float o[kTerms];
         for (int j = 0; j < 256; ++j) {
             float sm[kTerms] = {0}, cp[kTerms] = {0};
-            for (int i = 0; i < 4; ++i) {
-                const int p = 1024 * k + j + 256 * i;
+            for (int i = 0; i < PER; ++i) {
+                const int p = T * k + j + 256 * i;
                 if (p >= npx) continue;
                 icp_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, o);
                 if (o[kTerms - 1] == 0.f) continue;  // rejected: every term is zero

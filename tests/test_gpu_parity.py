@@ -553,7 +553,9 @@ def test_server_mixed_sessions_match_standalone_engines(gpu):
     """One server round after another over sessions of every kind side by side
     (320x240 / 640x480 / 1280x720 frames; dense S16, F16 and F32 volumes of 128^3 and
     256^3, a hash scene and a shifting volume): each session's poses and maps, and
-    the dense volumes, equal those of the same session run alone, bit for bit."""
+    the dense volumes, equal those of the same session run alone, bit for bit.  The
+    dense sessions' tracked frames run as one batched round (nuif_fusion_round) beside
+    the hash and shifting sessions on their own streams."""
     import dataclasses
     announce("server: 5 mixed sessions (320x240..1280x720, S16/F16/F32, hash, shift), 4 rounds vs standalone")
     full256 = dataclasses.replace(W.FULL, volume_res=256)
@@ -573,6 +575,8 @@ def test_server_mixed_sessions_match_standalone_engines(gpu):
         srv.submit(ids, [frames[k][r] for k in range(len(cases))])
         lat = srv.wait()
         assert (lat > 0).all()
+        # after their first frames the three plain dense sessions run as one batched round
+        assert srv.last_batch == (0 if r == 0 else 3), (r, srv.last_batch)
     for k, ((wl, d, dense), sid) in enumerate(zip(cases, ids)):
         ref = nuif.Engine(d)
         for r in range(n):
