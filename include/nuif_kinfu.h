@@ -78,8 +78,13 @@ int nuifk_run(void* h, const uint16_t* depth, int on_device, const uint8_t* bgra
 int nuifk_synchronize(void* h);                    /* 1: no failure so far, 0: a failure */
 int nuifk_reset(void* h);
 /* Pose history rows (NUIF_POSE_ROW floats) of the first n frames; returns the count.  The
- * translation is global (local + the shifting volume's offset, NuiCameraPos::getGlobalTranslation). */
+ * translation is global (local + the shifting volume's offset, NuiCameraPos::getGlobalTranslation).
+ * Every frame since the last reset has a row, a failed one too (ok = 0, the pose it kept); the
+ * reference returns before m_poses.push_back on a failed frame (VO/NuiKinfuTrackingEngine.cpp:107-139). */
 int nuifk_poses(void* h, float* out, int n);
+/* Rows of the device pose ring (default 65536) are copied to the host before the ring wraps
+ * over them, which synchronises the session at most once every `capacity` frames. */
+int nuifk_set_history_capacity(void* h, int capacity);
 int nuifk_state(void* h, nuif_icp_result* out);
 /* Copies the scene's device allocation: the dense volume, or the whole hash scene
  * (layout of nuif_hash_layout); nuifk_volume_bytes gives its size. */

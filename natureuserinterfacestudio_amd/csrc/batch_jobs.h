@@ -102,6 +102,11 @@ struct RayJob {
     unsigned block0;
 };
 
+__device__ __forceinline__ SurfaceBlocks in_global(SurfaceBlocks s) {
+    s.b = in_global(s.b);
+    return s;
+}
+
 // host-side launchers of the batch kernels (jobs: device arrays inside the round table,
 // hjobs: the same jobs on the host, for the grid sizes)
 int launch_preprocess_batch(const PreJob* jobs, const PreJob* hjobs, int n, hipStream_t st);

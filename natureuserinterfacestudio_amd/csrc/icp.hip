@@ -390,19 +390,21 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
     const IcpJob& J = L.job;
     const int level = L.level;
     const int npx = level_pixels(J.cam, level);
-    const float* __restrict__ vcur = J.vmap[level];
+    const float* __restrict__ vcur = in_global(J.vmap[level]);
+    const float* __restrict__ vref = in_global(J.vref);
+    const float* __restrict__ nref = in_global(J.nref);
 
     if constexpr (MODE == kRefOrder) {
         // four 128-pixel groups, two at a time (one per 128 threads)
         const int sub = tid >> 7, l = tid & 127, lane = l & 63, wave = l >> 6;
         const int ngroups = (npx + 127) >> 7;
-        float* part = reinterpret_cast<float*>(J.partials);
+        float* part = in_global(reinterpret_cast<float*>(J.partials));
         for (int r = 0; r < kTileGroups / 2; ++r) {
             const int g = (int)tile * kTileGroups + 2 * r + sub;
             const int p = g * 128 + l;
             float v[kTerms];
             const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
-            icp_terms(vc, L.pose, J.cam, J.vref, J.nref, J.dist, v);
+            icp_terms(vc, L.pose, J.cam, vref, nref, J.dist, v);
             // K/icp.cl:162-190 reduce(): step d = 64 through LDS, steps 32..1 in the group's wave 0
             if (wave == 1) {
 #pragma unroll
@@ -431,7 +433,7 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
         for (int i = 0; i < kPxPerThread; ++i) {
             const int p = p0 + i * kBlock;
             const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
-            ok[i] = icp_assoc(vc, L.pose, J.cam, J.vref, J.nref, J.dist, cf[i]);
+            ok[i] = icp_assoc(vc, L.pose, J.cam, vref, nref, J.dist, cf[i]);
         }
 #pragma unroll
         for (int i = 0; i < kPxPerThread; ++i) {
@@ -454,7 +456,7 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
             if (lane == 0) L.wsum[wv][k] = v;
         }
         __syncthreads();
-        double* part = reinterpret_cast<double*>(J.partials);
+        double* part = in_global(reinterpret_cast<double*>(J.partials));
         if (tid < kTerms)
             part[(size_t)tile * kTerms + tid] = ((L.wsum[0][tid] + L.wsum[1][tid]) + L.wsum[2][tid]) + L.wsum[3][tid];
         __syncthreads();  // wsum is reused by the next tile
@@ -489,7 +491,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int c0 = 0; c0 < n; c0 += kBlock) {
         const int s = c0 + tid;
         uint32_t v = 0u;
-        if (s < n) v = open_tiles(table[s], sync_of(table[s].st)->epoch, planned);
+        if (s < n) v = open_tiles(table[s], sync_of(in_global(table[s].st))->epoch, planned);
         // inclusive scan in the wave, then over the waves
         const int lane = tid & 63, wv = tid >> 6;
         uint32_t x = v;
@@ -531,14 +533,16 @@ __global__ void __launch_bounds__(kBlock)
             for (int w = tid; w < kJobWords; w += kBlock) jd[w] = js[w];
             const IcpJob& J = table[lo];
             float* dp = reinterpret_cast<float*>(&L.pose);
+            IcpState* st = in_global(J.st);
+            const NuiCLRigidTransform* pose = in_global(J.pose);
             if (tid < 12) {
-                dp[tid] = reinterpret_cast<const float*>(J.st)[tid];
+                dp[tid] = reinterpret_cast<const float*>(st)[tid];
             } else if (tid < 21) {
-                L.pose.Ri[tid - 12] = J.pose->R_inv[tid - 12];
+                L.pose.Ri[tid - 12] = pose->R_inv[tid - 12];
             } else if (tid < 24) {
-                L.pose.tp[tid - 21] = J.pose->t[tid - 21];
+                L.pose.tp[tid - 21] = pose->t[tid - 21];
             } else if (tid == 24) {
-                L.level = (int)(sync_of(J.st)->epoch / kEpochIters);
+                L.level = (int)(sync_of(st)->epoch / kEpochIters);
             }
             cur = lo;
             __syncthreads();
@@ -580,7 +584,7 @@ __global__ void __launch_bounds__(kFinBlock)
         // compute_sums' tree over each chunk of 128 group sums (one wave per chunk, its
         // 29 terms together), then the host's sequential float sum over the chunk
         // results (OCL/NuiKinfuOpenCLDepthTracker.cpp:394-402, 428-440)
-        const float* part = reinterpret_cast<const float*>(J.partials);
+        const float* part = in_global(reinterpret_cast<const float*>(J.partials));
         const int ngroups = (npx + 127) >> 7, nchunks = (ngroups + 127) >> 7;
         const int lane = tid & 63, wv = tid >> 6;
         for (int q = wv; q < nchunks; q += kFinBlock / 64) {
@@ -607,7 +611,7 @@ __global__ void __launch_bounds__(kFinBlock)
         __syncthreads();
         finish_epoch<MODE>(J, F.S, nullptr, F.sumsf, level, iter);
     } else {
-        const double* part = reinterpret_cast<const double*>(J.partials);
+        const double* part = in_global(reinterpret_cast<const double*>(J.partials));
         const int ntiles = level_tiles(J.cam, level);
         if (tid < kTerms * 8) {
             // term k, segment j: tiles j, j + 8, ... in order (loads 16 at a time)

@@ -73,6 +73,15 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
     return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 __device__ __forceinline__ bool nan3(f3 v) { return isnan(v.x) || isnan(v.y) || isnan(v.z); }
+// p, known to point into device (global) memory: accesses through the result
+// compile to global_* instructions.  A pointer read from LDS or from a job table
+// is otherwise generic, and its accesses compile to flat_* instructions (which
+// also wait on the LDS counter).  The integer round trip keeps the compiler from
+// folding the cast away.
+template <class T>
+__device__ __forceinline__ T* in_global(T* p) {
+    return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
 __device__ __forceinline__ f3 ld3(const float* p, size_t i) { return f3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 __device__ __forceinline__ void st3(float* p, size_t i, f3 v) {
     p[3 * i] = v.x;

@@ -95,7 +95,7 @@ __device__ __forceinline__ void preprocess_l0_body(const PreJob& J, int bx, int 
     const int W = J.W, H = J.H;
     const NuiCLCameraParams cam = J.cam;
     const int ox = bx * kTileW, oy = by * kTileH;
-    load_tile<true>(tile, tw, th, J.raw, W, H, ox, oy, cam.sensorDepthWorldMin, cam.sensorDepthWorldMax);
+    load_tile<true>(tile, tw, th, in_global(J.raw), W, H, ox, oy, cam.sensorDepthWorldMin, cam.sensorDepthWorldMax);
     __syncthreads();
     const int x = ox + threadIdx.x, y = oy + threadIdx.y;
     if (x >= W || y >= H) return;
@@ -108,13 +108,13 @@ __device__ __forceinline__ void preprocess_l0_body(const PreJob& J, int bx, int 
         const float lambda = sqrtf(xl * xl + yl * yl + 1.f);
         d = d * lambda;
     }
-    J.depth_out[i] = d;
+    in_global(J.depth_out)[i] = d;
     const float f = bilateral_px(tile, tw, x, y, ox, oy, W, H, J.g, J.r, J.sigma_depth, J.thr);
-    J.depth_f[i] = f;
+    in_global(J.depth_f)[i] = f;
     // K/transformMaps.cl:18-42 with div = 1
     f3 v = mk3(kNaN, kNaN, kNaN);
     if (!isnan(f)) v = mk3(f * (((float)x - cam.cx) * cam.fx_inv), f * (((float)y - cam.cy) * cam.fy_inv), f * 1.0f);
-    st3(J.vmap, i, v);
+    st3(in_global(J.vmap), i, v);
 }
 
 __global__ void __launch_bounds__(256) preprocess_l0_kernel(PreJob J) { preprocess_l0_body(J, blockIdx.x, blockIdx.y); }
@@ -147,7 +147,7 @@ __device__ __forceinline__ void half_sample_body(const HalfJob& J, int bx, int b
     const int Wd = J.Wd, Hd = J.Hd, r = J.r;
     const int x = bx * 16 + threadIdx.x, y = by * 16 + threadIdx.y;
     if (x >= Wd || y >= Hd) return;
-    const float* __restrict__ src = J.src;
+    const float* __restrict__ src = in_global(J.src);
     const float thr = J.thr;
     const int Ws = Wd << 1, Hs = Hd << 1;
     const int sx = x << 1, sy = y << 1;
@@ -166,8 +166,8 @@ __device__ __forceinline__ void half_sample_body(const HalfJob& J, int bx, int b
         }
     const float o = cnt > 0 ? s / (float)cnt : kNaN;
     const size_t i = (size_t)y * Wd + x;
-    J.dst[i] = o;
-    if (J.vmap) st3(J.vmap, i, vertex_at(o, x, y, J.cam, J.div));
+    in_global(J.dst)[i] = o;
+    if (J.vmap) st3(in_global(J.vmap), i, vertex_at(o, x, y, J.cam, J.div));
 }
 
 __global__ void half_sample_kernel(HalfJob J) { half_sample_body(J, blockIdx.x, blockIdx.y); }

@@ -222,9 +222,9 @@ __host__ __device__ __forceinline__ size_t bounds_offset(int W, int H, int L) {
 }
 
 __device__ __forceinline__ void depth_bounds_base_body(const BoundsJob& J, int b) {
-    if (J.gate && *J.gate == 0) return;
+    if (J.gate && *in_global(J.gate) == 0) return;
     const int W = J.W, H = J.H;
-    const float* __restrict__ depth = J.depth;
+    const float* __restrict__ depth = in_global(J.depth);
     const int cw = bounds_dim(W, 0), ch = bounds_dim(H, 0);
     const int i = b * blockDim.x + threadIdx.x;
     if (i >= cw * ch) return;
@@ -235,13 +235,13 @@ __device__ __forceinline__ void depth_bounds_base_body(const BoundsJob& J, int b
             const float d = depth[(size_t)y * W + x];
             if (!isnan(d)) m = fmaxf(m, d);
         }
-    J.out[i] = m;
+    in_global(J.out)[i] = m;
 }
 
 __device__ __forceinline__ void depth_bounds_levels_body(const BoundsJob& J) {
-    if (J.gate && *J.gate == 0) return;
+    if (J.gate && *in_global(J.gate) == 0) return;
     const int W = J.W, H = J.H;
-    float* __restrict__ out = J.out;
+    float* __restrict__ out = in_global(J.out);
     const int levels = bounds_levels(W, H);
     for (int L = 1; L < levels; ++L) {
         const float* src = out + bounds_offset(W, H, L - 1);
@@ -535,9 +535,11 @@ __global__ void __launch_bounds__(64) integrate_tile_batch_kernel(const IntJob* 
     const int bx = (int)(b % J.gx);
     b /= J.gx;
     const int by = (int)(b % J.gy), bz = (int)(b / J.gy);
-    integrate_tile_body<S, COLOR>(J.depth, J.bgra, reinterpret_cast<typename S::word*>(J.vol), J.cvol, J.p, J.cam,
-                                  J.pose, J.maxcw, J.gate, J.updated, J.bounds, J.surf, VoxelWrap{0, 0, 0}, J.walk,
-                                  bx, by, bz, (int)J.gx, (int)J.gz);
+    integrate_tile_body<S, COLOR>(in_global(J.depth), in_global(J.bgra),
+                                  in_global(reinterpret_cast<typename S::word*>(J.vol)), in_global(J.cvol), J.p, J.cam,
+                                  in_global(J.pose), J.maxcw, in_global(J.gate), in_global(J.updated),
+                                  in_global(J.bounds), in_global(J.surf), VoxelWrap{0, 0, 0}, in_global(J.walk), bx, by,
+                                  bz, (int)J.gx, (int)J.gz);
 }
 
 // column walk states of all sessions: block -> session by cblock0
@@ -552,7 +554,8 @@ __global__ void __launch_bounds__(256) column_states_batch_kernel(const IntJob* 
     }
     const IntJob& J = jobs[lo];
     if (!J.walk) return;
-    column_states_body(J.p, J.cam, J.pose, J.gate, J.walk, (int)(blockIdx.x - J.cblock0));
+    column_states_body(J.p, J.cam, in_global(J.pose), in_global(J.gate), in_global(J.walk),
+                       (int)(blockIdx.x - J.cblock0));
 }
 
 // Generic path (any resolution): the reference's one-column-per-thread walk.
@@ -924,8 +927,10 @@ template <class S>
 __global__ void __launch_bounds__(64) raycast_batch_kernel(const RayJob* __restrict__ jobs, int n) {
     const RayJob& J = jobs[job_of(jobs, n, blockIdx.x)];
     const unsigned b = blockIdx.x - J.block0;
-    raycast_body<S, false>(reinterpret_cast<const typename S::word*>(J.vol), J.cvol, J.p, J.cam, J.pose, J.vmap,
-                           J.nmap, J.cmap, J.enable, J.surf, VoxelWrap{0, 0, 0}, (int)(b % J.gx), (int)(b / J.gx));
+    raycast_body<S, false>(in_global(reinterpret_cast<const typename S::word*>(J.vol)), in_global(J.cvol), J.p, J.cam,
+                           in_global(J.pose), in_global(J.vmap), in_global(J.nmap), in_global(J.cmap),
+                           in_global(J.enable), in_global(J.surf), VoxelWrap{0, 0, 0}, (int)(b % J.gx),
+                           (int)(b / J.gx));
 }
 
 // ---- host launchers ------------------------------------------------------------

@@ -548,9 +548,12 @@ void NuiKinfuTrackingEngine::log(const std::string& fileName) const {
 
 // VO/NuiKinfuTrackingEngine.cpp:46-58
 void NuiKinfuTrackingEngine::reset() {
+    if (m_frames > m_drained) check(nuif_stream_synchronize(m_stream), "reset");
     m_frames = 0;
+    m_rows.clear();
+    m_drained = 0;
+    m_anyFailed = false;
     m_poses.clear();
-    m_posesDirty = false;
     if (m_graph) {
         nuif_graph_destroy(m_graph);
         m_graph = nullptr;
@@ -653,6 +656,7 @@ bool NuiKinfuTrackingEngine::RunTrackingDevice(INT64, const UINT16* dDepths, con
         check(nuif_memcpy_d2d(m_pFrame->GetRawBuffer(), dDepths, n * sizeof(UINT16), m_stream), "depth copy");
     if (dBgra && scene && scene->hasColorData()) m_pFrame->UpdateColorBuffersDevice(dBgra);
 
+    beforeFrame();
     if (m_frames == 0) {
         // VO/NuiKinfuTrackingEngine.cpp:93-104: first frame: feedback maps from the
         // frame itself, integrate, no raycast
@@ -663,7 +667,7 @@ bool NuiKinfuTrackingEngine::RunTrackingDevice(INT64, const UINT16* dDepths, con
             scene->integrateVolume(m_pFrame, m_pCameraState);
         }
         m_pTracker->RecordPose(m_pCameraState);
-    } else if (m_useGraph && !m_stageTiming) {
+    } else if (m_useGraph && !m_stageTiming && !(scene && scene->RequiresHostWork())) {
         if (!m_graph || m_graphScene != scene) {
             if (m_graph) nuif_graph_destroy(m_graph);
             m_graph = nullptr;
@@ -677,7 +681,6 @@ bool NuiKinfuTrackingEngine::RunTrackingDevice(INT64, const UINT16* dDepths, con
         enqueueTrackedFrame(scene);
     }
     m_frames++;
-    m_posesDirty = true;
     if (!sync) return true;
     check(nuif_stream_synchronize(m_stream), "frame");
     const nuif_icp_result r = m_pTracker->ReadState();
@@ -692,6 +695,7 @@ bool NuiKinfuTrackingEngine::PrepareRoundFrame(const UINT16* dDepths, NuiKinfuSc
     if (!scene || dynamic_cast<NuiKinfuHIPShiftScene*>(pScene)) return false;
     *out = nuif_round_session{};
     if (!scene->FillRoundScene(out)) return false;
+    beforeFrame();
     m_pCameraState->UpdateCameraParams(cameraParams, m_pFrame->GetWidth(), m_pFrame->GetHeight());
     m_pFrame->SetScaleDepthForTsdf(true);
     out->depth = dDepths;
@@ -708,42 +712,75 @@ bool NuiKinfuTrackingEngine::PrepareRoundFrame(const UINT16* dDepths, NuiKinfuSc
     out->nmap = m_pFeedbackFrame->GetNormalBuffer();
     out->cmap = out->color_volume ? m_pFeedbackFrame->GetColorBuffer() : nullptr;
     m_frames++;
-    m_posesDirty = true;
     return true;
 }
 
 bool NuiKinfuTrackingEngine::synchronize() {
     check(nuif_stream_synchronize(m_stream), "synchronize");
     if (m_pCameraState) m_pCameraState->SyncFromDevice();
-    std::vector<float> rows((size_t)std::min(m_frames, m_historyCap) * NUIF_POSE_ROW);
-    if (!rows.empty()) {
-        check(nuif_memcpy_d2h(rows.data(), m_history.get(), rows.size() * sizeof(float), m_stream), "history");
-        check(nuif_stream_synchronize(m_stream), "history");
+    drainHistory();
+    return !m_anyFailed;
+}
+
+// The next frame writes ring row m_frames % capacity: rows not yet copied to the host
+// are drained first when the ring is full.
+void NuiKinfuTrackingEngine::beforeFrame() {
+    if (m_frames - m_drained >= m_historyCap) drainHistory();
+}
+
+void NuiKinfuTrackingEngine::drainHistory() const {
+    if (m_drained >= m_frames || !m_history.get()) return;
+    check(nuif_stream_synchronize(m_stream), "history");
+    const size_t row = NUIF_POSE_ROW;
+    const size_t old = m_rows.size();
+    m_rows.resize(old + (size_t)(m_frames - m_drained) * row);
+    for (int f = m_drained; f < m_frames;) {
+        const int r = f % m_historyCap;
+        const int k = std::min(m_frames - f, m_historyCap - r);  // contiguous rows up to the ring's end
+        check(nuif_memcpy_d2h(&m_rows[old + (size_t)(f - m_drained) * row], m_history.as<float>() + (size_t)r * row,
+                              (size_t)k * row * sizeof(float), m_stream),
+              "history");
+        f += k;
     }
-    bool ok = true;
-    for (size_t i = 0; i < rows.size(); i += NUIF_POSE_ROW) ok = ok && rows[i + 12] != 0.f;
-    return ok;
+    check(nuif_stream_synchronize(m_stream), "history");
+    for (size_t i = old; i < m_rows.size(); i += row) m_anyFailed = m_anyFailed || m_rows[i + 12] == 0.f;
+    m_drained = m_frames;
+}
+
+void NuiKinfuTrackingEngine::setPoseHistoryCapacity(int capacity) {
+    if (capacity < 1) throw std::invalid_argument("pose history capacity must be at least 1");
+    if (m_frames > m_drained) {
+        check(nuif_stream_synchronize(m_stream), "history");
+        drainHistory();
+    }
+    m_historyCap = capacity;
+    if (m_history.get()) {
+        m_history = DeviceBuffer((size_t)m_historyCap * NUIF_POSE_ROW * sizeof(float));
+        if (m_pTracker) m_pTracker->SetHistory(m_history.as<float>(), m_historyCap);
+        if (m_graph) {  // the captured frame holds the old ring's address
+            nuif_graph_destroy(m_graph);
+            m_graph = nullptr;
+            m_graphScene = nullptr;
+        }
+    }
+}
+
+const std::vector<float>& NuiKinfuTrackingEngine::poseRows() const {
+    drainHistory();
+    return m_rows;
 }
 
 const NuiCameraPos& NuiKinfuTrackingEngine::getCameraPose(int time) const {
-    if (m_posesDirty) {
-        const int n = std::min(m_frames, m_historyCap);
-        std::vector<float> rows((size_t)n * NUIF_POSE_ROW);
-        if (n) {
-            check(nuif_memcpy_d2h(rows.data(), m_history.get(), rows.size() * sizeof(float), m_stream), "history");
-            check(nuif_stream_synchronize(m_stream), "history");
-        }
-        m_poses.clear();
-        for (int i = 0; i < n; ++i) {
-            NuiCameraPos p;
-            Matrix3frm R;
-            std::memcpy(R.data(), &rows[(size_t)i * NUIF_POSE_ROW], 9 * sizeof(float));
-            p.setRotation(R);
-            const float* t = &rows[(size_t)i * NUIF_POSE_ROW + 9];
-            p.setLocalTranslation(Vector3f(t[0], t[1], t[2]));
-            m_poses.push_back(p);
-        }
-        m_posesDirty = false;
+    drainHistory();
+    const size_t row = NUIF_POSE_ROW;
+    for (size_t i = m_poses.size(); i < m_rows.size() / row; ++i) {
+        NuiCameraPos p;
+        Matrix3frm R;
+        std::memcpy(R.data(), &m_rows[i * row], 9 * sizeof(float));
+        p.setRotation(R);
+        const float* t = &m_rows[i * row + 9];
+        p.setLocalTranslation(Vector3f(t[0], t[1], t[2]));
+        m_poses.push_back(p);
     }
     if (time == -1 || time >= (int)m_poses.size()) {
         if (m_poses.empty()) return m_pCameraState->GetCameraPos();

@@ -211,6 +211,9 @@ public:
     virtual float Truncation() const = 0;
     // running count of voxels written by integration (synchronises)
     virtual unsigned long long UpdatedVoxels() const = 0;
+    // the scene's integration waits for the frame's pose and decides on the host
+    // (shifting volume, host swapping): such frames cannot be replayed from a graph
+    virtual bool RequiresHostWork() const { return false; }
 
 protected:
     const int32_t* m_gateIntegrate = nullptr;
@@ -272,6 +275,7 @@ public:
     NuiKinfuHIPShiftScene(const NuiKinfuVolumeConfig& config, int storage, nuif_stream_t s);
     void reset() override;
     bool integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) override;
+    bool RequiresHostWork() const override { return true; }
     // the whole volume placed at its offset (the reference's fetchShiftVolumeKernel with wrap 0)
     bool Volume2CLVertices(NuiKinfuMesh* pData) override;
     void SetTracker(NuiKinfuHIPDepthTracker* tracker) { m_tracker = tracker; }
@@ -324,6 +328,7 @@ public:
     static nuif_hash_params ParamsOf(const NuiHashingSDFConfig& config, int storage, bool color);
     // host swapping (m_bUseSwapping): blocks held by the host store
     size_t StoredBlocks() const;
+    bool RequiresHostWork() const override { return m_cache != nullptr; }
 
 private:
     NuiHashingSDFConfig m_config;
@@ -411,6 +416,12 @@ public:
     nuif_kinfu::NuiKinfuHIPDepthTracker* tracker() const { return m_pTracker; }
     const float* poseHistoryDevice() const { return m_history.as<float>(); }
     int poseHistoryCapacity() const { return m_historyCap; }
+    // Rows of the device pose ring (written by the frame's ICP end, row frame % capacity)
+    // are copied to the host before the ring can wrap over them; capacity >= 1.
+    void setPoseHistoryCapacity(int capacity);
+    // Every frame's row (NUIF_POSE_ROW floats: R, t, ok, error, count, integrated) since
+    // the last reset, in frame order; synchronises the stream.
+    const std::vector<float>& poseRows() const;
     // A tracked frame of a plain dense scene as one session of a batched round
     // (nuif_fusion_round): fills *out from the session's buffers and counts the frame.
     // Returns false, doing nothing, when the frame must run alone: the first frame, a
@@ -426,6 +437,8 @@ public:
 
 private:
     void enqueueTrackedFrame(nuif_kinfu::NuiKinfuHIPSceneBase* scene);
+    void drainHistory() const;
+    void beforeFrame();
     void recordStage(int k);
     bool m_stageTiming = false;
     std::vector<nuif_event_t> m_events;  // 5 per timed frame
@@ -444,8 +457,10 @@ private:
     nuif_kinfu::NuiKinfuHIPSceneBase* m_graphScene = nullptr;
     nuif_kinfu::DeviceBuffer m_history;
     int m_historyCap = 1 << 16;
+    mutable std::vector<float> m_rows;  // host copy of the rows of frames [0, m_drained)
+    mutable int m_drained = 0;
+    mutable bool m_anyFailed = false;
     mutable std::vector<NuiCameraPos> m_poses;
-    mutable bool m_posesDirty = false;
 };
 
 }  // namespace NuiKinfuEngine

@@ -94,18 +94,23 @@ int nuifk_poses(void* h, float* out, int n) {
     return guarded([&] {
         Session* s = static_cast<Session*>(h);
         s->engine->synchronize();
-        const int m = std::min(n, std::min(s->engine->getFrameID(), s->engine->poseHistoryCapacity()));
+        const std::vector<float>& rows = s->engine->poseRows();
+        const int m = std::min(n, (int)(rows.size() / NUIF_POSE_ROW));
         if (m > 0) {
-            check(nuif_memcpy_d2h(out, s->engine->poseHistoryDevice(), (size_t)m * NUIF_POSE_ROW * sizeof(float),
-                                  s->engine->stream()),
-                  "pose history");
-            check(nuif_stream_synchronize(s->engine->stream()), "pose history");
+            std::memcpy(out, rows.data(), (size_t)m * NUIF_POSE_ROW * sizeof(float));
             // global translation = local + the shifting volume's offset after that frame
             if (s->offsets.size() >= (size_t)m * 3)
                 for (int i = 0; i < m; ++i)
                     for (int k = 0; k < 3; ++k) out[(size_t)i * NUIF_POSE_ROW + 9 + k] += s->offsets[(size_t)i * 3 + k];
         }
         return m;
+    });
+}
+
+int nuifk_set_history_capacity(void* h, int capacity) {
+    return guarded([&] {
+        static_cast<Session*>(h)->engine->setPoseHistoryCapacity(capacity);
+        return 0;
     });
 }
 

@@ -260,6 +260,7 @@ _KSIGS = {
     "nuifk_synchronize": (c_int, [c_void_p]),
     "nuifk_reset": (c_int, [c_void_p]),
     "nuifk_poses": (c_int, [c_void_p, c_void_p, c_int]),
+    "nuifk_set_history_capacity": (c_int, [c_void_p, c_int]),
     "nuifk_state": (c_int, [c_void_p, C.POINTER(IcpResult)]),
     "nuifk_read_volume": (c_int, [c_void_p, c_void_p, c_size_t]),
     "nuifk_read_maps": (c_int, [c_void_p, c_void_p, c_void_p]),
@@ -569,6 +570,10 @@ class Engine:
         c = np.empty((max(n, 1), 4), np.float32)
         self._check(K().nuifk_shift_cache(self._h, p.ctypes.data, c.ctypes.data, n), "shift cache")
         return p[:n], c[:n]
+
+    def set_history_capacity(self, capacity):
+        """Rows of the device pose ring; rows are copied to the host before it wraps."""
+        self._check(K().nuifk_set_history_capacity(self._h, capacity), "history capacity")
 
     def poses(self, n):
         out = np.zeros((n, POSE_ROW), np.float32)

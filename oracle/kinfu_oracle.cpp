@@ -503,6 +503,11 @@ OR_EXPORT long long or_engine_volume_words(void* hp, void* out) {
 }
 
 // Raw short2 volume of the compat engine (2 x int16 per voxel).
+// 1 when the last frame integrated (VO/NuiKinfuTrackingEngine.cpp:113-127 gate passed or first frame)
+OR_EXPORT int or_engine_integrated(void* hp) {
+    return with_engine((OrEngine*)hp, [&](auto& e) { return e.integrated_last ? 1 : 0; });
+}
+
 OR_EXPORT int or_engine_volume_raw(void* hp, int16_t* out) {
     OrEngine* h = (OrEngine*)hp;
     if (h->prec != 0) return -1;

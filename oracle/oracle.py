@@ -55,6 +55,7 @@ def lib():
             "or_engine_run": (ci, [vp, u16p, vp, f64p, f64p]),
             "or_engine_volume": (None, [vp, f32p, i32p]),
             "or_engine_volume_raw": (ci, [vp, i16p]),
+            "or_engine_integrated": (ci, [vp]),
             "or_engine_refmaps": (None, [vp, f32p, f32p]),
             "or_integrate_s16_f": (cll, [f32p, vp, i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, ci]),
             "or_raycast_s16_f": (None, [i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, f32p, f32p, vp]),
@@ -155,6 +156,10 @@ class OracleEngine:
         ok = lib().or_engine_run(self.h, np.ascontiguousarray(depth, np.uint16),
                                  None if bgra is None else np.ascontiguousarray(bgra).ctypes.data, pose, info)
         return ok, pose, info
+
+    def integrated(self):
+        """Whether the last frame integrated (first frame, or the gate passed)."""
+        return lib().or_engine_integrated(self.h) == 1
 
     def volume(self):
         n = self.wl.volume_res ** 3

@@ -503,12 +503,17 @@ def test_engine_full_size_against_oracles(gpu):
     t_r = ref[:, 0] / 32766.0
     m = (w_g > 0) & (ref[:, 1] > 0)
     assert np.abs(t_g[m] - t_r[m]).max() <= 1e-3
-    # raycast depth (reference maps for the next frame) within 0.5 mm
-    vg, _ = eng.maps()
-    vr, _ = o32.refmaps()
-    ok = ~np.isnan(vg[:, 2]) & ~np.isnan(vr[:, 2])
-    assert ok.sum() > 50000
-    assert np.quantile(np.abs(vg[ok] - vr[ok]).max(1), 0.999) < 5e-4
+    # raycast (reference maps for the next frame): bit for bit, NaN pattern included,
+    # as at 128^3 -- the superblock / surface-block skips and the walk states at full size
+    vg, ng = eng.maps()
+    vr, nr = o32.refmaps()
+    assert (~np.isnan(vr[:, 2])).sum() > 50000
+    for name, a, b in (("vmap", vg, vr), ("nmap", ng, nr)):
+        a = np.asarray(a, np.float32)
+        b = np.asarray(b, np.float32)
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=name)
+        m = ~np.isnan(a)
+        np.testing.assert_array_equal(a[m].view(np.uint32), b[m].view(np.uint32), err_msg=name)
 
 
 # ------------------------------------------------------------------ multi-session server
