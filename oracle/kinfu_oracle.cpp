@@ -656,6 +656,16 @@ OR_EXPORT void or_raycast_s16_wrap_f(const int16_t* vol2, const uint8_t* color_v
                            make_rigid<float>(R9, t3), vmap, nmap, cmap, wr);
 }
 OR_EXPORT float or_effective_trunc(int N, float dim, float trunc) { return make_tsdf<float>(N, dim, trunc).trunc; }
+// K/raycast.cl:36-120 interpolateTrilineary at world point p3 of a short2 volume
+OR_EXPORT float or_trilinear_s16_f(const int16_t* vol2, int N, float dim, float trunc, const float* p3) {
+    VolS16 v;
+    v.v.assign(vol2, vol2 + 2 * (size_t)N * N * N);
+    V3<float> p;
+    p.x = p3[0];
+    p.y = p3[1];
+    p.z = p3[2];
+    return trilinear<float, VolS16>(p, v, make_tsdf<float>(N, dim, trunc));
+}
 
 // Reproducible float exp / sin / cos (exposed for their accuracy tests)
 OR_EXPORT float or_expf_det(float x) { return nuif_det::expf_det(x); }

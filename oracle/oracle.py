@@ -60,6 +60,7 @@ def lib():
             "or_integrate_s16_f": (cll, [f32p, vp, i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, ci]),
             "or_raycast_s16_f": (None, [i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, f32p, f32p, vp]),
             "or_effective_trunc": (cf, [ci, cf, cf]),
+            "or_trilinear_s16_f": (cf, [i16p, ci, cf, cf, f32p]),
             "or_integrate_s16_wrap_f": (cll, [f32p, vp, i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, ci, i32p]),
             "or_raycast_s16_wrap_f": (None, [i16p, vp, ci, cf, cf, f32p, ci, ci, f32p, f32p, f32p, f32p, vp, i32p]),
             "or_points_dense_slice": (cll, [vp, vp, ci, ci, ci, ci, f32p, i32p, i32p, i32p, vp, vp, cll]),
