@@ -248,6 +248,9 @@ typedef struct nuif_icp_session {
     nuif_icp_config cfg;
     float* history;
     int capacity;
+    int32_t* outcome;                          /* NULL, or an int32 that the frame's end sets to 1
+                                                  (tracked) or 0 (failed): device memory or pinned
+                                                  host memory (nuif_host_alloc) */
 } nuif_icp_session;
 /* Device bytes of the session table of nuif_icp_estimate_batch. */
 size_t nuif_icp_batch_table_bytes(int n);

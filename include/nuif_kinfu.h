@@ -97,6 +97,9 @@ int nuifk_shift_state(void* h, int32_t* offset, float* offset_size);
 int nuifk_shift_cache(void* h, float* points, float* colors, size_t cap);
 /* Hash scene with swapping: blocks held by the host store (NuiKinfuOpenCLHashGlobalCache). */
 int nuifk_hash_stored_blocks(void* h, size_t* out);
+/* Hash scene: insertions that found no brick (pool exhausted) or no free bucket slot so far,
+ * which the reference drops silently; 0 for a dense scene.  Synchronises the session. */
+int nuifk_scene_failures(void* h, unsigned long long* out);
 /* VO/NuiTrackerConfig.cpp:26-115 key=value files; -1 if the file cannot be opened. */
 int nuifk_load_tracker_config(const char* path, nuifk_desc* d);
 int nuifk_save_tracker_config(const char* path, const nuifk_desc* d);
@@ -135,9 +138,19 @@ int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths
 int nuifs_set_batching(void* srv, int on);
 /* Frames of the last round that ran in the batch. */
 int nuifs_last_batch(void* srv);
-/* Wait for the last round; latency_ms[i] for the i-th submitted frame (may be NULL).
- * Tracking outcomes stay in each session's pose history (nuifk_poses). */
+/* Wait for the last round; latency_ms[i] for the i-th submitted frame (may be NULL).  Then
+ * every frame's tracking outcome is known (nuifs_last_outcomes) and the failure policy is
+ * applied to the sessions whose frame failed. */
 int nuifs_wait(void* srv, float* latency_ms);
+/* What a tracking failure does to its session.  The reference's tracking manager either
+ * resets the engine (m_bAutoReset) or stops (VO/NuiKinfuTrackingManager.cpp:94-106). */
+#define NUIFS_ON_FAILURE_KEEP 0    /* default: the next frame tracks from the last good pose */
+#define NUIFS_ON_FAILURE_RESET 1   /* engine reset and scene cleared: the next frame starts over */
+#define NUIFS_ON_FAILURE_EVICT 2   /* the session is closed and its device memory released */
+int nuifs_set_failure_policy(void* srv, int policy);
+/* Outcomes of the frames of the last waited round, in submission order: 1 tracked, 0 failed
+ * (a session's first frame always tracks).  Copies min(cap, n) and returns n. */
+int nuifs_last_outcomes(void* srv, int32_t* out, int cap);
 
 #ifdef __cplusplus
 }

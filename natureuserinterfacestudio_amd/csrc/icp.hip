@@ -112,6 +112,7 @@ struct IcpJob {
     int32_t mode;
     int32_t single;             // one epoch only (nuif_icp_iteration): finished after it
     int32_t first_epoch;        // single: the epoch to run
+    int32_t* outcome;           // NULL, or receives 1 (tracked) / 0 (failed) at the frame's end
 };
 
 __host__ __device__ __forceinline__ int level_pixels(const NuiCLCameraParams& c, int level) {
@@ -706,11 +707,12 @@ __device__ float rodrigues_norm(const float Rm[9]) {
 // A session whose epochs did not all run (fewer launched than its schedule needs)
 // fails with the incomplete flag set.
 __device__ void end_frame(IcpState* st, NuiCLRigidTransform* pose, float gate_threshold, float* history,
-                          int capacity) {
+                          int capacity, int32_t* outcome = nullptr) {
     if (sync_of(st)->epoch != kFinished) {
         st->failed = 1;
         st->timeout = 1;
     }
+    if (outcome) *outcome = st->failed ? 0 : 1;
     const int frame = st->frame;
     st->frame = frame + 1;
     float* row = history ? history + (size_t)(frame % capacity) * NUIF_POSE_ROW : nullptr;
@@ -759,7 +761,7 @@ __global__ void icp_end_kernel(const IcpJob* __restrict__ jobs, IcpJob single) {
     if (threadIdx.x != 0) return;
     __shared__ IcpJob J;
     J = jobs ? jobs[blockIdx.x] : single;
-    end_frame(J.st, J.pose, J.gate_threshold, J.history, J.capacity);
+    end_frame(J.st, J.pose, J.gate_threshold, J.history, J.capacity, J.outcome);
 }
 
 // A frame without tracking (the first frame): the pose is kept and the frame
@@ -886,6 +888,7 @@ int make_job(const nuif_icp_session& in, IcpJob* j) {
     J.pose = in.pose;
     J.history = in.history;
     J.capacity = in.capacity;
+    J.outcome = in.outcome;
     J.cam = in.cam;
     const float L = (float)cfg.levels;
     J.dist = cfg.dist_threshold + cfg.dist_threshold / L;  // :300-304

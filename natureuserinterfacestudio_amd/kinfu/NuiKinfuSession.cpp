@@ -122,7 +122,14 @@ NuiFusionServer::~NuiFusionServer() {
     if (m_start) nuif_event_destroy(m_start);
     if (m_batchEnd) nuif_event_destroy(m_batchEnd);
     if (m_staging) nuif_host_free(m_staging);
+    if (m_status) nuif_host_free(m_status);
     if (m_ctl) nuif_stream_destroy(m_ctl);
+}
+
+void NuiFusionServer::setFailurePolicy(int policy) {
+    if (policy != NUIFS_ON_FAILURE_KEEP && policy != NUIFS_ON_FAILURE_RESET && policy != NUIFS_ON_FAILURE_EVICT)
+        throw std::invalid_argument("server: unknown failure policy");
+    m_policy = policy;
 }
 
 void NuiFusionServer::ensureTable(int n) {
@@ -192,16 +199,29 @@ void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depth
             if (ids[j] == ids[i]) throw std::runtime_error("server submit: a session appears twice in a round");
     }
     m_round.assign(ids, ids + n);
+    m_slot.assign((size_t)n, -1);
+    m_outcomes.clear();
     check(nuif_event_record(m_start, m_ctl), "round start");
-    // the staging copy of the previous batched round must have been uploaded
+    // the staging copy and the outcome slots of the previous batched round are free
     if (m_batchPending) check(nuif_event_synchronize(m_batchEnd), "batched round");
     m_batchPending = false;
+    if (n > m_statusCap) {
+        if (m_status) nuif_host_free(m_status);
+        m_status = nullptr;
+        void* p = nullptr;
+        check(nuif_host_alloc(&p, (size_t)std::max(n, 2 * m_statusCap) * sizeof(int32_t)), "outcome slots");
+        m_status = static_cast<int32_t*>(p);
+        m_statusCap = std::max(n, 2 * m_statusCap);
+    }
     std::vector<nuif_round_session> batch;
     std::vector<int> batched;
     for (int i = 0; i < n; ++i) {
         NuiKinfuSession* s = m_sessions[ids[i]].get();
         nuif_round_session rs;
         if (m_batching && s->engine->PrepareRoundFrame(depths[i], s->scene.get(), s->cam, &rs)) {
+            m_slot[i] = (int)batch.size();
+            m_status[m_slot[i]] = -1;
+            rs.icp.outcome = m_status + m_slot[i];
             batch.push_back(rs);
             batched.push_back(ids[i]);
             continue;
@@ -234,6 +254,31 @@ int NuiFusionServer::wait(float* latencyMs) {
         const int id = m_round[i];
         check(nuif_event_synchronize(m_ends[id]), "frame wait");
         if (latencyMs) check(nuif_event_elapsed_ms(m_start, m_ends[id], &latencyMs[i]), "frame latency");
+    }
+    // outcomes: the batched frames' from their slots, the others' from their session's state
+    // (the first frame of a session records its pose and always tracks)
+    m_outcomes.assign(m_round.size(), 1);
+    for (size_t i = 0; i < m_round.size(); ++i) {
+        NuiKinfuSession* s = m_sessions[m_round[i]].get();
+        if (m_slot[i] >= 0) {
+            m_outcomes[i] = m_status[m_slot[i]] == 1 ? 1 : 0;
+        } else if (s->engine->getFrameID() > 1) {
+            m_outcomes[i] = s->engine->tracker()->ReadState().status == 0 ? 1 : 0;
+        }
+    }
+    if (m_policy == NUIFS_ON_FAILURE_KEEP) return 0;
+    for (size_t i = 0; i < m_round.size(); ++i) {
+        if (m_outcomes[i]) continue;
+        const int id = m_round[i];
+        if (m_policy == NUIFS_ON_FAILURE_EVICT) {
+            close(id);
+        } else {
+            NuiKinfuSession* s = m_sessions[id].get();
+            s->engine->synchronize();
+            s->engine->reset();
+            s->scene->reset();
+            s->offsets.clear();
+        }
     }
     return 0;
 }

@@ -54,8 +54,17 @@ public:
     // (nuif_fusion_round) on the server's stream; every other frame (a session's first,
     // hash and shifting scenes) runs on its session's own stream, overlapping the batch.
     void submit(int n, const int* ids, const uint16_t* const* depths);
-    // waits for the round; latencyMs[i] = round start -> end of the i-th frame
+    // waits for the round; latencyMs[i] = round start -> end of the i-th frame.  Then
+    // each frame's tracking outcome is known (lastOutcomes) and the failure policy is
+    // applied to the sessions whose frame failed.
     int wait(float* latencyMs);
+    // what a failed frame does to its session (VO/NuiKinfuTrackingManager.cpp:94-106):
+    // NUIFS_ON_FAILURE_KEEP (track the next frame from the last good pose), _RESET (the
+    // reference's m_bAutoReset, with a cleared scene: the next frame starts the session
+    // over), _EVICT (the reference's quit: the session is closed, its memory released)
+    void setFailurePolicy(int policy);
+    // outcome of each frame of the last waited round: 1 tracked, 0 failed
+    const std::vector<int32_t>& lastOutcomes() const { return m_outcomes; }
     // batched rounds on (default) or off (every frame on its session's stream)
     void setBatching(bool on) { m_batching = on; }
     // frames of the last round that ran in the batch
@@ -78,6 +87,11 @@ private:
     DeviceBuffer m_table;              // job table of the batched round (device)
     void* m_staging = nullptr;         // its pinned host copy
     int m_tableCap = 0;
+    int32_t* m_status = nullptr;       // pinned: the batched frames' outcomes, written by the ICP end
+    int m_statusCap = 0;
+    std::vector<int> m_slot;           // per frame of the round: its outcome slot (-1: not batched)
+    std::vector<int32_t> m_outcomes;
+    int m_policy = NUIFS_ON_FAILURE_KEEP;
 };
 
 }  // namespace nuif_kinfu

@@ -145,6 +145,20 @@ int nuifk_hash_stored_blocks(void* h, size_t* out) {
     });
 }
 
+int nuifk_scene_failures(void* h, unsigned long long* out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("nuifk_scene_failures: null output");
+        auto* hs = dynamic_cast<NuiKinfuHIPHashScene*>(static_cast<Session*>(h)->scene.get());
+        *out = 0;
+        if (hs) {
+            uint32_t st[4];
+            hs->Stats(st);
+            *out = st[2];
+        }
+        return 0;
+    });
+}
+
 int nuifk_state(void* h, nuif_icp_result* out) {
     return guarded([&] {
         *out = static_cast<Session*>(h)->engine->tracker()->ReadState();
@@ -305,6 +319,25 @@ int nuifs_wait(void* srv, float* latency_ms) {
     return guarded([&] {
         if (!srv) throw std::runtime_error("nuifs_wait: null server");
         return static_cast<NuiFusionServer*>(srv)->wait(latency_ms);
+    });
+}
+
+int nuifs_set_failure_policy(void* srv, int policy) {
+    return guarded([&] {
+        if (!srv) throw std::runtime_error("nuifs_set_failure_policy: null server");
+        static_cast<NuiFusionServer*>(srv)->setFailurePolicy(policy);
+        return 0;
+    });
+}
+
+int nuifs_last_outcomes(void* srv, int32_t* out, int cap) {
+    return guarded([&] {
+        if (!srv) throw std::runtime_error("nuifs_last_outcomes: null server");
+        const std::vector<int32_t>& o = static_cast<NuiFusionServer*>(srv)->lastOutcomes();
+        const int n = std::min(cap, (int)o.size());
+        if (n > 0 && !out) throw std::invalid_argument("nuifs_last_outcomes: null output");
+        for (int i = 0; i < n; ++i) out[i] = o[i];
+        return (int)o.size();
     });
 }
 

@@ -49,6 +49,7 @@ ICP_TERMS = 29
 ICP_MAX_LEVELS = 4
 POSE_ROW = 16
 TSDF_S16, TSDF_F16, TSDF_F32 = 0, 1, 2
+ON_FAILURE_KEEP, ON_FAILURE_RESET, ON_FAILURE_EVICT = 0, 1, 2
 
 
 class IcpResult(C.Structure):
@@ -286,6 +287,9 @@ _KSIGS = {
     "nuifs_engine": (c_void_p, [c_void_p, c_int]),
     "nuifs_submit": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p)]),
     "nuifs_wait": (c_int, [c_void_p, C.POINTER(c_float)]),
+    "nuifs_set_failure_policy": (c_int, [c_void_p, c_int]),
+    "nuifs_last_outcomes": (c_int, [c_void_p, c_void_p, c_int]),
+    "nuifk_scene_failures": (c_int, [c_void_p, c_void_p]),
     "nuifs_set_batching": (c_int, [c_void_p, c_int]),
     "nuifs_last_batch": (c_int, [c_void_p]),
     "nuif_io_write_frame_image": (c_int, [C.c_char_p, c_int, C.c_int64, C.c_uint32, C.c_uint32, c_void_p,
@@ -571,6 +575,12 @@ class Engine:
         self._check(K().nuifk_shift_cache(self._h, p.ctypes.data, c.ctypes.data, n), "shift cache")
         return p[:n], c[:n]
 
+    def scene_failures(self):
+        """Hash scene: insertions dropped for want of a brick or a bucket slot (0 if dense)."""
+        out = C.c_ulonglong()
+        self._check(K().nuifk_scene_failures(self._h, C.byref(out)), "scene failures")
+        return out.value
+
     def set_history_capacity(self, capacity):
         """Rows of the device pose ring; rows are copied to the host before it wraps."""
         self._check(K().nuifk_set_history_capacity(self._h, capacity), "history capacity")
@@ -698,10 +708,27 @@ class Server:
         return K().nuifs_last_batch(self._h)
 
     def wait(self):
-        """Waits for the round; returns per-frame latency (ms, round start -> frame end)."""
+        """Waits for the round; returns per-frame latency (ms, round start -> frame end).
+        The failure policy has then been applied (evicted sessions leave self.sessions)."""
         lat = (c_float * max(self._n, 1))()
         self._check(K().nuifs_wait(self._h, lat), "nuifs_wait")
+        if getattr(self, "_policy", ON_FAILURE_KEEP) == ON_FAILURE_EVICT:
+            for sid, ok in zip(list(self._ids), self.last_outcomes()):
+                if not ok and sid in self.sessions:
+                    self.sessions.pop(sid)._h = None
         return np.array(lat[:self._n], np.float64)
+
+    def set_failure_policy(self, policy):
+        """ON_FAILURE_KEEP / ON_FAILURE_RESET / ON_FAILURE_EVICT."""
+        self._check(K().nuifs_set_failure_policy(self._h, policy), "nuifs_set_failure_policy")
+        self._policy = policy
+
+    def last_outcomes(self):
+        """Tracking outcome of each frame of the last waited round (1 tracked, 0 failed)."""
+        n = self._check(K().nuifs_last_outcomes(self._h, None, 0), "nuifs_last_outcomes")
+        out = (C.c_int32 * max(n, 1))()
+        self._check(K().nuifs_last_outcomes(self._h, out, n), "nuifs_last_outcomes")
+        return np.array(out[:n], np.int32)
 
     def close(self):
         if self._h:
