@@ -119,6 +119,97 @@ def run_server(nuif, wl, frames, sessions, storage, warmup, rounds, budget):
     return out
 
 
+C5_SEED = 0x4E554905  # "NUI\0" + 5
+
+
+def c5_trace(sessions=64, rate=2.0, seed=C5_SEED):
+    """The multi-session arrival trace: Poisson arrivals at `rate` per second, 200-600
+    frames at 30 Hz per session, resolution uniform over 320x240 / 640x480 / 1280x720,
+    scene uniform over dense 256^3, dense 512^3 (3 m, FP16 volumes) and hash 5 mm."""
+    rng = np.random.default_rng(seed)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, sessions))
+    frames = rng.integers(200, 601, sessions)
+    res = rng.integers(0, 3, sessions)
+    kind = rng.integers(0, 3, sessions)
+    return arrive, frames, res, kind
+
+
+def run_c5(nuif, W, sessions, rate, budget, pool_frames=40):
+    """Virtual time in 30 Hz ticks: at each tick the sessions that have arrived are
+    admitted while the device-memory budget allows (the rest wait, first come first
+    served), then every active session submits its next frame as one server round.
+    A failed frame resets its session (the reference's auto reset).  Frames come from
+    per-(resolution, scene) pools played forwards and backwards, resident in HBM."""
+    arrive, nfr, res, kind = c5_trace(sessions, rate)
+    dense_wl = [dataclasses.replace(W.SMALL, volume_dim=3.0, trunc=0.025), W.FULL, W.HD]
+    hash_wl = [W.HASH_SMALL, W.HASH, W.HASH_HD]
+    pools = {}
+    for r in range(3):
+        for k in (0, 1):
+            wl = dense_wl[r] if k == 0 else hash_wl[r]
+            pools[(r, k)] = [nuif.DeviceArray.from_host(wl.render(f)[0]) for f in range(pool_frames)]
+
+    def desc(i):
+        r, k = int(res[i]), int(kind[i])
+        if k == 2:
+            return hash_wl[r].kinfu_desc(storage=STORAGES["f16"], gate=0.0)
+        wl = dataclasses.replace(dense_wl[r], volume_res=256 if k == 0 else 512)
+        return wl.kinfu_desc(storage=STORAGES["f16"], gate=0.0)
+
+    def frame(i, f):
+        period = 2 * pool_frames - 2
+        q = f % period
+        return pools[(int(res[i]), 1 if kind[i] == 2 else 0)][q if q < pool_frames else period - q]
+
+    srv = nuif.Server(budget)
+    srv.set_failure_policy(nuif.ON_FAILURE_RESET)
+    tick = 1.0 / 30
+    t, nxt, queue, active = 0.0, 0, [], {}
+    lat, round_ms, peak, peak_bytes, refused, failures, done, rounds_over = [], [], 0, 0, 0, 0, 0, 0
+    gpu_t = 0.0
+    while nxt < sessions or queue or active:
+        while nxt < sessions and arrive[nxt] <= t:
+            queue.append(nxt)
+            nxt += 1
+        while queue:
+            sid = srv.open(desc(queue[0]))
+            if sid is None:
+                refused += 1
+                break
+            active[sid] = [queue.pop(0), 0]
+        if active:
+            ids = list(active)
+            fr = [frame(active[s][0], active[s][1]) for s in ids]
+            t0 = time.perf_counter()
+            srv.submit(ids, fr)
+            lat.append(srv.wait())
+            dt = time.perf_counter() - t0
+            gpu_t += dt
+            round_ms.append(dt * 1e3)
+            rounds_over += dt > tick
+            failures += int((srv.last_outcomes() == 0).sum())
+            peak = max(peak, len(ids))
+            peak_bytes = max(peak_bytes, srv.bytes_in_use)
+            for s in ids:
+                active[s][1] += 1
+                if active[s][1] >= nfr[active[s][0]]:
+                    srv.close_session(s)
+                    del active[s]
+                    done += 1
+        t += tick
+    srv.close()
+    lat = np.concatenate(lat)
+    rm = np.array(round_ms)
+    return {"sessions": int(sessions), "arrival_rate_per_s": rate, "frames": int(lat.size),
+            "trace_s": round(t, 2), "completed": done, "peak_concurrent": peak,
+            "peak_bytes": int(peak_bytes), "admission_waits": refused, "tracking_failures": failures,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "max_ms": round(float(lat.max()), 3), "round_p99_ms": round(float(np.percentile(rm, 99)), 3),
+            "rounds_over_33ms": int(rounds_over), "aggregate_fps": round(lat.size / gpu_t, 1),
+            "mix": "resolution uniform over 320x240/640x480/1280x720; dense 256^3 / dense 512^3 over 3 m "
+                   "(FP16 volumes, compensated ICP) / hash 5 mm; failures reset the session"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +221,8 @@ def main():
     ap.add_argument("--server-storage", choices=sorted(STORAGES), default="f16")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="server device-memory budget")
     ap.add_argument("--cpu-frames", type=int, default=16)
+    ap.add_argument("--c5-sessions", type=int, default=64, help="arrival-trace sessions (0: skip that leg)")
+    ap.add_argument("--c5-rate", type=float, default=2.0, help="arrivals per second")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device to device and exit (scales the HBM byte counters)")
@@ -188,6 +281,13 @@ def main():
         server = run_server(nuif, wl, frames, args.sessions, STORAGES[args.server_storage], args.warmup, args.steps,
                             args.budget_gb * 2 ** 30)
 
+    c5 = None
+    if args.c5_sessions > 0:
+        print(f"[bench] arrival trace: {args.c5_sessions} sessions at {args.c5_rate}/s", file=sys.stderr, flush=True)
+        c5 = run_c5(nuif, W, args.c5_sessions, args.c5_rate, args.budget_gb * 2 ** 30)
+        print(f"[bench] arrival trace done: {c5['frames']} frames, peak {c5['peak_concurrent']} sessions",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         print(f"[bench] CPU baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
@@ -216,6 +316,7 @@ def main():
         "fp32_serial_fps": round(fp32["fps"], 3) if fp32 else None,
         "server_vs_fp32_serial": round(server["fps"] / fp32["fps"], 3) if server and fp32 else None,
         "roofline": roof,
+        "arrival_trace": c5,
         "cpu_baseline": cpu,
         "device": info["name"],
     }

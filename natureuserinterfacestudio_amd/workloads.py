@@ -139,6 +139,12 @@ HASH = Workload("hash-640x480-5mm", 640, 480, 525.0, 525.0, 319.5, 239.5, frames
 HASH_SMALL = Workload("hash-320x240-5mm", 320, 240, 262.5, 262.5, 159.5, 119.5, frames=30, scene_kind=1,
                       volume_res=0, volume_dim=0.0, trunc=0.02, iterations=[10, 5, 4], seed_index=2)
 
+# the third resolution of the multi-session workloads (16:9, the same field of view across)
+HD = Workload("hd-1280x720-512^3", 1280, 720, 1050.0, 1050.0, 639.5, 359.5, frames=300, orbit_radius=1.5,
+              volume_res=512, volume_dim=3.0, trunc=0.025, iterations=[10, 5, 4], seed_index=4)
+HASH_HD = Workload("hash-1280x720-5mm", 1280, 720, 1050.0, 1050.0, 639.5, 359.5, frames=500, scene_kind=1,
+                   volume_res=0, volume_dim=0.0, trunc=0.02, iterations=[10, 5, 4], seed_index=4)
+
 
 def tiny(width=64, height=48, res=32, frames=4):
     """A small workload for quick checks (same scene family as SMALL, scaled intrinsics)."""
