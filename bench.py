@@ -115,8 +115,52 @@ def run_server(nuif, wl, frames, sessions, storage, warmup, rounds, budget):
     out = {"sessions": len(ids), "fps": len(ids) * rounds / t_all, "round_ms": t_all / rounds * 1e3,
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "max_ms": float(lat.max()), "bytes_in_use": int(srv.bytes_in_use), "ok": bool(ok)}
+    out["stages"] = server_stages(nuif, srv, ids, wl, frames, warmup + rounds, min(rounds, 10), storage)
     srv.close()
     return out
+
+
+def server_stages(nuif, srv, ids, wl, frames, first, rounds, storage):
+    """Further rounds with the batched round's stages event-timed on the server stream,
+    and the algorithmic bytes of each stage from what the rounds did: updated voxels
+    (integration: 2 B n_upd + 4 W H per session), iterations run per pyramid level (ICP:
+    12 W_l H_l + 24 W H + 2 x 232 B per 512-pixel tile of partial sums, per iteration) and
+    distinct voxels read (raycast: B U + 28 W H per session, U counted at the last pose)."""
+    upd0 = [srv.sessions[i].updated_voxels() for i in ids]
+    srv.set_stage_timing(True)
+    srv.take_stage_times()
+    icp_bytes = 0.0
+    W, H = wl.width, wl.height
+    for r in range(first, first + rounds):
+        srv.submit(ids, [frames[r + i] for i in range(len(ids))])
+        srv.wait()
+        for i in ids:
+            its = srv.sessions[i].state().level_iterations
+            for lv in range(len(wl.iterations)):
+                px = (W >> lv) * (H >> lv)
+                icp_bytes += its[lv] * (12 * px + 24 * W * H + 2 * 232 * -(-px // 512))
+    n, ms = srv.take_stage_times()
+    srv.set_stage_timing(False)
+    B = nuif.L().nuif_tsdf_voxel_bytes(storage)
+    upd = sum(srv.sessions[i].updated_voxels() - u for i, u in zip(ids, upd0)) / max(n, 1)
+    U = [srv.sessions[i].raycast_footprint() for i in ids]
+    names = ["preprocess", "bounds", "icp", "integrate", "raycast"]
+    avg = {k: v / max(n, 1) for k, v in zip(names, ms)}
+    alg = {"icp": icp_bytes / max(n, 1), "integrate": 2 * B * upd + 4 * W * H * len(ids),
+           "raycast": B * sum(U) + 28 * W * H * len(ids)}
+    kernels = {"icp": "icp_tiles_kernel<1> + icp_finish_kernel<1> (all steps of the round)",
+               "integrate": f"column_states_batch_kernel + integrate_tile_batch_kernel<{storage}>",
+               "raycast": "raycast_batch_kernel"}
+    roofs = []
+    for k in ("icp", "integrate", "raycast"):
+        gbs = alg[k] / (avg[k] / 1e3) / 1e9
+        roofs.append({"stage": k, "kernel": kernels[k], "bound": "hbm", "achieved": round(gbs, 2),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5),
+                      "traffic": measured_traffic(f"server_{k}"), "algorithmic_bytes": int(alg[k]),
+                      "avg_ms": round(avg[k], 4)})
+    return {"rounds": n, "avg_ms": {k: round(v, 4) for k, v in avg.items()},
+            "updated_voxels_per_round": int(upd), "raycast_unique_voxels_per_session": int(np.mean(U)),
+            "rooflines": roofs}
 
 
 C5_SEED = 0x4E554905  # "NUI\0" + 5
@@ -249,6 +293,8 @@ def main():
     wl = W.FULL
     info = nuif.device_info()
     n = args.warmup + args.steps + max(args.sessions - 1, 0)
+    if args.sessions > 0:
+        n += min(args.steps, 10)  # the server's stage-timed rounds
     if n > wl.frames:
         wl = dataclasses.replace(wl, frames=n)
     print(f"[bench] {info['name']} CUs={info['cu']}; rendering {n} frames of {wl.name}", file=sys.stderr, flush=True)
@@ -293,6 +339,13 @@ def main():
         print(f"[bench] CPU baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
         cpu = cpu_baseline(wl, args.cpu_frames)
 
+    if server and server.get("stages"):
+        # the roofline of the round's dominant stage (the server is the headline value)
+        roofs = server["stages"]["rooflines"]
+        single_roof = roof
+        roof = dict(max(roofs, key=lambda r: r["avg_ms"]))
+        roof["rooflines"] = roofs
+        roof["single_session_integrate"] = single_roof
     if server:
         value, steps_ms = server["fps"], server["round_ms"]
         workload = (f"{wl.name}: {server['sessions']} resident sessions, one fused frame each per round "

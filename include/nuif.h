@@ -160,6 +160,7 @@ typedef struct nuif_icp_result {
     int32_t frame;       /* frames finalised */
     double sums[NUIF_ICP_TERMS]; /* 29 sums of the last executed iteration */
     double Rd[9], td[3]; /* compensated order: the pose in FP64 */
+    int32_t level_iterations[NUIF_ICP_MAX_LEVELS]; /* iterations that ran per level, last frame */
 } nuif_icp_result;
 
 /* Summation order of the 29 ICP sums */
@@ -296,6 +297,15 @@ int nuif_tsdf_integrate(const float* depth, const uint8_t* bgra, void* volume, u
 int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
                       const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap, float* nmap,
                       uint8_t* cmap, int storage, nuif_stream_t s);
+/* Measurement, no counterpart in the reference: the number of distinct voxels the raycast
+ * reads at this pose (with these surface blocks), i.e. U of its algorithmic bytes
+ * B * U + 28 * W * H.  The same march as nuif_tsdf_raycast (no wrap), with every voxel
+ * read marked in a bitset; vmap / nmap receive the raycast.  workspace:
+ * nuif_tsdf_footprint_bytes(params) of device memory.  Synchronises s; returns U. */
+size_t nuif_tsdf_footprint_bytes(const TsdfParams* params);
+long long nuif_tsdf_raycast_footprint(const void* volume, const TsdfParams* params, const NuiCLCameraParams* cam,
+                                      const NuiCLRigidTransform* pose, int storage, const uint8_t* surface_blocks,
+                                      float* vmap, float* nmap, void* workspace, nuif_stream_t s);
 /* Same, skipped entirely when the device int32 *enable is zero (tracking failure). */
 int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,
                             const NuiCLCameraParams* cam, const NuiCLRigidTransform* pose, float* vmap,
@@ -392,6 +402,11 @@ size_t nuif_fusion_round_table_bytes(int n);
 /* table / staging: nuif_fusion_round_table_bytes(n) each; the staging copy may be reused once
  * the stream has passed this call's upload (e.g. after waiting for the round). */
 int nuif_fusion_round(const nuif_round_session* sessions, int n, void* table, void* staging, nuif_stream_t s);
+/* nuif_fusion_round with 6 events recorded on s around its stages (no counterpart in the
+ * reference): [0] start, [1] after preprocessing and the pyramid, [2] after the depth bounds,
+ * [3] after the ICP, [4] after the integration, [5] after the raycast. */
+int nuif_fusion_round_timed(const nuif_round_session* sessions, int n, void* table, void* staging,
+                            nuif_event_t* events6, nuif_stream_t s);
 
 /* ---- prefix sum and compaction (OCL/NuiOpenCLPrefixSum.cpp:75-300, K/prefixSum.cl:163-232) ---- */
 

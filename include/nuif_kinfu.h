@@ -100,6 +100,9 @@ int nuifk_hash_stored_blocks(void* h, size_t* out);
 /* Hash scene: insertions that found no brick (pool exhausted) or no free bucket slot so far,
  * which the reference drops silently; 0 for a dense scene.  Synchronises the session. */
 int nuifk_scene_failures(void* h, unsigned long long* out);
+/* Dense scene: distinct voxels the raycast reads at the current camera pose
+ * (nuif_tsdf_raycast_footprint), for the raycast's algorithmic bytes. */
+int nuifk_raycast_footprint(void* h, unsigned long long* out);
 /* VO/NuiTrackerConfig.cpp:26-115 key=value files; -1 if the file cannot be opened. */
 int nuifk_load_tracker_config(const char* path, nuifk_desc* d);
 int nuifk_save_tracker_config(const char* path, const nuifk_desc* d);
@@ -138,6 +141,11 @@ int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths
 int nuifs_set_batching(void* srv, int on);
 /* Frames of the last round that ran in the batch. */
 int nuifs_last_batch(void* srv);
+/* Device time of the batched rounds' stages, event-timed on the server stream: sums (ms) of
+ * [preprocess + pyramid, depth bounds, ICP, integration, raycast] over the rounds timed since
+ * the last call; returns their number. */
+int nuifs_set_stage_timing(void* srv, int on);
+int nuifs_take_stage_times(void* srv, double* ms5);
 /* Wait for the last round; latency_ms[i] for the i-th submitted frame (may be NULL).  Then
  * every frame's tracking outcome is known (nuifs_last_outcomes) and the failure policy is
  * applied to the sessions whose frame failed. */

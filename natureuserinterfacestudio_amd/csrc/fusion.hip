@@ -81,6 +81,11 @@ extern "C" {
 size_t nuif_fusion_round_table_bytes(int n) { return n > 0 ? Layout(n).total : 0; }
 
 int nuif_fusion_round(const nuif_round_session* S, int n, void* table, void* staging, nuif_stream_t s) {
+    return nuif_fusion_round_timed(S, n, table, staging, nullptr, s);
+}
+
+int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, void* staging, nuif_event_t* ev,
+                            nuif_stream_t s) {
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(n >= 0 && (n == 0 || (S && table && staging)), "nuif_fusion_round: bad arguments");
     if (n == 0) return NUIF_OK;
@@ -225,6 +230,11 @@ int nuif_fusion_round(const nuif_round_session* S, int n, void* table, void* sta
     }
 
     NUIF_HIP(hipMemcpyAsync(table, staging, lay.total, hipMemcpyHostToDevice, st));
+    auto mark = [&](int k) -> int {
+        if (ev) NUIF_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(ev[k]), st));
+        return NUIF_OK;
+    };
+    if ((rc = mark(0))) return rc;
     rc = launch_preprocess_batch(reinterpret_cast<const PreJob*>(d + lay.pre), pre, n, st);
     if (rc) return rc;
     for (int l = 1; l < kLevels && nhalf[l] > 0; ++l) {
@@ -232,21 +242,25 @@ int nuif_fusion_round(const nuif_round_session* S, int n, void* table, void* sta
                                reinterpret_cast<const HalfJob*>(h + lay.half[l]), nhalf[l], st);
         if (rc) return rc;
     }
+    if ((rc = mark(1))) return rc;
     rc = launch_bounds_batch(reinterpret_cast<const BoundsJob*>(d + lay.bounds), bj, n, st);
     if (rc) return rc;
+    if ((rc = mark(2))) return rc;
     rc = launch_icp_jobs(d + lay.icp, h + lay.icp, n, nref, st);
     if (rc) return rc;
+    if ((rc = mark(3))) return rc;
     for (const Group& g : igroups) {
         rc = launch_integrate_batch(reinterpret_cast<const IntJob*>(d + lay.integ) + g.first, ij + g.first, g.count,
                                     g.storage, g.color, st);
         if (rc) return rc;
     }
+    if ((rc = mark(4))) return rc;
     for (const Group& g : rgroups) {
         rc = launch_raycast_batch(reinterpret_cast<const RayJob*>(d + lay.ray) + g.first, rj + g.first, g.count,
                                   g.storage, st);
         if (rc) return rc;
     }
-    return NUIF_OK;
+    return mark(5);
 }
 
 }  // extern "C"

@@ -80,6 +80,7 @@ struct IcpState {
     float off[3];          // shifting volume: offset of the local frame (metres), 0 otherwise
     float pad3;
     double Rd[9], td[3];   // compensated order: the pose in FP64
+    int32_t level_iters[NUIF_ICP_MAX_LEVELS];  // iterations that ran per level this frame
 };
 static_assert(sizeof(IcpState) % 8 == 0, "state is copied as 32-bit words");
 
@@ -220,6 +221,7 @@ __device__ bool ref_accept(IcpState& S, const float* sumsf, int level, int iter,
     S.error = err;
     S.count = cnt;
     S.iterations += 1;
+    S.level_iters[level] += 1;
     if (iter == 0) {  // per-level initialisation (:299-306)
         S.lambda = 1.0f;
         S.err_old = 1e20f;
@@ -306,6 +308,7 @@ __device__ void comp_update(IcpState& S, const double* sums, int level, uint32_t
     S.count = (float)cnt;
     S.error = cnt > 0.0 ? (float)(sqrt(sums[27]) / cnt) : 3.402823466e38f;
     S.iterations += 1;
+    S.level_iters[level] += 1;
     if (cnt < (double)count_threshold) {
         S.brk[level] = 1;
         return;
@@ -682,7 +685,7 @@ __global__ void icp_begin_kernel(const IcpJob* __restrict__ jobs, IcpJob single)
             for (int k = 0; k < 9; ++k) st->R[k] = pose->R[k];
             for (int k = 0; k < 3; ++k) st->t[k] = pose->t[k];
         }
-        for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
+        for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = st->level_iters[l] = 0;
         st->failed = 0;
         st->iterations = 0;
         st->ok = 0;
@@ -769,7 +772,7 @@ __global__ void icp_end_kernel(const IcpJob* __restrict__ jobs, IcpJob single) {
 __global__ void icp_record_kernel(IcpState* st, NuiCLRigidTransform* pose, float* history, int capacity) {
     for (int k = 0; k < 9; ++k) st->R[k] = pose->R[k];
     for (int k = 0; k < 3; ++k) st->t[k] = pose->t[k];
-    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = st->level_iters[l] = 0;
     st->failed = 0;
     st->iterations = 0;
     st->timeout = 0;
@@ -785,7 +788,7 @@ __global__ void icp_state_init_kernel(IcpState* st, const NuiCLRigidTransform* p
     for (int k = 0; k < 9; ++k) st->Rd[k] = (double)pose->R[k];
     for (int k = 0; k < 3; ++k) st->td[k] = (double)pose->t[k];
     for (int k = 0; k < 3; ++k) st->off[k] = 0.f;
-    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = 0;
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) st->brk[l] = st->level_iters[l] = 0;
     st->failed = 0;
     st->iterations = 0;
     st->gate = 1;
@@ -1082,6 +1085,7 @@ int nuif_icp_state_read(const void* state, nuif_icp_result* out, nuif_stream_t s
     for (int k = 0; k < kTerms; ++k) out->sums[k] = h.sums[k];
     for (int k = 0; k < 9; ++k) out->Rd[k] = h.Rd[k];
     for (int k = 0; k < 3; ++k) out->td[k] = h.td[k];
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) out->level_iterations[l] = h.level_iters[l];
     return NUIF_OK;
 }
 

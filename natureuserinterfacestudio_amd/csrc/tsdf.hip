@@ -535,11 +535,11 @@ __global__ void __launch_bounds__(64) integrate_tile_batch_kernel(const IntJob* 
     const int bx = (int)(b % J.gx);
     b /= J.gx;
     const int by = (int)(b % J.gy), bz = (int)(b / J.gy);
-    integrate_tile_body<S, COLOR>(in_global(J.depth), in_global(J.bgra),
-                                  in_global(reinterpret_cast<typename S::word*>(J.vol)), in_global(J.cvol), J.p, J.cam,
-                                  in_global(J.pose), J.maxcw, in_global(J.gate), in_global(J.updated),
-                                  in_global(J.bounds), in_global(J.surf), VoxelWrap{0, 0, 0}, in_global(J.walk), bx, by,
-                                  bz, (int)J.gx, (int)J.gz);
+    // generic pointers here: with device-memory (global_*) accesses the FP16 variant needs
+    // 134 VGPRs instead of 120 (three waves per SIMD instead of four) and runs slower
+    integrate_tile_body<S, COLOR>(J.depth, J.bgra, reinterpret_cast<typename S::word*>(J.vol), J.cvol, J.p, J.cam,
+                                  J.pose, J.maxcw, J.gate, J.updated, J.bounds, J.surf, VoxelWrap{0, 0, 0}, J.walk,
+                                  bx, by, bz, (int)J.gx, (int)J.gz);
 }
 
 // column walk states of all sessions: block -> session by cblock0
@@ -595,11 +595,15 @@ __global__ void integrate_column_kernel(const float* __restrict__ depth, const u
 
 // ---- raycast -------------------------------------------------------------------
 
-template <class S, bool WRAP> struct VolView {
+// COUNT: every voxel read is also marked in the bitset `mark` (the footprint
+// measurement of nuif_tsdf_raycast_footprint; the raycast kernels do not count)
+template <class S, bool WRAP, bool COUNT = false> struct VolView {
     const typename S::word* __restrict__ v;
     int Nx, Ny, Nz;
     VoxelWrap w;
+    uint32_t* mark = nullptr;
     __device__ __forceinline__ float sample(long long idx) const {
+        if (COUNT) atomicOr(&mark[idx >> 5], 1u << (idx & 31));
         const typename S::word w = v[idx];
         return S::weight(w) > 0 ? S::tsdf(w) : kNaN;
     }
@@ -644,7 +648,8 @@ __device__ __forceinline__ void get_voxel_fast(f3 pt, const TsdfParams& p, const
 // In a shifting volume every corner is wrapped on its own; the reference wraps only
 // the first corner and steps to the others by the unwrapped strides, which reads the
 // wrong voxels (or past the end of the volume) where a cell straddles the wrap.
-template <class S, bool WRAP> __device__ float trilinear(f3 pt, const VolView<S, WRAP>& vv, const TsdfParams& p) {
+template <class S, bool WRAP, bool COUNT>
+__device__ float trilinear(f3 pt, const VolView<S, WRAP, COUNT>& vv, const TsdfParams& p) {
     int g[3];
     get_voxel(pt, p, g);
     if (g[0] <= 0 || g[0] >= vv.Nx - 1) return kNaN;
@@ -736,13 +741,13 @@ __device__ __forceinline__ void superblock_span(const int g[3], const int N[3], 
 // superblock flags kept in LDS when the volume has at most this many superblocks
 constexpr int kSuperLds = 4096;
 
-template <class S, bool WRAP>
+template <class S, bool WRAP, bool COUNT = false>
 __device__ __forceinline__ void raycast_body(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol,
                                              const TsdfParams& p, const NuiCLCameraParams& cam,
                                              const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
                                              float* __restrict__ nmap, uchar4* __restrict__ cmap,
                                              const int32_t* __restrict__ enable, const SurfaceBlocks& surf,
-                                             VoxelWrap wr, int bx, int by) {
+                                             VoxelWrap wr, int bx, int by, uint32_t* mark = nullptr) {
     if (enable && *enable == 0) return;
     // the superblock flags, read by every lane at every superblock it enters
     __shared__ __align__(16) uint8_t sflags[kSuperLds];
@@ -763,7 +768,7 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
     if (x >= W || y >= H) return;
     const size_t id = (size_t)y * W + x;
     const f3 NaN3 = mk3(kNaN, kNaN, kNaN);
-    VolView<S, WRAP> vv{vol, (int)p.resolution[0], (int)p.resolution[1], (int)p.resolution[2], wr};
+    VolView<S, WRAP, COUNT> vv{vol, (int)p.resolution[0], (int)p.resolution[1], (int)p.resolution[2], wr, mark};
     const NuiCLRigidTransform g = *pose;
     const f3 off = mk3(p.dimension[0] / 2.0f, p.dimension[1] / 2.0f, 0.f);
     const f3 rs = add3(mk3(g.t[0], g.t[1], g.t[2]), off);
@@ -883,8 +888,8 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
         if (crossing) {
             do {
                 // both interpolations are evaluated (pure reads) so that their loads overlap
-                const float Ftdt = trilinear<S, WRAP>(add3(rs, mul3(dir, tcur + step)), vv, p);
-                const float Ft = trilinear<S, WRAP>(add3(rs, mul3(dir, tcur)), vv, p);
+                const float Ftdt = trilinear<S, WRAP, COUNT>(add3(rs, mul3(dir, tcur + step)), vv, p);
+                const float Ft = trilinear<S, WRAP, COUNT>(add3(rs, mul3(dir, tcur)), vv, p);
                 if (isnan(Ftdt) || isnan(Ft)) break;
                 const float Ts = tcur - step * Ft / (Ftdt - Ft);
                 const f3 vf = add3(rs, mul3(dir, Ts));
@@ -894,14 +899,14 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
                 get_voxel(add3(rs, mul3(dir, tcur)), p, gg);
                 if (gg[0] > 1 && gg[1] > 1 && gg[2] > 1 && gg[0] < vv.Nx - 2 && gg[1] < vv.Ny - 2 && gg[2] < vv.Nz - 2) {
                     f3 n, q;
-                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S, WRAP>(q, vv, p);
-                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S, WRAP>(q, vv, p);
+                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S, WRAP, COUNT>(q, vv, p);
+                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S, WRAP, COUNT>(q, vv, p);
                     n.x = fx1 - fx2;
-                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S, WRAP>(q, vv, p);
-                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S, WRAP>(q, vv, p);
+                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S, WRAP, COUNT>(q, vv, p);
+                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S, WRAP, COUNT>(q, vv, p);
                     n.y = fy1 - fy2;
-                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S, WRAP>(q, vv, p);
-                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S, WRAP>(q, vv, p);
+                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S, WRAP, COUNT>(q, vv, p);
+                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S, WRAP, COUNT>(q, vv, p);
                     n.z = fz1 - fz2;
                     nout = normalize3(n);
                 }
@@ -920,6 +925,24 @@ __global__ void __launch_bounds__(64)
                    float* __restrict__ nmap, uchar4* __restrict__ cmap, const int32_t* __restrict__ enable,
                    SurfaceBlocks surf, VoxelWrap wr) {
     raycast_body<S, WRAP>(vol, cvol, p, cam, pose, vmap, nmap, cmap, enable, surf, wr, blockIdx.x, blockIdx.y);
+}
+
+// the raycast of raycast_kernel (no wrap) marking every voxel it reads
+template <class S>
+__global__ void __launch_bounds__(64)
+    raycast_count_kernel(const typename S::word* __restrict__ vol, TsdfParams p, NuiCLCameraParams cam,
+                         const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
+                         float* __restrict__ nmap, SurfaceBlocks surf, uint32_t* __restrict__ mark) {
+    raycast_body<S, false, true>(vol, nullptr, p, cam, pose, vmap, nmap, nullptr, nullptr, surf, VoxelWrap{0, 0, 0},
+                                 blockIdx.x, blockIdx.y, mark);
+}
+
+__global__ void popcount_kernel(const uint32_t* __restrict__ bits, size_t n, unsigned long long* __restrict__ out) {
+    unsigned long long c = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        c += __popc(bits[i]);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
 }
 
 // all sessions' 8 x 8 pixel tiles of one storage (volumes without wrap)
@@ -995,6 +1018,19 @@ int launch_raycast(const void* volume, const uint8_t* color_volume, const TsdfPa
         raycast_kernel<S, true><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, none, wr);
     }
     NUIF_LAUNCHED("raycast_kernel");
+    return NUIF_OK;
+}
+
+template <class S>
+int launch_raycast_count(const void* volume, const TsdfParams& p, const NuiCLCameraParams& cam,
+                         const NuiCLRigidTransform* pose, float* vmap, float* nmap, const uint8_t* surface,
+                         uint32_t* bits, hipStream_t st) {
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    const SurfaceBlocks surf = surface_blocks_of(const_cast<uint8_t*>(surface), (int)p.resolution[0],
+                                                 (int)p.resolution[1], (int)p.resolution[2]);
+    raycast_count_kernel<S><<<dim3(div_up(W, 8), div_up(H, 8)), 64, 0, st>>>(
+        reinterpret_cast<const typename S::word*>(volume), p, cam, pose, vmap, nmap, surf, bits);
+    NUIF_LAUNCHED("raycast_count_kernel");
     return NUIF_OK;
 }
 
@@ -1240,6 +1276,41 @@ int nuif_tsdf_raycast_wrapped(const void* volume, const uint8_t* color_volume, c
                                                             surface_blocks, wr, st);
         default: set_error("nuif_tsdf_raycast: unknown storage %d", storage); return NUIF_ERR_INVALID;
     }
+}
+
+size_t nuif_tsdf_footprint_bytes(const TsdfParams* params) {
+    if (!params) return 0;
+    const size_t n = (size_t)params->resolution[0] * params->resolution[1] * params->resolution[2];
+    return ((n + 31) / 32 * sizeof(uint32_t) + 255) / 256 * 256 + 256;  // the bitset, then the count
+}
+
+long long nuif_tsdf_raycast_footprint(const void* volume, const TsdfParams* params, const NuiCLCameraParams* cam,
+                                      const NuiCLRigidTransform* pose, int storage, const uint8_t* surface_blocks,
+                                      float* vmap, float* nmap, void* workspace, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    int rc = check_params(params);
+    if (rc) return rc;
+    NUIF_CHECK_ARG(volume && cam && pose && vmap && nmap && workspace, "nuif_tsdf_raycast_footprint: null argument");
+    NUIF_CHECK_ARG(cam->depthImageWidth > 0 && cam->depthImageHeight > 0, "nuif_tsdf_raycast_footprint: empty image");
+    hipStream_t st = stream_of(s);
+    const size_t n = (size_t)params->resolution[0] * params->resolution[1] * params->resolution[2];
+    const size_t words = (n + 31) / 32;
+    uint32_t* bits = static_cast<uint32_t*>(workspace);
+    auto* count = reinterpret_cast<unsigned long long*>(static_cast<char*>(workspace) + (words * 4 + 255) / 256 * 256);
+    NUIF_HIP(hipMemsetAsync(workspace, 0, nuif_tsdf_footprint_bytes(params), st));
+    switch (storage) {
+        case NUIF_TSDF_S16: rc = launch_raycast_count<StS16>(volume, *params, *cam, pose, vmap, nmap, surface_blocks, bits, st); break;
+        case NUIF_TSDF_F16: rc = launch_raycast_count<StF16>(volume, *params, *cam, pose, vmap, nmap, surface_blocks, bits, st); break;
+        case NUIF_TSDF_F32: rc = launch_raycast_count<StF32>(volume, *params, *cam, pose, vmap, nmap, surface_blocks, bits, st); break;
+        default: set_error("nuif_tsdf_raycast_footprint: unknown storage %d", storage); return NUIF_ERR_INVALID;
+    }
+    if (rc) return rc;
+    popcount_kernel<<<(unsigned)std::min<size_t>(div_up(words, 256), 4096), 256, 0, st>>>(bits, words, count);
+    NUIF_LAUNCHED("popcount_kernel");
+    unsigned long long h = 0;
+    NUIF_HIP(hipMemcpyAsync(&h, count, sizeof(h), hipMemcpyDeviceToHost, st));
+    NUIF_HIP(hipStreamSynchronize(st));
+    return (long long)h;
 }
 
 int nuif_tsdf_raycast_gated(const void* volume, const uint8_t* color_volume, const TsdfParams* params,

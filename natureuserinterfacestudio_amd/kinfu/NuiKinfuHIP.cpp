@@ -348,6 +348,18 @@ void NuiKinfuHIPScene::raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiK
           "nuif_tsdf_raycast");
 }
 
+long long NuiKinfuHIPScene::RaycastFootprint(NuiKinfuHIPCameraState* cam) const {
+    if (m_wrap[0] || m_wrap[1] || m_wrap[2]) throw std::runtime_error("raycast footprint: wrapped volume");
+    const NuiCLCameraParams& c = cam->GetCameraParams();
+    const size_t n = (size_t)c.depthImageWidth * c.depthImageHeight;
+    DeviceBuffer ws(nuif_tsdf_footprint_bytes(&m_params)), v(n * 12), nm(n * 12);
+    const long long u = nuif_tsdf_raycast_footprint(m_volume.get(), &m_params, &c, cam->GetCameraTransformBuffer(),
+                                                    m_storage, m_surface.as<uint8_t>(), v.as<float>(), nm.as<float>(),
+                                                    ws.get(), m_stream);
+    if (u < 0) check((int)u, "nuif_tsdf_raycast_footprint");
+    return u;
+}
+
 // ------------------------------------------------------------ shifting volume
 
 NuiKinfuHIPShiftScene::NuiKinfuHIPShiftScene(const NuiKinfuVolumeConfig& config, int storage, nuif_stream_t s)

@@ -247,6 +247,9 @@ public:
     uint8_t* GetColorVolume() const { return m_color.as<uint8_t>(); }
     size_t VolumeBytes() const { return m_volume.bytes(); }
     unsigned long long UpdatedVoxels() const override;
+    // distinct voxels the raycast reads at the camera's current pose (nuif_tsdf_raycast_footprint;
+    // synchronises); not for a wrapped volume
+    long long RaycastFootprint(NuiKinfuHIPCameraState* cam) const;
     // the scene's part of a batched round's entry (volume, surface blocks, walk states);
     // false when the volume cannot be batched (a shifting volume, or dimensions the tile
     // integration does not take)
@@ -414,6 +417,7 @@ public:
     nuif_stream_t stream() const { return m_stream; }
     nuif_kinfu::NuiKinfuHIPFeedbackFrame* feedbackFrame() const { return m_pFeedbackFrame; }
     nuif_kinfu::NuiKinfuHIPDepthTracker* tracker() const { return m_pTracker; }
+    nuif_kinfu::NuiKinfuHIPCameraState* cameraState() const { return m_pCameraState; }
     const float* poseHistoryDevice() const { return m_history.as<float>(); }
     int poseHistoryCapacity() const { return m_historyCap; }
     // Rows of the device pose ring (written by the frame's ICP end, row frame % capacity)

@@ -110,6 +110,17 @@ NuiFusionServer::NuiFusionServer(size_t hbmBudget) : m_budget(hbmBudget) {
     check(nuif_stream_create(&m_ctl), "server stream");
     check(nuif_event_create(&m_start), "server event");
     check(nuif_event_create(&m_batchEnd), "server event");
+    for (nuif_event_t& e : m_stageEv) check(nuif_event_create(&e), "server event");
+}
+
+int NuiFusionServer::takeStageTimes(double ms[5]) {
+    for (int k = 0; k < 5; ++k) {
+        ms[k] = m_stageMs[k];
+        m_stageMs[k] = 0;
+    }
+    const int n = m_stageRounds;
+    m_stageRounds = 0;
+    return n;
 }
 
 NuiFusionServer::~NuiFusionServer() {
@@ -121,6 +132,8 @@ NuiFusionServer::~NuiFusionServer() {
         if (e) nuif_event_destroy(e);
     if (m_start) nuif_event_destroy(m_start);
     if (m_batchEnd) nuif_event_destroy(m_batchEnd);
+    for (nuif_event_t e : m_stageEv)
+        if (e) nuif_event_destroy(e);
     if (m_staging) nuif_host_free(m_staging);
     if (m_status) nuif_host_free(m_status);
     if (m_ctl) nuif_stream_destroy(m_ctl);
@@ -233,12 +246,16 @@ void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depth
         check(nuif_event_record(m_ends[ids[i]], st), "frame end");
     }
     m_lastBatch = (int)batch.size();
+    m_stagePending = false;
     if (batch.empty()) return;
     // the batch follows each session's previous frame; each session's stream follows the
     // batch (its later work and read-backs see the round's results)
     for (int id : batched) check(nuif_stream_wait_event(m_ctl, m_ends[id]), "batch wait");
     ensureTable((int)batch.size());
-    check(nuif_fusion_round(batch.data(), (int)batch.size(), m_table.get(), m_staging, m_ctl), "nuif_fusion_round");
+    check(nuif_fusion_round_timed(batch.data(), (int)batch.size(), m_table.get(), m_staging,
+                                  m_stageTiming ? m_stageEv : nullptr, m_ctl),
+          "nuif_fusion_round");
+    m_stagePending = m_stageTiming;
     check(nuif_event_record(m_batchEnd, m_ctl), "batch end");
     m_batchPending = true;
     for (int id : batched) {
@@ -254,6 +271,16 @@ int NuiFusionServer::wait(float* latencyMs) {
         const int id = m_round[i];
         check(nuif_event_synchronize(m_ends[id]), "frame wait");
         if (latencyMs) check(nuif_event_elapsed_ms(m_start, m_ends[id], &latencyMs[i]), "frame latency");
+    }
+    if (m_stagePending) {
+        check(nuif_event_synchronize(m_stageEv[5]), "stage events");
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0.f;
+            check(nuif_event_elapsed_ms(m_stageEv[k], m_stageEv[k + 1], &ms), "stage time");
+            m_stageMs[k] += ms;
+        }
+        m_stageRounds++;
+        m_stagePending = false;
     }
     // outcomes: the batched frames' from their slots, the others' from their session's state
     // (the first frame of a session records its pose and always tracks)

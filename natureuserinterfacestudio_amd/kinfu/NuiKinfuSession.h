@@ -65,6 +65,11 @@ public:
     void setFailurePolicy(int policy);
     // outcome of each frame of the last waited round: 1 tracked, 0 failed
     const std::vector<int32_t>& lastOutcomes() const { return m_outcomes; }
+    // device time of the batched round's stages (preprocess + pyramid, depth bounds, ICP,
+    // integration, raycast), event-timed on the server stream
+    void setStageTiming(bool on) { m_stageTiming = on; }
+    // per-stage sums (ms) over the batched rounds timed since the last call; returns their number
+    int takeStageTimes(double ms[5]);
     // batched rounds on (default) or off (every frame on its session's stream)
     void setBatching(bool on) { m_batching = on; }
     // frames of the last round that ran in the batch
@@ -92,6 +97,11 @@ private:
     std::vector<int> m_slot;           // per frame of the round: its outcome slot (-1: not batched)
     std::vector<int32_t> m_outcomes;
     int m_policy = NUIFS_ON_FAILURE_KEEP;
+    bool m_stageTiming = false;
+    bool m_stagePending = false;       // the last batched round recorded the stage events
+    nuif_event_t m_stageEv[6] = {};
+    double m_stageMs[5] = {0, 0, 0, 0, 0};
+    int m_stageRounds = 0;
 };
 
 }  // namespace nuif_kinfu

@@ -145,6 +145,17 @@ int nuifk_hash_stored_blocks(void* h, size_t* out) {
     });
 }
 
+int nuifk_raycast_footprint(void* h, unsigned long long* out) {
+    return guarded([&] {
+        Session* s = static_cast<Session*>(h);
+        auto* ds = dynamic_cast<NuiKinfuHIPScene*>(s->scene.get());
+        if (!ds || !out) throw std::invalid_argument("nuifk_raycast_footprint: not a dense scene");
+        s->engine->synchronize();
+        *out = (unsigned long long)ds->RaycastFootprint(s->engine->cameraState());
+        return 0;
+    });
+}
+
 int nuifk_scene_failures(void* h, unsigned long long* out) {
     return guarded([&] {
         if (!out) throw std::invalid_argument("nuifk_scene_failures: null output");
@@ -319,6 +330,21 @@ int nuifs_wait(void* srv, float* latency_ms) {
     return guarded([&] {
         if (!srv) throw std::runtime_error("nuifs_wait: null server");
         return static_cast<NuiFusionServer*>(srv)->wait(latency_ms);
+    });
+}
+
+int nuifs_set_stage_timing(void* srv, int on) {
+    return guarded([&] {
+        if (!srv) throw std::runtime_error("nuifs_set_stage_timing: null server");
+        static_cast<NuiFusionServer*>(srv)->setStageTiming(on != 0);
+        return 0;
+    });
+}
+
+int nuifs_take_stage_times(void* srv, double* ms5) {
+    return guarded([&] {
+        if (!srv || !ms5) throw std::runtime_error("nuifs_take_stage_times: null argument");
+        return static_cast<NuiFusionServer*>(srv)->takeStageTimes(ms5);
     });
 }
 

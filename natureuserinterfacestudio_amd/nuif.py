@@ -55,7 +55,8 @@ ON_FAILURE_KEEP, ON_FAILURE_RESET, ON_FAILURE_EVICT = 0, 1, 2
 class IcpResult(C.Structure):
     _fields_ = [("R", c_float * 9), ("t", c_float * 3), ("error", c_float), ("count", c_float),
                 ("status", C.c_int32), ("iterations", C.c_int32), ("integrate", C.c_int32), ("frame", C.c_int32),
-                ("sums", C.c_double * ICP_TERMS), ("Rd", C.c_double * 9), ("td", C.c_double * 3)]
+                ("sums", C.c_double * ICP_TERMS), ("Rd", C.c_double * 9), ("td", C.c_double * 3),
+                ("level_iterations", C.c_int32 * ICP_MAX_LEVELS)]
 
 
 class IcpConfig(C.Structure):
@@ -147,6 +148,10 @@ _SIGS = {
     "nuif_icp_estimate_static": (c_int, [c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                          c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
     "nuif_icp_estimate_batch": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "nuif_fusion_round_timed": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nuif_tsdf_footprint_bytes": (c_size_t, [c_void_p]),
+    "nuif_tsdf_raycast_footprint": (C.c_longlong, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                                    c_void_p, c_void_p, c_void_p]),
     "nuif_icp_estimate": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                   c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
     "nuif_icp_gate_flag": (c_void_p, [c_void_p]),
@@ -288,6 +293,9 @@ _KSIGS = {
     "nuifs_submit": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p)]),
     "nuifs_wait": (c_int, [c_void_p, C.POINTER(c_float)]),
     "nuifs_set_failure_policy": (c_int, [c_void_p, c_int]),
+    "nuifs_set_stage_timing": (c_int, [c_void_p, c_int]),
+    "nuifs_take_stage_times": (c_int, [c_void_p, c_void_p]),
+    "nuifk_raycast_footprint": (c_int, [c_void_p, c_void_p]),
     "nuifs_last_outcomes": (c_int, [c_void_p, c_void_p, c_int]),
     "nuifk_scene_failures": (c_int, [c_void_p, c_void_p]),
     "nuifs_set_batching": (c_int, [c_void_p, c_int]),
@@ -575,6 +583,12 @@ class Engine:
         self._check(K().nuifk_shift_cache(self._h, p.ctypes.data, c.ctypes.data, n), "shift cache")
         return p[:n], c[:n]
 
+    def raycast_footprint(self):
+        """Distinct voxels the dense raycast reads at the current pose (U of B*U + 28*W*H)."""
+        out = C.c_ulonglong()
+        self._check(K().nuifk_raycast_footprint(self._h, C.byref(out)), "raycast footprint")
+        return out.value
+
     def scene_failures(self):
         """Hash scene: insertions dropped for want of a brick or a bucket slot (0 if dense)."""
         out = C.c_ulonglong()
@@ -717,6 +731,15 @@ class Server:
                 if not ok and sid in self.sessions:
                     self.sessions.pop(sid)._h = None
         return np.array(lat[:self._n], np.float64)
+
+    def set_stage_timing(self, on):
+        self._check(K().nuifs_set_stage_timing(self._h, 1 if on else 0), "nuifs_set_stage_timing")
+
+    def take_stage_times(self):
+        """(rounds, [preprocess, bounds, icp, integrate, raycast] summed ms) of the batched rounds."""
+        ms = (C.c_double * 5)()
+        n = self._check(K().nuifs_take_stage_times(self._h, ms), "nuifs_take_stage_times")
+        return n, list(ms)
 
     def set_failure_policy(self, policy):
         """ON_FAILURE_KEEP / ON_FAILURE_RESET / ON_FAILURE_EVICT."""

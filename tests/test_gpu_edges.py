@@ -128,3 +128,20 @@ def test_pose_history_ring_wraps(gpu):
     np.testing.assert_array_equal(a, b)
     with pytest.raises(nuif.NuifError):
         small.set_history_capacity(0)
+
+
+def test_raycast_footprint_counts_the_voxels_read(gpu):
+    announce("raycast footprint, 320x240 128^3 after 3 frames")
+    wl = W.SMALL
+    eng = nuif.Engine(wl.kinfu_desc())
+    for f in range(3):
+        eng.run(_dev(wl.render(f)[0]), sync=True)
+    u1 = eng.raycast_footprint()
+    u2 = eng.raycast_footprint()
+    vg, _ = eng.maps()
+    hits = int((~np.isnan(vg[:, 2])).sum())
+    assert u1 == u2  # the same march at the same pose
+    # every hit reads the 8 corners of at least 2 trilinear cells; the march reads at
+    # most the whole volume
+    assert hits > 10000 and 8 <= u1 < 128 ** 3
+    assert u1 > hits // 8
