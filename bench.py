@@ -12,6 +12,7 @@ Prints one JSON line with the metric, the roofline of the dominant kernel
 bounded sample of the same workload.
 """
 import argparse
+import ctypes as C
 import dataclasses
 import json
 import os
@@ -339,6 +340,9 @@ def main():
         print(f"[bench] CPU baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
         cpu = cpu_baseline(wl, args.cpu_frames)
 
+    sch = (C.c_int32 * 4)()
+    nuif.check(nuif.L().nuif_schedule(sch), "schedule")
+    schedule = dict(zip(["ray_batch", "walk_batch", "rmw_batch", "seg_voxels"], list(sch)))
     if server and server.get("stages"):
         # the roofline of the round's dominant stage (the server is the headline value)
         roofs = server["stages"]["rooflines"]
@@ -359,6 +363,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload, "resolution": [wl.width, wl.height], "volume": [wl.volume_res] * 3,
                    "volume_m": wl.volume_dim, "sessions": server["sessions"] if server else 1},
+        "schedule": schedule,
         "p50_frame_latency_ms": round(server["p50_ms"], 3) if server else None,
         "p99_frame_latency_ms": round(server["p99_ms"], 3) if server else None,
         "server": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in server.items()} if server else None,

@@ -303,6 +303,12 @@ int nuif_tsdf_raycast(const void* volume, const uint8_t* color_volume, const Tsd
  * read marked in a bitset; vmap / nmap receive the raycast.  workspace:
  * nuif_tsdf_footprint_bytes(params) of device memory.  Synchronises s; returns U. */
 size_t nuif_tsdf_footprint_bytes(const TsdfParams* params);
+/* The launch schedule of the library: raycast samples in flight, integration walk steps in
+ * flight, integration column chunks in flight (compiled in), and the integration z-segment
+ * voxels of a single-session launch; nuif_schedule_segment gives the segment of a launch
+ * shared by that many sessions (a batched round). */
+int nuif_schedule(int32_t* out4);
+int nuif_schedule_segment(int sessions);
 long long nuif_tsdf_raycast_footprint(const void* volume, const TsdfParams* params, const NuiCLCameraParams* cam,
                                       const NuiCLRigidTransform* pose, int storage, const uint8_t* surface_blocks,
                                       float* vmap, float* nmap, void* workspace, nuif_stream_t s);

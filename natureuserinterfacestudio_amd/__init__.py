@@ -8,7 +8,8 @@ import ctypes
 import os
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.join(_PKG, "lib")
+# NUIF_LIB_DIR selects another build of the three libraries (the schedule tuner's variants)
+LIB_DIR = os.environ.get("NUIF_LIB_DIR") or os.path.join(_PKG, "lib")
 
 _libs = {}
 

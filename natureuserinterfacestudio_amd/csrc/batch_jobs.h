@@ -81,6 +81,7 @@ struct IntJob {
     const float* bounds;
     SurfaceBlocks surf;
     float2* walk;     // walk states (NULL: one z segment)
+    int seg;          // z-segment voxels (segment_voxels)
     unsigned gx, gy, gz;
     unsigned block0;  // tile kernel blocks
     unsigned cblock0; // column_states blocks

@@ -11,6 +11,7 @@
 // table from pinned host memory.
 
 #include "batch_jobs.h"
+#include "schedule.h"
 #include "nuif_internal.h"
 
 #include <algorithm>
@@ -22,7 +23,6 @@ namespace {
 
 constexpr int kLevels = NUIF_ICP_MAX_LEVELS;
 constexpr int kTileW = 32, kTileH = 8;  // preprocess_l0 tiles (preprocess.hip)
-constexpr int kSegVoxels = 128;         // integration z segments (tsdf.hip)
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
@@ -66,7 +66,7 @@ int check_session(const nuif_round_session& r, int i) {
                    "nuif_fusion_round: session %d: volume %d x %d x %d (needs multiples of 8, 8, 32)", i, Nx, Ny, Nz);
     NUIF_CHECK_ARG(r.storage == NUIF_TSDF_S16 || r.storage == NUIF_TSDF_F16 || r.storage == NUIF_TSDF_F32,
                    "nuif_fusion_round: session %d: unknown storage %d", i, r.storage);
-    NUIF_CHECK_ARG(Nz <= kSegVoxels || r.walk_states, "nuif_fusion_round: session %d needs walk states", i);
+    NUIF_CHECK_ARG(Nz <= kMinSegVoxels || r.walk_states, "nuif_fusion_round: session %d needs walk states", i);
     return NUIF_OK;
 }
 
@@ -162,6 +162,7 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
     int rc = make_icp_jobs(icp.data(), n, h + lay.icp, &nref);
     if (rc) return rc;
     // integration and raycast, grouped by storage (and colour for the integration)
+    const int seg = segment_voxels(n);  // integration z segment for a launch of n sessions
     std::vector<int> order((size_t)n);
     for (int i = 0; i < n; ++i) order[i] = i;
     auto color_of = [&](int i) { return S[i].color_volume && S[i].bgra; };
@@ -206,7 +207,8 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
         J.surf = surface_blocks_of(r.surface_blocks, Nx, Ny, Nz);
         J.gx = (unsigned)Nx / 8;
         J.gy = (unsigned)Ny / 8;
-        J.gz = div_up(Nz, kSegVoxels);
+        J.seg = seg;
+        J.gz = div_up(Nz, seg);
         J.walk = J.gz > 1 ? reinterpret_cast<float2*>(r.walk_states) : nullptr;
         J.block0 = ib;
         J.cblock0 = icb;

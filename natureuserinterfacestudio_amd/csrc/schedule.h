@@ -1,0 +1,48 @@
+// Launch schedule of the dense integration and raycast.  The values are the ones
+// natureuserinterfacestudio_amd/tune_schedules.py measured best on MI355X at
+// 640x480 / 512^3 (server round and single session); a build may override the
+// compile-time ones (-DNUIF_RAY_BATCH=... etc.) and nuif_schedule() reports them.
+#pragma once
+
+#ifndef NUIF_RAY_BATCH
+#define NUIF_RAY_BATCH 8     // raycast samples whose loads are in flight together
+#endif
+#ifndef NUIF_WALK_BATCH
+#define NUIF_WALK_BATCH 8    // integration walk steps whose depth reads are in flight together
+#endif
+#ifndef NUIF_RMW_BATCH
+#define NUIF_RMW_BATCH 2     // integration column chunks whose volume loads are in flight together
+#endif
+#include <cstdlib>
+
+namespace nuif {
+constexpr int kRayBatch = NUIF_RAY_BATCH;
+constexpr int kWalkBatch = NUIF_WALK_BATCH;
+constexpr int kRmwBatch = NUIF_RMW_BATCH;
+
+// Columns are split into z segments walked by separate waves: tiles that look through
+// long runs of free space would otherwise set the kernel's duration alone.  The length
+// is chosen per launch by the number of sessions sharing it: one session needs the
+// parallelism of short segments, a batched round has waves enough and gains from fewer
+// walk states.  Rules: from `min_sessions` sessions on, `voxels` per segment.  The
+// environment variable NUIF_SEG_VOXELS forces a length (the tuner's sweep).
+struct SegmentRule {
+    int min_sessions, voxels;
+};
+// measured at 640x480 / 512^3: 1 session 64, 8 sessions 128, 32 sessions 256
+constexpr SegmentRule kSegmentRules[] = {{1, 64}, {2, 128}, {16, 256}};
+constexpr int kMinSegVoxels = 64;  // walk-state buffers are sized for this length
+
+inline int segment_voxels(int sessions) {
+    static const int forced = [] {
+        const char* e = std::getenv("NUIF_SEG_VOXELS");
+        const int v = e ? std::atoi(e) : 0;
+        return (v >= kMinSegVoxels && v % 32 == 0) ? v : 0;
+    }();
+    if (forced) return forced;
+    int v = kSegmentRules[0].voxels;
+    for (const SegmentRule& r : kSegmentRules)
+        if (sessions >= r.min_sessions) v = r.voxels;
+    return v;
+}
+}  // namespace nuif

@@ -25,6 +25,7 @@
 
 #include "batch_jobs.h"
 #include "dense_volume.h"
+#include "schedule.h"
 
 namespace nuif {
 
@@ -89,11 +90,7 @@ __global__ void reset_slice_kernel(typename S::word* __restrict__ vol, uint32_t*
 // Column chunk of the integration: 32 voxels keep the LDS transpose buffer at 8.3 KB
 // per 64-column wave (5 waves per SIMD; 64 voxels allowed 3 and ran 20 % slower).
 constexpr int kZChunk = 32;
-// Columns are split into z segments walked by separate waves: tiles that look through
-// long runs of free space would otherwise set the kernel's duration alone.
-constexpr int kSegVoxels = 128;
-static_assert(kSegVoxels % kZChunk == 0, "segments hold whole chunks");
-constexpr int kRayBatch = 8;  // raycast samples whose loads are in flight together
+static_assert(kMinSegVoxels % kZChunk == 0, "segments hold whole chunks");
 constexpr int kLdsPitch = kZChunk + 1;
 
 struct ColumnWalk {  // the reference's per-column incremental state
@@ -320,7 +317,7 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
 __device__ __forceinline__ void column_states_body(const TsdfParams& p, const NuiCLCameraParams& cam,
                                                    const NuiCLRigidTransform* __restrict__ pose,
                                                    const int32_t* __restrict__ gate, float2* __restrict__ walk_states,
-                                                   int b) {
+                                                   int b, int seg) {
     if (gate && *gate == 0) return;
     const int Nx = (int)p.resolution[0], Ny = (int)p.resolution[1], Nz = (int)p.resolution[2];
     const int i = b * blockDim.x + threadIdx.x;
@@ -330,11 +327,11 @@ __device__ __forceinline__ void column_states_body(const TsdfParams& p, const Nu
     const NuiCLRigidTransform g = *pose;
     const ColumnWalk c0 = column_setup(x, y, Nx, Ny, p, cam, g);
     const float cz = p.cell_size[2];
-    const int nseg = (Nz + kSegVoxels - 1) / kSegVoxels;
+    const int nseg = (Nz + seg - 1) / seg;
     float vx = c0.vx, vy = c0.vy;
     for (int k = 1; k < nseg; ++k) {
 #pragma unroll 16
-        for (int j = 0; j < kSegVoxels; ++j) {
+        for (int j = 0; j < seg; ++j) {
             vx += c0.incx;
             vy += c0.incy;
         }
@@ -344,7 +341,7 @@ __device__ __forceinline__ void column_states_body(const TsdfParams& p, const Nu
         float vgz = c0.vgz, zs = c0.zs;
         for (int k = 1; k < nseg; ++k) {
 #pragma unroll 16
-            for (int j = 0; j < kSegVoxels; ++j) {
+            for (int j = 0; j < seg; ++j) {
                 vgz += cz;
                 zs += cz;
             }
@@ -355,8 +352,8 @@ __device__ __forceinline__ void column_states_body(const TsdfParams& p, const Nu
 
 __global__ void __launch_bounds__(256)
     column_states_kernel(TsdfParams p, NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose,
-                         const int32_t* __restrict__ gate, float2* __restrict__ walk_states) {
-    column_states_body(p, cam, pose, gate, walk_states, blockIdx.x);
+                         const int32_t* __restrict__ gate, float2* __restrict__ walk_states, int seg) {
+    column_states_body(p, cam, pose, gate, walk_states, blockIdx.x, seg);
 }
 
 // Fast path: Nx, Ny multiples of 8, Nz a multiple of the column chunk.
@@ -369,15 +366,15 @@ __device__ __forceinline__ void integrate_tile_body(const float* __restrict__ de
                                                     unsigned long long* __restrict__ updated,
                                                     const float* __restrict__ bounds, const SurfaceBlocks& surf,
                                                     VoxelWrap wr, const float2* __restrict__ walk_states, int bx,
-                                                    int by, int bz, int gdx, int gdz) {
+                                                    int by, int bz, int gdx, int gdz, int seg) {
     if (gate && *gate == 0) return;
     using word = typename S::word;
     constexpr int VPL = 16 / S::kBytes;      // voxels per lane per 16-byte vector
     constexpr int LPC = kZChunk / VPL;        // lanes covering one column chunk
     constexpr int CPI = 64 / LPC;             // columns per vector instruction
     constexpr int NIT = 64 / CPI;             // instructions per chunk
-    constexpr int BATCH = 2;  // column chunks of phase 2 whose loads are in flight together
-    constexpr int KB = 8;     // walk steps whose depth reads are in flight together
+    constexpr int BATCH = kRmwBatch;  // column chunks of phase 2 whose loads are in flight together
+    constexpr int KB = kWalkBatch;    // walk steps whose depth reads are in flight together
     static_assert(kZChunk % KB == 0, "walk batches tile the chunk");
     static_assert(NIT % BATCH == 0 && kZChunk % VPL == 0, "phase 2 covers the 64 columns exactly");
     __shared__ float ts[64 * kLdsPitch];
@@ -395,7 +392,7 @@ __device__ __forceinline__ void integrate_tile_body(const float* __restrict__ de
     // z segment of this wave (bz): the walk state at its start is the one the
     // same sequence of additions reaches, read from the per-column states of
     // column_states_kernel or replayed here
-    const int zbeg = bz * kSegVoxels, zend = min(Nz, zbeg + kSegVoxels);
+    const int zbeg = bz * seg, zend = min(Nz, zbeg + seg);
     if (zbeg > 0 && walk_states) {
         const float2 w = walk_states[(size_t)(bz - 1) * Nx * Ny + ((size_t)by * gdx + bx) * 64 + lane];
         const float2 z = walk_states[(size_t)(gdz - 1) * Nx * Ny + bz - 1];
@@ -522,9 +519,9 @@ __global__ void __launch_bounds__(64)
                           NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, int maxcw,
                           const int32_t* __restrict__ gate, unsigned long long* __restrict__ updated,
                           const float* __restrict__ bounds, SurfaceBlocks surf, VoxelWrap wr,
-                          const float2* __restrict__ walk_states) {
+                          const float2* __restrict__ walk_states, int seg) {
     integrate_tile_body<S, COLOR>(depth, bgra, vol, cvol, p, cam, pose, maxcw, gate, updated, bounds, surf, wr,
-                                  walk_states, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.z);
+                                  walk_states, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.z, seg);
 }
 
 // all sessions' tiles of one storage: block -> (session, tile x, tile y, z segment)
@@ -539,7 +536,7 @@ __global__ void __launch_bounds__(64) integrate_tile_batch_kernel(const IntJob* 
     // 134 VGPRs instead of 120 (three waves per SIMD instead of four) and runs slower
     integrate_tile_body<S, COLOR>(J.depth, J.bgra, reinterpret_cast<typename S::word*>(J.vol), J.cvol, J.p, J.cam,
                                   J.pose, J.maxcw, J.gate, J.updated, J.bounds, J.surf, VoxelWrap{0, 0, 0}, J.walk,
-                                  bx, by, bz, (int)J.gx, (int)J.gz);
+                                  bx, by, bz, (int)J.gx, (int)J.gz, J.seg);
 }
 
 // column walk states of all sessions: block -> session by cblock0
@@ -555,7 +552,7 @@ __global__ void __launch_bounds__(256) column_states_batch_kernel(const IntJob* 
     const IntJob& J = jobs[lo];
     if (!J.walk) return;
     column_states_body(J.p, J.cam, in_global(J.pose), in_global(J.gate), in_global(J.walk),
-                       (int)(blockIdx.x - J.cblock0));
+                       (int)(blockIdx.x - J.cblock0), J.seg);
 }
 
 // Generic path (any resolution): the reference's one-column-per-thread walk.
@@ -972,18 +969,19 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
     // the tile kernel's 16-byte vectors stay aligned and unsplit when w.z is a multiple of
     // the voxels per vector
     if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0 && wr.z % (16 / S::kBytes) == 0) {
-        dim3 grid(Nx / 8, Ny / 8, div_up(Nz, kSegVoxels));
+        const int seg = segment_voxels(1);
+        dim3 grid(Nx / 8, Ny / 8, div_up(Nz, seg));
         float2* ws = grid.z > 1 ? reinterpret_cast<float2*>(walk_states) : nullptr;
         if (ws) {
-            column_states_kernel<<<div_up(Nx * Ny, 256), 256, 0, st>>>(p, cam, pose, gate, ws);
+            column_states_kernel<<<div_up(Nx * Ny, 256), 256, 0, st>>>(p, cam, pose, gate, ws, seg);
             NUIF_LAUNCHED("column_states_kernel");
         }
         if (color)
             integrate_tile_kernel<S, true><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                 bounds, surf, wr, ws);
+                                                                 bounds, surf, wr, ws, seg);
         else
             integrate_tile_kernel<S, false><<<grid, 64, 0, st>>>(depth, bg, vol, cv, p, cam, pose, maxcw, gate, updated,
-                                                                  bounds, surf, wr, ws);
+                                                                  bounds, surf, wr, ws, seg);
         NUIF_LAUNCHED("integrate_tile_kernel");
     } else {
         dim3 block(16, 16), grid(div_up(Nx, 16), div_up(Ny, 16));
@@ -1161,7 +1159,7 @@ static VoxelWrap wrap_of(const int32_t* w, const TsdfParams& p) {
 
 size_t nuif_tsdf_walk_states_bytes(const TsdfParams* params) {
     if (!params || check_params(params)) return 0;
-    const size_t nseg = (size_t)div_up((int)params->resolution[2], kSegVoxels);
+    const size_t nseg = (size_t)div_up((int)params->resolution[2], kMinSegVoxels);
     return nseg > 1 ? (nseg - 1) * ((size_t)params->resolution[0] * (size_t)params->resolution[1] + 1) * sizeof(float2)
                     : 16;
 }
@@ -1276,6 +1274,17 @@ int nuif_tsdf_raycast_wrapped(const void* volume, const uint8_t* color_volume, c
                                                             surface_blocks, wr, st);
         default: set_error("nuif_tsdf_raycast: unknown storage %d", storage); return NUIF_ERR_INVALID;
     }
+}
+
+int nuif_schedule_segment(int sessions) { return sessions > 0 ? segment_voxels(sessions) : NUIF_ERR_INVALID; }
+
+int nuif_schedule(int32_t* out4) {
+    NUIF_CHECK_ARG(out4, "nuif_schedule: null argument");
+    out4[0] = kRayBatch;
+    out4[1] = kWalkBatch;
+    out4[2] = kRmwBatch;
+    out4[3] = segment_voxels(1);
+    return NUIF_OK;
 }
 
 size_t nuif_tsdf_footprint_bytes(const TsdfParams* params) {

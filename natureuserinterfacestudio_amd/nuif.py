@@ -150,6 +150,8 @@ _SIGS = {
     "nuif_icp_estimate_batch": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "nuif_fusion_round_timed": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nuif_tsdf_footprint_bytes": (c_size_t, [c_void_p]),
+    "nuif_schedule": (c_int, [c_void_p]),
+    "nuif_schedule_segment": (c_int, [c_int]),
     "nuif_tsdf_raycast_footprint": (C.c_longlong, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                                     c_void_p, c_void_p, c_void_p]),
     "nuif_icp_estimate": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,

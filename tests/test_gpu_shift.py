@@ -218,7 +218,7 @@ def test_walk_state_workspace_matches_the_replayed_walk(gpu, N, storage, wrap):
     cam = wl.camera_params()
     w = np.array(wrap, np.int32)
     ws_bytes = L.nuif_tsdf_walk_states_bytes(C.byref(p))
-    assert ws_bytes == ((N + 127) // 128 - 1) * (N * N + 1) * 8
+    assert ws_bytes == ((N + 63) // 64 - 1) * (N * N + 1) * 8  # sized for 64-voxel segments
     ws = nuif.DeviceArray((ws_bytes,), np.uint8)
     vb = N ** 3 * L.nuif_tsdf_voxel_bytes(storage)
     color = storage == 0

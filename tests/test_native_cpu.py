@@ -78,16 +78,31 @@ def test_voxel_bytes_and_params():
     assert lib.nuif_tsdf_params(0, 1, 1, 1.0, 1.0, 1.0, 0.03, C.byref(p)) == -1
 
 
-@pytest.mark.parametrize("shape,expect", [((128, 128, 128), 16), ((512, 512, 512), 3 * (512 * 512 + 1) * 8),
-                                          ((64, 96, 320), 2 * (64 * 96 + 1) * 8), ((8, 8, 129), (64 + 1) * 8)])
+@pytest.mark.parametrize("shape,expect", [((128, 128, 64), 16), ((128, 128, 128), (128 * 128 + 1) * 8),
+                                          ((512, 512, 512), 7 * (512 * 512 + 1) * 8),
+                                          ((64, 96, 320), 4 * (64 * 96 + 1) * 8), ((8, 8, 129), 2 * (64 + 1) * 8)])
 def test_walk_state_workspace_bytes(shape, expect):
-    """8 B per column and z segment of 128 voxels after the first, plus 8 B per such
-    segment; a single segment needs none (16 B placeholder)."""
+    """8 B per column and z segment after the first, plus 8 B per such segment, for the
+    shortest segment a launch may use (64 voxels); a single segment needs none (16 B
+    placeholder)."""
     lib = nuif.L()
     p = nuif.TsdfParams()
     assert lib.nuif_tsdf_params(*shape, 1.0, 1.0, 1.0, 0.03, C.byref(p)) == 0
     assert lib.nuif_tsdf_walk_states_bytes(C.byref(p)) == expect
     assert lib.nuif_tsdf_walk_states_bytes(None) == 0
+
+
+def test_schedule_table():
+    """The compiled launch schedule and the segment rule (short segments for a single
+    session, long ones for a batched round), as the schedule sweep chose them."""
+    lib = nuif.L()
+    out = (C.c_int32 * 4)()
+    assert lib.nuif_schedule(out) == 0
+    assert list(out) == [8, 8, 2, 64]
+    assert [lib.nuif_schedule_segment(n) for n in (1, 2, 8, 15, 16, 32)] == [64, 128, 128, 128, 256, 256]
+    assert lib.nuif_schedule_segment(0) == -1
+    from natureuserinterfacestudio_amd import tune_schedules as T
+    assert T.CANDIDATES["default"] == {} and set(T.SEGMENTS) == {64, 128, 256}
 
 
 def test_synthetic_frames_are_deterministic():
