@@ -8,12 +8,11 @@
 //   count   one wavefront per (y, x) cube column, 64 z per step: vertex count
 //   scan    column offsets in (y, x) order
 //   emit    triangle corners in traversal order: 64-bit edge key + position
-//   dedup   stable radix sort of (key, corner index); each corner learns the
-//           first corner with its key; first uses are numbered by a scan in
-//           corner order, which is the reference's first-use order
+//   dedup   stable radix sort of (key, corner index) (scan.hip); runs of equal
+//           keys are numbered by a scan of their heads, and each corner learns the
+//           first corner of its run; first uses are numbered by a scan in corner
+//           order, which is the reference's first-use order
 // Points and indices equal the host mesher's bit for bit.
-
-#include <hipcub/hipcub.hpp>
 
 #include "mc_tables.h"
 #include "nuif_internal.h"
@@ -141,17 +140,25 @@ __global__ void mesh_iota_kernel(uint32_t* __restrict__ v, uint32_t n) {
     if (i < n) v[i] = i;
 }
 
-// seg[j] = j at the first position of each run of equal sorted keys, 0 elsewhere
-__global__ void mesh_heads_kernel(const unsigned long long* __restrict__ sk, uint32_t* __restrict__ seg, uint32_t n) {
+// head[j] = 1 at the first position of each run of equal sorted keys
+__global__ void mesh_heads_kernel(const unsigned long long* __restrict__ sk, uint32_t* __restrict__ head, uint32_t n) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) seg[j] = (j == 0 || sk[j] != sk[j - 1]) ? j : 0u;
+    if (j < n) head[j] = (j == 0 || sk[j] != sk[j - 1]) ? 1u : 0u;
 }
 
-// the sort is stable, so the run's first element is the key's first corner
-__global__ void mesh_first_kernel(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ segstart,
+// run r = excl[j] (heads before j) starts at a head; the sort is stable, so the run's
+// first element is the key's first corner
+__global__ void mesh_runfirst_kernel(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ head,
+                                     const uint32_t* __restrict__ excl, uint32_t* __restrict__ runfirst, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n && head[j]) runfirst[excl[j]] = sv[j];
+}
+
+__global__ void mesh_first_kernel(const uint32_t* __restrict__ sv, const uint32_t* __restrict__ head,
+                                  const uint32_t* __restrict__ excl, const uint32_t* __restrict__ runfirst,
                                   uint32_t* __restrict__ first, uint32_t n) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) first[sv[j]] = sv[segstart[j]];
+    if (j < n) first[sv[j]] = runfirst[excl[j] + head[j] - 1u];
 }
 
 __global__ void mesh_isfirst_kernel(const uint32_t* __restrict__ first, uint32_t* __restrict__ flag, uint32_t n) {
@@ -174,22 +181,14 @@ struct MeshScratch {
     uint32_t* counts;
     void* scan;
     unsigned long long *keys, *keys2;
-    uint32_t *vals, *vals2, *seg, *first, *flag, *vid, *npoints;
+    uint32_t *vals, *vals2, *head, *excl, *runfirst, *first, *flag, *vid, *npoints;
     float* pos;
-    void* cub;
-    size_t cub_bytes, total;
+    void* sort;
+    void* scan2;
+    size_t total;
 };
 
 size_t a256(size_t v) { return (v + 255) / 256 * 256; }
-
-size_t cub_temp_bytes(uint32_t n) {
-    size_t sort = 0, maxscan = 0, sum = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, sort, (unsigned long long*)nullptr, (unsigned long long*)nullptr,
-                                       (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 64);
-    hipcub::DeviceScan::InclusiveScan(nullptr, maxscan, (uint32_t*)nullptr, (uint32_t*)nullptr, hipcub::Max(), (int)n);
-    hipcub::DeviceScan::ExclusiveSum(nullptr, sum, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n);
-    return std::max(sort, std::max(maxscan, sum));
-}
 
 MeshScratch mesh_scratch(void* base, const TsdfParams& p, uint32_t max_vertices) {
     const size_t ncols = (size_t)((int)p.resolution[0] - 1) * ((int)p.resolution[1] - 1);
@@ -208,14 +207,16 @@ MeshScratch mesh_scratch(void* base, const TsdfParams& p, uint32_t max_vertices)
     s.keys2 = (unsigned long long*)take(m * 8);
     s.vals = (uint32_t*)take(m * 4);
     s.vals2 = (uint32_t*)take(m * 4);
-    s.seg = (uint32_t*)take(m * 4);
+    s.head = (uint32_t*)take(m * 4);
+    s.excl = (uint32_t*)take(m * 4);
+    s.runfirst = (uint32_t*)take(m * 4);
     s.first = (uint32_t*)take(m * 4);
     s.flag = (uint32_t*)take(m * 4);
     s.vid = (uint32_t*)take(m * 4);
     s.npoints = (uint32_t*)take(256);
     s.pos = (float*)take(m * 12);
-    s.cub_bytes = cub_temp_bytes((uint32_t)m);
-    s.cub = take(s.cub_bytes);
+    s.sort = take(sort_scratch_bytes((uint32_t)m));
+    s.scan2 = take(nuif_scan_scratch_bytes(m));
     s.total = o;
     return s;
 }
@@ -271,19 +272,26 @@ int nuif_mesh_dense(const void* volume, const TsdfParams* p, int storage, uint32
     const unsigned g = div_up(n, 256);
     mesh_iota_kernel<<<g, 256, 0, st>>>(sc.vals, n);
     NUIF_LAUNCHED("mesh_iota_kernel");
-    size_t tb = sc.cub_bytes;
-    NUIF_HIP(hipcub::DeviceRadixSort::SortPairs(sc.cub, tb, sc.keys, sc.keys2, sc.vals, sc.vals2, (int)n, 0, 64, st));
-    mesh_heads_kernel<<<g, 256, 0, st>>>(sc.keys2, sc.seg, n);
+    // keys are (voxel a) << 32 | (voxel b) with voxel ids below Nx * Ny * Nz
+    int vbits = 1;
+    while (vbits < 32 && ((size_t)1 << vbits) < (size_t)Nx * Ny * Nz) ++vbits;
+    bool alt = false;
+    rc = launch_sort_pairs_u64(sc.keys, sc.keys2, sc.vals, sc.vals2, n, 32 + vbits, sc.sort, st, &alt);
+    if (rc) return rc;
+    const unsigned long long* sk = alt ? sc.keys2 : sc.keys;
+    const uint32_t* sv = alt ? sc.vals2 : sc.vals;
+    mesh_heads_kernel<<<g, 256, 0, st>>>(sk, sc.head, n);
     NUIF_LAUNCHED("mesh_heads_kernel");
-    tb = sc.cub_bytes;
-    uint32_t* segstart = sc.vals;  // the unsorted corner indices are no longer needed
-    NUIF_HIP(hipcub::DeviceScan::InclusiveScan(sc.cub, tb, sc.seg, segstart, hipcub::Max(), (int)n, st));
-    mesh_first_kernel<<<g, 256, 0, st>>>(sc.vals2, segstart, sc.first, n);
+    rc = launch_scan(sc.head, false, sc.excl, n, nullptr, sc.scan2, st);
+    if (rc) return rc;
+    mesh_runfirst_kernel<<<g, 256, 0, st>>>(sv, sc.head, sc.excl, sc.runfirst, n);
+    NUIF_LAUNCHED("mesh_runfirst_kernel");
+    mesh_first_kernel<<<g, 256, 0, st>>>(sv, sc.head, sc.excl, sc.runfirst, sc.first, n);
     NUIF_LAUNCHED("mesh_first_kernel");
     mesh_isfirst_kernel<<<g, 256, 0, st>>>(sc.first, sc.flag, n);
     NUIF_LAUNCHED("mesh_isfirst_kernel");
-    tb = sc.cub_bytes;
-    NUIF_HIP(hipcub::DeviceScan::ExclusiveSum(sc.cub, tb, sc.flag, sc.vid, (int)n, st));
+    rc = launch_scan(sc.flag, false, sc.vid, n, nullptr, sc.scan2, st);
+    if (rc) return rc;
     mesh_output_kernel<<<g, 256, 0, st>>>(sc.first, sc.vid, sc.pos, points, indices, n, sc.npoints);
     NUIF_LAUNCHED("mesh_output_kernel");
     uint32_t np = 0;

@@ -56,6 +56,12 @@ int static_scratch(void** out);
 // (flags=true); scratch of nuif_scan_scratch_bytes(n); in == out is allowed
 int launch_scan(const void* in, bool flags, uint32_t* out, size_t n, uint32_t* total, void* scratch,
                 hipStream_t st);
+// Stable LSD radix sort of n (key, value) pairs over the low key_bits bits of the keys
+// (scan.hip).  The result is in (keys, vals), or in (keys_alt, vals_alt) when
+// *result_in_alt; scratch: sort_scratch_bytes(n) of device memory.
+size_t sort_scratch_bytes(uint32_t n);
+int launch_sort_pairs_u64(unsigned long long* keys, unsigned long long* keys_alt, uint32_t* vals, uint32_t* vals_alt,
+                          uint32_t n, int key_bits, void* scratch, hipStream_t st, bool* result_in_alt);
 
 // ---- device math, mirroring the reference's float evaluation order ----------
 // Everything is compiled with -ffp-contract=off: no implicit FMA, so each

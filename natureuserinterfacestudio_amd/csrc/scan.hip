@@ -101,6 +101,87 @@ int launch_scan(const void* in, bool flags, uint32_t* out, size_t n, uint32_t* t
     return NUIF_OK;
 }
 
+// ---- stable radix sort of (64-bit key, 32-bit value) pairs --------------------------
+// Least significant digit first, 8-bit digits, one element per thread: per pass, every
+// block counts its digits (digit-major table, so that one exclusive scan of the table
+// gives each block's first output slot per digit), then scatters its elements; an
+// element's rank among the block's elements of its digit is its rank among its wave's
+// peers (lanes with the same digit, found with 8 ballots) plus the counts of the
+// earlier waves.  Every pass keeps the order of equal digits: the sort is stable.
+constexpr int kSortThreads = 256;  // == the number of digits
+
+__global__ void __launch_bounds__(kSortThreads)
+    sort_hist_kernel(const unsigned long long* __restrict__ keys, uint32_t n, int shift, uint32_t nblocks,
+                     uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kSortThreads];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+    sort_scatter_kernel(const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ vals,
+                        unsigned long long* __restrict__ okeys, uint32_t* __restrict__ ovals, uint32_t n, int shift,
+                        uint32_t nblocks, const uint32_t* __restrict__ offsets) {
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t cnt[kWaves][256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int k = tid; k < kWaves * 256; k += kSortThreads) (&cnt[0][0])[k] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kSortThreads + tid;
+    const bool valid = i < n;
+    const unsigned long long key = valid ? keys[i] : 0ull;
+    const uint32_t d = (uint32_t)(key >> shift) & 255u;
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const unsigned long long m = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t rank = (uint32_t)__popcll(peers & below);
+    if (valid && (peers >> lane) == 1ull) cnt[wv][d] = (uint32_t)__popcll(peers);  // the wave's last peer
+    __syncthreads();
+    if (!valid) return;
+    uint32_t before = 0;
+    for (int w = 0; w < wv; ++w) before += cnt[w][d];
+    const uint32_t pos = offsets[(size_t)d * nblocks + blockIdx.x] + before + rank;
+    okeys[pos] = key;
+    ovals[pos] = vals[i];
+}
+
+size_t sort_scratch_bytes(uint32_t n) {
+    const size_t table = (size_t)kSortThreads * ((n + kSortThreads - 1) / kSortThreads);
+    return (table * sizeof(uint32_t) + 255) / 256 * 256 + nuif_scan_scratch_bytes(table);
+}
+
+int launch_sort_pairs_u64(unsigned long long* keys, unsigned long long* keys_alt, uint32_t* vals, uint32_t* vals_alt,
+                          uint32_t n, int key_bits, void* scratch, hipStream_t st, bool* result_in_alt) {
+    const uint32_t nblocks = (n + kSortThreads - 1) / kSortThreads;
+    const size_t table = (size_t)kSortThreads * nblocks;
+    uint32_t* hist = static_cast<uint32_t*>(scratch);
+    void* scan_scratch = static_cast<char*>(scratch) + (table * sizeof(uint32_t) + 255) / 256 * 256;
+    bool alt = false;
+    for (int shift = 0; shift < key_bits && n > 0; shift += 8) {
+        const unsigned long long* ki = alt ? keys_alt : keys;
+        const uint32_t* vi = alt ? vals_alt : vals;
+        unsigned long long* ko = alt ? keys : keys_alt;
+        uint32_t* vo = alt ? vals : vals_alt;
+        sort_hist_kernel<<<nblocks, kSortThreads, 0, st>>>(ki, n, shift, nblocks, hist);
+        NUIF_LAUNCHED("sort_hist_kernel");
+        const int rc = launch_scan(hist, false, hist, table, nullptr, scan_scratch, st);
+        if (rc) return rc;
+        sort_scatter_kernel<<<nblocks, kSortThreads, 0, st>>>(ki, vi, ko, vo, n, shift, nblocks, hist);
+        NUIF_LAUNCHED("sort_scatter_kernel");
+        alt = !alt;
+    }
+    *result_in_alt = alt;
+    return NUIF_OK;
+}
+
 }  // namespace nuif
 
 using namespace nuif;
