@@ -102,6 +102,15 @@ int nuif_graph_destroy(nuif_graph_t g);
 
 /* ---- frame preprocessing (one entry per reference kernel) ------------------ */
 
+/* OCL/NuiKinfuOpenCLFrame.cpp:405-448 UpdateColorBuffers(ColorSpacePoint*, UINT*, UINT,
+ * const NuiColorImage&): the BGRA colour of each of the n depth pixels through the
+ * depth-to-colour map.  For pixel i, j = lut ? lut[i] : i; when j < n and
+ * ((int)(x_j + 0.5f), (int)(y_j + 0.5f)) lies in the iw x ih colour image, the output is that
+ * pixel with alpha 255, else 0.  map_xy: n (x, y) float pairs in colour-image pixels (the
+ * Kinect SDK's ColorSpacePoint), lut: n uint32 or NULL, image: iw*ih BGRA; device memory. */
+int nuif_map_depth_to_color(const float* map_xy, const uint32_t* lut, uint32_t n, const uint8_t* image, int iw,
+                            int ih, uint8_t* bgra_out, nuif_stream_t s);
+
 /* K/filters.cl:9-23 depth_passing_filter_kernel: depth[i] = raw[i]*0.001 m, NaN unless dmin < d < dmax */
 int nuif_depth_passing_filter(const uint16_t* raw, float* depth, size_t n, float dmin, float dmax,
                               nuif_stream_t s);

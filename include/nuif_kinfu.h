@@ -100,6 +100,17 @@ int nuifk_hash_stored_blocks(void* h, size_t* out);
 /* Hash scene: insertions that found no brick (pool exhausted) or no free bucket slot so far,
  * which the reference drops silently; 0 for a dense scene.  Synchronises the session. */
 int nuifk_scene_failures(void* h, unsigned long long* out);
+/* DS/NuiKinfuScene.h:43 Volume2Mesh(NuiMeshShape*) (src/SLAM/Backend/NuiSLAMController.cpp:121-127):
+ * the shared-vertex mesh of the scene, points (float3) then triangle indices.  Copies up to
+ * cap_points points and cap_indices indices (either pointer may be NULL) and returns the counts. */
+int nuifk_mesh_shape(void* h, float* points, size_t cap_points, int32_t* indices, size_t cap_indices,
+                     size_t* n_points, size_t* n_indices);
+/* DS/NuiKinfuFrame.h:22 UpdateColorBuffers(ColorSpacePoint*, UINT*, UINT, const NuiColorImage&): the
+ * frame's colours from a colour image through the depth-to-colour map (host arrays: W*H (x, y)
+ * floats, W*H uint32 lookup table or NULL, iw*ih BGRA image); nuifk_frame_colors reads the
+ * frame's W*H BGRA colours back. */
+int nuifk_map_color(void* h, const float* map_xy, const uint32_t* lut, const uint8_t* image, int iw, int ih);
+int nuifk_frame_colors(void* h, uint8_t* out);
 /* Dense scene: distinct voxels the raycast reads at the current camera pose
  * (nuif_tsdf_raycast_footprint), for the raycast's algorithmic bytes. */
 int nuifk_raycast_footprint(void* h, unsigned long long* out);

@@ -240,11 +240,43 @@ int launch_half_batch(const HalfJob* jobs, const HalfJob* hjobs, int n, hipStrea
     return NUIF_OK;
 }
 
+// OCL/NuiKinfuOpenCLFrame.cpp:405-448: the colour of depth pixel i through the
+// depth-to-colour map, read at map[lut[i]] (map[i] without a lookup table)
+__global__ void map_color_kernel(const float2* __restrict__ map, const uint32_t* __restrict__ lut, uint32_t n,
+                                 const uchar4* __restrict__ image, int iw, int ih, uchar4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = lut ? lut[i] : i;
+    uchar4 c = make_uchar4(0, 0, 0, 0);
+    if (j < n) {
+        const float2 q = map[j];
+        const int cx = (int)(q.x + 0.5f), cy = (int)(q.y + 0.5f);  // truncation toward zero, as written
+        if (cx >= 0 && cx < iw && cy >= 0 && cy < ih) {
+            c = image[(size_t)cy * iw + cx];
+            c.w = 255;
+        }
+    }
+    out[i] = c;
+}
+
 }  // namespace nuif
 
 using namespace nuif;
 
 extern "C" {
+
+int nuif_map_depth_to_color(const float* map_xy, const uint32_t* lut, uint32_t n, const uint8_t* image, int iw, int ih,
+                            uint8_t* bgra_out, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(map_xy && image && bgra_out, "nuif_map_depth_to_color: null argument");
+    NUIF_CHECK_ARG(iw > 0 && ih > 0, "nuif_map_depth_to_color: empty colour image");
+    if (n == 0) return NUIF_OK;
+    map_color_kernel<<<div_up(n, 256), 256, 0, stream_of(s)>>>(reinterpret_cast<const float2*>(map_xy), lut, n,
+                                                                reinterpret_cast<const uchar4*>(image), iw, ih,
+                                                                reinterpret_cast<uchar4*>(bgra_out));
+    NUIF_LAUNCHED("map_color_kernel");
+    return NUIF_OK;
+}
 
 int nuif_depth_passing_filter(const uint16_t* raw, float* depth, size_t n, float dmin, float dmax, nuif_stream_t s) {
     NUIF_REQUIRE_INIT();

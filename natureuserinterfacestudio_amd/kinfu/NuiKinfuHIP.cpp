@@ -154,6 +154,25 @@ void NuiKinfuHIPFrame::UpdateColorBuffers(BGRQUAD* pColors, UINT nNum) {
     m_hasColor = true;
 }
 
+void NuiKinfuHIPFrame::UpdateColorBuffers(ColorSpacePoint* pDepthToColor, UINT* pDepthDistortionLT, UINT nNum,
+                                          const NuiColorImage& image) {
+    if (!pDepthToColor || !image.GetBuffer()) return;
+    if (nNum != m_W * m_H) throw std::runtime_error("UpdateColorBuffers: frame size mismatch");
+    const size_t img = (size_t)image.GetWidth() * image.GetHeight() * 4;
+    if (m_colorMap.bytes() < (size_t)nNum * 8) m_colorMap = DeviceBuffer((size_t)nNum * 8);
+    if (pDepthDistortionLT && m_colorLut.bytes() < (size_t)nNum * 4) m_colorLut = DeviceBuffer((size_t)nNum * 4);
+    if (m_colorImage.bytes() < img) m_colorImage = DeviceBuffer(img);
+    check(nuif_memcpy_h2d(m_colorMap.get(), pDepthToColor, (size_t)nNum * 8, m_stream), "colour map upload");
+    if (pDepthDistortionLT)
+        check(nuif_memcpy_h2d(m_colorLut.get(), pDepthDistortionLT, (size_t)nNum * 4, m_stream), "lut upload");
+    check(nuif_memcpy_h2d(m_colorImage.get(), image.GetBuffer(), img, m_stream), "colour image upload");
+    check(nuif_map_depth_to_color(m_colorMap.as<float>(), pDepthDistortionLT ? m_colorLut.as<uint32_t>() : nullptr,
+                                  nNum, m_colorImage.as<uint8_t>(), (int)image.GetWidth(), (int)image.GetHeight(),
+                                  m_colors.as<uint8_t>(), m_stream),
+          "nuif_map_depth_to_color");
+    m_hasColor = true;
+}
+
 void NuiKinfuHIPFrame::UpdateColorBuffersDevice(const uint8_t* dBgra) {
     if (!dBgra) return;
     if (dBgra != m_colors.as<uint8_t>())

@@ -68,6 +68,9 @@ public:
     ~NuiKinfuHIPFrame() override;
     void UpdateVertexBuffers(UINT16* pDepths, UINT nNum, NuiKinfuCameraState* pCameraState) override;
     void UpdateColorBuffers(BGRQUAD* pColors, UINT nNum) override;
+    // OCL/NuiKinfuOpenCLFrame.cpp:405-448 on the device (nuif_map_depth_to_color)
+    void UpdateColorBuffers(ColorSpacePoint* pDepthToColor, UINT* pDepthDistortionLT, UINT nNum,
+                            const NuiColorImage& image) override;
     // device-resident inputs (no host copy)
     void UpdateVertexBuffersDevice(const UINT16* dDepths, NuiKinfuCameraState* pCameraState);
     void UpdateColorBuffersDevice(const uint8_t* dBgra);
@@ -94,6 +97,7 @@ public:
 
 private:
     void Preprocess(NuiKinfuCameraState* pCameraState);
+    DeviceBuffer m_colorMap, m_colorLut, m_colorImage;  // staging of the depth-to-colour mapping
     NuiTrackerConfig m_cfg;
     UINT m_W, m_H;
     int m_levels;
@@ -250,6 +254,7 @@ public:
     // distinct voxels the raycast reads at the camera's current pose (nuif_tsdf_raycast_footprint;
     // synchronises); not for a wrapped volume
     long long RaycastFootprint(NuiKinfuHIPCameraState* cam) const;
+    using NuiKinfuScene::Volume2Mesh;
     // the scene's part of a batched round's entry (volume, surface blocks, walk states);
     // false when the volume cannot be batched (a shifting volume, or dimensions the tile
     // integration does not take)
@@ -318,6 +323,7 @@ public:
     // K/KinfuHashVisualization.cl via nuif_hash_mesh over the visible list
     bool Volume2CLMesh(NuiKinfuMesh* pData) override;
     bool Volume2Mesh(NuiKinfuMesh* pMesh) override { return Volume2CLMesh(pMesh); }
+    using NuiKinfuScene::Volume2Mesh;
     bool NeedsRayLengthDepth() const override { return false; }
     void* DevicePointer() const override { return m_scene.get(); }
     size_t DeviceBytes() const override { return m_scene.bytes(); }
@@ -418,6 +424,7 @@ public:
     nuif_kinfu::NuiKinfuHIPFeedbackFrame* feedbackFrame() const { return m_pFeedbackFrame; }
     nuif_kinfu::NuiKinfuHIPDepthTracker* tracker() const { return m_pTracker; }
     nuif_kinfu::NuiKinfuHIPCameraState* cameraState() const { return m_pCameraState; }
+    nuif_kinfu::NuiKinfuHIPFrame* frame() const { return m_pFrame; }
     const float* poseHistoryDevice() const { return m_history.as<float>(); }
     int poseHistoryCapacity() const { return m_historyCap; }
     // Rows of the device pose ring (written by the frame's ICP end, row frame % capacity)

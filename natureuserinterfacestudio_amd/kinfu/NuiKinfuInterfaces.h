@@ -23,6 +23,31 @@ struct NuiKinfuMesh {
     size_t triangleCount() const { return positions.size() / 9; }
 };
 
+typedef Vector3f SgVec3f;
+
+// src/Shape/NuiMeshShape.h:7-61: shared points and triangle indices (Volume2Mesh's output for
+// the SLAM controller, src/SLAM/Backend/NuiSLAMController.cpp:121-127); UVs and colours are
+// not produced by the fusion path.
+class NuiMeshShape {
+public:
+    void appendPoint(const SgVec3f& point) { m_points.push_back(point); }
+    void appendTriangleIndex(int index) { m_triangle_indices.push_back(index); }
+    size_t pointsNum() const { return m_points.size(); }
+    const SgVec3f& getPoint(unsigned int id) const { return m_points.at(id); }
+    size_t trianglesNum() const { return m_triangle_indices.size() / 3; }
+    int triangleIndex(unsigned int id) const {
+        return id >= m_triangle_indices.size() ? -1 : m_triangle_indices.at(id);
+    }
+    void clear() {
+        m_points.clear();
+        m_triangle_indices.clear();
+    }
+
+private:
+    std::vector<SgVec3f> m_points;
+    std::vector<int> m_triangle_indices;
+};
+
 // DS/NuiKinfuFrame.h:11-29
 class NuiKinfuFrame {
 public:
@@ -31,6 +56,10 @@ public:
     void clearDirty() { m_dirty = false; }
     virtual void UpdateVertexBuffers(UINT16* pDepths, UINT nNum, NuiKinfuCameraState* pCameraState) = 0;
     virtual void UpdateColorBuffers(BGRQUAD* pColors, UINT nNum) = 0;
+    // the colour image seen through the depth-to-colour map (and the depth distortion
+    // lookup table, or NULL), DS/NuiKinfuFrame.h:22
+    virtual void UpdateColorBuffers(ColorSpacePoint* pDepthToColor, UINT* pDepthDistortionLT, UINT nNum,
+                                    const NuiColorImage& image) = 0;
     virtual UINT GetWidth() const = 0;
     virtual UINT GetHeight() const = 0;
 
@@ -78,9 +107,23 @@ public:
     virtual void reset() = 0;
     virtual bool integrateVolume(NuiKinfuFrame* pFrame, NuiKinfuCameraState* pCameraState) = 0;
     virtual void raycastRender(NuiKinfuFeedbackFrame* pFeedbackFrame, NuiKinfuCameraState* pCameraState) = 0;
+    // DS/NuiKinfuScene.h:39: the reference's hash scene defers its render to a GL thread that
+    // calls this; the render here happens in raycastRender, so there is nothing left to do
+    virtual void offlineRender() {}
     virtual bool Volume2CLVertices(NuiKinfuMesh* pData) { return false; }
     virtual bool Volume2CLMesh(NuiKinfuMesh* pData) = 0;
     virtual bool Volume2Mesh(NuiKinfuMesh* pMesh) = 0;
+    // DS/NuiKinfuScene.h:43: the shared-vertex mesh as a NuiMeshShape (points, then triangle
+    // indices), appended to pMesh
+    virtual bool Volume2Mesh(NuiMeshShape* pMesh) {
+        if (!pMesh) return false;
+        NuiKinfuMesh m;
+        if (!Volume2Mesh(&m)) return false;
+        for (size_t i = 0; i + 2 < m.positions.size(); i += 3)
+            pMesh->appendPoint(SgVec3f(m.positions[i], m.positions[i + 1], m.positions[i + 2]));
+        for (uint32_t k : m.indices) pMesh->appendTriangleIndex((int)k);
+        return true;
+    }
 
 protected:
     std::atomic<bool> m_dirty{false};

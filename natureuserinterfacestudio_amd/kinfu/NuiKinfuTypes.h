@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <vector>
 
 typedef unsigned int UINT;
 typedef uint16_t UINT16;
@@ -21,6 +22,35 @@ typedef int64_t INT64;
 struct BGRQUAD {
     uint8_t rgbBlue, rgbGreen, rgbRed, rgbReserved;
 };
+
+// The Kinect SDK's ColorSpacePoint: where a depth pixel lands in the colour image (pixels).
+struct ColorSpacePoint {
+    float X, Y;
+};
+
+// src/Shape/NuiImageBuffer.h:7-57: an image of width x height pixels of T.
+template <typename T>
+class NuiImageBuffer {
+public:
+    T* AllocateBuffer(UINT width, UINT height) {
+        m_nWidth = width;
+        m_nHeight = height;
+        m_buf.assign((size_t)width * height, T{});
+        return m_buf.data();
+    }
+    void Clear() {
+        m_buf.clear();
+        m_nWidth = m_nHeight = 0;
+    }
+    T* GetBuffer() const { return m_buf.empty() ? nullptr : const_cast<T*>(m_buf.data()); }
+    UINT GetWidth() const { return m_nWidth; }
+    UINT GetHeight() const { return m_nHeight; }
+
+private:
+    std::vector<T> m_buf;
+    UINT m_nWidth = 0, m_nHeight = 0;
+};
+typedef NuiImageBuffer<BGRQUAD> NuiColorImage;
 
 class Vector3f {
 public:

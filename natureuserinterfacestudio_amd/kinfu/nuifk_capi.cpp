@@ -145,6 +145,58 @@ int nuifk_hash_stored_blocks(void* h, size_t* out) {
     });
 }
 
+int nuifk_mesh_shape(void* h, float* points, size_t cap_points, int32_t* indices, size_t cap_indices,
+                     size_t* n_points, size_t* n_indices) {
+    return guarded([&] {
+        if (!n_points || !n_indices) throw std::invalid_argument("nuifk_mesh_shape: null count");
+        Session* s = static_cast<Session*>(h);
+        s->engine->synchronize();
+        NuiMeshShape mesh;
+        NuiKinfuScene* scene = s->scene.get();
+        if (!scene->Volume2Mesh(&mesh)) throw std::runtime_error("nuifk_mesh_shape: no mesh");
+        *n_points = mesh.pointsNum();
+        *n_indices = mesh.trianglesNum() * 3;
+        if (points)
+            for (size_t i = 0; i < std::min(cap_points, mesh.pointsNum()); ++i) {
+                const SgVec3f& p = mesh.getPoint((unsigned)i);
+                points[3 * i] = p[0];
+                points[3 * i + 1] = p[1];
+                points[3 * i + 2] = p[2];
+            }
+        if (indices)
+            for (size_t i = 0; i < std::min(cap_indices, *n_indices); ++i) indices[i] = mesh.triangleIndex((unsigned)i);
+        return 0;
+    });
+}
+
+int nuifk_map_color(void* h, const float* map_xy, const uint32_t* lut, const uint8_t* image, int iw, int ih) {
+    return guarded([&] {
+        Session* s = static_cast<Session*>(h);
+        if (!map_xy || !image || iw <= 0 || ih <= 0) throw std::invalid_argument("nuifk_map_color: bad argument");
+        NuiColorImage img;
+        BGRQUAD* dst = img.AllocateBuffer((UINT)iw, (UINT)ih);
+        std::memcpy(dst, image, (size_t)iw * ih * 4);
+        auto* frame = s->engine->frame();
+        frame->UpdateColorBuffers(reinterpret_cast<ColorSpacePoint*>(const_cast<float*>(map_xy)),
+                                  const_cast<UINT*>(reinterpret_cast<const UINT*>(lut)),
+                                  frame->GetWidth() * frame->GetHeight(), img);
+        return 0;
+    });
+}
+
+int nuifk_frame_colors(void* h, uint8_t* out) {
+    return guarded([&] {
+        Session* s = static_cast<Session*>(h);
+        auto* frame = s->engine->frame();
+        const uint8_t* c = frame->GetColorBuffer();
+        if (!out || !c) throw std::invalid_argument("nuifk_frame_colors: no colour buffer");
+        check(nuif_memcpy_d2h(out, c, (size_t)frame->GetWidth() * frame->GetHeight() * 4, s->engine->stream()),
+              "frame colours");
+        check(nuif_stream_synchronize(s->engine->stream()), "frame colours");
+        return 0;
+    });
+}
+
 int nuifk_raycast_footprint(void* h, unsigned long long* out) {
     return guarded([&] {
         Session* s = static_cast<Session*>(h);

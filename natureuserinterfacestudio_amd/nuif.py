@@ -151,6 +151,7 @@ _SIGS = {
     "nuif_fusion_round_timed": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nuif_tsdf_footprint_bytes": (c_size_t, [c_void_p]),
     "nuif_schedule": (c_int, [c_void_p]),
+    "nuif_map_depth_to_color": (c_int, [c_void_p, c_void_p, C.c_uint32, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "nuif_schedule_segment": (c_int, [c_int]),
     "nuif_tsdf_raycast_footprint": (C.c_longlong, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                                     c_void_p, c_void_p, c_void_p]),
@@ -298,6 +299,9 @@ _KSIGS = {
     "nuifs_set_stage_timing": (c_int, [c_void_p, c_int]),
     "nuifs_take_stage_times": (c_int, [c_void_p, c_void_p]),
     "nuifk_raycast_footprint": (c_int, [c_void_p, c_void_p]),
+    "nuifk_mesh_shape": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p]),
+    "nuifk_map_color": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int]),
+    "nuifk_frame_colors": (c_int, [c_void_p, c_void_p]),
     "nuifs_last_outcomes": (c_int, [c_void_p, c_void_p, c_int]),
     "nuifk_scene_failures": (c_int, [c_void_p, c_void_p]),
     "nuifs_set_batching": (c_int, [c_void_p, c_int]),
@@ -584,6 +588,29 @@ class Engine:
         c = np.empty((max(n, 1), 4), np.float32)
         self._check(K().nuifk_shift_cache(self._h, p.ctypes.data, c.ctypes.data, n), "shift cache")
         return p[:n], c[:n]
+
+    def mesh_shape(self):
+        """Volume2Mesh(NuiMeshShape*): (points float32[n,3], triangle indices int32[m])."""
+        npts, nidx = c_size_t(), c_size_t()
+        self._check(K().nuifk_mesh_shape(self._h, None, 0, None, 0, C.byref(npts), C.byref(nidx)), "mesh shape")
+        pts = np.empty((max(npts.value, 1), 3), np.float32)
+        idx = np.empty(max(nidx.value, 1), np.int32)
+        self._check(K().nuifk_mesh_shape(self._h, pts.ctypes.data, npts.value, idx.ctypes.data, nidx.value,
+                                         C.byref(npts), C.byref(nidx)), "mesh shape")
+        return pts[:npts.value], idx[:nidx.value]
+
+    def map_color(self, map_xy, image, lut=None):
+        """The frame's colours from `image` (HxWx4 BGRA) through the depth-to-colour map."""
+        m = np.ascontiguousarray(map_xy, np.float32)
+        img = np.ascontiguousarray(image, np.uint8)
+        lp = np.ascontiguousarray(lut, np.uint32).ctypes.data if lut is not None else None
+        self._check(K().nuifk_map_color(self._h, m.ctypes.data, lp, img.ctypes.data, img.shape[1], img.shape[0]),
+                    "map colour")
+
+    def frame_colors(self):
+        out = np.empty((self.desc.height, self.desc.width, 4), np.uint8)
+        self._check(K().nuifk_frame_colors(self._h, out.ctypes.data), "frame colours")
+        return out
 
     def raycast_footprint(self):
         """Distinct voxels the dense raycast reads at the current pose (U of B*U + 28*W*H)."""

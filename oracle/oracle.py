@@ -588,3 +588,26 @@ def points_dense_slice(words, color_volume, storage, res, cell, wrap, offset, rn
     cols = np.empty((max(n, 1), 4), np.float32)
     lib().or_points_dense_slice(*args, pts.ctypes.data, cols.ctypes.data, n)
     return pts[:n], cols[:n], n
+
+
+def map_depth_to_color(map_xy, lut, image):
+    """OCL/NuiKinfuOpenCLFrame.cpp:405-448 UpdateColorBuffers(ColorSpacePoint*, UINT*, UINT,
+    const NuiColorImage&), restated in numpy: depth pixel i reads the map at j = lut[i] (i
+    without a table); for j < n the colour pixel ((int)(x + 0.5f), (int)(y + 0.5f)) -- float
+    addition, then truncation toward zero -- is taken with alpha 255 when it lies in the
+    image, else the pixel is 0.  map_xy: (n, 2) float32, lut: (n,) uint32 or None,
+    image: (ih, iw, 4) uint8 BGRA.  Returns (n, 4) uint8."""
+    map_xy = np.asarray(map_xy, np.float32)
+    n = map_xy.shape[0]
+    ih, iw = image.shape[:2]
+    j = np.arange(n, dtype=np.int64) if lut is None else np.asarray(lut, np.uint32).astype(np.int64)
+    ok = j < n
+    q = map_xy[np.where(ok, j, 0)]
+    cx = np.trunc(q[:, 0] + np.float32(0.5)).astype(np.int64)
+    cy = np.trunc(q[:, 1] + np.float32(0.5)).astype(np.int64)
+    inside = ok & (cx >= 0) & (cx < iw) & (cy >= 0) & (cy < ih)
+    out = np.zeros((n, 4), np.uint8)
+    flat = image.reshape(-1, 4)
+    out[inside] = flat[cy[inside] * iw + cx[inside]]
+    out[inside, 3] = 255
+    return out

@@ -11,7 +11,9 @@
     projects into the image;
   * the trilinear interpolation (K/raycast.cl:36-120): its corner names follow an
     x-major layout over the y-major volume, so the x and y fractions weight swapped
-    corners; a field linear in x is interpolated with the y fraction.
+    corners; a field linear in x is interpolated with the y fraction;
+  * the depth-to-colour mapping (OCL/NuiKinfuOpenCLFrame.cpp:405-448): (int)(x + 0.5f)
+    truncates toward zero, so x in (-1.5, -0.5) still lands on column 0.
 Each is built so that the plausible slip (rounding to nearest, no channel swap, no
 re-check, the textbook corner weights) fails.
 """
@@ -232,3 +234,33 @@ def test_trilinear_weights_x_corners_with_the_y_fraction():
     vol[..., 0] = np.round((gy - 8) / 10.0 * 32766).astype(np.int16)
     got_y = lib().or_trilinear_s16_f(vol.reshape(-1), N, dim, trunc, p)
     assert abs(got_y - ((1 - a) * fx(7) + a * fx(8))) < 1e-5
+
+
+# ---------------------------------------------------------------- depth-to-colour mapping
+
+def test_depth_to_color_mapping_closed_forms():
+    rng = np.random.default_rng(5)
+    ih, iw = 6, 8
+    image = rng.integers(0, 255, (ih, iw, 4), dtype=np.uint8)
+    ys, xs = np.meshgrid(np.arange(ih), np.arange(iw), indexing="ij")
+    ident = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.float32)
+    out = O.map_depth_to_color(ident, None, image)
+    want = image.reshape(-1, 4).copy()
+    want[:, 3] = 255
+    np.testing.assert_array_equal(out, want)
+    # +0.49 stays on the pixel; the reversal table reverses the output
+    np.testing.assert_array_equal(O.map_depth_to_color(ident + np.float32(0.49), None, image), want)
+    rev = np.arange(ih * iw)[::-1].astype(np.uint32)
+    np.testing.assert_array_equal(O.map_depth_to_color(ident, rev, image), want[::-1])
+    # truncation toward zero: x in (-1.5, -0.5) still lands on column 0, -1.5 does not
+    m = np.array([[-0.7, 0.0], [-1.0, 0.0], [-1.49, 2.0], [-1.5, 0.0], [7.49, 5.0], [7.5, 0.0]], np.float32)
+    small = O.map_depth_to_color(m, None, image)
+    np.testing.assert_array_equal(small[:3, :3], image[[0, 0, 2], [0, 0, 0], :3])
+    assert (small[:3, 3] == 255).all()
+    assert (small[3] == 0).all() and (small[5] == 0).all()
+    np.testing.assert_array_equal(small[4, :3], image[5, 7, :3])
+    # a table entry beyond the frame gives 0
+    bad = np.zeros(6, np.uint32)
+    bad[2] = 6
+    out2 = O.map_depth_to_color(m, bad, image)
+    assert (out2[2] == 0).all() and (out2[0] == small[0]).all()
