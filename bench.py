@@ -121,6 +121,42 @@ def run_server(nuif, wl, frames, sessions, storage, warmup, rounds, budget):
     return out
 
 
+def run_hash(nuif, wl, warmup, steps):
+    """One voxel-hashing session (5 mm voxels in 8^3 FP16 + uint16 bricks) at the
+    workload's resolution, frames back to back with its stages event-timed: frames/s,
+    stage ms and the integration stage against HBM peak (each visible brick read and
+    written whole, 2 x 2 KiB + 16 B of entry per brick, plus the 4 W H bytes of depth)."""
+    frames = [nuif.DeviceArray.from_host(wl.render(f)[0]) for f in range(warmup + steps)]
+    eng = nuif.Engine(wl.kinfu_desc(storage=STORAGES["f16"], gate=0.0))
+    for f in range(warmup):
+        eng.run(frames[f], sync=False)
+    eng.synchronize()
+    eng.set_stage_timing(True)
+    eng.take_stage_times()
+    visible = []
+    t0 = time.perf_counter()
+    for f in range(warmup, warmup + steps):
+        eng.run(frames[f], sync=False)
+        visible.append(eng.hash_stats()[1])  # synchronises: the visible list of this frame
+    wall = time.perf_counter() - t0
+    ok = eng.synchronize()
+    timed, ms = eng.take_stage_times()
+    stats = eng.hash_stats()
+    eng.close()
+    avg = [x / max(timed, 1) for x in ms]
+    names = ["preprocess", "icp", "integrate", "raycast"]
+    device_ms = sum(avg)
+    vis = float(np.mean(visible))
+    int_bytes = vis * (2 * 2048 + 16) + 4 * wl.width * wl.height
+    return {"workload": wl.name, "frames": steps, "fps": round(1e3 / device_ms, 3),
+            "ms_per_frame": round(device_ms, 4), "wall_fps": round(steps / wall, 3), "tracking_ok": bool(ok),
+            "stage_ms": {k: round(v, 4) for k, v in zip(names, avg)}, "visible_bricks_per_frame": round(vis, 1),
+            "bricks_in_use": stats[0], "failed_insertions": stats[2],
+            "integrate_roofline": {"bound": "hbm", "algorithmic_bytes": int(int_bytes),
+                                   "achieved": round(int_bytes / (avg[2] / 1e3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": round(int_bytes / (avg[2] / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}}
+
+
 def server_stages(nuif, srv, ids, wl, frames, first, rounds, storage):
     """Further rounds with the batched round's stages event-timed on the server stream,
     and the algorithmic bytes of each stage from what the rounds did: updated voxels
@@ -268,6 +304,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--c5-sessions", type=int, default=64, help="arrival-trace sessions (0: skip that leg)")
     ap.add_argument("--c5-rate", type=float, default=2.0, help="arrivals per second")
+    ap.add_argument("--hash-steps", type=int, default=20, help="frames of the 640x480 hash session (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device to device and exit (scales the HBM byte counters)")
@@ -328,6 +365,11 @@ def main():
         server = run_server(nuif, wl, frames, args.sessions, STORAGES[args.server_storage], args.warmup, args.steps,
                             args.budget_gb * 2 ** 30)
 
+    hashed = None
+    if args.hash_steps > 0:
+        print(f"[bench] hash session: {W.HASH.name}", file=sys.stderr, flush=True)
+        hashed = run_hash(nuif, W.HASH, min(args.warmup, 5), args.hash_steps)
+
     c5 = None
     if args.c5_sessions > 0:
         print(f"[bench] arrival trace: {args.c5_sessions} sessions at {args.c5_rate}/s", file=sys.stderr, flush=True)
@@ -374,6 +416,7 @@ def main():
         "fp32_serial_fps": round(fp32["fps"], 3) if fp32 else None,
         "server_vs_fp32_serial": round(server["fps"] / fp32["fps"], 3) if server and fp32 else None,
         "roofline": roof,
+        "hash_session": hashed,
         "arrival_trace": c5,
         "cpu_baseline": cpu,
         "device": info["name"],

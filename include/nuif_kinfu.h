@@ -100,6 +100,9 @@ int nuifk_hash_stored_blocks(void* h, size_t* out);
 /* Hash scene: insertions that found no brick (pool exhausted) or no free bucket slot so far,
  * which the reference drops silently; 0 for a dense scene.  Synchronises the session. */
 int nuifk_scene_failures(void* h, unsigned long long* out);
+/* Hash scene: [bricks handed out, visible-list length of the last frame, failed insertions,
+ * occupied entries], as nuif_hash_stats; all zero for a dense scene.  Synchronises the session. */
+int nuifk_hash_stats(void* h, uint32_t* out4);
 /* DS/NuiKinfuScene.h:43 Volume2Mesh(NuiMeshShape*) (src/SLAM/Backend/NuiSLAMController.cpp:121-127):
  * the shared-vertex mesh of the scene, points (float3) then triangle indices.  Copies up to
  * cap_points points and cap_indices indices (either pointer may be NULL) and returns the counts. */

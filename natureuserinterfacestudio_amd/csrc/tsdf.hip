@@ -6,7 +6,9 @@
 // Integration.  The reference walks one (x,y) column per work-item along z,
 // updating the projection incrementally (K/tsdfVolume.cl:74-189); adjacent
 // work-items then touch addresses Nz*4 bytes apart, i.e. no coalescing.  Here
-// one wavefront owns an 8x8 tile of columns and processes z in chunks of 64:
+// one wavefront owns an 8x8 tile of columns and one z segment of them (64, 128
+// or 256 voxels, chosen per launch; the walk state at each segment's start comes
+// from a column-state pass), and processes the segment in chunks of 64:
 //   phase 1  every lane walks its own column for 64 steps with exactly the
 //            reference's incremental arithmetic (bit-identical projections) and
 //            parks the new tsdf sample (or "no update") in LDS;
@@ -16,7 +18,8 @@
 // Voxels that the reference does not update are neither read nor written
 // unless they share a 16-byte vector with one that is (then they are written
 // back unchanged).  Algorithmic traffic: 2 x bytes/voxel per updated voxel +
-// the depth image; the kernel is HBM-bound.
+// the depth image; the time is set by phase 1 (one dependent depth gather per
+// voxel along each column), far above what that traffic costs at HBM rate.
 //
 // Raycast.  One lane per pixel, 8x8 pixel tiles per wavefront so that
 // neighbouring rays share cache lines; the fixed-step march, zero-crossing

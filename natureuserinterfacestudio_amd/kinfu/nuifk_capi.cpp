@@ -222,6 +222,16 @@ int nuifk_scene_failures(void* h, unsigned long long* out) {
     });
 }
 
+int nuifk_hash_stats(void* h, uint32_t* out4) {
+    return guarded([&] {
+        if (!out4) throw std::invalid_argument("nuifk_hash_stats: null output");
+        auto* hs = dynamic_cast<NuiKinfuHIPHashScene*>(static_cast<Session*>(h)->scene.get());
+        for (int k = 0; k < 4; ++k) out4[k] = 0;
+        if (hs) hs->Stats(out4);
+        return 0;
+    });
+}
+
 int nuifk_state(void* h, nuif_icp_result* out) {
     return guarded([&] {
         *out = static_cast<Session*>(h)->engine->tracker()->ReadState();

@@ -304,6 +304,7 @@ _KSIGS = {
     "nuifk_frame_colors": (c_int, [c_void_p, c_void_p]),
     "nuifs_last_outcomes": (c_int, [c_void_p, c_void_p, c_int]),
     "nuifk_scene_failures": (c_int, [c_void_p, c_void_p]),
+    "nuifk_hash_stats": (c_int, [c_void_p, c_void_p]),
     "nuifs_set_batching": (c_int, [c_void_p, c_int]),
     "nuifs_last_batch": (c_int, [c_void_p]),
     "nuif_io_write_frame_image": (c_int, [C.c_char_p, c_int, C.c_int64, C.c_uint32, C.c_uint32, c_void_p,
@@ -623,6 +624,13 @@ class Engine:
         out = C.c_ulonglong()
         self._check(K().nuifk_scene_failures(self._h, C.byref(out)), "scene failures")
         return out.value
+
+    def hash_stats(self):
+        """Hash scene: (bricks handed out, visible blocks of the last frame, failed
+        insertions, occupied entries); zeros for a dense scene."""
+        out = (C.c_uint32 * 4)()
+        self._check(K().nuifk_hash_stats(self._h, out), "hash stats")
+        return tuple(int(x) for x in out)
 
     def set_history_capacity(self, capacity):
         """Rows of the device pose ring; rows are copied to the host before it wraps."""
