@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--c5-rate", type=float, default=2.0, help="arrivals per second")
     ap.add_argument("--hash-steps", type=int, default=20, help="frames of the 640x480 hash session (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--server-only", action="store_true",
+                    help="skip the single-session and FP32 serial legs (profiles of the batched round)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many MiB device to device and exit (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -314,6 +316,8 @@ def main():
               f"and this build measures one GPU's share (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
 
+    if args.server_only and args.sessions <= 0:
+        ap.error("--server-only needs --sessions > 0")
     from natureuserinterfacestudio_amd import nuif
     from natureuserinterfacestudio_amd import workloads as W
 
@@ -340,13 +344,14 @@ def main():
 
     # 1) one session, reference-precision volume, stage-timed: the roofline of the integration kernel
     storage = STORAGES[args.storage]
-    print(f"[bench] single session, storage {args.storage}", file=sys.stderr, flush=True)
-    single = run_single(nuif, wl, frames, storage, args.graph, args.warmup, args.steps)
-    B = nuif.L().nuif_tsdf_voxel_bytes(storage)
-    integ_bytes = 2 * B * single["updated_per_frame"] + 4 * wl.width * wl.height
     names = ["preprocess", "icp", "integrate", "raycast"]
-    roof = None
-    if single["stage_avg"]:
+    roof = single = None
+    if not args.server_only:
+        print(f"[bench] single session, storage {args.storage}", file=sys.stderr, flush=True)
+        single = run_single(nuif, wl, frames, storage, args.graph, args.warmup, args.steps)
+        B = nuif.L().nuif_tsdf_voxel_bytes(storage)
+        integ_bytes = 2 * B * single["updated_per_frame"] + 4 * wl.width * wl.height
+    if single and single["stage_avg"]:
         t_int = single["stage_avg"][2] / 1e3
         achieved = integ_bytes / t_int / 1e9
         roof = {"kernel": f"integrate_tile_kernel<{args.storage}>", "bound": "hbm", "achieved": round(achieved, 2),
@@ -358,8 +363,9 @@ def main():
     server = fp32 = None
     if args.sessions > 0:
         # 2) the plain FP32 pipeline, one session at a time (the server's reference point)
-        print("[bench] FP32 single session, serial", file=sys.stderr, flush=True)
-        fp32 = run_single(nuif, wl, frames, STORAGES["f32"], 0, args.warmup, args.steps, stage_timing=False)
+        if not args.server_only:
+            print("[bench] FP32 single session, serial", file=sys.stderr, flush=True)
+            fp32 = run_single(nuif, wl, frames, STORAGES["f32"], 0, args.warmup, args.steps, stage_timing=False)
         # 3) the multi-session server
         print(f"[bench] server: {args.sessions} sessions, storage {args.server_storage}", file=sys.stderr, flush=True)
         server = run_server(nuif, wl, frames, args.sessions, STORAGES[args.server_storage], args.warmup, args.steps,
@@ -412,7 +418,7 @@ def main():
         "single_session": {"storage": args.storage, "graph": bool(args.graph), "fps": round(single["fps"], 3),
                            "ms_per_frame": round(single["ms_per_frame"], 4), "tracking_ok": single["ok"],
                            "stage_ms": {k: round(v, 4) for k, v in zip(names, single["stage_avg"])}
-                           if single["stage_avg"] else None},
+                           if single["stage_avg"] else None} if single else None,
         "fp32_serial_fps": round(fp32["fps"], 3) if fp32 else None,
         "server_vs_fp32_serial": round(server["fps"] / fp32["fps"], 3) if server and fp32 else None,
         "roofline": roof,

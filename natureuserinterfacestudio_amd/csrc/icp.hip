@@ -41,6 +41,7 @@
 #include "det_math.h"
 #include "icp_math.h"
 #include "nuif_internal.h"
+#include "schedule.h"
 #include "se3_math.h"
 
 #include <algorithm>
@@ -476,10 +477,11 @@ __device__ __forceinline__ uint32_t open_tiles(const IcpJob& J, uint32_t e, uint
 
 // Every open tile of n sessions (jobs, or `single` when jobs is null).  The sessions'
 // open tiles are numbered one after another (the prefix of their counts in dynamic
-// LDS: n + 1 words) and block b takes the b-th of gridDim.x equal runs of them.
+// LDS: n + 1 words) and block b takes the xcd_block(b)-th of gridDim.x equal runs of
+// them (the b-th when gridDim.x is not a multiple of the XCD count).
 template <int MODE>
 __global__ void __launch_bounds__(kBlock)
-    icp_tiles_kernel(const IcpJob* __restrict__ jobs, int n, IcpJob single, uint32_t planned) {
+    icp_tiles_kernel(const IcpJob* __restrict__ jobs, int n, IcpJob single, uint32_t planned, int remap) {
     __shared__ IcpLds<MODE> L;
     extern __shared__ uint32_t pre[];
     const int tid = threadIdx.x;
@@ -516,8 +518,10 @@ __global__ void __launch_bounds__(kBlock)
     if (tid == 0) pre[n] = carry;
     __syncthreads();
     const uint32_t total = pre[n];
-    const uint32_t f0 = (uint32_t)((unsigned long long)blockIdx.x * total / gridDim.x);
-    const uint32_t f1 = (uint32_t)((unsigned long long)(blockIdx.x + 1) * total / gridDim.x);
+    // runs of consecutive tiles (one session, one band of its image) share an XCD's L2
+    const unsigned lb = remap && gridDim.x % kXcds == 0 ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t f0 = (uint32_t)((unsigned long long)lb * total / gridDim.x);
+    const uint32_t f1 = (uint32_t)((unsigned long long)(lb + 1) * total / gridDim.x);
     int cur = -1;
     for (uint32_t f = f0; f < f1; ++f) {
         // the session of tile f: the last with pre[s] <= f
@@ -965,11 +969,13 @@ void launch_steps(const IcpJob* dj, const IcpJob* hj, int n, hipStream_t st) {
     for (int s = 0; s < n; ++s) steps = std::max(steps, max_epochs(hj[s]));
     const size_t cap = (size_t)resident_tile_blocks<MODE>();
     const size_t lds = (size_t)(n + 1) * sizeof(uint32_t);
+    const int remap = xcd_remap(kXcdIcp) ? 1 : 0;
     for (int k = 0; k < steps; ++k) {
         size_t tiles = 0;
         for (int s = 0; s < n; ++s) tiles += step_tiles(hj[s], k);
-        const unsigned grid = (unsigned)std::max<size_t>(1, std::min(tiles, cap));
-        icp_tiles_kernel<MODE><<<grid, kBlock, lds, st>>>(dj, n, IcpJob{}, kAnyEpoch);
+        unsigned grid = (unsigned)std::max<size_t>(1, std::min(tiles, cap));
+        if (grid >= kXcds) grid -= grid % kXcds;  // whole rounds of the XCDs (see xcd_block)
+        icp_tiles_kernel<MODE><<<grid, kBlock, lds, st>>>(dj, n, IcpJob{}, kAnyEpoch, remap);
         icp_finish_kernel<MODE><<<n, kFinBlock, 0, st>>>(dj, IcpJob{}, kAnyEpoch);
     }
 }
@@ -1000,11 +1006,12 @@ int run_single(const IcpJob& J, hipStream_t st, bool end) {
         const uint32_t e = (uint32_t)(level * kEpochIters + iter);
         const unsigned grid = (unsigned)level_tiles(J.cam, level);
         const size_t lds = 2 * sizeof(uint32_t);
+        const int remap = xcd_remap(kXcdIcp) ? 1 : 0;
         if (J.mode == kRefOrder) {
-            icp_tiles_kernel<kRefOrder><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e);
+            icp_tiles_kernel<kRefOrder><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e, remap);
             icp_finish_kernel<kRefOrder><<<1, kFinBlock, 0, st>>>(nullptr, J, e);
         } else {
-            icp_tiles_kernel<kCompensated><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e);
+            icp_tiles_kernel<kCompensated><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e, remap);
             icp_finish_kernel<kCompensated><<<1, kFinBlock, 0, st>>>(nullptr, J, e);
         }
     };

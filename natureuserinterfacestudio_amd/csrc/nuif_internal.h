@@ -48,6 +48,15 @@ bool initialised();
 
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// The dispatcher hands the workgroups of a 1-D launch to the 8 XCDs in turn (workgroup b
+// to XCD b % 8), and every XCD has an L2 of its own.  For a launch of G workgroups, G a
+// multiple of 8, xcd_block(b, G) gives XCD x the logical blocks [x G/8, (x + 1) G/8): the
+// consecutive blocks of one session, or of one band of an image, then share one L2
+// instead of all eight fetching the same lines.  The map is a bijection of [0, G).
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G) { return (b % kXcds) * (G / kXcds) + b / kXcds; }
+inline unsigned xcd_grid(unsigned blocks) { return div_up(blocks, kXcds) * kXcds; }
+
 // device address of the allocation-free ICP entry point's static scratch on the
 // current device (resolved once per device)
 int static_scratch(void** out);

@@ -529,9 +529,12 @@ __global__ void __launch_bounds__(64)
 
 // all sessions' tiles of one storage: block -> (session, tile x, tile y, z segment)
 template <class S, bool COLOR>
-__global__ void __launch_bounds__(64) integrate_tile_batch_kernel(const IntJob* __restrict__ jobs, int n) {
-    const IntJob& J = jobs[job_of(jobs, n, blockIdx.x)];
-    unsigned b = blockIdx.x - J.block0;
+__global__ void __launch_bounds__(64) integrate_tile_batch_kernel(const IntJob* __restrict__ jobs, int n,
+                                                                   unsigned blocks, int remap) {
+    const unsigned lb = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    if (lb >= blocks) return;
+    const IntJob& J = jobs[job_of(jobs, n, lb)];
+    unsigned b = lb - J.block0;
     const int bx = (int)(b % J.gx);
     b /= J.gx;
     const int by = (int)(b % J.gy), bz = (int)(b / J.gy);
@@ -947,9 +950,12 @@ __global__ void popcount_kernel(const uint32_t* __restrict__ bits, size_t n, uns
 
 // all sessions' 8 x 8 pixel tiles of one storage (volumes without wrap)
 template <class S>
-__global__ void __launch_bounds__(64) raycast_batch_kernel(const RayJob* __restrict__ jobs, int n) {
-    const RayJob& J = jobs[job_of(jobs, n, blockIdx.x)];
-    const unsigned b = blockIdx.x - J.block0;
+__global__ void __launch_bounds__(64) raycast_batch_kernel(const RayJob* __restrict__ jobs, int n, unsigned blocks,
+                                                            int remap) {
+    const unsigned lb = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    if (lb >= blocks) return;
+    const RayJob& J = jobs[job_of(jobs, n, lb)];
+    const unsigned b = lb - J.block0;
     raycast_body<S, false>(in_global(reinterpret_cast<const typename S::word*>(J.vol)), in_global(J.cvol), J.p, J.cam,
                            in_global(J.pose), in_global(J.vmap), in_global(J.nmap), in_global(J.cmap),
                            in_global(J.enable), in_global(J.surf), VoxelWrap{0, 0, 0}, (int)(b % J.gx),
@@ -1056,18 +1062,19 @@ int launch_integrate_batch(const IntJob* jobs, const IntJob* hjobs, int n, int s
         NUIF_LAUNCHED("column_states_batch_kernel");
     }
     const unsigned blocks = last.block0 + last.gx * last.gy * last.gz;
+    const int remap = xcd_remap(kXcdIntegrate) ? 1 : 0;
     switch (storage) {
         case NUIF_TSDF_S16:
-            if (color) integrate_tile_batch_kernel<StS16, true><<<blocks, 64, 0, st>>>(jobs, n);
-            else integrate_tile_batch_kernel<StS16, false><<<blocks, 64, 0, st>>>(jobs, n);
+            if (color) integrate_tile_batch_kernel<StS16, true><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap);
+            else integrate_tile_batch_kernel<StS16, false><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap);
             break;
         case NUIF_TSDF_F16:
-            if (color) integrate_tile_batch_kernel<StF16, true><<<blocks, 64, 0, st>>>(jobs, n);
-            else integrate_tile_batch_kernel<StF16, false><<<blocks, 64, 0, st>>>(jobs, n);
+            if (color) integrate_tile_batch_kernel<StF16, true><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap);
+            else integrate_tile_batch_kernel<StF16, false><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap);
             break;
         default:
-            if (color) integrate_tile_batch_kernel<StF32, true><<<blocks, 64, 0, st>>>(jobs, n);
-            else integrate_tile_batch_kernel<StF32, false><<<blocks, 64, 0, st>>>(jobs, n);
+            if (color) integrate_tile_batch_kernel<StF32, true><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap);
+            else integrate_tile_batch_kernel<StF32, false><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap);
             break;
     }
     NUIF_LAUNCHED("integrate_tile_batch_kernel");
@@ -1078,10 +1085,11 @@ int launch_raycast_batch(const RayJob* jobs, const RayJob* hjobs, int n, int sto
     if (n <= 0) return NUIF_OK;
     const RayJob& last = hjobs[n - 1];
     const unsigned blocks = last.block0 + last.gx * div_up(last.cam.depthImageHeight, 8);
+    const int remap = xcd_remap(kXcdRaycast) ? 1 : 0;
     switch (storage) {
-        case NUIF_TSDF_S16: raycast_batch_kernel<StS16><<<blocks, 64, 0, st>>>(jobs, n); break;
-        case NUIF_TSDF_F16: raycast_batch_kernel<StF16><<<blocks, 64, 0, st>>>(jobs, n); break;
-        default: raycast_batch_kernel<StF32><<<blocks, 64, 0, st>>>(jobs, n); break;
+        case NUIF_TSDF_S16: raycast_batch_kernel<StS16><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap); break;
+        case NUIF_TSDF_F16: raycast_batch_kernel<StF16><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap); break;
+        default: raycast_batch_kernel<StF32><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap); break;
     }
     NUIF_LAUNCHED("raycast_batch_kernel");
     return NUIF_OK;

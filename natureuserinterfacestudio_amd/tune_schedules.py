@@ -14,6 +14,11 @@ and the segment rules of schedule.h.
     python -m natureuserinterfacestudio_amd.tune_schedules build   # here: variant libraries
     python -m natureuserinterfacestudio_amd.tune_schedules run     # on the GPU: sweep, table
     python -m natureuserinterfacestudio_amd.tune_schedules clean   # drop the variants
+    python -m natureuserinterfacestudio_amd.tune_schedules xcd     # on the GPU: XCD numbering sweep
+
+The XCD sweep forces each mask of XCD-aware block numbering (NUIF_XCD_REMAP: bit 0 ICP
+tiles, bit 1 batched integration, bit 2 batched raycast) for the 32-session server round
+and writes profiles/xcd_sweep_server_32x640x480_512.md.
 
 The variants live in natureuserinterfacestudio_amd/lib/tune/<name>/ (all three
 libraries, the kernel library rebuilt with the candidate's -D values); bench.py loads
@@ -45,9 +50,14 @@ CANDIDATES = {
 # session and server rounds of SEG_SESSIONS sessions
 SEGMENTS = (64, 128, 256)
 SEG_SESSIONS = (8, 32)
+# XCD-aware numbering masks, forced at run time (NUIF_XCD_REMAP); each mask is run twice
+# (interleaved) so that the spread between runs shows beside the differences
+XCD_MASKS = (0, 1, 2, 4, 7)
+XCD_PROFILE = os.path.join(ROOT, "profiles", "xcd_sweep_server_32x640x480_512.md")
+XCD_ARGS = ["--server-only", "--steps", "20", "--warmup", "5", "--no-cpu", "--c5-sessions", "0", "--hash-steps", "0"]
 # sources whose code depends on the schedule
 SCHEDULED = ["tsdf.hip", "fusion.hip"]
-BENCH_ARGS = ["--steps", "10", "--warmup", "3", "--no-cpu", "--c5-sessions", "0"]
+BENCH_ARGS = ["--steps", "10", "--warmup", "3", "--no-cpu", "--c5-sessions", "0", "--hash-steps", "0"]
 
 
 def build(names=None):
@@ -90,8 +100,8 @@ def run_segments():
         for seg in SEGMENTS:
             print(f"[tune] segment {seg}, {sessions} sessions", flush=True)
             b = _bench(dict(os.environ, NUIF_SEG_VOXELS=str(seg)),
-                       ["--steps", "10", "--warmup", "3", "--no-cpu", "--c5-sessions", "0", "--sessions",
-                        str(sessions)])
+                       ["--steps", "10", "--warmup", "3", "--no-cpu", "--c5-sessions", "0", "--hash-steps", "0",
+                        "--sessions", str(sessions)])
             rows.append({"seg": seg, "sessions": sessions, "server_fps": b["value"],
                          "server_integrate_ms": b["server"]["stages"]["avg_ms"]["integrate"],
                          "single_integrate_ms": b["single_session"]["stage_ms"]["integrate"],
@@ -159,16 +169,55 @@ def write_table(rows, segs):
     print(f"[tune] table written to {PROFILE}; best: {best['name']}", flush=True)
 
 
+def run_xcd():
+    """XCD-aware block numbering per stage (GPU, the in-tree libraries): the server round's
+    stage times and frames/s for each mask, two interleaved passes."""
+    rows = []
+    for rep in range(2):
+        for mask in XCD_MASKS:
+            print(f"[tune] xcd mask {mask} (pass {rep + 1})", flush=True)
+            b = _bench(dict(os.environ, NUIF_XCD_REMAP=str(mask)), XCD_ARGS)
+            st = b["server"]["stages"]["avg_ms"]
+            rows.append({"mask": mask, "pass": rep + 1, "fps": b["value"], "p99": b["p99_frame_latency_ms"],
+                         "icp": st["icp"], "integrate": st["integrate"], "raycast": st["raycast"]})
+    names = {1: "ICP tiles", 2: "integration", 4: "raycast"}
+    lines = ["# XCD-aware block numbering: 32-session server round, 640x480 / 512^3", "",
+             "Command (on the GPU): `python -m natureuserinterfacestudio_amd.tune_schedules xcd`; each mask runs "
+             f"`bench.py {' '.join(XCD_ARGS)}` with NUIF_XCD_REMAP set, every mask twice in interleaved passes.", "",
+             "Mask bits: 1 the ICP tile runs, 2 the batched integration, 4 the batched raycast (consecutive blocks "
+             "of a session on one XCD); 0 the dispatcher's own round-robin order.  Stage times are device times of "
+             "the batched round (events on the server stream).", "",
+             "| mask | stages renumbered | pass | frames/s | p99 ms | ICP ms | integration ms | raycast ms |",
+             "|---|---|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: (r["mask"], r["pass"])):
+        on = ", ".join(v for k, v in names.items() if r["mask"] & k) or "none"
+        lines.append(f"| {r['mask']} | {on} | {r['pass']} | {r['fps']:.1f} | {r['p99']:.3f} | {r['icp']:.4f} | "
+                     f"{r['integrate']:.4f} | {r['raycast']:.4f} |")
+    lines.append("")
+    for stage, bit in (("icp", 1), ("integrate", 2), ("raycast", 4)):
+        on = [r[stage] for r in rows if r["mask"] & bit]
+        off = [r[stage] for r in rows if not r["mask"] & bit]
+        lines.append(f"{stage}: renumbered {min(on):.4f}-{max(on):.4f} ms, dispatcher order "
+                     f"{min(off):.4f}-{max(off):.4f} ms.")
+    with open(XCD_PROFILE, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(f"[tune] table written to {XCD_PROFILE}", flush=True)
+    return rows
+
+
 def clean():
     shutil.rmtree(TUNE, ignore_errors=True)
 
 
 def main(argv):
-    if not argv or argv[0] not in ("build", "run", "clean"):
+    if not argv or argv[0] not in ("build", "run", "clean", "xcd"):
         print(__doc__)
         return 2
+    if argv[0] in ("clean", "xcd"):
+        {"clean": clean, "xcd": run_xcd}[argv[0]]()
+        return 0
     names = argv[1:] or None
-    {"build": build, "run": run, "clean": clean}[argv[0]](*([names] if argv[0] != "clean" else []))
+    {"build": build, "run": run}[argv[0]](names)
     return 0
 
 
