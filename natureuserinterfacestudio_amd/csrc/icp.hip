@@ -30,8 +30,9 @@
 //                then the LM step: float sums, Eigen-like LLT in FP64, float Euler
 //                increment.  Bit for bit the reference's arithmetic.
 //   compensated  thread j of tile k takes pixels 512k + j + 256i (i < 2) with
-//                Kahan-compensated FP32 sums, FP64 across threads, tiles and
-//                segments in a fixed order; Gauss-Newton on the undamped normal
+//                Kahan-compensated FP32 sums, FP64 across threads (runs of 16
+//                threads in order, then a 16-way tree), tiles and segments in a
+//                fixed order; Gauss-Newton on the undamped normal
 //                equations (FP64 LDL^T), increment applied as an SE(3)
 //                exponential to an FP64 pose (se3_math.h).
 // Every sum is taken in a fixed order: results do not depend on which block
@@ -202,7 +203,6 @@ __device__ __forceinline__ void icp_terms(const f3 vc, const TilePose& P, const 
     out[28] = 1.0f;
 }
 
-__device__ __forceinline__ double wave_tree64d(double v) { return wave_tree64_t(v); }
 
 // ---- the updates ------------------------------------------------------------------
 
@@ -383,8 +383,13 @@ template <int MODE> struct IcpLds;
 template <> struct IcpLds<kRefOrder> : IcpLdsBase {
     float xfer[2][64 * kTerms];  // the tree's d = 64 step of two groups
 };
+// compensated: the threads' FP64 values of up to kXtTerms terms at a time, term-major,
+// one double of padding after every 16 threads (conflict-free 16-thread runs)
+constexpr int kXtTerms = (kTerms + 1) / 2;  // two passes of 15 and 14 terms
+constexpr int kXtRow = kBlock + kBlock / 16;
+__device__ __forceinline__ int xt_index(int term, int thread) { return term * kXtRow + thread + (thread >> 4); }
 template <> struct IcpLds<kCompensated> : IcpLdsBase {
-    double wsum[4][kTerms];  // wave sums
+    double xt[kXtTerms * kXtRow];
 };
 
 // Tile `tile` of level L.level of job L.job (with L.pose loaded): the 29 terms of its
@@ -454,17 +459,31 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
             kahan_add(acc[27], cmp[27], cf[i][6] * cf[i][6]);
             acc[28] += 1.0f;
         }
-        const int lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-        for (int k = 0; k < kTerms; ++k) {
-            const double v = wave_tree64d((double)acc[k] - (double)cmp[k]);
-            if (lane == 0) L.wsum[wv][k] = v;
-        }
-        __syncthreads();
+        // across the tile's threads, a pass of up to kXtTerms terms at a time through
+        // LDS: 16 threads per term, thread q of a term summing the values of threads
+        // 16q .. 16q + 15 in order, then a tree over the 16 (row DPP, lane l takes
+        // lane l + d for d = 8, 4, 2, 1)
         double* part = in_global(reinterpret_cast<double*>(J.partials));
-        if (tid < kTerms)
-            part[(size_t)tile * kTerms + tid] = ((L.wsum[0][tid] + L.wsum[1][tid]) + L.wsum[2][tid]) + L.wsum[3][tid];
-        __syncthreads();  // wsum is reused by the next tile
+        const int q = tid & 15;
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+            const int t0 = pass * kXtTerms, nt = pass ? kTerms - kXtTerms : kXtTerms;
+#pragma unroll
+            for (int k = 0; k < nt; ++k)
+                L.xt[xt_index(k, tid)] = (double)acc[t0 + k] - (double)cmp[t0 + k];
+            __syncthreads();
+            const int tl = min(tid >> 4, nt - 1);  // rows past the pass's terms repeat its last
+            const double* row = &L.xt[xt_index(tl, 16 * q)];
+            double v = row[0];
+#pragma unroll
+            for (int k = 1; k < 16; ++k) v += row[k];
+            v += lane_from_above<8>(v);
+            v += lane_from_above<4>(v);
+            v += lane_from_above<2>(v);
+            v += lane_from_above<1>(v);
+            if (q == 0 && (tid >> 4) < nt) part[(size_t)tile * kTerms + t0 + tl] = v;
+            __syncthreads();  // xt is rewritten by the next pass or tile
+        }
     }
 }
 

@@ -466,9 +466,10 @@ IcpResult icp(const Cam<T>& cam, const IcpConfig& cfg, const T* const* vpyr, con
 // The GPU's compensated reduction of one iteration (nuif ICP kernel): tiles of T =
 // kCompTilePixels pixels; in tile k thread j (256 threads) takes pixels Tk + j + 256i,
 // i = 0..T/256-1, with Kahan-compensated FP32 sums (the count plainly); each thread's value s - c
-// in double; FP64 tree over each 64-lane wave (lane l takes lane l + d for d = 32,
-// 16, .., 1), the four wave sums added in order; then term by term the tile sums
-// k = j, j + 8, j + 16, .. in order for j = 0..7, and those eight in order.
+// in double; per term, the values of threads 16g .. 16g + 15 added in thread order
+// (g = 0..15), then a tree over the 16 run sums (item l takes item l + d for d = 8, 4,
+// 2, 1); then term by term the tile sums k = j, j + 8, j + 16, .. in order for
+// j = 0..7, and those eight in order.
 inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int npx, const float R[9], const float t[3],
                                  const float* vref, const float* nref, const Rigid<float>& prev, float distThr,
                                  double out[kTerms]) {
@@ -497,15 +498,15 @@ inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int n
             for (int q = 0; q < kTerms; ++q) v[(size_t)j * kTerms + q] = (double)sm[q] - (double)cp[q];
         }
         for (int q = 0; q < kTerms; ++q) {
-            double w[4];
-            for (int wv = 0; wv < 4; ++wv) {
-                double lane[64];
-                for (int l = 0; l < 64; ++l) lane[l] = v[(size_t)(64 * wv + l) * kTerms + q];
-                for (int d = 32; d > 0; d >>= 1)
-                    for (int l = 0; l < d; ++l) lane[l] = lane[l] + lane[l + d];
-                w[wv] = lane[0];
+            double run[16];
+            for (int g = 0; g < 16; ++g) {
+                double sacc = v[(size_t)(16 * g) * kTerms + q];
+                for (int j = 1; j < 16; ++j) sacc += v[(size_t)(16 * g + j) * kTerms + q];
+                run[g] = sacc;
             }
-            tiles[(size_t)k * kTerms + q] = ((w[0] + w[1]) + w[2]) + w[3];
+            for (int d = 8; d > 0; d >>= 1)
+                for (int l = 0; l < d; ++l) run[l] = run[l] + run[l + d];
+            tiles[(size_t)k * kTerms + q] = run[0];
         }
     }
     for (int q = 0; q < kTerms; ++q) {
