@@ -46,6 +46,7 @@
 #include "se3_math.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <mutex>
 
 namespace nuif {
@@ -385,6 +386,29 @@ template <> struct IcpLds<kRefOrder> : IcpLdsBase {
 };
 // compensated: the threads' FP64 values of up to kXtTerms terms at a time, term-major,
 // one double of padding after every 16 threads (conflict-free 16-thread runs)
+// product term k (k < 28) is c[a] * c[b] for the k-th pair a <= b < 7 with a < 6, then
+// (6, 6) (K/icp.cl:271-289 order)
+constexpr int term_a(int k) {
+    int a = 0;
+    while (a < 6 && k >= 7 - a) k -= 7 - a++;
+    return a;
+}
+constexpr int term_b(int k) {
+    int a = 0;
+    while (a < 6 && k >= 7 - a) k -= 7 - a++;
+    return a + k;
+}
+static_assert(term_a(6) == 0 && term_b(6) == 6 && term_a(7) == 1 && term_b(7) == 1 && term_a(26) == 5 &&
+                  term_b(26) == 6 && term_a(27) == 6 && term_b(27) == 6,
+              "term pairs");
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
 constexpr int kXtTerms = (kTerms + 1) / 2;  // two passes of 15 and 14 terms
 constexpr int kXtRow = kBlock + kBlock / 16;
 __device__ __forceinline__ int xt_index(int term, int thread) { return term * kXtRow + thread + (thread >> 4); }
@@ -433,9 +457,7 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
         }
     } else {
         // compensated: the thread's pixels in order, their gathers in flight together
-        float acc[kTerms], cmp[kTerms];
-#pragma unroll
-        for (int k = 0; k < kTerms; ++k) acc[k] = cmp[k] = 0.f;
+        static_assert(kPxPerThread == 2, "the folded Kahan sums below take two pixels");
         const int p0 = (int)tile * kTilePx + tid;
         float cf[kPxPerThread][7];
         bool ok[kPxPerThread];
@@ -445,32 +467,34 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
             const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
             ok[i] = icp_assoc(vc, L.pose, J.cam, vref, nref, J.dist, cf[i]);
         }
-#pragma unroll
-        for (int i = 0; i < kPxPerThread; ++i) {
-            if (!ok[i]) continue;
-            int s = 0;
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int b = a; b < 7; ++b) {
-                    kahan_add(acc[s], cmp[s], cf[i][a] * cf[i][b]);
-                    ++s;
-                }
-            kahan_add(acc[27], cmp[27], cf[i][6] * cf[i][6]);
-            acc[28] += 1.0f;
-        }
+        // Kahan sums of the two pixels from zero, folded: after the first pixel the sum is
+        // 0 + v0 (never -0) and the compensation exactly +0, so the second step is
+        // s = s1 + v1, c = (s - s1) - v1; a rejected pixel adds +0, which changes neither.
+        // Each term is formed as it goes to LDS: only the 14 coefficients stay live.
+        auto term = [&](auto kc) -> double {
+            constexpr int k = decltype(kc)::value;
+            if constexpr (k < kTerms - 1) {
+                constexpr int a = term_a(k), b = term_b(k);
+                const float v0 = ok[0] ? cf[0][a] * cf[0][b] : 0.f;
+                const float v1 = ok[1] ? cf[1][a] * cf[1][b] : 0.f;
+                const float s1 = 0.f + v0;
+                const float sm = s1 + v1;
+                return (double)sm - (double)((sm - s1) - v1);
+            } else {  // the count, summed plainly
+                return (double)((0.f + (ok[0] ? 1.f : 0.f)) + (ok[1] ? 1.f : 0.f));
+            }
+        };
         // across the tile's threads, a pass of up to kXtTerms terms at a time through
         // LDS: 16 threads per term, thread q of a term summing the values of threads
         // 16q .. 16q + 15 in order, then a tree over the 16 (row DPP, lane l takes
         // lane l + d for d = 8, 4, 2, 1)
         double* part = in_global(reinterpret_cast<double*>(J.partials));
         const int q = tid & 15;
-#pragma unroll
-        for (int pass = 0; pass < 2; ++pass) {
-            const int t0 = pass * kXtTerms, nt = pass ? kTerms - kXtTerms : kXtTerms;
-#pragma unroll
-            for (int k = 0; k < nt; ++k)
-                L.xt[xt_index(k, tid)] = (double)acc[t0 + k] - (double)cmp[t0 + k];
+        static_for<0, 2>([&](auto pc) {
+            constexpr int pass = decltype(pc)::value;
+            constexpr int t0 = pass * kXtTerms, nt = pass ? kTerms - kXtTerms : kXtTerms;
+            static_for<0, nt>(
+                [&](auto kc) { L.xt[xt_index(decltype(kc)::value, tid)] = term(std::integral_constant<int, t0 + decltype(kc)::value>{}); });
             __syncthreads();
             const int tl = min(tid >> 4, nt - 1);  // rows past the pass's terms repeat its last
             const double* row = &L.xt[xt_index(tl, 16 * q)];
@@ -483,7 +507,7 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
             v += lane_from_above<1>(v);
             if (q == 0 && (tid >> 4) < nt) part[(size_t)tile * kTerms + t0 + tl] = v;
             __syncthreads();  // xt is rewritten by the next pass or tile
-        }
+        });
     }
 }
 
