@@ -409,7 +409,8 @@ __device__ __forceinline__ void static_for(F&& f) {
         static_for<I + 1, N>(f);
     }
 }
-constexpr int kXtTerms = (kTerms + 1) / 2;  // two passes of 15 and 14 terms
+constexpr int kXtPasses = 3;                                      // passes of 10, 10 and 9 terms:
+constexpr int kXtTerms = (kTerms + kXtPasses - 1) / kXtPasses;  // 22 KB, five blocks per CU
 constexpr int kXtRow = kBlock + kBlock / 16;
 __device__ __forceinline__ int xt_index(int term, int thread) { return term * kXtRow + thread + (thread >> 4); }
 template <> struct IcpLds<kCompensated> : IcpLdsBase {
@@ -490,9 +491,9 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
         // lane l + d for d = 8, 4, 2, 1)
         double* part = in_global(reinterpret_cast<double*>(J.partials));
         const int q = tid & 15;
-        static_for<0, 2>([&](auto pc) {
-            constexpr int pass = decltype(pc)::value;
-            constexpr int t0 = pass * kXtTerms, nt = pass ? kTerms - kXtTerms : kXtTerms;
+        static_for<0, kXtPasses>([&](auto pc) {
+            constexpr int t0 = decltype(pc)::value * kXtTerms;
+            constexpr int nt = kTerms - t0 < kXtTerms ? kTerms - t0 : kXtTerms;
             static_for<0, nt>(
                 [&](auto kc) { L.xt[xt_index(decltype(kc)::value, tid)] = term(std::integral_constant<int, t0 + decltype(kc)::value>{}); });
             __syncthreads();
