@@ -122,12 +122,12 @@ def run_server(nuif, wl, frames, sessions, storage, warmup, rounds, budget):
 
 
 def run_hash(nuif, wl, warmup, steps):
-    """One voxel-hashing session (5 mm voxels in 8^3 FP16 + uint16 bricks) at the
-    workload's resolution, frames back to back with its stages event-timed: frames/s,
+    """One voxel-hashing session (5 mm voxels in 8^3 FP16 + uint16 bricks, 2^20 buckets) at
+    the workload's resolution, frames back to back with its stages event-timed: frames/s,
     stage ms and the integration stage against HBM peak (each visible brick read and
     written whole, 2 x 2 KiB + 16 B of entry per brick, plus the 4 W H bytes of depth)."""
     frames = [nuif.DeviceArray.from_host(wl.render(f)[0]) for f in range(warmup + steps)]
-    eng = nuif.Engine(wl.kinfu_desc(storage=STORAGES["f16"], gate=0.0))
+    eng = nuif.Engine(wl.kinfu_desc(storage=STORAGES["f16"], gate=0.0, bricks=1 << 18, buckets=1 << 20))
     for f in range(warmup):
         eng.run(frames[f], sync=False)
     eng.synchronize()
@@ -142,6 +142,9 @@ def run_hash(nuif, wl, warmup, steps):
     ok = eng.synchronize()
     timed, ms = eng.take_stage_times()
     stats = eng.hash_stats()
+    t0 = time.perf_counter()
+    pts, idx = eng.mesh_shape()  # Volume2Mesh over the visible list: marching cubes + host copy
+    mesh_ms = (time.perf_counter() - t0) * 1e3
     eng.close()
     avg = [x / max(timed, 1) for x in ms]
     names = ["preprocess", "icp", "integrate", "raycast"]
@@ -151,7 +154,8 @@ def run_hash(nuif, wl, warmup, steps):
     return {"workload": wl.name, "frames": steps, "fps": round(1e3 / device_ms, 3),
             "ms_per_frame": round(device_ms, 4), "wall_fps": round(steps / wall, 3), "tracking_ok": bool(ok),
             "stage_ms": {k: round(v, 4) for k, v in zip(names, avg)}, "visible_bricks_per_frame": round(vis, 1),
-            "bricks_in_use": stats[0], "failed_insertions": stats[2],
+            "bricks_in_use": stats[0], "failed_insertions": stats[2], "buckets": 1 << 20,
+            "mesh_wall_ms": round(mesh_ms, 2), "mesh_triangles": len(idx) // 3, "mesh_points": len(pts),
             "integrate_roofline": {"bound": "hbm", "algorithmic_bytes": int(int_bytes),
                                    "achieved": round(int_bytes / (avg[2] / 1e3) / 1e9, 2), "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": round(int_bytes / (avg[2] / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}}

@@ -56,6 +56,9 @@ typedef struct nuifk_desc {
     int voxel_shift;
     float volume_basis[3];
     int hash_swapping;              /* m_bUseSwapping: host swapping of hash bricks (no graph replay) */
+    int hash_buckets;               /* m_hashNumBuckets: the table has at least this many buckets of 4
+                                     * entries, and enough for a load of at most 1/2 of hash_bricks
+                                     * (0: the reference's default 512 x 512) */
 } nuifk_desc;
 
 #define NUIFK_SCENE_DENSE 0         /* OCL/NuiKinfuOpenCLScene: dense TSDF volume */

@@ -128,7 +128,7 @@ NuiHashingSDFConfig::NuiHashingSDFConfig()
 
 int NuiHashingSDFConfig::bucketsLog2() const {
     int b = 4;
-    while (b < 24 && ((size_t)4 << b) < (size_t)2 * m_numSDFBlocks) ++b;
+    while (b < 24 && (((size_t)4 << b) < (size_t)2 * m_numSDFBlocks || ((size_t)1 << b) < m_hashNumBuckets)) ++b;
     return b;
 }
 

@@ -53,10 +53,11 @@ struct NuiKinfuVolumeConfig {
     bool log(const std::string& fileName) const;
 };
 
-// VO/NuiHashingSDFConfig.cpp:3-15.  The table here has 2^b buckets of 4 entries
-// sized from m_numSDFBlocks (load <= 1/2); m_hashNumBuckets and
-// m_hashMaxCollisionLinkedListSize describe the reference's table and are kept
-// for the configuration interface only.
+// VO/NuiHashingSDFConfig.cpp:3-15.  The table here has 2^b buckets of 4 entries, the
+// smallest power of two with at least m_hashNumBuckets buckets and a load of at most
+// 1/2 for m_numSDFBlocks bricks; a full bucket probes the next ones instead of the
+// reference's linked excess list, so m_hashMaxCollisionLinkedListSize is kept for
+// the configuration interface only.
 struct NuiHashingSDFConfig {
     UINT m_hashNumBuckets;
     UINT m_numSDFBlocks;

@@ -113,15 +113,24 @@ public:
     virtual bool Volume2CLVertices(NuiKinfuMesh* pData) { return false; }
     virtual bool Volume2CLMesh(NuiKinfuMesh* pData) = 0;
     virtual bool Volume2Mesh(NuiKinfuMesh* pMesh) = 0;
-    // DS/NuiKinfuScene.h:43: the shared-vertex mesh as a NuiMeshShape (points, then triangle
-    // indices), appended to pMesh
+    // DS/NuiKinfuScene.h:43: the scene's mesh as a NuiMeshShape (points, then triangle
+    // indices), appended to pMesh.  A scene whose mesh has shared vertices (the dense
+    // volume) gives its indices; a triangle soup (the hash scene, whose reference
+    // Volume2Mesh(NuiMeshShape*) appends nothing) gets indices 0, 1, 2, ... so that every
+    // triangle of the marching cubes is in the shape.
     virtual bool Volume2Mesh(NuiMeshShape* pMesh) {
         if (!pMesh) return false;
         NuiKinfuMesh m;
         if (!Volume2Mesh(&m)) return false;
+        const int base = (int)pMesh->pointsNum();
         for (size_t i = 0; i + 2 < m.positions.size(); i += 3)
             pMesh->appendPoint(SgVec3f(m.positions[i], m.positions[i + 1], m.positions[i + 2]));
-        for (uint32_t k : m.indices) pMesh->appendTriangleIndex((int)k);
+        if (!m.indices.empty()) {
+            for (uint32_t k : m.indices) pMesh->appendTriangleIndex(base + (int)k);
+        } else {
+            const int n = (int)(m.positions.size() / 9) * 3;
+            for (int k = 0; k < n; ++k) pMesh->appendTriangleIndex(base + k);
+        }
         return true;
     }
 

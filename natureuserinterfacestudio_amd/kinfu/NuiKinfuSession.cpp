@@ -79,6 +79,7 @@ NuiHashingSDFConfig NuiKinfuSession::hashConfig(const nuifk_desc& d) {
     c.m_bUseForwardRender = d.hash_forward_render != NUIFK_RENDER_CASTRAY;
     c.m_bForwardReferenceProjection = d.hash_forward_render == NUIFK_RENDER_FORWARD_REFERENCE;
     c.m_bUseSwapping = d.hash_swapping != 0;
+    if (d.hash_buckets > 0) c.m_hashNumBuckets = (UINT)d.hash_buckets;
     return c;
 }
 
@@ -93,7 +94,10 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
     } else {  // dense or shifting volume
         const size_t voxels = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
         if (d.scene == NUIFK_SCENE_SHIFT) bytes += ((size_t)d.resolution[0] * d.resolution[1] + 64) * 4 + 65536;
-        bytes += (size_t)std::max(0, (d.resolution[2] + 127) / 128 - 1) * ((size_t)d.resolution[0] * d.resolution[1] + 1) * 8 + 16;
+        TsdfParams tp{};
+        if (nuif_tsdf_params(d.resolution[0], d.resolution[1], d.resolution[2], d.dimensions[0], d.dimensions[1],
+                             d.dimensions[2], d.tranc_dist, &tp) == NUIF_OK)
+            bytes += nuif_tsdf_walk_states_bytes(&tp);
         bytes += voxels * nuif_tsdf_voxel_bytes(d.storage) + (d.color ? voxels * 4 : 0);
         bytes += (size_t)((d.resolution[0] + 7) / 8) * ((d.resolution[1] + 7) / 8) * ((d.resolution[2] + 7) / 8) +
                  (size_t)((d.resolution[0] + 31) / 32) * ((d.resolution[1] + 31) / 32) * ((d.resolution[2] + 31) / 32);

@@ -82,7 +82,7 @@ class KinfuDesc(C.Structure):
                 ("use_graph", c_int), ("icp_reduction", c_int), ("scene", c_int), ("hash_voxel_size", c_float),
                 ("hash_trunc_scale", c_float), ("hash_truncation", c_float), ("hash_weight_max", c_int),
                 ("hash_bricks", c_int), ("hash_forward_render", c_int), ("voxel_shift", c_int),
-                ("volume_basis", c_float * 3), ("hash_swapping", c_int)]
+                ("volume_basis", c_float * 3), ("hash_swapping", c_int), ("hash_buckets", c_int)]
 
 
 SCENE_DENSE, SCENE_HASH, SCENE_SHIFT = 0, 1, 2
@@ -1001,9 +1001,10 @@ class HashScene:
 
 def hash_params_of_desc(d):
     """The nuif_hash_params a hash session builds from its descriptor (buckets: the
-    smallest 2^b with 4 * 2^b >= 2 * bricks)."""
+    smallest 2^b with 2^b >= hash_buckets (0: 512 x 512) and 4 * 2^b >= 2 * bricks)."""
+    want = d.hash_buckets if d.hash_buckets > 0 else 512 * 512
     b = 4
-    while b < 24 and (4 << b) < 2 * d.hash_bricks:
+    while b < 24 and ((4 << b) < 2 * d.hash_bricks or (1 << b) < want):
         b += 1
     return hash_params(voxel=d.hash_voxel_size, trunc_scale=d.hash_trunc_scale, truncation=d.hash_truncation,
                        weight_max=d.hash_weight_max, buckets_log2=b, bricks=d.hash_bricks, storage=d.storage,

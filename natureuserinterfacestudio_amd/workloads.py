@@ -86,7 +86,8 @@ class Workload:
         return nuif.camera_params(self.width, self.height, self.fx, self.fy, self.cx, self.cy, 0.5, 4.5)
 
     def kinfu_desc(self, storage=nuif.TSDF_S16, gate=0.0, color=False, use_graph=False, iterations=None,
-                   reduction=None, bricks=1 << 18, forward=0, shift=0, basis=(0.0, 0.0, 0.0), swapping=False):
+                   reduction=None, bricks=1 << 18, forward=0, shift=0, basis=(0.0, 0.0, 0.0), swapping=False,
+                   buckets=0):
         d = nuif.KinfuDesc()
         d.width, d.height = self.width, self.height
         d.fx, d.fy, d.cx, d.cy = self.fx, self.fy, self.cx, self.cy
@@ -124,6 +125,7 @@ class Workload:
         d.hash_voxel_size, d.hash_trunc_scale, d.hash_truncation = 0.005, self.trunc, 0.06
         d.hash_weight_max = 255
         d.hash_bricks = bricks
+        d.hash_buckets = buckets
         d.hash_forward_render = forward
         return d
 
