@@ -741,6 +741,28 @@ __device__ __forceinline__ void superblock_span(const int g[3], const int N[3], 
     *u_out = hi_t;
 }
 
+// The time after the march's steps t <- fl(t + step) that stay inside [t, s_out] and
+// below max_t, in one operation.  With t in the binade [2^e, E), E = 2^(e+1), of ulp
+// u = 2^(e-23), every sum t + step that stays at or below E - u rounds to a multiple of u
+// with the same offset: fl(t + step) = t + d, d = fl(t + step) - t (exact), unless step
+// lies exactly half way between two multiples of u (then the rounding to even alternates
+// and no jump is made).  So m steps give exactly t + m d while t + m d <= E - 2u; m is
+// taken one short of the float estimate and the result checked, so a rounding of the
+// estimate only ever shortens the jump.
+__device__ __forceinline__ float march_jump(float t, float step, float s_out, float max_t) {
+    const unsigned ex = __float_as_uint(t) & 0x7f800000u;
+    if (!(t > 0.f) || ex < (24u << 23) || ex >= (254u << 23)) return t;
+    const float d = (t + step) - t;
+    const float u = __uint_as_float(ex - (23u << 23));
+    const float top = __uint_as_float(ex + (1u << 23)) - 2.f * u;
+    if (fabsf(step - d) == 0.5f * u) return t;
+    const float lim = fminf(fminf(s_out, top), max_t);
+    const float m = floorf((lim - t) * __builtin_amdgcn_rcpf(d)) - 1.f;
+    if (!(m >= 1.f)) return t;
+    const float tm = t + m * d;
+    return (tm <= s_out && tm <= top && tm < max_t) ? tm : t;
+}
+
 // superblock flags kept in LDS when the volume has at most this many superblocks
 constexpr int kSuperLds = 4096;
 
@@ -834,6 +856,8 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
                         }
                     }
                     tn = un;
+                    // further steps inside the clear span only advance the time
+                    if (tn >= s_in && tn <= s_out) tn = march_jump(tn, step, s_out, max_t);
                 }
                 if (tn != tcur) {
                     int gs[3];
