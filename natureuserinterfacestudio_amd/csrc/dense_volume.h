@@ -69,6 +69,14 @@ struct SurfaceBlocks {
     __device__ __forceinline__ size_t sindex(int x, int y, int z) const {
         return nb + ((size_t)(y >> 5) * nsx + (x >> 5)) * nsz + (z >> 5);
     }
+    // the same in 32-bit arithmetic (at most 512^3 blocks: 4096 voxels a side)
+    __device__ __forceinline__ unsigned index32(int x, int y, int z) const {
+        return ((unsigned)(y >> 3) * (unsigned)nbx + (unsigned)(x >> 3)) * (unsigned)nbz + (unsigned)(z >> 3);
+    }
+    __device__ __forceinline__ unsigned sindex32(int x, int y, int z) const {
+        return (unsigned)nb + ((unsigned)(y >> 5) * (unsigned)nsx + (unsigned)(x >> 5)) * (unsigned)nsz +
+               (unsigned)(z >> 5);
+    }
     __device__ __forceinline__ void mark(int x, int y, int z) const {
         b[index(x, y, z)] = 1;
         b[sindex(x, y, z)] = 1;

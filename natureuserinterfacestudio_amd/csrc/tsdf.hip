@@ -30,6 +30,8 @@
 #include "dense_volume.h"
 #include "schedule.h"
 
+#include <algorithm>
+
 namespace nuif {
 
 constexpr int kMaxWeight = 128;  // K/tsdfVolume.cl:4
@@ -600,7 +602,8 @@ __global__ void integrate_column_kernel(const float* __restrict__ depth, const u
 
 // COUNT: every voxel read is also marked in the bitset `mark` (the footprint
 // measurement of nuif_tsdf_raycast_footprint; the raycast kernels do not count)
-template <class S, bool WRAP, bool COUNT = false> struct VolView {
+// I32: the volume has fewer than 2^31 voxels and linear indices use 32-bit arithmetic
+template <class S, bool WRAP, bool COUNT = false, bool I32 = false> struct VolView {
     const typename S::word* __restrict__ v;
     int Nx, Ny, Nz;
     VoxelWrap w;
@@ -613,6 +616,7 @@ template <class S, bool WRAP, bool COUNT = false> struct VolView {
     // stored index of the voxel at logical g (every component inside the volume)
     __device__ __forceinline__ long long index(int gx, int gy, int gz) const {
         if (WRAP) return (long long)wrapped_column(gx, gy, w, Nx, Ny, Nz) + wrap1(gz, w.z, Nz);
+        if (I32) return (long long)(unsigned)(((unsigned)gy * (unsigned)Nx + (unsigned)gx) * (unsigned)Nz + (unsigned)gz);
         return ((long long)gy * Nx + gx) * Nz + gz;
     }
 };
@@ -637,6 +641,16 @@ __device__ __forceinline__ int round_div(float x, float c, float rc) {
     if (d > 9.5367431640625e-7f * fmaxf(fabsf(a), 1.0f)) return to_int(r);
     return to_int(roundf(x / c));
 }
+// round_div's rounded quotient as a float (an integer value, or NaN / huge beyond the
+// int range where to_int would give INT_MIN): range tests on it decide what to_int's
+// result would, without the conversion's saturation
+__device__ __forceinline__ float round_div_f(float x, float c, float rc) {
+    const float a = x * rc;
+    const float r = rintf(a);
+    const float d = fabsf(fabsf(a - r) - 0.5f);
+    if (d > 9.5367431640625e-7f * fmaxf(fabsf(a), 1.0f)) return r;
+    return roundf(x / c);
+}
 struct CellInv {
     float x, y, z;
 };
@@ -651,8 +665,8 @@ __device__ __forceinline__ void get_voxel_fast(f3 pt, const TsdfParams& p, const
 // In a shifting volume every corner is wrapped on its own; the reference wraps only
 // the first corner and steps to the others by the unwrapped strides, which reads the
 // wrong voxels (or past the end of the volume) where a cell straddles the wrap.
-template <class S, bool WRAP, bool COUNT>
-__device__ float trilinear(f3 pt, const VolView<S, WRAP, COUNT>& vv, const TsdfParams& p) {
+template <class S, bool WRAP, bool COUNT, bool I32>
+__device__ float trilinear(f3 pt, const VolView<S, WRAP, COUNT, I32>& vv, const TsdfParams& p) {
     int g[3];
     get_voxel(pt, p, g);
     if (g[0] <= 0 || g[0] >= vv.Nx - 1) return kNaN;
@@ -681,7 +695,7 @@ __device__ float trilinear(f3 pt, const VolView<S, WRAP, COUNT>& vv, const TsdfP
         f100 = vv.sample(vv.index(g[0], g[1] + 1, g[2]));
     } else {
         const long long sx = vv.Nz, sy = (long long)vv.Nx * vv.Nz;
-        const long long idx = ((long long)g[1] * vv.Nx + g[0]) * vv.Nz + g[2];
+        const long long idx = vv.index(g[0], g[1], g[2]);
         f000 = vv.sample(idx);
         f001 = vv.sample(idx + 1);
         f011 = vv.sample(idx + sx + 1);
@@ -766,7 +780,7 @@ __device__ __forceinline__ float march_jump(float t, float step, float s_out, fl
 // superblock flags kept in LDS when the volume has at most this many superblocks
 constexpr int kSuperLds = 4096;
 
-template <class S, bool WRAP, bool COUNT = false>
+template <class S, bool WRAP, bool COUNT = false, bool I32 = false>
 __device__ __forceinline__ void raycast_body(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol,
                                              const TsdfParams& p, const NuiCLCameraParams& cam,
                                              const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
@@ -793,7 +807,7 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
     if (x >= W || y >= H) return;
     const size_t id = (size_t)y * W + x;
     const f3 NaN3 = mk3(kNaN, kNaN, kNaN);
-    VolView<S, WRAP, COUNT> vv{vol, (int)p.resolution[0], (int)p.resolution[1], (int)p.resolution[2], wr, mark};
+    VolView<S, WRAP, COUNT, I32> vv{vol, (int)p.resolution[0], (int)p.resolution[1], (int)p.resolution[2], wr, mark};
     const NuiCLRigidTransform g = *pose;
     const f3 off = mk3(p.dimension[0] / 2.0f, p.dimension[1] / 2.0f, 0.f);
     const f3 rs = add3(mk3(g.t[0], g.t[1], g.t[2]), off);
@@ -801,6 +815,7 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
     rn = add3(add3(rot3(g.R, rn), mk3(g.t[0], g.t[1], g.t[2])), off);
     const f3 dir = normalize3(sub3(rn, rs));
     const CellInv ri{1.0f / p.cell_size[0], 1.0f / p.cell_size[1], 1.0f / p.cell_size[2]};
+    const f3 Nm1 = mk3(p.resolution[0] - 1.f, p.resolution[1] - 1.f, p.resolution[2] - 1.f);
     // reciprocal directions for the superblock spans; a component below 1e-30 counts as
     // zero (it moves the ray by less than 1e-28 m over the whole march)
     const f3 inv_dir = mk3(fabsf(dir.x) >= 1e-30f ? 1.0f / dir.x : 0.f, fabsf(dir.y) >= 1e-30f ? 1.0f / dir.y : 0.f,
@@ -839,7 +854,7 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
             // only if its superblock is clear too.
             if (surf.b && !(tsdf < 0.f)) {
                 float tn = tcur, s_in = 1.f, s_out = 0.f;  // empty span
-                size_t cur = ~(size_t)0;                  // superblock of the span
+                unsigned cur = ~0u;                       // superblock of the span
                 while (tn < max_t) {
                     const float un = tn + step;
                     if (!(un >= s_in && un <= s_out)) {
@@ -848,7 +863,7 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
                         if (!(gq[0] >= 0 && gq[1] >= 0 && gq[2] >= 0 && gq[0] < vv.Nx && gq[1] < vv.Ny &&
                               gq[2] < vv.Nz))
                             break;
-                        const size_t si = surf.sindex(gq[0], gq[1], gq[2]);
+                        const unsigned si = surf.sindex32(gq[0], gq[1], gq[2]);
                         if (si != cur) {
                             if ((sup_lds ? sflags[si - surf.nb] : surf.b[si]) != 0) break;
                             superblock_span(gq, Nv, p.cell_size, rs, inv_dir, &s_in, &s_out);
@@ -874,17 +889,22 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
 #pragma unroll
             for (int k = 0; k < kRayBatch; ++k) {
                 tb[k] = tn;
-                get_voxel_fast(add3(rs, mul3(dir, tn + step)), p, ri, gv);
-                inb[k] = gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny && gv[2] < vv.Nz;
+                const f3 q = add3(rs, mul3(dir, tn + step));
+                const float rx = round_div_f(q.x, p.cell_size[0], ri.x), ry = round_div_f(q.y, p.cell_size[1], ri.y),
+                            rz = round_div_f(q.z, p.cell_size[2], ri.z);
+                inb[k] = rx >= 0.f && ry >= 0.f && rz >= 0.f && rx <= Nm1.x && ry <= Nm1.y && rz <= Nm1.z;
+                gv[0] = (int)rx;
+                gv[1] = (int)ry;
+                gv[2] = (int)rz;
                 ib[k] = inb[k] ? vv.index(gv[0], gv[1], gv[2]) : 0;
-                safe[k] = surf.b && inb[k] && surf.b[surf.index(gv[0], gv[1], gv[2])] == 0;
+                safe[k] = surf.b && inb[k] && surf.b[surf.index32(gv[0], gv[1], gv[2])] == 0;
                 tn += step;  // the reference's `time_curr += time_step`
             }
             safe[kRayBatch] = false;  // the first sample of the next batch
             if (surf.b) {
                 get_voxel_fast(add3(rs, mul3(dir, tn + step)), p, ri, gv);
                 safe[kRayBatch] = gv[0] >= 0 && gv[1] >= 0 && gv[2] >= 0 && gv[0] < vv.Nx && gv[1] < vv.Ny &&
-                                  gv[2] < vv.Nz && surf.b[surf.index(gv[0], gv[1], gv[2])] == 0;
+                                  gv[2] < vv.Nz && surf.b[surf.index32(gv[0], gv[1], gv[2])] == 0;
             }
 #pragma unroll
             for (int k = 0; k < kRayBatch; ++k) {
@@ -915,8 +935,8 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
         if (crossing) {
             do {
                 // both interpolations are evaluated (pure reads) so that their loads overlap
-                const float Ftdt = trilinear<S, WRAP, COUNT>(add3(rs, mul3(dir, tcur + step)), vv, p);
-                const float Ft = trilinear<S, WRAP, COUNT>(add3(rs, mul3(dir, tcur)), vv, p);
+                const float Ftdt = trilinear<S, WRAP, COUNT, I32>(add3(rs, mul3(dir, tcur + step)), vv, p);
+                const float Ft = trilinear<S, WRAP, COUNT, I32>(add3(rs, mul3(dir, tcur)), vv, p);
                 if (isnan(Ftdt) || isnan(Ft)) break;
                 const float Ts = tcur - step * Ft / (Ftdt - Ft);
                 const f3 vf = add3(rs, mul3(dir, Ts));
@@ -926,14 +946,14 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
                 get_voxel(add3(rs, mul3(dir, tcur)), p, gg);
                 if (gg[0] > 1 && gg[1] > 1 && gg[2] > 1 && gg[0] < vv.Nx - 2 && gg[1] < vv.Ny - 2 && gg[2] < vv.Nz - 2) {
                     f3 n, q;
-                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S, WRAP, COUNT>(q, vv, p);
-                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S, WRAP, COUNT>(q, vv, p);
+                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
+                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
                     n.x = fx1 - fx2;
-                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S, WRAP, COUNT>(q, vv, p);
-                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S, WRAP, COUNT>(q, vv, p);
+                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
+                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
                     n.y = fy1 - fy2;
-                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S, WRAP, COUNT>(q, vv, p);
-                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S, WRAP, COUNT>(q, vv, p);
+                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
+                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
                     n.z = fz1 - fz2;
                     nout = normalize3(n);
                 }
@@ -945,13 +965,19 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
     if (cmap) cmap[id] = cout;
 }
 
-template <class S, bool WRAP>
+// linear voxel indices fit 31 bits (I32 launches)
+inline bool index32(const TsdfParams& p) {
+    return (double)p.resolution[0] * (double)p.resolution[1] * (double)p.resolution[2] < 2147483648.0;
+}
+
+template <class S, bool WRAP, bool I32>
 __global__ void __launch_bounds__(64)
     raycast_kernel(const typename S::word* __restrict__ vol, const uchar4* __restrict__ cvol, TsdfParams p,
                    NuiCLCameraParams cam, const NuiCLRigidTransform* __restrict__ pose, float* __restrict__ vmap,
                    float* __restrict__ nmap, uchar4* __restrict__ cmap, const int32_t* __restrict__ enable,
                    SurfaceBlocks surf, VoxelWrap wr) {
-    raycast_body<S, WRAP>(vol, cvol, p, cam, pose, vmap, nmap, cmap, enable, surf, wr, blockIdx.x, blockIdx.y);
+    raycast_body<S, WRAP, false, I32>(vol, cvol, p, cam, pose, vmap, nmap, cmap, enable, surf, wr, blockIdx.x,
+                                      blockIdx.y);
 }
 
 // the raycast of raycast_kernel (no wrap) marking every voxel it reads
@@ -973,17 +999,17 @@ __global__ void popcount_kernel(const uint32_t* __restrict__ bits, size_t n, uns
 }
 
 // all sessions' 8 x 8 pixel tiles of one storage (volumes without wrap)
-template <class S>
+template <class S, bool I32>
 __global__ void __launch_bounds__(64) raycast_batch_kernel(const RayJob* __restrict__ jobs, int n, unsigned blocks,
                                                             int remap) {
     const unsigned lb = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     if (lb >= blocks) return;
     const RayJob& J = jobs[job_of(jobs, n, lb)];
     const unsigned b = lb - J.block0;
-    raycast_body<S, false>(in_global(reinterpret_cast<const typename S::word*>(J.vol)), in_global(J.cvol), J.p, J.cam,
-                           in_global(J.pose), in_global(J.vmap), in_global(J.nmap), in_global(J.cmap),
-                           in_global(J.enable), in_global(J.surf), VoxelWrap{0, 0, 0}, (int)(b % J.gx),
-                           (int)(b / J.gx));
+    raycast_body<S, false, false, I32>(in_global(reinterpret_cast<const typename S::word*>(J.vol)), in_global(J.cvol),
+                                       J.p, J.cam, in_global(J.pose), in_global(J.vmap), in_global(J.nmap),
+                                       in_global(J.cmap), in_global(J.enable), in_global(J.surf), VoxelWrap{0, 0, 0},
+                                       (int)(b % J.gx), (int)(b / J.gx));
 }
 
 // ---- host launchers ------------------------------------------------------------
@@ -1041,12 +1067,15 @@ int launch_raycast(const void* volume, const uint8_t* color_volume, const TsdfPa
     if (wr.zero()) {
         const SurfaceBlocks surf = surface_blocks_of(const_cast<uint8_t*>(surface), (int)p.resolution[0],
                                                      (int)p.resolution[1], (int)p.resolution[2]);
-        raycast_kernel<S, false><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, surf, wr);
+        if (index32(p))
+            raycast_kernel<S, false, true><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, surf, wr);
+        else
+            raycast_kernel<S, false, false><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, surf, wr);
     } else {
         // the surface-block skip works on logical positions: not used in a wrapped volume
         const SurfaceBlocks none = surface_blocks_of(nullptr, (int)p.resolution[0], (int)p.resolution[1],
                                                      (int)p.resolution[2]);
-        raycast_kernel<S, true><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, none, wr);
+        raycast_kernel<S, true, false><<<grid, 64, 0, st>>>(vol, cv, p, cam, pose, vmap, nmap, cm, enable, none, wr);
     }
     NUIF_LAUNCHED("raycast_kernel");
     return NUIF_OK;
@@ -1110,10 +1139,15 @@ int launch_raycast_batch(const RayJob* jobs, const RayJob* hjobs, int n, int sto
     const RayJob& last = hjobs[n - 1];
     const unsigned blocks = last.block0 + last.gx * div_up(last.cam.depthImageHeight, 8);
     const int remap = xcd_remap(kXcdRaycast) ? 1 : 0;
-    switch (storage) {
-        case NUIF_TSDF_S16: raycast_batch_kernel<StS16><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap); break;
-        case NUIF_TSDF_F16: raycast_batch_kernel<StF16><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap); break;
-        default: raycast_batch_kernel<StF32><<<xcd_grid(blocks), 64, 0, st>>>(jobs, n, blocks, remap); break;
+    const bool i32 = std::all_of(hjobs, hjobs + n, [](const RayJob& j) { return index32(j.p); });
+    const unsigned g = xcd_grid(blocks);
+    switch (storage * 2 + (i32 ? 1 : 0)) {
+        case NUIF_TSDF_S16 * 2: raycast_batch_kernel<StS16, false><<<g, 64, 0, st>>>(jobs, n, blocks, remap); break;
+        case NUIF_TSDF_S16 * 2 + 1: raycast_batch_kernel<StS16, true><<<g, 64, 0, st>>>(jobs, n, blocks, remap); break;
+        case NUIF_TSDF_F16 * 2: raycast_batch_kernel<StF16, false><<<g, 64, 0, st>>>(jobs, n, blocks, remap); break;
+        case NUIF_TSDF_F16 * 2 + 1: raycast_batch_kernel<StF16, true><<<g, 64, 0, st>>>(jobs, n, blocks, remap); break;
+        case NUIF_TSDF_F32 * 2: raycast_batch_kernel<StF32, false><<<g, 64, 0, st>>>(jobs, n, blocks, remap); break;
+        default: raycast_batch_kernel<StF32, true><<<g, 64, 0, st>>>(jobs, n, blocks, remap); break;
     }
     NUIF_LAUNCHED("raycast_batch_kernel");
     return NUIF_OK;
