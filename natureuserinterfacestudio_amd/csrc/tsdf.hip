@@ -541,10 +541,14 @@ __global__ void __launch_bounds__(64) integrate_tile_batch_kernel(const IntJob* 
     b /= J.gx;
     const int by = (int)(b % J.gy), bz = (int)(b / J.gy);
     // generic pointers here: with device-memory (global_*) accesses the FP16 variant needs
-    // 134 VGPRs instead of 120 (three waves per SIMD instead of four) and runs slower
-    integrate_tile_body<S, COLOR>(J.depth, J.bgra, reinterpret_cast<typename S::word*>(J.vol), J.cvol, J.p, J.cam,
-                                  J.pose, J.maxcw, J.gate, J.updated, J.bounds, J.surf, VoxelWrap{0, 0, 0}, J.walk,
-                                  bx, by, bz, (int)J.gx, (int)J.gz, J.seg);
+    // 134 VGPRs instead of 120 (three waves per SIMD instead of four) and runs slower.  The
+    // volume and camera parameters are copied once: read through the job table they would
+    // be reloaded after every store the walk makes (the stores might alias them)
+    const TsdfParams p = J.p;
+    const NuiCLCameraParams cam = J.cam;
+    integrate_tile_body<S, COLOR>(J.depth, J.bgra, reinterpret_cast<typename S::word*>(J.vol), J.cvol, p, cam, J.pose,
+                                  J.maxcw, J.gate, J.updated, J.bounds, J.surf, VoxelWrap{0, 0, 0}, J.walk, bx, by,
+                                  bz, (int)J.gx, (int)J.gz, J.seg);
 }
 
 // column walk states of all sessions: block -> session by cblock0
