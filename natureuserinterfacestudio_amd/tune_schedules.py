@@ -48,7 +48,7 @@ CANDIDATES = {
 }
 # integration z-segment lengths, forced at run time (NUIF_SEG_VOXELS) for the single
 # session and server rounds of SEG_SESSIONS sessions
-SEGMENTS = (64, 128, 256)
+SEGMENTS = (64, 128, 256, 512)
 SEG_SESSIONS = (8, 32)
 # XCD-aware numbering masks, forced at run time (NUIF_XCD_REMAP); each mask is run twice
 # (interleaved) so that the spread between runs shows beside the differences

@@ -554,6 +554,39 @@ def test_server_sessions_match_standalone_engine(gpu):
     srv.close()
 
 
+def test_server_32_sessions_match_standalone_engines(gpu):
+    """32 sessions in one batched round (the launch sizes of the server's workload: a whole
+    column per integration segment, blocks numbered per XCD) give the standalone engines'
+    poses, maps and volumes bit for bit (a session alone walks its columns in 64-voxel
+    segments from precomputed walk states)."""
+    announce("server: 32 F16 sessions 320x240 128^3, 4 rounds vs standalone engines")
+    wl = W.SMALL
+    S, n = 32, 4
+    assert gpu.L().nuif_schedule_segment(S) >= wl.volume_res
+    frames = [nuif.DeviceArray.from_host(wl.render(f)[0]) for f in range(n + S)]
+    desc = wl.kinfu_desc(storage=nuif.TSDF_F16, use_graph=True)
+    srv = nuif.Server(S * nuif.Server.session_bytes(desc) + (1 << 30))
+    ids = [srv.open(desc) for _ in range(S)]
+    assert None not in ids
+    for r in range(n):
+        srv.submit(ids, [frames[r + k] for k in range(S)])  # session k plays frames k, k+1, ...
+        srv.wait()
+        assert srv.last_batch == (0 if r == 0 else S)
+    for k, sid in enumerate(ids):
+        ref = nuif.Engine(desc)
+        for r in range(n):
+            ref.run(frames[r + k], sync=False)
+        ref.synchronize()
+        s = srv.sessions[sid]
+        np.testing.assert_array_equal(s.poses(n).view(np.uint32), ref.poses(n).view(np.uint32), err_msg=str(k))
+        for a, b in zip(s.maps(), ref.maps()):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=str(k))
+        if k in (0, 13, 31):
+            np.testing.assert_array_equal(s.volume_raw(), ref.volume_raw(), err_msg=str(k))
+        ref.close()
+    srv.close()
+
+
 def test_server_mixed_sessions_match_standalone_engines(gpu):
     """One server round after another over sessions of every kind side by side
     (320x240 / 640x480 / 1280x720 frames; dense S16, F16 and F32 volumes of 128^3 and
