@@ -7,8 +7,23 @@
 
 #include "dense_volume.h"
 #include "nuif_internal.h"
+#include "schedule.h"
 
 namespace nuif {
+
+// The integration's walk-state workspace: per-column states at the starts of z
+// segments 1, 2, ... of the shortest segments (kMinSegVoxels), then the z table: the z
+// terms (vgz, zs) at every z, float2[Nz], the same for every column -- the integration
+// takes a walked chunk's z terms from it and estimates the columns' numerators to test
+// chunks it then skips, instead of stepping every column through them.
+__host__ __device__ __forceinline__ size_t ztable_offset(int Nx, int Ny, int Nz) {
+    const size_t nseg = (size_t)((Nz + kMinSegVoxels - 1) / kMinSegVoxels);
+    return (nseg - 1) * ((size_t)Nx * Ny + 1);
+}
+// blocks of the column-states launch: the per-column states only with several segments
+inline unsigned column_state_blocks(bool walk, int Nx, int Ny, unsigned gz) {
+    return walk ? (gz > 1 ? div_up((size_t)Nx * Ny, 256) : 1u) : 0u;
+}
 
 // the job whose blocks contain flat block b (jobs sorted by block0, jobs[0].block0 == 0)
 template <class Job>

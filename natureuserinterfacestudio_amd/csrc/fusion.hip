@@ -209,11 +209,11 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
         J.gy = (unsigned)Ny / 8;
         J.seg = seg;
         J.gz = div_up(Nz, seg);
-        J.walk = J.gz > 1 ? reinterpret_cast<float2*>(r.walk_states) : nullptr;
+        J.walk = reinterpret_cast<float2*>(r.walk_states);
         J.block0 = ib;
         J.cblock0 = icb;
         ib += J.gx * J.gy * J.gz;
-        icb += J.walk ? div_up((size_t)Nx * Ny, 256) : 0u;
+        icb += column_state_blocks(J.walk != nullptr, Nx, Ny, J.gz);
         RayJob& R = rj[k];
         R = RayJob{};
         R.vol = r.volume;

@@ -164,7 +164,10 @@ __device__ __forceinline__ void column_advance(ColumnWalk& c, float cz) {
 // of the column is affine in z, hence so are the projection's numerators and its
 // depth; with the depth of one sign over the segment the projected coordinates
 // are monotonic and the end points bound them.  The margins cover the rounding
-// of the serial accumulation (relative 1e-4 of the magnitudes, far above 64 ulps).
+// of the serial accumulation (relative 1e-4 of the magnitudes, far above 64 ulps)
+// and of the quotients here (hardware reciprocals, 1 ulp); slack_x / slack_y bound
+// how far the numerators given may lie from the walk's own (an estimate of a later
+// state, see integrate_tile_body).
 //   outside: no step can land inside the image (column_step returns NaN for all)
 //   rect:    the pixel rectangle [x0, x1] x [y0, y1] holds every pixel a step can read
 struct ChunkFootprint {
@@ -173,7 +176,8 @@ struct ChunkFootprint {
 };
 
 __device__ __forceinline__ ChunkFootprint chunk_footprint(const ColumnWalk& c, float R8, float cz, int n,
-                                                          const NuiCLCameraParams& cam) {
+                                                          const NuiCLCameraParams& cam, float slack_x = 0.f,
+                                                          float slack_y = 0.f) {
     ChunkFootprint f{false, false, 0, 0, 0, 0};
     const float steps = (float)(n - 1);
     const float z0 = c.vz + R8 * c.zs;
@@ -187,9 +191,10 @@ __device__ __forceinline__ ChunkFootprint chunk_footprint(const ColumnWalk& c, f
     const float W = (float)cam.depthImageWidth, H = (float)cam.depthImageHeight;
     const float x0 = c.vx, x1 = c.vx + steps * c.incx;
     const float y0 = c.vy, y1 = c.vy + steps * c.incy;
-    const float u0 = x0 / z0, u1 = x1 / z1, v0 = y0 / z0, v1 = y1 / z1;
-    const float mu = 2.f + 1e-4f * (fabsf(x0) + fabsf(steps * c.incx)) / fminf(z0, z1);
-    const float mv = 2.f + 1e-4f * (fabsf(y0) + fabsf(steps * c.incy)) / fminf(z0, z1);
+    const float r0 = __builtin_amdgcn_rcpf(z0), r1 = __builtin_amdgcn_rcpf(z1), rmin = fmaxf(r0, r1);
+    const float u0 = x0 * r0, u1 = x1 * r1, v0 = y0 * r0, v1 = y1 * r1;
+    const float mu = 2.f + (1e-4f * (fabsf(x0) + fabsf(steps * c.incx)) + slack_x) * rmin;
+    const float mv = 2.f + (1e-4f * (fabsf(y0) + fabsf(steps * c.incy)) + slack_y) * rmin;
     // pixel u is hit iff round(u' + cx) in [0, W-1], i.e. u' + cx in [-0.5, W - 0.5)
     const float ulo = fminf(u0, u1) + cam.cx, uhi = fmaxf(u0, u1) + cam.cx;
     const float vlo = fminf(v0, v1) + cam.cy, vhi = fmaxf(v0, v1) + cam.cy;
@@ -306,7 +311,7 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
     const float a = c.vgz, e = c.vgz + (float)(n - 1) * cz;
     const float lo = fminf(a, e), hi = fmaxf(a, e);
     const float m2 = (lo <= 0.f && hi >= 0.f) ? 0.f : fminf(lo * lo, hi * hi);
-    const float dmin = sqrtf(m2 + c.part);
+    const float dmin = __builtin_amdgcn_sqrtf(m2 + c.part);  // 1 ulp, far inside the margin
     const float dmax = bounds_max(bounds, W, H, f.x0, f.x1, f.y0, f.y1);
     return dmax + trunc < dmin * (1.0f - 1e-5f) - 2e-4f;
 }
@@ -319,6 +324,7 @@ __device__ __forceinline__ bool chunk_behind_surface(const ColumnWalk& c, float 
 // terms (vgz, zs) are the same for every column and follow as
 // float2[(nseg - 1) * Nx * Ny + k - 1], written by column 0.  Nx, Ny are
 // multiples of 8 (the tile kernel's condition).
+// After the space of the shortest segments follows the z table (ztable_offset).
 __device__ __forceinline__ void column_states_body(const TsdfParams& p, const NuiCLCameraParams& cam,
                                                    const NuiCLRigidTransform* __restrict__ pose,
                                                    const int32_t* __restrict__ gate, float2* __restrict__ walk_states,
@@ -343,6 +349,13 @@ __device__ __forceinline__ void column_states_body(const TsdfParams& p, const Nu
         walk_states[(size_t)(k - 1) * Nx * Ny + i] = make_float2(vx, vy);
     }
     if (i == 0) {
+        float2* zt = walk_states + ztable_offset(Nx, Ny, Nz);
+        float tz = c0.vgz, tzs = c0.zs;
+        for (int z = 0; z < Nz; ++z) {
+            zt[z] = make_float2(tz, tzs);
+            tz += cz;
+            tzs += cz;
+        }
         float vgz = c0.vgz, zs = c0.zs;
         for (int k = 1; k < nseg; ++k) {
 #pragma unroll 16
@@ -409,17 +422,47 @@ __device__ __forceinline__ void integrate_tile_body(const float* __restrict__ de
 #pragma unroll 1
         for (int k = 0; k < zbeg; ++k) column_advance(c, cz);
     }
+    // With the z table (walk states given) a chunk that no voxel of the tile can update is
+    // tested on an estimate of the columns' numerators at its start, x + k incx (the slack
+    // below bounds its distance from the serial sums, (k + 1) 2^-23 of the magnitudes), and
+    // skipped without stepping the columns through it: the serial additions run only up to
+    // a chunk that is walked (zx: the z where c holds the walk's own state).
+    const float2* __restrict__ ztab = walk_states ? walk_states + ztable_offset(Nx, Ny, Nz) : nullptr;
+    int zx = zbeg;
     for (int zc = zbeg; zc < zend; zc += kZChunk) {
         // phase 1: serial walk of this lane's column; chunks that cannot project
         // into the image only advance the walk
-        const ChunkFootprint fp = chunk_footprint(c, R8, cz, kZChunk, cam);
-        const bool outside = fp.outside || chunk_behind_surface(c, cz, kZChunk, fp, bounds,
+        ColumnWalk a = c;
+        float sx = 0.f, sy = 0.f;
+        if (ztab && zc > zx) {
+            const float k = (float)(zc - zx);
+            a.vx = c.vx + k * c.incx;
+            a.vy = c.vy + k * c.incy;
+            sx = (k + 1.f) * 1.1920929e-7f * (fabsf(c.vx) + k * fabsf(c.incx));
+            sy = (k + 1.f) * 1.1920929e-7f * (fabsf(c.vy) + k * fabsf(c.incy));
+            const float2 zt = ztab[zc];
+            a.vgz = zt.x;
+            a.zs = zt.y;
+        }
+        const ChunkFootprint fp = chunk_footprint(a, R8, cz, kZChunk, cam, sx, sy);
+        const bool outside = fp.outside || chunk_behind_surface(a, cz, kZChunk, fp, bounds,
                                                                 (int)cam.depthImageWidth,
                                                                 (int)cam.depthImageHeight, trunc);
         if (__all(outside)) {
-            for (int k = 0; k < kZChunk; ++k) column_advance(c, cz);
+            if (!ztab)
+                for (int k = 0; k < kZChunk; ++k) column_advance(c, cz);
             continue;
         }
+        if (ztab && zc > zx) {  // the walk's own state at zc: the numerators' serial sums
+            for (; zx < zc; ++zx) {
+                c.vx += c.incx;
+                c.vy += c.incy;
+            }
+            const float2 zt = ztab[zc];
+            c.vgz = zt.x;
+            c.zs = zt.y;
+        }
+        zx = zc + kZChunk;  // the walk below steps c through this chunk
         bool any = false;
         if (outside) {
             for (int k = 0; k < kZChunk; ++k) {
@@ -1034,9 +1077,10 @@ int launch_integrate(const float* depth, const uint8_t* bgra, void* volume, uint
     if (Nx % 8 == 0 && Ny % 8 == 0 && Nz % kZChunk == 0 && wr.z % (16 / S::kBytes) == 0) {
         const int seg = segment_voxels(1);
         dim3 grid(Nx / 8, Ny / 8, div_up(Nz, seg));
-        float2* ws = grid.z > 1 ? reinterpret_cast<float2*>(walk_states) : nullptr;
+        float2* ws = reinterpret_cast<float2*>(walk_states);
         if (ws) {
-            column_states_kernel<<<div_up(Nx * Ny, 256), 256, 0, st>>>(p, cam, pose, gate, ws, seg);
+            column_states_kernel<<<column_state_blocks(true, Nx, Ny, grid.z), 256, 0, st>>>(p, cam, pose, gate, ws,
+                                                                                            seg);
             NUIF_LAUNCHED("column_states_kernel");
         }
         if (color)
@@ -1112,8 +1156,8 @@ int launch_bounds_batch(const BoundsJob* jobs, const BoundsJob* hjobs, int n, hi
 int launch_integrate_batch(const IntJob* jobs, const IntJob* hjobs, int n, int storage, bool color, hipStream_t st) {
     if (n <= 0) return NUIF_OK;
     const IntJob& last = hjobs[n - 1];
-    const unsigned cblocks =
-        last.cblock0 + (last.walk ? div_up((size_t)last.p.resolution[0] * last.p.resolution[1], 256) : 0u);
+    const unsigned cblocks = last.cblock0 + column_state_blocks(last.walk != nullptr, (int)last.p.resolution[0],
+                                                                (int)last.p.resolution[1], last.gz);
     if (cblocks > 0) {
         column_states_batch_kernel<<<cblocks, 256, 0, st>>>(jobs, n);
         NUIF_LAUNCHED("column_states_batch_kernel");
@@ -1232,9 +1276,8 @@ static VoxelWrap wrap_of(const int32_t* w, const TsdfParams& p) {
 
 size_t nuif_tsdf_walk_states_bytes(const TsdfParams* params) {
     if (!params || check_params(params)) return 0;
-    const size_t nseg = (size_t)div_up((int)params->resolution[2], kMinSegVoxels);
-    return nseg > 1 ? (nseg - 1) * ((size_t)params->resolution[0] * (size_t)params->resolution[1] + 1) * sizeof(float2)
-                    : 16;
+    const int Nx = (int)params->resolution[0], Ny = (int)params->resolution[1], Nz = (int)params->resolution[2];
+    return (ztable_offset(Nx, Ny, Nz) + (size_t)Nz) * sizeof(float2);  // column states, then the z table
 }
 
 int nuif_tsdf_integrate_wrapped(const float* depth, const uint8_t* bgra, void* volume, uint8_t* color_volume,

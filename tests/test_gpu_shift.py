@@ -209,8 +209,9 @@ def test_shift_engine_matches_oracle_engine(gpu, shift, color):
 
 @pytest.mark.parametrize("N,storage,wrap", [(256, 0, (0, 0, 0)), (384, 0, (5, 7, 16)), (320, 2, (3, 250, 2))])
 def test_walk_state_workspace_matches_the_replayed_walk(gpu, N, storage, wrap):
-    """The per-column walk states computed once per z segment give the same volume,
-    colour volume and updated-voxel count as every segment replaying the walk."""
+    """The per-column walk states computed once per z segment, with the z table that lets
+    the integration skip chunks on estimated states, give the same volume, colour volume
+    and updated-voxel count as every segment replaying the whole walk."""
     wl = W.SMALL
     announce(f"walk-state workspace vs replay: storage {storage}, {N}^3, wrap {wrap}, 3 frames")
     L = gpu.L()
@@ -218,7 +219,7 @@ def test_walk_state_workspace_matches_the_replayed_walk(gpu, N, storage, wrap):
     cam = wl.camera_params()
     w = np.array(wrap, np.int32)
     ws_bytes = L.nuif_tsdf_walk_states_bytes(C.byref(p))
-    assert ws_bytes == ((N + 63) // 64 - 1) * (N * N + 1) * 8  # sized for 64-voxel segments
+    assert ws_bytes == ((N + 63) // 64 - 1) * (N * N + 1) * 8 + N * 8  # 64-voxel segments, then the z table
     ws = nuif.DeviceArray((ws_bytes,), np.uint8)
     vb = N ** 3 * L.nuif_tsdf_voxel_bytes(storage)
     color = storage == 0

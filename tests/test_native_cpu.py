@@ -78,13 +78,13 @@ def test_voxel_bytes_and_params():
     assert lib.nuif_tsdf_params(0, 1, 1, 1.0, 1.0, 1.0, 0.03, C.byref(p)) == -1
 
 
-@pytest.mark.parametrize("shape,expect", [((128, 128, 64), 16), ((128, 128, 128), (128 * 128 + 1) * 8),
-                                          ((512, 512, 512), 7 * (512 * 512 + 1) * 8),
-                                          ((64, 96, 320), 4 * (64 * 96 + 1) * 8), ((8, 8, 129), 2 * (64 + 1) * 8)])
+@pytest.mark.parametrize("shape,expect", [((128, 128, 64), 64 * 8), ((128, 128, 128), (128 * 128 + 1) * 8 + 128 * 8),
+                                          ((512, 512, 512), 7 * (512 * 512 + 1) * 8 + 512 * 8),
+                                          ((64, 96, 320), 4 * (64 * 96 + 1) * 8 + 320 * 8),
+                                          ((8, 8, 129), 2 * (64 + 1) * 8 + 129 * 8)])
 def test_walk_state_workspace_bytes(shape, expect):
     """8 B per column and z segment after the first, plus 8 B per such segment, for the
-    shortest segment a launch may use (64 voxels); a single segment needs none (16 B
-    placeholder)."""
+    shortest segment a launch may use (64 voxels), then the z table: 8 B per z."""
     lib = nuif.L()
     p = nuif.TsdfParams()
     assert lib.nuif_tsdf_params(*shape, 1.0, 1.0, 1.0, 0.03, C.byref(p)) == 0
