@@ -95,6 +95,7 @@ __global__ void reset_slice_kernel(typename S::word* __restrict__ vol, uint32_t*
 // Column chunk of the integration: 32 voxels keep the LDS transpose buffer at 8.3 KB
 // per 64-column wave (5 waves per SIMD; 64 voxels allowed 3 and ran 20 % slower).
 constexpr int kZChunk = 32;
+constexpr int kCullGroup = 4;  // chunks per coarse cull test of the integration
 static_assert(kMinSegVoxels % kZChunk == 0, "segments hold whole chunks");
 constexpr int kLdsPitch = kZChunk + 1;
 
@@ -429,21 +430,41 @@ __device__ __forceinline__ void integrate_tile_body(const float* __restrict__ de
     // a chunk that is walked (zx: the z where c holds the walk's own state).
     const float2* __restrict__ ztab = walk_states ? walk_states + ztable_offset(Nx, Ny, Nz) : nullptr;
     int zx = zbeg;
-    for (int zc = zbeg; zc < zend; zc += kZChunk) {
-        // phase 1: serial walk of this lane's column; chunks that cannot project
-        // into the image only advance the walk
-        ColumnWalk a = c;
-        float sx = 0.f, sy = 0.f;
-        if (ztab && zc > zx) {
-            const float k = (float)(zc - zx);
+    // The estimated state at z (exact when z == zx) and its slack.
+    auto estimate = [&](int z, ColumnWalk& a, float& sx, float& sy) {
+        a = c;
+        sx = sy = 0.f;
+        if (ztab && z > zx) {
+            const float k = (float)(z - zx);
             a.vx = c.vx + k * c.incx;
             a.vy = c.vy + k * c.incy;
             sx = (k + 1.f) * 1.1920929e-7f * (fabsf(c.vx) + k * fabsf(c.incx));
             sy = (k + 1.f) * 1.1920929e-7f * (fabsf(c.vy) + k * fabsf(c.incy));
-            const float2 zt = ztab[zc];
+            const float2 zt = ztab[z];
             a.vgz = zt.x;
             a.zs = zt.y;
         }
+    };
+    for (int zc = zbeg; zc < zend; zc += kZChunk) {
+        // groups of kCullGroup chunks are tested first, and skipped whole when no voxel of
+        // the tile can update (the same test over the longer run)
+        if (ztab && (zc - zbeg) % (kCullGroup * kZChunk) == 0 && zend - zc > kZChunk) {
+            const int n = min(kCullGroup * kZChunk, zend - zc);
+            ColumnWalk a;
+            float sx, sy;
+            estimate(zc, a, sx, sy);
+            const ChunkFootprint fg = chunk_footprint(a, R8, cz, n, cam, sx, sy);
+            if (__all(fg.outside || chunk_behind_surface(a, cz, n, fg, bounds, (int)cam.depthImageWidth,
+                                                         (int)cam.depthImageHeight, trunc))) {
+                zc += n - kZChunk;
+                continue;
+            }
+        }
+        // phase 1: serial walk of this lane's column; chunks that cannot project
+        // into the image only advance the walk
+        ColumnWalk a;
+        float sx, sy;
+        estimate(zc, a, sx, sy);
         const ChunkFootprint fp = chunk_footprint(a, R8, cz, kZChunk, cam, sx, sy);
         const bool outside = fp.outside || chunk_behind_surface(a, cz, kZChunk, fp, bounds,
                                                                 (int)cam.depthImageWidth,
