@@ -13,12 +13,16 @@
 #ifndef NUIF_RMW_BATCH
 #define NUIF_RMW_BATCH 2     // integration column chunks whose volume loads are in flight together
 #endif
+#ifndef NUIF_CULL_GROUP
+#define NUIF_CULL_GROUP 4    // integration chunks per coarse cull test
+#endif
 #include <cstdlib>
 
 namespace nuif {
 constexpr int kRayBatch = NUIF_RAY_BATCH;
 constexpr int kWalkBatch = NUIF_WALK_BATCH;
 constexpr int kRmwBatch = NUIF_RMW_BATCH;
+constexpr int kCullGroup = NUIF_CULL_GROUP;
 
 // Columns are split into z segments walked by separate waves: tiles that look through
 // long runs of free space would otherwise set the kernel's duration alone.  The length
@@ -29,9 +33,9 @@ constexpr int kRmwBatch = NUIF_RMW_BATCH;
 struct SegmentRule {
     int min_sessions, voxels;
 };
-// measured at 640x480 / 512^3: 1 session 64, 8 sessions 128, 32 sessions 512 (a whole
-// column per wave: 7 % faster than 256 there)
-constexpr SegmentRule kSegmentRules[] = {{1, 64}, {2, 128}, {16, 256}, {32, 512}};
+// measured at 640x480 / 512^3: 1 session 64, 8 and 32 sessions 512 (a whole column per
+// wave: 7-10 % faster than 256 there); 2 to 7 sessions keep 128
+constexpr SegmentRule kSegmentRules[] = {{1, 64}, {2, 128}, {8, 512}};
 constexpr int kMinSegVoxels = 64;  // walk-state buffers are sized for this length
 
 // XCD-aware block numbering (xcd_block: consecutive blocks of a session share one XCD's

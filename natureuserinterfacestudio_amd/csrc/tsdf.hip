@@ -95,7 +95,6 @@ __global__ void reset_slice_kernel(typename S::word* __restrict__ vol, uint32_t*
 // Column chunk of the integration: 32 voxels keep the LDS transpose buffer at 8.3 KB
 // per 64-column wave (5 waves per SIMD; 64 voxels allowed 3 and ran 20 % slower).
 constexpr int kZChunk = 32;
-constexpr int kCullGroup = 4;  // chunks per coarse cull test of the integration
 static_assert(kMinSegVoxels % kZChunk == 0, "segments hold whole chunks");
 constexpr int kLdsPitch = kZChunk + 1;
 

@@ -1,9 +1,9 @@
 """Schedule tuner of the dense integration and raycast.
 
 The launch schedule (natureuserinterfacestudio_amd/csrc/schedule.h) sets how many raycast
-samples, integration walk steps and integration column chunks each wave keeps in flight
-(compiled in) and how long a z segment of the integration is (chosen per launch by the
-number of sessions sharing it).  The tuner builds one variant of the kernel library per
+samples, integration walk steps and integration column chunks each wave keeps in flight,
+how many integration chunks a coarse cull test covers (compiled in) and how long a z
+segment of the integration is (chosen per launch by the number of sessions sharing it).  The tuner builds one variant of the kernel library per
 candidate of the compiled values and times each with bench.py on the GPU: the fused
 server round of 32 sessions at 640x480 / 512^3 (FP16 volumes, per-stage device times)
 and the single session (reference volume); then it forces each segment length at run
@@ -45,6 +45,9 @@ CANDIDATES = {
     "ray16": {"NUIF_RAY_BATCH": 16},
     "walk16": {"NUIF_WALK_BATCH": 16},
     "rmw4": {"NUIF_RMW_BATCH": 4},
+    "cull2": {"NUIF_CULL_GROUP": 2},
+    "cull8": {"NUIF_CULL_GROUP": 8},
+    "cull16": {"NUIF_CULL_GROUP": 16},
 }
 # integration z-segment lengths, forced at run time (NUIF_SEG_VOXELS) for the single
 # session and server rounds of SEG_SESSIONS sessions

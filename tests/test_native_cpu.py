@@ -99,8 +99,7 @@ def test_schedule_table():
     out = (C.c_int32 * 4)()
     assert lib.nuif_schedule(out) == 0
     assert list(out) == [8, 8, 2, 64]
-    assert [lib.nuif_schedule_segment(n) for n in (1, 2, 8, 15, 16, 31, 32, 64)] == [64, 128, 128, 128, 256, 256, 512,
-                                                                                     512]
+    assert [lib.nuif_schedule_segment(n) for n in (1, 2, 7, 8, 16, 32, 64)] == [64, 128, 128, 512, 512, 512, 512]
     assert lib.nuif_schedule_segment(0) == -1
     from natureuserinterfacestudio_amd import tune_schedules as T
     assert T.CANDIDATES["default"] == {} and set(T.SEGMENTS) == {64, 128, 256, 512}
