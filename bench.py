@@ -364,12 +364,17 @@ def main():
                 "algorithmic_bytes": int(integ_bytes), "avg_ms": round(single["stage_avg"][2], 4),
                 "updated_voxels_per_launch": int(single["updated_per_frame"])}
 
-    server = fp32 = None
+    server = fp32 = fp32c = None
     if args.sessions > 0:
-        # 2) the plain FP32 pipeline, one session at a time (the server's reference point)
+        # 2) the plain FP32 pipeline, one session at a time (the server's reference point):
+        #    FP32 volume, the reference's float ICP sums and LM update; beside it the same
+        #    session with the server's compensated ICP
         if not args.server_only:
             print("[bench] FP32 single session, serial", file=sys.stderr, flush=True)
-            fp32 = run_single(nuif, wl, frames, STORAGES["f32"], 0, args.warmup, args.steps, stage_timing=False)
+            fp32 = run_single(nuif, wl, frames, STORAGES["f32"], 0, args.warmup, args.steps,
+                              reduction=nuif.ICP_REDUCE_REFERENCE, stage_timing=False)
+            fp32c = run_single(nuif, wl, frames, STORAGES["f32"], 0, args.warmup, args.steps,
+                               reduction=nuif.ICP_REDUCE_COMPENSATED, stage_timing=False)
         # 3) the multi-session server
         print(f"[bench] server: {args.sessions} sessions, storage {args.server_storage}", file=sys.stderr, flush=True)
         server = run_server(nuif, wl, frames, args.sessions, STORAGES[args.server_storage], args.warmup, args.steps,
@@ -425,6 +430,7 @@ def main():
                            if single["stage_avg"] else None} if single else None,
         "fp32_serial_fps": round(fp32["fps"], 3) if fp32 else None,
         "server_vs_fp32_serial": round(server["fps"] / fp32["fps"], 3) if server and fp32 else None,
+        "fp32_serial_compensated_fps": round(fp32c["fps"], 3) if fp32c else None,
         "roofline": roof,
         "hash_session": hashed,
         "arrival_trace": c5,
