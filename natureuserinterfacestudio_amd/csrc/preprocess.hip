@@ -31,7 +31,8 @@ __global__ void passing_filter_kernel(const uint16_t* __restrict__ raw, float* _
 // K/bilateralFilter.cl:68-119 for the pixel (x,y) of an LDS tile whose origin
 // is (ox - R, oy - R).  Summation order: rows, then columns (as the reference).
 __device__ __forceinline__ float bilateral_px(const float* tile, int tw, int x, int y, int ox, int oy, int W,
-                                              int H, const Gauss& g, int r, float sigma_depth, float thr) {
+                                              int H, const Gauss& g, int r, float sigma_depth, float thr,
+                                              const double* exp_tab) {
     const float c = tile[(y - oy + kMaxR) * tw + (x - ox + kMaxR)];
     if (isnan(c)) return kNaN;
     const int x0 = max(x - r, 0), x1 = min(x + r + 1, W);
@@ -45,13 +46,20 @@ __device__ __forceinline__ float bilateral_px(const float* tile, int tw, int x, 
             const float diff = fabsf(c - n);
             if (n > 0.0f && diff < thr) {
                 const float d2 = diff * diff;
-                const float w = g.g[cx + r - x] * g.g[cy + r - y] * nuif_det::expf_det(-(d2 * sigma_depth));
+                const float w = g.g[cx + r - x] * g.g[cy + r - y] * nuif_det::expf_det_t(-(d2 * sigma_depth), exp_tab);
                 sd += n * w;
                 sw += w;
             }
         }
     }
     return sd / sw;
+}
+
+// The exp table of nuif_det::expf_det into LDS (every lane of the block reads it at
+// data-dependent indices); the caller synchronises.
+__device__ __forceinline__ void load_exp_table(double* t) {
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if (tid < 64) t[tid] = nuif_det::kExp2J64[tid];
 }
 
 // Loads the block's tile (with a kMaxR halo) into LDS.  FROM_RAW applies the
@@ -79,23 +87,28 @@ __device__ __forceinline__ void load_tile(float* tile, int tw, int th, const voi
 __global__ void __launch_bounds__(256) bilateral_kernel(const float* __restrict__ in, float* __restrict__ out, int W,
                                                         int H, Gauss g, int r, float sigma_depth, float thr) {
     __shared__ float tile[(kTileW + 2 * kMaxR) * (kTileH + 2 * kMaxR)];
+    __shared__ double etab[64];
     const int tw = kTileW + 2 * kMaxR, th = kTileH + 2 * kMaxR;
     const int ox = blockIdx.x * kTileW, oy = blockIdx.y * kTileH;
     load_tile<false>(tile, tw, th, in, W, H, ox, oy, 0.f, 0.f);
+    load_exp_table(etab);
     __syncthreads();
     const int x = ox + threadIdx.x, y = oy + threadIdx.y;
-    if (x < W && y < H) out[(size_t)y * W + x] = bilateral_px(tile, tw, x, y, ox, oy, W, H, g, r, sigma_depth, thr);
+    if (x < W && y < H)
+        out[(size_t)y * W + x] = bilateral_px(tile, tw, x, y, ox, oy, W, H, g, r, sigma_depth, thr, etab);
 }
 
 // Fused passing filter + bilateral + level-0 vertices + ray-length depth: reads the raw
 // depth once.  Body of one 32 x 8 tile (bx, by) of job J.
 __device__ __forceinline__ void preprocess_l0_body(const PreJob& J, int bx, int by) {
     __shared__ float tile[(kTileW + 2 * kMaxR) * (kTileH + 2 * kMaxR)];
+    __shared__ double etab[64];
     const int tw = kTileW + 2 * kMaxR, th = kTileH + 2 * kMaxR;
     const int W = J.W, H = J.H;
     const NuiCLCameraParams cam = J.cam;
     const int ox = bx * kTileW, oy = by * kTileH;
     load_tile<true>(tile, tw, th, in_global(J.raw), W, H, ox, oy, cam.sensorDepthWorldMin, cam.sensorDepthWorldMax);
+    load_exp_table(etab);
     __syncthreads();
     const int x = ox + threadIdx.x, y = oy + threadIdx.y;
     if (x >= W || y >= H) return;
@@ -109,7 +122,7 @@ __device__ __forceinline__ void preprocess_l0_body(const PreJob& J, int bx, int 
         d = d * lambda;
     }
     in_global(J.depth_out)[i] = d;
-    const float f = bilateral_px(tile, tw, x, y, ox, oy, W, H, J.g, J.r, J.sigma_depth, J.thr);
+    const float f = bilateral_px(tile, tw, x, y, ox, oy, W, H, J.g, J.r, J.sigma_depth, J.thr, etab);
     in_global(J.depth_f)[i] = f;
     // K/transformMaps.cl:18-42 with div = 1
     f3 v = mk3(kNaN, kNaN, kNaN);
