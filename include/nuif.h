@@ -180,6 +180,10 @@ typedef struct nuif_icp_result {
  * reject decisions at the noise floor of the residual make float and FP64 runs diverge
  * by millimetres; this loop stays within hundredths of a millimetre of its FP64 run. */
 #define NUIF_ICP_REDUCE_COMPENSATED 1
+/* The compensated order with a photometric row per pixel beside the geometric one
+ * (nuif_icp_photometric): joint geometric + photometric tracking.  The reference's joint
+ * kernel (K/icp.cl:352-480) has no host loop; this is the project's own formulation. */
+#define NUIF_ICP_REDUCE_JOINT 2
 
 typedef struct nuif_icp_config {
     int levels;                            /* pyramid levels (<= 4) */
@@ -188,6 +192,23 @@ typedef struct nuif_icp_config {
     float gate_threshold;                  /* integration metric threshold (0 = always) */
     int reduction;                         /* NUIF_ICP_REDUCE_* */
 } nuif_icp_config;
+
+/* Inputs of the joint order's photometric rows.  For a pixel whose geometric association
+ * holds (current vertex p at the estimate, projected through the previous pose to the
+ * full-resolution pixel (u, v)): I the current intensity at the pixel's level, I_ref and
+ * g = (dI/du, dI/dv) the model's intensity and its gradient bilinearly at (u, v); with
+ * |I - I_ref| < max_difference and |g|^2 > min_gradient^2 the row is
+ * sqrt(weight) [J x p, J, I - I_ref], J the gradient carried to world coordinates through
+ * the previous camera, added to the normal equations beside the geometric row
+ * [n x p, n, n . (q - p)]. */
+typedef struct nuif_icp_photometric {
+    const float* intensity_pyr[NUIF_ICP_MAX_LEVELS]; /* current intensities, W>>l x H>>l per level */
+    const float* ref_intensity;                      /* the model's intensities, W x H (previous pose) */
+    const float* ref_gradient;                       /* W x H float2: nuif_intensity_derivatives of it */
+    float weight;                                    /* photometric weight (0.1) */
+    float max_difference;
+    float min_gradient;
+} nuif_icp_photometric;
 
 /* State block: the tracking state plus the current iteration ("epoch") of the frame
  * (device memory, initialised by nuif_icp_state_init). */
@@ -241,6 +262,12 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
  * process ends; an image whose nuif_icp_partials_bytes exceed it is refused with
  * NUIF_ERR_INVALID (1280 x 720 fits).  Nothing is allocated or freed.  Every call on a
  * GPU shares that GPU's array: calls that overlap on one GPU (different streams) race. */
+/* nuif_icp_estimate in the joint order (cfg->reduction NUIF_ICP_REDUCE_JOINT). */
+int nuif_icp_estimate_joint(void* state, float* partials, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
+                            const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
+                            const nuif_icp_config* cfg, const nuif_icp_photometric* photometric, float* history,
+                            int capacity, nuif_stream_t s);
+
 size_t nuif_icp_static_scratch_bytes(void);
 int nuif_icp_estimate_static(void* state, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
                              const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
@@ -261,6 +288,7 @@ typedef struct nuif_icp_session {
     int32_t* outcome;                          /* NULL, or an int32 that the frame's end sets to 1
                                                   (tracked) or 0 (failed): device memory or pinned
                                                   host memory (nuif_host_alloc) */
+    nuif_icp_photometric photometric;          /* cfg.reduction NUIF_ICP_REDUCE_JOINT only */
 } nuif_icp_session;
 /* Device bytes of the session table of nuif_icp_estimate_batch. */
 size_t nuif_icp_batch_table_bytes(int n);
@@ -654,6 +682,10 @@ int nuif_bgra_to_intensity(const uint8_t* bgra, float* intensities, size_t n, nu
  * reference steps by the image height and leaves those pixels unwritten). */
 int nuif_intensity_derivatives(const float* intensities, float* derivatives, int width, int height,
                                nuif_stream_t s);
+/* Intensities one pyramid level down (no counterpart in the reference): dst (dst_width x
+ * dst_height) = the mean of the valid (non-NaN) ones of the 2 x 2 source pixels, summed in
+ * row order, NaN when none is valid; src is 2 dst_width x 2 dst_height. */
+int nuif_intensity_half_sample(const float* src, float* dst, int dst_width, int dst_height, nuif_stream_t s);
 size_t nuif_intensity_icp_workspace_bytes(int width, int height);
 /* K/icp.cl:352-480 intensity_icp_block_kernel + the 128-wide reduction trees + the
  * sequential sum of the second-level results, at `level` (all maps W >> level by H >>

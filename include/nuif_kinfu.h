@@ -93,6 +93,13 @@ int nuifk_state(void* h, nuif_icp_result* out);
  * (layout of nuif_hash_layout); nuifk_volume_bytes gives its size. */
 int nuifk_read_volume(void* h, void* out, size_t bytes);
 int nuifk_read_maps(void* h, float* vmap, float* nmap);
+/* The feedback frame's colours (W*H BGRA): the last raycast's, or the first frame's
+ * (OCL/NuiKinfuOpenCLFeedbackFrame.cpp colour buffer). */
+int nuifk_read_feedback_colors(void* h, uint8_t* out);
+/* The joint ICP order's photometric weight (default 0.1), largest intensity difference
+ * (0.2) and minimum gradient (0.002) (nuif_icp_photometric); before the next frame.  No
+ * counterpart in the reference. */
+int nuifk_set_photometric(void* h, float weight, float max_difference, float min_gradient);
 /* Shifting volume: voxel offset (int32[3]) and its size in metres (float[3]). */
 int nuifk_shift_state(void* h, int32_t* offset, float* offset_size);
 /* Shifting volume: points of the evicted slabs (NuiKinfuVertexCache); returns their number
@@ -154,6 +161,12 @@ void* nuifs_engine(void* srv, int id);
  * the server's stream; every other frame (a session's first frame, hash and shifting
  * scenes) runs on its session's own stream, overlapping the batch. */
 int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths);
+/* nuifs_submit with each frame's device-resident BGRA colours (W*H*4 bytes, or NULL for a
+ * frame without): integrated into sessions with a colour volume, and the photometric
+ * rows' input of sessions in the joint ICP order (icp_reduction NUIF_ICP_REDUCE_JOINT,
+ * which require them).  No counterpart in the reference. */
+int nuifs_submit_color(void* srv, int n, const int* ids, const uint16_t* const* depths,
+                       const uint8_t* const* bgras);
 /* Batched rounds on (default, non-zero) or off: every frame on its session's stream. */
 int nuifs_set_batching(void* srv, int on);
 /* Frames of the last round that ran in the batch. */

@@ -134,8 +134,11 @@ int launch_raycast_batch(const RayJob* jobs, const RayJob* hjobs, int n, int sto
 // memory, reference order first (*nref of them); launch_icp_jobs runs them from the
 // device copy dtable
 size_t icp_job_bytes();
-int make_icp_jobs(const nuif_icp_session* sessions, int n, void* host_out, int* nref);
-int launch_icp_jobs(const void* dtable, const void* hjobs, int n, int nref, hipStream_t st);
+int make_icp_jobs(const nuif_icp_session* sessions, int n, void* host_out, int* counts);
+int launch_icp_jobs(const void* dtable, const void* hjobs, int n, const int* counts, hipStream_t st);
+// the joint ICP order's inputs of a frame (color.hip)
+int launch_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W,
+                              int H, int levels, hipStream_t st);
 int make_gauss(const float* gaussian, int radius, Gauss* g);
 
 }  // namespace nuif

@@ -65,6 +65,12 @@ __device__ __forceinline__ int cvt_i32_sat(float v) {
     return (int)v;
 }
 
+// The joint ICP order's inputs of a frame (nuif_icp_photometric): the current intensities
+// at every level from the frame's BGRA, the model's intensities from the previous
+// raycast's colours and their Sobel gradients.
+int launch_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W,
+                              int H, int levels, hipStream_t st);
+
 // ---- pyramid and gradients ------------------------------------------------------
 
 // K/transformMaps.cl:405-446 half_sample_bgra_kernel
@@ -475,6 +481,25 @@ __global__ void intensity_derivatives_kernel(const float* __restrict__ I, float2
     out[c] = r;
 }
 
+// Intensities one level down: the mean of the valid ones of the 2 x 2 source pixels,
+// summed in row order (no counterpart in the reference)
+__global__ void intensity_half_sample_kernel(const float* __restrict__ src, float* __restrict__ dst, int Wd, int Hd) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= Wd || y >= Hd) return;
+    const int Ws = 2 * Wd;
+    const size_t i = (size_t)(2 * y) * Ws + 2 * x;
+    const float v[4] = {src[i], src[i + 1], src[i + Ws], src[i + Ws + 1]};
+    float sum = 0.f;
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (!isnan(v[k])) {
+            sum += v[k];
+            ++cnt;
+        }
+    dst[(size_t)y * Wd + x] = cnt ? sum / (float)cnt : kNaN;
+}
+
 // K/icp.cl:328-350 evalR_dAlpha / evalR_dBeta / evalR_dGamma as written
 __device__ __forceinline__ void eval_phis(const float s[3], const float c[3], f3 p, f3* pa, f3* pb, f3* pg) {
     const float sx = s[0], cx = c[0], sy = s[1], cy = c[1], sz = s[2], cz = c[2];
@@ -628,6 +653,27 @@ __global__ void __launch_bounds__(128 * kIntGroups)
 __global__ void intensity_sync_clear_kernel(ColorIcpSync* sy) {
     for (int i = threadIdx.x; i < kCMaxGroups2; i += blockDim.x) sy->arrive[i] = 0u;
     if (threadIdx.x == 0) sy->arrive_all = 0u;
+}
+
+int launch_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W,
+                              int H, int levels, hipStream_t st) {
+    NUIF_CHECK_ARG(bgra && ref_bgra && ph.ref_intensity && ph.ref_gradient, "joint ICP: missing colour inputs");
+    const size_t n = (size_t)W * H;
+    float* i0 = const_cast<float*>(ph.intensity_pyr[0]);
+    bgra_to_intensity_kernel<<<div_up(n, 256), 256, 0, st>>>(reinterpret_cast<const uchar4*>(bgra), i0, n);
+    for (int l = 1; l < levels; ++l) {
+        const int Wd = W >> l, Hd = H >> l;
+        const dim3 block(16, 16), grid(div_up(Wd, 16), div_up(Hd, 16));
+        intensity_half_sample_kernel<<<grid, block, 0, st>>>(ph.intensity_pyr[l - 1],
+                                                             const_cast<float*>(ph.intensity_pyr[l]), Wd, Hd);
+    }
+    float* ir = const_cast<float*>(ph.ref_intensity);
+    bgra_to_intensity_kernel<<<div_up(n, 256), 256, 0, st>>>(reinterpret_cast<const uchar4*>(ref_bgra), ir, n);
+    const dim3 block(16, 16), grid(div_up(W, 16), div_up(H, 16));
+    intensity_derivatives_kernel<<<grid, block, 0, st>>>(ir, reinterpret_cast<float2*>(const_cast<float*>(ph.ref_gradient)),
+                                                         W, H);
+    NUIF_LAUNCHED("joint ICP inputs");
+    return NUIF_OK;
 }
 
 }  // namespace nuif
@@ -784,6 +830,16 @@ int nuif_intensity_derivatives(const float* intensities, float* derivatives, int
     intensity_derivatives_kernel<<<grid, block, 0, stream_of(s)>>>(intensities, reinterpret_cast<float2*>(derivatives),
                                                                    width, height);
     NUIF_LAUNCHED("intensity_derivatives_kernel");
+    return NUIF_OK;
+}
+
+int nuif_intensity_half_sample(const float* src, float* dst, int dst_width, int dst_height, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(src && dst && dst_width > 0 && dst_height > 0 && (const void*)src != dst,
+                   "nuif_intensity_half_sample: bad arguments");
+    const dim3 block(16, 16), grid(div_up(dst_width, 16), div_up(dst_height, 16));
+    intensity_half_sample_kernel<<<grid, block, 0, stream_of(s)>>>(src, dst, dst_width, dst_height);
+    NUIF_LAUNCHED("intensity_half_sample_kernel");
     return NUIF_OK;
 }
 

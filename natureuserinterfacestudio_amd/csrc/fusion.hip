@@ -67,6 +67,14 @@ int check_session(const nuif_round_session& r, int i) {
     NUIF_CHECK_ARG(r.storage == NUIF_TSDF_S16 || r.storage == NUIF_TSDF_F16 || r.storage == NUIF_TSDF_F32,
                    "nuif_fusion_round: session %d: unknown storage %d", i, r.storage);
     NUIF_CHECK_ARG(Nz <= kMinSegVoxels || r.walk_states, "nuif_fusion_round: session %d needs walk states", i);
+    if (r.icp.cfg.reduction == NUIF_ICP_REDUCE_JOINT) {
+        const nuif_icp_photometric& ph = r.icp.photometric;
+        NUIF_CHECK_ARG(r.bgra && r.cmap && ph.ref_intensity && ph.ref_gradient,
+                       "nuif_fusion_round: session %d: the joint ICP needs the frame's colours, the feedback "
+                       "colours and the intensity buffers", i);
+        for (int l = 0; l < L; ++l)
+            NUIF_CHECK_ARG(ph.intensity_pyr[l], "nuif_fusion_round: session %d: intensity level %d is NULL", i, l);
+    }
     return NUIF_OK;
 }
 
@@ -158,8 +166,8 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
     // ICP
     std::vector<nuif_icp_session> icp((size_t)n);
     for (int i = 0; i < n; ++i) icp[i] = S[i].icp;
-    int nref = 0;
-    int rc = make_icp_jobs(icp.data(), n, h + lay.icp, &nref);
+    int counts[2] = {0, 0};
+    int rc = make_icp_jobs(icp.data(), n, h + lay.icp, counts);
     if (rc) return rc;
     // integration and raycast, grouped by storage (and colour for the integration)
     const int seg = segment_voxels(n);  // integration z segment for a launch of n sessions
@@ -248,7 +256,14 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
     rc = launch_bounds_batch(reinterpret_cast<const BoundsJob*>(d + lay.bounds), bj, n, st);
     if (rc) return rc;
     if ((rc = mark(2))) return rc;
-    rc = launch_icp_jobs(d + lay.icp, h + lay.icp, n, nref, st);
+    for (int i = 0; i < n; ++i) {  // the joint order's photometric inputs
+        const nuif_round_session& r = S[i];
+        if (r.icp.cfg.reduction != NUIF_ICP_REDUCE_JOINT) continue;
+        rc = launch_photometric_inputs(r.bgra, r.cmap, r.icp.photometric, (int)r.icp.cam.depthImageWidth,
+                                       (int)r.icp.cam.depthImageHeight, r.icp.cfg.levels, st);
+        if (rc) return rc;
+    }
+    rc = launch_icp_jobs(d + lay.icp, h + lay.icp, n, counts, st);
     if (rc) return rc;
     if ((rc = mark(3))) return rc;
     for (const Group& g : igroups) {

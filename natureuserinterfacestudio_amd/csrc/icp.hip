@@ -67,6 +67,7 @@ static_assert(128 % kTileGroups == 0, "a tile lies in one compute_sums chunk");
 // Reduction orders (nuif_icp_config::reduction)
 constexpr int kRefOrder = 0;      // the reference's float trees and LM loop, bit for bit
 constexpr int kCompensated = 1;   // Kahan FP32 + FP64, Gauss-Newton with an FP64 pose
+constexpr int kJoint = 2;         // the compensated order with photometric rows
 
 struct IcpState {
     float R[9], t[3];          // current estimate (compensated: the FP64 pose rounded)
@@ -117,6 +118,11 @@ struct IcpJob {
     int32_t single;             // one epoch only (nuif_icp_iteration): finished after it
     int32_t first_epoch;        // single: the epoch to run
     int32_t* outcome;           // NULL, or receives 1 (tracked) / 0 (failed) at the frame's end
+    // joint order: the photometric inputs (nuif_icp_photometric)
+    const float* icur[NUIF_ICP_MAX_LEVELS];
+    const float* iref;
+    const float2* gref;
+    float psw, pdiff, pgrad2;   // sqrt(weight), max |I - I_ref|, min |g|^2
 };
 
 __host__ __device__ __forceinline__ int level_pixels(const NuiCLCameraParams& c, int level) {
@@ -158,16 +164,24 @@ struct TilePose {
     float R[9], t[3], Ri[9], tp[3];
 };
 
+// Where a pixel's association lands: the current vertex at the estimate (pv), in the
+// previous camera (pp), and its full-resolution projection (px, py).
+struct AssocAt {
+    f3 pv, pp;
+    float px, py;
+};
+
 // One pixel of K/icp.cl:243-289: its association and its 7 coefficients
 // c = (n x p, n, n . (q - p)); false when rejected.
 __device__ __forceinline__ bool icp_assoc(const f3 vc, const TilePose& P, const NuiCLCameraParams& cam,
                                           const float* __restrict__ vref, const float* __restrict__ nref,
-                                          float distThr, float c[7]) {
+                                          float distThr, float c[7], AssocAt* at = nullptr) {
     if (nan3(vc)) return false;
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
     const f3 pv = add3(rot3(P.R, vc), mk3(P.t[0], P.t[1], P.t[2]));
     const f3 pp = rot3(P.Ri, sub3(pv, mk3(P.tp[0], P.tp[1], P.tp[2])));
     const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
+    if (at) *at = AssocAt{pv, pp, px, py};
     if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) return false;
     // both lookups are issued before either is tested (pure reads)
     const f3 rn = bilinear_holes(nref, px, py, W, H);
@@ -184,6 +198,61 @@ __device__ __forceinline__ bool icp_assoc(const f3 vc, const TilePose& P, const 
     c[4] = rn.y;
     c[5] = rn.z;
     c[6] = dot3(rn, diff);
+    return true;
+}
+
+// Bilinear sample of a scalar / float2 image with the holes rule of bilinear_holes
+// (an index past the buffer or a NaN corner gives NaN).
+__device__ __forceinline__ float bilinear1(const float* __restrict__ src, float px, float py, int W, int H) {
+    const int x = (int)floorf(px), y = (int)floorf(py);
+    const float dx = px - (float)x, dy = py - (float)y;
+    const long long n = (long long)W * H;
+    const long long ia = x + (long long)y * W, ib = ia + 1, ic = ia + W, id = ic + 1;
+    if (ia < 0 || id >= n) return kNaN;
+    const float a = src[ia], b = src[ib], c = src[ic], d = src[id];
+    if (isnan(a) || isnan(b) || isnan(c) || isnan(d)) return kNaN;
+    return a * (1.0f - dx) * (1.0f - dy) + b * dx * (1.0f - dy) + c * (1.0f - dx) * dy + d * dx * dy;
+}
+__device__ __forceinline__ float2 bilinear2(const float2* __restrict__ src, float px, float py, int W, int H) {
+    const int x = (int)floorf(px), y = (int)floorf(py);
+    const float dx = px - (float)x, dy = py - (float)y;
+    const long long n = (long long)W * H;
+    const long long ia = x + (long long)y * W, ib = ia + 1, ic = ia + W, id = ic + 1;
+    if (ia < 0 || id >= n) return make_float2(kNaN, kNaN);
+    const float2 a = src[ia], b = src[ib], c = src[ic], d = src[id];
+    if (isnan(a.x) || isnan(b.x) || isnan(c.x) || isnan(d.x) || isnan(a.y) || isnan(b.y) || isnan(c.y) || isnan(d.y))
+        return make_float2(kNaN, kNaN);
+    return make_float2(a.x * (1.0f - dx) * (1.0f - dy) + b.x * dx * (1.0f - dy) + c.x * (1.0f - dx) * dy + d.x * dx * dy,
+                       a.y * (1.0f - dx) * (1.0f - dy) + b.y * dx * (1.0f - dy) + c.y * (1.0f - dx) * dy + d.y * dx * dy);
+}
+
+// The joint order's photometric row of an associated pixel (nuif_icp_photometric): false
+// when the pixel gives none.  ic: the current intensity at the pixel.
+__device__ __forceinline__ bool photometric_row(float ic, const AssocAt& at, const TilePose& P,
+                                                const NuiCLCameraParams& cam, const float* __restrict__ iref,
+                                                const float2* __restrict__ gref, float psw, float pdiff, float pgrad2,
+                                                float c[7]) {
+    if (isnan(ic)) return false;
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    const float ir = bilinear1(iref, at.px, at.py, W, H);
+    const float2 g = bilinear2(gref, at.px, at.py, W, H);
+    if (isnan(ir) || isnan(g.x)) return false;
+    const float di = ic - ir;
+    if (!(fabsf(di) < pdiff) || !(g.x * g.x + g.y * g.y > pgrad2)) return false;
+    // d(intensity)/d(point in the previous camera), then to world coordinates: pp = Ri (pv - tp)
+    const float iz = 1.0f / at.pp.z;
+    const float ax = g.x * cam.fx, ay = g.y * cam.fy;
+    const f3 jc = mk3(ax * iz, ay * iz, -((ax * at.pp.x + ay * at.pp.y) * iz) * iz);
+    const f3 jw = mk3((jc.x * P.Ri[0] + jc.y * P.Ri[3]) + jc.z * P.Ri[6], (jc.x * P.Ri[1] + jc.y * P.Ri[4]) + jc.z * P.Ri[7],
+                      (jc.x * P.Ri[2] + jc.y * P.Ri[5]) + jc.z * P.Ri[8]);
+    const f3 a = cross3(jw, at.pv);
+    c[0] = psw * a.x;
+    c[1] = psw * a.y;
+    c[2] = psw * a.z;
+    c[3] = psw * jw.x;
+    c[4] = psw * jw.y;
+    c[5] = psw * jw.z;
+    c[6] = psw * di;
     return true;
 }
 
@@ -416,6 +485,7 @@ __device__ __forceinline__ int xt_index(int term, int thread) { return term * kX
 template <> struct IcpLds<kCompensated> : IcpLdsBase {
     double xt[kXtTerms * kXtRow];
 };
+template <> struct IcpLds<kJoint> : IcpLds<kCompensated> {};
 
 // Tile `tile` of level L.level of job L.job (with L.pose loaded): the 29 terms of its
 // pixels and its partial sums.
@@ -462,11 +532,21 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
         const int p0 = (int)tile * kTilePx + tid;
         float cf[kPxPerThread][7];
         bool ok[kPxPerThread];
+        float pc[MODE == kJoint ? kPxPerThread : 1][7];
+        bool ph[kPxPerThread] = {};
 #pragma unroll
         for (int i = 0; i < kPxPerThread; ++i) {
             const int p = p0 + i * kBlock;
             const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
-            ok[i] = icp_assoc(vc, L.pose, J.cam, vref, nref, J.dist, cf[i]);
+            if constexpr (MODE == kJoint) {
+                AssocAt at;
+                ok[i] = icp_assoc(vc, L.pose, J.cam, vref, nref, J.dist, cf[i], &at);
+                const float ic = ok[i] ? in_global(J.icur[level])[p] : kNaN;
+                ph[i] = ok[i] && photometric_row(ic, at, L.pose, J.cam, in_global(J.iref), in_global(J.gref), J.psw,
+                                                 J.pdiff, J.pgrad2, pc[i]);
+            } else {
+                ok[i] = icp_assoc(vc, L.pose, J.cam, vref, nref, J.dist, cf[i]);
+            }
         }
         // Kahan sums of the two pixels from zero, folded: after the first pixel the sum is
         // 0 + v0 (never -0) and the compensation exactly +0, so the second step is
@@ -476,8 +556,12 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
             constexpr int k = decltype(kc)::value;
             if constexpr (k < kTerms - 1) {
                 constexpr int a = term_a(k), b = term_b(k);
-                const float v0 = ok[0] ? cf[0][a] * cf[0][b] : 0.f;
-                const float v1 = ok[1] ? cf[1][a] * cf[1][b] : 0.f;
+                float v0 = ok[0] ? cf[0][a] * cf[0][b] : 0.f;
+                float v1 = ok[1] ? cf[1][a] * cf[1][b] : 0.f;
+                if constexpr (MODE == kJoint) {  // a pixel's value: geometric + photometric product
+                    if (ph[0]) v0 = v0 + pc[0][a] * pc[0][b];
+                    if (ph[1]) v1 = v1 + pc[MODE == kJoint ? 1 : 0][a] * pc[MODE == kJoint ? 1 : 0][b];
+                }
                 const float s1 = 0.f + v0;
                 const float sm = s1 + v1;
                 return (double)sm - (double)((sm - s1) - v1);
@@ -719,7 +803,7 @@ __global__ void icp_begin_kernel(const IcpJob* __restrict__ jobs, IcpJob single)
     IcpSync* sy = sync_of(st);
     const NuiCLRigidTransform* pose = J.pose;
     if (!J.single) {
-        if (J.mode == kCompensated) {
+        if (J.mode != kRefOrder) {
             bool same = true;
             for (int k = 0; k < 9; ++k) same = same && (float)st->Rd[k] == pose->R[k];
             for (int k = 0; k < 3; ++k) same = same && (float)st->td[k] == pose->t[k];
@@ -915,7 +999,8 @@ int make_job(const nuif_icp_session& in, IcpJob* j) {
     const nuif_icp_config& cfg = in.cfg;
     NUIF_CHECK_ARG(cfg.levels >= 1 && cfg.levels <= NUIF_ICP_MAX_LEVELS, "ICP session: bad level count %d",
                    cfg.levels);
-    NUIF_CHECK_ARG(cfg.reduction == NUIF_ICP_REDUCE_REFERENCE || cfg.reduction == NUIF_ICP_REDUCE_COMPENSATED,
+    NUIF_CHECK_ARG(cfg.reduction == NUIF_ICP_REDUCE_REFERENCE || cfg.reduction == NUIF_ICP_REDUCE_COMPENSATED ||
+                       cfg.reduction == NUIF_ICP_REDUCE_JOINT,
                    "ICP session: unknown reduction %d", cfg.reduction);
     NUIF_CHECK_ARG(!in.history || in.capacity > 0, "ICP session: history capacity must be positive");
     const int W = (int)in.cam.depthImageWidth, H = (int)in.cam.depthImageHeight;
@@ -948,6 +1033,20 @@ int make_job(const nuif_icp_session& in, IcpJob* j) {
     if (rc) return rc;
     J.levels = cfg.levels;
     J.mode = cfg.reduction;
+    if (J.mode == kJoint) {
+        const nuif_icp_photometric& ph = in.photometric;
+        NUIF_CHECK_ARG(ph.ref_intensity && ph.ref_gradient && ph.weight >= 0.f,
+                       "ICP session: the joint order needs the photometric inputs");
+        for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) {
+            J.icur[l] = l < cfg.levels ? ph.intensity_pyr[l] : nullptr;
+            NUIF_CHECK_ARG(J.iters[l] == 0 || J.icur[l], "ICP session: level %d has no intensities", l);
+        }
+        J.iref = ph.ref_intensity;
+        J.gref = reinterpret_cast<const float2*>(ph.ref_gradient);
+        J.psw = sqrtf(ph.weight);
+        J.pdiff = ph.max_difference;
+        J.pgrad2 = ph.min_gradient * ph.min_gradient;
+    }
     return NUIF_OK;
 }
 
@@ -989,9 +1088,9 @@ uint32_t step_tiles(const IcpJob& J, int k) {
 
 size_t icp_job_bytes() { return sizeof(IcpJob); }
 
-// The jobs of n sessions, those of the reference order first (one series of steps per
-// order); *nref receives their number.
-int make_icp_jobs(const nuif_icp_session* sessions, int n, void* host_out, int* nref) {
+// The jobs of n sessions grouped by order (reference, compensated, joint: one series of
+// steps per order); counts[0..1] receive the reference and compensated numbers.
+int make_icp_jobs(const nuif_icp_session* sessions, int n, void* host_out, int* counts) {
     IcpJob* jobs = static_cast<IcpJob*>(host_out);
     for (int i = 0; i < n; ++i) {
         const int rc = make_job(sessions[i], &jobs[i]);
@@ -999,8 +1098,9 @@ int make_icp_jobs(const nuif_icp_session* sessions, int n, void* host_out, int* 
         for (int j = 0; j < i; ++j)
             NUIF_CHECK_ARG(jobs[j].st != jobs[i].st, "ICP batch: a state appears twice");
     }
-    std::stable_partition(jobs, jobs + n, [](const IcpJob& j) { return j.mode == kRefOrder; });
-    *nref = (int)std::count_if(jobs, jobs + n, [](const IcpJob& j) { return j.mode == kRefOrder; });
+    std::stable_sort(jobs, jobs + n, [](const IcpJob& a, const IcpJob& b) { return a.mode < b.mode; });
+    counts[0] = (int)std::count_if(jobs, jobs + n, [](const IcpJob& j) { return j.mode == kRefOrder; });
+    counts[1] = (int)std::count_if(jobs, jobs + n, [](const IcpJob& j) { return j.mode == kCompensated; });
     return NUIF_OK;
 }
 
@@ -1026,15 +1126,17 @@ void launch_steps(const IcpJob* dj, const IcpJob* hj, int n, hipStream_t st) {
 
 // begin (all sessions), the steps of each order over its part of the table, end (all
 // sessions); dtable: device copy of hjobs
-int launch_icp_jobs(const void* dtable, const void* hjobs, int n, int nref, hipStream_t st) {
+int launch_icp_jobs(const void* dtable, const void* hjobs, int n, const int* counts, hipStream_t st) {
     if (n <= 0) return NUIF_OK;
     NUIF_CHECK_ARG(n <= kMaxBatch, "ICP batch: %d sessions exceed the limit of %d", n, kMaxBatch);
     const IcpJob* dj = static_cast<const IcpJob*>(dtable);
     const IcpJob* hj = static_cast<const IcpJob*>(hjobs);
     icp_begin_kernel<<<n, 64, 0, st>>>(dj, IcpJob{});
     NUIF_LAUNCHED("icp_begin_kernel");
+    const int nref = counts[0], ncomp = counts[1], njoint = n - nref - ncomp;
     if (nref > 0) launch_steps<kRefOrder>(dj, hj, nref, st);
-    if (nref < n) launch_steps<kCompensated>(dj + nref, hj + nref, n - nref, st);
+    if (ncomp > 0) launch_steps<kCompensated>(dj + nref, hj + nref, ncomp, st);
+    if (njoint > 0) launch_steps<kJoint>(dj + nref + ncomp, hj + nref + ncomp, njoint, st);
     NUIF_LAUNCHED("icp_tiles_kernel / icp_finish_kernel");
     icp_end_kernel<<<n, 64, 0, st>>>(dj, IcpJob{});
     NUIF_LAUNCHED("icp_end_kernel");
@@ -1054,6 +1156,9 @@ int run_single(const IcpJob& J, hipStream_t st, bool end) {
         if (J.mode == kRefOrder) {
             icp_tiles_kernel<kRefOrder><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e, remap);
             icp_finish_kernel<kRefOrder><<<1, kFinBlock, 0, st>>>(nullptr, J, e);
+        } else if (J.mode == kJoint) {
+            icp_tiles_kernel<kJoint><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e, remap);
+            icp_finish_kernel<kJoint><<<1, kFinBlock, 0, st>>>(nullptr, J, e);
         } else {
             icp_tiles_kernel<kCompensated><<<grid, kBlock, lds, st>>>(nullptr, 1, J, e, remap);
             icp_finish_kernel<kCompensated><<<1, kFinBlock, 0, st>>>(nullptr, J, e);
@@ -1251,6 +1356,32 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
     return run_single(J, stream_of(s), true);
 }
 
+int nuif_icp_estimate_joint(void* state, float* partials, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
+                            const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
+                            const nuif_icp_config* cfg, const nuif_icp_photometric* photometric, float* history,
+                            int capacity, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(state && partials && cam && vmap_pyr && ref_vmap && ref_nmap && pose && cfg && photometric,
+                   "nuif_icp_estimate_joint: null argument");
+    NUIF_CHECK_ARG(cfg->reduction == NUIF_ICP_REDUCE_JOINT, "nuif_icp_estimate_joint: the reduction must be joint");
+    nuif_icp_session in{};
+    in.state = state;
+    in.partials = partials;
+    in.cam = *cam;
+    for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) in.vmap_pyr[l] = l < cfg->levels ? vmap_pyr[l] : nullptr;
+    in.ref_vmap = ref_vmap;
+    in.ref_nmap = ref_nmap;
+    in.pose = pose;
+    in.cfg = *cfg;
+    in.history = history;
+    in.capacity = capacity;
+    in.photometric = *photometric;
+    IcpJob J;
+    const int rc = make_job(in, &J);
+    if (rc) return rc;
+    return run_single(J, stream_of(s), true);
+}
+
 size_t nuif_icp_static_scratch_bytes(void) { return kStaticScratchBytes; }
 
 int nuif_icp_estimate_static(void* state, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
@@ -1276,12 +1407,12 @@ int nuif_icp_estimate_batch(const nuif_icp_session* sessions, int n, void* table
     NUIF_CHECK_ARG(n >= 0 && (n == 0 || (sessions && table)), "nuif_icp_estimate_batch: bad arguments");
     if (n == 0) return NUIF_OK;
     std::vector<IcpJob> jobs((size_t)n);
-    int nref = 0;
-    int rc = make_icp_jobs(sessions, n, jobs.data(), &nref);
+    int counts[2] = {0, 0};
+    int rc = make_icp_jobs(sessions, n, jobs.data(), counts);
     if (rc) return rc;
     hipStream_t st = stream_of(s);
     NUIF_HIP(hipMemcpyAsync(table, jobs.data(), (size_t)n * sizeof(IcpJob), hipMemcpyHostToDevice, st));
-    return launch_icp_jobs(table, jobs.data(), n, nref, st);
+    return launch_icp_jobs(table, jobs.data(), n, counts, st);
 }
 
 }  // extern "C"

@@ -248,11 +248,62 @@ bool NuiKinfuHIPDepthTracker::EstimatePose(NuiKinfuFrame* pFrame, NuiKinfuFeedba
         cam->UpdateCameraTransform(hint->rotation(), hint->translation());
     }
     const NuiCLCameraParams& c = cam->GetCameraParams();
+    if (m_icfg.reduction == NUIF_ICP_REDUCE_JOINT) {
+        preparePhotometric(f, fb);
+        check(nuif_icp_estimate_joint(m_state.get(), m_partials.as<float>(), &c, f->GetVertexPyramid(),
+                                      fb->GetVertexBuffer(), fb->GetNormalBuffer(), cam->GetCameraTransformBuffer(),
+                                      &m_icfg, &m_photo, m_history, m_historyCap, m_stream),
+              "nuif_icp_estimate_joint");
+        return true;
+    }
     check(nuif_icp_estimate(m_state.get(), m_partials.as<float>(), &c, f->GetVertexPyramid(), fb->GetVertexBuffer(),
                             fb->GetNormalBuffer(), cam->GetCameraTransformBuffer(), &m_icfg, m_history, m_historyCap,
                             m_stream),
           "nuif_icp_estimate");
     return true;
+}
+
+void NuiKinfuHIPDepthTracker::SetReduction(int mode) {
+    if (mode != NUIF_ICP_REDUCE_REFERENCE && mode != NUIF_ICP_REDUCE_COMPENSATED && mode != NUIF_ICP_REDUCE_JOINT)
+        throw std::runtime_error("unknown ICP reduction");
+    m_icfg.reduction = mode;
+    if (mode != NUIF_ICP_REDUCE_JOINT || m_refIntensity.get()) return;
+    for (int l = 0; l < m_icfg.levels; ++l) {
+        m_intensity[l] = DeviceBuffer((size_t)(m_W >> l) * (m_H >> l) * sizeof(float));
+        m_photo.intensity_pyr[l] = m_intensity[l].as<float>();
+    }
+    m_refIntensity = DeviceBuffer((size_t)m_W * m_H * sizeof(float));
+    m_refGradient = DeviceBuffer((size_t)m_W * m_H * 2 * sizeof(float));
+    m_photo.ref_intensity = m_refIntensity.as<float>();
+    m_photo.ref_gradient = m_refGradient.as<float>();
+    if (m_photo.weight == 0.f) SetPhotometric(0.1f, 0.2f, 0.002f);
+}
+
+void NuiKinfuHIPDepthTracker::SetPhotometric(float weight, float maxDifference, float minGradient) {
+    if (!(weight >= 0.f) || !(maxDifference > 0.f) || !(minGradient >= 0.f))
+        throw std::runtime_error("bad photometric parameters");
+    m_photo.weight = weight;
+    m_photo.max_difference = maxDifference;
+    m_photo.min_gradient = minGradient;
+}
+
+// The joint order's inputs: intensities of the frame's colours at every level, and of
+// the feedback colours (the previous raycast's, or the first frame's) with their gradients.
+void NuiKinfuHIPDepthTracker::preparePhotometric(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb) {
+    if (!f->GetColorBuffer())
+        throw std::runtime_error("the joint ICP needs the frame's colours (and a scene with a colour volume)");
+    const size_t n = (size_t)m_W * m_H;
+    check(nuif_bgra_to_intensity(f->GetColorBuffer(), m_intensity[0].as<float>(), n, m_stream),
+          "nuif_bgra_to_intensity");
+    for (int l = 1; l < m_icfg.levels; ++l)
+        check(nuif_intensity_half_sample(m_intensity[l - 1].as<float>(), m_intensity[l].as<float>(), (int)(m_W >> l),
+                                         (int)(m_H >> l), m_stream),
+              "nuif_intensity_half_sample");
+    check(nuif_bgra_to_intensity(fb->GetColorBuffer(), m_refIntensity.as<float>(), n, m_stream),
+          "nuif_bgra_to_intensity");
+    check(nuif_intensity_derivatives(m_refIntensity.as<float>(), m_refGradient.as<float>(), (int)m_W, (int)m_H,
+                                     m_stream),
+          "nuif_intensity_derivatives");
 }
 
 void NuiKinfuHIPDepthTracker::FillIcpSession(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb,
@@ -268,6 +319,7 @@ void NuiKinfuHIPDepthTracker::FillIcpSession(NuiKinfuHIPFrame* f, NuiKinfuHIPFee
     out->cfg = m_icfg;
     out->history = m_history;
     out->capacity = m_historyCap;
+    if (m_icfg.reduction == NUIF_ICP_REDUCE_JOINT) out->photometric = m_photo;  // inputs made by the round
 }
 
 nuif_icp_result NuiKinfuHIPDepthTracker::ReadState() const {
@@ -719,7 +771,7 @@ bool NuiKinfuTrackingEngine::RunTrackingDevice(INT64, const UINT16* dDepths, con
     return r.status == 0;
 }
 
-bool NuiKinfuTrackingEngine::PrepareRoundFrame(const UINT16* dDepths, NuiKinfuScene* pScene,
+bool NuiKinfuTrackingEngine::PrepareRoundFrame(const UINT16* dDepths, const uint8_t* dBgra, NuiKinfuScene* pScene,
                                                const NuiCameraParams& cameraParams, nuif_round_session* out) {
     if (!m_pTracker || !m_pFrame || !m_pCameraState || !dDepths || m_frames == 0 || m_stageTiming) return false;
     auto* scene = dynamic_cast<NuiKinfuHIPScene*>(pScene);
@@ -742,6 +794,9 @@ bool NuiKinfuTrackingEngine::PrepareRoundFrame(const UINT16* dDepths, NuiKinfuSc
     out->vmap = m_pFeedbackFrame->GetVertexBuffer();
     out->nmap = m_pFeedbackFrame->GetNormalBuffer();
     out->cmap = out->color_volume ? m_pFeedbackFrame->GetColorBuffer() : nullptr;
+    out->bgra = out->color_volume ? dBgra : nullptr;
+    if (m_pTracker->Reduction() == NUIF_ICP_REDUCE_JOINT && !out->bgra)
+        throw std::runtime_error("the joint ICP needs the frame's colours");
     m_frames++;
     return true;
 }

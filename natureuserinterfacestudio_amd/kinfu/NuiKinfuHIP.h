@@ -145,9 +145,14 @@ public:
     // frame without ICP (first frame): records the pose and advances the frame count
     void RecordPose(NuiKinfuHIPCameraState* cam);
     void SetIntegrationThreshold(float v) { m_icfg.gate_threshold = v; }
-    // NUIF_ICP_REDUCE_REFERENCE (default: the reference's float summation order) or
-    // NUIF_ICP_REDUCE_COMPENSATED (Kahan FP32 + FP64)
-    void SetReduction(int mode) { m_icfg.reduction = mode; }
+    // NUIF_ICP_REDUCE_REFERENCE (default: the reference's float summation order),
+    // NUIF_ICP_REDUCE_COMPENSATED (Kahan FP32 + FP64) or NUIF_ICP_REDUCE_JOINT (the
+    // compensated order with photometric rows against the feedback colours: the frame
+    // and the feedback frame must carry colours)
+    void SetReduction(int mode);
+    int Reduction() const { return m_icfg.reduction; }
+    // the joint order's photometric weight, intensity difference and minimum gradient
+    void SetPhotometric(float weight, float maxDifference, float minGradient);
     void SetHistory(float* dHistory, int capacity) { m_history = dHistory; m_historyCap = capacity; }
     nuif_icp_result ReadState() const;
     // the arguments of this tracker's nuif_icp_estimate for a frame (a batched round's entry)
@@ -164,6 +169,10 @@ private:
     DeviceBuffer m_state, m_partials;
     float* m_history = nullptr;
     int m_historyCap = 0;
+    // the joint order: intensity pyramid of the frame, the feedback intensities and gradients
+    DeviceBuffer m_intensity[NUIF_ICP_MAX_LEVELS], m_refIntensity, m_refGradient;
+    nuif_icp_photometric m_photo{};
+    void preparePhotometric(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb);
 };
 
 // OCL/NuiKinfuOpenCLColorTracker.*: photometric tracker against the feedback
@@ -437,8 +446,9 @@ public:
     // (nuif_fusion_round): fills *out from the session's buffers and counts the frame.
     // Returns false, doing nothing, when the frame must run alone: the first frame, a
     // scene other than a plain dense volume of batchable size, or stage timing on.
-    bool PrepareRoundFrame(const UINT16* dDepths, NuiKinfuScene* pScene, const NuiCameraParams& cameraParams,
-                           nuif_round_session* out);
+    // dBgra: the frame's colours (device) or NULL.
+    bool PrepareRoundFrame(const UINT16* dDepths, const uint8_t* dBgra, NuiKinfuScene* pScene,
+                           const NuiCameraParams& cameraParams, nuif_round_session* out);
     // Per-stage device timing of tracked frames (events around preprocessing, ICP,
     // integration and raycast on the session stream; frames then run un-captured).
     void setStageTiming(bool on);

@@ -33,6 +33,8 @@ std::unique_ptr<NuiKinfuSession> NuiKinfuSession::create(const nuifk_desc& d) {
     s->engine->setIntegrationMetricThreshold(d.integration_threshold);
     s->engine->initialize(trackerConfig(d), false, (UINT)d.width, (UINT)d.height);
     s->engine->setUseGraph(d.use_graph != 0);
+    if (d.icp_reduction == NUIF_ICP_REDUCE_JOINT && !d.color)
+        throw std::runtime_error("session: the joint ICP needs the colour volume (color = 1)");
     s->engine->tracker()->SetReduction(d.icp_reduction);
     if (d.hash_forward_render < NUIFK_RENDER_CASTRAY || d.hash_forward_render > NUIFK_RENDER_FORWARD_REFERENCE)
         throw std::runtime_error("session: unknown hash_forward_render mode");
@@ -106,6 +108,10 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
     for (int l = 0; l < d.levels; ++l) bytes += (size_t)(d.width >> l) * (d.height >> l) * (4 + 12);
     bytes += n * (12 + 12 + 4);  // feedback vertex / normal / colour maps
     bytes += nuif_icp_state_bytes() + nuif_icp_partials_bytes(d.width, d.height);
+    if (d.icp_reduction == NUIF_ICP_REDUCE_JOINT) {  // intensity pyramid, feedback intensities and gradients
+        for (int l = 0; l < d.levels; ++l) bytes += (size_t)(d.width >> l) * (d.height >> l) * 4;
+        bytes += n * 12;
+    }
     bytes += (size_t)(1 << 16) * NUIF_POSE_ROW * sizeof(float) + 256;  // pose history, camera pose
     return bytes;
 }
@@ -207,7 +213,7 @@ int NuiFusionServer::sessionCount() const {
     return n;
 }
 
-void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depths) {
+void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depths, const uint8_t* const* bgras) {
     if (n < 0 || (n > 0 && (!ids || !depths))) throw std::runtime_error("server submit: bad arguments");
     for (int i = 0; i < n; ++i) {
         session(ids[i]);
@@ -235,7 +241,8 @@ void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depth
     for (int i = 0; i < n; ++i) {
         NuiKinfuSession* s = m_sessions[ids[i]].get();
         nuif_round_session rs;
-        if (m_batching && s->engine->PrepareRoundFrame(depths[i], s->scene.get(), s->cam, &rs)) {
+        const uint8_t* bgra = bgras ? bgras[i] : nullptr;
+        if (m_batching && s->engine->PrepareRoundFrame(depths[i], bgra, s->scene.get(), s->cam, &rs)) {
             m_slot[i] = (int)batch.size();
             m_status[m_slot[i]] = -1;
             rs.icp.outcome = m_status + m_slot[i];
@@ -245,7 +252,7 @@ void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depth
         }
         const nuif_stream_t st = s->engine->stream();
         check(nuif_stream_wait_event(st, m_start), "round wait");
-        s->engine->RunTrackingDevice(0, depths[i], nullptr, s->scene.get(), s->cam, false);
+        s->engine->RunTrackingDevice(0, depths[i], bgra, s->scene.get(), s->cam, false);
         s->afterFrame();
         check(nuif_event_record(m_ends[ids[i]], st), "frame end");
     }

@@ -53,7 +53,7 @@ public:
     // tracked frames of plain dense sessions run together as one batched round
     // (nuif_fusion_round) on the server's stream; every other frame (a session's first,
     // hash and shifting scenes) runs on its session's own stream, overlapping the batch.
-    void submit(int n, const int* ids, const uint16_t* const* depths);
+    void submit(int n, const int* ids, const uint16_t* const* depths, const uint8_t* const* bgras = nullptr);
     // waits for the round; latencyMs[i] = round start -> end of the i-th frame.  Then
     // each frame's tracking outcome is known (lastOutcomes) and the failure policy is
     // applied to the sessions whose frame failed.

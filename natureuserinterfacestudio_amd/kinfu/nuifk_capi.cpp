@@ -262,6 +262,26 @@ int nuifk_read_maps(void* h, float* vmap, float* nmap) {
     });
 }
 
+int nuifk_set_photometric(void* h, float weight, float max_difference, float min_gradient) {
+    return guarded([&] {
+        Session* s = static_cast<Session*>(h);
+        s->engine->tracker()->SetPhotometric(weight, max_difference, min_gradient);
+        return 0;
+    });
+}
+
+int nuifk_read_feedback_colors(void* h, uint8_t* out) {
+    return guarded([&] {
+        Session* s = static_cast<Session*>(h);
+        if (!out) throw std::runtime_error("nuifk_read_feedback_colors: null output");
+        check(nuif_memcpy_d2h(out, s->engine->feedbackFrame()->GetColorBuffer(), (size_t)s->W * s->H * 4,
+                              s->engine->stream()),
+              "feedback colours");
+        check(nuif_stream_synchronize(s->engine->stream()), "feedback colours");
+        return 0;
+    });
+}
+
 // NuiTrackerConfig::load / log (VO/NuiTrackerConfig.cpp:26-115) into / from a descriptor.
 // Keys absent from the file keep the descriptor's values; returns -1 if the file cannot be opened.
 int nuifk_load_tracker_config(const char* path, nuifk_desc* d) {
@@ -374,6 +394,15 @@ int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths
     return guarded([&] {
         if (!srv) throw std::runtime_error("nuifs_submit: null server");
         static_cast<NuiFusionServer*>(srv)->submit(n, ids, depths);
+        return 0;
+    });
+}
+
+int nuifs_submit_color(void* srv, int n, const int* ids, const uint16_t* const* depths,
+                       const uint8_t* const* bgras) {
+    return guarded([&] {
+        if (!srv) throw std::runtime_error("nuifs_submit_color: null server");
+        static_cast<NuiFusionServer*>(srv)->submit(n, ids, depths, bgras);
         return 0;
     });
 }

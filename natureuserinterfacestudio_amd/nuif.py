@@ -64,7 +64,13 @@ class IcpConfig(C.Structure):
                 ("gate_threshold", c_float), ("reduction", c_int)]
 
 
-ICP_REDUCE_REFERENCE, ICP_REDUCE_COMPENSATED = 0, 1
+ICP_REDUCE_REFERENCE, ICP_REDUCE_COMPENSATED, ICP_REDUCE_JOINT = 0, 1, 2
+
+
+class IcpPhotometric(C.Structure):
+    _fields_ = [("intensity_pyr", c_void_p * ICP_MAX_LEVELS), ("ref_intensity", c_void_p),
+                ("ref_gradient", c_void_p), ("weight", c_float), ("max_difference", c_float),
+                ("min_gradient", c_float)]
 
 
 class ColorIcpResult(C.Structure):
@@ -225,6 +231,10 @@ _SIGS = {
     "nuif_color_icp_read": (c_int, [c_void_p, C.POINTER(ColorIcpResult), c_void_p]),
     "nuif_bgra_to_intensity": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "nuif_intensity_derivatives": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "nuif_intensity_half_sample": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "nuif_icp_estimate_joint": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p,
+                                        c_void_p, c_void_p, C.POINTER(IcpConfig), C.POINTER(IcpPhotometric),
+                                        c_void_p, c_int, c_void_p]),
     "nuif_intensity_icp_workspace_bytes": (c_size_t, [c_int, c_int]),
     "nuif_intensity_icp_sums": (c_int, [C.POINTER(NuiCLCameraParams), c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, C.POINTER(c_float),
@@ -273,6 +283,8 @@ _KSIGS = {
     "nuifk_state": (c_int, [c_void_p, C.POINTER(IcpResult)]),
     "nuifk_read_volume": (c_int, [c_void_p, c_void_p, c_size_t]),
     "nuifk_read_maps": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "nuifk_read_feedback_colors": (c_int, [c_void_p, c_void_p]),
+    "nuifk_set_photometric": (c_int, [c_void_p, c_float, c_float, c_float]),
     "nuifk_stream": (c_void_p, [c_void_p]),
     "nuifk_volume_ptr": (c_void_p, [c_void_p]),
     "nuifk_volume_bytes": (c_size_t, [c_void_p]),
@@ -294,6 +306,7 @@ _KSIGS = {
     "nuifs_session_count": (c_int, [c_void_p]),
     "nuifs_engine": (c_void_p, [c_void_p, c_int]),
     "nuifs_submit": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p)]),
+    "nuifs_submit_color": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p), C.POINTER(c_void_p)]),
     "nuifs_wait": (c_int, [c_void_p, C.POINTER(c_float)]),
     "nuifs_set_failure_policy": (c_int, [c_void_p, c_int]),
     "nuifs_set_stage_timing": (c_int, [c_void_p, c_int]),
@@ -659,6 +672,16 @@ class Engine:
         self._check(K().nuifk_read_maps(self._h, v.ctypes.data, nm.ctypes.data), "maps")
         return v, nm
 
+    def set_photometric(self, weight, max_difference=0.2, min_gradient=0.002):
+        """The joint ICP order's photometric weight, intensity difference and minimum gradient."""
+        self._check(K().nuifk_set_photometric(self._h, weight, max_difference, min_gradient), "photometric")
+
+    def feedback_colors(self):
+        """The feedback frame's BGRA colours (H*W x 4 uint8)."""
+        out = np.empty((self.desc.width * self.desc.height, 4), np.uint8)
+        self._check(K().nuifk_read_feedback_colors(self._h, out.ctypes.data), "feedback colours")
+        return out
+
     def set_stage_timing(self, on):
         self._check(K().nuifk_set_stage_timing(self._h, 1 if on else 0), "stage timing")
 
@@ -742,13 +765,18 @@ class Server:
     def bytes_in_use(self):
         return K().nuifs_bytes_in_use(self._h)
 
-    def submit(self, ids, frames):
-        """ids: session ids; frames: DeviceArray uint16 depth per id."""
+    def submit(self, ids, frames, colors=None):
+        """ids: session ids; frames: DeviceArray uint16 depth per id; colors: None or a
+        DeviceArray of BGRA bytes (or None) per id."""
         n = len(ids)
         self._ids = (c_int * n)(*ids)
         self._ptrs = (c_void_p * n)(*[f.ptr for f in frames])
         self._n = n
-        self._check(K().nuifs_submit(self._h, n, self._ids, self._ptrs), "nuifs_submit")
+        if colors is None:
+            self._check(K().nuifs_submit(self._h, n, self._ids, self._ptrs), "nuifs_submit")
+            return
+        self._cptrs = (c_void_p * n)(*[c.ptr if c is not None else None for c in colors])
+        self._check(K().nuifs_submit_color(self._h, n, self._ids, self._ptrs, self._cptrs), "nuifs_submit_color")
 
     def set_batching(self, on):
         """Batched rounds (nuif_fusion_round) on or off."""

@@ -186,15 +186,24 @@ template <class T> V3<T> bilinear_holes(const T* src, T px, T py, int W, int H) 
     return r;
 }
 
-// K/icp.cl:222-292 icp_block_kernel, one pixel: fills the 29 terms (zeros if rejected)
+// Where a pixel's association lands: the current vertex at the estimate (pv), in the
+// previous camera (pp), and its full-resolution projection (px, py).
+template <class T> struct AssocAt {
+    V3<T> pv, pp;
+    T px, py;
+};
+
+// K/icp.cl:222-292 icp_block_kernel, one pixel: fills the 29 terms (zeros if rejected);
+// `at` (optional) receives where the association lands once it is projected.
 template <class T>
 void icp_pixel(const Cam<T>& cam, V3<T> vc, const T R[9], const T t[3], const T* vref, const T* nref,
-               const Rigid<T>& prev, T distThr, T out[kTerms]) {
+               const Rigid<T>& prev, T distThr, T out[kTerms], AssocAt<T>* at = nullptr, T* coef = nullptr) {
     for (int k = 0; k < kTerms; ++k) out[k] = 0;
     if (isnan3(vc)) return;
     V3<T> pv = rotate(R, vc) + mk(t[0], t[1], t[2]);
     V3<T> pp = transform_inverse(prev, pv);
     T px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
+    if (at) *at = AssocAt<T>{pv, pp, px, py};
     if (!(px >= 0 && px < (T)cam.W && py >= 0 && py < (T)cam.H)) return;
     V3<T> rn = bilinear_holes(nref, px, py, cam.W, cam.H);
     if (isnan3(rn)) return;
@@ -205,6 +214,8 @@ void icp_pixel(const Cam<T>& cam, V3<T> vc, const T R[9], const T t[3], const T*
     if (!(dist < distThr)) return;
     V3<T> row0 = cross(rn, pv);
     T c[7] = {row0.x, row0.y, row0.z, rn.x, rn.y, rn.z, dot(rn, diff)};
+    if (coef)
+        for (int k = 0; k < 7; ++k) coef[k] = c[k];
     int s = 0;
     for (int i = 0; i < 6; ++i)
         for (int j = i; j < 7; ++j) out[s++] = c[i] * c[j];
@@ -463,6 +474,115 @@ IcpResult icp(const Cam<T>& cam, const IcpConfig& cfg, const T* const* vpyr, con
 //     applied as an SE(3) exponential in FP64 (nuif_se3::gn_step); a step below
 //     1e-6 ends the level; a singular system is a tracking failure.
 
+// ---------------------------------------------------------------- ICP, joint order
+//
+// The joint order (no counterpart in the reference: its photometric terms,
+// K/icp.cl:352-480, have no host loop) is the compensated order with a photometric row
+// per associated pixel added to the geometric row's products.  Parity unpinned against
+// the reference; the rows are pinned by closed forms in the tests (a constant image
+// gives no rows, a linear image's row against finite differences of the projection).
+
+// Intensities one level down: the mean of the valid ones of the 2 x 2 source pixels,
+// summed in row order, NaN when none is valid (no counterpart in the reference).
+template <class T> void intensity_half_sample(const T* src, T* dst, int Wd, int Hd) {
+    const int Ws = 2 * Wd;
+    for (int y = 0; y < Hd; ++y)
+        for (int x = 0; x < Wd; ++x) {
+            const size_t i = (size_t)(2 * y) * Ws + 2 * x;
+            const T v[4] = {src[i], src[i + 1], src[i + Ws], src[i + Ws + 1]};
+            T sum = 0;
+            int cnt = 0;
+            for (int k = 0; k < 4; ++k)
+                if (!std::isnan(v[k])) {
+                    sum += v[k];
+                    ++cnt;
+                }
+            dst[(size_t)y * Wd + x] = cnt ? sum / (T)cnt : qnan<T>();
+        }
+}
+
+// Bilinear samples of a scalar / float2 image with the holes rule of bilinear_holes.
+template <class T> T bilinear1(const T* src, T px, T py, int W, int H) {
+    const int x = (int)std::floor(px), y = (int)std::floor(py);
+    const T dx = px - (T)x, dy = py - (T)y;
+    const long long n = (long long)W * H;
+    const long long ia = x + (long long)y * W, ib = ia + 1, ic = ia + W, id = ic + 1;
+    if (ia < 0 || id >= n) return qnan<T>();
+    const T a = src[ia], b = src[ib], c = src[ic], d = src[id];
+    if (std::isnan(a) || std::isnan(b) || std::isnan(c) || std::isnan(d)) return qnan<T>();
+    return a * (T(1) - dx) * (T(1) - dy) + b * dx * (T(1) - dy) + c * (T(1) - dx) * dy + d * dx * dy;
+}
+template <class T> bool bilinear2(const T* src, T px, T py, int W, int H, T g[2]) {
+    const int x = (int)std::floor(px), y = (int)std::floor(py);
+    const T dx = px - (T)x, dy = py - (T)y;
+    const long long n = (long long)W * H;
+    const long long ia = x + (long long)y * W, ib = ia + 1, ic = ia + W, id = ic + 1;
+    if (ia < 0 || id >= n) return false;
+    for (int k = 0; k < 2; ++k) {
+        const T a = src[2 * ia + k], b = src[2 * ib + k], c = src[2 * ic + k], d = src[2 * id + k];
+        if (std::isnan(a) || std::isnan(b) || std::isnan(c) || std::isnan(d)) return false;
+    }
+    for (int k = 0; k < 2; ++k) {
+        const T a = src[2 * ia + k], b = src[2 * ib + k], c = src[2 * ic + k], d = src[2 * id + k];
+        g[k] = a * (T(1) - dx) * (T(1) - dy) + b * dx * (T(1) - dy) + c * (T(1) - dx) * dy + d * dx * dy;
+    }
+    return true;
+}
+
+// The photometric inputs of one level: the current intensities at the level, the
+// model's intensities and gradients (full resolution), sqrt(weight), the largest
+// intensity difference and the squared minimum gradient.
+template <class T> struct PhotoIn {
+    const T* icur;
+    const T* iref;
+    const T* gref;
+    T psw, pdiff, pgrad2;
+};
+
+// The photometric row of an associated pixel: false when it gives none.  ic: the
+// current intensity at the pixel.  I(u, v) with u = fx x / z + cx, v = fy y / z + cy in
+// the previous camera (p' = Ri (pv - tp)): dI/dp' = (gx fx / z, gy fy / z,
+// -(gx fx x + gy fy y) / z^2), carried to world coordinates by Ri^T; the row is
+// sqrt(w) [J x pv, J, I - I_ref].
+template <class T>
+bool photometric_row(T ic, const AssocAt<T>& at, const Rigid<T>& prev, const Cam<T>& cam, const PhotoIn<T>& ph,
+                     T c[7]) {
+    if (std::isnan(ic)) return false;
+    const T ir = bilinear1(ph.iref, at.px, at.py, cam.W, cam.H);
+    T g[2];
+    const bool gok = bilinear2(ph.gref, at.px, at.py, cam.W, cam.H, g);
+    if (std::isnan(ir) || !gok) return false;
+    const T di = ic - ir;
+    if (!(std::fabs(di) < ph.pdiff) || !(g[0] * g[0] + g[1] * g[1] > ph.pgrad2)) return false;
+    const T iz = T(1) / at.pp.z;
+    const T ax = g[0] * cam.fx, ay = g[1] * cam.fy;
+    const T jc[3] = {ax * iz, ay * iz, -((ax * at.pp.x + ay * at.pp.y) * iz) * iz};
+    const T* Ri = prev.R_inv;
+    const V3<T> jw = mk((jc[0] * Ri[0] + jc[1] * Ri[3]) + jc[2] * Ri[6], (jc[0] * Ri[1] + jc[1] * Ri[4]) + jc[2] * Ri[7],
+                        (jc[0] * Ri[2] + jc[1] * Ri[5]) + jc[2] * Ri[8]);
+    const V3<T> a = cross(jw, at.pv);
+    const T r[7] = {a.x, a.y, a.z, jw.x, jw.y, jw.z, di};
+    for (int k = 0; k < 7; ++k) c[k] = ph.psw * r[k];
+    return true;
+}
+
+// A pixel's 29 values in the joint order: the geometric products plus, where the
+// photometric row exists, its products (the count only for the geometric row).
+template <class T>
+void joint_pixel(const Cam<T>& cam, V3<T> vc, const T R[9], const T t[3], const T* vref, const T* nref,
+                 const Rigid<T>& prev, T distThr, const PhotoIn<T>& ph, int p, T out[kTerms]) {
+    AssocAt<T> at;
+    T cf[7];
+    icp_pixel(cam, vc, R, t, vref, nref, prev, distThr, out, &at, cf);
+    if (out[kTerms - 1] == T(0)) return;
+    T pc[7];
+    if (!photometric_row(ph.icur[p], at, prev, cam, ph, pc)) return;
+    int s = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j, ++s) out[s] = out[s] + pc[i] * pc[j];
+    out[s] = out[s] + pc[6] * pc[6];
+}
+
 // The GPU's compensated reduction of one iteration (nuif ICP kernel): tiles of T =
 // kCompTilePixels pixels; in tile k thread j (256 threads) takes pixels Tk + j + 256i,
 // i = 0..T/256-1, with Kahan-compensated FP32 sums (the count plainly); each thread's value s - c
@@ -472,7 +592,7 @@ IcpResult icp(const Cam<T>& cam, const IcpConfig& cfg, const T* const* vpyr, con
 // j = 0..7, and those eight in order.
 inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int npx, const float R[9], const float t[3],
                                  const float* vref, const float* nref, const Rigid<float>& prev, float distThr,
-                                 double out[kTerms]) {
+                                 double out[kTerms], const PhotoIn<float>* ph = nullptr) {
     constexpr int T = nuif_se3::kCompTilePixels, PER = T / 256;
     const int ntiles = (npx + T - 1) / T;
     std::vector<double> tiles((size_t)ntiles * kTerms);
@@ -485,7 +605,10 @@ inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int n
             for (int i = 0; i < PER; ++i) {
                 const int p = T * k + j + 256 * i;
                 if (p >= npx) continue;
-                icp_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, o);
+                if (ph)
+                    joint_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, *ph, p, o);
+                else
+                    icp_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, o);
                 if (o[kTerms - 1] == 0.f) continue;  // rejected: every term is zero
                 for (int q = 0; q < kTerms - 1; ++q) {
                     const float y = o[q] - cp[q];
@@ -524,11 +647,14 @@ inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int n
 
 inline void icp_sums_compensated(const Cam<double>& cam, const double* vcur, int npx, const double R[9],
                                  const double t[3], const double* vref, const double* nref, const Rigid<double>& prev,
-                                 double distThr, double out[kTerms]) {
+                                 double distThr, double out[kTerms], const PhotoIn<double>* ph = nullptr) {
     for (int q = 0; q < kTerms; ++q) out[q] = 0.0;
     double o[kTerms];
     for (int p = 0; p < npx; ++p) {
-        icp_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, o);
+        if (ph)
+            joint_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, *ph, p, o);
+        else
+            icp_pixel(cam, load3(vcur, p), R, t, vref, nref, prev, distThr, o);
         for (int q = 0; q < kTerms; ++q) out[q] += o[q];
     }
 }
@@ -536,7 +662,8 @@ inline void icp_sums_compensated(const Cam<double>& cam, const double* vcur, int
 // The compensated mode's iteration loop; (Rd, td) in/out, the pose in FP64.
 template <class T>
 IcpResult icp_compensated(const Cam<T>& cam, const IcpConfig& cfg, const T* const* vpyr, const T* vref,
-                          const T* nref, const Rigid<T>& prev, double Rd[9], double td[3]) {
+                          const T* nref, const Rigid<T>& prev, double Rd[9], double td[3],
+                          const PhotoIn<T>* photo = nullptr, const T* const* ipyr = nullptr) {
     IcpResult res{1, 0, 0, 0};
     const int L = cfg.levels;
     const T step = (T)cfg.dist_threshold / (T)L;
@@ -550,7 +677,12 @@ IcpResult icp_compensated(const Cam<T>& cam, const IcpConfig& cfg, const T* cons
             for (int k = 0; k < 9; ++k) R[k] = (T)Rd[k];
             for (int k = 0; k < 3; ++k) t[k] = (T)td[k];
             double sums[kTerms];
-            icp_sums_compensated(cam, vpyr[level], npx, R, t, vref, nref, prev, distThr, sums);
+            PhotoIn<T> ph{};
+            if (photo) {  // the joint order: the level's current intensities
+                ph = *photo;
+                ph.icur = ipyr[level];
+            }
+            icp_sums_compensated(cam, vpyr[level], npx, R, t, vref, nref, prev, distThr, sums, photo ? &ph : nullptr);
             res.total_iters++;
             const int groups = (npx + 127) / 128;
             kgs = (size_t)((groups + 127) / 128) * 128;

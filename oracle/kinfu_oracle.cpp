@@ -145,11 +145,17 @@ struct EngineCfg {
     int max_color_weight;
     int voxel_shift;       // > 0: shifting volume (VO/NuiKinfuVolumeConfig.cpp:13 voxel_shift)
     float basis[3];        // translateBasis
-    int icp_mode = 0;      // 0: the reference's ICP loop (icp), 1: the compensated mode (icp_compensated)
+    int icp_mode = 0;      // 0: the reference's ICP loop (icp), 1: the compensated mode (icp_compensated),
+                           // 2: the joint order (icp_compensated with photometric rows; float engines)
+    float photo[3] = {0.1f, 0.2f, 0.002f};  // joint order: weight, intensity difference, minimum gradient
 };
 
 // OCL/NuiKinfuOpenCLShiftScene.cpp:119-245 fetchSlice reads the slab with
 // K/extract.cl:80-190; the restatement lives in points_oracle.cpp.
+// K/transformMaps.cl:372-384 and :448-511, restated in color_oracle.cpp.
+extern "C" void or_bgra_to_intensity(const uint8_t* bgra, float* out, long long n);
+extern "C" void or_intensity_derivatives(const float* I, float* out, int W, int H);
+
 extern "C" long long or_points_dense_slice(const void* vol, const uint8_t* cvol, int storage, int Nx, int Ny, int Nz,
                                            const float* cell, const int* wrap, const int* offset, const int* range,
                                            float* pts, float* cols, long long cap);
@@ -174,6 +180,11 @@ template <class T, class Vol> struct Engine {
     // shifting volume: m_voxel_offset and the evicted points (NuiKinfuVertexCache)
     int off[3] = {0, 0, 0};
     std::vector<float> cache_pts, cache_cols;
+    // joint order: the feedback colours (first frame's, then the raycast's), the frame's
+    // intensity pyramid, the feedback intensities and their gradients
+    std::vector<uint8_t> cfb;
+    std::vector<std::vector<float>> ipyr;
+    std::vector<float> iref, gref;
 
     explicit Engine(const EngineCfg& cfg) : c(cfg) {
         cam = make_cam<T>(c.cam, c.W, c.H);
@@ -274,6 +285,35 @@ template <class T, class Vol> struct Engine {
         integrated_last = true;
     }
 
+    // The joint order's ICP: intensities of the frame's colours per level and of the
+    // feedback colours with their gradients, then icp_compensated with photometric rows.
+    IcpResult joint_icp(const IcpConfig& ic, const T* const* vp, const Rigid<T>& pose, const uint8_t* bgra, double Rn[9],
+                        double tn[3]) {
+        if constexpr (!std::is_same<T, float>::value) {
+            (void)ic, (void)vp, (void)pose, (void)bgra, (void)Rn, (void)tn;
+            return IcpResult{0, 0, 0, 0};  // the joint order is a float engine's
+        } else {
+            if (!bgra || cfb.empty()) return IcpResult{0, 0, 0, 0};  // the joint order needs colours
+            const size_t n = (size_t)c.W * c.H;
+            ipyr.resize(c.levels);
+            ipyr[0].resize(n);
+            or_bgra_to_intensity(bgra, ipyr[0].data(), (long long)n);
+            for (int l = 1; l < c.levels; ++l) {
+                ipyr[l].resize((size_t)(c.W >> l) * (c.H >> l));
+                intensity_half_sample<float>(ipyr[l - 1].data(), ipyr[l].data(), c.W >> l, c.H >> l);
+            }
+            iref.resize(n);
+            gref.resize(2 * n);
+            or_bgra_to_intensity(cfb.data(), iref.data(), (long long)n);
+            or_intensity_derivatives(iref.data(), gref.data(), c.W, c.H);
+            PhotoIn<float> ph{nullptr, iref.data(), gref.data(), std::sqrt(c.photo[0]), c.photo[1],
+                              c.photo[2] * c.photo[2]};
+            std::vector<const float*> ip(c.levels);
+            for (int l = 0; l < c.levels; ++l) ip[l] = ipyr[l].data();
+            return icp_compensated<float>(cam, ic, vp, vref.data(), nref.data(), pose, Rn, tn, &ph, ip.data());
+        }
+    }
+
     // returns 1 ok, 0 tracking failure
     int run(const uint16_t* raw, const uint8_t* bgra) {
         const int W = c.W, H = c.H;
@@ -290,6 +330,7 @@ template <class T, class Vol> struct Engine {
             std::vector<T> nm(3 * n);
             normals_simple<T>(vpyr[0].data(), nm.data(), W, H, (T)c.depth_threshold);
             transform_maps<T>(vpyr[0].data(), nm.data(), vref.data(), nref.data(), n, pose);
+            if (bgra) cfb.assign(bgra, bgra + 4 * n);
             integrate_frame(bgra);
         } else {
             // NuiKinfuOpenCLDepthTracker::EstimatePose: pyramid, vertices, ICP
@@ -303,11 +344,14 @@ template <class T, class Vol> struct Engine {
             ic.dist_threshold = c.dist_threshold;
             std::vector<const T*> vp(c.levels);
             for (int l = 0; l < c.levels; ++l) vp[l] = vpyr[l].data();
-            if (c.icp_mode == 1) {
+            if (c.icp_mode == 1 || c.icp_mode == 2) {
                 double Rn[9], tn[3];
                 std::memcpy(Rn, Rd, sizeof(Rn));
                 std::memcpy(tn, td, sizeof(tn));
-                last = icp_compensated<T>(cam, ic, vp.data(), vref.data(), nref.data(), pose, Rn, tn);
+                if (c.icp_mode == 2)
+                    last = joint_icp(ic, vp.data(), pose, bgra, Rn, tn);
+                else
+                    last = icp_compensated<T>(cam, ic, vp.data(), vref.data(), nref.data(), pose, Rn, tn);
                 if (!last.ok) return 0;
                 std::memcpy(Rd, Rn, sizeof(Rn));
                 std::memcpy(td, tn, sizeof(tn));
@@ -339,8 +383,9 @@ template <class T, class Vol> struct Engine {
                 std::memcpy(tlast, t, sizeof(tlast));
             }
             const Rigid<T> np = make_rigid<T>(R, t);
-            raycast<T, Vol>(vol, bgra ? cvol.data() : nullptr, prm, cam, np, vref.data(), nref.data(), nullptr,
-                            wrap());
+            if (bgra && !cvol.empty() && cfb.size() != 4 * n) cfb.assign(4 * n, 0);
+            raycast<T, Vol>(vol, bgra ? cvol.data() : nullptr, prm, cam, np, vref.data(), nref.data(),
+                            bgra && !cvol.empty() ? cfb.data() : nullptr, wrap());
         }
         frames++;
         return 1;
@@ -425,8 +470,8 @@ OR_EXPORT int or_engine_run(void* hp, const uint16_t* raw, const uint8_t* bgra, 
     return with_engine((OrEngine*)hp, [&](auto& e) {
         const int ok = e.run(raw, bgra);
         if (pose12) {
-            for (int k = 0; k < 9; ++k) pose12[k] = e.c.icp_mode == 1 ? e.Rd[k] : (double)e.R[k];
-            for (int k = 0; k < 3; ++k) pose12[9 + k] = e.c.icp_mode == 1 ? e.td[k] : (double)e.t[k];
+            for (int k = 0; k < 9; ++k) pose12[k] = e.c.icp_mode != 0 ? e.Rd[k] : (double)e.R[k];
+            for (int k = 0; k < 3; ++k) pose12[9 + k] = e.c.icp_mode != 0 ? e.td[k] : (double)e.t[k];
         }
         if (icp_info3) {
             icp_info3[0] = e.last.error;
@@ -437,10 +482,21 @@ OR_EXPORT int or_engine_run(void* hp, const uint16_t* raw, const uint8_t* bgra, 
     });
 }
 
-// ICP loop: 0 the reference's (icp), 1 the compensated mode (icp_compensated); before the first frame.
+// ICP loop: 0 the reference's (icp), 1 the compensated mode (icp_compensated), 2 the joint
+// order (float engines, colour frames); before the first frame.
 OR_EXPORT void or_engine_set_icp_mode(void* hp, int mode) {
     with_engine((OrEngine*)hp, [&](auto& e) {
         e.c.icp_mode = mode;
+        return 0;
+    });
+}
+
+// The joint order's photometric weight, intensity difference and minimum gradient.
+OR_EXPORT void or_engine_set_photometric(void* hp, float weight, float max_difference, float min_gradient) {
+    with_engine((OrEngine*)hp, [&](auto& e) {
+        e.c.photo[0] = weight;
+        e.c.photo[1] = max_difference;
+        e.c.photo[2] = min_gradient;
         return 0;
     });
 }
@@ -514,6 +570,45 @@ OR_EXPORT int or_engine_volume_raw(void* hp, int16_t* out) {
     auto& e = ((EngineBox<float, VolS16>*)h->p)->e;
     std::memcpy(out, e.vol.v.data(), e.vol.v.size() * sizeof(int16_t));
     return 0;
+}
+
+// The joint order's helpers on their own, in double (for the closed-form checks).
+OR_EXPORT void or_intensity_half_sample(const float* src, float* dst, int Wd, int Hd) {
+    intensity_half_sample<float>(src, dst, Wd, Hd);
+}
+// at8: pv (3), pp (3), px, py; Rinv9: the previous camera's inverse rotation; cam6: fx, fy,
+// cx, cy (+ unused); iref W*H, gref 2*W*H; returns 1 and c7 when the pixel gives a row.
+OR_EXPORT int or_photometric_row_d(double ic, const double* at8, const double* Rinv9, const float* cam6, int W, int H,
+                                   const double* iref, const double* gref, double psw, double pdiff, double pgrad2,
+                                   double* c7) {
+    Cam<double> cam = make_cam<double>(cam6, W, H);
+    Rigid<double> prev{};
+    for (int k = 0; k < 9; ++k) prev.R_inv[k] = Rinv9[k];
+    AssocAt<double> at{mk(at8[0], at8[1], at8[2]), mk(at8[3], at8[4], at8[5]), at8[6], at8[7]};
+    PhotoIn<double> ph{nullptr, iref, gref, psw, pdiff, pgrad2};
+    return photometric_row<double>(ic, at, prev, cam, ph, c7) ? 1 : 0;
+}
+
+// The feedback colours (W*H BGRA) after the last frame; 0 when there are none.
+OR_EXPORT int or_engine_feedback_colors(void* hp, uint8_t* out) {
+    return with_engine((OrEngine*)hp, [&](auto& e) {
+        if (e.cfb.empty()) return 0;
+        std::memcpy(out, e.cfb.data(), e.cfb.size());
+        return 1;
+    });
+}
+
+// The joint order's intensity pyramid level and the feedback intensities / gradients of
+// the last tracked frame (float engines); returns the element count copied.
+OR_EXPORT long long or_engine_photometric(void* hp, int which, float* out) {
+    return with_engine((OrEngine*)hp, [&](auto& e) -> long long {
+        const std::vector<float>* v = which < 0    ? &e.iref
+                                      : which == 4 ? &e.gref
+                                      : which < (int)e.ipyr.size() ? &e.ipyr[which] : nullptr;
+        if (!v) return 0;
+        if (out) std::memcpy(out, v->data(), v->size() * sizeof(float));
+        return (long long)v->size();
+    });
 }
 
 // Reference maps (world vmap/nmap, float) after the last frame.

@@ -66,6 +66,12 @@ def lib():
             "or_points_dense_slice": (cll, [vp, vp, ci, ci, ci, ci, f32p, i32p, i32p, i32p, vp, vp, cll]),
             "or_engine_set_shift": (None, [vp, ci, f32p]),
             "or_engine_set_icp_mode": (None, [vp, ci]),
+            "or_engine_feedback_colors": (ci, [vp, vp]),
+            "or_engine_set_photometric": (None, [vp, cf, cf, cf]),
+            "or_engine_photometric": (cll, [vp, ci, vp]),
+            "or_intensity_half_sample": (None, [f32p, f32p, ci, ci]),
+            "or_photometric_row_d": (ci, [C.c_double, f64p, f64p, f32p, ci, ci, f64p, f64p, C.c_double, C.c_double,
+                                          C.c_double, f64p]),
             "or_engine_volume_words": (cll, [vp, vp]),
             "or_engine_refmaps_d": (None, [vp, f64p, f64p]),
             "or_f16_from_float": (C.c_uint16, [cf]),
@@ -139,7 +145,8 @@ class OracleEngine:
     reference's loop.
     """
 
-    def __init__(self, wl, prec=0, gate=0.0, color=False, iterations=None, compensated=False, storage=0):
+    def __init__(self, wl, prec=0, gate=0.0, color=False, iterations=None, compensated=False, storage=0,
+                 joint=False):
         its = np.array(list(iterations or wl.iterations) + [0] * 8, np.int32)[:8]
         self.wl = wl
         self.prec = prec
@@ -148,7 +155,10 @@ class OracleEngine:
         self.h = lib().or_engine_create(code, wl.width, wl.height, wl.cam6(), 3, 0.08, 50.0, 0.3, 0.1,
                                         len(iterations or wl.iterations), its, wl.volume_res, wl.volume_dim,
                                         wl.trunc, gate, 1 if color else 0)
-        if compensated:
+        if joint:  # the joint order: photometric rows (weight 0.1) beside the compensated sums
+            assert color and prec == 0, "the joint order runs on a float engine with colours"
+            lib().or_engine_set_icp_mode(self.h, 2)
+        elif compensated:
             lib().or_engine_set_icp_mode(self.h, 1)
 
     def run(self, depth, bgra=None):
@@ -175,6 +185,23 @@ class OracleEngine:
         out = np.empty(max(n, 0), np.uint8)
         if n > 0:
             lib().or_engine_volume_words(self.h, out.ctypes.data)
+        return out
+
+    def set_photometric(self, weight, max_difference=0.2, min_gradient=0.002):
+        lib().or_engine_set_photometric(self.h, weight, max_difference, min_gradient)
+
+    def feedback_colors(self):
+        """The feedback colours (H*W x 4 BGRA) after the last frame, or None."""
+        out = np.empty((self.wl.width * self.wl.height, 4), np.uint8)
+        return out if lib().or_engine_feedback_colors(self.h, out.ctypes.data) else None
+
+    def photometric(self, which):
+        """The joint order's inputs of the last tracked frame: which = level l (0..3) of
+        the intensity pyramid, -1 the feedback intensities, 4 their gradients."""
+        n = lib().or_engine_photometric(self.h, which, None)
+        out = np.empty(n, np.float32)
+        if n:
+            lib().or_engine_photometric(self.h, which, out.ctypes.data)
         return out
 
     def refmaps_d(self):
@@ -508,6 +535,24 @@ def color_icp(wl, vmap, colors, frame_bgra, R, t, iterations=None, rot_only=None
 
 
 # ---- joint depth + intensity ICP terms ------------------------------------------
+
+def intensity_half_sample(src, wd, hd):
+    """Intensities one level down (mean of the valid ones of each 2 x 2 block)."""
+    out = np.empty(wd * hd, np.float32)
+    lib().or_intensity_half_sample(_f32(src), out, wd, hd)
+    return out
+
+
+def photometric_row_d(ic, pv, pp, px, py, Rinv, cam6, w, h, iref, gref, weight, pdiff, min_grad):
+    """The joint order's photometric row of one pixel in double: (ok, c[7])."""
+    at = np.array(list(pv) + list(pp) + [px, py], np.float64)
+    out = np.zeros(7, np.float64)
+    ok = lib().or_photometric_row_d(float(ic), at, np.ascontiguousarray(Rinv, np.float64).ravel(),
+                                    np.ascontiguousarray(cam6, np.float32), w, h,
+                                    np.ascontiguousarray(iref, np.float64).ravel(),
+                                    np.ascontiguousarray(gref, np.float64).ravel(), float(np.sqrt(weight)),
+                                    float(pdiff), float(min_grad) ** 2, out)
+    return ok == 1, out
 
 def bgra_to_intensity(bgra):
     b = _u8(bgra)

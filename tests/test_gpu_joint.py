@@ -1,0 +1,128 @@
+"""The joint ICP order (geometric plus photometric rows, weight 0.1; no counterpart in
+the reference, whose photometric terms K/icp.cl:352-480 have no host loop) on the GPU
+against the oracle's float restatement, bit for bit:
+
+  * the intensity half-sample kernel (nuif_intensity_half_sample) on a ragged image
+    with holes;
+  * a session alone (each frame's pipeline captured as a graph): every frame's pose,
+    the volume's storage words, the raycast maps and the feedback colours over 12
+    frames of the 320x240 orbit with colour, FP16 and FP32 volumes;
+  * the 16-session mix (6 x 320x240, 6 x 640x480, 4 x 1280x720, FP16 256^3 with
+    colour) in batched server rounds, where all sessions' joint iterations run in
+    lockstep launches: each session's poses, maps and feedback colours equal the same
+    session run alone."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from natureuserinterfacestudio_amd import nuif
+from natureuserinterfacestudio_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def same_bits_or_nan(a, b, msg=""):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=msg)
+    m = ~np.isnan(a)
+    np.testing.assert_array_equal(a[m].view(np.uint32), b[m].view(np.uint32), err_msg=msg)
+
+
+def test_intensity_half_sample_matches_oracle(gpu):
+    announce("intensity half-sample 161x97 with holes vs oracle")
+    rng = np.random.default_rng(5)
+    wd, hd = 161, 97
+    src = rng.uniform(0.0, 1.0, (2 * hd, 2 * wd)).astype(np.float32)
+    src[rng.uniform(size=src.shape) < 0.35] = np.nan
+    d_src = nuif.DeviceArray.from_host(src)
+    d_dst = nuif.DeviceArray((hd, wd), np.float32)
+    nuif.check(gpu.L().nuif_intensity_half_sample(d_src.ptr, d_dst.ptr, wd, hd, None), "half sample")
+    nuif.check(gpu.L().nuif_stream_synchronize(None), "sync")
+    out = d_dst.download().ravel()
+    same_bits_or_nan(out, O.intensity_half_sample(src, wd, hd))
+    assert np.isnan(out).sum() > 0
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("storage", [nuif.TSDF_F16, nuif.TSDF_F32])
+def test_joint_engine_matches_oracle(gpu, storage):
+    wl = W.SMALL
+    frames = 12
+    sname = {nuif.TSDF_F16: "f16", nuif.TSDF_F32: "f32"}[storage]
+    announce(f"joint engine {wl.name} {sname} colour, {frames} frames (graph) vs oracle")
+    desc = wl.kinfu_desc(storage=storage, gate=0.0, color=True, use_graph=True, reduction=nuif.ICP_REDUCE_JOINT)
+    eng = nuif.Engine(desc)
+    ref = O.OracleEngine(wl, prec=0, color=True, joint=True, storage={nuif.TSDF_F16: 1, nuif.TSDF_F32: 2}[storage])
+    rendered = [wl.render(f, color=True) for f in range(frames)]
+    for d, bgra, _ in rendered:
+        eng.run(nuif.DeviceArray.from_host(d), nuif.DeviceArray.from_host(bgra), sync=False)
+    assert eng.synchronize() == 1
+    poses = eng.poses(frames)
+    for f, (d, bgra, _) in enumerate(rendered):
+        ok, pr, _ = ref.run(d, bgra)
+        assert ok == 1 and poses[f, 12] == 1.0, f
+        np.testing.assert_array_equal(poses[f, :12], pr.astype(np.float32), err_msg=f"frame {f}")
+    st = eng.state()
+    np.testing.assert_array_equal(np.array(st.Rd), pr[:9])
+    np.testing.assert_array_equal(np.array(st.td), pr[9:])
+    np.testing.assert_array_equal(eng.volume_raw(), ref.volume_words())
+    vg, ng = eng.maps()
+    vr, nr = ref.refmaps()
+    same_bits_or_nan(vg, vr, "vmap")
+    same_bits_or_nan(ng, nr, "nmap")
+    cf = ref.feedback_colors()
+    assert cf is not None and (cf[:, 3] > 0).sum() > wl.width * wl.height // 4
+    np.testing.assert_array_equal(eng.feedback_colors(), cf)
+    eng.close()
+
+
+@pytest.mark.timeout(900)
+def test_server_joint_mix_matches_standalone_engines(gpu):
+    mix = W.joint_mix()
+    S, n = len(mix), 4
+    announce(f"server: {S} joint sessions (6x320x240, 6x640x480, 4x1280x720, F16 256^3 colour), {n} rounds "
+             "vs standalone engines")
+    frames = {}
+    for wl, _ in mix:
+        if wl.name not in frames:
+            frames[wl.name] = [tuple(nuif.DeviceArray.from_host(a) for a in wl.render(f, color=True)[:2])
+                               for f in range(n + 6)]
+    srv = nuif.Server(sum(nuif.Server.session_bytes(d) for _, d in mix) + (1 << 30))
+    ids = [srv.open(d) for _, d in mix]
+    assert None not in ids
+
+    def frame(k, r):  # session k of a resolution plays frames k, k+1, ... of it
+        wl = mix[k][0]
+        j = k - [m[0].name for m in mix].index(wl.name)
+        return frames[wl.name][r + j]
+
+    for r in range(n):
+        fr = [frame(k, r) for k in range(S)]
+        srv.submit(ids, [f[0] for f in fr], [f[1] for f in fr])
+        srv.wait()
+        assert srv.last_batch == (0 if r == 0 else S), (r, srv.last_batch)
+        assert (srv.last_outcomes() == 1).all(), r
+    for k, sid in enumerate(ids):
+        ref = nuif.Engine(mix[k][1])
+        for r in range(n):
+            d, b = frame(k, r)
+            ref.run(d, b, sync=False)
+        assert ref.synchronize() == 1
+        s = srv.sessions[sid]
+        np.testing.assert_array_equal(s.poses(n).view(np.uint32), ref.poses(n).view(np.uint32), err_msg=str(k))
+        for a, b in zip(s.maps(), ref.maps()):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=str(k))
+        np.testing.assert_array_equal(s.feedback_colors(), ref.feedback_colors(), err_msg=str(k))
+        if k in (0, 7, 15):
+            np.testing.assert_array_equal(s.volume_raw(), ref.volume_raw(), err_msg=str(k))
+        ref.close()
+    srv.close()
+    announce(f"server joint mix: {S} sessions x {n} rounds equal the standalone engines")
