@@ -204,6 +204,85 @@ def server_stages(nuif, srv, ids, wl, frames, first, rounds, storage):
             "rooflines": roofs}
 
 
+def run_c4(nuif, W, warmup, rounds, budget, pool_frames=16):
+    """The joint-ICP mix: 16 colour sessions (6 x 320x240 256^3, 6 x 640x480 and
+    4 x 1280x720 512^3, FP16 volumes), closed-loop rounds of one frame per session with
+    its BGRA colours.  Measured three ways: the joint order batched (all sessions' joint
+    iterations in lockstep launches of one round), the same sessions in the geometric
+    (compensated) order batched, and the joint sessions unbatched (each session's
+    graph-captured frame pipeline on its own stream).  Frames come from per-resolution
+    pools resident in HBM; session k of a resolution plays frames k, k+1, ..."""
+    mix = W.joint_mix()
+    pools = {}
+    for wl, _ in mix:
+        if wl.name not in pools:
+            pools[wl.name] = [tuple(nuif.DeviceArray.from_host(a) for a in wl.render(f, color=True)[:2])
+                              for f in range(pool_frames)]
+    first = {}
+    for k, (wl, _) in enumerate(mix):
+        first.setdefault(wl.name, k)
+
+    def frame(k, r):
+        wl = mix[k][0]
+        period = 2 * pool_frames - 2
+        q = (r + k - first[wl.name]) % period
+        return pools[wl.name][q if q < pool_frames else period - q]
+
+    def leg(reduction, batching, stages=False):
+        srv = nuif.Server(budget)
+        srv.set_batching(batching)
+        ids = []
+        for wl, _ in mix:
+            ids.append(srv.open(wl.kinfu_desc(storage=nuif.TSDF_F16, gate=0.0, color=True, use_graph=True,
+                                              reduction=reduction)))
+        assert None not in ids
+
+        def submit(r):
+            fr = [frame(k, r) for k in range(len(ids))]
+            srv.submit(ids, [f[0] for f in fr], [f[1] for f in fr])
+
+        for r in range(warmup):
+            submit(r)
+            srv.wait()
+        nuif.check(nuif.L().nuif_device_synchronize(), "sync")
+        lat = []
+        t0 = time.perf_counter()
+        for r in range(warmup, warmup + rounds):
+            submit(r)
+            lat.append(srv.wait())
+        nuif.check(nuif.L().nuif_device_synchronize(), "sync")
+        t_all = time.perf_counter() - t0
+        ok = all((srv.sessions[i].synchronize() == 1) for i in ids)
+        lat = np.concatenate(lat)
+        # ICP iterations the last frame ran per level (mean over the sessions)
+        its = np.mean([list(srv.sessions[i].state().level_iterations)[:3] for i in ids], axis=0)
+        out = {"fps": round(len(ids) * rounds / t_all, 1), "round_ms": round(t_all / rounds * 1e3, 3),
+               "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "tracking_ok": bool(ok), "icp_iterations_per_level": [round(float(v), 2) for v in its]}
+        if stages:
+            srv.set_stage_timing(True)
+            srv.take_stage_times()
+            n = min(rounds, 10)
+            for r in range(warmup + rounds, warmup + rounds + n):
+                submit(r)
+                srv.wait()
+            m, ms = srv.take_stage_times()
+            srv.set_stage_timing(False)
+            names = ["preprocess", "bounds", "icp", "integrate", "raycast"]
+            out["stage_ms"] = {k: round(v / max(m, 1), 4) for k, v in zip(names, ms)}
+        srv.close()
+        return out
+
+    joint = leg(nuif.ICP_REDUCE_JOINT, True, stages=True)
+    geometric = leg(nuif.ICP_REDUCE_COMPENSATED, True, stages=True)
+    unbatched = leg(nuif.ICP_REDUCE_JOINT, False)
+    return {"sessions": len(mix), "mix": "6 x 320x240 (256^3), 6 x 640x480 (512^3), 4 x 1280x720 (512^3); "
+            "FP16 volumes with colour over 3 m; colour weight 0.1", "rounds": rounds,
+            "joint_batched": joint, "geometric_batched": geometric, "joint_unbatched": unbatched,
+            "joint_over_unbatched": round(joint["fps"] / unbatched["fps"], 3),
+            "photometric_icp_cost": round(joint["stage_ms"]["icp"] / geometric["stage_ms"]["icp"], 3)}
+
+
 C5_SEED = 0x4E554905  # "NUI\0" + 5
 
 
@@ -306,6 +385,7 @@ def main():
     ap.add_argument("--server-storage", choices=sorted(STORAGES), default="f16")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="server device-memory budget")
     ap.add_argument("--cpu-frames", type=int, default=16)
+    ap.add_argument("--c4-rounds", type=int, default=30, help="joint-ICP mix rounds (0: skip that leg)")
     ap.add_argument("--c5-sessions", type=int, default=64, help="arrival-trace sessions (0: skip that leg)")
     ap.add_argument("--c5-rate", type=float, default=2.0, help="arrivals per second")
     ap.add_argument("--hash-steps", type=int, default=20, help="frames of the 640x480 hash session (0: skip)")
@@ -385,6 +465,11 @@ def main():
         print(f"[bench] hash session: {W.HASH.name}", file=sys.stderr, flush=True)
         hashed = run_hash(nuif, W.HASH, min(args.warmup, 5), args.hash_steps)
 
+    c4 = None
+    if args.c4_rounds > 0:
+        print(f"[bench] joint-ICP mix: 16 sessions, {args.c4_rounds} rounds", file=sys.stderr, flush=True)
+        c4 = run_c4(nuif, W, 5, args.c4_rounds, args.budget_gb * 2 ** 30)
+        print(f"[bench] joint-ICP mix done: {c4['joint_batched']['fps']} fps batched", file=sys.stderr, flush=True)
     c5 = None
     if args.c5_sessions > 0:
         print(f"[bench] arrival trace: {args.c5_sessions} sessions at {args.c5_rate}/s", file=sys.stderr, flush=True)
@@ -433,6 +518,7 @@ def main():
         "fp32_serial_compensated_fps": round(fp32c["fps"], 3) if fp32c else None,
         "roofline": roof,
         "hash_session": hashed,
+        "joint_icp_mix": c4,
         "arrival_trace": c5,
         "cpu_baseline": cpu,
         "device": info["name"],

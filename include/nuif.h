@@ -203,12 +203,19 @@ typedef struct nuif_icp_config {
  * [n x p, n, n . (q - p)]. */
 typedef struct nuif_icp_photometric {
     const float* intensity_pyr[NUIF_ICP_MAX_LEVELS]; /* current intensities, W>>l x H>>l per level */
-    const float* ref_intensity;                      /* the model's intensities, W x H (previous pose) */
-    const float* ref_gradient;                       /* W x H float2: nuif_intensity_derivatives of it */
+    const float* ref_model;                          /* W x H float4: the model's intensity (previous
+                                                        pose), dI/du, dI/dv, 0 */
     float weight;                                    /* photometric weight (0.1) */
     float max_difference;
     float min_gradient;
 } nuif_icp_photometric;
+/* The joint order's inputs of a frame in one launch (no counterpart in the reference):
+ * intensity_pyr[0] = nuif_bgra_to_intensity of the frame's colours, level l the
+ * nuif_intensity_half_sample of level l - 1; ref_model = the intensities of ref_bgra (the
+ * previous raycast's colours) with their nuif_intensity_derivatives.  width and height
+ * must be multiples of 2^(levels - 1). */
+int nuif_icp_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, int width, int height, int levels,
+                                const nuif_icp_photometric* photometric, nuif_stream_t s);
 
 /* State block: the tracking state plus the current iteration ("epoch") of the frame
  * (device memory, initialised by nuif_icp_state_init). */

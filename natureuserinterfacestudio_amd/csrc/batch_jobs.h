@@ -136,9 +136,12 @@ int launch_raycast_batch(const RayJob* jobs, const RayJob* hjobs, int n, int sto
 size_t icp_job_bytes();
 int make_icp_jobs(const nuif_icp_session* sessions, int n, void* host_out, int* counts);
 int launch_icp_jobs(const void* dtable, const void* hjobs, int n, const int* counts, hipStream_t st);
-// the joint ICP order's inputs of a frame (color.hip)
-int launch_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W,
-                              int H, int levels, hipStream_t st);
+// the joint ICP order's inputs (color.hip): job k of a host table for a frame (its blocks
+// numbered from *blocks, which advances past them), and one launch for n jobs
+size_t photo_job_bytes();
+int make_photo_job_at(void* hjobs, int k, const uint8_t* bgra, const uint8_t* ref_bgra,
+                      const nuif_icp_photometric& ph, int W, int H, int levels, unsigned* blocks);
+int launch_photo_jobs(const void* djobs, int n, unsigned blocks, hipStream_t st);
 int make_gauss(const float* gaussian, int radius, Gauss* g);
 
 }  // namespace nuif

@@ -65,12 +65,6 @@ __device__ __forceinline__ int cvt_i32_sat(float v) {
     return (int)v;
 }
 
-// The joint ICP order's inputs of a frame (nuif_icp_photometric): the current intensities
-// at every level from the frame's BGRA, the model's intensities from the previous
-// raycast's colours and their Sobel gradients.
-int launch_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W,
-                              int H, int levels, hipStream_t st);
-
 // ---- pyramid and gradients ------------------------------------------------------
 
 // K/transformMaps.cl:405-446 half_sample_bgra_kernel
@@ -451,26 +445,27 @@ __global__ void color_icp_end_kernel(const ColorIcpState* st, NuiCLRigidTransfor
 
 // ---- joint depth + intensity ICP (K/icp.cl:328-480; no host in the reference) ------
 
-// K/transformMaps.cl:372-384 bgra_to_intensity_kernel
+// K/transformMaps.cl:372-384 bgra_to_intensity_kernel, one pixel
+__device__ __forceinline__ float intensity_of(uchar4 c) {
+    return c.w > 0 ? 0.299f * ((float)c.z / 255.0f) + 0.587f * ((float)c.y / 255.0f) + 0.114f * ((float)c.x / 255.0f)
+                   : kNaN;
+}
+
 __global__ void bgra_to_intensity_kernel(const uchar4* __restrict__ bgra, float* __restrict__ out, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uchar4 c = bgra[i];
-    out[i] = c.w > 0 ? 0.299f * ((float)c.z / 255.0f) + 0.587f * ((float)c.y / 255.0f) + 0.114f * ((float)c.x / 255.0f)
-                     : kNaN;
+    out[i] = intensity_of(bgra[i]);
 }
 
-// K/transformMaps.cl:448-511 intensity_derivatives_kernel: neighbours one row (the
-// width) apart, result at the centre; NaN where the reference writes nothing
-__global__ void intensity_derivatives_kernel(const float* __restrict__ I, float2* __restrict__ out, int W, int H) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= W || y >= H) return;
-    const size_t c = (size_t)y * W + x;
+// K/transformMaps.cl:448-511 intensity_derivatives_kernel, one pixel: I(dx, dy) the
+// intensity at the offset from the centre; NaN where the reference writes nothing
+template <class F>
+__device__ __forceinline__ float2 sobel8(F I, bool inside) {
     float2 r = make_float2(kNaN, kNaN);
-    if (!isnan(I[c]) && x > 0 && x < W - 1 && y > 0 && y < H - 1) {
-        const float i00 = I[c - W - 1], i10 = I[c - W], i20 = I[c - W + 1];
-        const float i01 = I[c - 1], i21 = I[c + 1];
-        const float i02 = I[c + W - 1], i12 = I[c + W], i22 = I[c + W + 1];
+    if (!isnan(I(0, 0)) && inside) {
+        const float i00 = I(-1, -1), i10 = I(0, -1), i20 = I(1, -1);
+        const float i01 = I(-1, 0), i21 = I(1, 0);
+        const float i02 = I(-1, 1), i12 = I(0, 1), i22 = I(1, 1);
         if (!(isnan(i00) || isnan(i10) || isnan(i20) || isnan(i01) || isnan(i21) || isnan(i02) || isnan(i12) ||
               isnan(i22))) {
             const float dx = (-1.0f) * i00 + (1.0f) * i20 + (-2.0f) * i01 + (2.0f) * i21 + (-1.0f) * i02 + (1.0f) * i22;
@@ -478,17 +473,20 @@ __global__ void intensity_derivatives_kernel(const float* __restrict__ I, float2
             r = make_float2(dx / 8.0f, dy / 8.0f);
         }
     }
-    out[c] = r;
+    return r;
 }
 
-// Intensities one level down: the mean of the valid ones of the 2 x 2 source pixels,
-// summed in row order (no counterpart in the reference)
-__global__ void intensity_half_sample_kernel(const float* __restrict__ src, float* __restrict__ dst, int Wd, int Hd) {
+// neighbours one row (the width) apart, result at the centre
+__global__ void intensity_derivatives_kernel(const float* __restrict__ I, float2* __restrict__ out, int W, int H) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= Wd || y >= Hd) return;
-    const int Ws = 2 * Wd;
-    const size_t i = (size_t)(2 * y) * Ws + 2 * x;
-    const float v[4] = {src[i], src[i + 1], src[i + Ws], src[i + Ws + 1]};
+    if (x >= W || y >= H) return;
+    const size_t c = (size_t)y * W + x;
+    out[c] = sobel8([&](int dx, int dy) { return I[(long long)c + dy * W + dx]; },
+                    x > 0 && x < W - 1 && y > 0 && y < H - 1);
+}
+
+// The mean of the valid ones of four intensities, summed in order; NaN when none is
+__device__ __forceinline__ float mean_valid4(const float v[4]) {
     float sum = 0.f;
     int cnt = 0;
 #pragma unroll
@@ -497,7 +495,122 @@ __global__ void intensity_half_sample_kernel(const float* __restrict__ src, floa
             sum += v[k];
             ++cnt;
         }
-    dst[(size_t)y * Wd + x] = cnt ? sum / (float)cnt : kNaN;
+    return cnt ? sum / (float)cnt : kNaN;
+}
+
+// Intensities one level down: the mean of the valid ones of the 2 x 2 source pixels in
+// row order (no counterpart in the reference)
+__global__ void intensity_half_sample_kernel(const float* __restrict__ src, float* __restrict__ dst, int Wd, int Hd) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= Wd || y >= Hd) return;
+    const int Ws = 2 * Wd;
+    const size_t i = (size_t)(2 * y) * Ws + 2 * x;
+    const float v[4] = {src[i], src[i + 1], src[i + Ws], src[i + Ws + 1]};
+    dst[(size_t)y * Wd + x] = mean_valid4(v);
+}
+
+// The intensity pyramid's level-L value at (x, y) from the level-0 colours (width W, a
+// multiple of 2^L): the half-sample rule applied level by level.
+template <int L> __device__ float pyramid_value(const uchar4* __restrict__ c, int W, int x, int y) {
+    if constexpr (L == 0) {
+        return intensity_of(c[(size_t)y * W + x]);
+    } else {
+        const float v[4] = {pyramid_value<L - 1>(c, W, 2 * x, 2 * y), pyramid_value<L - 1>(c, W, 2 * x + 1, 2 * y),
+                            pyramid_value<L - 1>(c, W, 2 * x, 2 * y + 1),
+                            pyramid_value<L - 1>(c, W, 2 * x + 1, 2 * y + 1)};
+        return mean_valid4(v);
+    }
+}
+
+// One frame's joint-order inputs (nuif_icp_photometric) in 16 x 16 tiles of level-0
+// pixels numbered from block0.
+struct PhotoJob {
+    const uchar4* bgra;
+    const uchar4* ref;
+    float* pyr[NUIF_ICP_MAX_LEVELS];
+    float4* model;
+    int W, H, levels, gx;
+    unsigned block0;
+};
+
+// All sessions' inputs in one launch (jobs == NULL: the one job `single`).  A thread
+// writes its pixel's level-0 intensity and model texel (the reference intensity and its
+// Sobel gradient from the 3 x 3 reference colours); the thread at (2^l x, 2^l y) also
+// writes level l's pixel (x, y), recomputed from the level-0 colours.
+__global__ void __launch_bounds__(256)
+    photometric_inputs_kernel(const PhotoJob* __restrict__ jobs, int n, PhotoJob single) {
+    int j = 0;
+    if (jobs) {  // the last job whose first block is at or below this block
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].block0 <= blockIdx.x) lo = mid;
+            else hi = mid - 1;
+        }
+        j = lo;
+    }
+    const PhotoJob& J = jobs ? jobs[j] : single;
+    const unsigned b = blockIdx.x - J.block0;
+    const int x = (int)(b % (unsigned)J.gx) * 16 + (int)(threadIdx.x & 15);
+    const int y = (int)(b / (unsigned)J.gx) * 16 + (int)(threadIdx.x >> 4);
+    const int W = J.W, H = J.H;
+    if (x >= W || y >= H) return;
+    const size_t i = (size_t)y * W + x;
+    J.pyr[0][i] = intensity_of(J.bgra[i]);
+    const uchar4* __restrict__ ref = J.ref;
+    const float ir = intensity_of(ref[i]);
+    const float2 g = sobel8([&](int dx, int dy) { return dx == 0 && dy == 0 ? ir : intensity_of(ref[(long long)i + dy * W + dx]); },
+                            x > 0 && x < W - 1 && y > 0 && y < H - 1);
+    J.model[i] = make_float4(ir, g.x, g.y, 0.f);
+    if (J.levels > 1 && !(x & 1) && !(y & 1))
+        J.pyr[1][(size_t)(y >> 1) * (W >> 1) + (x >> 1)] = pyramid_value<1>(J.bgra, W, x >> 1, y >> 1);
+    if (J.levels > 2 && !(x & 3) && !(y & 3))
+        J.pyr[2][(size_t)(y >> 2) * (W >> 2) + (x >> 2)] = pyramid_value<2>(J.bgra, W, x >> 2, y >> 2);
+    if (J.levels > 3 && !(x & 7) && !(y & 7))
+        J.pyr[3][(size_t)(y >> 3) * (W >> 3) + (x >> 3)] = pyramid_value<3>(J.bgra, W, x >> 3, y >> 3);
+}
+
+int make_photo_job(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W, int H,
+                   int levels, unsigned block0, PhotoJob* out) {
+    NUIF_CHECK_ARG(bgra && ref_bgra && ph.ref_model && W > 0 && H > 0 && levels >= 1 && levels <= NUIF_ICP_MAX_LEVELS,
+                   "joint ICP inputs: bad arguments");
+    NUIF_CHECK_ARG(W % (1 << (levels - 1)) == 0 && H % (1 << (levels - 1)) == 0,
+                   "joint ICP inputs: %d x %d is not divisible by 2^(levels - 1)", W, H);
+    PhotoJob J{};
+    J.bgra = reinterpret_cast<const uchar4*>(bgra);
+    J.ref = reinterpret_cast<const uchar4*>(ref_bgra);
+    for (int l = 0; l < levels; ++l) {
+        NUIF_CHECK_ARG(ph.intensity_pyr[l], "joint ICP inputs: intensity level %d is NULL", l);
+        J.pyr[l] = const_cast<float*>(ph.intensity_pyr[l]);
+    }
+    J.model = reinterpret_cast<float4*>(const_cast<float*>(ph.ref_model));
+    J.W = W;
+    J.H = H;
+    J.levels = levels;
+    J.gx = div_up(W, 16);
+    J.block0 = block0;
+    *out = J;
+    return NUIF_OK;
+}
+
+unsigned photo_job_blocks(const PhotoJob& J) { return (unsigned)J.gx * (unsigned)div_up(J.H, 16); }
+
+size_t photo_job_bytes() { return sizeof(PhotoJob); }
+
+int make_photo_job_at(void* hjobs, int k, const uint8_t* bgra, const uint8_t* ref_bgra,
+                      const nuif_icp_photometric& ph, int W, int H, int levels, unsigned* blocks) {
+    PhotoJob* J = static_cast<PhotoJob*>(hjobs) + k;
+    const int rc = make_photo_job(bgra, ref_bgra, ph, W, H, levels, *blocks, J);
+    if (rc) return rc;
+    *blocks += photo_job_blocks(*J);
+    return NUIF_OK;
+}
+
+int launch_photo_jobs(const void* djobs, int n, unsigned blocks, hipStream_t st) {
+    if (n <= 0) return NUIF_OK;
+    photometric_inputs_kernel<<<blocks, 256, 0, st>>>(static_cast<const PhotoJob*>(djobs), n, PhotoJob{});
+    NUIF_LAUNCHED("photometric_inputs_kernel");
+    return NUIF_OK;
 }
 
 // K/icp.cl:328-350 evalR_dAlpha / evalR_dBeta / evalR_dGamma as written
@@ -653,27 +766,6 @@ __global__ void __launch_bounds__(128 * kIntGroups)
 __global__ void intensity_sync_clear_kernel(ColorIcpSync* sy) {
     for (int i = threadIdx.x; i < kCMaxGroups2; i += blockDim.x) sy->arrive[i] = 0u;
     if (threadIdx.x == 0) sy->arrive_all = 0u;
-}
-
-int launch_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W,
-                              int H, int levels, hipStream_t st) {
-    NUIF_CHECK_ARG(bgra && ref_bgra && ph.ref_intensity && ph.ref_gradient, "joint ICP: missing colour inputs");
-    const size_t n = (size_t)W * H;
-    float* i0 = const_cast<float*>(ph.intensity_pyr[0]);
-    bgra_to_intensity_kernel<<<div_up(n, 256), 256, 0, st>>>(reinterpret_cast<const uchar4*>(bgra), i0, n);
-    for (int l = 1; l < levels; ++l) {
-        const int Wd = W >> l, Hd = H >> l;
-        const dim3 block(16, 16), grid(div_up(Wd, 16), div_up(Hd, 16));
-        intensity_half_sample_kernel<<<grid, block, 0, st>>>(ph.intensity_pyr[l - 1],
-                                                             const_cast<float*>(ph.intensity_pyr[l]), Wd, Hd);
-    }
-    float* ir = const_cast<float*>(ph.ref_intensity);
-    bgra_to_intensity_kernel<<<div_up(n, 256), 256, 0, st>>>(reinterpret_cast<const uchar4*>(ref_bgra), ir, n);
-    const dim3 block(16, 16), grid(div_up(W, 16), div_up(H, 16));
-    intensity_derivatives_kernel<<<grid, block, 0, st>>>(ir, reinterpret_cast<float2*>(const_cast<float*>(ph.ref_gradient)),
-                                                         W, H);
-    NUIF_LAUNCHED("joint ICP inputs");
-    return NUIF_OK;
 }
 
 }  // namespace nuif
@@ -840,6 +932,18 @@ int nuif_intensity_half_sample(const float* src, float* dst, int dst_width, int 
     const dim3 block(16, 16), grid(div_up(dst_width, 16), div_up(dst_height, 16));
     intensity_half_sample_kernel<<<grid, block, 0, stream_of(s)>>>(src, dst, dst_width, dst_height);
     NUIF_LAUNCHED("intensity_half_sample_kernel");
+    return NUIF_OK;
+}
+
+int nuif_icp_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, int width, int height, int levels,
+                                const nuif_icp_photometric* photometric, nuif_stream_t s) {
+    NUIF_REQUIRE_INIT();
+    NUIF_CHECK_ARG(photometric, "nuif_icp_photometric_inputs: NULL inputs");
+    PhotoJob J;
+    const int rc = make_photo_job(bgra, ref_bgra, *photometric, width, height, levels, 0, &J);
+    if (rc) return rc;
+    photometric_inputs_kernel<<<photo_job_blocks(J), 256, 0, stream_of(s)>>>(nullptr, 1, J);
+    NUIF_LAUNCHED("photometric_inputs_kernel");
     return NUIF_OK;
 }
 

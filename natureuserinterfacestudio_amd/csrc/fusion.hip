@@ -28,7 +28,7 @@ size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 // byte offsets of the table's parts for n sessions
 struct Layout {
-    size_t pre, half[kLevels], icp, bounds, integ, ray, total;
+    size_t pre, half[kLevels], icp, photo, bounds, integ, ray, total;
     explicit Layout(int n) {
         size_t o = 0;
         pre = o;
@@ -39,6 +39,8 @@ struct Layout {
         }
         icp = o;
         o = align256(o + (size_t)n * icp_job_bytes());
+        photo = o;
+        o = align256(o + (size_t)n * photo_job_bytes());
         bounds = o;
         o = align256(o + (size_t)n * sizeof(BoundsJob));
         integ = o;
@@ -69,7 +71,7 @@ int check_session(const nuif_round_session& r, int i) {
     NUIF_CHECK_ARG(Nz <= kMinSegVoxels || r.walk_states, "nuif_fusion_round: session %d needs walk states", i);
     if (r.icp.cfg.reduction == NUIF_ICP_REDUCE_JOINT) {
         const nuif_icp_photometric& ph = r.icp.photometric;
-        NUIF_CHECK_ARG(r.bgra && r.cmap && ph.ref_intensity && ph.ref_gradient,
+        NUIF_CHECK_ARG(r.bgra && r.cmap && ph.ref_model,
                        "nuif_fusion_round: session %d: the joint ICP needs the frame's colours, the feedback "
                        "colours and the intensity buffers", i);
         for (int l = 0; l < L; ++l)
@@ -169,6 +171,17 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
     int counts[2] = {0, 0};
     int rc = make_icp_jobs(icp.data(), n, h + lay.icp, counts);
     if (rc) return rc;
+    // the joint order's photometric inputs
+    int nphoto = 0;
+    unsigned pblocks = 0;
+    for (int i = 0; i < n; ++i) {
+        const nuif_round_session& r = S[i];
+        if (r.icp.cfg.reduction != NUIF_ICP_REDUCE_JOINT) continue;
+        rc = make_photo_job_at(h + lay.photo, nphoto++, r.bgra, r.cmap, r.icp.photometric,
+                               (int)r.icp.cam.depthImageWidth, (int)r.icp.cam.depthImageHeight, r.icp.cfg.levels,
+                               &pblocks);
+        if (rc) return rc;
+    }
     // integration and raycast, grouped by storage (and colour for the integration)
     const int seg = segment_voxels(n);  // integration z segment for a launch of n sessions
     std::vector<int> order((size_t)n);
@@ -256,13 +269,8 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
     rc = launch_bounds_batch(reinterpret_cast<const BoundsJob*>(d + lay.bounds), bj, n, st);
     if (rc) return rc;
     if ((rc = mark(2))) return rc;
-    for (int i = 0; i < n; ++i) {  // the joint order's photometric inputs
-        const nuif_round_session& r = S[i];
-        if (r.icp.cfg.reduction != NUIF_ICP_REDUCE_JOINT) continue;
-        rc = launch_photometric_inputs(r.bgra, r.cmap, r.icp.photometric, (int)r.icp.cam.depthImageWidth,
-                                       (int)r.icp.cam.depthImageHeight, r.icp.cfg.levels, st);
-        if (rc) return rc;
-    }
+    rc = launch_photo_jobs(d + lay.photo, nphoto, pblocks, st);
+    if (rc) return rc;
     rc = launch_icp_jobs(d + lay.icp, h + lay.icp, n, counts, st);
     if (rc) return rc;
     if ((rc = mark(3))) return rc;

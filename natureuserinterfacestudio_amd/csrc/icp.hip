@@ -120,8 +120,7 @@ struct IcpJob {
     int32_t* outcome;           // NULL, or receives 1 (tracked) / 0 (failed) at the frame's end
     // joint order: the photometric inputs (nuif_icp_photometric)
     const float* icur[NUIF_ICP_MAX_LEVELS];
-    const float* iref;
-    const float2* gref;
+    const float4* model;        // the model's intensity and gradient per pixel
     float psw, pdiff, pgrad2;   // sqrt(weight), max |I - I_ref|, min |g|^2
 };
 
@@ -175,13 +174,12 @@ struct AssocAt {
 // c = (n x p, n, n . (q - p)); false when rejected.
 __device__ __forceinline__ bool icp_assoc(const f3 vc, const TilePose& P, const NuiCLCameraParams& cam,
                                           const float* __restrict__ vref, const float* __restrict__ nref,
-                                          float distThr, float c[7], AssocAt* at = nullptr) {
+                                          float distThr, float c[7]) {
     if (nan3(vc)) return false;
     const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
     const f3 pv = add3(rot3(P.R, vc), mk3(P.t[0], P.t[1], P.t[2]));
     const f3 pp = rot3(P.Ri, sub3(pv, mk3(P.tp[0], P.tp[1], P.tp[2])));
     const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
-    if (at) *at = AssocAt{pv, pp, px, py};
     if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) return false;
     // both lookups are issued before either is tested (pure reads)
     const f3 rn = bilinear_holes(nref, px, py, W, H);
@@ -201,42 +199,12 @@ __device__ __forceinline__ bool icp_assoc(const f3 vc, const TilePose& P, const 
     return true;
 }
 
-// Bilinear sample of a scalar / float2 image with the holes rule of bilinear_holes
-// (an index past the buffer or a NaN corner gives NaN).
-__device__ __forceinline__ float bilinear1(const float* __restrict__ src, float px, float py, int W, int H) {
-    const int x = (int)floorf(px), y = (int)floorf(py);
-    const float dx = px - (float)x, dy = py - (float)y;
-    const long long n = (long long)W * H;
-    const long long ia = x + (long long)y * W, ib = ia + 1, ic = ia + W, id = ic + 1;
-    if (ia < 0 || id >= n) return kNaN;
-    const float a = src[ia], b = src[ib], c = src[ic], d = src[id];
-    if (isnan(a) || isnan(b) || isnan(c) || isnan(d)) return kNaN;
-    return a * (1.0f - dx) * (1.0f - dy) + b * dx * (1.0f - dy) + c * (1.0f - dx) * dy + d * dx * dy;
-}
-__device__ __forceinline__ float2 bilinear2(const float2* __restrict__ src, float px, float py, int W, int H) {
-    const int x = (int)floorf(px), y = (int)floorf(py);
-    const float dx = px - (float)x, dy = py - (float)y;
-    const long long n = (long long)W * H;
-    const long long ia = x + (long long)y * W, ib = ia + 1, ic = ia + W, id = ic + 1;
-    if (ia < 0 || id >= n) return make_float2(kNaN, kNaN);
-    const float2 a = src[ia], b = src[ib], c = src[ic], d = src[id];
-    if (isnan(a.x) || isnan(b.x) || isnan(c.x) || isnan(d.x) || isnan(a.y) || isnan(b.y) || isnan(c.y) || isnan(d.y))
-        return make_float2(kNaN, kNaN);
-    return make_float2(a.x * (1.0f - dx) * (1.0f - dy) + b.x * dx * (1.0f - dy) + c.x * (1.0f - dx) * dy + d.x * dx * dy,
-                       a.y * (1.0f - dx) * (1.0f - dy) + b.y * dx * (1.0f - dy) + c.y * (1.0f - dx) * dy + d.y * dx * dy);
-}
-
-// The joint order's photometric row of an associated pixel (nuif_icp_photometric): false
-// when the pixel gives none.  ic: the current intensity at the pixel.
-__device__ __forceinline__ bool photometric_row(float ic, const AssocAt& at, const TilePose& P,
-                                                const NuiCLCameraParams& cam, const float* __restrict__ iref,
-                                                const float2* __restrict__ gref, float psw, float pdiff, float pgrad2,
+// The joint order's photometric row of an associated pixel (nuif_icp_photometric) from
+// the current intensity ic and the model's intensity ir and gradient g at the
+// projection: false when the pixel gives none.
+__device__ __forceinline__ bool photometric_row(float ic, float ir, float2 g, const AssocAt& at, const TilePose& P,
+                                                const NuiCLCameraParams& cam, float psw, float pdiff, float pgrad2,
                                                 float c[7]) {
-    if (isnan(ic)) return false;
-    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
-    const float ir = bilinear1(iref, at.px, at.py, W, H);
-    const float2 g = bilinear2(gref, at.px, at.py, W, H);
-    if (isnan(ir) || isnan(g.x)) return false;
     const float di = ic - ir;
     if (!(fabsf(di) < pdiff) || !(g.x * g.x + g.y * g.y > pgrad2)) return false;
     // d(intensity)/d(point in the previous camera), then to world coordinates: pp = Ri (pv - tp)
@@ -254,6 +222,57 @@ __device__ __forceinline__ bool photometric_row(float ic, const AssocAt& at, con
     c[5] = psw * jw.z;
     c[6] = psw * di;
     return true;
+}
+
+// One pixel of the joint order: icp_assoc's geometric row (ok) and, for an associated
+// pixel, its photometric row (ph), with every gather -- the reference normals and
+// vertices, the model's texels (intensity, dI/du, dI/dv; bilinear with the holes rule
+// of bilinear_holes) and the current intensity -- issued before any test.
+__device__ __forceinline__ void joint_assoc(const f3 vc, const TilePose& P, const NuiCLCameraParams& cam,
+                                            const float* __restrict__ vref, const float* __restrict__ nref,
+                                            const float4* __restrict__ model, const float* __restrict__ icur, int p,
+                                            float distThr, float psw, float pdiff, float pgrad2, float c[7],
+                                            float pc[7], bool& ok, bool& ph) {
+    ok = ph = false;
+    if (nan3(vc)) return;
+    const int W = (int)cam.depthImageWidth, H = (int)cam.depthImageHeight;
+    const f3 pv = add3(rot3(P.R, vc), mk3(P.t[0], P.t[1], P.t[2]));
+    const f3 pp = rot3(P.Ri, sub3(pv, mk3(P.tp[0], P.tp[1], P.tp[2])));
+    const float px = pp.x * cam.fx / pp.z + cam.cx, py = pp.y * cam.fy / pp.z + cam.cy;
+    if (!(px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H)) return;
+    const int x = (int)floorf(px), y = (int)floorf(py);
+    const float dx = px - (float)x, dy = py - (float)y;
+    const long long n = (long long)W * H;
+    const long long ia = x + (long long)y * W, ib = ia + 1, ic = ia + W, id = ic + 1;
+    if (ia < 0 || id >= n) return;
+    const f3 na = ld3(nref, ia), nb = ld3(nref, ib), nc = ld3(nref, ic), nd = ld3(nref, id);
+    const f3 va = ld3(vref, ia), vb = ld3(vref, ib), vcn = ld3(vref, ic), vd = ld3(vref, id);
+    const float4 ma = model[ia], mb = model[ib], mc = model[ic], md = model[id];
+    const float ival = icur[p];
+    auto mix = [&](float a, float b, float c_, float d) {
+        return a * (1.0f - dx) * (1.0f - dy) + b * dx * (1.0f - dy) + c_ * (1.0f - dx) * dy + d * dx * dy;
+    };
+    if (nan3(na) || nan3(nb) || nan3(nc) || nan3(nd) || nan3(va) || nan3(vb) || nan3(vcn) || nan3(vd)) return;
+    const f3 rn = mk3(mix(na.x, nb.x, nc.x, nd.x), mix(na.y, nb.y, nc.y, nd.y), mix(na.z, nb.z, nc.z, nd.z));
+    const f3 rv = mk3(mix(va.x, vb.x, vcn.x, vd.x), mix(va.y, vb.y, vcn.y, vd.y), mix(va.z, vb.z, vcn.z, vd.z));
+    const f3 diff = sub3(rv, pv);
+    const float dist = sqrtf(dot3(diff, diff));
+    if (!(dist < distThr)) return;
+    const f3 row0 = cross3(rn, pv);
+    c[0] = row0.x;
+    c[1] = row0.y;
+    c[2] = row0.z;
+    c[3] = rn.x;
+    c[4] = rn.y;
+    c[5] = rn.z;
+    c[6] = dot3(rn, diff);
+    ok = true;
+    if (isnan(ival) || isnan(ma.x) || isnan(mb.x) || isnan(mc.x) || isnan(md.x) || isnan(ma.y) || isnan(mb.y) ||
+        isnan(mc.y) || isnan(md.y) || isnan(ma.z) || isnan(mb.z) || isnan(mc.z) || isnan(md.z))
+        return;
+    const float ir = mix(ma.x, mb.x, mc.x, md.x);
+    const float2 g = make_float2(mix(ma.y, mb.y, mc.y, md.y), mix(ma.z, mb.z, mc.z, md.z));
+    ph = photometric_row(ival, ir, g, AssocAt{pv, pp, px, py}, P, cam, psw, pdiff, pgrad2, pc);
 }
 
 // The 29 terms of a pixel (zeros when rejected)
@@ -539,11 +558,8 @@ __device__ __forceinline__ void process_tile(IcpLds<MODE>& L, uint32_t tile) {
             const int p = p0 + i * kBlock;
             const f3 vc = p < npx ? ld3(vcur, p) : mk3(kNaN, kNaN, kNaN);
             if constexpr (MODE == kJoint) {
-                AssocAt at;
-                ok[i] = icp_assoc(vc, L.pose, J.cam, vref, nref, J.dist, cf[i], &at);
-                const float ic = ok[i] ? in_global(J.icur[level])[p] : kNaN;
-                ph[i] = ok[i] && photometric_row(ic, at, L.pose, J.cam, in_global(J.iref), in_global(J.gref), J.psw,
-                                                 J.pdiff, J.pgrad2, pc[i]);
+                joint_assoc(vc, L.pose, J.cam, vref, nref, in_global(J.model), in_global(J.icur[level]), p, J.dist,
+                            J.psw, J.pdiff, J.pgrad2, cf[i], pc[i], ok[i], ph[i]);
             } else {
                 ok[i] = icp_assoc(vc, L.pose, J.cam, vref, nref, J.dist, cf[i]);
             }
@@ -1035,14 +1051,14 @@ int make_job(const nuif_icp_session& in, IcpJob* j) {
     J.mode = cfg.reduction;
     if (J.mode == kJoint) {
         const nuif_icp_photometric& ph = in.photometric;
-        NUIF_CHECK_ARG(ph.ref_intensity && ph.ref_gradient && ph.weight >= 0.f,
-                       "ICP session: the joint order needs the photometric inputs");
+        NUIF_CHECK_ARG(ph.ref_model && ph.weight >= 0.f, "ICP session: the joint order needs the photometric inputs");
+        NUIF_CHECK_ARG(W % (1 << (cfg.levels - 1)) == 0 && H % (1 << (cfg.levels - 1)) == 0,
+                       "ICP session: the joint order needs an image size divisible by 2^(levels - 1)");
         for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) {
             J.icur[l] = l < cfg.levels ? ph.intensity_pyr[l] : nullptr;
             NUIF_CHECK_ARG(J.iters[l] == 0 || J.icur[l], "ICP session: level %d has no intensities", l);
         }
-        J.iref = ph.ref_intensity;
-        J.gref = reinterpret_cast<const float2*>(ph.ref_gradient);
+        J.model = reinterpret_cast<const float4*>(ph.ref_model);
         J.psw = sqrtf(ph.weight);
         J.pdiff = ph.max_difference;
         J.pgrad2 = ph.min_gradient * ph.min_gradient;

@@ -267,15 +267,15 @@ void NuiKinfuHIPDepthTracker::SetReduction(int mode) {
     if (mode != NUIF_ICP_REDUCE_REFERENCE && mode != NUIF_ICP_REDUCE_COMPENSATED && mode != NUIF_ICP_REDUCE_JOINT)
         throw std::runtime_error("unknown ICP reduction");
     m_icfg.reduction = mode;
-    if (mode != NUIF_ICP_REDUCE_JOINT || m_refIntensity.get()) return;
+    if (mode != NUIF_ICP_REDUCE_JOINT || m_refModel.get()) return;
+    const UINT m = 1u << (m_icfg.levels - 1);
+    if (m_W % m || m_H % m) throw std::runtime_error("the joint ICP needs an image size divisible by 2^(levels - 1)");
     for (int l = 0; l < m_icfg.levels; ++l) {
         m_intensity[l] = DeviceBuffer((size_t)(m_W >> l) * (m_H >> l) * sizeof(float));
         m_photo.intensity_pyr[l] = m_intensity[l].as<float>();
     }
-    m_refIntensity = DeviceBuffer((size_t)m_W * m_H * sizeof(float));
-    m_refGradient = DeviceBuffer((size_t)m_W * m_H * 2 * sizeof(float));
-    m_photo.ref_intensity = m_refIntensity.as<float>();
-    m_photo.ref_gradient = m_refGradient.as<float>();
+    m_refModel = DeviceBuffer((size_t)m_W * m_H * 4 * sizeof(float));
+    m_photo.ref_model = m_refModel.as<float>();
     if (m_photo.weight == 0.f) SetPhotometric(0.1f, 0.2f, 0.002f);
 }
 
@@ -292,18 +292,9 @@ void NuiKinfuHIPDepthTracker::SetPhotometric(float weight, float maxDifference, 
 void NuiKinfuHIPDepthTracker::preparePhotometric(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb) {
     if (!f->GetColorBuffer())
         throw std::runtime_error("the joint ICP needs the frame's colours (and a scene with a colour volume)");
-    const size_t n = (size_t)m_W * m_H;
-    check(nuif_bgra_to_intensity(f->GetColorBuffer(), m_intensity[0].as<float>(), n, m_stream),
-          "nuif_bgra_to_intensity");
-    for (int l = 1; l < m_icfg.levels; ++l)
-        check(nuif_intensity_half_sample(m_intensity[l - 1].as<float>(), m_intensity[l].as<float>(), (int)(m_W >> l),
-                                         (int)(m_H >> l), m_stream),
-              "nuif_intensity_half_sample");
-    check(nuif_bgra_to_intensity(fb->GetColorBuffer(), m_refIntensity.as<float>(), n, m_stream),
-          "nuif_bgra_to_intensity");
-    check(nuif_intensity_derivatives(m_refIntensity.as<float>(), m_refGradient.as<float>(), (int)m_W, (int)m_H,
-                                     m_stream),
-          "nuif_intensity_derivatives");
+    check(nuif_icp_photometric_inputs(f->GetColorBuffer(), fb->GetColorBuffer(), (int)m_W, (int)m_H, m_icfg.levels,
+                                      &m_photo, m_stream),
+          "nuif_icp_photometric_inputs");
 }
 
 void NuiKinfuHIPDepthTracker::FillIcpSession(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb,

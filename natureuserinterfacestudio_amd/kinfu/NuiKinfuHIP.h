@@ -170,7 +170,7 @@ private:
     float* m_history = nullptr;
     int m_historyCap = 0;
     // the joint order: intensity pyramid of the frame, the feedback intensities and gradients
-    DeviceBuffer m_intensity[NUIF_ICP_MAX_LEVELS], m_refIntensity, m_refGradient;
+    DeviceBuffer m_intensity[NUIF_ICP_MAX_LEVELS], m_refModel;
     nuif_icp_photometric m_photo{};
     void preparePhotometric(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb);
 };

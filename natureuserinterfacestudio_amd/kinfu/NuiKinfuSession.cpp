@@ -110,7 +110,7 @@ size_t NuiKinfuSession::deviceBytes(const nuifk_desc& d) {
     bytes += nuif_icp_state_bytes() + nuif_icp_partials_bytes(d.width, d.height);
     if (d.icp_reduction == NUIF_ICP_REDUCE_JOINT) {  // intensity pyramid, feedback intensities and gradients
         for (int l = 0; l < d.levels; ++l) bytes += (size_t)(d.width >> l) * (d.height >> l) * 4;
-        bytes += n * 12;
+        bytes += n * 16;  // the model: intensity and gradient per pixel
     }
     bytes += (size_t)(1 << 16) * NUIF_POSE_ROW * sizeof(float) + 256;  // pose history, camera pose
     return bytes;

@@ -68,9 +68,8 @@ ICP_REDUCE_REFERENCE, ICP_REDUCE_COMPENSATED, ICP_REDUCE_JOINT = 0, 1, 2
 
 
 class IcpPhotometric(C.Structure):
-    _fields_ = [("intensity_pyr", c_void_p * ICP_MAX_LEVELS), ("ref_intensity", c_void_p),
-                ("ref_gradient", c_void_p), ("weight", c_float), ("max_difference", c_float),
-                ("min_gradient", c_float)]
+    _fields_ = [("intensity_pyr", c_void_p * ICP_MAX_LEVELS), ("ref_model", c_void_p), ("weight", c_float),
+                ("max_difference", c_float), ("min_gradient", c_float)]
 
 
 class ColorIcpResult(C.Structure):
@@ -232,6 +231,8 @@ _SIGS = {
     "nuif_bgra_to_intensity": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "nuif_intensity_derivatives": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "nuif_intensity_half_sample": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "nuif_icp_photometric_inputs": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, C.POINTER(IcpPhotometric),
+                                            c_void_p]),
     "nuif_icp_estimate_joint": (c_int, [c_void_p, c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p,
                                         c_void_p, c_void_p, C.POINTER(IcpConfig), C.POINTER(IcpPhotometric),
                                         c_void_p, c_int, c_void_p]),

@@ -148,15 +148,16 @@ HASH_HD = Workload("hash-1280x720-5mm", 1280, 720, 1050.0, 1050.0, 639.5, 359.5,
                    volume_res=0, volume_dim=0.0, trunc=0.02, iterations=[10, 5, 4], seed_index=4)
 
 
-def joint_mix(volume_res=256):
+def joint_mix(volume_res=(256, 512, 512)):
     """The joint-ICP session mix: 6 x 320x240, 6 x 640x480 and 4 x 1280x720 sessions with
-    colour, FP16 volumes of volume_res^3 over 3 m, the joint ICP order (geometric plus
-    photometric rows, weight 0.1); each frame's pipeline captured as a graph when the
-    session runs alone.  Returns [(workload, descriptor)] for the 16 sessions."""
+    colour, FP16 volumes over 3 m (volume_res^3 per resolution), the joint ICP order
+    (geometric plus photometric rows, weight 0.1); each frame's pipeline captured as a
+    graph when the session runs alone.  Returns [(workload, descriptor)] for the 16
+    sessions."""
     import dataclasses
     out = []
-    for base, count in ((SMALL, 6), (FULL, 6), (HD, 4)):
-        wl = dataclasses.replace(base, volume_res=volume_res, volume_dim=3.0, trunc=0.025)
+    for (base, count), res in zip(((SMALL, 6), (FULL, 6), (HD, 4)), volume_res):
+        wl = dataclasses.replace(base, volume_res=res, volume_dim=3.0, trunc=0.025)
         d = wl.kinfu_desc(storage=nuif.TSDF_F16, gate=0.0, color=True, use_graph=True,
                           reduction=nuif.ICP_REDUCE_JOINT)
         out += [(wl, d)] * count

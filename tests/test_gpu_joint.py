@@ -3,12 +3,13 @@ the reference, whose photometric terms K/icp.cl:352-480 have no host loop) on th
 against the oracle's float restatement, bit for bit:
 
   * the intensity half-sample kernel (nuif_intensity_half_sample) on a ragged image
-    with holes;
+    with holes, and the joint inputs in one launch (nuif_icp_photometric_inputs: the
+    intensity pyramid, the model's intensities and gradients);
   * a session alone (each frame's pipeline captured as a graph): every frame's pose,
     the volume's storage words, the raycast maps and the feedback colours over 12
     frames of the 320x240 orbit with colour, FP16 and FP32 volumes;
-  * the 16-session mix (6 x 320x240, 6 x 640x480, 4 x 1280x720, FP16 256^3 with
-    colour) in batched server rounds, where all sessions' joint iterations run in
+  * the 16-session mix (6 x 320x240 with 256^3, 6 x 640x480 and 4 x 1280x720 with 512^3
+    FP16 volumes, colour) in batched server rounds, where all sessions' joint iterations run in
     lockstep launches: each session's poses, maps and feedback colours equal the same
     session run alone."""
 import sys
@@ -51,6 +52,42 @@ def test_intensity_half_sample_matches_oracle(gpu):
     assert np.isnan(out).sum() > 0
 
 
+@pytest.mark.parametrize("levels", [3, 4])
+def test_photometric_inputs_match_oracle(gpu, levels):
+    """nuif_icp_photometric_inputs (one launch) against the oracle's intensities,
+    half-samples and derivatives of the same colours, 200 x 152 with alpha-0 holes."""
+    announce(f"photometric inputs 200x152, {levels} levels vs oracle")
+    import ctypes as C
+    rng = np.random.default_rng(9 + levels)
+    w, h = 200, 152
+    bgra = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    ref = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    bgra[..., 3] = np.where(rng.uniform(size=(h, w)) < 0.1, 0, 255)
+    ref[..., 3] = np.where(rng.uniform(size=(h, w)) < 0.05, 0, 255)
+    d_b, d_r = nuif.DeviceArray.from_host(bgra), nuif.DeviceArray.from_host(ref)
+    pyr = [nuif.DeviceArray(((h >> l), (w >> l)), np.float32) for l in range(levels)]
+    model = nuif.DeviceArray((h * w, 4), np.float32)
+    ph = nuif.IcpPhotometric()
+    for l in range(levels):
+        ph.intensity_pyr[l] = pyr[l].ptr
+    ph.ref_model = model.ptr
+    ph.weight, ph.max_difference, ph.min_gradient = 0.1, 0.2, 0.002
+    nuif.check(gpu.L().nuif_icp_photometric_inputs(d_b.ptr, d_r.ptr, w, h, levels, C.byref(ph), None), "inputs")
+    nuif.check(gpu.L().nuif_stream_synchronize(None), "sync")
+    i0 = O.bgra_to_intensity(bgra)
+    exp = [i0]
+    for l in range(1, levels):
+        exp.append(O.intensity_half_sample(exp[-1], w >> l, h >> l))
+    for l in range(levels):
+        same_bits_or_nan(pyr[l].download().ravel(), np.asarray(exp[l]).ravel(), f"level {l}")
+    ir = O.bgra_to_intensity(ref)
+    g = O.intensity_derivatives(ir, w, h).reshape(-1, 2)
+    m = model.download()
+    same_bits_or_nan(m[:, 0], np.asarray(ir).ravel(), "model intensity")
+    same_bits_or_nan(m[:, 1:3], g, "model gradient")
+    assert (m[:, 3] == 0).all() and np.isnan(g[:, 0]).sum() > 0 and (~np.isnan(g[:, 0])).sum() > w * h // 2
+
+
 @pytest.mark.timeout(600)
 @pytest.mark.parametrize("storage", [nuif.TSDF_F16, nuif.TSDF_F32])
 def test_joint_engine_matches_oracle(gpu, storage):
@@ -88,7 +125,7 @@ def test_joint_engine_matches_oracle(gpu, storage):
 def test_server_joint_mix_matches_standalone_engines(gpu):
     mix = W.joint_mix()
     S, n = len(mix), 4
-    announce(f"server: {S} joint sessions (6x320x240, 6x640x480, 4x1280x720, F16 256^3 colour), {n} rounds "
+    announce(f"server: {S} joint sessions (6x320x240, 6x640x480, 4x1280x720, F16 256^3/512^3 colour), {n} rounds "
              "vs standalone engines")
     frames = {}
     for wl, _ in mix:
