@@ -509,19 +509,6 @@ __global__ void intensity_half_sample_kernel(const float* __restrict__ src, floa
     dst[(size_t)y * Wd + x] = mean_valid4(v);
 }
 
-// The intensity pyramid's level-L value at (x, y) from the level-0 colours (width W, a
-// multiple of 2^L): the half-sample rule applied level by level.
-template <int L> __device__ float pyramid_value(const uchar4* __restrict__ c, int W, int x, int y) {
-    if constexpr (L == 0) {
-        return intensity_of(c[(size_t)y * W + x]);
-    } else {
-        const float v[4] = {pyramid_value<L - 1>(c, W, 2 * x, 2 * y), pyramid_value<L - 1>(c, W, 2 * x + 1, 2 * y),
-                            pyramid_value<L - 1>(c, W, 2 * x, 2 * y + 1),
-                            pyramid_value<L - 1>(c, W, 2 * x + 1, 2 * y + 1)};
-        return mean_valid4(v);
-    }
-}
-
 // One frame's joint-order inputs (nuif_icp_photometric) in 16 x 16 tiles of level-0
 // pixels numbered from block0.
 struct PhotoJob {
@@ -533,12 +520,21 @@ struct PhotoJob {
     unsigned block0;
 };
 
-// All sessions' inputs in one launch (jobs == NULL: the one job `single`).  A thread
-// writes its pixel's level-0 intensity and model texel (the reference intensity and its
-// Sobel gradient from the 3 x 3 reference colours); the thread at (2^l x, 2^l y) also
-// writes level l's pixel (x, y), recomputed from the level-0 colours.
+// All sessions' inputs in one launch (jobs == NULL: the one job `single`), one 16 x 16
+// tile of level-0 pixels per block.  The intensities come from per-channel tables of
+// intensity_of's three products (the same roundings, added in the same order); the
+// tile's intensities stay in LDS for the pyramid (a 16-pixel tile holds whole 2^l x 2^l
+// blocks for l <= 3, the image size being a multiple of 2^(levels - 1)), and the
+// reference intensities of the tile and its one-pixel halo for the Sobel gradient.
 __global__ void __launch_bounds__(256)
     photometric_inputs_kernel(const PhotoJob* __restrict__ jobs, int n, PhotoJob single) {
+    __shared__ float tab[3][256];
+    __shared__ float lev[16 * 16 + 8 * 8 + 4 * 4 + 2 * 2];
+    __shared__ float refi[18][19];
+    const int tid = threadIdx.x;
+    tab[0][tid] = 0.299f * ((float)tid / 255.0f);  // red
+    tab[1][tid] = 0.587f * ((float)tid / 255.0f);  // green
+    tab[2][tid] = 0.114f * ((float)tid / 255.0f);  // blue
     int j = 0;
     if (jobs) {  // the last job whose first block is at or below this block
         int lo = 0, hi = n - 1;
@@ -551,23 +547,50 @@ __global__ void __launch_bounds__(256)
     }
     const PhotoJob& J = jobs ? jobs[j] : single;
     const unsigned b = blockIdx.x - J.block0;
-    const int x = (int)(b % (unsigned)J.gx) * 16 + (int)(threadIdx.x & 15);
-    const int y = (int)(b / (unsigned)J.gx) * 16 + (int)(threadIdx.x >> 4);
+    const int x0 = (int)(b % (unsigned)J.gx) * 16, y0 = (int)(b / (unsigned)J.gx) * 16;
     const int W = J.W, H = J.H;
-    if (x >= W || y >= H) return;
-    const size_t i = (size_t)y * W + x;
-    J.pyr[0][i] = intensity_of(J.bgra[i]);
+    __syncthreads();
+    auto inten = [&](uchar4 c) { return c.w > 0 ? (tab[0][c.z] + tab[1][c.y]) + tab[2][c.x] : kNaN; };
+    // the reference intensities of the tile and its halo (18 x 18, NaN outside the image)
     const uchar4* __restrict__ ref = J.ref;
-    const float ir = intensity_of(ref[i]);
-    const float2 g = sobel8([&](int dx, int dy) { return dx == 0 && dy == 0 ? ir : intensity_of(ref[(long long)i + dy * W + dx]); },
-                            x > 0 && x < W - 1 && y > 0 && y < H - 1);
-    J.model[i] = make_float4(ir, g.x, g.y, 0.f);
-    if (J.levels > 1 && !(x & 1) && !(y & 1))
-        J.pyr[1][(size_t)(y >> 1) * (W >> 1) + (x >> 1)] = pyramid_value<1>(J.bgra, W, x >> 1, y >> 1);
-    if (J.levels > 2 && !(x & 3) && !(y & 3))
-        J.pyr[2][(size_t)(y >> 2) * (W >> 2) + (x >> 2)] = pyramid_value<2>(J.bgra, W, x >> 2, y >> 2);
-    if (J.levels > 3 && !(x & 7) && !(y & 7))
-        J.pyr[3][(size_t)(y >> 3) * (W >> 3) + (x >> 3)] = pyramid_value<3>(J.bgra, W, x >> 3, y >> 3);
+    for (int k = tid; k < 18 * 18; k += 256) {
+        const int hx = x0 - 1 + k % 18, hy = y0 - 1 + k / 18;
+        refi[k / 18][k % 18] =
+            hx >= 0 && hx < W && hy >= 0 && hy < H ? inten(ref[(size_t)hy * W + hx]) : kNaN;
+    }
+    const int tx = tid & 15, ty = tid >> 4;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool in = x < W && y < H;
+    const size_t i = (size_t)y * W + x;
+    const float i0 = in ? inten(J.bgra[i]) : kNaN;
+    lev[tid] = i0;
+    __syncthreads();
+    if (in) {
+        J.pyr[0][i] = i0;
+        const float ir = refi[ty + 1][tx + 1];
+        const float2 g = sobel8([&](int dx, int dy) { return refi[ty + 1 + dy][tx + 1 + dx]; },
+                                x > 0 && x < W - 1 && y > 0 && y < H - 1);
+        J.model[i] = make_float4(ir, g.x, g.y, 0.f);
+    }
+    // levels 1..3 from the tile's own intensities, each (16 >> l)^2 after the previous
+    float* src = lev;
+    float* dst = lev + 256;
+    for (int l = 1; l < J.levels; ++l) {
+        const int w = 16 >> l;
+        if (tid < w * w) {
+            const int lx = tid % w, ly = tid / w;
+            const int sw = 2 * w;
+            const float v[4] = {src[(2 * ly) * sw + 2 * lx], src[(2 * ly) * sw + 2 * lx + 1],
+                                src[(2 * ly + 1) * sw + 2 * lx], src[(2 * ly + 1) * sw + 2 * lx + 1]};
+            const float m = mean_valid4(v);
+            dst[tid] = m;
+            const int gx = (x0 >> l) + lx, gy = (y0 >> l) + ly;
+            if (gx < (W >> l) && gy < (H >> l)) J.pyr[l][(size_t)gy * (W >> l) + gx] = m;
+        }
+        __syncthreads();
+        src = dst;
+        dst += w * w;
+    }
 }
 
 int make_photo_job(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W, int H,
