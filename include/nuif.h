@@ -269,7 +269,9 @@ int nuif_icp_estimate(void* state, float* partials, const NuiCLCameraParams* cam
  * process ends; an image whose nuif_icp_partials_bytes exceed it is refused with
  * NUIF_ERR_INVALID (1280 x 720 fits).  Nothing is allocated or freed.  Every call on a
  * GPU shares that GPU's array: calls that overlap on one GPU (different streams) race. */
-/* nuif_icp_estimate in the joint order (cfg->reduction NUIF_ICP_REDUCE_JOINT). */
+/* nuif_icp_estimate in the joint order (cfg->reduction NUIF_ICP_REDUCE_JOINT; photometric
+ * inputs from nuif_icp_photometric_inputs).  No counterpart in the reference: its
+ * photometric terms (K/icp.cl:352-480) have no host loop. */
 int nuif_icp_estimate_joint(void* state, float* partials, const NuiCLCameraParams* cam, const float* const* vmap_pyr,
                             const float* ref_vmap, const float* ref_nmap, NuiCLRigidTransform* pose,
                             const nuif_icp_config* cfg, const nuif_icp_photometric* photometric, float* history,
