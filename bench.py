@@ -36,14 +36,84 @@ def cpu_baseline(wl, frames):
     eng = O.OracleEngine(wl, prec=0, gate=0.0)
     depths = [wl.render(f)[0] for f in range(frames)]
     eng.run(depths[0])  # first frame: integrate only
+    eng.stage_seconds()
     t0 = time.perf_counter()
     for d in depths[1:]:
         eng.run(d)
     dt = time.perf_counter() - t0
+    st = eng.stage_seconds() / (frames - 1) * 1e3
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": round((frames - 1) / dt, 3), "unit": "frames/s", "cores": cores, "kind": "port",
+            "cpu_model": model,
+            "stage_ms": {k: round(float(v), 2) for k, v in zip(["preprocess", "icp", "integrate", "raycast"], st)},
             "sample": f"{frames - 1} fused frames (after frame 0) of the same 640x480 / 512^3 workload, "
                       f"float oracle with the reference's short2 volume, OpenMP"}
+
+
+def fp64_check(nuif, wl, frames, storage):
+    """The server's session type (FP16 or FP32 volume, compensated ICP) on the GPU against
+    the FP64 oracle on the same frames -- the oracle as the checker, on the host: pose
+    drift per frame against the FP64 engine, the TSDF against an FP64 volume integrated at
+    the GPU's poses (voxels observed by both with equal weights, in units of the
+    truncation distance) and the raycast vertices against an FP64 raycast of that volume
+    at the last pose (rays hit by both)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    depths = [wl.render(f)[0] for f in range(frames)]
+    eng = nuif.Engine(wl.kinfu_desc(storage=storage, gate=0.0, reduction=nuif.ICP_REDUCE_COMPENSATED))
+    for d in depths:
+        eng.run(nuif.DeviceArray.from_host(d), sync=False)
+    ok = eng.synchronize() == 1
+    poses = eng.poses(frames)
+    o64 = O.OracleEngine(wl, prec=1, gate=0.0, compensated=True)
+    vd = O.OracleVolumeD(wl)
+    dt_max, ang_max = 0.0, 0.0
+    for f, d in enumerate(depths):
+        _, p64, _ = o64.run(d)
+        R, t = poses[f, :9].reshape(3, 3).astype(np.float64), poses[f, 9:12].astype(np.float64)
+        R64, t64 = p64[:9].reshape(3, 3), p64[9:]
+        dt_max = max(dt_max, float(np.linalg.norm(t - t64)))
+        c = np.clip((np.trace(R @ R64.T) - 1.0) / 2.0, -1.0, 1.0)
+        ang_max = max(ang_max, float(np.degrees(np.arccos(c))))
+        vd.integrate(d, poses[f, :9].reshape(3, 3), poses[f, 9:12])
+    o64.close()
+    raw = eng.volume_raw()
+    if storage == nuif.TSDF_F16:
+        w32 = raw.view(np.uint32)
+        t_g = (w32 & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float64)
+        w_g = (w32 >> 16).astype(np.int64)
+    else:
+        f2 = raw.view(np.float32).reshape(-1, 2)
+        t_g, w_g = f2[:, 0].astype(np.float64), f2[:, 1].astype(np.int64)
+    t_d, w_d = vd.volume()
+    same = (w_g > 0) & (w_d > 0) & (w_g == w_d)
+    terr = np.abs(t_g[same] - t_d[same])
+    vg, _ = eng.maps()
+    vr, _ = vd.raycast(poses[-1, :9].reshape(3, 3), poses[-1, 9:12])
+    vd.close()
+    eng.close()
+    hg, hd = ~np.isnan(vg[:, 2]), ~np.isnan(vr[:, 2])
+    hit = hg & hd
+    verr = np.linalg.norm(vg[hit].astype(np.float64) - vr[hit], axis=1)
+    return {"session": f"{'FP16' if storage == nuif.TSDF_F16 else 'FP32'} volume, compensated ICP",
+            "frames": frames, "tracking_ok": bool(ok),
+            "pose_drift_max_mm": round(dt_max * 1e3, 5), "pose_drift_max_deg": round(ang_max, 6),
+            "tsdf_err_over_mu": {"voxels": int(same.sum()), "p99": float(np.quantile(terr, 0.99)),
+                                 "p999": float(np.quantile(terr, 0.999)), "max": float(terr.max()),
+                                 "beyond_1e-3": round(float((terr > 1e-3).mean()), 6)},
+            "raycast_vertex_err_mm": {"rays": int(hit.sum()), "p99": round(float(np.quantile(verr, 0.99)) * 1e3, 4),
+                                      "p999": round(float(np.quantile(verr, 0.999)) * 1e3, 4),
+                                      "beyond_0.5mm": round(float((verr > 5e-4).mean()), 6),
+                                      "hit_pattern_differs": round(float((hg != hd).sum() / max(hd.sum(), 1)), 6)}}
 
 
 def measured_traffic(key):
@@ -481,6 +551,9 @@ def main():
     if not args.no_cpu:
         print(f"[bench] CPU baseline on {args.cpu_frames} frames", file=sys.stderr, flush=True)
         cpu = cpu_baseline(wl, args.cpu_frames)
+        # the same host leg checks the server's session type against the FP64 oracle
+        print(f"[bench] FP64 check of the FP16 session on {args.cpu_frames} frames", file=sys.stderr, flush=True)
+        cpu["fp64_check"] = fp64_check(nuif, wl, args.cpu_frames, STORAGES["f16"])
 
     sch = (C.c_int32 * 4)()
     nuif.check(nuif.L().nuif_schedule(sch), "schedule")

@@ -7,6 +7,7 @@
 
 #include "kinfu_core.h"
 
+#include <chrono>
 #include <type_traits>
 
 
@@ -185,6 +186,13 @@ template <class T, class Vol> struct Engine {
     std::vector<uint8_t> cfb;
     std::vector<std::vector<float>> ipyr;
     std::vector<float> iref, gref;
+    // host seconds per stage summed over the frames run: preprocess, ICP (pyramid
+    // included), integration, raycast
+    double stage_s[4] = {0, 0, 0, 0};
+    using Clock = std::chrono::steady_clock;
+    static double since(Clock::time_point t0) {
+        return std::chrono::duration<double>(Clock::now() - t0).count();
+    }
 
     explicit Engine(const EngineCfg& cfg) : c(cfg) {
         cam = make_cam<T>(c.cam, c.W, c.H);
@@ -318,11 +326,13 @@ template <class T, class Vol> struct Engine {
     int run(const uint16_t* raw, const uint8_t* bgra) {
         const int W = c.W, H = c.H;
         const size_t n = (size_t)W * H;
+        auto t0 = Clock::now();
         // NuiKinfuOpenCLFrame::UpdateVertexBuffers (OCL/NuiKinfuOpenCLFrame.cpp:314-367)
         passing_filter<T>(raw, depth.data(), n, cam.dmin, cam.dmax);
         bilateral<T>(depth.data(), dpyr[0].data(), W, H, g.data(), c.filter_radius, (T)c.sigma_depth,
                      (T)c.depth_threshold);
         depth2vertex<T>(dpyr[0].data(), vpyr[0].data(), W, H, cam, 1);
+        stage_s[0] += since(t0);
         const Rigid<T> pose = make_rigid<T>(R, t);
         integrated_last = false;
         if (frames == 0) {
@@ -333,6 +343,7 @@ template <class T, class Vol> struct Engine {
             if (bgra) cfb.assign(bgra, bgra + 4 * n);
             integrate_frame(bgra);
         } else {
+            t0 = Clock::now();
             // NuiKinfuOpenCLDepthTracker::EstimatePose: pyramid, vertices, ICP
             for (int l = 1; l < c.levels; ++l) {
                 half_sample<T>(dpyr[l - 1].data(), dpyr[l].data(), W >> l, H >> l, 1, (T)c.depth_threshold);
@@ -366,6 +377,8 @@ template <class T, class Vol> struct Engine {
                 std::memcpy(R, Rn, sizeof(Rn));
                 std::memcpy(t, tn, sizeof(tn));
             }
+            stage_s[1] += since(t0);
+            t0 = Clock::now();
             // integration gate (VO/NuiKinfuTrackingEngine.cpp:113-127) on global translations
             // (local + the shifting volume's offset; NuiCameraPos::getGlobalTranslation)
             T Rinv[9], Rrel[9];
@@ -382,10 +395,13 @@ template <class T, class Vol> struct Engine {
                 std::memcpy(Rlast, R, sizeof(Rlast));
                 std::memcpy(tlast, t, sizeof(tlast));
             }
+            stage_s[2] += since(t0);
+            t0 = Clock::now();
             const Rigid<T> np = make_rigid<T>(R, t);
             if (bgra && !cvol.empty() && cfb.size() != 4 * n) cfb.assign(4 * n, 0);
             raycast<T, Vol>(vol, bgra ? cvol.data() : nullptr, prm, cam, np, vref.data(), nref.data(),
                             bgra && !cvol.empty() ? cfb.data() : nullptr, wrap());
+            stage_s[3] += since(t0);
         }
         frames++;
         return 1;
@@ -570,6 +586,18 @@ OR_EXPORT int or_engine_volume_raw(void* hp, int16_t* out) {
     auto& e = ((EngineBox<float, VolS16>*)h->p)->e;
     std::memcpy(out, e.vol.v.data(), e.vol.v.size() * sizeof(int16_t));
     return 0;
+}
+
+// Host seconds per stage summed over the frames run since the last call (preprocess, ICP,
+// integration, raycast), then cleared.
+OR_EXPORT void or_engine_stage_seconds(void* hp, double* out4) {
+    with_engine((OrEngine*)hp, [&](auto& e) {
+        for (int k = 0; k < 4; ++k) {
+            out4[k] = e.stage_s[k];
+            e.stage_s[k] = 0;
+        }
+        return 0;
+    });
 }
 
 // The joint order's helpers on their own, in double (for the closed-form checks).

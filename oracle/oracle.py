@@ -68,6 +68,7 @@ def lib():
             "or_engine_set_icp_mode": (None, [vp, ci]),
             "or_engine_feedback_colors": (ci, [vp, vp]),
             "or_engine_set_photometric": (None, [vp, cf, cf, cf]),
+            "or_engine_stage_seconds": (None, [vp, f64p]),
             "or_engine_photometric": (cll, [vp, ci, vp]),
             "or_intensity_half_sample": (None, [f32p, f32p, ci, ci]),
             "or_photometric_row_d": (ci, [C.c_double, f64p, f64p, f32p, ci, ci, f64p, f64p, C.c_double, C.c_double,
@@ -185,6 +186,13 @@ class OracleEngine:
         out = np.empty(max(n, 0), np.uint8)
         if n > 0:
             lib().or_engine_volume_words(self.h, out.ctypes.data)
+        return out
+
+    def stage_seconds(self):
+        """Host seconds per stage (preprocess, ICP, integration, raycast) summed over the
+        frames run since the last call."""
+        out = np.zeros(4, np.float64)
+        lib().or_engine_stage_seconds(self.h, out)
         return out
 
     def set_photometric(self, weight, max_difference=0.2, min_gradient=0.002):
