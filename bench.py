@@ -82,8 +82,11 @@ def fp64_check(nuif, wl, frames, storage):
         R, t = poses[f, :9].reshape(3, 3).astype(np.float64), poses[f, 9:12].astype(np.float64)
         R64, t64 = p64[:9].reshape(3, 3), p64[9:]
         dt_max = max(dt_max, float(np.linalg.norm(t - t64)))
-        c = np.clip((np.trace(R @ R64.T) - 1.0) / 2.0, -1.0, 1.0)
-        ang_max = max(ang_max, float(np.degrees(np.arccos(c))))
+        # the angle as atan2(|vee(skew part)|, (tr - 1) / 2): arccos of the trace would
+        # turn the float32 rounding of R into ~0.01 deg near zero
+        Rr = R.T @ R64
+        w = 0.5 * np.array([Rr[2, 1] - Rr[1, 2], Rr[0, 2] - Rr[2, 0], Rr[1, 0] - Rr[0, 1]])
+        ang_max = max(ang_max, float(np.degrees(np.arctan2(np.linalg.norm(w), (np.trace(Rr) - 1.0) / 2.0))))
         vd.integrate(d, poses[f, :9].reshape(3, 3), poses[f, 9:12])
     o64.close()
     raw = eng.volume_raw()
