@@ -705,10 +705,12 @@ __global__ void __launch_bounds__(kBlock)
 
 // ---- the finish launch -------------------------------------------------------------
 
+constexpr int kFinSegs = 32;  // compensated: tile segments per term, summed in parallel
+static_assert(kTerms * kFinSegs <= kFinBlock, "one thread per term and segment");
 struct FinLds {
     IcpJob job;
     uint32_t epoch;
-    double seg[kTerms * 8];                // compensated: segment sums
+    double seg[kTerms * kFinSegs];         // compensated: segment sums
     double sums[kTerms];
     float sumsf[kTerms];
     float chunk[kMaxChunks * kTerms];      // reference: compute_sums results
@@ -765,26 +767,28 @@ __global__ void __launch_bounds__(kFinBlock)
     } else {
         const double* part = in_global(reinterpret_cast<const double*>(J.partials));
         const int ntiles = level_tiles(J.cam, level);
-        if (tid < kTerms * 8) {
-            // term k, segment j: tiles j, j + 8, ... in order (loads 16 at a time)
-            const int k = tid >> 3, j = tid & 7;
+        if (tid < kTerms * kFinSegs) {
+            // term k, segment j: tiles j, j + kFinSegs, ... in order, 16 loads in flight
+            // (past the last tile a load is replaced by +0, which leaves the sum as it is)
+            const int k = tid / kFinSegs, j = tid % kFinSegs;
             const double* pk = part + k;
             double s = 0.0;
-            int q = j;
-            for (; q + 8 * 15 < ntiles; q += 8 * 16) {
+            for (int q0 = j; q0 < ntiles; q0 += kFinSegs * 16) {
                 double v[16];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) v[u] = pk[(size_t)(q + 8 * u) * kTerms];
+                for (int u = 0; u < 16; ++u) {
+                    const int q = q0 + kFinSegs * u;
+                    v[u] = q < ntiles ? pk[(size_t)q * kTerms] : 0.0;
+                }
 #pragma unroll
                 for (int u = 0; u < 16; ++u) s += v[u];
             }
-            for (; q < ntiles; q += 8) s += pk[(size_t)q * kTerms];
-            F.seg[k * 8 + j] = s;
+            F.seg[k * kFinSegs + j] = s;
         }
         __syncthreads();
         if (tid < kTerms) {
             double s = 0.0;
-            for (int j = 0; j < 8; ++j) s += F.seg[tid * 8 + j];
+            for (int j = 0; j < kFinSegs; ++j) s += F.seg[tid * kFinSegs + j];
             F.sums[tid] = s;
         }
         __syncthreads();

@@ -588,8 +588,8 @@ void joint_pixel(const Cam<T>& cam, V3<T> vc, const T R[9], const T t[3], const 
 // i = 0..T/256-1, with Kahan-compensated FP32 sums (the count plainly); each thread's value s - c
 // in double; per term, the values of threads 16g .. 16g + 15 added in thread order
 // (g = 0..15), then a tree over the 16 run sums (item l takes item l + d for d = 8, 4,
-// 2, 1); then term by term the tile sums k = j, j + 8, j + 16, .. in order for
-// j = 0..7, and those eight in order.
+// 2, 1); then term by term the tile sums k = j, j + 32, j + 64, .. in order for
+// j = 0..31, and those 32 in order.
 inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int npx, const float R[9], const float t[3],
                                  const float* vref, const float* nref, const Rigid<float>& prev, float distThr,
                                  double out[kTerms], const PhotoIn<float>* ph = nullptr) {
@@ -633,14 +633,14 @@ inline void icp_sums_compensated(const Cam<float>& cam, const float* vcur, int n
         }
     }
     for (int q = 0; q < kTerms; ++q) {
-        double seg[8];
-        for (int j = 0; j < 8; ++j) {
+        double seg[32];
+        for (int j = 0; j < 32; ++j) {
             double sacc = 0.0;
-            for (int k = j; k < ntiles; k += 8) sacc += tiles[(size_t)k * kTerms + q];
+            for (int k = j; k < ntiles; k += 32) sacc += tiles[(size_t)k * kTerms + q];
             seg[j] = sacc;
         }
         double tot = 0.0;
-        for (int j = 0; j < 8; ++j) tot += seg[j];
+        for (int j = 0; j < 32; ++j) tot += seg[j];
         out[q] = tot;
     }
 }
