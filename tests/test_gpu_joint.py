@@ -7,7 +7,8 @@ against the oracle's float restatement, bit for bit:
     intensity pyramid, the model's intensities and gradients);
   * a session alone (each frame's pipeline captured as a graph): every frame's pose,
     the volume's storage words, the raycast maps and the feedback colours over 12
-    frames of the 320x240 orbit with colour, FP16 and FP32 volumes;
+    frames of the 320x240 orbit with colour, FP16 and FP32 volumes, and over 8 frames of
+    the 640x480 / 512^3 orbit with an FP16 volume;
   * the 16-session mix (6 x 320x240 with 256^3, 6 x 640x480 and 4 x 1280x720 with 512^3
     FP16 volumes, colour) in batched server rounds, where all sessions' joint iterations run in
     lockstep launches: each session's poses, maps and feedback colours equal the same
@@ -119,6 +120,35 @@ def test_joint_engine_matches_oracle(gpu, storage):
     assert cf is not None and (cf[:, 3] > 0).sum() > wl.width * wl.height // 4
     np.testing.assert_array_equal(eng.feedback_colors(), cf)
     eng.close()
+
+
+@pytest.mark.timeout(900)
+def test_joint_engine_full_size_matches_oracle(gpu):
+    """640x480 / 512^3 FP16 with colour, 8 frames: every pose, the volume words, the
+    raycast maps and the feedback colours bit-exact vs the oracle's joint order."""
+    wl = W.FULL
+    frames = 8
+    announce(f"joint engine {wl.name} f16 colour, {frames} frames vs oracle")
+    desc = wl.kinfu_desc(storage=nuif.TSDF_F16, gate=0.0, color=True, use_graph=True, reduction=nuif.ICP_REDUCE_JOINT)
+    eng = nuif.Engine(desc)
+    rendered = [wl.render(f, color=True) for f in range(frames)]
+    for d, bgra, _ in rendered:
+        eng.run(nuif.DeviceArray.from_host(d), nuif.DeviceArray.from_host(bgra), sync=False)
+    assert eng.synchronize() == 1
+    poses = eng.poses(frames)
+    ref = O.OracleEngine(wl, prec=0, color=True, joint=True, storage=1)
+    for f, (d, bgra, _) in enumerate(rendered):
+        ok, pr, _ = ref.run(d, bgra)
+        assert ok == 1 and poses[f, 12] == 1.0, f
+        np.testing.assert_array_equal(poses[f, :12], pr.astype(np.float32), err_msg=f"frame {f}")
+    np.testing.assert_array_equal(eng.volume_raw(), ref.volume_words())
+    vg, ng = eng.maps()
+    vr, nr = ref.refmaps()
+    same_bits_or_nan(vg, vr, "vmap")
+    same_bits_or_nan(ng, nr, "nmap")
+    np.testing.assert_array_equal(eng.feedback_colors(), ref.feedback_colors())
+    eng.close()
+    announce("joint engine full size: poses, volume, maps and colours exact")
 
 
 @pytest.mark.timeout(900)
