@@ -733,9 +733,9 @@ __device__ __forceinline__ void get_voxel_fast(f3 pt, const TsdfParams& p, const
 // the first corner and steps to the others by the unwrapped strides, which reads the
 // wrong voxels (or past the end of the volume) where a cell straddles the wrap.
 template <class S, bool WRAP, bool COUNT, bool I32>
-__device__ float trilinear(f3 pt, const VolView<S, WRAP, COUNT, I32>& vv, const TsdfParams& p) {
+__device__ float trilinear(f3 pt, const VolView<S, WRAP, COUNT, I32>& vv, const TsdfParams& p, const CellInv& ri) {
     int g[3];
-    get_voxel(pt, p, g);
+    get_voxel_fast(pt, p, ri, g);
     if (g[0] <= 0 || g[0] >= vv.Nx - 1) return kNaN;
     if (g[1] <= 0 || g[1] >= vv.Ny - 1) return kNaN;
     if (g[2] <= 0 || g[2] >= vv.Nz - 1) return kNaN;
@@ -1002,25 +1002,25 @@ __device__ __forceinline__ void raycast_body(const typename S::word* __restrict_
         if (crossing) {
             do {
                 // both interpolations are evaluated (pure reads) so that their loads overlap
-                const float Ftdt = trilinear<S, WRAP, COUNT, I32>(add3(rs, mul3(dir, tcur + step)), vv, p);
-                const float Ft = trilinear<S, WRAP, COUNT, I32>(add3(rs, mul3(dir, tcur)), vv, p);
+                const float Ftdt = trilinear<S, WRAP, COUNT, I32>(add3(rs, mul3(dir, tcur + step)), vv, p, ri);
+                const float Ft = trilinear<S, WRAP, COUNT, I32>(add3(rs, mul3(dir, tcur)), vv, p, ri);
                 if (isnan(Ftdt) || isnan(Ft)) break;
                 const float Ts = tcur - step * Ft / (Ftdt - Ft);
                 const f3 vf = add3(rs, mul3(dir, Ts));
                 vout = sub3(vf, off);
                 if (cvol && cmap) cout = cvol[idx];
                 int gg[3];
-                get_voxel(add3(rs, mul3(dir, tcur)), p, gg);
+                get_voxel_fast(add3(rs, mul3(dir, tcur)), p, ri, gg);
                 if (gg[0] > 1 && gg[1] > 1 && gg[2] > 1 && gg[0] < vv.Nx - 2 && gg[1] < vv.Ny - 2 && gg[2] < vv.Nz - 2) {
                     f3 n, q;
-                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
-                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
+                    q = vf; q.x += p.cell_size[0]; const float fx1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p, ri);
+                    q = vf; q.x -= p.cell_size[0]; const float fx2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p, ri);
                     n.x = fx1 - fx2;
-                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
-                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
+                    q = vf; q.y += p.cell_size[1]; const float fy1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p, ri);
+                    q = vf; q.y -= p.cell_size[1]; const float fy2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p, ri);
                     n.y = fy1 - fy2;
-                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
-                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p);
+                    q = vf; q.z += p.cell_size[2]; const float fz1 = trilinear<S, WRAP, COUNT, I32>(q, vv, p, ri);
+                    q = vf; q.z -= p.cell_size[2]; const float fz2 = trilinear<S, WRAP, COUNT, I32>(q, vv, p, ri);
                     n.z = fz1 - fz2;
                     nout = normalize3(n);
                 }
