@@ -94,7 +94,8 @@ int nuifk_state(void* h, nuif_icp_result* out);
 int nuifk_read_volume(void* h, void* out, size_t bytes);
 int nuifk_read_maps(void* h, float* vmap, float* nmap);
 /* The feedback frame's colours (W*H BGRA): the last raycast's, or the first frame's
- * (OCL/NuiKinfuOpenCLFeedbackFrame.cpp colour buffer). */
+ * (m_colorsCL of OCL/NuiKinfuOpenCLFeedbackFrame.cpp:48, filled by CopyColors :90,186-200
+ * or by the raycast). */
 int nuifk_read_feedback_colors(void* h, uint8_t* out);
 /* The joint ICP order's photometric weight (default 0.1), largest intensity difference
  * (0.2) and minimum gradient (0.002) (nuif_icp_photometric); before the next frame.  No
