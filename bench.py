@@ -58,27 +58,31 @@ def cpu_baseline(wl, frames):
                       f"float oracle with the reference's short2 volume, OpenMP"}
 
 
-def fp64_check(nuif, wl, frames, storage):
+def fp64_check(nuif, wl, frames, storage, joint=False):
     """The server's session type (FP16 or FP32 volume, compensated ICP) on the GPU against
     the FP64 oracle on the same frames -- the oracle as the checker, on the host: pose
     drift per frame against the FP64 engine, the TSDF against an FP64 volume integrated at
     the GPU's poses (voxels observed by both with equal weights, in units of the
     truncation distance) and the raycast vertices against an FP64 raycast of that volume
-    at the last pose (rays hit by both)."""
+    at the last pose (rays hit by both).  joint: the joint ICP order with colour frames
+    against the FP64 instance of the joint order."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    depths = [wl.render(f)[0] for f in range(frames)]
-    eng = nuif.Engine(wl.kinfu_desc(storage=storage, gate=0.0, reduction=nuif.ICP_REDUCE_COMPENSATED))
-    for d in depths:
-        eng.run(nuif.DeviceArray.from_host(d), sync=False)
+    rendered = [wl.render(f, color=joint)[:2] for f in range(frames)]
+    depths = [r[0] for r in rendered]
+    red = nuif.ICP_REDUCE_JOINT if joint else nuif.ICP_REDUCE_COMPENSATED
+    eng = nuif.Engine(wl.kinfu_desc(storage=storage, gate=0.0, reduction=red, color=joint))
+    for d, b in rendered:
+        eng.run(nuif.DeviceArray.from_host(d), nuif.DeviceArray.from_host(b) if joint else None, sync=False)
     ok = eng.synchronize() == 1
     poses = eng.poses(frames)
-    o64 = O.OracleEngine(wl, prec=1, gate=0.0, compensated=True)
+    o64 = O.OracleEngine(wl, prec=1, gate=0.0, compensated=not joint, joint=joint, color=joint)
     vd = O.OracleVolumeD(wl)
     dt_max, ang_max = 0.0, 0.0
-    for f, d in enumerate(depths):
-        _, p64, _ = o64.run(d)
+    drift = []
+    for f, (d, b) in enumerate(rendered):
+        _, p64, _ = o64.run(d, b)
         R, t = poses[f, :9].reshape(3, 3).astype(np.float64), poses[f, 9:12].astype(np.float64)
         R64, t64 = p64[:9].reshape(3, 3), p64[9:]
         dt_max = max(dt_max, float(np.linalg.norm(t - t64)))
@@ -87,6 +91,7 @@ def fp64_check(nuif, wl, frames, storage):
         Rr = R.T @ R64
         w = 0.5 * np.array([Rr[2, 1] - Rr[1, 2], Rr[0, 2] - Rr[2, 0], Rr[1, 0] - Rr[0, 1]])
         ang_max = max(ang_max, float(np.degrees(np.arctan2(np.linalg.norm(w), (np.trace(Rr) - 1.0) / 2.0))))
+        drift.append(round(float(np.linalg.norm(t - t64)) * 1e3, 4))
         vd.integrate(d, poses[f, :9].reshape(3, 3), poses[f, 9:12])
     o64.close()
     raw = eng.volume_raw()
@@ -107,8 +112,9 @@ def fp64_check(nuif, wl, frames, storage):
     hg, hd = ~np.isnan(vg[:, 2]), ~np.isnan(vr[:, 2])
     hit = hg & hd
     verr = np.linalg.norm(vg[hit].astype(np.float64) - vr[hit], axis=1)
-    return {"session": f"{'FP16' if storage == nuif.TSDF_F16 else 'FP32'} volume, compensated ICP",
-            "frames": frames, "tracking_ok": bool(ok),
+    return {"session": f"{'FP16' if storage == nuif.TSDF_F16 else 'FP32'} volume, "
+                       f"{'joint ICP (colour weight 0.1)' if joint else 'compensated ICP'}",
+            "frames": frames, "tracking_ok": bool(ok), "pose_drift_mm_per_frame": drift,
             "pose_drift_max_mm": round(dt_max * 1e3, 5), "pose_drift_max_deg": round(ang_max, 6),
             "tsdf_err_over_mu": {"voxels": int(same.sum()), "p99": float(np.quantile(terr, 0.99)),
                                  "p999": float(np.quantile(terr, 0.999)), "max": float(terr.max()),
@@ -557,6 +563,9 @@ def main():
         # the same host leg checks the server's session type against the FP64 oracle
         print(f"[bench] FP64 check of the FP16 session on {args.cpu_frames} frames", file=sys.stderr, flush=True)
         cpu["fp64_check"] = fp64_check(nuif, wl, args.cpu_frames, STORAGES["f16"])
+        print(f"[bench] FP64 check of the joint-order FP16 session on {args.cpu_frames} frames", file=sys.stderr,
+              flush=True)
+        cpu["fp64_check_joint"] = fp64_check(nuif, wl, args.cpu_frames, STORAGES["f16"], joint=True)
 
     sch = (C.c_int32 * 4)()
     nuif.check(nuif.L().nuif_schedule(sch), "schedule")

@@ -153,6 +153,33 @@ struct EngineCfg {
 
 // OCL/NuiKinfuOpenCLShiftScene.cpp:119-245 fetchSlice reads the slab with
 // K/extract.cl:80-190; the restatement lives in points_oracle.cpp.
+// K/transformMaps.cl:372-384 intensities in double (the FP64 engine's joint order)
+void intensities_d(const uint8_t* bgra, double* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        const uint8_t* q = bgra + 4 * i;
+        out[i] = q[3] > 0 ? 0.299 * (q[2] / 255.0) + 0.587 * (q[1] / 255.0) + 0.114 * (q[0] / 255.0)
+                          : qnan<double>();
+    }
+}
+// K/transformMaps.cl:448-511 derivatives in double: Sobel / 8 at the centre, NaN on the
+// border and next to a NaN intensity (as the float restatement in color_oracle.cpp)
+void derivatives_d(const double* I, double* out, int W, int H) {
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t q = (size_t)y * W + x;
+            out[2 * q] = out[2 * q + 1] = qnan<double>();
+            if (std::isnan(I[q]) || !(x > 0 && x < W - 1 && y > 0 && y < H - 1)) continue;
+            auto at = [&](int dx, int dy) { return I[(size_t)(y + dy) * W + x + dx]; };
+            const double i00 = at(-1, -1), i10 = at(0, -1), i20 = at(1, -1), i01 = at(-1, 0), i21 = at(1, 0),
+                         i02 = at(-1, 1), i12 = at(0, 1), i22 = at(1, 1);
+            if (std::isnan(i00) || std::isnan(i10) || std::isnan(i20) || std::isnan(i01) || std::isnan(i21) ||
+                std::isnan(i02) || std::isnan(i12) || std::isnan(i22))
+                continue;
+            out[2 * q] = ((i20 - i00) + 2.0 * (i21 - i01) + (i22 - i02)) / 8.0;
+            out[2 * q + 1] = ((i02 - i00) + 2.0 * (i12 - i10) + (i22 - i20)) / 8.0;
+        }
+}
+
 // K/transformMaps.cl:372-384 and :448-511, restated in color_oracle.cpp.
 extern "C" void or_bgra_to_intensity(const uint8_t* bgra, float* out, long long n);
 extern "C" void or_intensity_derivatives(const float* I, float* out, int W, int H);
@@ -186,6 +213,8 @@ template <class T, class Vol> struct Engine {
     std::vector<uint8_t> cfb;
     std::vector<std::vector<float>> ipyr;
     std::vector<float> iref, gref;
+    std::vector<std::vector<double>> ipyrd;  // the FP64 engine's
+    std::vector<double> irefd, grefd;
     // host seconds per stage summed over the frames run: preprocess, ICP (pyramid
     // included), integration, raycast
     double stage_s[4] = {0, 0, 0, 0};
@@ -298,8 +327,26 @@ template <class T, class Vol> struct Engine {
     IcpResult joint_icp(const IcpConfig& ic, const T* const* vp, const Rigid<T>& pose, const uint8_t* bgra, double Rn[9],
                         double tn[3]) {
         if constexpr (!std::is_same<T, float>::value) {
-            (void)ic, (void)vp, (void)pose, (void)bgra, (void)Rn, (void)tn;
-            return IcpResult{0, 0, 0, 0};  // the joint order is a float engine's
+            // FP64: the same inputs and rows evaluated in double
+            if (!bgra || cfb.empty()) return IcpResult{0, 0, 0, 0};
+            const size_t n = (size_t)c.W * c.H;
+            ipyrd.resize(c.levels);
+            ipyrd[0].resize(n);
+            intensities_d(bgra, ipyrd[0].data(), n);
+            for (int l = 1; l < c.levels; ++l) {
+                ipyrd[l].resize((size_t)(c.W >> l) * (c.H >> l));
+                intensity_half_sample<double>(ipyrd[l - 1].data(), ipyrd[l].data(), c.W >> l, c.H >> l);
+            }
+            irefd.resize(n);
+            grefd.resize(2 * n);
+            intensities_d(cfb.data(), irefd.data(), n);
+            derivatives_d(irefd.data(), grefd.data(), c.W, c.H);
+            const double mg = (double)c.photo[2];
+            PhotoIn<double> ph{nullptr, irefd.data(), grefd.data(), std::sqrt((double)c.photo[0]), (double)c.photo[1],
+                               mg * mg};
+            std::vector<const double*> ip(c.levels);
+            for (int l = 0; l < c.levels; ++l) ip[l] = ipyrd[l].data();
+            return icp_compensated<double>(cam, ic, vp, vref.data(), nref.data(), pose, Rn, tn, &ph, ip.data());
         } else {
             if (!bgra || cfb.empty()) return IcpResult{0, 0, 0, 0};  // the joint order needs colours
             const size_t n = (size_t)c.W * c.H;

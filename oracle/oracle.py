@@ -157,7 +157,7 @@ class OracleEngine:
                                         len(iterations or wl.iterations), its, wl.volume_res, wl.volume_dim,
                                         wl.trunc, gate, 1 if color else 0)
         if joint:  # the joint order: photometric rows (weight 0.1) beside the compensated sums
-            assert color and prec == 0, "the joint order runs on a float engine with colours"
+            assert color, "the joint order needs colours"
             lib().or_engine_set_icp_mode(self.h, 2)
         elif compensated:
             lib().or_engine_set_icp_mode(self.h, 1)

@@ -153,3 +153,16 @@ def test_joint_engine_tracks_the_orbit():
     assert len(joint.photometric(0)) == wl.width * wl.height
     assert len(joint.photometric(1)) == (wl.width // 2) * (wl.height // 2)
     assert len(joint.photometric(4)) == 2 * wl.width * wl.height
+
+
+def test_fp64_joint_engine_weight_zero_is_the_fp64_compensated_order():
+    """The FP64 instance of the joint order (intensities, half-samples and derivatives
+    in double) reduces to the FP64 compensated order when the photometric weight is 0."""
+    wl = W.SMALL
+    frames = 4
+    comp = O.OracleEngine(wl, prec=1, color=True, compensated=True)
+    joint = O.OracleEngine(wl, prec=1, color=True, joint=True)
+    joint.set_photometric(0.0)
+    np.testing.assert_array_equal(_poses(joint, frames, wl), _poses(comp, frames, wl))
+    joint.close()
+    comp.close()
