@@ -196,15 +196,16 @@ typedef struct nuif_icp_config {
 /* Inputs of the joint order's photometric rows.  For a pixel whose geometric association
  * holds (current vertex p at the estimate, projected through the previous pose to the
  * full-resolution pixel (u, v)): I the current intensity at the pixel's level, I_ref and
- * g = (dI/du, dI/dv) the model's intensity and its gradient bilinearly at (u, v); with
+ * g = (dI/du, dI/dv) the previous frame's intensity and its gradient bilinearly at (u, v)
+ * (the previous frame's colours were taken at the previous pose); with
  * |I - I_ref| < max_difference and |g|^2 > min_gradient^2 the row is
  * sqrt(weight) [J x p, J, I - I_ref], J the gradient carried to world coordinates through
  * the previous camera, added to the normal equations beside the geometric row
  * [n x p, n, n . (q - p)]. */
 typedef struct nuif_icp_photometric {
     const float* intensity_pyr[NUIF_ICP_MAX_LEVELS]; /* current intensities, W>>l x H>>l per level */
-    const float* ref_model;                          /* W x H float4: the model's intensity (previous
-                                                        pose), dI/du, dI/dv, 0 */
+    const float* ref_model;                          /* W x H float4: the previous frame's
+                                                        intensity, dI/du, dI/dv, 0 */
     float weight;                                    /* photometric weight (0.1) */
     float max_difference;
     float min_gradient;
@@ -212,7 +213,7 @@ typedef struct nuif_icp_photometric {
 /* The joint order's inputs of a frame in one launch (no counterpart in the reference):
  * intensity_pyr[0] = nuif_bgra_to_intensity of the frame's colours, level l the
  * nuif_intensity_half_sample of level l - 1; ref_model = the intensities of ref_bgra (the
- * previous raycast's colours) with their nuif_intensity_derivatives.  width and height
+ * previous frame's colours) with their nuif_intensity_derivatives.  width and height
  * must be multiples of 2^(levels - 1). */
 int nuif_icp_photometric_inputs(const uint8_t* bgra, const uint8_t* ref_bgra, int width, int height, int levels,
                                 const nuif_icp_photometric* photometric, nuif_stream_t s);
@@ -447,6 +448,9 @@ typedef struct nuif_round_session {
     float* vmap;                               /* feedback maps: raycast output, W*H float3 */
     float* nmap;
     uint8_t* cmap;                             /* W*H uchar4 or NULL */
+    uint8_t* prev_bgra;                        /* joint ICP order: the previous frame's colours
+                                                  (W*H BGRA), the photometric rows' model;
+                                                  replaced by this frame's colours */
 } nuif_round_session;
 /* Bytes of the round's job table (device memory) and of its host staging copy (pinned host
  * memory, nuif_host_alloc) for n sessions. */

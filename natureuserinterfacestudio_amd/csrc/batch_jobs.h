@@ -137,10 +137,11 @@ size_t icp_job_bytes();
 int make_icp_jobs(const nuif_icp_session* sessions, int n, void* host_out, int* counts);
 int launch_icp_jobs(const void* dtable, const void* hjobs, int n, const int* counts, hipStream_t st);
 // the joint ICP order's inputs (color.hip): job k of a host table for a frame (its blocks
-// numbered from *blocks, which advances past them), and one launch for n jobs
+// numbered from *blocks, which advances past them; ref_bgra, the previous frame's
+// colours, receives this frame's after the inputs are made), and the launches for n jobs
 size_t photo_job_bytes();
-int make_photo_job_at(void* hjobs, int k, const uint8_t* bgra, const uint8_t* ref_bgra,
-                      const nuif_icp_photometric& ph, int W, int H, int levels, unsigned* blocks);
+int make_photo_job_at(void* hjobs, int k, const uint8_t* bgra, uint8_t* ref_bgra, const nuif_icp_photometric& ph,
+                      int W, int H, int levels, unsigned* blocks);
 int launch_photo_jobs(const void* djobs, int n, unsigned blocks, hipStream_t st);
 int make_gauss(const float* gaussian, int radius, Gauss* g);
 

@@ -516,6 +516,7 @@ struct PhotoJob {
     const uchar4* ref;
     float* pyr[NUIF_ICP_MAX_LEVELS];
     float4* model;
+    uchar4* keep;  // photometric_keep_kernel: receives the frame's colours (the next frame's ref)
     int W, H, levels, gx;
     unsigned block0;
 };
@@ -593,6 +594,22 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// The frame's colours into `keep` of every job, after photometric_inputs_kernel has read
+// the reference colours there (the next frame's model).
+__global__ void __launch_bounds__(256) photometric_keep_kernel(const PhotoJob* __restrict__ jobs, int n) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].block0 <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const PhotoJob& J = jobs[lo];
+    const unsigned b = blockIdx.x - J.block0;
+    const int x = (int)(b % (unsigned)J.gx) * 16 + (int)(threadIdx.x & 15);
+    const int y = (int)(b / (unsigned)J.gx) * 16 + (int)(threadIdx.x >> 4);
+    if (J.keep && x < J.W && y < J.H) J.keep[(size_t)y * J.W + x] = J.bgra[(size_t)y * J.W + x];
+}
+
 int make_photo_job(const uint8_t* bgra, const uint8_t* ref_bgra, const nuif_icp_photometric& ph, int W, int H,
                    int levels, unsigned block0, PhotoJob* out) {
     NUIF_CHECK_ARG(bgra && ref_bgra && ph.ref_model && W > 0 && H > 0 && levels >= 1 && levels <= NUIF_ICP_MAX_LEVELS,
@@ -620,11 +637,12 @@ unsigned photo_job_blocks(const PhotoJob& J) { return (unsigned)J.gx * (unsigned
 
 size_t photo_job_bytes() { return sizeof(PhotoJob); }
 
-int make_photo_job_at(void* hjobs, int k, const uint8_t* bgra, const uint8_t* ref_bgra,
-                      const nuif_icp_photometric& ph, int W, int H, int levels, unsigned* blocks) {
+int make_photo_job_at(void* hjobs, int k, const uint8_t* bgra, uint8_t* ref_bgra, const nuif_icp_photometric& ph,
+                      int W, int H, int levels, unsigned* blocks) {
     PhotoJob* J = static_cast<PhotoJob*>(hjobs) + k;
     const int rc = make_photo_job(bgra, ref_bgra, ph, W, H, levels, *blocks, J);
     if (rc) return rc;
+    J->keep = reinterpret_cast<uchar4*>(ref_bgra);
     *blocks += photo_job_blocks(*J);
     return NUIF_OK;
 }
@@ -632,7 +650,8 @@ int make_photo_job_at(void* hjobs, int k, const uint8_t* bgra, const uint8_t* re
 int launch_photo_jobs(const void* djobs, int n, unsigned blocks, hipStream_t st) {
     if (n <= 0) return NUIF_OK;
     photometric_inputs_kernel<<<blocks, 256, 0, st>>>(static_cast<const PhotoJob*>(djobs), n, PhotoJob{});
-    NUIF_LAUNCHED("photometric_inputs_kernel");
+    photometric_keep_kernel<<<blocks, 256, 0, st>>>(static_cast<const PhotoJob*>(djobs), n);
+    NUIF_LAUNCHED("photometric_inputs_kernel / photometric_keep_kernel");
     return NUIF_OK;
 }
 

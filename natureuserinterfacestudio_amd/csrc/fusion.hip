@@ -71,9 +71,9 @@ int check_session(const nuif_round_session& r, int i) {
     NUIF_CHECK_ARG(Nz <= kMinSegVoxels || r.walk_states, "nuif_fusion_round: session %d needs walk states", i);
     if (r.icp.cfg.reduction == NUIF_ICP_REDUCE_JOINT) {
         const nuif_icp_photometric& ph = r.icp.photometric;
-        NUIF_CHECK_ARG(r.bgra && r.cmap && ph.ref_model,
-                       "nuif_fusion_round: session %d: the joint ICP needs the frame's colours, the feedback "
-                       "colours and the intensity buffers", i);
+        NUIF_CHECK_ARG(r.bgra && r.prev_bgra && ph.ref_model && (const void*)r.bgra != r.prev_bgra,
+                       "nuif_fusion_round: session %d: the joint ICP needs the frame's colours, the previous "
+                       "frame's colours and the intensity buffers", i);
         for (int l = 0; l < L; ++l)
             NUIF_CHECK_ARG(ph.intensity_pyr[l], "nuif_fusion_round: session %d: intensity level %d is NULL", i, l);
     }
@@ -177,7 +177,7 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
     for (int i = 0; i < n; ++i) {
         const nuif_round_session& r = S[i];
         if (r.icp.cfg.reduction != NUIF_ICP_REDUCE_JOINT) continue;
-        rc = make_photo_job_at(h + lay.photo, nphoto++, r.bgra, r.cmap, r.icp.photometric,
+        rc = make_photo_job_at(h + lay.photo, nphoto++, r.bgra, r.prev_bgra, r.icp.photometric,
                                (int)r.icp.cam.depthImageWidth, (int)r.icp.cam.depthImageHeight, r.icp.cfg.levels,
                                &pblocks);
         if (rc) return rc;

@@ -276,6 +276,7 @@ void NuiKinfuHIPDepthTracker::SetReduction(int mode) {
     }
     m_refModel = DeviceBuffer((size_t)m_W * m_H * 4 * sizeof(float));
     m_photo.ref_model = m_refModel.as<float>();
+    m_prevColors = DeviceBuffer((size_t)m_W * m_H * 4);
     if (m_photo.weight == 0.f) SetPhotometric(0.1f, 0.2f, 0.002f);
 }
 
@@ -288,13 +289,22 @@ void NuiKinfuHIPDepthTracker::SetPhotometric(float weight, float maxDifference, 
 }
 
 // The joint order's inputs: intensities of the frame's colours at every level, and of
-// the feedback colours (the previous raycast's, or the first frame's) with their gradients.
-void NuiKinfuHIPDepthTracker::preparePhotometric(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb) {
+// the previous frame's colours with their gradients; then this frame's colours are kept
+// for the next frame.
+void NuiKinfuHIPDepthTracker::preparePhotometric(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame*) {
     if (!f->GetColorBuffer())
         throw std::runtime_error("the joint ICP needs the frame's colours (and a scene with a colour volume)");
-    check(nuif_icp_photometric_inputs(f->GetColorBuffer(), fb->GetColorBuffer(), (int)m_W, (int)m_H, m_icfg.levels,
-                                      &m_photo, m_stream),
+    check(nuif_icp_photometric_inputs(f->GetColorBuffer(), m_prevColors.as<uint8_t>(), (int)m_W, (int)m_H,
+                                      m_icfg.levels, &m_photo, m_stream),
           "nuif_icp_photometric_inputs");
+    KeepColors(f);
+}
+
+void NuiKinfuHIPDepthTracker::KeepColors(NuiKinfuHIPFrame* f) {
+    if (!m_prevColors.get()) return;
+    if (!f->GetColorBuffer())
+        throw std::runtime_error("the joint ICP needs the frame's colours (and a scene with a colour volume)");
+    check(nuif_memcpy_d2d(m_prevColors.get(), f->GetColorBuffer(), (size_t)m_W * m_H * 4, m_stream), "colour copy");
 }
 
 void NuiKinfuHIPDepthTracker::FillIcpSession(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb,
@@ -740,6 +750,7 @@ bool NuiKinfuTrackingEngine::RunTrackingDevice(INT64, const UINT16* dDepths, con
             scene->SetGates(nullptr, nullptr);
             scene->integrateVolume(m_pFrame, m_pCameraState);
         }
+        if (m_pTracker->Reduction() == NUIF_ICP_REDUCE_JOINT) m_pTracker->KeepColors(m_pFrame);
         m_pTracker->RecordPose(m_pCameraState);
     } else if (m_useGraph && !m_stageTiming && !(scene && scene->RequiresHostWork())) {
         if (!m_graph || m_graphScene != scene) {
@@ -786,8 +797,10 @@ bool NuiKinfuTrackingEngine::PrepareRoundFrame(const UINT16* dDepths, const uint
     out->nmap = m_pFeedbackFrame->GetNormalBuffer();
     out->cmap = out->color_volume ? m_pFeedbackFrame->GetColorBuffer() : nullptr;
     out->bgra = out->color_volume ? dBgra : nullptr;
-    if (m_pTracker->Reduction() == NUIF_ICP_REDUCE_JOINT && !out->bgra)
-        throw std::runtime_error("the joint ICP needs the frame's colours");
+    if (m_pTracker->Reduction() == NUIF_ICP_REDUCE_JOINT) {
+        if (!out->bgra) throw std::runtime_error("the joint ICP needs the frame's colours");
+        out->prev_bgra = m_pTracker->PreviousColors();
+    }
     m_frames++;
     return true;
 }

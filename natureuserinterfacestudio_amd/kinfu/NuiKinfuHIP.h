@@ -151,6 +151,9 @@ public:
     // and the feedback frame must carry colours)
     void SetReduction(int mode);
     int Reduction() const { return m_icfg.reduction; }
+    // the joint order's model: the previous frame's colours (kept from each frame)
+    void KeepColors(NuiKinfuHIPFrame* f);
+    uint8_t* PreviousColors() const { return m_prevColors.as<uint8_t>(); }
     // the joint order's photometric weight, intensity difference and minimum gradient
     void SetPhotometric(float weight, float maxDifference, float minGradient);
     void SetHistory(float* dHistory, int capacity) { m_history = dHistory; m_historyCap = capacity; }
@@ -170,7 +173,7 @@ private:
     float* m_history = nullptr;
     int m_historyCap = 0;
     // the joint order: intensity pyramid of the frame, the feedback intensities and gradients
-    DeviceBuffer m_intensity[NUIF_ICP_MAX_LEVELS], m_refModel;
+    DeviceBuffer m_intensity[NUIF_ICP_MAX_LEVELS], m_refModel, m_prevColors;
     nuif_icp_photometric m_photo{};
     void preparePhotometric(NuiKinfuHIPFrame* f, NuiKinfuHIPFeedbackFrame* fb);
 };

@@ -208,9 +208,10 @@ template <class T, class Vol> struct Engine {
     // shifting volume: m_voxel_offset and the evicted points (NuiKinfuVertexCache)
     int off[3] = {0, 0, 0};
     std::vector<float> cache_pts, cache_cols;
-    // joint order: the feedback colours (first frame's, then the raycast's), the frame's
-    // intensity pyramid, the feedback intensities and their gradients
-    std::vector<uint8_t> cfb;
+    // the feedback colours (first frame's, then the raycast's); joint order: the previous
+    // frame's colours (the photometric model), the frame's intensity pyramid, the model's
+    // intensities and their gradients
+    std::vector<uint8_t> cfb, prevc;
     std::vector<std::vector<float>> ipyr;
     std::vector<float> iref, gref;
     std::vector<std::vector<double>> ipyrd;  // the FP64 engine's
@@ -323,12 +324,13 @@ template <class T, class Vol> struct Engine {
     }
 
     // The joint order's ICP: intensities of the frame's colours per level and of the
-    // feedback colours with their gradients, then icp_compensated with photometric rows.
+    // previous frame's colours with their gradients, then icp_compensated with photometric
+    // rows.
     IcpResult joint_icp(const IcpConfig& ic, const T* const* vp, const Rigid<T>& pose, const uint8_t* bgra, double Rn[9],
                         double tn[3]) {
         if constexpr (!std::is_same<T, float>::value) {
             // FP64: the same inputs and rows evaluated in double
-            if (!bgra || cfb.empty()) return IcpResult{0, 0, 0, 0};
+            if (!bgra || prevc.empty()) return IcpResult{0, 0, 0, 0};
             const size_t n = (size_t)c.W * c.H;
             ipyrd.resize(c.levels);
             ipyrd[0].resize(n);
@@ -339,7 +341,8 @@ template <class T, class Vol> struct Engine {
             }
             irefd.resize(n);
             grefd.resize(2 * n);
-            intensities_d(cfb.data(), irefd.data(), n);
+            intensities_d(prevc.data(), irefd.data(), n);
+            prevc.assign(bgra, bgra + 4 * n);  // the next frame's model, whatever this ICP's outcome
             derivatives_d(irefd.data(), grefd.data(), c.W, c.H);
             const double mg = (double)c.photo[2];
             PhotoIn<double> ph{nullptr, irefd.data(), grefd.data(), std::sqrt((double)c.photo[0]), (double)c.photo[1],
@@ -348,7 +351,7 @@ template <class T, class Vol> struct Engine {
             for (int l = 0; l < c.levels; ++l) ip[l] = ipyrd[l].data();
             return icp_compensated<double>(cam, ic, vp, vref.data(), nref.data(), pose, Rn, tn, &ph, ip.data());
         } else {
-            if (!bgra || cfb.empty()) return IcpResult{0, 0, 0, 0};  // the joint order needs colours
+            if (!bgra || prevc.empty()) return IcpResult{0, 0, 0, 0};  // the joint order needs colours
             const size_t n = (size_t)c.W * c.H;
             ipyr.resize(c.levels);
             ipyr[0].resize(n);
@@ -359,7 +362,8 @@ template <class T, class Vol> struct Engine {
             }
             iref.resize(n);
             gref.resize(2 * n);
-            or_bgra_to_intensity(cfb.data(), iref.data(), (long long)n);
+            or_bgra_to_intensity(prevc.data(), iref.data(), (long long)n);
+            prevc.assign(bgra, bgra + 4 * n);  // the next frame's model, whatever this ICP's outcome
             or_intensity_derivatives(iref.data(), gref.data(), c.W, c.H);
             PhotoIn<float> ph{nullptr, iref.data(), gref.data(), std::sqrt(c.photo[0]), c.photo[1],
                               c.photo[2] * c.photo[2]};
@@ -388,6 +392,7 @@ template <class T, class Vol> struct Engine {
             normals_simple<T>(vpyr[0].data(), nm.data(), W, H, (T)c.depth_threshold);
             transform_maps<T>(vpyr[0].data(), nm.data(), vref.data(), nref.data(), n, pose);
             if (bgra) cfb.assign(bgra, bgra + 4 * n);
+            if (bgra && c.icp_mode == 2) prevc.assign(bgra, bgra + 4 * n);
             integrate_frame(bgra);
         } else {
             t0 = Clock::now();

@@ -8,7 +8,8 @@ against the oracle's float restatement, bit for bit:
   * a session alone (each frame's pipeline captured as a graph): every frame's pose,
     the volume's storage words, the raycast maps and the feedback colours over 12
     frames of the 320x240 orbit with colour, FP16 and FP32 volumes, and over 8 frames of
-    the 640x480 / 512^3 orbit with an FP16 volume;
+    the 640x480 / 512^3 orbit with an FP16 volume; and against the FP64 instance of the
+    joint order over 16 frames at 640x480 / 512^3 (0.1 mm / 0.01 deg per frame);
   * the 16-session mix (6 x 320x240 with 256^3, 6 x 640x480 and 4 x 1280x720 with 512^3
     FP16 volumes, colour) in batched server rounds, where all sessions' joint iterations run in
     lockstep launches: each session's poses, maps and feedback colours equal the same
@@ -149,6 +150,38 @@ def test_joint_engine_full_size_matches_oracle(gpu):
     np.testing.assert_array_equal(eng.feedback_colors(), ref.feedback_colors())
     eng.close()
     announce("joint engine full size: poses, volume, maps and colours exact")
+
+
+@pytest.mark.timeout(900)
+def test_joint_engine_against_fp64(gpu):
+    """640x480 / 512^3 FP16 with colour, 16 frames, against the FP64 instance of the joint
+    order (intensities, derivatives and rows in double): pose within 0.1 mm / 0.01 deg
+    of the FP64 engine at every frame."""
+    wl = W.FULL
+    frames = 16
+    announce(f"joint engine {wl.name} f16 colour, {frames} frames vs FP64 joint oracle")
+    desc = wl.kinfu_desc(storage=nuif.TSDF_F16, gate=0.0, color=True, use_graph=True, reduction=nuif.ICP_REDUCE_JOINT)
+    eng = nuif.Engine(desc)
+    rendered = [wl.render(f, color=True) for f in range(frames)]
+    for d, bgra, _ in rendered:
+        eng.run(nuif.DeviceArray.from_host(d), nuif.DeviceArray.from_host(bgra), sync=False)
+    assert eng.synchronize() == 1
+    poses = eng.poses(frames)
+    eng.close()
+    o64 = O.OracleEngine(wl, prec=1, color=True, joint=True)
+    worst = (0.0, 0.0)
+    for f, (d, bgra, _) in enumerate(rendered):
+        ok, p64, _ = o64.run(d, bgra)
+        assert ok == 1
+        R, t = poses[f, :9].reshape(3, 3).astype(np.float64), poses[f, 9:12].astype(np.float64)
+        Rr = R.T @ p64[:9].reshape(3, 3)
+        w = 0.5 * np.array([Rr[2, 1] - Rr[1, 2], Rr[0, 2] - Rr[2, 0], Rr[1, 0] - Rr[0, 1]])
+        ang = float(np.degrees(np.arctan2(np.linalg.norm(w), (np.trace(Rr) - 1) / 2)))
+        dt = float(np.linalg.norm(t - p64[9:]))
+        worst = (max(worst[0], dt), max(worst[1], ang))
+        assert dt <= 1e-4 and ang <= 0.01, (f, dt * 1e3, ang)
+    o64.close()
+    announce(f"joint vs FP64: pose max {worst[0] * 1e3:.4f} mm {worst[1]:.5f} deg over {frames} frames")
 
 
 @pytest.mark.timeout(900)

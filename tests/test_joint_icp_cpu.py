@@ -12,8 +12,8 @@ have no host loop).
     sqrt(weight) is I - I_ref; rows are refused for NaN, a large difference or a small
     gradient;
   * engines: the joint order with weight 0 runs the compensated order's poses bit for
-    bit; with weight 0.1 it differs from it and follows the same motion within 5 mm
-    and 1 degree.
+    bit (float and FP64 instances); with weight 0.1 it differs from it and follows the
+    same motion within 2 mm and 0.2 degrees at 640x480 / 512^3.
 """
 import numpy as np
 import pytest
@@ -138,21 +138,25 @@ def test_joint_engine_weight_zero_is_the_compensated_order():
 
 
 def test_joint_engine_tracks_the_orbit():
-    wl = W.SMALL
-    frames = 8
-    comp = O.OracleEngine(wl, prec=0, color=True, compensated=True, storage=2)
-    joint = O.OracleEngine(wl, prec=0, color=True, joint=True, storage=2)
+    # at 640x480 / 512^3 (at 320x240 / 128^3 with 1.6 cm voxels even the float and FP64
+    # instances of one order differ by millimetres, so the orders are compared here)
+    wl = W.FULL
+    frames = 5
+    comp = O.OracleEngine(wl, prec=0, color=True, compensated=True, storage=1)
+    joint = O.OracleEngine(wl, prec=0, color=True, joint=True, storage=1)
     pj, pc = _poses(joint, frames, wl), _poses(comp, frames, wl)
     assert not np.array_equal(pj, pc)  # the photometric rows take part
-    # the same motion as the geometric order's (the orbit moves 4 cm and 2 deg a frame)
+    # the same motion as the geometric order's (the orbit moves 5 cm and 2 deg a frame)
     for f in range(frames):
-        assert np.linalg.norm(pj[f, 9:] - pc[f, 9:]) < 5e-3, f
-        Rj, Rc = pj[f, :9].reshape(3, 3), pc[f, :9].reshape(3, 3)
-        ang = np.degrees(np.arccos(np.clip((np.trace(Rj @ Rc.T) - 1) / 2, -1, 1)))
-        assert ang < 1.0, f
+        assert np.linalg.norm(pj[f, 9:] - pc[f, 9:]) < 2e-3, f
+        Rr = pj[f, :9].reshape(3, 3).T @ pc[f, :9].reshape(3, 3)
+        w = 0.5 * np.array([Rr[2, 1] - Rr[1, 2], Rr[0, 2] - Rr[2, 0], Rr[1, 0] - Rr[0, 1]])
+        assert np.degrees(np.arctan2(np.linalg.norm(w), (np.trace(Rr) - 1) / 2)) < 0.2, f
     assert len(joint.photometric(0)) == wl.width * wl.height
     assert len(joint.photometric(1)) == (wl.width // 2) * (wl.height // 2)
     assert len(joint.photometric(4)) == 2 * wl.width * wl.height
+    joint.close()
+    comp.close()
 
 
 def test_fp64_joint_engine_weight_zero_is_the_fp64_compensated_order():
