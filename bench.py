@@ -283,7 +283,7 @@ def server_stages(nuif, srv, ids, wl, frames, first, rounds, storage):
             "rooflines": roofs}
 
 
-def run_c4(nuif, W, warmup, rounds, budget, pool_frames=16):
+def run_c4(nuif, W, warmup, rounds, budget, pool_frames=16, unbatched=True):
     """The joint-ICP mix: 16 colour sessions (6 x 320x240 256^3, 6 x 640x480 and
     4 x 1280x720 512^3, FP16 volumes), closed-loop rounds of one frame per session with
     its BGRA colours.  Measured three ways: the joint order batched (all sessions' joint
@@ -354,11 +354,11 @@ def run_c4(nuif, W, warmup, rounds, budget, pool_frames=16):
 
     joint = leg(nuif.ICP_REDUCE_JOINT, True, stages=True)
     geometric = leg(nuif.ICP_REDUCE_COMPENSATED, True, stages=True)
-    unbatched = leg(nuif.ICP_REDUCE_JOINT, False)
+    unb = leg(nuif.ICP_REDUCE_JOINT, False) if unbatched else None
     return {"sessions": len(mix), "mix": "6 x 320x240 (256^3), 6 x 640x480 (512^3), 4 x 1280x720 (512^3); "
             "FP16 volumes with colour over 3 m; colour weight 0.1", "rounds": rounds,
-            "joint_batched": joint, "geometric_batched": geometric, "joint_unbatched": unbatched,
-            "joint_over_unbatched": round(joint["fps"] / unbatched["fps"], 3),
+            "joint_batched": joint, "geometric_batched": geometric, "joint_unbatched": unb,
+            "joint_over_unbatched": round(joint["fps"] / unb["fps"], 3) if unb else None,
             "photometric_icp_cost": round(joint["stage_ms"]["icp"] / geometric["stage_ms"]["icp"], 3)}
 
 
@@ -465,6 +465,7 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0, help="server device-memory budget")
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--c4-rounds", type=int, default=30, help="joint-ICP mix rounds (0: skip that leg)")
+    ap.add_argument("--c4-batched-only", action="store_true", help="joint-ICP mix: skip the unbatched comparison")
     ap.add_argument("--c5-sessions", type=int, default=64, help="arrival-trace sessions (0: skip that leg)")
     ap.add_argument("--c5-rate", type=float, default=2.0, help="arrivals per second")
     ap.add_argument("--hash-steps", type=int, default=20, help="frames of the 640x480 hash session (0: skip)")
@@ -547,7 +548,7 @@ def main():
     c4 = None
     if args.c4_rounds > 0:
         print(f"[bench] joint-ICP mix: 16 sessions, {args.c4_rounds} rounds", file=sys.stderr, flush=True)
-        c4 = run_c4(nuif, W, 5, args.c4_rounds, args.budget_gb * 2 ** 30)
+        c4 = run_c4(nuif, W, 5, args.c4_rounds, args.budget_gb * 2 ** 30, unbatched=not args.c4_batched_only)
         print(f"[bench] joint-ICP mix done: {c4['joint_batched']['fps']} fps batched", file=sys.stderr, flush=True)
     c5 = None
     if args.c5_sessions > 0:
