@@ -540,6 +540,18 @@ def main():
         server = run_server(nuif, wl, frames, args.sessions, STORAGES[args.server_storage], args.warmup, args.steps,
                             args.budget_gb * 2 ** 30)
 
+    small = None
+    if not args.server_only:
+        # the small configuration: 320x240, dense FP32 volume of 128^3 over 2 m, the
+        # reference's ICP order, its 30 frames (5 warm-up, 25 timed)
+        print(f"[bench] small session: {W.SMALL.name}", file=sys.stderr, flush=True)
+        sf = [nuif.DeviceArray.from_host(W.SMALL.render(f)[0]) for f in range(W.SMALL.frames)]
+        r = run_single(nuif, W.SMALL, sf, STORAGES["f32"], 0, 5, W.SMALL.frames - 5,
+                       reduction=nuif.ICP_REDUCE_REFERENCE)
+        small = {"workload": W.SMALL.name, "storage": "f32", "icp": "reference order", "fps": round(r["fps"], 1),
+                 "ms_per_frame": round(r["ms_per_frame"], 4), "tracking_ok": r["ok"],
+                 "stage_ms": {k: round(v, 4) for k, v in zip(names, r["stage_avg"])} if r["stage_avg"] else None}
+
     hashed = None
     if args.hash_steps > 0:
         print(f"[bench] hash session: {W.HASH.name}", file=sys.stderr, flush=True)
@@ -603,6 +615,7 @@ def main():
         "server_vs_fp32_serial": round(server["fps"] / fp32["fps"], 3) if server and fp32 else None,
         "fp32_serial_compensated_fps": round(fp32c["fps"], 3) if fp32c else None,
         "roofline": roof,
+        "small_session": small,
         "hash_session": hashed,
         "joint_icp_mix": c4,
         "arrival_trace": c5,
