@@ -302,13 +302,15 @@ typedef struct nuif_icp_session {
 } nuif_icp_session;
 /* Device bytes of the session table of nuif_icp_estimate_batch. */
 size_t nuif_icp_batch_table_bytes(int n);
-/* The frame's ICP of n <= 4096 sessions (any image sizes, either order) in lockstep steps:
+/* The frame's ICP of n <= 4096 sessions (any image sizes, any NUIF_ICP_REDUCE_* order) in lockstep steps:
  * step k is two launches for all sessions of an order, the 512-pixel tiles of every
  * session's k-th iteration (whatever its level: each session moves through its levels on
  * its own and is done after a break at its finest level or a failure) spread over the
  * grid, then one block per session for the reduction and the update.  The steps are as
  * many as the most iterations a session is configured for.  Results are bit for bit
- * those of nuif_icp_estimate.  No host synchronisation.  table:
+ * those of nuif_icp_estimate (nuif_icp_estimate_joint for a joint-order session, whose
+ * photometric inputs are made beforehand, e.g. by nuif_icp_photometric_inputs).  No host
+ * synchronisation.  table:
  * nuif_icp_batch_table_bytes(n) of device memory, written on the stream (calls on one
  * stream may reuse it). */
 int nuif_icp_estimate_batch(const nuif_icp_session* sessions, int n, void* table, nuif_stream_t s);

@@ -77,8 +77,8 @@ struct IcpState {
     float error, count;
     int32_t brk[NUIF_ICP_MAX_LEVELS];
     int32_t failed, iterations, gate, frame, ok, timeout;
-    double good[27];       // unused by both orders today; kept for the layout
-    double sums[kTerms];   // sums of the last executed iteration (both orders)
+    double good[27];       // unused by every order today; kept for the layout
+    double sums[kTerms];   // sums of the last executed iteration (every order)
     float goodf[27];       // reference order: goodCores as the reference keeps them (float)
     float pad2;
     float off[3];          // shifting volume: offset of the local frame (metres), 0 otherwise
