@@ -7,7 +7,7 @@ Two oracles (CPU, oracle/):
   * the float instance of the same algorithm, with the GPU's storage words and
     the GPU's summation order: poses (float and FP64), volume words and raycast
     maps are compared BIT FOR BIT, every frame;
-  * the FP64 instance: the north star's error bars, stated per test --
+  * the FP64 instance: the session type's error bars, stated per test --
       pose             <= 0.1 mm and 0.01 deg per frame (FP64 engine run alongside),
       TSDF             <= 1e-3 of the truncation distance (normalised tsdf), FP64
                        volume integrated from the same frames at the GPU's poses,
