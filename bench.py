@@ -167,10 +167,11 @@ def run_single(nuif, wl, frames, storage, graph, warmup, steps, reduction=None, 
             "updated_per_frame": upd / max(steps, 1)}
 
 
-def run_server(nuif, wl, frames, sessions, storage, warmup, rounds, budget):
+def run_server(nuif, wl, frames, sessions, storage, warmup, rounds, budget, round_graph=True):
     """`sessions` resident sessions; each round submits one frame per session and
     waits for all of them (closed loop).  Session s plays frames s, s+1, ..."""
     srv = nuif.Server(budget)
+    srv.set_round_graph(round_graph)
     desc = wl.kinfu_desc(storage=storage, gate=0.0, use_graph=True)
     ids = []
     for _ in range(sessions):
@@ -466,6 +467,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--c4-rounds", type=int, default=30, help="joint-ICP mix rounds (0: skip that leg)")
     ap.add_argument("--c4-batched-only", action="store_true", help="joint-ICP mix: skip the unbatched comparison")
+    ap.add_argument("--no-round-graph", action="store_true", help="server: enqueue batched rounds launch by launch")
     ap.add_argument("--c5-sessions", type=int, default=64, help="arrival-trace sessions (0: skip that leg)")
     ap.add_argument("--c5-rate", type=float, default=2.0, help="arrivals per second")
     ap.add_argument("--hash-steps", type=int, default=20, help="frames of the 640x480 hash session (0: skip)")
@@ -538,7 +540,7 @@ def main():
         # 3) the multi-session server
         print(f"[bench] server: {args.sessions} sessions, storage {args.server_storage}", file=sys.stderr, flush=True)
         server = run_server(nuif, wl, frames, args.sessions, STORAGES[args.server_storage], args.warmup, args.steps,
-                            args.budget_gb * 2 ** 30)
+                            args.budget_gb * 2 ** 30, round_graph=not args.no_round_graph)
 
     small = None
     if not args.server_only:

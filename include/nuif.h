@@ -460,6 +460,13 @@ size_t nuif_fusion_round_table_bytes(int n);
 /* table / staging: nuif_fusion_round_table_bytes(n) each; the staging copy may be reused once
  * the stream has passed this call's upload (e.g. after waiting for the round). */
 int nuif_fusion_round(const nuif_round_session* sessions, int n, void* table, void* staging, nuif_stream_t s);
+/* The host part of nuif_fusion_round alone: writes the job table into staging and returns
+ * a signature of the round's launches (grids, groups, kernel choices, the table and staging
+ * addresses).  A round captured into a graph (nuif_graph_begin / nuif_fusion_round /
+ * nuif_graph_end) replays a later round whose signature is the same, its table written by
+ * this call first.  No device work.  No counterpart in the reference. */
+int nuif_fusion_round_prepare(const nuif_round_session* sessions, int n, void* table, void* staging,
+                              uint64_t* signature);
 /* nuif_fusion_round with 6 events recorded on s around its stages (no counterpart in the
  * reference): [0] start, [1] after preprocessing and the pyramid, [2] after the depth bounds,
  * [3] after the ICP, [4] after the integration, [5] after the raycast. */

@@ -168,6 +168,9 @@ int nuifs_submit(void* srv, int n, const int* ids, const uint16_t* const* depths
  * which require them).  No counterpart in the reference. */
 int nuifs_submit_color(void* srv, int n, const int* ids, const uint16_t* const* depths,
                        const uint8_t* const* bgras);
+/* Batched rounds captured once as a graph and replayed while their launches stay the same
+ * (default, non-zero) or enqueued launch by launch (0).  No counterpart in the reference. */
+int nuifs_set_round_graph(void* srv, int on);
 /* Batched rounds on (default, non-zero) or off: every frame on its session's stream. */
 int nuifs_set_batching(void* srv, int on);
 /* Frames of the last round that ran in the batch. */

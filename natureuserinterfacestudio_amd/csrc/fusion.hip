@@ -94,8 +94,71 @@ int nuif_fusion_round(const nuif_round_session* S, int n, void* table, void* sta
     return nuif_fusion_round_timed(S, n, table, staging, nullptr, s);
 }
 
+}  // extern "C"
+
+namespace {
+
+// FNV-1a over the values that decide a round's launches (grids, groups, kernel choices and
+// the table and staging addresses baked into them)
+struct Sig {
+    uint64_t h = 1469598103934665603ull;
+    void add(uint64_t v) {
+        for (int k = 0; k < 8; ++k) {
+            h ^= (v >> (8 * k)) & 0xff;
+            h *= 1099511628211ull;
+        }
+    }
+};
+
+uint64_t round_signature(const nuif_round_session* S, int n, const void* table, const void* staging) {
+    Sig g;
+    g.add((uint64_t)n);
+    g.add((uint64_t)(uintptr_t)table);
+    g.add((uint64_t)(uintptr_t)staging);
+    for (int i = 0; i < n; ++i) {
+        const nuif_round_session& r = S[i];
+        g.add(r.icp.cam.depthImageWidth);
+        g.add(r.icp.cam.depthImageHeight);
+        g.add((uint64_t)r.icp.cfg.levels);
+        for (int l = 0; l < NUIF_ICP_MAX_LEVELS; ++l) g.add((uint64_t)r.icp.cfg.iterations[l]);
+        g.add((uint64_t)r.icp.cfg.reduction);
+        g.add((uint64_t)r.storage);
+        g.add((uint64_t)(r.color_volume && r.bgra));
+        g.add((uint64_t)(r.color_volume != nullptr));
+        for (int k = 0; k < 3; ++k) g.add((uint64_t)r.params.resolution[k]);
+        g.add((uint64_t)(r.walk_states != nullptr));
+        g.add((uint64_t)(r.surface_blocks != nullptr));
+        g.add((uint64_t)r.radius);
+    }
+    return g.h;
+}
+
+int fusion_round_impl(const nuif_round_session* S, int n, void* table, void* staging, nuif_event_t* ev,
+                      nuif_stream_t s, bool launch);
+
+}  // namespace
+
+extern "C" {
+
+int nuif_fusion_round_prepare(const nuif_round_session* S, int n, void* table, void* staging, uint64_t* signature) {
+    NUIF_CHECK_ARG(signature, "nuif_fusion_round_prepare: NULL signature");
+    const int rc = fusion_round_impl(S, n, table, staging, nullptr, nullptr, false);
+    if (rc) return rc;
+    *signature = round_signature(S, n, table, staging);
+    return NUIF_OK;
+}
+
 int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, void* staging, nuif_event_t* ev,
                             nuif_stream_t s) {
+    return fusion_round_impl(S, n, table, staging, ev, s, true);
+}
+
+}  // extern "C"
+
+namespace {
+
+int fusion_round_impl(const nuif_round_session* S, int n, void* table, void* staging, nuif_event_t* ev,
+                      nuif_stream_t s, bool launch) {
     NUIF_REQUIRE_INIT();
     NUIF_CHECK_ARG(n >= 0 && (n == 0 || (S && table && staging)), "nuif_fusion_round: bad arguments");
     if (n == 0) return NUIF_OK;
@@ -252,6 +315,7 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
         rb += R.gx * div_up(r.icp.cam.depthImageHeight, 8);
     }
 
+    if (!launch) return NUIF_OK;  // the host table only (a captured round replays the rest)
     NUIF_HIP(hipMemcpyAsync(table, staging, lay.total, hipMemcpyHostToDevice, st));
     auto mark = [&](int k) -> int {
         if (ev) NUIF_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(ev[k]), st));
@@ -288,4 +352,5 @@ int nuif_fusion_round_timed(const nuif_round_session* S, int n, void* table, voi
     return mark(5);
 }
 
-}  // extern "C"
+}  // namespace
+

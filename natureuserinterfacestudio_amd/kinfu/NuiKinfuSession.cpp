@@ -144,6 +144,7 @@ NuiFusionServer::~NuiFusionServer() {
     if (m_batchEnd) nuif_event_destroy(m_batchEnd);
     for (nuif_event_t e : m_stageEv)
         if (e) nuif_event_destroy(e);
+    if (m_roundGraph) nuif_graph_destroy(m_roundGraph);
     if (m_staging) nuif_host_free(m_staging);
     if (m_status) nuif_host_free(m_status);
     if (m_ctl) nuif_stream_destroy(m_ctl);
@@ -263,9 +264,7 @@ void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depth
     // batch (its later work and read-backs see the round's results)
     for (int id : batched) check(nuif_stream_wait_event(m_ctl, m_ends[id]), "batch wait");
     ensureTable((int)batch.size());
-    check(nuif_fusion_round_timed(batch.data(), (int)batch.size(), m_table.get(), m_staging,
-                                  m_stageTiming ? m_stageEv : nullptr, m_ctl),
-          "nuif_fusion_round");
+    launchRound(batch.data(), (int)batch.size());
     m_stagePending = m_stageTiming;
     check(nuif_event_record(m_batchEnd, m_ctl), "batch end");
     m_batchPending = true;
@@ -275,6 +274,41 @@ void NuiFusionServer::submit(int n, const int* ids, const uint16_t* const* depth
         check(nuif_event_record(m_ends[id], m_ctl), "frame end");
         s->afterFrame();
     }
+}
+
+void NuiFusionServer::setRoundGraph(bool on) {
+    m_roundGraphs = on;
+    if (!on && m_roundGraph) {
+        if (m_batchPending) check(nuif_event_synchronize(m_batchEnd), "batched round");
+        nuif_graph_destroy(m_roundGraph);
+        m_roundGraph = nullptr;
+    }
+}
+
+// The batched round: with stage timing, enqueued with its events; else the host table is
+// written and the captured round replayed, captured anew when its launches changed (other
+// sessions' sizes or orders, a reallocated table)
+void NuiFusionServer::launchRound(const nuif_round_session* batch, int n) {
+    if (m_stageTiming || !m_roundGraphs) {
+        check(nuif_fusion_round_timed(batch, n, m_table.get(), m_staging, m_stageTiming ? m_stageEv : nullptr, m_ctl),
+              "nuif_fusion_round");
+        return;
+    }
+    uint64_t sig = 0;
+    check(nuif_fusion_round_prepare(batch, n, m_table.get(), m_staging, &sig), "nuif_fusion_round_prepare");
+    if (!m_roundGraph || sig != m_roundSig) {
+        if (m_roundGraph) nuif_graph_destroy(m_roundGraph);
+        m_roundGraph = nullptr;
+        check(nuif_graph_begin(m_ctl), "round capture");
+        const int rc = nuif_fusion_round(batch, n, m_table.get(), m_staging, m_ctl);
+        nuif_graph_t g = nullptr;
+        const int rc_end = nuif_graph_end(m_ctl, &g);
+        check(rc, "nuif_fusion_round");
+        check(rc_end, "round capture");
+        m_roundGraph = g;
+        m_roundSig = sig;
+    }
+    check(nuif_graph_launch(m_roundGraph, m_ctl), "round replay");
 }
 
 int NuiFusionServer::wait(float* latencyMs) {

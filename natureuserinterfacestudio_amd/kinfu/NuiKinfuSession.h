@@ -74,6 +74,9 @@ public:
     void setBatching(bool on) { m_batching = on; }
     // frames of the last round that ran in the batch
     int lastBatchSize() const { return m_lastBatch; }
+    // batched rounds captured once as a graph and replayed while the round's launches stay
+    // the same (default on; stage timing runs rounds un-captured)
+    void setRoundGraph(bool on);
 
 private:
     void ensureTable(int n);
@@ -102,6 +105,10 @@ private:
     nuif_event_t m_stageEv[6] = {};
     double m_stageMs[5] = {0, 0, 0, 0, 0};
     int m_stageRounds = 0;
+    bool m_roundGraphs = true;
+    nuif_graph_t m_roundGraph = nullptr;  // the captured batched round and its launch signature
+    uint64_t m_roundSig = 0;
+    void launchRound(const nuif_round_session* batch, int n);
 };
 
 }  // namespace nuif_kinfu

@@ -407,6 +407,14 @@ int nuifs_submit_color(void* srv, int n, const int* ids, const uint16_t* const* 
     });
 }
 
+int nuifs_set_round_graph(void* srv, int on) {
+    return guarded([&] {
+        if (!srv) throw std::runtime_error("nuifs_set_round_graph: null server");
+        static_cast<NuiFusionServer*>(srv)->setRoundGraph(on != 0);
+        return 0;
+    });
+}
+
 int nuifs_set_batching(void* srv, int on) {
     return guarded([&] {
         if (!srv) throw std::runtime_error("nuifs_set_batching: null server");

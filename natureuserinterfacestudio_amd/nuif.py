@@ -150,6 +150,7 @@ _SIGS = {
     "nuif_icp_static_scratch_bytes": (c_size_t, []),
     "nuif_fusion_round_table_bytes": (c_size_t, [c_int]),
     "nuif_fusion_round": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "nuif_fusion_round_prepare": (c_int, [c_void_p, c_int, c_void_p, c_void_p, C.POINTER(C.c_uint64)]),
     "nuif_icp_estimate_static": (c_int, [c_void_p, C.POINTER(NuiCLCameraParams), c_void_p, c_void_p, c_void_p,
                                          c_void_p, C.POINTER(IcpConfig), c_void_p, c_int, c_void_p]),
     "nuif_icp_estimate_batch": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
@@ -308,6 +309,7 @@ _KSIGS = {
     "nuifs_engine": (c_void_p, [c_void_p, c_int]),
     "nuifs_submit": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p)]),
     "nuifs_submit_color": (c_int, [c_void_p, c_int, C.POINTER(c_int), C.POINTER(c_void_p), C.POINTER(c_void_p)]),
+    "nuifs_set_round_graph": (c_int, [c_void_p, c_int]),
     "nuifs_wait": (c_int, [c_void_p, C.POINTER(c_float)]),
     "nuifs_set_failure_policy": (c_int, [c_void_p, c_int]),
     "nuifs_set_stage_timing": (c_int, [c_void_p, c_int]),
@@ -778,6 +780,10 @@ class Server:
             return
         self._cptrs = (c_void_p * n)(*[c.ptr if c is not None else None for c in colors])
         self._check(K().nuifs_submit_color(self._h, n, self._ids, self._ptrs, self._cptrs), "nuifs_submit_color")
+
+    def set_round_graph(self, on):
+        """Batched rounds replayed from a captured graph (default) or enqueued launch by launch."""
+        self._check(K().nuifs_set_round_graph(self._h, 1 if on else 0), "nuifs_set_round_graph")
 
     def set_batching(self, on):
         """Batched rounds (nuif_fusion_round) on or off."""
